@@ -1,0 +1,457 @@
+"""graph_analyzer_amd - MI355X-native triangle counting, triangle-density batch peeling and (k,triangle)-core.
+
+Python face of the C ABI in include/ga_amd.h (libga_amd.so, HIP kernels for gfx950). The C++ face, which restates
+the reference's graph.h / densest.h / dsg.h API, is in graph_analyzer_amd/host/. There is no CPU fallback: if the
+HIP library is missing or no GPU is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libga_amd.so")
+GEN_PATH = os.path.join(_HERE, "libga_gen.so")
+
+GA_OK, GA_ERR_ARG, GA_ERR_HIP, GA_ERR_OOM, GA_ERR_ARITH, GA_ERR_STATE = range(6)
+
+# triangle-count variants (ga_amd.h)
+TC_AUTO, TC_MERGE, TC_GROUP4, TC_GROUP8, TC_GROUP16, TC_GROUP64, TC_HASH, TC_SORTED = range(8)
+TC_VARIANTS = {
+    "merge": TC_MERGE,
+    "group4": TC_GROUP4,
+    "group8": TC_GROUP8,
+    "group16": TC_GROUP16,
+    "group64": TC_GROUP64,
+    "hash": TC_HASH,
+    "sorted": TC_SORTED,
+}
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+
+
+class GAError(RuntimeError):
+    """A non-zero status from the C ABI; .code is the GA_ERR_* value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class TcTuning(ctypes.Structure):
+    _fields_ = [("lanes", ctypes.c_int32 * 4), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
+                ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "load_shift": self.load_shift,
+                "hash_cap": self.hash_cap}
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "ga_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+            "ga_ctx_destroy": (None, [_vp]),
+            "ga_ctx_last_error": (ctypes.c_char_p, [_vp]),
+            "ga_ctx_sync": (ctypes.c_int, [_vp]),
+            "ga_ctx_device_info": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                  ctypes.POINTER(ctypes.c_int), _u64p]),
+            "ga_graph_from_pairs": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_vp),
+                                                   ctypes.POINTER(ctypes.c_float)]),
+            "ga_graph_from_csr": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.POINTER(_vp)]),
+            "ga_graph_destroy": (None, [_vp]),
+            "ga_graph_info": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
+            "ga_graph_download": (ctypes.c_int, [_vp, _vp, _u32p, _u32p, _u32p]),
+            "ga_graph_undirected": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
+            "ga_graph_edge_order": (ctypes.c_int, [_vp, _vp, _u32p]),
+            "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
+            "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_tc_prepare": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+            "ga_tc_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
+            "ga_tc_set_tuning": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning)]),
+            "ga_tc_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
+            "ga_tc_vertex": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u8p, _u32p, _u64p]),
+            "ga_tc_edge_support": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u32p, _u64p]),
+            "ga_peel_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+            "ga_peel_destroy": (None, [_vp]),
+            "ga_peel_reset_state": (ctypes.c_int, [_vp]),
+            "ga_peel_begin_round": (ctypes.c_int, [_vp]),
+            "ga_peel_run_tc": (ctypes.c_int, [_vp]),
+            "ga_peel_run_mark": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32]),
+            "ga_peel_read_stats": (ctypes.c_int, [_vp, _u64p, _u64p]),
+            "ga_peel_download_removed_round": (ctypes.c_int, [_vp, _u32p]),
+            "ga_peel_run": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _u64p,
+                                           ctypes.POINTER(ctypes.c_float)]),
+            "ga_core_create": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(_vp)]),
+            "ga_core_destroy": (None, [_vp]),
+            "ga_core_reset_state": (ctypes.c_int, [_vp]),
+            "ga_core_begin_round": (ctypes.c_int, [_vp]),
+            "ga_core_run_tc": (ctypes.c_int, [_vp]),
+            "ga_core_run_min": (ctypes.c_int, [_vp]),
+            "ga_core_run_peel": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+            "ga_core_read_stats": (ctypes.c_int, [_vp, _u64p, _u64p, _u64p]),
+            "ga_core_init_tri": (ctypes.c_int, [_vp, _u64p]),
+            "ga_core_run_find": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+            "ga_core_run_update": (ctypes.c_int, [_vp]),
+            "ga_core_run_retire": (ctypes.c_int, [_vp]),
+            "ga_core_download": (ctypes.c_int, [_vp, _u32p]),
+            "ga_core_run": (ctypes.c_int, [_vp, ctypes.c_uint64, _u32p, ctypes.POINTER(ctypes.c_float)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class Context:
+    """One HIP device + stream (ga_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._h = _vp()
+        rc = lib().ga_ctx_create(device, ctypes.byref(self._h))
+        if rc != GA_OK:
+            raise GAError(rc, f"ga_ctx_create({device}) failed (no HIP device?)")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int, what: str = ""):
+        if rc != GA_OK:
+            msg = lib().ga_ctx_last_error(self._h)
+            raise GAError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def sync(self):
+        self.check(lib().ga_ctx_sync(self._h), "sync")
+
+    def device_info(self) -> dict:
+        name = ctypes.create_string_buffer(256)
+        cus, wave, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        self.check(lib().ga_ctx_device_info(self._h, name, 256, ctypes.byref(cus), ctypes.byref(wave), ctypes.byref(hbm)), "device_info")
+        return {"name": name.value.decode(), "cus": cus.value, "wave": wave.value, "hbm_bytes": hbm.value}
+
+    def close(self):
+        if self._h:
+            lib().ga_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class PeelResult:
+    """Mirror of PeelResult (densest.h:46-54)."""
+    best_num: int
+    best_den: int
+    best_round: int
+    rounds: int
+    removed_round: np.ndarray
+
+    def same(self, o: "PeelResult") -> bool:
+        return (self.best_num == o.best_num and self.best_den == o.best_den and self.best_round == o.best_round
+                and self.rounds == o.rounds and np.array_equal(self.removed_round, o.removed_round))
+
+
+class DeviceGraph:
+    """Oriented CSR resident in HBM (ga_graph)."""
+
+    def __init__(self, ctx: Context, handle, build_ms: float = 0.0):
+        self.ctx = ctx
+        self._h = handle
+        self.build_ms = build_ms
+        n, m, mo = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        ctx.check(lib().ga_graph_info(self._h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(mo)), "graph_info")
+        self.n, self.m, self.max_out_degree = n.value, m.value, mo.value
+
+    @classmethod
+    def from_pairs(cls, ctx: Context, pairs: np.ndarray, n: Optional[int] = None) -> "DeviceGraph":
+        """n=None: load_graph semantics on raw pairs; else build_graph_from_edges semantics (unique pairs, ids < n)."""
+        p = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1)
+        h = _vp()
+        ms = ctypes.c_float()
+        ctx.check(lib().ga_graph_from_pairs(ctx.handle, _ptr(p, _u32p), p.size // 2, -1 if n is None else int(n),
+                                            ctypes.byref(h), ctypes.byref(ms)), "graph_from_pairs")
+        return cls(ctx, h, ms.value)
+
+    @classmethod
+    def from_csr(cls, ctx: Context, rowptr: np.ndarray, col: np.ndarray) -> "DeviceGraph":
+        r = np.ascontiguousarray(rowptr, dtype=np.uint32)
+        c = np.ascontiguousarray(col, dtype=np.uint32)
+        h = _vp()
+        ctx.check(lib().ga_graph_from_csr(ctx.handle, r.size - 1, c.size, _ptr(r, _u32p), _ptr(c, _u32p), ctypes.byref(h)), "graph_from_csr")
+        return cls(ctx, h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def download(self):
+        rowptr = np.zeros(self.n + 1, np.uint32)
+        col = np.zeros(max(self.m, 1), np.uint32)
+        src = np.zeros(max(self.m, 1), np.uint32)
+        self.ctx.check(lib().ga_graph_download(self.ctx.handle, self._h, _ptr(rowptr, _u32p), _ptr(col, _u32p), _ptr(src, _u32p)), "download")
+        return rowptr, col[: self.m], src[: self.m]
+
+    def undirected(self):
+        urow = np.zeros(self.n + 1, np.uint32)
+        ucol = np.zeros(max(2 * self.m, 1), np.uint32)
+        self.ctx.check(lib().ga_graph_undirected(self.ctx.handle, self._h, _ptr(urow, _u32p), _ptr(ucol, _u32p)), "undirected")
+        return urow, ucol[: 2 * self.m]
+
+    def edge_order(self) -> np.ndarray:
+        perm = np.zeros(max(self.m, 1), np.uint32)
+        self.ctx.check(lib().ga_graph_edge_order(self.ctx.handle, self._h, _ptr(perm, _u32p)), "edge_order")
+        return perm[: self.m]
+
+    # ---- triangle counting ----
+    def count(self, variant: int = TC_AUTO):
+        T = ctypes.c_uint64()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_tc_count(self.ctx.handle, self._h, variant, ctypes.byref(T), ctypes.byref(ms)), f"tc_count[{variant}]")
+        return T.value, ms.value
+
+    def prepare(self) -> float:
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_tc_prepare(self.ctx.handle, self._h, ctypes.byref(ms)), "tc_prepare")
+        return ms.value
+
+    def get_tuning(self) -> TcTuning:
+        t = TcTuning()
+        self.ctx.check(lib().ga_tc_get_tuning(self._h, ctypes.byref(t)), "get_tuning")
+        return t
+
+    def set_tuning(self, t: TcTuning):
+        self.ctx.check(lib().ga_tc_set_tuning(self.ctx.handle, self._h, ctypes.byref(t)), "set_tuning")
+
+    def tune(self):
+        t = TcTuning()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_tc_tune(self.ctx.handle, self._h, ctypes.byref(t), ctypes.byref(ms)), "tc_tune")
+        return t, ms.value
+
+    def vertex_participation(self, alive: Optional[np.ndarray] = None, variant: int = TC_AUTO):
+        tri = np.zeros(max(self.n, 1), np.uint32)
+        T = ctypes.c_uint64()
+        ap = None
+        if alive is not None:
+            a = np.ascontiguousarray(alive, dtype=np.uint8)
+            assert a.size == self.n
+            ap = _ptr(a, _u8p)
+        self.ctx.check(lib().ga_tc_vertex(self.ctx.handle, self._h, variant, ap, _ptr(tri, _u32p), ctypes.byref(T)), "tc_vertex")
+        return T.value, tri[: self.n]
+
+    def edge_support(self, variant: int = TC_AUTO):
+        sup = np.zeros(max(self.m, 1), np.uint32)
+        T = ctypes.c_uint64()
+        self.ctx.check(lib().ga_tc_edge_support(self.ctx.handle, self._h, variant, _ptr(sup, _u32p), ctypes.byref(T)), "tc_edge_support")
+        return T.value, sup[: self.m]
+
+    # ---- peeling and core ----
+    def batch_peel(self, num: int = 1, den: int = 10, T_full: Optional[int] = None, fused: bool = True) -> PeelResult:
+        """Triangle-density batch peeling (densest.h:136-160 semantics)."""
+        p = Peel(self)
+        try:
+            return p.run(num, den, T_full) if fused else p.run_rounds(num, den, T_full)
+        finally:
+            p.close()
+
+    def core_decomposition(self, incremental: bool = True, fused: bool = True, T_full: Optional[int] = None):
+        """(k,triangle)-core numbers; returns (core, rounds, kmax)."""
+        c = Core(self, incremental)
+        try:
+            if fused and incremental:
+                return c.run(T_full)
+            return c.run_rounds(T_full)
+        finally:
+            c.close()
+
+    def close(self):
+        if self._h:
+            lib().ga_graph_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Peel:
+    """Batch-peeling state (ga_peel) with the reference's backend calls and the fused device loop."""
+
+    def __init__(self, g: DeviceGraph):
+        self.g = g
+        self.ctx = g.ctx
+        self._h = _vp()
+        self.ctx.check(lib().ga_peel_create(self.ctx.handle, g.handle, ctypes.byref(self._h)), "peel_create")
+        self.last_ms = 0.0
+
+    # backend concept (densest.h:265-278)
+    def reset_state(self):
+        self.ctx.check(lib().ga_peel_reset_state(self._h), "peel_reset_state")
+
+    def begin_round(self):
+        self.ctx.check(lib().ga_peel_begin_round(self._h), "peel_begin_round")
+
+    def run_tc(self):
+        self.ctx.check(lib().ga_peel_run_tc(self._h), "peel_run_tc")
+
+    def run_mark(self, nalive, num, den, rnd):
+        self.ctx.check(lib().ga_peel_run_mark(self._h, nalive, num, den, rnd), "peel_run_mark")
+
+    def read_stats(self):
+        T, r = ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(lib().ga_peel_read_stats(self._h, ctypes.byref(T), ctypes.byref(r)), "peel_read_stats")
+        return T.value, r.value
+
+    def removed_round(self) -> np.ndarray:
+        rr = np.zeros(max(self.g.n, 1), np.uint32)
+        self.ctx.check(lib().ga_peel_download_removed_round(self._h, _ptr(rr, _u32p)), "download_removed_round")
+        return rr[: self.g.n]
+
+    def run(self, num=1, den=10, T_full: Optional[int] = None) -> PeelResult:
+        stats = np.zeros(4, np.uint64)
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_peel_run(self._h, (1 << 64) - 1 if T_full is None else T_full, num, den, _ptr(stats, _u64p),
+                                         ctypes.byref(ms)), "peel_run")
+        self.last_ms = ms.value
+        return PeelResult(int(stats[0]), int(stats[1]), int(stats[2]), int(stats[3]), self.removed_round())
+
+    def run_rounds(self, num=1, den=10, T_full: Optional[int] = None) -> PeelResult:
+        """batch_peel_loop (densest.h:279-303) driven from the host, one readback per round."""
+        n = self.g.n
+        self.reset_state()
+        nalive = n
+        best_num = best_den = best_round = rounds = 0
+        rnd = 0
+        while nalive > 0:
+            self.begin_round()
+            self.run_tc()
+            self.run_mark(nalive, num, den, rnd)
+            T, removed = self.read_stats()
+            if rnd == 0 and T_full is not None and T != T_full:
+                raise GAError(GA_ERR_STATE, "device triangle count in round 0 differs from CPU count")
+            if best_den == 0 or T * best_den > best_num * nalive:
+                best_num, best_den, best_round = T, nalive, rnd
+            if removed == 0 or removed > nalive:
+                raise GAError(GA_ERR_STATE, "bad removed count from device (0 or more than alive)")
+            nalive -= removed
+            rounds = rnd + 1
+            rnd += 1
+        return PeelResult(best_num, best_den, best_round, rounds, self.removed_round())
+
+    def close(self):
+        if self._h:
+            lib().ga_peel_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Core:
+    """(k,triangle)-core state (ga_core): recount or incremental backend calls, fused incremental loop."""
+
+    def __init__(self, g: DeviceGraph, incremental: bool = True):
+        self.g = g
+        self.ctx = g.ctx
+        self.incremental = incremental
+        self._h = _vp()
+        self.ctx.check(lib().ga_core_create(self.ctx.handle, g.handle, 1 if incremental else 0, ctypes.byref(self._h)), "core_create")
+        self.last_ms = 0.0
+
+    def _stats(self):
+        T, r, k = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self.ctx.check(lib().ga_core_read_stats(self._h, ctypes.byref(T), ctypes.byref(r), ctypes.byref(k)), "core_read_stats")
+        return T.value, r.value, k.value
+
+    def core(self) -> np.ndarray:
+        c = np.zeros(max(self.g.n, 1), np.uint32)
+        self.ctx.check(lib().ga_core_download(self._h, _ptr(c, _u32p)), "core_download")
+        return c[: self.g.n]
+
+    def run(self, T_full: Optional[int] = None):
+        stats = np.zeros(2, np.uint32)
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_core_run(self._h, (1 << 64) - 1 if T_full is None else T_full, _ptr(stats, _u32p), ctypes.byref(ms)), "core_run")
+        self.last_ms = ms.value
+        return self.core(), int(stats[0]), int(stats[1])
+
+    def run_rounds(self, T_full: Optional[int] = None):
+        """core_decomp_loop / core_decomp_loop_inc (dsg.h:694-751) driven from the host."""
+        L = lib()
+        h = self._h
+        self.ctx.check(L.ga_core_reset_state(h), "core_reset_state")
+        n = self.g.n
+        if self.incremental:
+            T0 = ctypes.c_uint64()
+            self.ctx.check(L.ga_core_init_tri(h, ctypes.byref(T0)), "core_init_tri")
+            if T_full is not None and T0.value != T_full:
+                raise GAError(GA_ERR_STATE, "device triangle count differs from CPU count")
+        nalive, k, rounds, rnd = n, 0, 0, 0
+        while nalive > 0:
+            self.ctx.check(L.ga_core_begin_round(h), "core_begin_round")
+            if self.incremental:
+                self.ctx.check(L.ga_core_run_min(h), "run_min")
+                self.ctx.check(L.ga_core_run_find(h, k), "run_find")
+                self.ctx.check(L.ga_core_run_update(h), "run_update")
+                self.ctx.check(L.ga_core_run_retire(h), "run_retire")
+                _, removed, k_new = self._stats()
+            else:
+                self.ctx.check(L.ga_core_run_tc(h), "run_tc")
+                self.ctx.check(L.ga_core_run_min(h), "run_min")
+                self.ctx.check(L.ga_core_run_peel(h, k), "run_peel")
+                T, removed, k_new = self._stats()
+                if rnd == 0 and T_full is not None and T != T_full:
+                    raise GAError(GA_ERR_STATE, "device triangle count in round 0 differs from CPU count")
+            if removed == 0 or removed > nalive or k_new < k:
+                raise GAError(GA_ERR_STATE, "bad stats from device (removed 0, removed > alive, or level decreased)")
+            nalive -= removed
+            k = k_new
+            rounds = rnd + 1
+            rnd += 1
+        return self.core(), rounds, k
+
+    def close(self):
+        if self._h:
+            lib().ga_core_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def variant_name(v: int) -> str:
+    return lib().ga_tc_variant_name(v).decode()

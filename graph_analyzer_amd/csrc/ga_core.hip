@@ -1,0 +1,363 @@
+// ga_core.hip - (k,triangle)-core decomposition, level-synchronous (dsg.h:665-751; kernels.cuh:461-626).
+//
+// Level rule: k = max(k, min over alive of tri); every alive v with tri[v] <= k is removed from the same snapshot
+// with core[v] = k. Recount design: tri[] recounted on the alive subgraph every round. Incremental design: one full
+// count, then each round subtracts exactly the triangles that touch the round's frontier, each triangle handled by
+// its smallest-id frontier member (kernels.cuh:536-541), so tri[] after a round equals a recount.
+//
+// MI355X shape: wave64 min reduction with one atomic per wave; the frontier is appended with one atomic per wave
+// (ballot + prefix popcount); the update kernel gives each frontier vertex u one wave whose lanes split u's sorted
+// neighbour list, and each lane intersects N(u) after v with N(v) after v by walking the shorter list and binary-
+// searching the longer one (cost min(|A|,|B|) log, so hubs on either side stay cheap).
+// ga_core_run keeps k / nalive / rounds on the device (CoreDev) and enqueues rounds in blocks, one readback per block.
+#include "ga_internal.hpp"
+
+struct CoreDev {
+    unsigned long long T;          // stats[0]: count of the recount / init pass
+    unsigned long long removed;    // stats[1]: removed this round (= frontier size, incremental)
+    unsigned long long k;          // stats[2]: level after this round
+    unsigned long long nalive, k_prev, T_full;
+    uint32_t min_tri, done, error, rounds;
+};
+
+enum { CORE_ERR_T = 1, CORE_ERR_BIG = 2, CORE_ERR_STATS = 3 };
+
+struct ga_core {
+    ga_ctx* ctx = nullptr;
+    ga_graph* g = nullptr;
+    int incremental = 1;
+    uint8_t* state = nullptr;      // 0 removed, 1 alive, 2 frontier of this round (recount uses 0/1)
+    uint32_t* core = nullptr;
+    uint32_t* tri = nullptr;
+    uint32_t* frontier = nullptr;
+    CoreDev* st = nullptr;
+    int variant = GA_TC_HASH;
+};
+
+template <bool FUSED>
+__global__ void k_core_begin(CoreDev* st, uint32_t* __restrict__ tri, uint32_t n, int reset_tri) {
+    if (FUSED && st->done) return;
+    if (reset_tri)
+        for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) tri[v] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (reset_tri) st->T = 0;
+        st->removed = 0;
+        st->k = 0;
+        st->min_tri = 0xFFFFFFFFu;
+    }
+}
+
+// min over alive (state == 1) of tri (kernels.cuh:480-491).
+template <bool FUSED>
+__global__ void k_core_min(const uint32_t* __restrict__ tri, const uint8_t* __restrict__ state, uint32_t n, CoreDev* st) {
+    if (FUSED && st->done) return;
+    uint32_t mn = 0xFFFFFFFFu;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        if (state[v] == 1 && tri[v] < mn) mn = tri[v];
+    mn = wave_min_u32(mn);
+    if (lane_id() == 0 && mn != 0xFFFFFFFFu) atomicMin(&st->min_tri, mn);
+}
+
+// Recount level step (kernels.cuh:493-517).
+__global__ void k_core_peel_level(const uint32_t* __restrict__ tri, uint8_t* __restrict__ alive, uint32_t* __restrict__ core,
+                                  uint32_t n, unsigned long long k_in, CoreDev* st) {
+    const unsigned long long mt = st->min_tri;
+    const unsigned long long k = mt > k_in ? mt : k_in;
+    uint64_t local = 0;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        if (alive[v] == 1 && (unsigned long long)tri[v] <= k) {
+            alive[v] = 0;
+            core[v] = (uint32_t)k;
+            local++;
+        }
+    }
+    local = wave_sum_u64(local);
+    if (lane_id() == 0 && local) atomicAdd(&st->removed, (unsigned long long)local);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
+}
+
+// Frontier: alive v with tri <= k (kernels.cuh:544-562); appended with one atomic per wave.
+template <bool FUSED>
+__global__ void k_core_find(const uint32_t* __restrict__ tri, uint8_t* __restrict__ state, uint32_t* __restrict__ core, uint32_t n,
+                            unsigned long long k_in, uint32_t* __restrict__ frontier, CoreDev* st) {
+    if (FUSED && st->done) return;
+    const unsigned long long kp = FUSED ? st->k_prev : k_in;
+    const unsigned long long mt = st->min_tri;
+    const unsigned long long k = mt > kp ? mt : kp;
+    const uint32_t lane = lane_id();
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < n; base += nwaves * 64) {   // wave-uniform loop
+        const uint64_t v = base + lane;
+        bool join = false;
+        if (v < n && state[v] == 1 && (unsigned long long)tri[v] <= k) {
+            join = true;
+            state[v] = 2;
+            core[v] = (uint32_t)k;
+        }
+        const uint64_t mask = __ballot(join);
+        if (mask) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+            unsigned long long pos = 0;
+            if (lane == leader) pos = atomicAdd(&st->removed, (unsigned long long)__popcll(mask));
+            pos = __shfl(pos, leader, 64);
+            if (join) frontier[pos + mask_rank(mask)] = (uint32_t)v;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
+}
+
+// Triangles {u, v, w} through frontier vertex u with w > v: subtract once, by the smallest-id frontier member
+// (kernels.cuh:574-618). One wave per frontier vertex, lanes over N(u).
+template <bool FUSED>
+__global__ void k_core_update(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol, const uint8_t* __restrict__ state,
+                              const uint32_t* __restrict__ frontier, CoreDev* st, uint32_t* __restrict__ tri) {
+    if (FUSED && st->done) return;
+    const unsigned long long F = st->removed;
+    const uint32_t lane = lane_id();
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t i = wave0; i < F; i += nwaves) {
+        const uint32_t u = frontier[i];
+        const uint32_t ub = urow[u], ue = urow[u + 1];
+        for (uint32_t p = ub + lane; p < ue; p += 64) {
+            const uint32_t v = ucol[p];
+            const uint8_t sv = state[v];
+            if (sv == 0) continue;
+            uint32_t a = p + 1;                                              // N(u) after v: all > v
+            const uint32_t vb = urow[v], ve = urow[v + 1];
+            uint32_t b = upper_bound_u32(ucol, vb, ve, v);                    // N(v) after v
+            const bool a_short = (ue - a) <= (ve - b);
+            uint32_t s = a_short ? a : b;
+            const uint32_t se = a_short ? ue : ve;
+            uint32_t l = a_short ? b : a;
+            const uint32_t le = a_short ? ve : ue;
+            for (; s < se && l < le; s++) {
+                const uint32_t w = ucol[s];
+                l = lower_bound_u32(ucol, l, le, w);
+                if (l >= le) break;
+                if (ucol[l] != w) continue;
+                const uint8_t sw = state[w];
+                if (sw == 0) continue;
+                const bool smaller_frontier = (sv == 2 && v < u) || (sw == 2 && w < u);
+                if (smaller_frontier) continue;
+                if (sv == 1) atomicSub(&tri[v], 1u);
+                if (sw == 1) atomicSub(&tri[w], 1u);
+            }
+        }
+    }
+}
+
+template <bool FUSED>
+__global__ void k_core_retire(uint8_t* __restrict__ state, const uint32_t* __restrict__ frontier, CoreDev* st) {
+    if (FUSED && st->done) return;
+    const unsigned long long F = st->removed;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < F; i += (unsigned long long)gridDim.x * blockDim.x)
+        state[frontier[i]] = 0;
+}
+
+__global__ void k_core_init(CoreDev* st, unsigned long long n, unsigned long long T_full) {
+    st->T = 0;
+    st->removed = 0;
+    st->k = 0;
+    st->nalive = n;
+    st->k_prev = 0;
+    st->T_full = T_full;
+    st->min_tri = 0xFFFFFFFFu;
+    st->done = n == 0;
+    st->error = 0;
+    st->rounds = 0;
+}
+
+__global__ void k_core_check_T(CoreDev* st) {
+    if (st->done) return;
+    if (st->T_full != ~0ull && st->T != st->T_full) { st->error = CORE_ERR_T; st->done = 1; return; }
+    if (st->T >= (1ull << 32)) { st->error = CORE_ERR_BIG; st->done = 1; }
+}
+
+// End of a fused round: the rules of core_decomp_loop_inc (dsg.h:744-747), then the next round's resets.
+__global__ void k_core_end(CoreDev* st) {
+    if (st->done) return;
+    const unsigned long long removed = st->removed, k = st->k, nalive = st->nalive;
+    if (removed == 0 || removed > nalive || k < st->k_prev) { st->error = CORE_ERR_STATS; st->done = 1; return; }
+    st->nalive = nalive - removed;
+    st->k_prev = k;
+    st->rounds++;
+    st->removed = 0;
+    st->min_tri = 0xFFFFFFFFu;
+    if (st->nalive == 0) st->done = 1;
+}
+
+static unsigned cgrid(ga_ctx* ctx, uint64_t n) { return grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 8u); }
+static unsigned ugrid(ga_ctx* ctx) { return (unsigned)std::max(ctx->cus, 1) * 8u; }
+
+extern "C" int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core** out) {
+    *out = nullptr;
+    if (!ctx || !g) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    ga_core* c = new ga_core;
+    c->ctx = ctx;
+    c->g = g;
+    c->incremental = incremental ? 1 : 0;
+    const size_t n = g->n;
+    if (ga_malloc(ctx, (void**)&c->state, n + 16) || ga_malloc(ctx, (void**)&c->core, (n + 1) * 4) ||
+        ga_malloc(ctx, (void**)&c->tri, (n + 4) * 4) || ga_malloc(ctx, (void**)&c->frontier, (n + 1) * 4) ||
+        ga_malloc(ctx, (void**)&c->st, sizeof(CoreDev))) {
+        ga_core_destroy(c);
+        return GA_ERR_OOM;
+    }
+    int rc = ga_tc_ensure_plan(ctx, g);
+    if (!rc && c->incremental) rc = ga_graph_ensure_undirected(ctx, g);
+    if (!rc) {
+        k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, ~0ull);
+        rc = ga_launch_check(ctx, "k_core_init");
+    }
+    if (rc) { ga_core_destroy(c); return rc; }
+    *out = c;
+    return GA_OK;
+}
+
+extern "C" void ga_core_destroy(ga_core* c) {
+    if (!c) return;
+    if (c->ctx) (void)hipStreamSynchronize(c->ctx->stream);
+    ga_free(c->state);
+    ga_free(c->core);
+    ga_free(c->tri);
+    ga_free(c->frontier);
+    ga_free(c->st);
+    delete c;
+}
+
+extern "C" int ga_core_reset_state(ga_core* c) {
+    GA_TRY(ga_fill_u8(c->ctx, c->state, c->g->n, 1));
+    GA_TRY(ga_fill_u32(c->ctx, c->core, c->g->n, UINT32_MAX));
+    return GA_OK;
+}
+
+extern "C" int ga_core_begin_round(ga_core* c) {
+    ga_ctx* ctx = c->ctx;
+    k_core_begin<false><<<cgrid(ctx, c->g->n), 256, 0, ctx->stream>>>(c->st, c->tri, c->g->n, c->incremental ? 0 : 1);
+    return ga_launch_check(ctx, "k_core_begin");
+}
+
+extern "C" int ga_core_run_tc(ga_core* c) {
+    if (c->incremental) return ga_fail(c->ctx, GA_ERR_ARG, "run_tc belongs to the recount design");
+    return ga_tc_launch(c->ctx, c->g, c->variant, &c->st->T, c->state, c->tri, nullptr, nullptr);
+}
+
+extern "C" int ga_core_run_min(ga_core* c) {
+    ga_ctx* ctx = c->ctx;
+    k_core_min<false><<<cgrid(ctx, c->g->n), 256, 0, ctx->stream>>>(c->tri, c->state, c->g->n, c->st);
+    return ga_launch_check(ctx, "k_core_min");
+}
+
+extern "C" int ga_core_run_peel(ga_core* c, uint64_t k) {
+    if (c->incremental) return ga_fail(c->ctx, GA_ERR_ARG, "run_peel belongs to the recount design");
+    ga_ctx* ctx = c->ctx;
+    k_core_peel_level<<<cgrid(ctx, c->g->n), 256, 0, ctx->stream>>>(c->tri, c->state, c->core, c->g->n, k, c->st);
+    return ga_launch_check(ctx, "k_core_peel_level");
+}
+
+extern "C" int ga_core_read_stats(ga_core* c, uint64_t* T, uint64_t* removed, uint64_t* k) {
+    ga_ctx* ctx = c->ctx;
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, c->st, 24, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (T) *T = ctx->hstat[0];
+    if (removed) *removed = ctx->hstat[1];
+    if (k) *k = ctx->hstat[2];
+    return GA_OK;
+}
+
+static int init_tri_async(ga_core* c) {
+    ga_ctx* ctx = c->ctx;
+    k_core_begin<false><<<cgrid(ctx, c->g->n), 256, 0, ctx->stream>>>(c->st, c->tri, c->g->n, 1);
+    GA_TRY(ga_launch_check(ctx, "k_core_begin"));
+    return ga_tc_launch(ctx, c->g, c->variant, &c->st->T, c->state, c->tri, nullptr, nullptr);
+}
+
+extern "C" int ga_core_init_tri(ga_core* c, uint64_t* T) {
+    if (!c->incremental) return ga_fail(c->ctx, GA_ERR_ARG, "init_tri belongs to the incremental design");
+    GA_TRY(init_tri_async(c));
+    return ga_core_read_stats(c, T, nullptr, nullptr);
+}
+
+extern "C" int ga_core_run_find(ga_core* c, uint64_t k) {
+    if (!c->incremental) return ga_fail(c->ctx, GA_ERR_ARG, "run_find belongs to the incremental design");
+    ga_ctx* ctx = c->ctx;
+    k_core_find<false><<<cgrid(ctx, c->g->n), 256, 0, ctx->stream>>>(c->tri, c->state, c->core, c->g->n, k, c->frontier, c->st);
+    return ga_launch_check(ctx, "k_core_find");
+}
+
+extern "C" int ga_core_run_update(ga_core* c) {
+    if (!c->incremental) return ga_fail(c->ctx, GA_ERR_ARG, "run_update belongs to the incremental design");
+    ga_ctx* ctx = c->ctx;
+    k_core_update<false><<<ugrid(ctx), 256, 0, ctx->stream>>>(c->g->urow, c->g->ucol, c->state, c->frontier, c->st, c->tri);
+    return ga_launch_check(ctx, "k_core_update");
+}
+
+extern "C" int ga_core_run_retire(ga_core* c) {
+    if (!c->incremental) return ga_fail(c->ctx, GA_ERR_ARG, "run_retire belongs to the incremental design");
+    ga_ctx* ctx = c->ctx;
+    k_core_retire<false><<<ugrid(ctx), 256, 0, ctx->stream>>>(c->state, c->frontier, c->st);
+    return ga_launch_check(ctx, "k_core_retire");
+}
+
+extern "C" int ga_core_download(ga_core* c, uint32_t* core) {
+    ga_ctx* ctx = c->ctx;
+    if (c->g->n) GA_HIP(ctx, hipMemcpyAsync(core, c->core, (size_t)c->g->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}
+
+// Whole decomposition with the round loop on the device (incremental design; the recount design is available
+// through the per-round calls).
+extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms) {
+    ga_ctx* ctx = c->ctx;
+    ga_graph* g = c->g;
+    if (!c->incremental) return ga_fail(ctx, GA_ERR_ARG, "ga_core_run runs the incremental design");
+    if (T_full != UINT64_MAX && T_full >= (1ull << 32))
+        return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_TRY(ga_core_reset_state(c));
+    k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, T_full);
+    GA_TRY(ga_launch_check(ctx, "k_core_init"));
+    if (g->n) {
+        GA_TRY(init_tri_async(c));
+        k_core_check_T<<<1, 1, 0, ctx->stream>>>(c->st);
+        GA_TRY(ga_launch_check(ctx, "k_core_check_T"));
+    }
+    int block_rounds = 8;
+    const unsigned gv = cgrid(ctx, g->n), gu = ugrid(ctx);
+    for (uint64_t guard = 0;; guard++) {
+        for (int r = 0; r < block_rounds; r++) {
+            k_core_min<true><<<gv, 256, 0, ctx->stream>>>(c->tri, c->state, g->n, c->st);
+            k_core_find<true><<<gv, 256, 0, ctx->stream>>>(c->tri, c->state, c->core, g->n, 0, c->frontier, c->st);
+            k_core_update<true><<<gu, 256, 0, ctx->stream>>>(g->urow, g->ucol, c->state, c->frontier, c->st, c->tri);
+            k_core_retire<true><<<gu, 256, 0, ctx->stream>>>(c->state, c->frontier, c->st);
+            k_core_end<<<1, 1, 0, ctx->stream>>>(c->st);
+        }
+        GA_TRY(ga_launch_check(ctx, "core round"));
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, c->st, sizeof(CoreDev), hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const CoreDev* h = (const CoreDev*)ctx->hstat;
+        if (h->done) break;
+        if (guard > (uint64_t)g->n + 16) return ga_fail(ctx, GA_ERR_STATE, "core decomposition did not finish");
+        block_rounds = std::min(block_rounds * 2, 256);
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    const CoreDev* h = (const CoreDev*)ctx->hstat;
+    switch (h->error) {
+        case 0: break;
+        case CORE_ERR_T: return ga_fail(ctx, GA_ERR_STATE, "device triangle count differs from the given count");
+        case CORE_ERR_BIG: return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
+        default: return ga_fail(ctx, GA_ERR_STATE, "bad stats from device (removed 0, removed > alive, or level decreased)");
+    }
+    if (stats) {
+        stats[0] = h->rounds;
+        stats[1] = (uint32_t)h->k_prev;
+    }
+    return GA_OK;
+}

@@ -1,0 +1,146 @@
+// ga_internal.hpp - shared host structures, error plumbing and wave64 device helpers of libga_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ga_amd.h"
+
+// ---------------------------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------------------------
+struct ga_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t* hstat = nullptr;          // pinned host staging for per-round statistics (512 bytes)
+    uint64_t* dsmall = nullptr;         // 64 device words for counters of one call (never reallocated)
+    void* scratch = nullptr;            // grow-only device scratch for library temporaries (sort / scan storage)
+    size_t scratch_bytes = 0;
+    std::string err;
+};
+
+// Device-resident count plan of GA_TC_HASH: work items grouped by the endpoint whose out-list is hashed.
+struct TcPlan {
+    bool ready = false;
+    uint32_t nitems[4] = {0, 0, 0, 0};   // work items per hashed bucket
+    uint32_t nhuge = 0;                  // work items whose d+(h) is too large for an LDS table (binary search)
+    uint32_t* items = nullptr;           // 3 words per item (h, begin, end) into wcol, bucket-major
+    uint32_t* wcol = nullptr;            // probe-side vertex per work entry (grouped by hashed vertex)
+    uint32_t* wedge = nullptr;           // oriented edge id of each work entry
+    uint32_t nwork = 0;                  // work entries (edges with min(d+) > 0)
+    ga_tc_tuning tune = {};             // block == 0: defaults not applied yet
+    uint32_t* perm = nullptr;            // GA_TC_SORTED work order (ga_graph_edge_order), lazily
+};
+
+struct ga_graph {
+    ga_ctx* ctx = nullptr;
+    uint32_t n = 0, m = 0, max_out = 0;
+    uint32_t *rowptr = nullptr, *col = nullptr, *src = nullptr;
+    uint32_t *urow = nullptr, *ucol = nullptr;   // undirected CSR, built on demand
+    TcPlan plan;
+};
+
+int ga_fail(ga_ctx* ctx, int code, const char* fmt, ...);
+
+#define GA_HIP(ctx, call)                                                                                 \
+    do {                                                                                                  \
+        hipError_t e_ = (call);                                                                           \
+        if (e_ != hipSuccess) return ga_fail((ctx), GA_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                                             hipGetErrorString(e_), __FILE__, __LINE__);                  \
+    } while (0)
+
+#define GA_TRY(call)                 \
+    do {                             \
+        int rc_ = (call);            \
+        if (rc_ != GA_OK) return rc_; \
+    } while (0)
+
+int ga_malloc(ga_ctx* ctx, void** p, size_t bytes);
+void ga_free(void* p);
+int ga_scratch(ga_ctx* ctx, size_t bytes, void** p);   // returns a pointer to at least `bytes` of scratch
+int ga_launch_check(ga_ctx* ctx, const char* what);
+
+// device-side primitives implemented in ga_prims.hip
+int ga_sort_u64(ga_ctx* ctx, uint64_t* keys, uint64_t* tmp, size_t count, int end_bit, uint64_t** result);
+int ga_sort_pairs_u32(ga_ctx* ctx, uint32_t* keys, uint32_t* keys_tmp, uint32_t* vals, uint32_t* vals_tmp, size_t count,
+                      int end_bit, uint32_t** keys_out, uint32_t** vals_out);
+int ga_exclusive_scan_u32(ga_ctx* ctx, const uint32_t* in, uint32_t* out, size_t count);
+int ga_exclusive_scan_u64(ga_ctx* ctx, const uint64_t* in, uint64_t* out, size_t count);
+int ga_fill_u32(ga_ctx* ctx, uint32_t* p, size_t count, uint32_t v);
+int ga_fill_u8(ga_ctx* ctx, uint8_t* p, size_t count, uint8_t v);
+int ga_fill_u64(ga_ctx* ctx, uint64_t* p, size_t count, uint64_t v);
+int ga_max_u32(ga_ctx* ctx, const uint32_t* p, size_t count, uint32_t* out_host);
+
+// graph helpers shared between files
+int ga_graph_ensure_undirected(ga_ctx* ctx, ga_graph* g);
+int ga_graph_ensure_perm(ga_ctx* ctx, ga_graph* g);
+int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g);
+int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* d_alive, uint32_t* d_tri,
+                 uint32_t* d_sup, const uint32_t* d_done);
+
+inline unsigned grid_for(uint64_t threads, unsigned block, unsigned cap = 1u << 20) {
+    uint64_t b = (threads + block - 1) / block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned)b;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// device side: wave64 helpers
+// ---------------------------------------------------------------------------------------------------------------
+#define GA_WAVE 64
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t t = __shfl_xor(v, o, 64);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Number of active lanes below this one in a ballot mask.
+__device__ __forceinline__ uint32_t mask_rank(uint64_t mask) { return (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull)); }
+
+// First index in sorted a[lo, hi) with a[i] >= x.
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// First index in sorted a[lo, hi) with a[i] > x.
+__device__ __forceinline__ uint32_t upper_bound_u32(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t x) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}

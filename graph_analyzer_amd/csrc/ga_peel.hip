@@ -1,0 +1,259 @@
+// ga_peel.hip - triangle-density batch peeling (densest.h:1-44 algorithm, 265-303 host loop; kernels.cuh:360-459).
+//
+// Per-round backend calls mirror the reference's backend concept (reset_state / begin_round / run_tc / run_mark /
+// read_stats), so the unchanged host loop batch_peel_loop can drive them with one 16-byte readback per round.
+// ga_peel_run keeps the whole round state on the device instead (PeelDev): every kernel of a round reads nalive,
+// round and eps from it and returns at once when `done` is set, a one-thread epilogue kernel applies the host loop's
+// rules (round-0 count check, strict-improvement best, removed-count sanity, nalive update), and the host enqueues
+// rounds in growing blocks with one readback per block instead of one per round.
+#include "ga_internal.hpp"
+
+struct PeelDev {
+    unsigned long long T;          // triangles of the current alive subgraph (written by the count kernel)
+    unsigned long long removed;    // vertices removed this round (written by the mark kernel)
+    unsigned long long nalive, best_num, best_den, T_full, num, den;
+    uint32_t round, best_round, rounds, done, error, pad[3];
+};
+
+enum { PEEL_ERR_T = 1, PEEL_ERR_ARITH = 2, PEEL_ERR_REMOVED = 3 };
+
+struct ga_peel {
+    ga_ctx* ctx = nullptr;
+    ga_graph* g = nullptr;
+    uint8_t* alive = nullptr;
+    uint32_t* removed_round = nullptr;
+    uint32_t* tri = nullptr;
+    PeelDev* st = nullptr;
+    int variant = GA_TC_HASH;
+};
+
+// tri[v] * nalive * den <= 3 * (den + num) * T  (kernels.cuh:427-459, densest.h:151-154): remove v, record the round.
+template <bool FUSED>
+__global__ void k_mark_peel(const uint32_t* __restrict__ tri, uint8_t* __restrict__ alive, uint32_t* __restrict__ removed_round,
+                            uint32_t n, unsigned long long nalive_a, unsigned long long num_a, unsigned long long den_a,
+                            uint32_t round_a, PeelDev* __restrict__ st) {
+    if (FUSED && st->done) return;
+    const unsigned long long nalive = FUSED ? st->nalive : nalive_a;
+    const unsigned long long num = FUSED ? st->num : num_a;
+    const unsigned long long den = FUSED ? st->den : den_a;
+    const uint32_t round = FUSED ? st->round : round_a;
+    const unsigned long long rhs = 3ull * (den + num) * st->T;
+    const unsigned long long mul = nalive * den;
+    uint64_t local = 0;
+    // four vertices per lane: one 32-bit load of alive[] and one 128-bit load of tri[] when aligned
+    const uint32_t nq = n / 4;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+        uint32_t a4 = reinterpret_cast<const uint32_t*>(alive)[q];
+        if (!a4) continue;
+        const uint4 t4 = reinterpret_cast<const uint4*>(tri)[q];
+        const uint32_t tv[4] = {t4.x, t4.y, t4.z, t4.w};
+        uint32_t na4 = a4;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (((a4 >> (8 * k)) & 0xFF) && (unsigned long long)tv[k] * mul <= rhs) {
+                na4 &= ~(0xFFu << (8 * k));
+                removed_round[4 * q + k] = round;
+                local++;
+            }
+        }
+        if (na4 != a4) reinterpret_cast<uint32_t*>(alive)[q] = na4;
+    }
+    for (uint32_t v = nq * 4 + blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        if (alive[v] && (unsigned long long)tri[v] * mul <= rhs) {
+            alive[v] = 0;
+            removed_round[v] = round;
+            local++;
+        }
+    }
+    local = wave_sum_u64(local);
+    if (lane_id() == 0 && local) atomicAdd(&st->removed, (unsigned long long)local);
+}
+
+// Zeroes tri[] and the round counters (replaces the two cudaMemset calls of gpu_densest.cu:101-104).
+template <bool FUSED>
+__global__ void k_peel_begin(uint32_t* __restrict__ tri, uint32_t n, PeelDev* __restrict__ st) {
+    if (FUSED && st->done) return;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) tri[v] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->T = 0;
+        st->removed = 0;
+    }
+}
+
+__global__ void k_peel_init(PeelDev* st, unsigned long long n, unsigned long long T_full, unsigned long long num, unsigned long long den) {
+    st->T = 0;
+    st->removed = 0;
+    st->nalive = n;
+    st->best_num = 0;
+    st->best_den = 0;
+    st->T_full = T_full;
+    st->num = num;
+    st->den = den;
+    st->round = 0;
+    st->best_round = 0;
+    st->rounds = 0;
+    st->done = n == 0;
+    st->error = 0;
+}
+
+// Device restatement of peel_arith_ok (densest.h:65-72) for the round-0 count.
+__device__ bool peel_arith_ok_dev(unsigned long long n, unsigned long long T0, unsigned long long num, unsigned long long den) {
+    if (den < 1 || den > (1ull << 20) || num > (1ull << 20)) return false;
+    if (T0 >= (1ull << 32)) return false;
+    if (n > 0 && T0 > ~0ull / (n * den)) return false;
+    if (T0 > ~0ull / (3ull * (den + num))) return false;
+    return true;
+}
+
+// End of a fused round: the rules of batch_peel_loop (densest.h:288-301).
+__global__ void k_peel_end(PeelDev* st, unsigned long long n) {
+    if (st->done) return;
+    const unsigned long long T = st->T, removed = st->removed, nalive = st->nalive;
+    const uint32_t round = st->round;
+    if (round == 0) {
+        if (st->T_full != ~0ull && T != st->T_full) { st->error = PEEL_ERR_T; st->done = 1; return; }
+        if (!peel_arith_ok_dev(n, T, st->num, st->den)) { st->error = PEEL_ERR_ARITH; st->done = 1; return; }
+    }
+    if (st->best_den == 0 || T * st->best_den > st->best_num * nalive) {
+        st->best_num = T;
+        st->best_den = nalive;
+        st->best_round = round;
+    }
+    if (removed == 0 || removed > nalive) { st->error = PEEL_ERR_REMOVED; st->done = 1; return; }
+    st->nalive = nalive - removed;
+    st->rounds = round + 1;
+    st->round = round + 1;
+    if (st->nalive == 0) st->done = 1;
+}
+
+static unsigned vgrid(ga_ctx* ctx, uint32_t n) { return grid_for(((uint64_t)n + 3) / 4, 256, (unsigned)std::max(ctx->cus, 1) * 8u); }
+
+extern "C" int ga_peel_create(ga_ctx* ctx, ga_graph* g, ga_peel** out) {
+    *out = nullptr;
+    if (!ctx || !g) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    ga_peel* p = new ga_peel;
+    p->ctx = ctx;
+    p->g = g;
+    const size_t n = (size_t)g->n;
+    // alive padded to a multiple of 4 bytes and tri to 16 bytes for the vector loads of the mark kernel
+    if (ga_malloc(ctx, (void**)&p->alive, n + 16) || ga_malloc(ctx, (void**)&p->removed_round, (n + 1) * 4) ||
+        ga_malloc(ctx, (void**)&p->tri, (n + 4) * 4) || ga_malloc(ctx, (void**)&p->st, sizeof(PeelDev))) {
+        ga_peel_destroy(p);
+        return GA_ERR_OOM;
+    }
+    int rc = ga_tc_ensure_plan(ctx, g);
+    if (rc) { ga_peel_destroy(p); return rc; }
+    *out = p;
+    return GA_OK;
+}
+
+extern "C" void ga_peel_destroy(ga_peel* p) {
+    if (!p) return;
+    if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+    ga_free(p->alive);
+    ga_free(p->removed_round);
+    ga_free(p->tri);
+    ga_free(p->st);
+    delete p;
+}
+
+extern "C" int ga_peel_reset_state(ga_peel* p) {
+    ga_ctx* ctx = p->ctx;
+    GA_TRY(ga_fill_u8(ctx, p->alive, p->g->n, 1));
+    GA_TRY(ga_fill_u32(ctx, p->removed_round, p->g->n, UINT32_MAX));
+    return GA_OK;
+}
+
+extern "C" int ga_peel_begin_round(ga_peel* p) {
+    ga_ctx* ctx = p->ctx;
+    k_peel_begin<false><<<vgrid(ctx, p->g->n), 256, 0, ctx->stream>>>(p->tri, p->g->n, p->st);
+    return ga_launch_check(ctx, "k_peel_begin");
+}
+
+extern "C" int ga_peel_run_tc(ga_peel* p) {
+    return ga_tc_launch(p->ctx, p->g, p->variant, &p->st->T, p->alive, p->tri, nullptr, nullptr);
+}
+
+extern "C" int ga_peel_run_mark(ga_peel* p, uint64_t nalive, uint64_t num, uint64_t den, uint32_t round) {
+    ga_ctx* ctx = p->ctx;
+    k_mark_peel<false><<<vgrid(ctx, p->g->n), 256, 0, ctx->stream>>>(p->tri, p->alive, p->removed_round, p->g->n, nalive, num, den, round, p->st);
+    return ga_launch_check(ctx, "k_mark_peel");
+}
+
+extern "C" int ga_peel_read_stats(ga_peel* p, uint64_t* T, uint64_t* removed) {
+    ga_ctx* ctx = p->ctx;
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, p->st, 16, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (T) *T = ctx->hstat[0];
+    if (removed) *removed = ctx->hstat[1];
+    return GA_OK;
+}
+
+extern "C" int ga_peel_download_removed_round(ga_peel* p, uint32_t* out) {
+    ga_ctx* ctx = p->ctx;
+    if (p->g->n) GA_HIP(ctx, hipMemcpyAsync(out, p->removed_round, (size_t)p->g->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}
+
+// Host check that mirrors peel_arith_ok (densest.h:65-72).
+static const char* peel_arith_why(uint64_t n, uint64_t T0, uint64_t num, uint64_t den) {
+    if (den < 1 || den > (1ull << 20) || num > (1ull << 20)) return "eps = num/den needs 1 <= den <= 2^20 and num <= 2^20";
+    if (T0 >= (1ull << 32)) return "triangle count >= 2^32: uint32 tri[] counters could overflow";
+    if (n > 0 && T0 > UINT64_MAX / (n * den)) return "tri*nalive*den could overflow uint64";
+    if (T0 > UINT64_MAX / (3ull * (den + num))) return "3*(den+num)*T could overflow uint64";
+    return nullptr;
+}
+
+extern "C" int ga_peel_run(ga_peel* p, uint64_t T_full, uint64_t num, uint64_t den, uint64_t* stats, float* ms) {
+    ga_ctx* ctx = p->ctx;
+    ga_graph* g = p->g;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    if (T_full != UINT64_MAX) {
+        const char* why = peel_arith_why(g->n, T_full, num, den);
+        if (why) return ga_fail(ctx, GA_ERR_ARITH, "%s", why);
+    } else if (den < 1 || den > (1ull << 20) || num > (1ull << 20)) {
+        return ga_fail(ctx, GA_ERR_ARITH, "eps = num/den needs 1 <= den <= 2^20 and num <= 2^20");
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_TRY(ga_peel_reset_state(p));
+    k_peel_init<<<1, 1, 0, ctx->stream>>>(p->st, g->n, T_full, num, den);
+    GA_TRY(ga_launch_check(ctx, "k_peel_init"));
+    const uint32_t* done = &p->st->done;
+    int block_rounds = 4;
+    for (int guard = 0;; guard++) {
+        for (int r = 0; r < block_rounds; r++) {
+            k_peel_begin<true><<<vgrid(ctx, g->n), 256, 0, ctx->stream>>>(p->tri, g->n, p->st);
+            GA_TRY(ga_launch_check(ctx, "k_peel_begin"));
+            GA_TRY(ga_tc_launch(ctx, g, p->variant, &p->st->T, p->alive, p->tri, nullptr, done));
+            k_mark_peel<true><<<vgrid(ctx, g->n), 256, 0, ctx->stream>>>(p->tri, p->alive, p->removed_round, g->n, 0, 0, 0, 0, p->st);
+            GA_TRY(ga_launch_check(ctx, "k_mark_peel"));
+            k_peel_end<<<1, 1, 0, ctx->stream>>>(p->st, g->n);
+            GA_TRY(ga_launch_check(ctx, "k_peel_end"));
+        }
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, p->st, sizeof(PeelDev), hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const PeelDev* h = (const PeelDev*)ctx->hstat;
+        if (h->done) break;
+        if (guard > 1000000) return ga_fail(ctx, GA_ERR_STATE, "peel did not finish");
+        block_rounds = std::min(block_rounds * 2, 64);
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    const PeelDev* h = (const PeelDev*)ctx->hstat;
+    switch (h->error) {
+        case 0: break;
+        case PEEL_ERR_T: return ga_fail(ctx, GA_ERR_STATE, "device triangle count in round 0 differs from the given count");
+        case PEEL_ERR_ARITH: return ga_fail(ctx, GA_ERR_ARITH, "%s", peel_arith_why(g->n, h->T, num, den) ? peel_arith_why(g->n, h->T, num, den) : "arithmetic bound");
+        default: return ga_fail(ctx, GA_ERR_STATE, "bad removed count from device (0 or more than alive)");
+    }
+    if (stats) {
+        stats[0] = h->best_num;
+        stats[1] = h->best_den;
+        stats[2] = h->best_round;
+        stats[3] = h->rounds;
+    }
+    return GA_OK;
+}

@@ -1,0 +1,202 @@
+// ga_prims.hip - context, error plumbing, device memory, and the device-wide primitives the CSR builder and the
+// count plan use (LSD radix sort and exclusive scans from rocPRIM; fill and max kernels written here).
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include <cstring>
+
+#include "ga_internal.hpp"
+
+int ga_fail(ga_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int ga_malloc(ga_ctx* ctx, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return ga_fail(ctx, GA_ERR_OOM, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    return GA_OK;
+}
+
+void ga_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int ga_scratch(ga_ctx* ctx, size_t bytes, void** p) {
+    if (bytes > ctx->scratch_bytes) {
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ga_free(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        size_t want = bytes + bytes / 4 + 4096;
+        GA_TRY(ga_malloc(ctx, &ctx->scratch, want));
+        ctx->scratch_bytes = want;
+    }
+    *p = ctx->scratch;
+    return GA_OK;
+}
+
+int ga_launch_check(ga_ctx* ctx, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ga_fail(ctx, GA_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+    return GA_OK;
+}
+
+extern "C" int ga_ctx_create(int device, ga_ctx** out) {
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return GA_ERR_HIP;
+    if (device < 0 || device >= count) return GA_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return GA_ERR_HIP;
+    ga_ctx* c = new ga_ctx;
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return GA_ERR_HIP; }
+    c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipHostMalloc((void**)&c->hstat, 512, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&c->dsmall, 512) != hipSuccess) {
+        ga_ctx_destroy(c);
+        return GA_ERR_HIP;
+    }
+    *out = c;
+    return GA_OK;
+}
+
+extern "C" void ga_ctx_destroy(ga_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ga_free(c->scratch);
+    ga_free(c->dsmall);
+    if (c->hstat) (void)hipHostFree(c->hstat);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* ga_ctx_last_error(const ga_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int ga_ctx_sync(ga_ctx* ctx) {
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}
+
+extern "C" int ga_ctx_device_info(ga_ctx* ctx, char* name, int len, int* cus, int* wave, uint64_t* hbm_bytes) {
+    hipDeviceProp_t prop;
+    GA_HIP(ctx, hipGetDeviceProperties(&prop, ctx->device));
+    if (name && len > 0) {
+        snprintf(name, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cus) *cus = prop.multiProcessorCount;
+    if (wave) *wave = prop.warpSize;
+    if (hbm_bytes) *hbm_bytes = (uint64_t)prop.totalGlobalMem;
+    return GA_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// sorts and scans (rocPRIM device primitives, temporary storage from the context scratch)
+// ---------------------------------------------------------------------------------------------------------------
+int ga_sort_u64(ga_ctx* ctx, uint64_t* keys, uint64_t* tmp, size_t count, int end_bit, uint64_t** result) {
+    *result = keys;
+    if (count <= 1) return GA_OK;
+    rocprim::double_buffer<uint64_t> db(keys, tmp);
+    size_t bytes = 0;
+    GA_HIP(ctx, rocprim::radix_sort_keys(nullptr, bytes, db, count, 0, end_bit, ctx->stream));
+    void* s = nullptr;
+    GA_TRY(ga_scratch(ctx, bytes, &s));
+    GA_HIP(ctx, rocprim::radix_sort_keys(s, bytes, db, count, 0, end_bit, ctx->stream));
+    *result = db.current();
+    return GA_OK;
+}
+
+int ga_sort_pairs_u32(ga_ctx* ctx, uint32_t* keys, uint32_t* keys_tmp, uint32_t* vals, uint32_t* vals_tmp, size_t count,
+                      int end_bit, uint32_t** keys_out, uint32_t** vals_out) {
+    *keys_out = keys;
+    *vals_out = vals;
+    if (count <= 1) return GA_OK;
+    rocprim::double_buffer<uint32_t> kb(keys, keys_tmp), vb(vals, vals_tmp);
+    size_t bytes = 0;
+    GA_HIP(ctx, rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, count, 0, end_bit, ctx->stream));
+    void* s = nullptr;
+    GA_TRY(ga_scratch(ctx, bytes, &s));
+    GA_HIP(ctx, rocprim::radix_sort_pairs(s, bytes, kb, vb, count, 0, end_bit, ctx->stream));
+    *keys_out = kb.current();
+    *vals_out = vb.current();
+    return GA_OK;
+}
+
+int ga_exclusive_scan_u32(ga_ctx* ctx, const uint32_t* in, uint32_t* out, size_t count) {
+    if (count == 0) return GA_OK;
+    size_t bytes = 0;
+    GA_HIP(ctx, rocprim::exclusive_scan(nullptr, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), ctx->stream));
+    void* s = nullptr;
+    GA_TRY(ga_scratch(ctx, bytes, &s));
+    GA_HIP(ctx, rocprim::exclusive_scan(s, bytes, in, out, 0u, count, rocprim::plus<uint32_t>(), ctx->stream));
+    return GA_OK;
+}
+
+int ga_exclusive_scan_u64(ga_ctx* ctx, const uint64_t* in, uint64_t* out, size_t count) {
+    if (count == 0) return GA_OK;
+    size_t bytes = 0;
+    GA_HIP(ctx, rocprim::exclusive_scan(nullptr, bytes, in, out, (uint64_t)0, count, rocprim::plus<uint64_t>(), ctx->stream));
+    void* s = nullptr;
+    GA_TRY(ga_scratch(ctx, bytes, &s));
+    GA_HIP(ctx, rocprim::exclusive_scan(s, bytes, in, out, (uint64_t)0, count, rocprim::plus<uint64_t>(), ctx->stream));
+    return GA_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// fills (used instead of hipMemsetAsync so every clear is an ordinary kernel node) and a max reduction
+// ---------------------------------------------------------------------------------------------------------------
+template <class T>
+__global__ void k_fill(T* __restrict__ p, size_t count, T v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <class T>
+static int fill_impl(ga_ctx* ctx, T* p, size_t count, T v) {
+    if (count == 0) return GA_OK;
+    k_fill<T><<<grid_for(count, 256, 4096), 256, 0, ctx->stream>>>(p, count, v);
+    return ga_launch_check(ctx, "k_fill");
+}
+
+int ga_fill_u32(ga_ctx* ctx, uint32_t* p, size_t count, uint32_t v) { return fill_impl(ctx, p, count, v); }
+int ga_fill_u8(ga_ctx* ctx, uint8_t* p, size_t count, uint8_t v) { return fill_impl(ctx, p, count, v); }
+int ga_fill_u64(ga_ctx* ctx, uint64_t* p, size_t count, uint64_t v) { return fill_impl(ctx, p, count, v); }
+
+__global__ void k_max_u32(const uint32_t* __restrict__ p, size_t count, uint32_t* __restrict__ out) {
+    uint32_t mx = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        mx = p[i] > mx ? p[i] : mx;
+    mx = wave_max_u32(mx);
+    if (lane_id() == 0 && mx) atomicMax(out, mx);
+}
+
+int ga_max_u32(ga_ctx* ctx, const uint32_t* p, size_t count, uint32_t* out_host) {
+    *out_host = 0;
+    if (count == 0) return GA_OK;
+    uint32_t* d = (uint32_t*)(ctx->dsmall + 63);
+    GA_TRY(ga_fill_u32(ctx, d, 1, 0));
+    k_max_u32<<<grid_for(count, 256, 1024), 256, 0, ctx->stream>>>(p, count, d);
+    GA_TRY(ga_launch_check(ctx, "k_max_u32"));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, d, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out_host = (uint32_t)(ctx->hstat[0] & 0xFFFFFFFFu);
+    return GA_OK;
+}

@@ -1,0 +1,129 @@
+"""Seeded synthetic graphs for the five workloads (libga_gen.so, host C++ with OpenMP).
+
+Every generator returns a raw (k, 2) uint32 pair array: duplicates, both orientations and self loops may occur; the
+CSR builder canonicalises them exactly like the reference's text loader (graph.h:71-96). Outputs do not depend on
+the OpenMP thread count.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import GEN_PATH
+
+
+class _Edges(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.POINTER(ctypes.c_uint32)), ("count", ctypes.c_uint64), ("n", ctypes.c_uint32)]
+
+
+_gl = None
+
+
+def _gen():
+    global _gl
+    if _gl is None:
+        if not os.path.exists(GEN_PATH):
+            raise RuntimeError(f"{GEN_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(GEN_PATH)
+        E = ctypes.POINTER(_Edges)
+        L.ga_gen_chung_lu.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, E]
+        L.ga_gen_chung_lu_exact.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, E]
+        L.ga_gen_chung_lu_offset.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]
+        L.ga_gen_chung_lu_offset.restype = ctypes.c_double
+        L.ga_gen_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, E]
+        L.ga_gen_er.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, E]
+        L.ga_gen_communities.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, E]
+        L.ga_gen_free.argtypes = [E]
+        L.ga_gen_free.restype = None
+        _gl = L
+    return _gl
+
+
+def _take(fn, *args) -> np.ndarray:
+    e = _Edges()
+    rc = fn(*args, ctypes.byref(e))
+    if rc != 0:
+        raise ValueError(f"generator failed with {rc}")
+    try:
+        if e.count == 0:
+            return np.zeros((0, 2), np.uint32)
+        a = np.ctypeslib.as_array(e.pairs, shape=(int(e.count) * 2,)).copy()
+        return a.reshape(-1, 2)
+    finally:
+        _gen().ga_gen_free(ctypes.byref(e))
+
+
+def chung_lu(n: int, samples: int, gamma: float, i0: float = 0.0, seed: int = 1, permute: bool = True) -> np.ndarray:
+    return _take(_gen().ga_gen_chung_lu, n, samples, gamma, i0, seed, 1 if permute else 0)
+
+
+def chung_lu_exact(n: int, m: int, gamma: float, i0: float = 0.0, seed: int = 1) -> np.ndarray:
+    """Exactly m unique loop-free undirected edges."""
+    return _take(_gen().ga_gen_chung_lu_exact, n, m, gamma, i0, seed)
+
+
+def chung_lu_offset(n: int, m: int, gamma: float, dmax: float) -> float:
+    return _gen().ga_gen_chung_lu_offset(n, m, gamma, dmax)
+
+
+def rmat(scale: int, samples: int, a=0.57, b=0.19, c=0.19, seed: int = 1) -> np.ndarray:
+    return _take(_gen().ga_gen_rmat, scale, samples, a, b, c, seed)
+
+
+def erdos_renyi(n: int, samples: int, seed: int = 1) -> np.ndarray:
+    return _take(_gen().ga_gen_er, n, samples, seed)
+
+
+def communities(n: int, count: int, smin: int, smax: int, p: float, seed: int = 1) -> np.ndarray:
+    return _take(_gen().ga_gen_communities, n, count, smin, smax, p, seed)
+
+
+# ---- the named workloads ----
+
+def powerlaw_small(seed: int = 2101) -> np.ndarray:
+    """Chung-Lu n=8,298, exactly m=100,762 unique edges, exponent 2.1, weight offset for max expected degree ~1,000."""
+    n, m = 8298, 100762
+    i0 = chung_lu_offset(n, m, 2.1, 1000.0)
+    return chung_lu_exact(n, m, 2.1, i0, seed)
+
+
+def rmat21(seed: int = 2102) -> np.ndarray:
+    """R-MAT scale 21, (0.57, 0.19, 0.19, 0.05), edge factor 8: 16,777,216 raw samples."""
+    return rmat(21, 8 << 21, 0.57, 0.19, 0.19, seed)
+
+
+def powerlaw_communities(seed: int = 2103, n: int = 4_000_000, background: int = 16_000_000, count: int = 20_000) -> np.ndarray:
+    """Chung-Lu n=4.0M, exponent 2.3 (max expected degree capped near 20k by a weight offset) plus `count` planted
+    communities of 10-200 vertices (log-uniform sizes) with intra-community edge probability 0.3. The background
+    sample count is chosen so the de-duplicated total lands near 34.7M edges."""
+    i0 = chung_lu_offset(n, background, 2.3, 20000.0)
+    bg = chung_lu(n, background, 2.3, i0, seed)
+    cm = communities(n, count, 10, 200, 0.3, seed + 1)
+    return np.concatenate([bg, cm])
+
+
+def batch_mix_specs(count: int = 4096, seed: int = 2104, nmin: int = 1000, nmax: int = 200_000):
+    """Per-graph parameters of the mix: n log-uniform in [nmin, nmax], mean degree uniform in [4, 64], half
+    Erdos-Renyi and half Chung-Lu with exponent uniform in [2.0, 2.5]."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        deg = float(rng.uniform(4, 64))
+        kind = "er" if i % 2 == 0 else "cl"
+        gamma = float(rng.uniform(2.0, 2.5))
+        specs.append({"n": n, "deg": deg, "kind": kind, "gamma": gamma, "seed": seed * 100003 + i})
+    return specs
+
+
+def batch_graph(spec) -> np.ndarray:
+    n, deg = spec["n"], spec["deg"]
+    samples = int(n * deg / 2)
+    if spec["kind"] == "er":
+        return erdos_renyi(n, samples, spec["seed"])
+    # cap the heaviest expected degree at sqrt(n * deg) * 4 to keep counts within uint32 tri[] counters
+    i0 = chung_lu_offset(n, samples, spec["gamma"], min(n / 2.0, 4.0 * math.sqrt(n * deg)))
+    return chung_lu(n, samples, spec["gamma"], i0, spec["seed"])

@@ -1,0 +1,188 @@
+// dsg.h (graph_analyzer_amd) - drop-in for the (k,triangle)-core part of the reference's dsg.h API (dsg.h:34-58,
+// 647-751): Dens and its exact comparisons, dsg_row, CoreLoopResult, the backend-templated level loops
+// core_decomp_loop / core_decomp_loop_inc, the GPU backends of gpu_dsg.cu:88-199 over the C ABI, a fused gpu_core_run,
+// and core_level_density on the device. The paper pipeline (Greedy++, Exact, CoreExact) is not part of this library.
+#pragma once
+#include "densest.h"
+
+typedef unsigned __int128 u128;
+
+struct Dens {
+    uint64_t num = 0, den = 0;   // triangles / vertices; den == 0 = no subgraph (density 0)
+};
+inline int dens_cmp(Dens a, Dens b) {
+    if (a.den == 0) a = Dens{0, 1};
+    if (b.den == 0) b = Dens{0, 1};
+    const u128 l = (u128)a.num * b.den, r = (u128)b.num * a.den;
+    return l < r ? -1 : (l > r ? 1 : 0);
+}
+inline bool dens_gt(Dens a, Dens b) { return dens_cmp(a, b) > 0; }
+inline uint64_t dens_ceil(Dens d) { return d.den == 0 ? 0 : (d.num + d.den - 1) / d.den; }
+inline double dens_value(Dens d) { return density_value(d.num, d.den); }
+
+// impl,graph,n,m,triangles,vertices,density,time_ms,h2d_ms,total_ms,rounds,kmax (dsg.h:650-663)
+inline std::string dsg_row(const char* impl, const char* graph, uint32_t n, uint32_t m, uint64_t tri, uint64_t verts, double time_ms,
+                           double h2d_ms, uint64_t rounds, uint64_t kmax) {
+    char buf[1024];
+    snprintf(buf, sizeof buf, "%s,%s,%u,%u,%llu,%llu,%.6f,%.3f,%.3f,%.3f,%llu,%llu", impl, graph, n, m, (unsigned long long)tri,
+             (unsigned long long)verts, density_value(tri, verts), time_ms, h2d_ms, time_ms + h2d_ms, (unsigned long long)rounds,
+             (unsigned long long)kmax);
+    return buf;
+}
+
+struct CoreLoopResult {
+    uint32_t rounds = 0, kmax = 0;
+    bool ok = true;
+    std::string error;
+};
+
+// Recount level loop (dsg.h:694-717 semantics): per round begin_round, run_tc, run_min, run_peel(k),
+// read_stats(T, removed, k_new); round-0 T == T_full; removed in 1..nalive and the level never decreases.
+template <class B>
+inline CoreLoopResult core_decomp_loop(uint32_t n, uint64_t T_full, B& be, bool verbose = false, FILE* diag = nullptr) {
+    CoreLoopResult r;
+    if (T_full >= (1ull << 32)) { r.ok = false; r.error = "triangle count >= 2^32: uint32 tri[] counters could overflow"; return r; }
+    be.reset_state();
+    uint64_t nalive = n, k = 0;
+    for (uint32_t round = 0; nalive > 0; round++) {
+        be.begin_round();
+        be.run_tc();
+        be.run_min();
+        be.run_peel(k);
+        uint64_t T = 0, removed = 0, k_new = 0;
+        be.read_stats(T, removed, k_new);
+        if (round == 0 && T != T_full) { r.ok = false; r.error = "device triangle count in round 0 differs from CPU count"; return r; }
+        if (verbose && diag)
+            fprintf(diag, "round %6u  alive=%10llu  T=%12llu  level k=%llu  removed=%10llu\n", round, (unsigned long long)nalive,
+                    (unsigned long long)T, (unsigned long long)k_new, (unsigned long long)removed);
+        if (removed == 0 || removed > nalive || k_new < k) {
+            r.ok = false;
+            r.error = "bad stats from device (removed 0, removed > alive, or level decreased)";
+            return r;
+        }
+        nalive -= removed;
+        k = k_new;
+        r.rounds = round + 1;
+    }
+    r.kmax = (uint32_t)k;
+    return r;
+}
+
+// Incremental level loop (dsg.h:725-751 semantics): init_tri(T) once, then per round begin_round, run_min,
+// run_find(k), run_update, run_retire, read_stats(removed, k_new).
+template <class B>
+inline CoreLoopResult core_decomp_loop_inc(uint32_t n, uint64_t T_full, B& be, bool verbose = false, FILE* diag = nullptr) {
+    CoreLoopResult r;
+    if (T_full >= (1ull << 32)) { r.ok = false; r.error = "triangle count >= 2^32: uint32 tri[] counters could overflow"; return r; }
+    be.reset_state();
+    uint64_t T0 = 0;
+    be.init_tri(T0);
+    if (T0 != T_full) { r.ok = false; r.error = "device triangle count differs from CPU count"; return r; }
+    uint64_t nalive = n, k = 0;
+    for (uint32_t round = 0; nalive > 0; round++) {
+        be.begin_round();
+        be.run_min();
+        be.run_find(k);
+        be.run_update();
+        be.run_retire();
+        uint64_t removed = 0, k_new = 0;
+        be.read_stats(removed, k_new);
+        if (verbose && diag)
+            fprintf(diag, "round %6u  alive=%10llu  level k=%llu  removed=%10llu\n", round, (unsigned long long)nalive,
+                    (unsigned long long)k_new, (unsigned long long)removed);
+        if (removed == 0 || removed > nalive || k_new < k) {
+            r.ok = false;
+            r.error = "bad stats from device (removed 0, removed > alive, or level decreased)";
+            return r;
+        }
+        nalive -= removed;
+        k = k_new;
+        r.rounds = round + 1;
+    }
+    r.kmax = (uint32_t)k;
+    return r;
+}
+
+// Recount backend (gpu_dsg.cu:88-134) over ga_core_* with incremental = 0.
+struct GpuCoreBackend {
+    ga_ctx* ctx = nullptr;
+    ga_core* c = nullptr;
+    uint32_t n = 0;
+    void init(const gaa::DeviceGraph& g) {
+        ctx = g.ctx();
+        n = g.n();
+        gaa::check(ctx, ga_core_create(ctx, g.handle(), 0, &c), "ga_core_create");
+    }
+    ~GpuCoreBackend() { if (c) ga_core_destroy(c); }
+    void reset_state() { gaa::check(ctx, ga_core_reset_state(c), "ga_core_reset_state"); }
+    void begin_round() { gaa::check(ctx, ga_core_begin_round(c), "ga_core_begin_round"); }
+    void run_tc() { gaa::check(ctx, ga_core_run_tc(c), "ga_core_run_tc"); }
+    void run_min() { gaa::check(ctx, ga_core_run_min(c), "ga_core_run_min"); }
+    void run_peel(uint64_t k) { gaa::check(ctx, ga_core_run_peel(c, k), "ga_core_run_peel"); }
+    void read_stats(uint64_t& T, uint64_t& removed, uint64_t& k) { gaa::check(ctx, ga_core_read_stats(c, &T, &removed, &k), "ga_core_read_stats"); }
+    void download_core(std::vector<uint32_t>& out) {
+        out.assign(n, 0);
+        gaa::check(ctx, ga_core_download(c, n ? out.data() : nullptr), "ga_core_download");
+    }
+};
+
+// Incremental backend (gpu_dsg.cu:138-199) over ga_core_* with incremental = 1.
+struct GpuCoreBackendInc {
+    ga_ctx* ctx = nullptr;
+    ga_core* c = nullptr;
+    uint32_t n = 0;
+    void init(const gaa::DeviceGraph& g) {
+        ctx = g.ctx();
+        n = g.n();
+        gaa::check(ctx, ga_core_create(ctx, g.handle(), 1, &c), "ga_core_create");
+    }
+    ~GpuCoreBackendInc() { if (c) ga_core_destroy(c); }
+    void reset_state() { gaa::check(ctx, ga_core_reset_state(c), "ga_core_reset_state"); }
+    void init_tri(uint64_t& T) { gaa::check(ctx, ga_core_init_tri(c, &T), "ga_core_init_tri"); }
+    void begin_round() { gaa::check(ctx, ga_core_begin_round(c), "ga_core_begin_round"); }
+    void run_min() { gaa::check(ctx, ga_core_run_min(c), "ga_core_run_min"); }
+    void run_find(uint64_t k) { gaa::check(ctx, ga_core_run_find(c, k), "ga_core_run_find"); }
+    void run_update() { gaa::check(ctx, ga_core_run_update(c), "ga_core_run_update"); }
+    void run_retire() { gaa::check(ctx, ga_core_run_retire(c), "ga_core_run_retire"); }
+    void read_stats(uint64_t& removed, uint64_t& k) {
+        uint64_t T = 0;
+        gaa::check(ctx, ga_core_read_stats(c, &T, &removed, &k), "ga_core_read_stats");
+    }
+    void download_core(std::vector<uint32_t>& out) {
+        out.assign(n, 0);
+        gaa::check(ctx, ga_core_download(c, n ? out.data() : nullptr), "ga_core_download");
+    }
+};
+
+// Whole incremental decomposition with the level loop on the device (ga_core_run).
+inline CoreLoopResult gpu_core_run(const gaa::DeviceGraph& g, uint64_t T_full, std::vector<uint32_t>& core, float* ms = nullptr) {
+    CoreLoopResult r;
+    ga_core* c = nullptr;
+    gaa::check(g.ctx(), ga_core_create(g.ctx(), g.handle(), 1, &c), "ga_core_create");
+    uint32_t st[2] = {0, 0};
+    float t = 0;
+    const int rc = ga_core_run(c, T_full, st, &t);
+    if (rc != GA_OK) {
+        r.ok = false;
+        r.error = ga_ctx_last_error(g.ctx());
+    } else {
+        r.rounds = st[0];
+        r.kmax = st[1];
+        core.assign(g.n(), 0);
+        gaa::check(g.ctx(), ga_core_download(c, g.n() ? core.data() : nullptr), "ga_core_download");
+    }
+    ga_core_destroy(c);
+    if (ms) *ms = t;
+    return r;
+}
+
+// Triangles and vertices of {v : core[v] >= k} (dsg.h:169-174), counted on the device.
+inline Dens core_level_density(const gaa::DeviceGraph& g, const std::vector<uint32_t>& core, uint32_t k) {
+    std::vector<uint8_t> in(g.n());
+    uint64_t cnt = 0;
+    for (uint32_t v = 0; v < g.n(); v++) { in[v] = core[v] >= k; cnt += in[v]; }
+    std::vector<uint32_t> tri(std::max<uint32_t>(g.n(), 1));
+    uint64_t T = 0;
+    gaa::check(g.ctx(), ga_tc_vertex(g.ctx(), g.handle(), GA_TC_AUTO, g.n() ? in.data() : nullptr, tri.data(), &T), "ga_tc_vertex");
+    return Dens{T, cnt};
+}

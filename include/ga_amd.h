@@ -1,0 +1,137 @@
+/* ga_amd.h - C ABI of the MI355X-native triangle-count / triangle-density peeling / (k,triangle)-core library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an int status (GA_OK = 0). Host
+ * buffers are caller-owned with explicit lengths; device memory is owned by the handles and freed by *_destroy.
+ * A context owns one HIP stream on one device; a handle is used by one host thread at a time.
+ *
+ * Each declaration names the reference interface it replaces (file:line in graph_analyzer's src/). The reference binds
+ * these as C++ functions and backend methods; the C++ headers in graph_analyzer_amd/host/ restate that API on top of
+ * this ABI. */
+#ifndef GA_AMD_H
+#define GA_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GA_OK = 0,
+    GA_ERR_ARG = 1,    /* bad argument (sizes, ids >= n, duplicate pairs where unique pairs are required, ...) */
+    GA_ERR_HIP = 2,    /* HIP runtime error; the message names the call */
+    GA_ERR_OOM = 3,    /* device allocation failed */
+    GA_ERR_ARITH = 4,  /* an exact-arithmetic bound does not hold (peel_arith_ok, T >= 2^32 for u32 counters) */
+    GA_ERR_STATE = 5,  /* device-side sanity check failed (removed 0 / more than alive, level decreased, T mismatch) */
+};
+
+typedef struct ga_ctx ga_ctx;       /* device + stream + scratch */
+typedef struct ga_graph ga_graph;   /* oriented CSR resident in HBM (+ optional undirected CSR, count plan) */
+typedef struct ga_peel ga_peel;     /* batch-peeling state of one graph */
+typedef struct ga_core ga_core;     /* (k,triangle)-core state of one graph */
+typedef struct ga_batch ga_batch;   /* many graphs resident together, counted / peeled in shared launches */
+
+/* ---- context ------------------------------------------------------------------------------------------------ */
+/* Replaces the driver set-up of gpu_tc.cu:83-85 (device properties) and the implicit default stream. */
+int ga_ctx_create(int device, ga_ctx** out);
+void ga_ctx_destroy(ga_ctx* ctx);
+const char* ga_ctx_last_error(const ga_ctx* ctx);
+int ga_ctx_sync(ga_ctx* ctx);
+/* name (NUL-terminated, truncated to len), compute units, warp (wave) size, total HBM bytes */
+int ga_ctx_device_info(ga_ctx* ctx, char* name, int len, int* cus, int* wave, uint64_t* hbm_bytes);
+
+/* ---- graphs (graph.h:19-96, densest.h:162-171) -------------------------------------------------------------- */
+/* n < 0: load_graph semantics on a raw pair list (graph.h:71-96 after the text parse): self loops dropped, pairs
+ *        canonicalised to (min,max) and de-duplicated, n = max_id + 1 (1 for an empty list).
+ * n >= 0: build_graph_from_edges semantics (graph.h:38-67): pairs must be unique, loop-free, ids < n; violations
+ *        return GA_ERR_ARG.
+ * Both: (degree, id) orientation, rows sorted ascending, src[] and max_out_degree, all on the GPU.
+ * build_ms (may be NULL): device time of the build, excluding the host-to-device copy of the pairs. */
+int ga_graph_from_pairs(ga_ctx* ctx, const uint32_t* pairs, uint64_t npairs, int64_t n, ga_graph** out, float* build_ms);
+/* Upload of an oriented CSR built elsewhere (gpu_tc.cu:42-62 upload of rowptr/col/src); src is derived on device. */
+int ga_graph_from_csr(ga_ctx* ctx, uint32_t n, uint32_t m, const uint32_t* rowptr, const uint32_t* col, ga_graph** out);
+void ga_graph_destroy(ga_graph* g);
+int ga_graph_info(const ga_graph* g, uint32_t* n, uint32_t* m, uint32_t* max_out_degree);
+/* Download of the Graph fields (graph.h:19-26); any pointer may be NULL. Sizes n+1, m, m. */
+int ga_graph_download(ga_ctx* ctx, const ga_graph* g, uint32_t* rowptr, uint32_t* col, uint32_t* src);
+/* Both-direction sorted adjacency on the device (build_undirected_csr, densest.h:163-171); optional download of
+ * urow (n+1) and ucol (2m). */
+int ga_graph_undirected(ga_ctx* ctx, ga_graph* g, uint32_t* urow, uint32_t* ucol);
+/* Work order of edges by floor(log2(min(d+u,d+v)+1)), heaviest bucket first, stable (build_edge_order,
+ * graph.h:156-171), computed on the device; perm has m entries. */
+int ga_graph_edge_order(ga_ctx* ctx, ga_graph* g, uint32_t* perm);
+
+/* ---- triangle counting (kernels.cuh:52-358, gpu_tc.cu:128-228) --------------------------------------------- */
+enum {
+    GA_TC_AUTO = 0,        /* the tuned default (currently GA_TC_HASH) */
+    GA_TC_MERGE = 1,       /* one lane per oriented edge, two-pointer merge (role of v1) */
+    GA_TC_GROUP4 = 2,      /* 4 lanes per edge, strided binary search of the shorter list (role of v5) */
+    GA_TC_GROUP8 = 3,
+    GA_TC_GROUP16 = 4,
+    GA_TC_GROUP64 = 5,     /* a full wave per edge (role of v2) */
+    GA_TC_HASH = 6,        /* longer list of each edge in an LDS hash set, shorter list streamed by lane groups
+                              sized per degree bucket (roles of v3/v8 generalised, v4/v9 work ordering on device) */
+    GA_TC_SORTED = 7,      /* one lane per edge in the device-built work order of ga_graph_edge_order (role of v9) */
+    GA_TC_NVARIANTS = 8
+};
+const char* ga_tc_variant_name(int variant);
+/* T = sum over oriented edges (u,v) of |N+(u) ∩ N+(v)| (graph.h:110-121). ms (may be NULL): device time of the
+ * counting launch(es), measured with HIP events on the context stream. Preparation of the count plan (GA_TC_HASH,
+ * GA_TC_SORTED) happens on first use and is excluded; call ga_tc_prepare to time it. */
+int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, float* ms);
+/* Builds the device-side count plan (hash-side work lists, degree buckets, work order); ms = its device time. */
+int ga_tc_prepare(ga_ctx* ctx, ga_graph* g, float* ms);
+/* Lane-group width / hash capacity / block shape per degree bucket (the autotuner's output, see ga_tc_tune). */
+typedef struct ga_tc_tuning {
+    int32_t lanes[4];      /* lanes per work item for hash buckets 0..3 (each one of 4, 8, 16, 32, 64) */
+    int32_t block;         /* threads per workgroup (64..1024, multiple of 64) */
+    int32_t work_chunk;    /* probe elements per work item before a hub's list is split */
+    int32_t load_shift;    /* hash table slots = next_pow2(d+(h)) << load_shift (1..3) */
+    int32_t hash_cap;      /* largest d+(h) kept in an LDS table; larger rows use binary search in HBM */
+} ga_tc_tuning;
+int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
+int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t);
+/* Times candidate tunings on this graph (each verified against the first count) and keeps the fastest. */
+int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* best_ms);
+/* variant: GA_TC_MERGE or GA_TC_HASH (GA_TC_AUTO = GA_TC_HASH) for the two calls below.
+ * Per-vertex triangle participation in the subgraph induced by alive (n bytes; NULL = all alive):
+ * cpu_triangle_participation (densest.h:81-101) / tc_per_vertex_alive (kernels.cuh:372-425). tri has n entries. */
+int ga_tc_vertex(ga_ctx* ctx, ga_graph* g, int variant, const uint8_t* alive, uint32_t* tri, uint64_t* T);
+/* Per-oriented-edge triangle support: sup[e] = triangles containing edge e (indices of for_each_triangle,
+ * dsg.h:62-79). sup has m entries. */
+int ga_tc_edge_support(ga_ctx* ctx, ga_graph* g, int variant, uint32_t* sup, uint64_t* T);
+
+/* ---- batch peeling: the backend concept of batch_peel_loop (densest.h:265-303, gpu_densest.cu:70-126) ------- */
+int ga_peel_create(ga_ctx* ctx, ga_graph* g, ga_peel** out);
+void ga_peel_destroy(ga_peel* p);
+int ga_peel_reset_state(ga_peel* p);                 /* alive = 1, removed_round = UINT32_MAX */
+int ga_peel_begin_round(ga_peel* p);                 /* tri = 0, counters = 0 (fill kernels, no memset) */
+int ga_peel_run_tc(ga_peel* p);                      /* participation of the alive subgraph */
+int ga_peel_run_mark(ga_peel* p, uint64_t nalive, uint64_t num, uint64_t den, uint32_t round);
+int ga_peel_read_stats(ga_peel* p, uint64_t* T, uint64_t* removed);   /* the one host sync per round */
+int ga_peel_download_removed_round(ga_peel* p, uint32_t* removed_round);
+/* Whole peel with the round loop on the device (rounds enqueued in blocks, one host sync per block).
+ * stats = {best_num, best_den, best_round, rounds}; T_full = the caller's full count for the round-0 check
+ * (UINT64_MAX = skip the check). Errors as batch_peel_loop: GA_ERR_ARITH, GA_ERR_STATE. */
+int ga_peel_run(ga_peel* p, uint64_t T_full, uint64_t num, uint64_t den, uint64_t* stats, float* ms);
+
+/* ---- (k,triangle)-core: backends of core_decomp_loop / core_decomp_loop_inc (dsg.h:665-751, gpu_dsg.cu:88-199) - */
+int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core** out);
+void ga_core_destroy(ga_core* c);
+int ga_core_reset_state(ga_core* c);                 /* alive/state = 1, core = UINT32_MAX */
+int ga_core_begin_round(ga_core* c);
+int ga_core_run_tc(ga_core* c);                      /* recount only */
+int ga_core_run_min(ga_core* c);
+int ga_core_run_peel(ga_core* c, uint64_t k);        /* recount only */
+int ga_core_read_stats(ga_core* c, uint64_t* T, uint64_t* removed, uint64_t* k);
+int ga_core_init_tri(ga_core* c, uint64_t* T);       /* incremental only: the one full count */
+int ga_core_run_find(ga_core* c, uint64_t k);        /* incremental only */
+int ga_core_run_update(ga_core* c);                  /* incremental only */
+int ga_core_run_retire(ga_core* c);                  /* incremental only */
+int ga_core_download(ga_core* c, uint32_t* core);
+/* Whole decomposition with the level loop on the device; stats = {rounds, kmax}. T_full as in ga_peel_run. */
+int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

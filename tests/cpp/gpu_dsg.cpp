@@ -1,0 +1,110 @@
+// gpu_dsg.cpp - (k,triangle)-core driver with the core part of the reference's contract (gpu_dsg.cu:1-46, 201-316):
+//   usage: gpu_dsg graph.txt [--repeats N] [--with-recount] [--no-burnin] [--verbose]
+//   rows impl,graph,n,m,triangles,vertices,density,time_ms,h2d_ms,total_ms,rounds,kmax for cpu_core_decomp (Alg 3),
+//   gpu_core_decomp (incremental host loop, one sync per round), gpu_core_decomp_fused (ga_core_run) and, with
+//   --with-recount, gpu_core_decomp_recount. Rows = the kmax-core. Printed only if every GPU core array equals
+//   CPU Alg 3 element-wise and the round counts of all GPU designs agree; exit 1 otherwise.
+// The paper pipeline after the decomposition (greedy, pruning, CoreExact) is not part of this build.
+// Integration test: links the CPU oracle as the checker and as the CPU row.
+#include "../../graph_analyzer_amd/host/dsg.h"
+#include "../../oracle/oracle.hpp"
+
+int main(int argc, char** argv) {
+    std::vector<const char*> pos;
+    bool recount = false, burnin = true, verbose = false;
+    int repeats = 5;
+    for (int i = 1; i < argc; i++) {
+        if (strcmp(argv[i], "--with-recount") == 0) recount = true;
+        else if (strcmp(argv[i], "--no-burnin") == 0) burnin = false;
+        else if (strcmp(argv[i], "--verbose") == 0) verbose = true;
+        else if (strcmp(argv[i], "--repeats") == 0 && i + 1 < argc) repeats = std::max(1, atoi(argv[++i]));
+        else if (argv[i][0] == '-') {}
+        else pos.push_back(argv[i]);
+    }
+    if (pos.size() != 1) { fprintf(stderr, "usage: %s graph.txt [--repeats N] [--with-recount] [--no-burnin] [--verbose]\n", argv[0]); return 2; }
+    const char* path = pos[0];
+    ga_ctx* ctx = gaa::default_ctx();
+    std::vector<uint32_t> pairs;
+    if (!gaa::parse_edge_file(path, pairs)) return 1;
+    orc::Graph og;
+    orc::graph_from_pairs(pairs.data(), pairs.size() / 2, og);
+    fprintf(stderr, "loaded n=%u m=%u max_out_deg=%u\n", og.n, og.m, og.max_out_degree);
+    const uint64_t T_full = orc::triangles_serial(og);
+
+    std::vector<std::string> rows;
+    std::vector<uint32_t> core_cpu;
+    uint32_t kmax = 0;
+    double t0 = now_ms();
+    if (!orc::core_decomposition(og, core_cpu, &kmax)) { fprintf(stderr, "core number does not fit uint32\n"); return 1; }
+    const double t_cpu = now_ms() - t0;
+    uint64_t ktri = 0, kverts = 0;
+    orc::core_level_density(og, core_cpu, kmax, ktri, kverts);
+    rows.push_back(dsg_row("cpu_core_decomp", path, og.n, og.m, ktri, kverts, t_cpu, 0, 0, kmax));
+
+    gaa::DeviceGraph dg = gaa::DeviceGraph::from_pairs(ctx, pairs.data(), pairs.size() / 2, -1);
+    GpuCoreBackendInc bi;
+    bi.init(dg);
+    GpuCoreBackend br;
+    if (recount) br.init(dg);
+    if (burnin) {
+        bi.reset_state();
+        const double tb = now_ms();
+        uint64_t Tb = 0;
+        do { bi.init_tri(Tb); } while (now_ms() - tb < 1500.0);
+    }
+    bool ok = true;
+    std::vector<uint32_t> core_gpu;
+    auto check = [&](const char* what, const CoreLoopResult& r) {
+        if (!r.ok) { fprintf(stderr, "%s error: %s\n", what, r.error.c_str()); ok = false; return; }
+        if (core_gpu != core_cpu || r.kmax != kmax) {
+            size_t bad = 0;
+            while (bad < core_gpu.size() && bad < core_cpu.size() && core_gpu[bad] == core_cpu[bad]) bad++;
+            fprintf(stderr, "MISMATCH %s core numbers vs CPU Alg 3: kmax %u vs %u, first differing vertex %zu\n", what, r.kmax, kmax, bad);
+            ok = false;
+        }
+    };
+    CoreLoopResult inc;
+    double best_inc = 1e300, best_fused = 1e300, best_rc = 1e300;
+    for (int i = -1; i < repeats && ok; i++) {
+        gaa::check(ctx, ga_ctx_sync(ctx), "sync");
+        const double ts = now_ms();
+        inc = core_decomp_loop_inc(dg.n(), T_full, bi, verbose && i < 0, stderr);
+        const double ms = now_ms() - ts;
+        bi.download_core(core_gpu);
+        check("gpu incremental", inc);
+        if (i >= 0) best_inc = std::min(best_inc, ms);
+    }
+    uint32_t fused_rounds = 0;
+    for (int i = -1; i < repeats && ok; i++) {
+        const double ts = now_ms();
+        CoreLoopResult f = gpu_core_run(dg, T_full, core_gpu);
+        const double ms = now_ms() - ts;
+        check("gpu fused", f);
+        fused_rounds = f.rounds;
+        if (i >= 0) best_fused = std::min(best_fused, ms);
+    }
+    uint32_t rc_rounds = inc.rounds;
+    if (recount) {
+        for (int i = -1; i < repeats && ok; i++) {
+            const double ts = now_ms();
+            CoreLoopResult r = core_decomp_loop(dg.n(), T_full, br);
+            const double ms = now_ms() - ts;
+            br.download_core(core_gpu);
+            check("gpu recount", r);
+            rc_rounds = r.rounds;
+            if (i >= 0) best_rc = std::min(best_rc, ms);
+        }
+    }
+    if (ok && (fused_rounds != inc.rounds || rc_rounds != inc.rounds)) {
+        fprintf(stderr, "MISMATCH round counts: incremental %u, fused %u, recount %u\n", inc.rounds, fused_rounds, rc_rounds);
+        ok = false;
+    }
+    if (!ok) { fprintf(stderr, "VERIFICATION FAILED - no timing rows printed\n"); return 1; }
+    rows.push_back(dsg_row("gpu_core_decomp", path, og.n, og.m, ktri, kverts, best_inc, 0, inc.rounds, kmax));
+    rows.push_back(dsg_row("gpu_core_decomp_fused", path, og.n, og.m, ktri, kverts, best_fused, 0, fused_rounds, kmax));
+    if (recount) rows.push_back(dsg_row("gpu_core_decomp_recount", path, og.n, og.m, ktri, kverts, best_rc, 0, rc_rounds, kmax));
+    for (auto& r : rows) puts(r.c_str());
+    fflush(stdout);
+    fprintf(stderr, "GPU core numbers verified element-wise against CPU Alg 3 (kmax %u, %u rounds)\n", kmax, inc.rounds);
+    return 0;
+}

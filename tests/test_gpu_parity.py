@@ -1,0 +1,212 @@
+"""GPU parity: every device result equals the CPU oracle (itself pinned by tests/test_oracle.py) bit for bit.
+
+Covers the GPU CSR builder (byte-identical rowptr/col/src), all count variants, per-vertex participation on random
+alive masks, per-edge support, batch peeling (fused and host-driven) and the (k,triangle)-core (fused incremental,
+host-driven incremental and recount), on the reference's fixtures, random graphs with duplicates and self loops,
+hub graphs, empty and degenerate inputs, and several lane-group / LDS / block / chunk shapes."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import graph_analyzer_amd as ga
+import oracle
+from conftest import clique, random_graph
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+ALL_VARIANTS = [ga.TC_MERGE, ga.TC_GROUP4, ga.TC_GROUP8, ga.TC_GROUP16, ga.TC_GROUP64, ga.TC_HASH, ga.TC_SORTED]
+EPS = [(1, 10), (1, 2), (1, 1), (3, 1), (0, 1)]
+
+
+def check_graph(ctx, raw, n=None, eps=EPS, variants=ALL_VARIANTS, core=True, masks=True, tag=""):
+    og = oracle.OGraph.from_pairs(raw, n)
+    g = ga.DeviceGraph.from_pairs(ctx, raw, n)
+    r, c, s = g.download()
+    orr, oc, os_ = og.arrays()
+    assert (g.n, g.m, g.max_out_degree) == (og.n, og.m, og.max_out_degree), tag
+    assert np.array_equal(r, orr) and np.array_equal(c, oc) and np.array_equal(s, os_), f"CSR differs {tag}"
+    T = og.triangles()
+    for v in variants:
+        assert g.count(v)[0] == T, f"variant {ga.variant_name(v)} {tag}"
+    rng = np.random.default_rng(g.n + g.m)
+    probs = (1.0, 0.9, 0.6, 0.3, 0.05, 0.0) if masks else (1.0,)
+    for p in probs:
+        alive = (rng.random(g.n) < p).astype(np.uint8)
+        To, to = og.participation(alive)
+        for v in (ga.TC_HASH, ga.TC_MERGE):
+            Tg, tg = g.vertex_participation(alive, v)
+            assert Tg == To and np.array_equal(tg, to), f"participation p={p} {ga.variant_name(v)} {tag}"
+    Ts, so = og.edge_support()
+    for v in (ga.TC_HASH, ga.TC_MERGE):
+        Tg, sg = g.edge_support(v)
+        assert Tg == Ts and np.array_equal(sg, so), f"edge support {ga.variant_name(v)} {tag}"
+    for num, den in eps:
+        bn, bd, br, rounds, rr = og.batch_peel(num, den)
+        for fused in (True, False):
+            pr = g.batch_peel(num, den, T_full=T, fused=fused)
+            assert (pr.best_num, pr.best_den, pr.best_round, pr.rounds) == (bn, bd, br, rounds), f"peel {num}/{den} fused={fused} {tag}"
+            assert np.array_equal(pr.removed_round, rr), f"removed_round {num}/{den} {tag}"
+    if core:
+        oc_, okmax = og.core()
+        _, orounds, _ = og.core_levels()
+        for inc, fused in ((True, True), (True, False), (False, False)):
+            cg, rounds, kmax = g.core_decomposition(incremental=inc, fused=fused, T_full=T)
+            assert np.array_equal(cg, oc_) and kmax == okmax, f"core inc={inc} fused={fused} {tag}"
+            assert rounds == orounds, f"core rounds {rounds} vs {orounds} {tag}"
+    g.close()
+    return T
+
+
+def test_golden_fixtures(ctx):
+    """The reference's own outputs (tests/golden) reproduced on the GPU."""
+    with open(GOLDEN) as f:
+        recs = json.load(f)["graphs"]
+    for rec in recs:
+        raw = np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2)
+        g = ga.DeviceGraph.from_pairs(ctx, raw)
+        r, c, s = g.download()
+        assert r.tolist() == rec["rowptr"] and c.tolist() == rec["col"] and s.tolist() == rec["src"], rec["name"]
+        for v in ALL_VARIANTS:
+            assert g.count(v)[0] == rec["T"], (rec["name"], v)
+        assert g.vertex_participation()[1].tolist() == rec["tri"]
+        assert g.edge_support()[1].tolist() == rec["sup"]
+        assert g.edge_order().tolist() == rec["edge_order"]
+        for key, p in rec["peel"].items():
+            num, den = map(int, key.split("/"))
+            pr = g.batch_peel(num, den, T_full=rec["T"])
+            assert (pr.best_num, pr.best_den, pr.best_round, pr.rounds) == (p["best_num"], p["best_den"], p["best_round"], p["rounds"])
+            assert pr.removed_round.tolist() == p["removed_round"]
+        cg, rounds, kmax = g.core_decomposition(T_full=rec["T"])
+        assert cg.tolist() == rec["core"] and kmax == rec["kmax"], rec["name"]
+        g.close()
+
+
+def test_closed_forms(ctx):
+    import math
+    for k in (3, 4, 7, 20, 70):
+        e = np.array(clique(k), np.uint32)
+        g = ga.DeviceGraph.from_pairs(ctx, e)
+        for v in ALL_VARIANTS:
+            assert g.count(v)[0] == math.comb(k, 3)
+        assert set(g.edge_support()[1].tolist()) == {k - 2}
+        core, rounds, kmax = g.core_decomposition()
+        assert set(core.tolist()) == {math.comb(k - 1, 2)} and rounds == 1
+        g.close()
+
+
+def test_degenerate_inputs(ctx):
+    # empty list -> n = 1, m = 0 (graph.h:76,94)
+    for raw in (np.zeros((0, 2), np.uint32), np.array([[3, 3], [5, 5]], np.uint32)):
+        g = ga.DeviceGraph.from_pairs(ctx, raw)
+        assert (g.n, g.m) == (1, 0)
+        assert g.count(ga.TC_HASH)[0] == 0 and g.count(ga.TC_MERGE)[0] == 0
+        pr = g.batch_peel(1, 10, T_full=0)
+        assert (pr.best_num, pr.best_den, pr.best_round, pr.rounds) == (0, 1, 0, 1)
+        core, rounds, kmax = g.core_decomposition(T_full=0)
+        assert core.tolist() == [0] and rounds == 1 and kmax == 0
+        g.close()
+    check_graph(ctx, np.array([[0, 1]], np.uint32), tag="single edge")
+    check_graph(ctx, np.array([[5, 9]], np.uint32), tag="single edge, isolated low ids")
+    # build_graph_from_edges semantics: n given, pairs must be unique and loop-free
+    e = np.array([[0, 1], [1, 2], [0, 2]], np.uint32)
+    g = ga.DeviceGraph.from_pairs(ctx, e, 10)
+    assert (g.n, g.m, g.count()[0]) == (10, 3, 1)
+    g.close()
+    for bad in (np.array([[0, 1], [1, 0]], np.uint32), np.array([[0, 0]], np.uint32), np.array([[0, 12]], np.uint32)):
+        with pytest.raises(ga.GAError) as ei:
+            ga.DeviceGraph.from_pairs(ctx, bad, 10)
+        assert ei.value.code == ga.GA_ERR_ARG
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_graphs(ctx, seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(10, 1500))
+    raw = random_graph(rng, n, float(rng.uniform(0.005, 0.2)), dup=0.3, loops=3)
+    check_graph(ctx, raw, tag=f"seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_powerlaw_graphs(ctx, seed):
+    from graph_analyzer_amd import gen
+    raw = gen.chung_lu(int(20000 + 10000 * seed), 150000, 2.0 + 0.15 * seed, 0.0, seed=seed + 11)
+    check_graph(ctx, raw, eps=[(1, 10), (1, 100)], masks=True, tag=f"chung-lu {seed}")
+
+
+def test_hub_graphs(ctx):
+    """Hubs with 40-3000 neighbours: lane striding past 64 positions in the core update, hash items split by work."""
+    rng = np.random.default_rng(99)
+    for it in range(6):
+        leaves = int(rng.integers(40, 3000))
+        cl = int(rng.integers(8, 40))
+        hubs = int(rng.integers(1, 4))
+        n = hubs + leaves + cl
+        perm = rng.permutation(n).astype(np.uint32)
+        e = []
+        c0 = hubs + leaves
+        for a in range(c0, n):
+            for b in range(a + 1, n):
+                if rng.random() < 0.7:
+                    e.append((a, b))
+        for h in range(hubs):
+            for l in range(leaves):
+                if rng.random() < 0.9:
+                    e.append((hubs + l, h))
+            for _ in range(int(rng.integers(2, 6))):
+                e.append((h, c0 + int(rng.integers(cl))))
+            for l in range(0, leaves, 7):     # leaves that also close triangles through the hub
+                e.append((hubs + l, hubs + (l + 1) % leaves))
+        raw = perm[np.array(e, np.uint32)]
+        check_graph(ctx, raw, eps=[(1, 10)], masks=False, tag=f"hub {it}")
+
+
+def test_tuning_shapes(ctx):
+    """Every lane-group width, block size, load factor and work chunk gives the same exact count; the tuner too."""
+    from graph_analyzer_amd import gen
+    raw = gen.chung_lu(30000, 300000, 2.05, 0.0, seed=4)
+    og = oracle.OGraph.from_pairs(raw)
+    T = og.triangles()
+    _, tri_ref = og.participation()
+    _, sup_ref = og.edge_support()
+    g = ga.DeviceGraph.from_pairs(ctx, raw)
+    base = g.get_tuning()
+    for lanes in ([4, 4, 8, 32], [8, 16, 32, 64], [16, 32, 64, 64], [64, 64, 64, 64], [4, 8, 16, 32]):
+        for block in (64, 256, 1024):
+            for chunk, shift in ((64, 1), (4096, 2), (1 << 20, 3)):
+                t = ga.TcTuning()
+                t.lanes[:] = lanes
+                t.block, t.work_chunk, t.load_shift = block, chunk, shift
+                for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
+                    t.hash_cap = cap
+                    g.set_tuning(t)
+                    assert g.count(ga.TC_HASH)[0] == T, t.as_dict()
+        assert np.array_equal(g.vertex_participation()[1], tri_ref), lanes
+        assert np.array_equal(g.edge_support()[1], sup_ref), lanes
+    g.set_tuning(base)
+    best, ms = g.tune()
+    assert g.count(ga.TC_HASH)[0] == T and ms > 0
+    g.close()
+
+
+def test_peel_errors(ctx):
+    e = np.array(clique(5), np.uint32)
+    g = ga.DeviceGraph.from_pairs(ctx, e)
+    for num, den in ((1, 0), (1, (1 << 20) + 1), ((1 << 20) + 1, 10)):
+        with pytest.raises(ga.GAError) as ei:
+            g.batch_peel(num, den, T_full=10)
+        assert ei.value.code == ga.GA_ERR_ARITH
+    with pytest.raises(ga.GAError) as ei:
+        g.batch_peel(1, 10, T_full=11)           # wrong round-0 count is caught on the device
+    assert ei.value.code == ga.GA_ERR_STATE
+    with pytest.raises(ga.GAError):
+        g.core_decomposition(T_full=11)
+    g.close()
+
+
+def test_powerlaw_small_workload(ctx):
+    """The 8,298-vertex / 100,762-edge power-law workload end to end."""
+    from graph_analyzer_amd import gen
+    check_graph(ctx, gen.powerlaw_small(), eps=[(1, 10), (1, 100), (1, 20)], tag="powerlaw_small")

@@ -1,0 +1,204 @@
+"""Pins the CPU oracle (oracle/oracle.hpp) before it is trusted as the checker of the GPU path:
+golden vectors written by the reference's own code, the reference build itself (oracle/_ref) on fresh inputs,
+closed forms, brute force, and invariants chosen so that a dropped term, a wrong index or a flipped rule fails."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import clique, random_graph, zoo
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def test_golden_vectors(golden):
+    """Every oracle output equals the reference's on the committed fixtures (loader, CSR, counts, peel, core)."""
+    assert len(golden["graphs"]) >= 30
+    for rec in golden["graphs"]:
+        raw = np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2)
+        g = oracle.OGraph.from_pairs(raw)
+        r, c, s = g.arrays()
+        assert g.n == rec["n"] and g.m == rec["m"] and g.max_out_degree == rec["max_out_degree"], rec["name"]
+        assert r.tolist() == rec["rowptr"] and c.tolist() == rec["col"] and s.tolist() == rec["src"], rec["name"]
+        assert g.triangles() == rec["T"] and g.triangles_omp() == rec["T"]
+        T, tri = g.participation()
+        assert T == rec["T"] and tri.tolist() == rec["tri"]
+        T2, tri2 = g.participation(omp=True)
+        assert T2 == rec["T"] and tri2.tolist() == rec["tri"]
+        Ts, sup = g.edge_support()
+        assert Ts == rec["T"] and sup.tolist() == rec["sup"]
+        core, kmax = g.core()
+        assert core.tolist() == rec["core"] and kmax == rec["kmax"]
+        assert list(g.core_level_density(core, kmax)) == rec["kmax_core"]
+        assert g.edge_order().tolist() == rec["edge_order"]
+        for key, p in rec["peel"].items():
+            num, den = map(int, key.split("/"))
+            bn, bd, br, rounds, rr = g.batch_peel(num, den)
+            assert (bn, bd, br, rounds) == (p["best_num"], p["best_den"], p["best_round"], p["rounds"]), (rec["name"], key)
+            assert rr.tolist() == p["removed_round"]
+
+
+def test_text_loader_matches_reference(tmp_path):
+    """The oracle's text loader equals the reference's on comments, blank lines, one-number lines, loops, duplicates."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    path = tmp_path / "g.txt"
+    path.write_text("# c\n% c\n\n1 2\n2 1\n3 3\n4\t5\n5 4 extra\n7\n 8 9\n10 x\n9 8\n")
+    o = oracle.OGraph.load_text(str(path))
+    r = oracle.RGraph.load_text(str(path))
+    assert (o.n, o.m) == (r.n, r.m) == (10, 3)
+    for a, b in zip(o.arrays(), r.arrays()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_equals_reference_build(seed):
+    """Fresh seeded graphs (not in the fixtures): every oracle function equals the reference build."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(50, 700))
+    raw = random_graph(rng, n, float(rng.uniform(0.01, 0.12)), dup=0.3, loops=2)
+    o = oracle.OGraph.from_pairs(raw)
+    uniq = oracle.canonical_unique(raw)
+    r = oracle.RGraph.build(o.n, uniq)
+    for a, b in zip(o.arrays(), r.arrays()):
+        assert np.array_equal(a, b)
+    assert o.triangles() == r.triangles() == r.triangles_omp()
+    for p in (1.0, 0.9, 0.6, 0.3, 0.05, 0.0):
+        alive = (rng.random(o.n) < p).astype(np.uint8)
+        To, to = o.participation(alive)
+        Tr, tr = r.participation(alive)
+        assert To == Tr and np.array_equal(to, tr)
+    assert o.edge_support()[1].tolist() == r.edge_support()[1].tolist()
+    for num, den in ((1, 10), (1, 100), (1, 20), (3, 7)):
+        assert o.batch_peel(num, den)[:4] == r.batch_peel(num, den)[:4]
+        assert np.array_equal(o.batch_peel(num, den)[4], r.batch_peel(num, den)[4])
+    co, ko = o.core()
+    cr, kr = r.core()
+    assert np.array_equal(co, cr) and ko == kr
+    assert np.array_equal(o.edge_order(), r.edge_order())
+    for a, b in zip(o.undirected(), r.undirected()):
+        assert np.array_equal(a, b)
+
+
+def test_closed_forms():
+    for k in range(3, 16):
+        g = oracle.OGraph.from_pairs(np.array(clique(k), np.uint32))
+        T = g.triangles()
+        assert T == math.comb(k, 3)
+        _, sup = g.edge_support()
+        assert set(sup.tolist()) == {k - 2}                       # every edge of K_k lies in k-2 triangles
+        _, tri = g.participation()
+        assert set(tri.tolist()) == {math.comb(k - 1, 2)}
+        core, kmax = g.core()
+        assert set(core.tolist()) == {math.comb(k - 1, 2)} and kmax == math.comb(k - 1, 2)
+        bn, bd, br, rounds, rr = g.batch_peel(1, 10)
+        assert (bn, bd, br, rounds) == (math.comb(k, 3), k, 0, 1) and set(rr.tolist()) == {0}   # K_k peels in round 0
+    for a, b in ((3, 4), (5, 5), (2, 9)):
+        g = oracle.OGraph.from_pairs(np.array([(i, a + j) for i in range(a) for j in range(b)], np.uint32))
+        assert g.triangles() == 0                                  # bipartite: no triangle
+    for c in range(4, 20):                                         # wheel: hub + c-cycle has c triangles
+        e = [(0, i) for i in range(1, c + 1)] + [(i, i % c + 1) for i in range(1, c + 1)]
+        assert oracle.OGraph.from_pairs(np.array(e, np.uint32)).triangles() == c
+    # disjoint K_a + K_b (a > b): the larger clique survives one round longer and is the best set
+    e = clique(6) + clique(4, 6)
+    bn, bd, br, rounds, rr = oracle.OGraph.from_pairs(np.array(e, np.uint32)).batch_peel(1, 10)
+    assert (bn, bd) == (math.comb(6, 3), 6) and rr[:6].max() > rr[6:].max()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_brute_force_counts(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(20, 400))
+    raw = random_graph(rng, n, float(rng.uniform(0.02, 0.3)), dup=0.2, loops=1)
+    g = oracle.OGraph.from_pairs(raw)
+    uniq = oracle.canonical_unique(raw)
+    T = oracle.brute_triangles(g.n, uniq)
+    assert g.triangles() == T == g.triangles_omp()
+    _, sup = g.edge_support()
+    assert int(sup.sum()) == 3 * T
+    for p in (1.0, 0.7, 0.3, 0.0):
+        alive = (rng.random(g.n) < p).astype(np.uint8)
+        Tb, tb = oracle.brute_participation(g.n, uniq, alive)
+        To, to = g.participation(alive)
+        assert To == Tb and np.array_equal(to, tb) and int(to.sum()) == 3 * To
+
+
+def test_brute_force_support():
+    rng = np.random.default_rng(5)
+    raw = random_graph(rng, 120, 0.15)
+    g = oracle.OGraph.from_pairs(raw)
+    r, c, s = g.arrays()
+    adj = np.zeros((g.n, g.n), bool)
+    adj[s, c] = adj[c, s] = True
+    _, sup = g.edge_support()
+    for e in range(g.m):
+        u, v = s[e], c[e]
+        assert sup[e] == int(np.sum(adj[u] & adj[v]))
+
+
+def test_tiny_exhaustive_peel_and_core():
+    """n <= 12: core numbers by definition over all subsets, fixpoint definition and Alg 3 agree; the peel result is
+    within 3(1+eps) of the brute-force optimum and never above it (the reference's tiny-suite checks)."""
+    rng = np.random.default_rng(777)
+    graphs = [(name, n, e) for name, n, e in zoo() if n <= 12]
+    for it in range(120):
+        n = int(rng.integers(3, 13))
+        graphs.append((f"rand{it}", n, oracle.canonical_unique(random_graph(rng, n, float(rng.uniform(0.1, 0.9))))))
+    with_tri = 0
+    for name, n, e in graphs:
+        g = oracle.OGraph.from_pairs(e, n)
+        core, _ = g.core()
+        assert np.array_equal(core, oracle.brute_core(n, e)), name
+        assert np.array_equal(core, g.core_fixpoint()), name
+        c2, rounds, _ = g.core_levels()
+        assert np.array_equal(core, c2) and rounds >= (1 if n else 0)
+        on, od = oracle.brute_densest(n, e)
+        with_tri += on > 0
+        for num, den in ((1, 10), (1, 2), (1, 1), (3, 1), (0, 1)):
+            bn, bd, br, rounds, rr = g.batch_peel(num, den)
+            assert bn * od <= on * bd                             # never above the optimum
+            assert 3 * (den + num) * bn * od >= on * bd * den      # 3(1+eps) guarantee
+            assert (rr < rounds).all()
+    assert with_tri > 60
+
+
+def test_arith_guards():
+    assert oracle.peel_arith_ok(10, 5, 1, 10)
+    assert not oracle.peel_arith_ok(10, 5, 1, 0)
+    assert not oracle.peel_arith_ok(10, 5, 1, (1 << 20) + 1)
+    assert not oracle.peel_arith_ok(10, 5, (1 << 20) + 1, 10)
+    assert not oracle.peel_arith_ok(10, 1 << 32, 1, 10)
+    assert not oracle.peel_arith_ok(4_000_000_000, 4_000_000_000, 1, 1 << 20)
+    if oracle.ref_available():
+        L = oracle.rlib()
+        for args in ((10, 5, 1, 10), (10, 5, 1, 0), (10, 1 << 32, 1, 10), (4_000_000_000, 4_000_000_000, 1, 1 << 20), (7, 3, 0, 1)):
+            assert bool(L.ref_peel_arith_ok(*args)) == oracle.peel_arith_ok(*args)
+
+
+def test_planted_mistake_is_caught():
+    """A mutated restatement (orientation rule flipped to the larger (degree, id) end) must fail the checks above:
+    rows stop being the reference's and the per-edge index sets change, although T is orientation-independent."""
+    rng = np.random.default_rng(3)
+    raw = random_graph(rng, 200, 0.08)
+    g = oracle.OGraph.from_pairs(raw)
+    r, c, s = g.arrays()
+    # flipped orientation built from the same edges
+    flipped_src, flipped_dst = c.copy(), s.copy()
+    order = np.lexsort((flipped_dst, flipped_src))
+    fr = np.zeros(g.n + 1, np.uint32)
+    np.add.at(fr, flipped_src.astype(np.int64) + 1, 1)
+    fr = np.cumsum(fr).astype(np.uint32)
+    gf = oracle.OGraph.from_csr(fr, flipped_dst[order])
+    assert gf.triangles() == g.triangles()                        # T is invariant ...
+    assert not np.array_equal(gf.arrays()[0], r)                  # ... but the CSR is not the reference's

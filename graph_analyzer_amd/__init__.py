@@ -455,3 +455,90 @@ class Core:
 
 def variant_name(v: int) -> str:
     return lib().ga_tc_variant_name(v).decode()
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return
+    sig = {
+        "ga_batch_create": (ctypes.c_int, [_vp, ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]),
+        "ga_batch_destroy": (None, [_vp]),
+        "ga_batch_info": (ctypes.c_int, [_vp, _u32p, _u64p, _u64p]),
+        "ga_batch_graph_info": (ctypes.c_int, [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p]),
+        "ga_batch_download_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, _u32p]),
+        "ga_batch_count": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
+        "ga_batch_count_per_graph": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
+        "ga_batch_peel": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
+        "ga_batch_peel_per_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p,
+                                                   ctypes.POINTER(ctypes.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L._batch_bound = True
+
+
+class Batch:
+    """Many graphs resident together (ga_batch): per-graph counts and batch peeling with shared launches."""
+
+    def __init__(self, ctx: Context, graphs: Sequence[np.ndarray]):
+        L = lib()
+        _bind_batch(L)
+        self.ctx = ctx
+        sizes = [int(np.asarray(p).reshape(-1, 2).shape[0]) for p in graphs]
+        off = np.zeros(len(graphs) + 1, np.uint64)
+        off[1:] = np.cumsum(sizes, dtype=np.uint64)
+        flat = np.concatenate([np.ascontiguousarray(p, dtype=np.uint32).reshape(-1) for p in graphs]) if graphs else np.zeros(0, np.uint32)
+        flat = np.ascontiguousarray(flat, dtype=np.uint32)
+        self._h = _vp()
+        ms = ctypes.c_float()
+        ctx.check(L.ga_batch_create(ctx.handle, len(graphs), _ptr(off, _u64p), _ptr(flat, _u32p) if flat.size else None,
+                                    ctypes.byref(self._h), ctypes.byref(ms)), "batch_create")
+        self.build_ms = ms.value
+        c, tn, tm = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
+        L.ga_batch_info(self._h, ctypes.byref(c), ctypes.byref(tn), ctypes.byref(tm))
+        self.count, self.total_n, self.total_m = c.value, tn.value, tm.value
+        self.n = np.zeros(self.count, np.int64)
+        self.m = np.zeros(self.count, np.int64)
+        for i in range(self.count):
+            n, m, mo = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+            L.ga_batch_graph_info(self._h, i, ctypes.byref(n), ctypes.byref(m), ctypes.byref(mo))
+            self.n[i], self.m[i] = n.value, m.value
+        self.voff = np.concatenate([[0], np.cumsum(self.n)])
+
+    def graph(self, i: int):
+        rowptr = np.zeros(int(self.n[i]) + 1, np.uint32)
+        col = np.zeros(max(int(self.m[i]), 1), np.uint32)
+        src = np.zeros(max(int(self.m[i]), 1), np.uint32)
+        self.ctx.check(lib().ga_batch_download_graph(self.ctx.handle, self._h, i, _ptr(rowptr, _u32p), _ptr(col, _u32p), _ptr(src, _u32p)),
+                       "batch_download_graph")
+        return rowptr, col[: self.m[i]], src[: self.m[i]]
+
+    def count_all(self, per_graph: bool = False):
+        T = np.zeros(self.count, np.uint64)
+        ms = ctypes.c_float()
+        f = lib().ga_batch_count_per_graph if per_graph else lib().ga_batch_count
+        self.ctx.check(f(self.ctx.handle, self._h, _ptr(T, _u64p), ctypes.byref(ms)), "batch_count")
+        return T, ms.value
+
+    def peel_all(self, num=1, den=10, T_full=None, per_graph: bool = False, removed_round: bool = True):
+        stats = np.zeros(4 * self.count, np.uint64)
+        rr = np.zeros(max(int(self.total_n), 1), np.uint32) if removed_round else None
+        tf = None if T_full is None else np.ascontiguousarray(T_full, dtype=np.uint64)
+        ms = ctypes.c_float()
+        f = lib().ga_batch_peel_per_graph if per_graph else lib().ga_batch_peel
+        self.ctx.check(f(self.ctx.handle, self._h, num, den, None if tf is None else _ptr(tf, _u64p), _ptr(stats, _u64p),
+                         None if rr is None else _ptr(rr, _u32p), ctypes.byref(ms)), "batch_peel")
+        return stats.reshape(-1, 4), (rr[: self.total_n] if rr is not None else None), ms.value
+
+    def close(self):
+        if self._h:
+            lib().ga_batch_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -153,23 +153,17 @@ int finish_rows(ga_ctx* ctx, ga_graph* g, uint32_t* outdeg) {
 
 }  // namespace
 
-// Raw pairs (host) -> oriented CSR (device).
-extern "C" int ga_graph_from_pairs(ga_ctx* ctx, const uint32_t* pairs, uint64_t k, int64_t n_in, ga_graph** out, float* build_ms) {
+// Device pairs -> oriented CSR (device). n_in < 0: n = max_id + 1 over non-loop pairs. strict: pairs must be unique,
+// loop-free and < n (build_graph_from_edges); otherwise loops, duplicates and both orientations are folded.
+int ga_build_from_device_pairs(ga_ctx* ctx, const uint32_t* dp, uint64_t k, int64_t n_in, bool strict, ga_graph** out, float* build_ms) {
     *out = nullptr;
-    if (!ctx) return GA_ERR_ARG;
-    if (k > 0 && !pairs) return ga_fail(ctx, GA_ERR_ARG, "pairs is NULL");
-    if (n_in > (int64_t)UINT32_MAX) return ga_fail(ctx, GA_ERR_ARG, "n too large");
-    GA_HIP(ctx, hipSetDevice(ctx->device));
-    DevBuf dpairs, keys_a, keys_b, flag, pos, counters;
-    GA_TRY(ga_malloc(ctx, &dpairs.p, std::max<uint64_t>(k, 1) * 8));
-    if (k) GA_HIP(ctx, hipMemcpyAsync(dpairs.p, pairs, k * 8, hipMemcpyHostToDevice, ctx->stream));
+    DevBuf keys_a, keys_b, flag, pos, counters;
     GA_TRY(ga_malloc(ctx, &counters.p, 64));
     uint32_t* cnt = (uint32_t*)counters.p;   // [0] max id, [1] any, [2] bad, [3] dup
     GA_TRY(ga_fill_u32(ctx, cnt, 16, 0));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
 
     uint32_t n;
-    const uint32_t* dp = (const uint32_t*)dpairs.p;
     if (n_in < 0) {
         if (k) {
             k_max_id<<<grid_for(k, 256, 4096), 256, 0, ctx->stream>>>(dp, k, cnt, cnt + 1);
@@ -224,7 +218,7 @@ extern "C" int ga_graph_from_pairs(ga_ctx* ctx, const uint32_t* pairs, uint64_t 
             return fail_g(ga_fail(ctx, GA_ERR_HIP, "reading the unique-edge count failed"));
         const uint64_t mu = ctx->hstat[0] & 0xFFFFFFFFu;
         const uint32_t bad = (uint32_t)(ctx->hstat[1] & 0xFFFFFFFFu), dup = (uint32_t)(ctx->hstat[1] >> 32);
-        if (n_in >= 0 && (bad || dup))
+        if (strict && (bad || dup))
             return fail_g(ga_fail(ctx, GA_ERR_ARG, "build_graph_from_edges needs unique, loop-free pairs with ids < n "
                                   "(%u self loops / out-of-range ids, %u duplicates)", bad, dup));
         if (mu > UINT32_MAX) return fail_g(ga_fail(ctx, GA_ERR_ARG, "more than 2^32-1 unique edges"));
@@ -251,6 +245,19 @@ extern "C" int ga_graph_from_pairs(ga_ctx* ctx, const uint32_t* pairs, uint64_t 
     if (build_ms) GA_HIP(ctx, hipEventElapsedTime(build_ms, ctx->ev0, ctx->ev1));
     *out = g;
     return GA_OK;
+}
+
+// Raw pairs (host) -> oriented CSR (device).
+extern "C" int ga_graph_from_pairs(ga_ctx* ctx, const uint32_t* pairs, uint64_t k, int64_t n_in, ga_graph** out, float* build_ms) {
+    *out = nullptr;
+    if (!ctx) return GA_ERR_ARG;
+    if (k > 0 && !pairs) return ga_fail(ctx, GA_ERR_ARG, "pairs is NULL");
+    if (n_in > (int64_t)UINT32_MAX) return ga_fail(ctx, GA_ERR_ARG, "n too large");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf dpairs;
+    GA_TRY(ga_malloc(ctx, &dpairs.p, std::max<uint64_t>(k, 1) * 8));
+    if (k) GA_HIP(ctx, hipMemcpyAsync(dpairs.p, pairs, k * 8, hipMemcpyHostToDevice, ctx->stream));
+    return ga_build_from_device_pairs(ctx, (const uint32_t*)dpairs.p, k, n_in, n_in >= 0, out, build_ms);
 }
 
 extern "C" int ga_graph_from_csr(ga_ctx* ctx, uint32_t n, uint32_t m, const uint32_t* rowptr, const uint32_t* col, ga_graph** out) {

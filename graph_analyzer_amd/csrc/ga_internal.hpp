@@ -78,11 +78,15 @@ int ga_fill_u64(ga_ctx* ctx, uint64_t* p, size_t count, uint64_t v);
 int ga_max_u32(ga_ctx* ctx, const uint32_t* p, size_t count, uint32_t* out_host);
 
 // graph helpers shared between files
+int ga_build_from_device_pairs(ga_ctx* ctx, const uint32_t* dpairs, uint64_t k, int64_t n_in, bool strict, ga_graph** out,
+                               float* build_ms);
 int ga_graph_ensure_undirected(ga_ctx* ctx, ga_graph* g);
 int ga_graph_ensure_perm(ga_ctx* ctx, ga_graph* g);
 int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g);
+// One count of `variant` into *d_total (zeroed by the caller). Optional: alive mask, per-vertex tri, per-edge sup, a device
+// `done` flag that turns the launch into a no-op, and per-graph counts gcnt[gof[h]] for a disjoint union of graphs.
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* d_alive, uint32_t* d_tri,
-                 uint32_t* d_sup, const uint32_t* d_done);
+                 uint32_t* d_sup, const uint32_t* d_done, const uint32_t* gof = nullptr, unsigned long long* gcnt = nullptr);
 
 inline unsigned grid_for(uint64_t threads, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (threads + block - 1) / block;

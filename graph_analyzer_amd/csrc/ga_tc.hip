@@ -14,7 +14,7 @@
 //                     size (the tuning table), and hub rows are split into work-bounded chunks.
 #include "ga_internal.hpp"
 
-enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4 };
+enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4, F_GRAPH = 8 };
 
 #define GA_EMPTY 0xFFFFFFFFu
 
@@ -105,7 +105,7 @@ __global__ void k_tc_hash(const uint32_t* __restrict__ items, uint32_t nitems, c
                           const uint32_t* __restrict__ col, const uint32_t* __restrict__ wcol, const uint32_t* __restrict__ wedge,
                           const uint8_t* __restrict__ alive, uint32_t* __restrict__ tri, uint32_t* __restrict__ sup,
                           unsigned long long* __restrict__ total, uint32_t ts_max, uint32_t load_shift,
-                          const uint32_t* __restrict__ done) {
+                          const uint32_t* __restrict__ done, const uint32_t* __restrict__ gof, unsigned long long* __restrict__ gcnt) {
     static_assert(G >= 4 && G <= 64 && (64 % G) == 0, "G must divide 64");
     extern __shared__ uint32_t lds[];
     if (done && *done) return;
@@ -194,8 +194,8 @@ __global__ void k_tc_hash(const uint32_t* __restrict__ items, uint32_t nitems, c
                     }
                     if (hit) {
                         local++;
+                        if (FLAGS & (F_TRI | F_GRAPH)) hcount++;
                         if (FLAGS & F_TRI) {
-                            hcount++;
                             atomicAdd(&tri[x], 1u);
                             atomicAdd(&tri[o_p], 1u);
                         }
@@ -210,10 +210,13 @@ __global__ void k_tc_hash(const uint32_t* __restrict__ items, uint32_t nitems, c
                 }
             }
         }
-        if (FLAGS & F_TRI) {
+        if (FLAGS & (F_TRI | F_GRAPH)) {   // triangles of this item: h is a corner of each, h's graph owns them
 #pragma unroll
             for (uint32_t o = G / 2; o > 0; o >>= 1) hcount += __shfl_xor(hcount, o, G);
-            if (sub == 0 && hcount) atomicAdd(&tri[h], hcount);
+            if (sub == 0 && hcount) {
+                if (FLAGS & F_TRI) atomicAdd(&tri[h], hcount);
+                if (FLAGS & F_GRAPH) atomicAdd(&gcnt[gof[h]], (unsigned long long)hcount);
+            }
         }
         group_sync();   // every lane is done with the table before the next item clears it
     }
@@ -443,7 +446,8 @@ static uint32_t bucket_slots(uint32_t bound, uint32_t shift) {
 
 template <int G, bool HASHED, int FLAGS>
 static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, uint32_t bound,
-                              unsigned long long* total, const uint8_t* alive, uint32_t* tri, uint32_t* sup, const uint32_t* done) {
+                              unsigned long long* total, const uint8_t* alive, uint32_t* tri, uint32_t* sup, const uint32_t* done,
+                              const uint32_t* gof, unsigned long long* gcnt) {
     if (!nitems) return GA_OK;
     const TcPlan& P = g->plan;
     uint32_t block = (uint32_t)P.tune.block;
@@ -456,19 +460,20 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     const unsigned grid = grid_for((uint64_t)nitems * G, block, (unsigned)std::max(ctx->cus, 1) * 64u);
     (void)gpb;
     k_tc_hash<G, HASHED, FLAGS><<<grid, block, shmem, ctx->stream>>>(items, nitems, g->rowptr, g->col, P.wcol, P.wedge, alive, tri, sup,
-                                                                      total, ts, (uint32_t)P.tune.load_shift, done);
+                                                                      total, ts, (uint32_t)P.tune.load_shift, done, gof, gcnt);
     return ga_launch_check(ctx, "k_tc_hash");
 }
 
 template <int G, int FLAGS>
 static int launch_hash_g(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, uint32_t bound, unsigned long long* total,
-                         const uint8_t* alive, uint32_t* tri, uint32_t* sup, const uint32_t* done) {
-    return launch_hash_bucket<G, true, FLAGS>(ctx, g, items, nitems, bound, total, alive, tri, sup, done);
+                         const uint8_t* alive, uint32_t* tri, uint32_t* sup, const uint32_t* done, const uint32_t* gof,
+                         unsigned long long* gcnt) {
+    return launch_hash_bucket<G, true, FLAGS>(ctx, g, items, nitems, bound, total, alive, tri, sup, done, gof, gcnt);
 }
 
 template <int FLAGS>
 static int launch_hash(ga_ctx* ctx, ga_graph* g, unsigned long long* total, const uint8_t* alive, uint32_t* tri, uint32_t* sup,
-                       const uint32_t* done) {
+                       const uint32_t* done, const uint32_t* gof, unsigned long long* gcnt) {
     const TcPlan& P = g->plan;
     uint32_t off = 0;
     for (int b = 0; b < 4; b++) {
@@ -477,16 +482,16 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, unsigned long long* total, cons
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
         int rc = GA_OK;
         switch (P.tune.lanes[b]) {
-            case 4: rc = launch_hash_g<4, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done); break;
-            case 8: rc = launch_hash_g<8, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done); break;
-            case 16: rc = launch_hash_g<16, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done); break;
-            case 32: rc = launch_hash_g<32, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done); break;
-            default: rc = launch_hash_g<64, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done); break;
+            case 4: rc = launch_hash_g<4, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
+            case 8: rc = launch_hash_g<8, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
+            case 16: rc = launch_hash_g<16, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
+            case 32: rc = launch_hash_g<32, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
+            default: rc = launch_hash_g<64, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
         }
         GA_TRY(rc);
         off += ni;
     }
-    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 3ull * off, P.nhuge, 0, total, alive, tri, sup, done);
+    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 3ull * off, P.nhuge, 0, total, alive, tri, sup, done, gof, gcnt);
 }
 
 template <int FLAGS>
@@ -506,20 +511,23 @@ static int launch_group(ga_ctx* ctx, ga_graph* g, unsigned long long* total) {
 
 // Launches one count of `variant` into *d_total (zeroed by the caller). alive/tri/sup as in the flags.
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* alive, uint32_t* tri, uint32_t* sup,
-                 const uint32_t* done) {
+                 const uint32_t* done, const uint32_t* gof, unsigned long long* gcnt) {
     unsigned long long* total = d_total;
     if (variant == GA_TC_AUTO) variant = GA_TC_HASH;
-    const int flags = (alive ? F_ALIVE : 0) | (tri ? F_TRI : 0) | (sup ? F_SUP : 0);
+    const int flags = (alive ? F_ALIVE : 0) | (tri ? F_TRI : 0) | (sup ? F_SUP : 0) | (gcnt ? F_GRAPH : 0);
     if (variant == GA_TC_HASH) {
         GA_TRY(ga_tc_ensure_plan(ctx, g));
         switch (flags) {
-            case 0: return launch_hash<0>(ctx, g, total, alive, tri, sup, done);
-            case F_TRI: return launch_hash<F_TRI>(ctx, g, total, alive, tri, sup, done);
-            case F_ALIVE | F_TRI: return launch_hash<F_ALIVE | F_TRI>(ctx, g, total, alive, tri, sup, done);
-            case F_SUP: return launch_hash<F_SUP>(ctx, g, total, alive, tri, sup, done);
+            case 0: return launch_hash<0>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
+            case F_TRI: return launch_hash<F_TRI>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
+            case F_ALIVE | F_TRI: return launch_hash<F_ALIVE | F_TRI>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
+            case F_SUP: return launch_hash<F_SUP>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
+            case F_GRAPH: return launch_hash<F_GRAPH>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
+            case F_ALIVE | F_TRI | F_GRAPH: return launch_hash<F_ALIVE | F_TRI | F_GRAPH>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
             default: return ga_fail(ctx, GA_ERR_ARG, "unsupported flag combination %d for the hash variant", flags);
         }
     }
+    if (gcnt) return ga_fail(ctx, GA_ERR_ARG, "per-graph counts need the hash variant");
     if (variant == GA_TC_MERGE || variant == GA_TC_SORTED) {
         const uint32_t* order = nullptr;
         if (variant == GA_TC_SORTED) {

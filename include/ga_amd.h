@@ -131,6 +131,29 @@ int ga_core_download(ga_core* c, uint32_t* core);
 /* Whole decomposition with the level loop on the device; stats = {rounds, kmax}. T_full as in ga_peel_run. */
 int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms);
 
+/* ---- batching: many graphs resident together (no reference counterpart: the reference runs one graph per process;
+ *      per-graph semantics are exactly load_graph + the calls above, graph by graph) ----------------------------- */
+/* Graph i's raw pairs are pairs[2*pair_offsets[i] .. 2*pair_offsets[i+1]) with load_graph semantics per graph
+ * (n_i = max_id + 1, loops dropped, de-duplicated). All graphs are built in one GPU pass; build_ms = host wall time. */
+int ga_batch_create(ga_ctx* ctx, uint32_t count, const uint64_t* pair_offsets, const uint32_t* pairs, ga_batch** out, float* build_ms);
+void ga_batch_destroy(ga_batch* b);
+int ga_batch_info(const ga_batch* b, uint32_t* count, uint64_t* total_n, uint64_t* total_m);
+int ga_batch_graph_info(const ga_batch* b, uint32_t i, uint32_t* n, uint32_t* m, uint32_t* max_out_degree);
+/* Graph i's oriented CSR with local ids (equals ga_graph_download of the graph built alone). */
+int ga_batch_download_graph(ga_ctx* ctx, ga_batch* b, uint32_t i, uint32_t* rowptr, uint32_t* col, uint32_t* src);
+/* Per-graph T (count entries) from one shared launch set; ms = host wall time from launch to result. */
+int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
+/* The same counts with one ga_tc_count per graph (the per-graph-launch baseline). */
+int ga_batch_count_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
+/* Batch peeling of every graph with shared launches per round. stats: 4 per graph {best_num, best_den, best_round,
+ * rounds}; removed_round: total_n entries, graph i at its vertex offset (or NULL); T_full: per-graph counts for the
+ * round-0 check (or NULL). */
+int ga_batch_peel(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t den, const uint64_t* T_full, uint64_t* stats,
+                  uint32_t* removed_round, float* ms);
+/* The same with one ga_peel_run per graph (the per-graph-launch baseline). */
+int ga_batch_peel_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t den, const uint64_t* T_full, uint64_t* stats,
+                            uint32_t* removed_round, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,84 @@
+"""Batching parity: every graph of a batch gets exactly the result it gets alone (oracle, per-graph launches, and a batch
+of one), for the CSR, the count and the batch peel."""
+import numpy as np
+import pytest
+
+import graph_analyzer_amd as ga
+import oracle
+from conftest import clique, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def mix(count, seed, nmax=3000):
+    from graph_analyzer_amd import gen
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(20), np.log(nmax))))
+        deg = float(rng.uniform(2, 24))
+        if i % 3 == 0:
+            graphs.append(gen.erdos_renyi(n, int(n * deg / 2), seed=seed * 1000 + i))
+        elif i % 3 == 1:
+            graphs.append(gen.chung_lu(n, int(n * deg / 2), float(rng.uniform(2.0, 2.5)), 1.0, seed=seed * 1000 + i))
+        else:
+            graphs.append(random_graph(rng, n if n < 600 else 600, float(rng.uniform(0.01, 0.2)), dup=0.3, loops=2))
+    graphs[1] = np.zeros((0, 2), np.uint32)                       # empty graph: n = 1
+    graphs[2] = np.array([[4, 4], [7, 7]], np.uint32)             # loops only: n = 1
+    graphs[3] = np.array(clique(30), np.uint32)
+    return graphs
+
+
+def test_batch_matches_oracle_and_per_graph(ctx):
+    graphs = mix(48, 3)
+    b = ga.Batch(ctx, graphs)
+    assert b.count == len(graphs)
+    T_batch, _ = b.count_all()
+    T_each, _ = b.count_all(per_graph=True)
+    assert np.array_equal(T_batch, T_each)
+    oracles = [oracle.OGraph.from_pairs(p) for p in graphs]
+    for i, og in enumerate(oracles):
+        r, c, s = b.graph(i)
+        orr, oc, os_ = og.arrays()
+        assert np.array_equal(r, orr) and np.array_equal(c, oc) and np.array_equal(s, os_), i
+        assert T_batch[i] == og.triangles(), i
+    for num, den in ((1, 10), (1, 20)):
+        stats, rr, _ = b.peel_all(num, den, T_full=T_batch)
+        stats2, rr2, _ = b.peel_all(num, den, per_graph=True)
+        assert np.array_equal(stats, stats2) and np.array_equal(rr, rr2)
+        for i, og in enumerate(oracles):
+            bn, bd, br, rounds, orr_ = og.batch_peel(num, den)
+            assert tuple(int(x) for x in stats[i]) == (bn, bd, br, rounds), (i, num, den)
+            assert np.array_equal(rr[b.voff[i]:b.voff[i + 1]], orr_), i
+    b.close()
+
+
+def test_alone_equals_in_mix(ctx):
+    graphs = mix(24, 8)
+    b = ga.Batch(ctx, graphs)
+    T, _ = b.count_all()
+    stats, rr, _ = b.peel_all(1, 10)
+    for i in (0, 3, 5, 17, 23):
+        one = ga.Batch(ctx, [graphs[i]])
+        T1, _ = one.count_all()
+        s1, rr1, _ = one.peel_all(1, 10)
+        assert T1[0] == T[i] and np.array_equal(s1[0], stats[i]) and np.array_equal(rr1, rr[b.voff[i]:b.voff[i + 1]])
+        g = ga.DeviceGraph.from_pairs(ctx, graphs[i])
+        assert g.count()[0] == T[i]
+        pr = g.batch_peel(1, 10)
+        assert (pr.best_num, pr.best_den, pr.best_round, pr.rounds) == tuple(int(x) for x in stats[i])
+        g.close()
+        one.close()
+    b.close()
+
+
+def test_batch_peel_errors(ctx):
+    graphs = [np.array(clique(6), np.uint32), np.array(clique(5), np.uint32)]
+    b = ga.Batch(ctx, graphs)
+    with pytest.raises(ga.GAError) as ei:
+        b.peel_all(1, 0)
+    assert ei.value.code == ga.GA_ERR_ARITH
+    with pytest.raises(ga.GAError) as ei:
+        b.peel_all(1, 10, T_full=np.array([20, 11], np.uint64))   # graph 1 has 10 triangles
+    assert ei.value.code == ga.GA_ERR_STATE and "graph 1" in str(ei.value)
+    b.close()

@@ -46,12 +46,12 @@ class GAError(RuntimeError):
 
 
 class TcTuning(ctypes.Structure):
-    _fields_ = [("lanes", ctypes.c_int32 * 4), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
-                ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32)]
+    _fields_ = [("lanes", ctypes.c_int32 * 5), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
+                ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32)]
 
     def as_dict(self):
-        return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "load_shift": self.load_shift,
-                "hash_cap": self.hash_cap}
+        return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "batches": self.batches,
+                "load_shift": self.load_shift, "hash_cap": self.hash_cap}
 
 
 _lib = None
@@ -85,6 +85,7 @@ def lib() -> ctypes.CDLL:
             "ga_tc_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
             "ga_tc_set_tuning": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning)]),
             "ga_tc_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
+            "ga_tc_plan_info": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
             "ga_tc_vertex": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u8p, _u32p, _u64p]),
             "ga_tc_edge_support": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u32p, _u64p]),
             "ga_peel_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
@@ -239,6 +240,12 @@ class DeviceGraph:
         ms = ctypes.c_float()
         self.ctx.check(lib().ga_tc_prepare(self.ctx.handle, self._h, ctypes.byref(ms)), "tc_prepare")
         return ms.value
+
+    def plan_info(self):
+        nw = ctypes.c_uint32()
+        it = np.zeros(6, np.uint32)
+        self.ctx.check(lib().ga_tc_plan_info(self.ctx.handle, self._h, ctypes.byref(nw), _ptr(it, _u32p)), "tc_plan_info")
+        return nw.value, it.tolist()
 
     def get_tuning(self) -> TcTuning:
         t = TcTuning()

@@ -309,8 +309,299 @@ extern "C" int ga_core_download(ga_core* c, uint32_t* core) {
     return GA_OK;
 }
 
-// Whole decomposition with the round loop on the device (incremental design; the recount design is available
-// through the per-round calls).
+// ---------------------------------------------------------------------------------------------------------------
+// fused incremental decomposition
+// ---------------------------------------------------------------------------------------------------------------
+// The level loop runs on the device with three structures that shrink with the alive set:
+//   alist        the ids still alive at the last compaction (min / find scan it instead of all n vertices)
+//   cbeg / cend  per-vertex ranges into a compacted copy of the undirected adjacency holding only neighbours that
+//                were alive at the last compaction (sorted order kept), so late-round hubs carry short lists
+//   entries      each frontier vertex u is split into (u, chunk) entries of at most CORE_CHUNK neighbours, one wave
+//                per entry, so a hub's triangles are enumerated by many waves at once
+// Compaction runs on the device (no host size readback) whenever at most half of alist is still alive: alist is
+// filtered, and each surviving vertex copies its surviving neighbours into the other adjacency buffer, reserving
+// space with one atomic per vertex. Every kernel returns at once once `done` is set.
+#define CORE_CHUNK 256u
+
+struct CoreFusedDev {
+    uint32_t na;          // alist length
+    uint32_t alist_cur;   // which alist buffer is current (0/1)
+    uint32_t adj_cur;     // which adjacency buffer is current (0 = the graph's own ucol, 1/2 = copies)
+    uint32_t compact;     // 1: compact after this round
+    uint32_t na_new;      // compaction cursor for the new alist
+    uint32_t nfv;         // frontier vertices this round
+    uint32_t nent;        // frontier entries this round
+    uint32_t pad;
+    unsigned long long adj_cursor;   // compaction cursor in the new adjacency buffer
+};
+
+struct CoreBufs {
+    uint32_t* alist[2];
+    const uint32_t* adj[3];   // adj[0] = ucol; adj[1], adj[2] writable copies
+    uint32_t* adjw[3];
+    uint32_t* cbeg;
+    uint32_t* cend;
+    uint32_t* fvert;
+    uint32_t* ent;            // 2 words per entry: u, chunk
+};
+
+__global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32_t* __restrict__ cbeg, uint32_t* __restrict__ cend,
+                          const uint32_t* __restrict__ urow, uint32_t n) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        alist0[v] = v;
+        cbeg[v] = urow[v];
+        cend[v] = urow[v + 1];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        f->na = n;
+        f->alist_cur = 0;
+        f->adj_cur = 0;
+        f->compact = 0;
+        f->na_new = 0;
+        f->nfv = 0;
+        f->nent = 0;
+        f->adj_cursor = 0;
+    }
+}
+
+__global__ void k_cf_min(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint32_t* __restrict__ tri, const uint8_t* __restrict__ state,
+                         CoreDev* st) {
+    if (st->done) return;
+    const uint32_t* __restrict__ al = B.alist[f->alist_cur];
+    const uint32_t na = f->na;
+    uint32_t mn = 0xFFFFFFFFu;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+        const uint32_t v = al[i];
+        if (state[v] == 1 && tri[v] < mn) mn = tri[v];
+    }
+    mn = wave_min_u32(mn);
+    if (lane_id() == 0 && mn != 0xFFFFFFFFu) atomicMin(&st->min_tri, mn);
+}
+
+// Frontier = alive v with tri <= k; appends v to fvert and its chunk entries (one atomic per wave for each).
+__global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restrict__ tri, uint8_t* __restrict__ state,
+                          uint32_t* __restrict__ core, CoreDev* st) {
+    if (st->done) return;
+    const unsigned long long mt = st->min_tri, kp = st->k_prev;
+    const unsigned long long k = mt > kp ? mt : kp;
+    const uint32_t* __restrict__ al = B.alist[f->alist_cur];
+    const uint32_t na = f->na;
+    const uint32_t lane = lane_id();
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
+        const uint64_t i = base + lane;
+        bool join = false;
+        uint32_t v = 0, nc = 0;
+        if (i < na) {
+            v = al[i];
+            if (state[v] == 1 && (unsigned long long)tri[v] <= k) {
+                join = true;
+                state[v] = 2;
+                core[v] = (uint32_t)k;
+                nc = (B.cend[v] - B.cbeg[v] + CORE_CHUNK - 1) / CORE_CHUNK;
+            }
+        }
+        const uint64_t mask = __ballot(join);
+        if (!mask) continue;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+        // inclusive scan of nc over the wave
+        uint32_t inc = nc;
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
+        }
+        const uint32_t tot = __shfl(inc, 63, 64);
+        uint32_t vpos = 0, epos = 0;
+        if (lane == leader) {
+            vpos = atomicAdd(&f->nfv, (uint32_t)__popcll(mask));
+            epos = tot ? atomicAdd(&f->nent, tot) : 0u;
+        }
+        vpos = __shfl(vpos, leader, 64);
+        epos = __shfl(epos, leader, 64);
+        if (join) {
+            B.fvert[vpos + mask_rank(mask)] = v;
+            const uint32_t e0 = epos + inc - nc;
+            for (uint32_t c = 0; c < nc; c++) {
+                B.ent[2 * (e0 + c)] = v;
+                B.ent[2 * (e0 + c) + 1] = c;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
+}
+
+// First index in sorted a[lo, hi) with a[i] >= x, searching forward from lo with doubling steps.
+__device__ __forceinline__ uint32_t gallop_lb(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t x) {
+    if (lo >= hi || a[lo] >= x) return lo;
+    uint32_t prev = lo, cur = lo + 1, step = 1;   // a[prev] < x
+    while (cur < hi && a[cur] < x) {
+        prev = cur;
+        step <<= 1;
+        cur = (hi - cur > step) ? cur + step : hi;
+    }
+    return lower_bound_u32(a, prev + 1, cur, x);   // cur == hi or a[cur] >= x
+}
+
+// Triangles {u, v, w} (w > v) through entry (u, chunk): subtract once, by the smallest-id frontier member.
+__global__ void k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state, uint32_t* __restrict__ tri,
+                            const CoreDev* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
+    const uint32_t nent = f->nent;
+    const uint32_t lane = lane_id();
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t ei = wave0; ei < nent; ei += nwaves) {
+        const uint32_t u = B.ent[2 * ei], chunk = B.ent[2 * ei + 1];
+        const uint32_t ub = B.cbeg[u], ue = B.cend[u];
+        const uint32_t pb = ub + chunk * CORE_CHUNK;
+        const uint32_t pe = min(ue, pb + CORE_CHUNK);
+        for (uint32_t p = pb + lane; p < pe; p += 64) {
+            const uint32_t v = adj[p];
+            const uint8_t sv = state[v];
+            if (sv == 0) continue;
+            const uint32_t vb = B.cbeg[v], ve = B.cend[v];
+            uint32_t a = p + 1;                                       // cN(u) after v: all > v
+            uint32_t b = upper_bound_u32(adj, vb, ve, v);             // cN(v) after v
+            const bool a_short = (ue - a) <= (ve - b);
+            uint32_t s = a_short ? a : b;
+            const uint32_t se = a_short ? ue : ve;
+            uint32_t l = a_short ? b : a;
+            const uint32_t le = a_short ? ve : ue;
+            for (; s < se && l < le; s++) {
+                const uint32_t w = adj[s];
+                l = gallop_lb(adj, l, le, w);
+                if (l >= le) break;
+                if (adj[l] != w) continue;
+                const uint8_t sw = state[w];
+                if (sw == 0) continue;
+                if ((sv == 2 && v < u) || (sw == 2 && w < u)) continue;   // a smaller frontier member owns it
+                if (sv == 1) atomicSub(&tri[v], 1u);
+                if (sw == 1) atomicSub(&tri[w], 1u);
+            }
+        }
+    }
+}
+
+__global__ void k_cf_retire(CoreBufs B, const CoreFusedDev* __restrict__ f, uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t nfv = f->nfv;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nfv; i += gridDim.x * blockDim.x) state[B.fvert[i]] = 0;
+}
+
+// Round epilogue (dsg.h:744-747 rules) and the compaction decision.
+__global__ void k_cf_end(CoreFusedDev* f, CoreDev* st) {
+    if (st->done) return;
+    const unsigned long long removed = f->nfv, k = st->k, nalive = st->nalive;
+    st->removed = removed;
+    if (removed == 0 || removed > nalive || k < st->k_prev) { st->error = CORE_ERR_STATS; st->done = 1; return; }
+    st->nalive = nalive - removed;
+    st->k_prev = k;
+    st->rounds++;
+    st->min_tri = 0xFFFFFFFFu;
+    f->nfv = 0;
+    f->nent = 0;
+    if (st->nalive == 0) { st->done = 1; return; }
+    f->compact = 2ull * st->nalive <= f->na ? 1u : 0u;
+    f->na_new = 0;
+    f->adj_cursor = 0;
+}
+
+// Compaction 1/2: alive entries of alist go to the other alist buffer.
+__global__ void k_cf_compact_list(CoreBufs B, CoreFusedDev* f, const uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
+    if (st->done || !f->compact) return;
+    const uint32_t* __restrict__ al = B.alist[f->alist_cur];
+    uint32_t* __restrict__ out = B.alist[f->alist_cur ^ 1u];
+    const uint32_t na = f->na;
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {
+        const uint64_t i = base + lane_id();
+        uint32_t v = 0;
+        const bool keep = i < na && state[(v = al[i])] == 1;
+        const uint64_t mask = __ballot(keep);
+        if (!mask) continue;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+        uint32_t pos = 0;
+        if (lane_id() == leader) pos = atomicAdd(&f->na_new, (uint32_t)__popcll(mask));
+        pos = __shfl(pos, leader, 64);
+        if (keep) out[pos + mask_rank(mask)] = v;
+    }
+}
+
+// Compaction 2/2: one wave per surviving vertex copies its alive neighbours (in order) into the next adjacency buffer.
+__global__ void k_cf_compact_adj(CoreBufs B, CoreFusedDev* f, const uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
+    if (st->done || !f->compact) return;
+    const uint32_t* __restrict__ al = B.alist[f->alist_cur ^ 1u];
+    const uint32_t na = f->na_new;
+    const uint32_t cur = f->adj_cur, nxt = cur == 1 ? 2u : 1u;
+    const uint32_t* __restrict__ src = B.adj[cur];
+    uint32_t* __restrict__ dst = B.adjw[nxt];
+    const uint32_t lane = lane_id();
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t i = wave0; i < na; i += nwaves) {
+        const uint32_t v = al[i];
+        const uint32_t b = B.cbeg[v], e = B.cend[v];
+        uint32_t cnt = 0;
+        for (uint32_t p = b + lane; p < e; p += 64) cnt += state[src[p]] == 1;
+        cnt = (uint32_t)wave_sum_u64(cnt);
+        unsigned long long base = 0;
+        if (lane == 0 && cnt) base = atomicAdd(&f->adj_cursor, (unsigned long long)cnt);
+        base = __shfl(base, 0, 64);
+        uint32_t w = (uint32_t)base;
+        for (uint32_t p0 = b; p0 < e; p0 += 64) {   // wave-uniform, keeps the sorted order
+            const uint32_t p = p0 + lane;
+            const bool keep = p < e && state[src[p]] == 1;
+            const uint64_t mask = __ballot(keep);
+            if (keep) dst[w + mask_rank(mask)] = src[p];
+            w += (uint32_t)__popcll(mask);
+        }
+        if (lane == 0) {
+            B.cbeg[v] = (uint32_t)base;
+            B.cend[v] = (uint32_t)base + cnt;
+        }
+    }
+}
+
+// Switch to the compacted buffers (runs after both compaction kernels).
+__global__ void k_cf_swap(CoreFusedDev* f, const CoreDev* __restrict__ st) {
+    if (st->done || !f->compact) return;
+    f->na = f->na_new;
+    f->alist_cur ^= 1u;
+    f->adj_cur = f->adj_cur == 1 ? 2u : 1u;
+    f->compact = 0;
+}
+
+struct CoreFusedMem {
+    ga_ctx* ctx = nullptr;
+    void* block = nullptr;
+    CoreBufs B{};
+    CoreFusedDev* f = nullptr;
+    ~CoreFusedMem() { ga_free(block); }
+};
+
+static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
+    const size_t n = g->n, m2 = 2ull * g->m;
+    const size_t nent = n + m2 / CORE_CHUNK + 16;
+    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 2 * n + n + 2 * nent + 64 + 16;
+    GA_TRY(ga_malloc(ctx, &M.block, words * 4));
+    uint32_t* p = (uint32_t*)M.block;
+    M.B.alist[0] = p; p += n;
+    M.B.alist[1] = p; p += n;
+    M.B.adj[0] = g->ucol;
+    M.B.adjw[0] = nullptr;
+    M.B.adjw[1] = p; M.B.adj[1] = p; p += std::max<size_t>(m2, 1);
+    M.B.adjw[2] = p; M.B.adj[2] = p; p += std::max<size_t>(m2, 1);
+    M.B.cbeg = p; p += n;
+    M.B.cend = p; p += n;
+    M.B.fvert = p; p += n;
+    M.B.ent = p; p += 2 * nent;
+    p = (uint32_t*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
+    M.f = (CoreFusedDev*)p;
+    return GA_OK;
+}
+
+// Whole decomposition with the round loop on the device (incremental design with compaction; the recount design is
+// available through the per-round calls).
 extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms) {
     ga_ctx* ctx = c->ctx;
     ga_graph* g = c->g;
@@ -318,10 +609,14 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     if (T_full != UINT64_MAX && T_full >= (1ull << 32))
         return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
     GA_HIP(ctx, hipSetDevice(ctx->device));
+    CoreFusedMem M;
+    GA_TRY(cf_alloc(ctx, g, M));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_core_reset_state(c));
     k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, T_full);
     GA_TRY(ga_launch_check(ctx, "k_core_init"));
+    k_cf_init<<<cgrid(ctx, g->n), 256, 0, ctx->stream>>>(M.f, M.B.alist[0], M.B.cbeg, M.B.cend, g->urow, g->n);
+    GA_TRY(ga_launch_check(ctx, "k_cf_init"));
     if (g->n) {
         GA_TRY(init_tri_async(c));
         k_core_check_T<<<1, 1, 0, ctx->stream>>>(c->st);
@@ -331,11 +626,14 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     const unsigned gv = cgrid(ctx, g->n), gu = ugrid(ctx);
     for (uint64_t guard = 0;; guard++) {
         for (int r = 0; r < block_rounds; r++) {
-            k_core_min<true><<<gv, 256, 0, ctx->stream>>>(c->tri, c->state, g->n, c->st);
-            k_core_find<true><<<gv, 256, 0, ctx->stream>>>(c->tri, c->state, c->core, g->n, 0, c->frontier, c->st);
-            k_core_update<true><<<gu, 256, 0, ctx->stream>>>(g->urow, g->ucol, c->state, c->frontier, c->st, c->tri);
-            k_core_retire<true><<<gu, 256, 0, ctx->stream>>>(c->state, c->frontier, c->st);
-            k_core_end<<<1, 1, 0, ctx->stream>>>(c->st);
+            k_cf_min<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
+            k_cf_find<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->core, c->st);
+            k_cf_update<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
+            k_cf_retire<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
+            k_cf_end<<<1, 1, 0, ctx->stream>>>(M.f, c->st);
+            k_cf_compact_list<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
+            k_cf_compact_adj<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
+            k_cf_swap<<<1, 1, 0, ctx->stream>>>(M.f, c->st);
         }
         GA_TRY(ga_launch_check(ctx, "core round"));
         GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, c->st, sizeof(CoreDev), hipMemcpyDeviceToHost, ctx->stream));

@@ -63,17 +63,31 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ keys, uint64_t k, ui
     if (lane_id() == 0 && nd) atomicAdd(dup, nd);
 }
 
-// Compacts unique keys and counts undirected degrees.
+// Compacts unique keys and counts undirected degrees. Keys are sorted, so the smaller endpoints of a wave's edges form
+// runs: each run adds its length with one atomic (hubs with low ids would otherwise serialise on one address).
 __global__ void k_compact_degrees(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
                                   const uint32_t* __restrict__ pos, uint64_t k, int nb, uint64_t* __restrict__ ukeys,
                                   uint32_t* __restrict__ deg) {
     const uint64_t mask = (1ull << nb) - 1ull;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x) {
-        if (!flag[i]) continue;
-        const uint64_t x = keys[i];
-        ukeys[pos[i]] = x;
-        atomicAdd(&deg[(uint32_t)(x >> nb)], 1u);
-        atomicAdd(&deg[(uint32_t)(x & mask)], 1u);
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < k; base += nwaves * 64) {   // wave-uniform loop
+        const uint64_t i = base + lane_id();
+        const bool f = i < k && flag[i];
+        uint32_t u = 0;
+        if (f) {
+            const uint64_t x = keys[i];
+            ukeys[pos[i]] = x;
+            u = (uint32_t)(x >> nb);
+            atomicAdd(&deg[(uint32_t)(x & mask)], 1u);
+        }
+        uint64_t todo = __ballot(f);
+        while (todo) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1;
+            const uint32_t ul = __shfl(u, leader, 64);
+            const uint64_t same = __ballot(f && u == ul);
+            if (lane_id() == leader) atomicAdd(&deg[ul], (uint32_t)__popcll(same));
+            todo &= ~same;
+        }
     }
 }
 
@@ -296,6 +310,8 @@ static void free_plan(TcPlan& p) {
     ga_free(p.items);
     ga_free(p.wcol);
     ga_free(p.wedge);
+    ga_free(p.wps);
+    ga_free(p.wlen);
     ga_free(p.perm);
     p = TcPlan();
 }

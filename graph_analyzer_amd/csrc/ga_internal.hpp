@@ -16,6 +16,7 @@
 struct ga_ctx {
     int device = 0;
     int cus = 0;
+    size_t lds_max = 65536;             // dynamic LDS bytes one workgroup may use
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t* hstat = nullptr;          // pinned host staging for per-round statistics (512 bytes)
@@ -28,11 +29,13 @@ struct ga_ctx {
 // Device-resident count plan of GA_TC_HASH: work items grouped by the endpoint whose out-list is hashed.
 struct TcPlan {
     bool ready = false;
-    uint32_t nitems[4] = {0, 0, 0, 0};   // work items per hashed bucket
+    uint32_t nitems[5] = {0, 0, 0, 0, 0};   // work items per hashed bucket
     uint32_t nhuge = 0;                  // work items whose d+(h) is too large for an LDS table (binary search)
     uint32_t* items = nullptr;           // 3 words per item (h, begin, end) into wcol, bucket-major
-    uint32_t* wcol = nullptr;            // probe-side vertex per work entry (grouped by hashed vertex)
+    uint32_t* wcol = nullptr;            // probe-side vertex per work entry (rows of the hashed vertex, longest first)
     uint32_t* wedge = nullptr;           // oriented edge id of each work entry
+    uint32_t* wps = nullptr;             // rowptr[p] of each work entry (start of the probe list)
+    uint32_t* wlen = nullptr;            // d+(p) of each work entry (length of the probe list)
     uint32_t nwork = 0;                  // work entries (edges with min(d+) > 0)
     ga_tc_tuning tune = {};             // block == 0: defaults not applied yet
     uint32_t* perm = nullptr;            // GA_TC_SORTED work order (ga_graph_edge_order), lazily
@@ -68,6 +71,8 @@ int ga_launch_check(ga_ctx* ctx, const char* what);
 
 // device-side primitives implemented in ga_prims.hip
 int ga_sort_u64(ga_ctx* ctx, uint64_t* keys, uint64_t* tmp, size_t count, int end_bit, uint64_t** result);
+int ga_sort_pairs_u64(ga_ctx* ctx, unsigned long long* keys, unsigned long long* keys_tmp, uint32_t* vals, uint32_t* vals_tmp,
+                      size_t count, int end_bit, unsigned long long** keys_out, uint32_t** vals_out);
 int ga_sort_pairs_u32(ga_ctx* ctx, uint32_t* keys, uint32_t* keys_tmp, uint32_t* vals, uint32_t* vals_tmp, size_t count,
                       int end_bit, uint32_t** keys_out, uint32_t** vals_out);
 int ga_exclusive_scan_u32(ga_ctx* ctx, const uint32_t* in, uint32_t* out, size_t count);

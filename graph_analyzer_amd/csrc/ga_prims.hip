@@ -66,6 +66,7 @@ extern "C" int ga_ctx_create(int device, ga_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return GA_ERR_HIP; }
     c->cus = prop.multiProcessorCount;
+    c->lds_max = prop.sharedMemPerBlock;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipHostMalloc((void**)&c->hstat, 512, hipHostMallocDefault) != hipSuccess ||
@@ -131,6 +132,23 @@ int ga_sort_pairs_u32(ga_ctx* ctx, uint32_t* keys, uint32_t* keys_tmp, uint32_t*
     *vals_out = vals;
     if (count <= 1) return GA_OK;
     rocprim::double_buffer<uint32_t> kb(keys, keys_tmp), vb(vals, vals_tmp);
+    size_t bytes = 0;
+    GA_HIP(ctx, rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, count, 0, end_bit, ctx->stream));
+    void* s = nullptr;
+    GA_TRY(ga_scratch(ctx, bytes, &s));
+    GA_HIP(ctx, rocprim::radix_sort_pairs(s, bytes, kb, vb, count, 0, end_bit, ctx->stream));
+    *keys_out = kb.current();
+    *vals_out = vb.current();
+    return GA_OK;
+}
+
+int ga_sort_pairs_u64(ga_ctx* ctx, unsigned long long* keys, unsigned long long* keys_tmp, uint32_t* vals, uint32_t* vals_tmp,
+                      size_t count, int end_bit, unsigned long long** keys_out, uint32_t** vals_out) {
+    *keys_out = keys;
+    *vals_out = vals;
+    if (count <= 1) return GA_OK;
+    rocprim::double_buffer<unsigned long long> kb(keys, keys_tmp);
+    rocprim::double_buffer<uint32_t> vb(vals, vals_tmp);
     size_t bytes = 0;
     GA_HIP(ctx, rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, count, 0, end_bit, ctx->stream));
     void* s = nullptr;

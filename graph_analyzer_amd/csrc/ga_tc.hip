@@ -83,13 +83,19 @@ __global__ void k_tc_group(const uint32_t* __restrict__ rowptr, const uint32_t* 
 // ---------------------------------------------------------------------------------------------------------------
 // hash variant
 // ---------------------------------------------------------------------------------------------------------------
-// Items: 3 words (h, begin, end) into wcol/wedge. Each lane group of G lanes owns one item at a time:
-//   1. clear a table of ts = next_pow2(d+(h)) << load_shift slots (>= 2x the entries, so every probe chain ends)
-//   2. insert N+(h) (LDS compare-and-swap, linear probing; dead vertices are not inserted)
-//   3. for each batch of G work edges (h, p): scan the lengths d+(p), then walk the concatenated lists G elements at a
-//      time; the owner of flat element f is found by a binary search over the group's inclusive lengths (shuffles),
-//      so consecutive lanes read consecutive col[] addresses.
-// HASHED=false is the bucket for huge d+(h): no table, each element is binary-searched in N+(h) in global memory.
+// Work entries: every oriented edge (u,v) with min(d+u, d+v) > 0, filed under the endpoint h with the LONGER out-list
+// (ties: u), the other endpoint p being the probe side. Entries are sorted by (h, d+(p) descending), so each row of h
+// lists its probe lists longest first. Rows are cut into items (h, begin, end) of bounded probe work and bounded
+// batch count, and items are bucketed by d+(h) (the LDS table size).
+//
+// One lane group of G lanes owns an item:
+//   1. clear a table of next_pow2(d+(h)) << load_shift slots (at least twice the entries: every probe chain ends)
+//   2. insert N+(h) by LDS compare-and-swap with linear probing (dead vertices are not inserted)
+//   3. take the item's probe lists G at a time (a batch), compact the non-empty ones in LDS (they stay longest first),
+//      and stream them in phases: lists of length in (S/2, S] are read by sub-groups of S lanes (G/S lists at once,
+//      one element per lane, consecutive lanes on consecutive addresses); lists longer than G/2 take the whole group.
+//      Every element probes the table; a hit is the triangle (h, p, x).
+// HASHED = false is the bucket for d+(h) above the LDS cap: the element is binary-searched in N+(h) in HBM instead.
 
 __device__ __forceinline__ void group_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -100,32 +106,64 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t x, uint32_t tbits) {
     return (x * 2654435769u) >> (32u - tbits);
 }
 
+// This lane group's bits of a wave ballot, shifted down to bit 0.
+template <int G>
+__device__ __forceinline__ uint64_t group_bits(uint64_t wave_mask) {
+    if (G == 64) return wave_mask;
+    const uint32_t goff = lane_id() & ~(uint32_t)(G - 1);
+    return (wave_mask >> goff) & ((1ull << G) - 1ull);
+}
+
+struct TcArgs {
+    const uint32_t* items;
+    uint32_t nitems;
+    uint32_t ts_max;       // table slots reserved per group
+    uint32_t load_shift;
+    const uint32_t* rowptr;
+    const uint32_t* col;
+    const uint32_t* wcol;
+    const uint32_t* wedge;
+    const uint32_t* wps;     // rowptr[p] of each work entry
+    const uint32_t* wlen;    // d+(p) of each work entry
+    const uint8_t* alive;
+    uint32_t* tri;
+    uint32_t* sup;
+    unsigned long long* total;
+    const uint32_t* done;
+    const uint32_t* gof;
+    unsigned long long* gcnt;
+};
+
 template <int G, bool HASHED, int FLAGS>
-__global__ void k_tc_hash(const uint32_t* __restrict__ items, uint32_t nitems, const uint32_t* __restrict__ rowptr,
-                          const uint32_t* __restrict__ col, const uint32_t* __restrict__ wcol, const uint32_t* __restrict__ wedge,
-                          const uint8_t* __restrict__ alive, uint32_t* __restrict__ tri, uint32_t* __restrict__ sup,
-                          unsigned long long* __restrict__ total, uint32_t ts_max, uint32_t load_shift,
-                          const uint32_t* __restrict__ done, const uint32_t* __restrict__ gof, unsigned long long* __restrict__ gcnt) {
+__global__ void k_tc_hash(const TcArgs a) {
     static_assert(G >= 4 && G <= 64 && (64 % G) == 0, "G must divide 64");
     extern __shared__ uint32_t lds[];
-    if (done && *done) return;
+    if (a.done && *a.done) return;
+    const uint32_t* __restrict__ rowptr = a.rowptr;
+    const uint32_t* __restrict__ col = a.col;
+    const uint8_t* __restrict__ alive = a.alive;
     const uint32_t sub = threadIdx.x % G;
     const uint32_t gib = threadIdx.x / G;
     const uint32_t gpb = blockDim.x / G;
-    uint32_t* tab = lds + (size_t)gib * ts_max * ((FLAGS & F_SUP) ? 2u : 1u);
-    uint32_t* tix = tab + ts_max;   // position of each key in N+(h) (F_SUP only)
+    const uint32_t tab_words = HASHED ? a.ts_max * ((FLAGS & (F_SUP | F_TRI)) ? 2u : 1u) : 0u;
+    uint32_t* tab = lds + (size_t)gib * (tab_words + 4u * G);
+    uint32_t* tix = tab + a.ts_max;          // F_SUP: position of each key in N+(h); F_TRI: hits per key
+    uint32_t* stg = tab + tab_words;         // batch staging: ps[G] | len[G] | p[G] | work index[G]
     uint64_t local = 0;
-    for (uint32_t it = blockIdx.x * gpb + gib; it < nitems; it += gridDim.x * gpb) {
-        const uint32_t h = items[3 * it], b = items[3 * it + 1], e = items[3 * it + 2];
+    for (uint32_t it = blockIdx.x * gpb + gib; it < a.nitems; it += gridDim.x * gpb) {
+        const uint32_t h = a.items[3 * it], b = a.items[3 * it + 1], e = a.items[3 * it + 2];
         if ((FLAGS & F_ALIVE) && !alive[h]) continue;
         const uint32_t hs = rowptr[h], dh = rowptr[h + 1] - hs;
         uint32_t tbits = 0, tsz = 1;
         if (HASHED) {
             // ceil(log2(dh)) + shift, at least 2: never more than the bucket's slots (bucket_slots uses the same rule)
-            tbits = (dh > 1 ? 32u - (uint32_t)__clz(dh - 1) : 0u) + load_shift;
+            tbits = (dh > 1 ? 32u - (uint32_t)__clz(dh - 1) : 0u) + a.load_shift;
             if (tbits < 2) tbits = 2;
             tsz = 1u << tbits;
-            for (uint32_t s = sub; s < tsz; s += G) tab[s] = GA_EMPTY;
+            for (uint32_t s = sub; s < tsz; s += G) {
+                tab[s] = GA_EMPTY;
+                if (FLAGS & F_TRI) tix[s] = 0;
+            }
             group_sync();
             for (uint32_t k = sub; k < dh; k += G) {
                 const uint32_t x = col[hs + k];
@@ -143,94 +181,137 @@ __global__ void k_tc_hash(const uint32_t* __restrict__ items, uint32_t nitems, c
             group_sync();
         }
         uint32_t hcount = 0;
+        // batch metadata of the first batch; later batches are fetched one batch ahead
+        uint32_t n_p = 0, n_ps = 0, n_len = 0;
+        {
+            const uint32_t j = b + sub;
+            if (j < e) {
+                n_ps = a.wps[j];
+                n_len = a.wlen[j];
+                if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[j];
+            }
+        }
         for (uint32_t base = b; base < e; base += G) {
             const uint32_t j = base + sub;
-            uint32_t p = 0, ps = 0, len = 0;
-            if (j < e) {
-                p = wcol[j];
-                if (!(FLAGS & F_ALIVE) || alive[p]) {
-                    ps = rowptr[p];
-                    len = rowptr[p + 1] - ps;
+            const uint32_t p = n_p, ps = n_ps;
+            uint32_t len = n_len;
+            if ((FLAGS & F_ALIVE) && len && !alive[p]) len = 0;
+            const uint64_t gm = group_bits<G>(__ballot(len > 0));
+            const uint32_t cnt = (uint32_t)__popcll(gm);
+            if (len > 0) {   // stable compaction keeps the batch longest first
+                const uint32_t r = (uint32_t)__popcll(gm & ((1ull << sub) - 1ull));
+                stg[r] = ps;
+                stg[G + r] = len;
+                if (FLAGS & F_TRI) stg[2 * G + r] = p;
+                if (FLAGS & F_SUP) stg[3 * G + r] = j;
+            }
+            {   // prefetch the next batch while this one is streamed
+                const uint32_t jn = base + G + sub;
+                n_p = n_ps = n_len = 0;
+                if (jn < e) {
+                    n_ps = a.wps[jn];
+                    n_len = a.wlen[jn];
+                    if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[jn];
                 }
             }
-            uint32_t incl = len;   // inclusive scan of len over the group
+            group_sync();
+            const uint32_t lk = sub < cnt ? stg[G + sub] : 0u;
+            uint32_t start = 0;
 #pragma unroll
-            for (uint32_t o = 1; o < G; o <<= 1) {
-                const uint32_t t = __shfl_up(incl, o, G);
-                if (sub >= o) incl += t;
+            for (uint32_t S = G; S >= 1; S >>= 1) {
+                // entries [start, end) have length in (S/2, S] (for S = G: every length above G/2); each list is read
+                // by W = S/U lanes (at least 1), every lane holding up to U loads in flight
+                constexpr uint32_t U = 4;
+                const uint32_t W = S >= U ? S / U : 1u;
+                const uint32_t end = (uint32_t)__popcll(group_bits<G>(__ballot(lk > S / 2)));
+                for (uint32_t k0 = start; k0 < end; k0 += G / W) {
+                    const uint32_t k = k0 + sub / W, r = sub % W;
+                    uint32_t c = 0;
+                    if (k < end) {
+                        const uint32_t kps = stg[k], klen = stg[G + k];
+                        for (uint32_t off = r; off < klen; off += U * W) {
+                            uint32_t x[U];
+#pragma unroll
+                            for (uint32_t u = 0; u < U; u++) x[u] = off + u * W < klen ? col[kps + off + u * W] : 0u;
+#pragma unroll
+                            for (uint32_t u = 0; u < U; u++) {
+                                if (off + u * W >= klen) break;
+                                const uint32_t addr = kps + off + u * W;
+                                bool hit = false;
+                                uint32_t hk = 0;
+                                if (HASHED) {
+                                    uint32_t s = hash_slot(x[u], tbits);
+                                    for (uint32_t probe = 0; probe < tsz; probe++) {
+                                        const uint32_t t = tab[s];
+                                        if (t == x[u]) {
+                                        hit = true;
+                                        if (FLAGS & F_SUP) hk = tix[s];
+                                        if (FLAGS & F_TRI) atomicAdd(&tix[s], 1u);   // x's hits gather in LDS
+                                        break;
+                                    }
+                                        if (t == GA_EMPTY) break;
+                                        s = (s + 1) & (tsz - 1);
+                                    }
+                                } else {
+                                    const uint32_t q = lower_bound_u32(col, hs, hs + dh, x[u]);
+                                    hit = q < hs + dh && col[q] == x[u] && (!(FLAGS & F_ALIVE) || alive[x[u]]);
+                                    hk = q - hs;
+                                }
+                                if (hit) {
+                                    c++;
+                                    if (!HASHED && (FLAGS & F_TRI)) atomicAdd(&a.tri[x[u]], 1u);
+                                    if (FLAGS & F_SUP) {
+                                        atomicAdd(&a.sup[hs + hk], 1u);
+                                        atomicAdd(&a.sup[addr], 1u);
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    local += c;
+                    if (FLAGS & (F_TRI | F_GRAPH)) hcount += c;
+                    if (FLAGS & (F_TRI | F_SUP)) {   // hits of list k, summed over its W lanes, go to p and edge (h,p)
+#pragma unroll
+                        for (uint32_t o = W / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, G);
+                        if (r == 0 && k < end && c) {
+                            if (FLAGS & F_TRI) atomicAdd(&a.tri[stg[2 * G + k]], c);
+                            if (FLAGS & F_SUP) atomicAdd(&a.sup[a.wedge[stg[3 * G + k]]], c);
+                        }
+                    }
+                }
+                start = end;
+                if (S == 1) break;
             }
-            const uint32_t tot = __shfl(incl, G - 1, G);
-            for (uint32_t f0 = 0; f0 < tot; f0 += G) {
-                const uint32_t f = f0 + sub;
-                uint32_t lo = 0, hi = G - 1;   // owner = first lane with incl > f
-#pragma unroll
-                for (uint32_t st = G; st > 1; st >>= 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    const uint32_t v = __shfl(incl, mid, G);
-                    if (v <= f) lo = mid + 1;
-                    else hi = mid;
-                }
-                const uint32_t o_ps = __shfl(ps, lo, G);
-                const uint32_t o_end = __shfl(incl, lo, G);
-                const uint32_t o_len = __shfl(len, lo, G);
-                const uint32_t o_p = __shfl(p, lo, G);
-                if (f < tot) {
-                    const uint32_t addr = o_ps + (f - (o_end - o_len));
-                    const uint32_t x = col[addr];
-                    bool hit = false;
-                    uint32_t hk = 0;
-                    if (HASHED) {
-                        uint32_t s = hash_slot(x, tbits);
-                        for (uint32_t probe = 0; probe < tsz; probe++) {
-                            const uint32_t t = tab[s];
-                            if (t == x) { hit = true; if (FLAGS & F_SUP) hk = tix[s]; break; }
-                            if (t == GA_EMPTY) break;
-                            s = (s + 1) & (tsz - 1);
-                        }
-                    } else {
-                        const uint32_t q = lower_bound_u32(col, hs, hs + dh, x);
-                        hit = q < hs + dh && col[q] == x && (!(FLAGS & F_ALIVE) || alive[x]);
-                        hk = q - hs;
-                    }
-                    if (hit) {
-                        local++;
-                        if (FLAGS & (F_TRI | F_GRAPH)) hcount++;
-                        if (FLAGS & F_TRI) {
-                            atomicAdd(&tri[x], 1u);
-                            atomicAdd(&tri[o_p], 1u);
-                        }
-                        if (FLAGS & F_SUP) {
-                            // work entry of this owner lane: its oriented edge is (h,p) or (p,h)
-                            const uint32_t oj = base + lo;
-                            atomicAdd(&sup[wedge[oj]], 1u);
-                            atomicAdd(&sup[hs + hk], 1u);
-                            atomicAdd(&sup[addr], 1u);
-                        }
-                    }
-                }
+            group_sync();   // staging is rewritten by the next batch
+        }
+        if (HASHED && (FLAGS & F_TRI)) {   // one atomic per hit key of N+(h) instead of one per triangle
+            group_sync();
+            for (uint32_t s = sub; s < tsz; s += G) {
+                const uint32_t cx = tix[s];
+                if (cx) atomicAdd(&a.tri[tab[s]], cx);
             }
         }
         if (FLAGS & (F_TRI | F_GRAPH)) {   // triangles of this item: h is a corner of each, h's graph owns them
 #pragma unroll
             for (uint32_t o = G / 2; o > 0; o >>= 1) hcount += __shfl_xor(hcount, o, G);
             if (sub == 0 && hcount) {
-                if (FLAGS & F_TRI) atomicAdd(&tri[h], hcount);
-                if (FLAGS & F_GRAPH) atomicAdd(&gcnt[gof[h]], (unsigned long long)hcount);
+                if (FLAGS & F_TRI) atomicAdd(&a.tri[h], hcount);
+                if (FLAGS & F_GRAPH) atomicAdd(&a.gcnt[a.gof[h]], (unsigned long long)hcount);
             }
         }
         group_sync();   // every lane is done with the table before the next item clears it
     }
     local = wave_sum_u64(local);
-    if (lane_id() == 0 && local) atomicAdd(total, (unsigned long long)local);
+    if (lane_id() == 0 && local) atomicAdd(a.total, (unsigned long long)local);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// count plan: work entries grouped by hashed endpoint, chunked by work, bucketed by d+(h)
+// count plan
 // ---------------------------------------------------------------------------------------------------------------
 namespace {
 
-const int NBUCKET = 5;                       // 4 hashed buckets + 1 binary-search bucket for huge d+(h)
-const uint32_t BUCKET_DEG[4] = {8, 32, 128, 4096};   // upper d+(h) of hashed buckets 0..3 (bucket 3: hash_cap)
+const int NHASH = 5;                                        // hashed buckets
+const uint32_t BUCKET_DEG[NHASH] = {8, 32, 128, 512, 4096};  // upper d+(h) per hashed bucket (the last one: hash_cap)
 
 __device__ __forceinline__ void hash_side(const uint32_t* __restrict__ rowptr, uint32_t u, uint32_t v, uint32_t& h, uint32_t& p,
                                           uint32_t& dmin) {
@@ -241,60 +322,86 @@ __device__ __forceinline__ void hash_side(const uint32_t* __restrict__ rowptr, u
     dmin = hu ? dv : du;
 }
 
-__global__ void k_plan_count(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
-                             uint32_t m, uint32_t* __restrict__ hcnt) {
+__device__ __forceinline__ uint32_t bucket_of(uint32_t dh, uint32_t cap) {
+    if (dh > cap) return NHASH;
+    return dh <= 8 ? 0u : dh <= 32 ? 1u : dh <= 128 ? 2u : dh <= 512 ? 3u : 4u;
+}
+
+struct PlanShape {   // entries per item before a row is cut, per bucket (lanes x batches)
+    uint32_t pmax[NHASH + 1];
+};
+
+// key = (h << 32) | ~d+(p): rows of h, longest probe list first; edges without work get the all-ones key (last).
+__global__ void k_plan_keys(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                            uint32_t m, unsigned long long* __restrict__ key, uint32_t* __restrict__ val, uint32_t* __restrict__ nwork) {
+    uint32_t cnt = 0;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
         uint32_t h, p, dmin;
         hash_side(rowptr, src[e], col[e], h, p, dmin);
-        if (dmin) atomicAdd(&hcnt[h], 1u);
+        key[e] = dmin ? (((unsigned long long)h << 32) | (uint32_t)~dmin) : ~0ull;
+        val[e] = e;
+        cnt += dmin != 0;
     }
+    cnt = (uint32_t)wave_sum_u64(cnt);
+    if (lane_id() == 0 && cnt) atomicAdd(nwork, cnt);
 }
 
-__global__ void k_plan_scatter(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
-                               uint32_t m, const uint32_t* __restrict__ hrow, uint32_t* __restrict__ cursor, uint32_t* __restrict__ wcol,
-                               uint32_t* __restrict__ wedge, uint32_t* __restrict__ wh, unsigned long long* __restrict__ wt) {
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
-        uint32_t h, p, dmin;
-        hash_side(rowptr, src[e], col[e], h, p, dmin);
-        if (!dmin) continue;
-        const uint32_t pos = hrow[h] + atomicAdd(&cursor[h], 1u);
-        wcol[pos] = p;
-        wedge[pos] = e;
-        wh[pos] = h;
-        wt[pos] = dmin;
-    }
-}
-
-// flag[j] = 1 where a work item starts: at each row start, and where the work prefix crosses a multiple of chunk.
-__global__ void k_plan_flags(const uint32_t* __restrict__ hrow, const uint32_t* __restrict__ wh, const unsigned long long* __restrict__ wsum,
-                             uint32_t nwork, uint32_t chunk, uint32_t* __restrict__ flag) {
+__global__ void k_plan_fill(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                            const uint32_t* __restrict__ val, uint32_t nwork, uint32_t* __restrict__ wcol, uint32_t* __restrict__ wedge,
+                            uint32_t* __restrict__ wps, uint32_t* __restrict__ wlen, uint32_t* __restrict__ wh,
+                            unsigned long long* __restrict__ wt) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) {
-        const uint32_t r0 = hrow[wh[j]];
+        const uint32_t e = val[j];
+        uint32_t h, p, dmin;
+        hash_side(rowptr, src[e], col[e], h, p, dmin);
+        wcol[j] = p;
+        wedge[j] = e;
+        wps[j] = rowptr[p];
+        wlen[j] = dmin;
+        wh[j] = h;
+        wt[j] = dmin;
+    }
+}
+
+__global__ void k_plan_rowstart(const uint32_t* __restrict__ wh, uint32_t nwork, uint32_t* __restrict__ hstart) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x)
+        if (j == 0 || wh[j - 1] != wh[j]) hstart[wh[j]] = j;
+}
+
+// flag[j] = 1 where an item starts: at a row start, where the row's work prefix crosses a multiple of `chunk`, and
+// every `pmax` entries of the row (pmax = lanes x batches of h's bucket).
+__global__ void k_plan_flags(const uint32_t* __restrict__ hstart, const uint32_t* __restrict__ wh, const unsigned long long* __restrict__ wsum,
+                             const uint32_t* __restrict__ rowptr, uint32_t nwork, uint32_t chunk, uint32_t cap, PlanShape shape,
+                             uint32_t* __restrict__ flag) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) {
+        const uint32_t h = wh[j];
+        const uint32_t r0 = hstart[h];
         uint32_t f = 1;
         if (j != r0) {
             const unsigned long long base = wsum[r0];
-            f = (wsum[j] - base) / chunk != (wsum[j - 1] - base) / chunk;
+            const uint32_t pmax = shape.pmax[bucket_of(rowptr[h + 1] - rowptr[h], cap)];
+            f = ((wsum[j] - base) / chunk != (wsum[j - 1] - base) / chunk) || ((j - r0) % pmax == 0);
         }
         flag[j] = f;
     }
 }
 
-__device__ __forceinline__ uint32_t bucket_of(uint32_t dh, uint32_t cap) {
-    if (dh > cap) return 4;
-    if (dh <= 8) return 0;
-    if (dh <= 32) return 1;
-    if (dh <= 128) return 2;
-    if (dh <= 4096) return 3;
-    return 4;
-}
-
-// Item starts -> per-bucket counts.
+// Item starts -> per-bucket counts (one atomic per bucket per wave).
 __global__ void k_plan_item_count(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ rowptr,
                                   uint32_t nwork, uint32_t cap, uint32_t* __restrict__ bcount) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) {
-        if (!flag[j]) continue;
-        const uint32_t h = wh[j];
-        atomicAdd(&bcount[bucket_of(rowptr[h + 1] - rowptr[h], cap)], 1u);
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < nwork; base += nwaves * 64) {
+        const uint64_t j = base + lane_id();
+        uint32_t bk = 0xFFu;
+        if (j < nwork && flag[j]) {
+            const uint32_t h = wh[j];
+            bk = bucket_of(rowptr[h + 1] - rowptr[h], cap);
+        }
+#pragma unroll
+        for (uint32_t b = 0; b <= NHASH; b++) {
+            const uint64_t mk = __ballot(bk == b);
+            if (mk && lane_id() == (uint32_t)__ffsll((unsigned long long)mk) - 1) atomicAdd(&bcount[b], (uint32_t)__popcll(mk));
+        }
     }
 }
 
@@ -302,16 +409,30 @@ __global__ void k_plan_item_count(const uint32_t* __restrict__ flag, const uint3
 __global__ void k_plan_items(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ fpos, const uint32_t* __restrict__ starts_by_rank,
                              const uint32_t* __restrict__ wh, const uint32_t* __restrict__ rowptr, uint32_t nwork, uint32_t nall,
                              uint32_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ items) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) {
-        if (!flag[j]) continue;
-        const uint32_t r = fpos[j];
-        const uint32_t end = r + 1 < nall ? starts_by_rank[r + 1] : nwork;
-        const uint32_t h = wh[j];
-        const uint32_t bk = bucket_of(rowptr[h + 1] - rowptr[h], cap);
-        const uint32_t pos = boff[bk] + atomicAdd(&bcur[bk], 1u);
-        items[3 * pos] = h;
-        items[3 * pos + 1] = j;
-        items[3 * pos + 2] = end;
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < nwork; base += nwaves * 64) {
+        const uint64_t j = base + lane_id();
+        uint32_t bk = 0xFFu, h = 0;
+        if (j < nwork && flag[j]) {
+            h = wh[j];
+            bk = bucket_of(rowptr[h + 1] - rowptr[h], cap);
+        }
+#pragma unroll
+        for (uint32_t b = 0; b <= NHASH; b++) {
+            const uint64_t mk = __ballot(bk == b);
+            if (!mk) continue;
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mk) - 1;
+            uint32_t pos0 = 0;
+            if (lane_id() == leader) pos0 = atomicAdd(&bcur[b], (uint32_t)__popcll(mk));
+            pos0 = __shfl(pos0, leader, 64);
+            if (bk == b) {
+                const uint32_t r = fpos[j];
+                const uint32_t pos = boff[b] + pos0 + mask_rank(mk);
+                items[3 * pos] = h;
+                items[3 * pos + 1] = (uint32_t)j;
+                items[3 * pos + 2] = r + 1 < nall ? starts_by_rank[r + 1] : nwork;
+            }
+        }
     }
 }
 
@@ -331,18 +452,27 @@ ga_tc_tuning default_tuning() {
     t.lanes[1] = 16;
     t.lanes[2] = 32;
     t.lanes[3] = 64;
+    t.lanes[4] = 64;
     t.block = 256;
-    t.work_chunk = 4096;
-    t.load_shift = 1;
+    t.work_chunk = 8192;
+    t.batches = 8;
+    t.load_shift = 2;
     t.hash_cap = 4096;
     return t;
 }
 
 bool tuning_valid(const ga_tc_tuning& t) {
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < NHASH; i++)
         if (t.lanes[i] != 4 && t.lanes[i] != 8 && t.lanes[i] != 16 && t.lanes[i] != 32 && t.lanes[i] != 64) return false;
-    return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.load_shift >= 1 && t.load_shift <= 3 &&
-           t.hash_cap >= 1 && t.hash_cap <= 4096;
+    return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
+           t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096;
+}
+
+PlanShape plan_shape(const ga_tc_tuning& t) {
+    PlanShape s;
+    for (int b = 0; b < NHASH; b++) s.pmax[b] = (uint32_t)(t.lanes[b] * t.batches);
+    s.pmax[NHASH] = (uint32_t)(64 * t.batches);
+    return s;
 }
 
 }  // namespace
@@ -351,80 +481,96 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     TcPlan& P = g->plan;
     if (!P.tune.block) P.tune = default_tuning();
     const uint32_t n = g->n, m = g->m;
-    DevBuf hcnt_b, hrow_b, wh_b, wt_b, wsum_b, flag_b, fpos_b, starts_b, bc_b;
-    GA_TRY(ga_malloc(ctx, &hcnt_b.p, ((size_t)n + 1) * 4));
-    GA_TRY(ga_malloc(ctx, &hrow_b.p, ((size_t)n + 1) * 4));
-    uint32_t* hcnt = (uint32_t*)hcnt_b.p;
-    uint32_t* hrow = (uint32_t*)hrow_b.p;
-    GA_TRY(ga_fill_u32(ctx, hcnt, (size_t)n + 1, 0));
-    if (m) {
-        k_plan_count<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, m, hcnt);
-        GA_TRY(ga_launch_check(ctx, "k_plan_count"));
-    }
-    GA_TRY(ga_exclusive_scan_u32(ctx, hcnt, hrow, (size_t)n + 1));
-    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, hrow + n, 4, hipMemcpyDeviceToHost, ctx->stream));
-    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const uint32_t nwork = (uint32_t)(ctx->hstat[0] & 0xFFFFFFFFu);
     ga_free(P.wcol);
     ga_free(P.wedge);
+    ga_free(P.wps);
+    ga_free(P.wlen);
     ga_free(P.items);
-    P.wcol = P.wedge = P.items = nullptr;
+    P.wcol = P.wedge = P.wps = P.wlen = P.items = nullptr;
+    for (int b = 0; b < NHASH; b++) P.nitems[b] = 0;
+    P.nhuge = 0;
+    P.nwork = 0;
+    DevBuf keys, keys2, vals, vals2, hstart_b, wh_b, wt_b, wsum_b, flag_b, fpos_b, starts_b, bc_b;
+    GA_TRY(ga_malloc(ctx, &bc_b.p, 128));
+    uint32_t* bc = (uint32_t*)bc_b.p;   // [0..6) bucket counts, [8..14) cursors, [16..24) offsets, [31] nwork
+    GA_TRY(ga_fill_u32(ctx, bc, 32, 0));
+    uint32_t nwork = 0;
+    uint32_t* sorted_vals = nullptr;
+    if (m) {
+        GA_TRY(ga_malloc(ctx, &keys.p, (size_t)m * 8));
+        GA_TRY(ga_malloc(ctx, &keys2.p, (size_t)m * 8));
+        GA_TRY(ga_malloc(ctx, &vals.p, (size_t)m * 4));
+        GA_TRY(ga_malloc(ctx, &vals2.p, (size_t)m * 4));
+        k_plan_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, m, (unsigned long long*)keys.p,
+                                                                     (uint32_t*)vals.p, bc + 31);
+        GA_TRY(ga_launch_check(ctx, "k_plan_keys"));
+        int nb = 1;
+        while (nb < 32 && (1ull << nb) < (uint64_t)n) nb++;
+        unsigned long long* ko = nullptr;
+        GA_TRY(ga_sort_pairs_u64(ctx, (unsigned long long*)keys.p, (unsigned long long*)keys2.p, (uint32_t*)vals.p, (uint32_t*)vals2.p, m,
+                                 32 + nb, &ko, &sorted_vals));
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, bc + 31, 4, hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        nwork = (uint32_t)(ctx->hstat[0] & 0xFFFFFFFFu);
+    }
     GA_TRY(ga_malloc(ctx, (void**)&P.wcol, std::max<size_t>(nwork, 1) * 4));
     GA_TRY(ga_malloc(ctx, (void**)&P.wedge, std::max<size_t>(nwork, 1) * 4));
+    GA_TRY(ga_malloc(ctx, (void**)&P.wps, std::max<size_t>(nwork, 1) * 4));
+    GA_TRY(ga_malloc(ctx, (void**)&P.wlen, std::max<size_t>(nwork, 1) * 4));
     P.nwork = nwork;
-    for (int b = 0; b < 4; b++) P.nitems[b] = 0;
-    P.nhuge = 0;
     if (nwork == 0) {
         GA_TRY(ga_malloc(ctx, (void**)&P.items, 12));
         P.ready = true;
         return GA_OK;
     }
+    GA_TRY(ga_malloc(ctx, &hstart_b.p, ((size_t)n + 1) * 4));
     GA_TRY(ga_malloc(ctx, &wh_b.p, (size_t)nwork * 4));
     GA_TRY(ga_malloc(ctx, &wt_b.p, ((size_t)nwork + 1) * 8));
     GA_TRY(ga_malloc(ctx, &wsum_b.p, ((size_t)nwork + 1) * 8));
     GA_TRY(ga_malloc(ctx, &flag_b.p, ((size_t)nwork + 1) * 4));
     GA_TRY(ga_malloc(ctx, &fpos_b.p, ((size_t)nwork + 1) * 4));
-    GA_TRY(ga_malloc(ctx, &bc_b.p, 64));
+    uint32_t* hstart = (uint32_t*)hstart_b.p;
     uint32_t* wh = (uint32_t*)wh_b.p;
     unsigned long long* wt = (unsigned long long*)wt_b.p;
     unsigned long long* wsum = (unsigned long long*)wsum_b.p;
     uint32_t* flag = (uint32_t*)flag_b.p;
     uint32_t* fpos = (uint32_t*)fpos_b.p;
-    uint32_t* bc = (uint32_t*)bc_b.p;   // [0..5) counts, [8..13) cursors
-    GA_TRY(ga_fill_u32(ctx, hcnt, (size_t)n + 1, 0));   // reused as the scatter cursor
+    k_plan_fill<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, sorted_vals, nwork, P.wcol, P.wedge, P.wps,
+                                                                     P.wlen, wh, wt);
+    GA_TRY(ga_launch_check(ctx, "k_plan_fill"));
+    k_plan_rowstart<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(wh, nwork, hstart);
+    GA_TRY(ga_launch_check(ctx, "k_plan_rowstart"));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nwork, 1, 0));
-    k_plan_scatter<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, m, hrow, hcnt, P.wcol, P.wedge, wh, wt);
-    GA_TRY(ga_launch_check(ctx, "k_plan_scatter"));
     GA_TRY(ga_exclusive_scan_u64(ctx, (const uint64_t*)wt, (uint64_t*)wsum, (size_t)nwork + 1));
-    k_plan_flags<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(hrow, wh, wsum, nwork, (uint32_t)P.tune.work_chunk, flag);
+    const uint32_t cap = (uint32_t)P.tune.hash_cap;
+    k_plan_flags<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(hstart, wh, wsum, g->rowptr, nwork, (uint32_t)P.tune.work_chunk, cap,
+                                                                      plan_shape(P.tune), flag);
     GA_TRY(ga_launch_check(ctx, "k_plan_flags"));
     GA_TRY(ga_fill_u32(ctx, flag + nwork, 1, 0));
     GA_TRY(ga_exclusive_scan_u32(ctx, flag, fpos, (size_t)nwork + 1));
-    GA_TRY(ga_fill_u32(ctx, bc, 16, 0));
-    const uint32_t cap = (uint32_t)P.tune.hash_cap;
     k_plan_item_count<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, wh, g->rowptr, nwork, cap, bc);
     GA_TRY(ga_launch_check(ctx, "k_plan_item_count"));
-    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, bc, 20, hipMemcpyDeviceToHost, ctx->stream));
-    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat + 3, fpos + nwork, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, bc, 32, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat + 4, fpos + nwork, 4, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    uint32_t cnt[NBUCKET];
+    uint32_t cnt[NHASH + 1];
     const uint32_t* hc = (const uint32_t*)ctx->hstat;
-    for (int b = 0; b < NBUCKET; b++) cnt[b] = hc[b];
-    const uint32_t nall = (uint32_t)(ctx->hstat[3] & 0xFFFFFFFFu);
+    for (int b = 0; b <= NHASH; b++) cnt[b] = hc[b];
+    const uint32_t nall = (uint32_t)(ctx->hstat[4] & 0xFFFFFFFFu);
     uint32_t off[8] = {0};
-    for (int b = 0; b < NBUCKET; b++) off[b + 1] = off[b] + cnt[b];
-    if (off[NBUCKET] != nall) return ga_fail(ctx, GA_ERR_STATE, "count plan: %u items bucketed, %u expected", off[NBUCKET], nall);
+    for (int b = 0; b <= NHASH; b++) off[b + 1] = off[b] + cnt[b];
+    if (off[NHASH + 1] != nall) return ga_fail(ctx, GA_ERR_STATE, "count plan: %u items bucketed, %u expected", off[NHASH + 1], nall);
     GA_TRY(ga_malloc(ctx, &starts_b.p, (size_t)nall * 4));
     GA_TRY(ga_malloc(ctx, (void**)&P.items, (size_t)nall * 12));
     k_plan_rank<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, fpos, nwork, (uint32_t*)starts_b.p);
     GA_TRY(ga_launch_check(ctx, "k_plan_rank"));
-    GA_HIP(ctx, hipMemcpyAsync(bc, off, 8 * 4, hipMemcpyHostToDevice, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(bc + 16, off, 8 * 4, hipMemcpyHostToDevice, ctx->stream));
     k_plan_items<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, cap,
-                                                                      bc, bc + 8, P.items);
+                                                                      bc + 16, bc + 8, P.items);
     GA_TRY(ga_launch_check(ctx, "k_plan_items"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    for (int b = 0; b < 4; b++) P.nitems[b] = cnt[b];
-    P.nhuge = cnt[4];
+    for (int b = 0; b < NHASH; b++) P.nitems[b] = cnt[b];
+    P.nhuge = cnt[NHASH];
     P.ready = true;
     return GA_OK;
 }
@@ -445,53 +591,45 @@ static uint32_t bucket_slots(uint32_t bound, uint32_t shift) {
 }
 
 template <int G, bool HASHED, int FLAGS>
-static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, uint32_t bound,
-                              unsigned long long* total, const uint8_t* alive, uint32_t* tri, uint32_t* sup, const uint32_t* done,
-                              const uint32_t* gof, unsigned long long* gcnt) {
+static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, uint32_t bound, TcArgs a) {
     if (!nitems) return GA_OK;
     const TcPlan& P = g->plan;
     uint32_t block = (uint32_t)P.tune.block;
     const uint32_t ts = HASHED ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), (uint32_t)P.tune.load_shift) : 0;
-    const uint32_t per_group = ts * 4u * ((FLAGS & F_SUP) ? 2u : 1u);
+    const uint32_t per_group = (ts * ((FLAGS & (F_SUP | F_TRI)) ? 2u : 1u) + 4u * G) * 4u;
     while (block > 64 && (size_t)(block / G) * per_group > 64 * 1024) block >>= 1;
     const size_t shmem = (size_t)(block / G) * per_group;
-    if (shmem > 160 * 1024) return ga_fail(ctx, GA_ERR_ARG, "hash table of %u slots does not fit LDS", ts);
-    const uint32_t gpb = block / G;
+    if (HASHED && shmem > ctx->lds_max)   // the tables do not fit one workgroup: binary-search this bucket in HBM
+        return launch_hash_bucket<64, false, FLAGS>(ctx, g, items, nitems, bound, a);
     const unsigned grid = grid_for((uint64_t)nitems * G, block, (unsigned)std::max(ctx->cus, 1) * 64u);
-    (void)gpb;
-    k_tc_hash<G, HASHED, FLAGS><<<grid, block, shmem, ctx->stream>>>(items, nitems, g->rowptr, g->col, P.wcol, P.wedge, alive, tri, sup,
-                                                                      total, ts, (uint32_t)P.tune.load_shift, done, gof, gcnt);
+    a.items = items;
+    a.nitems = nitems;
+    a.ts_max = ts;
+    a.load_shift = (uint32_t)P.tune.load_shift;
+    k_tc_hash<G, HASHED, FLAGS><<<grid, block, shmem, ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_hash");
 }
 
-template <int G, int FLAGS>
-static int launch_hash_g(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, uint32_t bound, unsigned long long* total,
-                         const uint8_t* alive, uint32_t* tri, uint32_t* sup, const uint32_t* done, const uint32_t* gof,
-                         unsigned long long* gcnt) {
-    return launch_hash_bucket<G, true, FLAGS>(ctx, g, items, nitems, bound, total, alive, tri, sup, done, gof, gcnt);
-}
-
 template <int FLAGS>
-static int launch_hash(ga_ctx* ctx, ga_graph* g, unsigned long long* total, const uint8_t* alive, uint32_t* tri, uint32_t* sup,
-                       const uint32_t* done, const uint32_t* gof, unsigned long long* gcnt) {
+static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a) {
     const TcPlan& P = g->plan;
     uint32_t off = 0;
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < NHASH; b++) {
         const uint32_t* items = P.items + 3ull * off;
         const uint32_t ni = P.nitems[b];
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
         int rc = GA_OK;
         switch (P.tune.lanes[b]) {
-            case 4: rc = launch_hash_g<4, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
-            case 8: rc = launch_hash_g<8, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
-            case 16: rc = launch_hash_g<16, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
-            case 32: rc = launch_hash_g<32, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
-            default: rc = launch_hash_g<64, FLAGS>(ctx, g, items, ni, bound, total, alive, tri, sup, done, gof, gcnt); break;
+            case 4: rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, items, ni, bound, a); break;
+            case 8: rc = launch_hash_bucket<8, true, FLAGS>(ctx, g, items, ni, bound, a); break;
+            case 16: rc = launch_hash_bucket<16, true, FLAGS>(ctx, g, items, ni, bound, a); break;
+            case 32: rc = launch_hash_bucket<32, true, FLAGS>(ctx, g, items, ni, bound, a); break;
+            default: rc = launch_hash_bucket<64, true, FLAGS>(ctx, g, items, ni, bound, a); break;
         }
         GA_TRY(rc);
         off += ni;
     }
-    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 3ull * off, P.nhuge, 0, total, alive, tri, sup, done, gof, gcnt);
+    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 3ull * off, P.nhuge, 0, a);
 }
 
 template <int FLAGS>
@@ -517,13 +655,27 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
     const int flags = (alive ? F_ALIVE : 0) | (tri ? F_TRI : 0) | (sup ? F_SUP : 0) | (gcnt ? F_GRAPH : 0);
     if (variant == GA_TC_HASH) {
         GA_TRY(ga_tc_ensure_plan(ctx, g));
+        TcArgs a{};
+        a.rowptr = g->rowptr;
+        a.col = g->col;
+        a.wcol = g->plan.wcol;
+        a.wedge = g->plan.wedge;
+        a.wps = g->plan.wps;
+        a.wlen = g->plan.wlen;
+        a.alive = alive;
+        a.tri = tri;
+        a.sup = sup;
+        a.total = total;
+        a.done = done;
+        a.gof = gof;
+        a.gcnt = gcnt;
         switch (flags) {
-            case 0: return launch_hash<0>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
-            case F_TRI: return launch_hash<F_TRI>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
-            case F_ALIVE | F_TRI: return launch_hash<F_ALIVE | F_TRI>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
-            case F_SUP: return launch_hash<F_SUP>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
-            case F_GRAPH: return launch_hash<F_GRAPH>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
-            case F_ALIVE | F_TRI | F_GRAPH: return launch_hash<F_ALIVE | F_TRI | F_GRAPH>(ctx, g, total, alive, tri, sup, done, gof, gcnt);
+            case 0: return launch_hash<0>(ctx, g, a);
+            case F_TRI: return launch_hash<F_TRI>(ctx, g, a);
+            case F_ALIVE | F_TRI: return launch_hash<F_ALIVE | F_TRI>(ctx, g, a);
+            case F_SUP: return launch_hash<F_SUP>(ctx, g, a);
+            case F_GRAPH: return launch_hash<F_GRAPH>(ctx, g, a);
+            case F_ALIVE | F_TRI | F_GRAPH: return launch_hash<F_ALIVE | F_TRI | F_GRAPH>(ctx, g, a);
             default: return ga_fail(ctx, GA_ERR_ARG, "unsupported flag combination %d for the hash variant", flags);
         }
     }
@@ -597,6 +749,18 @@ extern "C" int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, f
     return GA_OK;
 }
 
+extern "C" int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32_t* nitems6) {
+    if (!ctx || !g) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    GA_TRY(ga_tc_ensure_plan(ctx, g));
+    if (nwork) *nwork = g->plan.nwork;
+    if (nitems6) {
+        for (int b = 0; b < NHASH; b++) nitems6[b] = g->plan.nitems[b];
+        nitems6[NHASH] = g->plan.nhuge;
+    }
+    return GA_OK;
+}
+
 extern "C" int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out) {
     if (!g || !out) return GA_ERR_ARG;
     *out = g->plan.tune.block ? g->plan.tune : default_tuning();
@@ -606,7 +770,9 @@ extern "C" int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out) {
 extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t) {
     if (!ctx || !g || !t) return GA_ERR_ARG;
     if (!tuning_valid(*t)) return ga_fail(ctx, GA_ERR_ARG, "invalid tuning");
-    const bool replan = !g->plan.tune.block || g->plan.tune.work_chunk != t->work_chunk || g->plan.tune.hash_cap != t->hash_cap;
+    bool replan = !g->plan.tune.block || g->plan.tune.work_chunk != t->work_chunk || g->plan.tune.hash_cap != t->hash_cap ||
+                  g->plan.tune.batches != t->batches;
+    for (int b = 0; b < NHASH; b++) replan = replan || g->plan.tune.lanes[b] != t->lanes[b];
     g->plan.tune = *t;
     if (replan) g->plan.ready = false;
     return GA_OK;
@@ -636,22 +802,25 @@ extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* b
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
-    // coordinate search: lanes per bucket, then block, load shift and work chunk
+    // coordinate search: lanes per bucket, then block, load shift, batches and work chunk
     GA_TRY(trial(base));
     const int lane_opts[5] = {4, 8, 16, 32, 64};
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < NHASH; b++) {
+        if (!g->plan.nitems[b]) continue;
         for (int li = 0; li < 5; li++) {
             ga_tc_tuning t = bt;
             t.lanes[b] = lane_opts[li];
             if (t.lanes[b] == bt.lanes[b]) continue;
-            if ((b == 3 && t.lanes[b] < 32) || (b == 2 && t.lanes[b] < 8)) continue;   // tables would not fit LDS
+            if (b >= 3 && t.lanes[b] < 16) continue;   // large tables per few lanes would not fit LDS
             GA_TRY(trial(t));
         }
     }
     const int blocks[3] = {128, 256, 512};
     for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.block = blocks[bi]; if (t.block != bt.block) GA_TRY(trial(t)); }
-    for (int ls = 1; ls <= 2; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
-    const int chunks[3] = {1024, 4096, 16384};
+    for (int ls = 1; ls <= 3; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
+    const int bat[3] = {2, 8, 32};
+    for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
+    const int chunks[3] = {2048, 8192, 32768};
     for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
     GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
     GA_TRY(ga_tc_ensure_plan(ctx, g));

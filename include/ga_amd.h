@@ -82,14 +82,18 @@ int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, float* ms);
 int ga_tc_prepare(ga_ctx* ctx, ga_graph* g, float* ms);
 /* Lane-group width / hash capacity / block shape per degree bucket (the autotuner's output, see ga_tc_tune). */
 typedef struct ga_tc_tuning {
-    int32_t lanes[4];      /* lanes per work item for hash buckets 0..3 (each one of 4, 8, 16, 32, 64) */
+    int32_t lanes[5];      /* lanes per work item for the hashed buckets d+(h) <= 8, 32, 128, 512, hash_cap
+                              (each one of 4, 8, 16, 32, 64) */
     int32_t block;         /* threads per workgroup (64..1024, multiple of 64) */
-    int32_t work_chunk;    /* probe elements per work item before a hub's list is split */
+    int32_t work_chunk;    /* probe elements per work item before a row is cut */
+    int32_t batches;       /* lane-group batches (lanes probe lists each) per work item before a row is cut */
     int32_t load_shift;    /* hash table slots = next_pow2(d+(h)) << load_shift (1..3) */
-    int32_t hash_cap;      /* largest d+(h) kept in an LDS table; larger rows use binary search in HBM */
+    int32_t hash_cap;      /* largest d+(h) kept in an LDS table (<= 4096); larger rows use binary search in HBM */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
 int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t);
+/* Count plan statistics: work entries (edges with min(d+) > 0), items per bucket (5 hashed + 1 binary-search). */
+int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32_t* nitems6);
 /* Times candidate tunings on this graph (each verified against the first count) and keeps the fastest. */
 int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* best_ms);
 /* variant: GA_TC_MERGE or GA_TC_HASH (GA_TC_AUTO = GA_TC_HASH) for the two calls below.

@@ -173,12 +173,12 @@ def test_tuning_shapes(ctx):
     _, sup_ref = og.edge_support()
     g = ga.DeviceGraph.from_pairs(ctx, raw)
     base = g.get_tuning()
-    for lanes in ([4, 4, 8, 32], [8, 16, 32, 64], [16, 32, 64, 64], [64, 64, 64, 64], [4, 8, 16, 32]):
+    for lanes in ([4, 4, 8, 16, 32], [8, 16, 32, 64, 64], [16, 32, 64, 64, 64], [64, 64, 64, 64, 64], [4, 8, 16, 32, 64]):
         for block in (64, 256, 1024):
-            for chunk, shift in ((64, 1), (4096, 2), (1 << 20, 3)):
+            for chunk, shift, batches in ((64, 1, 1), (4096, 2, 8), (1 << 20, 3, 1024)):
                 t = ga.TcTuning()
                 t.lanes[:] = lanes
-                t.block, t.work_chunk, t.load_shift = block, chunk, shift
+                t.block, t.work_chunk, t.load_shift, t.batches = block, chunk, shift, batches
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)

@@ -12,6 +12,15 @@
 // ga_core_run keeps k / nalive / rounds on the device (CoreDev) and enqueues rounds in blocks, one readback per block.
 #include "ga_internal.hpp"
 
+__device__ __forceinline__ void group_sync_wave() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t hash_slot_u32(uint32_t x, uint32_t tbits) { return (x * 2654435769u) >> (32u - tbits); }
+
+#define GA_EMPTY 0xFFFFFFFFu
+
 struct CoreDev {
     unsigned long long T;          // stats[0]: count of the recount / init pass
     unsigned long long removed;    // stats[1]: removed this round (= frontier size, incremental)
@@ -321,7 +330,7 @@ extern "C" int ga_core_download(ga_core* c, uint32_t* core) {
 // Compaction runs on the device (no host size readback) whenever at most half of alist is still alive: alist is
 // filtered, and each surviving vertex copies its surviving neighbours into the other adjacency buffer, reserving
 // space with one atomic per vertex. Every kernel returns at once once `done` is set.
-#define CORE_CHUNK 256u
+#define CORE_CHUNK 32u
 
 struct CoreFusedDev {
     uint32_t na;          // alist length
@@ -331,11 +340,14 @@ struct CoreFusedDev {
     uint32_t na_new;      // compaction cursor for the new alist
     uint32_t nfv;         // frontier vertices this round
     uint32_t nent;        // frontier entries this round
-    uint32_t pad;
+    uint32_t tcursor;     // words of the round's table arena in use
     unsigned long long adj_cursor;   // compaction cursor in the new adjacency buffer
 };
 
 struct CoreBufs {
+    const uint32_t* hubid;    // bitmap index of each vertex or GA_EMPTY
+    const uint32_t* bitmaps;  // bm_words words per hub
+    size_t bm_words;
     uint32_t* alist[2];
     const uint32_t* adj[3];   // adj[0] = ucol; adj[1], adj[2] writable copies
     uint32_t* adjw[3];
@@ -343,6 +355,9 @@ struct CoreBufs {
     uint32_t* cend;
     uint32_t* fvert;
     uint32_t* ent;            // 2 words per entry: u, chunk
+    uint32_t* toff;           // per vertex: offset of its table in the arena (frontier vertices of this round)
+    uint32_t* tbits;          // per vertex: log2 of its table size
+    uint32_t* tables;         // the round's table arena
 };
 
 __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32_t* __restrict__ cbeg, uint32_t* __restrict__ cend,
@@ -360,6 +375,7 @@ __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32
         f->na_new = 0;
         f->nfv = 0;
         f->nent = 0;
+        f->tcursor = 0;
         f->adj_cursor = 0;
     }
 }
@@ -378,7 +394,8 @@ __global__ void k_cf_min(CoreBufs B, const CoreFusedDev* __restrict__ f, const u
     if (lane_id() == 0 && mn != 0xFFFFFFFFu) atomicMin(&st->min_tri, mn);
 }
 
-// Frontier = alive v with tri <= k; appends v to fvert and its chunk entries (one atomic per wave for each).
+// Frontier = alive v with tri <= k; appends v to fvert, its chunk entries and (if it is in a triangle) a hash table
+// of 2^(ceil(log2 |cN(v)|) + 1) slots in the round's table arena (one atomic per wave for each).
 __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restrict__ tri, uint8_t* __restrict__ state,
                           uint32_t* __restrict__ core, CoreDev* st) {
     if (st->done) return;
@@ -391,36 +408,46 @@ __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restric
     for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
         const uint64_t i = base + lane;
         bool join = false;
-        uint32_t v = 0, nc = 0;
+        uint32_t v = 0, nc = 0, tb = 0, ts = 0;
         if (i < na) {
             v = al[i];
             if (state[v] == 1 && (unsigned long long)tri[v] <= k) {
                 join = true;
                 state[v] = 2;
                 core[v] = (uint32_t)k;
-                nc = (B.cend[v] - B.cbeg[v] + CORE_CHUNK - 1) / CORE_CHUNK;
+                // a vertex in no alive triangle has nothing to subtract: no entries, no table
+                const uint32_t d = B.cend[v] - B.cbeg[v];
+                if (tri[v] && d) {
+                    nc = (d + CORE_CHUNK - 1) / CORE_CHUNK;
+                    tb = (d > 1 ? 32u - (uint32_t)__clz(d - 1) : 0u) + 1u;
+                    if (tb < 2) tb = 2;
+                    ts = 1u << tb;
+                }
             }
         }
         const uint64_t mask = __ballot(join);
         if (!mask) continue;
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-        // inclusive scan of nc over the wave
-        uint32_t inc = nc;
+        uint32_t inc = nc, tinc = ts;   // inclusive scans over the wave
 #pragma unroll
         for (uint32_t o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += t;
+            const uint32_t t = __shfl_up(inc, o, 64), t2 = __shfl_up(tinc, o, 64);
+            if (lane >= o) { inc += t; tinc += t2; }
         }
-        const uint32_t tot = __shfl(inc, 63, 64);
-        uint32_t vpos = 0, epos = 0;
+        const uint32_t tot = __shfl(inc, 63, 64), ttot = __shfl(tinc, 63, 64);
+        uint32_t vpos = 0, epos = 0, tpos = 0;
         if (lane == leader) {
             vpos = atomicAdd(&f->nfv, (uint32_t)__popcll(mask));
             epos = tot ? atomicAdd(&f->nent, tot) : 0u;
+            tpos = ttot ? atomicAdd(&f->tcursor, ttot) : 0u;
         }
         vpos = __shfl(vpos, leader, 64);
         epos = __shfl(epos, leader, 64);
+        tpos = __shfl(tpos, leader, 64);
         if (join) {
             B.fvert[vpos + mask_rank(mask)] = v;
+            B.toff[v] = tpos + tinc - ts;
+            B.tbits[v] = tb;
             const uint32_t e0 = epos + inc - nc;
             for (uint32_t c = 0; c < nc; c++) {
                 B.ent[2 * (e0 + c)] = v;
@@ -431,21 +458,15 @@ __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
 }
 
-// First index in sorted a[lo, hi) with a[i] >= x, searching forward from lo with doubling steps.
-__device__ __forceinline__ uint32_t gallop_lb(const uint32_t* __restrict__ a, uint32_t lo, uint32_t hi, uint32_t x) {
-    if (lo >= hi || a[lo] >= x) return lo;
-    uint32_t prev = lo, cur = lo + 1, step = 1;   // a[prev] < x
-    while (cur < hi && a[cur] < x) {
-        prev = cur;
-        step <<= 1;
-        cur = (hi - cur > step) ? cur + step : hi;
-    }
-    return lower_bound_u32(a, prev + 1, cur, x);   // cur == hi or a[cur] >= x
+// Clears the round's table arena (its size is only known on the device).
+__global__ void k_cf_tclear(CoreBufs B, const CoreFusedDev* __restrict__ f, const CoreDev* __restrict__ st) {
+    if (st->done) return;
+    const uint32_t used = f->tcursor;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < used; i += gridDim.x * blockDim.x) B.tables[i] = GA_EMPTY;
 }
 
-// Triangles {u, v, w} (w > v) through entry (u, chunk): subtract once, by the smallest-id frontier member.
-__global__ void k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state, uint32_t* __restrict__ tri,
-                            const CoreDev* __restrict__ st) {
+// Each entry inserts its chunk of cN(u) (alive members only) into u's table.
+__global__ void k_cf_tinsert(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
     if (st->done) return;
     const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
     const uint32_t nent = f->nent;
@@ -453,32 +474,166 @@ __global__ void k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, cons
     const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
     for (uint64_t ei = wave0; ei < nent; ei += nwaves) {
         const uint32_t u = B.ent[2 * ei], chunk = B.ent[2 * ei + 1];
+        const uint32_t pb = B.cbeg[u] + chunk * CORE_CHUNK, pe = min(B.cend[u], pb + CORE_CHUNK);
+        const uint32_t tb = B.tbits[u], tsz = 1u << tb;
+        uint32_t* tab = B.tables + B.toff[u];
+        for (uint32_t p = pb + lane; p < pe; p += 64) {
+            const uint32_t x = adj[p];
+            if (state[x] == 0) continue;
+            uint32_t s = hash_slot_u32(x, tb);
+            for (uint32_t probe = 0; probe < tsz; probe++) {
+                if (atomicCAS(&tab[s], GA_EMPTY, x) == GA_EMPTY) break;
+                s = (s + 1) & (tsz - 1);
+            }
+        }
+    }
+}
+
+// Triangles {u, v, w} (w > v) through entry (u, chunk): subtract once, by the smallest-id frontier member.
+// One wave per entry, 64 neighbours v at a time. For each pair the SHORTER of cN(u) after v and cN(v) after v is
+// streamed and its elements are tested for membership on the other side:
+//   in N(u): the LDS hash set of cN(u) when |cN(u)| <= CORE_HCAP, else u's membership bitmap
+//   in N(v): v's membership bitmap when v has one (original degree > CORE_HCAP), else binary search in cN(v)
+// Streaming follows the count kernel: lists compacted in LDS longest first, read by sub-groups sized to their length,
+// 4 loads in flight per lane. A hit w is a triangle {u, v, w}; its alive corners lose one triangle each.
+#define CORE_HCAP 2048u
+#define CORE_TSLOTS (2u * CORE_HCAP)
+
+__device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t w) { return (bm[w >> 5] >> (w & 31)) & 1u; }
+
+__global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
+                                                  uint32_t* __restrict__ tri, const CoreDev* __restrict__ st) {
+    __shared__ uint32_t stg[7 * 64];   // ps | len | v | sv + (mode << 8) | lo | hi | hub
+    if (st->done) return;
+    const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
+    const uint32_t nent = f->nent;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t ei = blockIdx.x; ei < nent; ei += gridDim.x) {
+        const uint32_t u = B.ent[2 * ei], chunk = B.ent[2 * ei + 1];
         const uint32_t ub = B.cbeg[u], ue = B.cend[u];
         const uint32_t pb = ub + chunk * CORE_CHUNK;
         const uint32_t pe = min(ue, pb + CORE_CHUNK);
-        for (uint32_t p = pb + lane; p < pe; p += 64) {
-            const uint32_t v = adj[p];
-            const uint8_t sv = state[v];
-            if (sv == 0) continue;
-            const uint32_t vb = B.cbeg[v], ve = B.cend[v];
-            uint32_t a = p + 1;                                       // cN(u) after v: all > v
-            uint32_t b = upper_bound_u32(adj, vb, ve, v);             // cN(v) after v
-            const bool a_short = (ue - a) <= (ve - b);
-            uint32_t s = a_short ? a : b;
-            const uint32_t se = a_short ? ue : ve;
-            uint32_t l = a_short ? b : a;
-            const uint32_t le = a_short ? ve : ue;
-            for (; s < se && l < le; s++) {
-                const uint32_t w = adj[s];
-                l = gallop_lb(adj, l, le, w);
-                if (l >= le) break;
-                if (adj[l] != w) continue;
-                const uint8_t sw = state[w];
-                if (sw == 0) continue;
-                if ((sv == 2 && v < u) || (sw == 2 && w < u)) continue;   // a smaller frontier member owns it
-                if (sv == 1) atomicSub(&tri[v], 1u);
-                if (sw == 1) atomicSub(&tri[w], 1u);
+        const uint32_t tbits = B.tbits[u], tsz = 1u << tbits;
+        const uint32_t* __restrict__ tab = B.tables + B.toff[u];
+        for (uint32_t base = pb; base < pe; base += 64) {
+            const uint32_t p = base + lane;
+            uint32_t v = 0, sv = 0, ps = 0, len = 0, mode = 0, lo = 0, hi = 0, hub = GA_EMPTY;
+            if (p < pe) {
+                v = adj[p];
+                sv = state[v];
+                if (sv) {
+                    const uint32_t vb = B.cbeg[v], ve = B.cend[v];
+                    lo = upper_bound_u32(adj, vb, ve, v);        // cN(v) after v
+                    hi = ve;
+                    hub = B.hubid[v];
+                    const uint32_t a = ue - (p + 1), bl = hi - lo;
+                    if (bl <= a) { ps = lo; len = bl; mode = 0; }   // stream cN(v) after v, test in N(u)
+                    else { ps = p + 1; len = a; mode = 1; }         // stream cN(u) after v, test in N(v)
+                }
             }
+            const uint64_t gm = __ballot(len > 0);
+            const uint32_t cnt = (uint32_t)__popcll(gm);
+            if (len > 0) {
+                const uint32_t r = mask_rank(gm);
+                stg[r] = ps;
+                stg[64 + r] = len;
+                stg[128 + r] = v;
+                stg[192 + r] = sv | (mode << 8);
+                stg[256 + r] = lo;
+                stg[320 + r] = hi;
+                stg[384 + r] = hub;
+            }
+            group_sync_wave();
+            const uint32_t lk = lane < cnt ? stg[64 + lane] : 0u;
+            uint32_t start = 0;
+#pragma unroll
+            for (uint32_t S = 64; S >= 1; S >>= 1) {
+                constexpr uint32_t U = 4;
+                const uint32_t W = S >= U ? S / U : 1u;
+                const uint32_t end = (uint32_t)__popcll(__ballot(lk > S / 2));
+                for (uint32_t k0 = start; k0 < end; k0 += 64 / W) {
+                    const uint32_t k = k0 + lane / W, r = lane % W;
+                    uint32_t c = 0, kv = 0, ksv = 0;
+                    if (k < end) {
+                        const uint32_t kps = stg[k], klen = stg[64 + k];
+                        kv = stg[128 + k];
+                        const uint32_t sm = stg[192 + k];
+                        ksv = sm & 0xFF;
+                        const uint32_t kmode = sm >> 8;
+                        const uint32_t klo = stg[256 + k], khi = stg[320 + k], khub = stg[384 + k];
+                        const uint32_t* __restrict__ vbm = khub != GA_EMPTY ? B.bitmaps + (size_t)khub * B.bm_words : nullptr;
+                        for (uint32_t off = r; off < klen; off += U * W) {
+                            uint32_t x[U];
+#pragma unroll
+                            for (uint32_t q = 0; q < U; q++) x[q] = off + q * W < klen ? adj[kps + off + q * W] : 0u;
+#pragma unroll
+                            for (uint32_t q = 0; q < U; q++) {
+                                if (off + q * W >= klen) break;
+                                const uint32_t w = x[q];
+                                bool hit = false;
+                                if (kmode == 0) {            // w in N(u)?
+                                    uint32_t s = hash_slot_u32(w, tbits);
+                                    for (uint32_t probe = 0; probe < tsz; probe++) {
+                                        const uint32_t t = tab[s];
+                                        if (t == w) { hit = true; break; }
+                                        if (t == GA_EMPTY) break;
+                                        s = (s + 1) & (tsz - 1);
+                                    }
+                                } else {                      // w in N(v)?  (w > v by construction)
+                                    if (vbm) {
+                                        hit = bit_test(vbm, w);
+                                    } else {
+                                        const uint32_t qi = lower_bound_u32(adj, klo, khi, w);
+                                        hit = qi < khi && adj[qi] == w;
+                                    }
+                                }
+                                if (!hit) continue;
+                                const uint8_t sw = state[w];
+                                if (sw == 0) continue;
+                                if ((ksv == 2 && kv < u) || (sw == 2 && w < u)) continue;   // a smaller frontier member owns it
+                                if (sw == 1) atomicSub(&tri[w], 1u);
+                                c++;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (uint32_t o = W / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+                    if (r == 0 && k < end && c && ksv == 1) atomicSub(&tri[kv], c);
+                }
+                start = end;
+                if (S == 1) break;
+            }
+            group_sync_wave();
+        }
+    }
+}
+
+// Membership bitmaps of the vertices whose ORIGINAL degree exceeds CORE_HCAP: hubid[v] = index or GA_EMPTY.
+__global__ void k_cf_hub_ids(const uint32_t* __restrict__ urow, uint32_t n, uint32_t* __restrict__ hubid, uint32_t* __restrict__ nhub) {
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t base = wave0 * 64; base < n; base += nwaves * 64) {
+        const uint64_t v = base + lane_id();
+        const bool hub = v < n && urow[v + 1] - urow[v] > CORE_HCAP;
+        const uint64_t mask = __ballot(hub);
+        uint32_t pos = 0;
+        if (mask) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+            if (lane_id() == leader) pos = atomicAdd(nhub, (uint32_t)__popcll(mask));
+            pos = __shfl(pos, leader, 64);
+        }
+        if (v < n) hubid[v] = hub ? pos + mask_rank(mask) : GA_EMPTY;
+    }
+}
+
+__global__ void k_cf_hub_bits(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol, const uint32_t* __restrict__ hubid, uint32_t n,
+                              uint32_t* __restrict__ bitmaps, size_t words) {
+    for (uint32_t v = blockIdx.x; v < n; v += gridDim.x) {
+        const uint32_t h = hubid[v];
+        if (h == GA_EMPTY) continue;
+        uint32_t* bm = bitmaps + (size_t)h * words;
+        for (uint32_t p = urow[v] + threadIdx.x; p < urow[v + 1]; p += blockDim.x) {
+            const uint32_t w = ucol[p];
+            atomicOr(&bm[w >> 5], 1u << (w & 31));
         }
     }
 }
@@ -501,6 +656,7 @@ __global__ void k_cf_end(CoreFusedDev* f, CoreDev* st) {
     st->min_tri = 0xFFFFFFFFu;
     f->nfv = 0;
     f->nent = 0;
+    f->tcursor = 0;
     if (st->nalive == 0) { st->done = 1; return; }
     f->compact = 2ull * st->nalive <= f->na ? 1u : 0u;
     f->na_new = 0;
@@ -574,15 +730,55 @@ __global__ void k_cf_swap(CoreFusedDev* f, const CoreDev* __restrict__ st) {
 struct CoreFusedMem {
     ga_ctx* ctx = nullptr;
     void* block = nullptr;
+    void* hubs = nullptr;
+    void* block_bm = nullptr;
     CoreBufs B{};
     CoreFusedDev* f = nullptr;
-    ~CoreFusedMem() { ga_free(block); }
+    ~CoreFusedMem() {
+        ga_free(block);
+        ga_free(hubs);
+        ga_free(block_bm);
+    }
 };
+
+static int cf_hubs(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
+    const size_t n = g->n;
+    void* idb = nullptr;
+    GA_TRY(ga_malloc(ctx, &idb, (n + 1) * 4 + 64));
+    uint32_t* hubid = (uint32_t*)idb;
+    uint32_t* cnt = hubid + n + 1;
+    GA_TRY(ga_fill_u32(ctx, cnt, 1, 0));
+    k_cf_hub_ids<<<cgrid(ctx, n), 256, 0, ctx->stream>>>(g->urow, g->n, hubid, cnt);
+    int rc = ga_launch_check(ctx, "k_cf_hub_ids");
+    if (!rc && hipMemcpyAsync(ctx->hstat, cnt, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
+    if (rc) { ga_free(idb); return ga_fail(ctx, rc ? rc : GA_ERR_HIP, "hub detection failed"); }
+    const size_t nh = ctx->hstat[0] & 0xFFFFFFFFu;
+    const size_t words = (n + 31) / 32;
+    M.hubs = idb;
+    M.B.hubid = hubid;
+    M.B.bm_words = words;
+    M.B.bitmaps = nullptr;
+    if (nh) {
+        void* bm = nullptr;
+        if (ga_malloc(ctx, &bm, nh * words * 4)) return GA_ERR_OOM;
+        // keep both allocations owned by M: bitmaps live in their own block
+        M.block_bm = bm;
+        GA_TRY(ga_fill_u32(ctx, (uint32_t*)bm, nh * words, 0));
+        k_cf_hub_bits<<<std::min<uint32_t>(g->n, 65535), 256, 0, ctx->stream>>>(g->urow, g->ucol, hubid, g->n, (uint32_t*)bm, words);
+        GA_TRY(ga_launch_check(ctx, "k_cf_hub_bits"));
+        M.B.bitmaps = (const uint32_t*)bm;
+    }
+    return GA_OK;
+}
 
 static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     const size_t n = g->n, m2 = 2ull * g->m;
     const size_t nent = n + m2 / CORE_CHUNK + 16;
-    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 2 * n + n + 2 * nent + 64 + 16;
+    const size_t arena = 4 * std::max<size_t>(m2, 1) + 4 * n + 64;   // tables: at most 4 slots per listed neighbour (+4 per vertex)
+    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 2 * n + n + 2 * nent + 2 * n + arena + 64 + 16;
+    if (arena >= 0xFFFFFFFFull || nent >= 0x7FFFFFFFull)
+        return ga_fail(ctx, GA_ERR_ARG, "graph too large for the fused core decomposition (32-bit table offsets)");
     GA_TRY(ga_malloc(ctx, &M.block, words * 4));
     uint32_t* p = (uint32_t*)M.block;
     M.B.alist[0] = p; p += n;
@@ -595,6 +791,9 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     M.B.cend = p; p += n;
     M.B.fvert = p; p += n;
     M.B.ent = p; p += 2 * nent;
+    M.B.toff = p; p += n;
+    M.B.tbits = p; p += n;
+    M.B.tables = p; p += arena;
     p = (uint32_t*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
     M.f = (CoreFusedDev*)p;
     return GA_OK;
@@ -611,6 +810,7 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     GA_HIP(ctx, hipSetDevice(ctx->device));
     CoreFusedMem M;
     GA_TRY(cf_alloc(ctx, g, M));
+    GA_TRY(cf_hubs(ctx, g, M));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_core_reset_state(c));
     k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, T_full);
@@ -628,7 +828,9 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
         for (int r = 0; r < block_rounds; r++) {
             k_cf_min<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
             k_cf_find<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->core, c->st);
-            k_cf_update<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
+            k_cf_tclear<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->st);
+            k_cf_tinsert<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
+            k_cf_update<<<gu * 4, 64, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
             k_cf_retire<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
             k_cf_end<<<1, 1, 0, ctx->stream>>>(M.f, c->st);
             k_cf_compact_list<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);

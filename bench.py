@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--variant", default="hash")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single MI355X)", file=sys.stderr)
@@ -160,6 +162,13 @@ def main():
         ref_T = og.triangles_omp(0)
         if ref_T != T0:
             raise SystemExit(f"MISMATCH: GPU count {T0} != CPU count {ref_T}")
+    if args.copy_probe:
+        import torch
+        x = torch.ones(args.copy_probe << 18, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
     traffic = load_traffic(cfg["workload"])
     out = {
         "metric": "oriented edges/s (exact triangle count, one resident graph)",

@@ -47,11 +47,13 @@ class GAError(RuntimeError):
 
 class TcTuning(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_int32 * 5), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
-                ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32)]
+                ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32),
+                ("flat_chunk", ctypes.c_int32), ("flat_block", ctypes.c_int32)]
 
     def as_dict(self):
         return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "batches": self.batches,
-                "load_shift": self.load_shift, "hash_cap": self.hash_cap}
+                "load_shift": self.load_shift, "hash_cap": self.hash_cap, "flat_chunk": self.flat_chunk,
+                "flat_block": self.flat_block}
 
 
 _lib = None

@@ -330,7 +330,7 @@ extern "C" int ga_core_download(ga_core* c, uint32_t* core) {
 // Compaction runs on the device (no host size readback) whenever at most half of alist is still alive: alist is
 // filtered, and each surviving vertex copies its surviving neighbours into the other adjacency buffer, reserving
 // space with one atomic per vertex. Every kernel returns at once once `done` is set.
-#define CORE_CHUNK 32u
+#define CORE_CHUNK 8u
 
 struct CoreFusedDev {
     uint32_t na;          // alist length

@@ -312,6 +312,10 @@ static void free_plan(TcPlan& p) {
     ga_free(p.wedge);
     ga_free(p.wps);
     ga_free(p.wlen);
+    ga_free(p.wh);
+    ga_free(p.wu);
+    ga_free(p.cstart);
+    ga_free(p.frec);
     ga_free(p.perm);
     p = TcPlan();
 }

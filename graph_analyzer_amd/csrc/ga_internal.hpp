@@ -37,6 +37,13 @@ struct TcPlan {
     uint32_t* wps = nullptr;             // rowptr[p] of each work entry (start of the probe list)
     uint32_t* wlen = nullptr;            // d+(p) of each work entry (length of the probe list)
     uint32_t nwork = 0;                  // work entries (edges with min(d+) > 0)
+    uint32_t* wh = nullptr;              // hashed vertex h of each work entry
+    uint32_t nlight = 0;                 // entries [0, nlight) have d+(h) <= 128 (the flat kernel's class)
+    unsigned long long* wu = nullptr;    // flat unit offset of each light entry (nlight + 1, exclusive prefix)
+    uint32_t* cstart = nullptr;          // light entry holding the first unit of each flat chunk (nch + 1)
+    uint32_t* frec = nullptr;            // 4 words per light entry: the flat kernel's entry record
+    uint32_t nch = 0, chunk_used = 0;    // flat chunks and the chunk size they were cut with
+    unsigned long long utot = 0;         // flat units of all light entries
     ga_tc_tuning tune = {};             // block == 0: defaults not applied yet
     uint32_t* perm = nullptr;            // GA_TC_SORTED work order (ga_graph_edge_order), lazily
 };

@@ -306,6 +306,159 @@ __global__ void k_tc_hash(const TcArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// flat kernel for light rows (d+(h) <= FLAT_MAX), plain counts
+// ---------------------------------------------------------------------------------------------------------------
+// The light work entries (plan order: rows of h, h ascending) are laid out on one line of "units":
+//   row of h:   [2^flat_bits(d+(h)) table units]  then for each of its entries j: [1 header unit][wlen[j] element units]
+// The line is cut into chunks of UPT x blockDim units; one workgroup takes a chunk, so every workgroup gets the same
+// work however the probe lists are distributed, and thread t handles units t, t + blockDim, ... (consecutive threads
+// on consecutive units, so element loads of one list are coalesced):
+//   1. mark where each overlapping entry starts and prefix-sum the marks in LDS: unit -> entry
+//   2. every thread reads its units' entry records (16 bytes, shared by a list's units, so mostly cache hits) and
+//      issues all its loads at once: table units load N+(h), element units load their probe element
+//   3. table units insert N+(h) into an LDS table that sits exactly on those units (the row whose table began in an
+//      earlier chunk is rebuilt in a spare region after the chunk)
+//   4. element units probe their row's table.
+// Entry record (frec): {first unit, first entry of the row ? rowptr[h] : units back to the row's table, rowptr[p],
+// bits | d+(h) << 8 | first << 16}. LDS: chunk + FLAT_TMAX table slots and chunk u16 owner indices.
+#define FLAT_MAX 128u
+#define FLAT_SHIFT 2u
+#define FLAT_TMAX (FLAT_MAX << FLAT_SHIFT)
+#define FLAT_UNIT_LIMIT 0xFFF00000ull   // unit offsets are 32-bit
+
+__host__ __device__ __forceinline__ uint32_t flat_bits(uint32_t dh) {
+    uint32_t lg = 0;
+    while ((1u << lg) < dh) lg++;
+    const uint32_t b = lg + FLAT_SHIFT;
+    return b < 2u ? 2u : b;
+}
+
+struct FlatArgs {
+    const uint32_t* rowptr;
+    const uint32_t* col;
+    const uint32_t* wh;
+    const uint4* frec;
+    const uint32_t* cstart;
+    uint32_t nch, nlight, utot;
+    unsigned long long* total;
+    const uint32_t* done;
+};
+
+static size_t flat_shmem(uint32_t chunk) { return (size_t)(chunk + FLAT_TMAX) * 4 + (size_t)chunk * 2; }
+
+__device__ __forceinline__ void flat_insert(uint32_t* tab, uint32_t tbits, uint32_t x) {
+    const uint32_t tsz = 1u << tbits;
+    uint32_t s = hash_slot(x, tbits);
+    for (uint32_t probe = 0; probe < tsz; probe++) {
+        if (atomicCAS(&tab[s], GA_EMPTY, x) == GA_EMPTY) return;
+        s = (s + 1) & (tsz - 1);
+    }
+}
+
+template <int UPT>
+__global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t wtot[16];
+    if (a.done && *a.done) return;
+    const uint32_t T = blockDim.x, CH = UPT * T, tid = threadIdx.x;
+    uint32_t* tab = lds;                                // CH + FLAT_TMAX slots ([CH, CH + FLAT_TMAX) = carried row)
+    uint16_t* own = (uint16_t*)(tab + CH + FLAT_TMAX);  // unit -> entry
+    const uint32_t lane = lane_id(), wv = tid / 64;
+    uint64_t local = 0;
+    for (uint32_t c = blockIdx.x; c < a.nch; c += gridDim.x) {
+        const uint32_t u0 = c * CH;
+        const uint32_t clen = min(CH, a.utot - u0);
+        const uint32_t j0 = a.cstart[c];
+        const uint32_t ne = min(a.cstart[c + 1] + 1, a.nlight) - j0;
+        // the row of entry j0: carried when its table begins before this chunk
+        const uint4 r0 = a.frec[j0];
+        const bool first0 = (r0.w >> 16) & 1u;
+        const uint32_t tu0 = first0 ? r0.x : r0.x - r0.y;
+        const bool carried = tu0 < u0;
+        const uint32_t hs0 = carried ? (first0 ? r0.y : a.rowptr[a.wh[j0]]) : 0u;
+        for (uint32_t u = tid; u < CH + FLAT_TMAX; u += T) tab[u] = GA_EMPTY;
+        for (uint32_t u = tid; u < CH; u += T) own[u] = 0;
+        __syncthreads();
+        for (uint32_t i = tid + 1; i < ne; i += T) {
+            const uint32_t U = a.frec[j0 + i].x - u0;   // > 0: entry j0 holds unit u0
+            if (U < clen) own[U] = 1;
+        }
+        __syncthreads();
+        {   // inclusive prefix sum of the marks: UPT contiguous units per thread
+            uint32_t run = 0;
+#pragma unroll
+            for (int k = 0; k < UPT; k++) run += own[tid * UPT + k];
+            uint32_t inc = run;
+#pragma unroll
+            for (uint32_t o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            if (lane == 63) wtot[wv] = inc;
+            __syncthreads();
+            uint32_t acc = inc - run;
+            for (uint32_t w = 0; w < wv; w++) acc += wtot[w];
+#pragma unroll
+            for (int k = 0; k < UPT; k++) {
+                acc += own[tid * UPT + k];
+                own[tid * UPT + k] = (uint16_t)acc;
+            }
+        }
+        __syncthreads();
+        // classify this thread's units and issue their loads
+        uint32_t x[UPT], meta[UPT];   // meta: kind (bit 31 insert, bit 30 probe) | bits << 16 | table base
+#pragma unroll
+        for (int k = 0; k < UPT; k++) {
+            const uint32_t u = k * T + tid;
+            meta[k] = 0;
+            x[k] = 0;
+            if (u >= clen) continue;
+            const uint4 r = a.frec[j0 + own[u]];
+            const uint32_t tb = r.w & 31u, dh = (r.w >> 8) & 0xFFu;
+            const bool first = (r.w >> 16) & 1u;
+            const uint32_t U = r.x - u0;   // unsigned: wraps for the entry that began before the chunk
+            const uint32_t q = u - U;
+            const uint32_t t = first ? (1u << tb) : 0u;
+            if (q < t) {   // table unit of a row whose first entry is in this chunk
+                if (q < dh && t <= clen && U <= clen - t) {   // the whole table lies in this chunk
+                    x[k] = a.col[r.y + q];
+                    meta[k] = 0x80000000u | (tb << 16) | U;
+                }
+            } else if (q > t) {   // element unit
+                x[k] = a.col[r.z + (q - t - 1)];
+                const uint32_t tu = first ? r.x : r.x - r.y;
+                const uint32_t base = tu >= u0 ? tu - u0 : CH;
+                meta[k] = 0x40000000u | (tb << 16) | base;
+            }
+        }
+        if (carried) {
+            const uint32_t tb = r0.w & 31u, dh = (r0.w >> 8) & 0xFFu;
+            for (uint32_t k = tid; k < dh; k += T) flat_insert(tab + CH, tb, a.col[hs0 + k]);
+        }
+#pragma unroll
+        for (int k = 0; k < UPT; k++)
+            if (meta[k] & 0x80000000u) flat_insert(tab + (meta[k] & 0xFFFFu), (meta[k] >> 16) & 31u, x[k]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < UPT; k++) {
+            if (!(meta[k] & 0x40000000u)) continue;
+            const uint32_t tb = (meta[k] >> 16) & 31u, tsz = 1u << tb;
+            const uint32_t* tt = tab + (meta[k] & 0xFFFFu);
+            uint32_t s = hash_slot(x[k], tb);
+            for (uint32_t probe = 0; probe < tsz; probe++) {
+                const uint32_t v = tt[s];
+                if (v == x[k]) { local++; break; }
+                if (v == GA_EMPTY) break;
+                s = (s + 1) & (tsz - 1);
+            }
+        }
+        __syncthreads();
+    }
+    local = wave_sum_u64(local);
+    if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // count plan
 // ---------------------------------------------------------------------------------------------------------------
 namespace {
@@ -331,19 +484,57 @@ struct PlanShape {   // entries per item before a row is cut, per bucket (lanes 
     uint32_t pmax[NHASH + 1];
 };
 
-// key = (h << 32) | ~d+(p): rows of h, longest probe list first; edges without work get the all-ones key (last).
+// key = (light << (32 + nb)) | (h << 32) | ~d+(p): light rows (d+(h) <= FLAT_MAX) first, then rows of h, longest
+// probe list first; edges without work get the all-ones key (last). light = 0 for every row when split is false.
 __global__ void k_plan_keys(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
-                            uint32_t m, unsigned long long* __restrict__ key, uint32_t* __restrict__ val, uint32_t* __restrict__ nwork) {
-    uint32_t cnt = 0;
+                            uint32_t m, int nb, bool split, unsigned long long* __restrict__ key, uint32_t* __restrict__ val,
+                            uint32_t* __restrict__ nwork) {
+    uint32_t cnt = 0, light = 0;
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
         uint32_t h, p, dmin;
         hash_side(rowptr, src[e], col[e], h, p, dmin);
-        key[e] = dmin ? (((unsigned long long)h << 32) | (uint32_t)~dmin) : ~0ull;
+        const bool heavy = split && rowptr[h + 1] - rowptr[h] > FLAT_MAX;
+        key[e] = dmin ? (((unsigned long long)heavy << (32 + nb)) | ((unsigned long long)h << 32) | (uint32_t)~dmin) : ~0ull;
         val[e] = e;
         cnt += dmin != 0;
+        light += dmin != 0 && !heavy;
     }
     cnt = (uint32_t)wave_sum_u64(cnt);
+    light = (uint32_t)wave_sum_u64(light);
     if (lane_id() == 0 && cnt) atomicAdd(nwork, cnt);
+    if (lane_id() == 0 && light) atomicAdd(nwork + 1, light);
+}
+
+// Flat units of each light entry: its row's table (first entry of the row only), one header unit, its probe list.
+__global__ void k_plan_units(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ wlen,
+                             uint32_t nlight, unsigned long long* __restrict__ wt) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
+        const uint32_t h = wh[j];
+        const bool first = j == 0 || wh[j - 1] != h;
+        wt[j] = (first ? (1ull << flat_bits(rowptr[h + 1] - rowptr[h])) : 0ull) + 1ull + wlen[j];
+    }
+}
+
+// Entry records of the flat kernel (see k_tc_flat).
+__global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ wps,
+                            const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ hstart, uint32_t nlight,
+                            uint4* __restrict__ frec) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
+        const uint32_t h = wh[j];
+        const uint32_t jf = hstart[h];
+        const bool first = jf == j;
+        const uint32_t dh = rowptr[h + 1] - rowptr[h];
+        frec[j] = make_uint4((uint32_t)wu[j], first ? rowptr[h] : (uint32_t)(wu[j] - wu[jf]), wps[j],
+                             flat_bits(dh) | (dh << 8) | ((first ? 1u : 0u) << 16));
+    }
+}
+
+// cstart[c] = the light entry whose units contain unit c * chunk.
+__global__ void k_plan_chunks(const unsigned long long* __restrict__ wu, uint32_t nlight, uint32_t chunk, uint32_t* __restrict__ cstart) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
+        const unsigned long long a = wu[j], b = wu[j + 1];
+        for (unsigned long long c = (a + chunk - 1) / chunk; c * chunk < b; c++) cstart[c] = j;
+    }
 }
 
 __global__ void k_plan_fill(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
@@ -386,53 +577,71 @@ __global__ void k_plan_flags(const uint32_t* __restrict__ hstart, const uint32_t
     }
 }
 
-// Item starts -> per-bucket counts (one atomic per bucket per wave).
+// Item starts -> per-(bucket, block) counts, bcount[bucket * gridDim.x + block]; block b covers entries
+// [b * span, (b + 1) * span) (LDS counters: no global atomics).
 __global__ void k_plan_item_count(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ rowptr,
-                                  uint32_t nwork, uint32_t cap, uint32_t* __restrict__ bcount) {
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
-    for (uint64_t base = wave0 * 64; base < nwork; base += nwaves * 64) {
-        const uint64_t j = base + lane_id();
+                                  uint32_t nwork, uint32_t span, uint32_t cap, uint32_t* __restrict__ bcount) {
+    __shared__ uint32_t cnt[NHASH + 1];
+    if (threadIdx.x <= NHASH) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t j0 = (uint64_t)blockIdx.x * span, j1 = min((uint64_t)nwork, j0 + span);
+    for (uint64_t base = j0; base < j1; base += blockDim.x) {   // block-uniform loop
+        const uint64_t j = base + threadIdx.x;
         uint32_t bk = 0xFFu;
-        if (j < nwork && flag[j]) {
+        if (j < j1 && flag[j]) {
             const uint32_t h = wh[j];
             bk = bucket_of(rowptr[h + 1] - rowptr[h], cap);
         }
 #pragma unroll
         for (uint32_t b = 0; b <= NHASH; b++) {
             const uint64_t mk = __ballot(bk == b);
-            if (mk && lane_id() == (uint32_t)__ffsll((unsigned long long)mk) - 1) atomicAdd(&bcount[b], (uint32_t)__popcll(mk));
+            if (mk && lane_id() == (uint32_t)__ffsll((unsigned long long)mk) - 1) atomicAdd(&cnt[b], (uint32_t)__popcll(mk));
         }
     }
+    __syncthreads();
+    if (threadIdx.x <= NHASH) bcount[threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Items placed bucket-major: (h, begin, end). end = next item start (rows are contiguous and every row starts an item).
+// Items placed bucket-major: (h, begin, end), end = next item start (rows are contiguous and every row starts an
+// item). Each block writes its range's items in order from its offset boff[bucket * gridDim.x + block].
 __global__ void k_plan_items(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ fpos, const uint32_t* __restrict__ starts_by_rank,
                              const uint32_t* __restrict__ wh, const uint32_t* __restrict__ rowptr, uint32_t nwork, uint32_t nall,
-                             uint32_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ bcur, uint32_t* __restrict__ items) {
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
-    for (uint64_t base = wave0 * 64; base < nwork; base += nwaves * 64) {
-        const uint64_t j = base + lane_id();
-        uint32_t bk = 0xFFu, h = 0;
-        if (j < nwork && flag[j]) {
+                             uint32_t span, uint32_t cap, const uint32_t* __restrict__ boff, uint32_t* __restrict__ items) {
+    __shared__ uint32_t run[NHASH + 1];
+    __shared__ uint32_t wc[16][NHASH + 1];
+    const uint32_t wv = threadIdx.x / 64, nw = blockDim.x / 64;
+    if (threadIdx.x <= NHASH) run[threadIdx.x] = boff[threadIdx.x * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint64_t j0 = (uint64_t)blockIdx.x * span, j1 = min((uint64_t)nwork, j0 + span);
+    for (uint64_t base = j0; base < j1; base += blockDim.x) {   // block-uniform loop
+        const uint64_t j = base + threadIdx.x;
+        uint32_t bk = 0xFFu, h = 0, rank = 0;
+        if (j < j1 && flag[j]) {
             h = wh[j];
             bk = bucket_of(rowptr[h + 1] - rowptr[h], cap);
         }
 #pragma unroll
         for (uint32_t b = 0; b <= NHASH; b++) {
             const uint64_t mk = __ballot(bk == b);
-            if (!mk) continue;
-            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mk) - 1;
-            uint32_t pos0 = 0;
-            if (lane_id() == leader) pos0 = atomicAdd(&bcur[b], (uint32_t)__popcll(mk));
-            pos0 = __shfl(pos0, leader, 64);
-            if (bk == b) {
-                const uint32_t r = fpos[j];
-                const uint32_t pos = boff[b] + pos0 + mask_rank(mk);
-                items[3 * pos] = h;
-                items[3 * pos + 1] = (uint32_t)j;
-                items[3 * pos + 2] = r + 1 < nall ? starts_by_rank[r + 1] : nwork;
-            }
+            if (bk == b) rank = mask_rank(mk);
+            if (lane_id() == 0) wc[wv][b] = (uint32_t)__popcll(mk);
         }
+        __syncthreads();
+        if (bk <= NHASH) {
+            uint32_t pos = run[bk] + rank;
+            for (uint32_t w = 0; w < wv; w++) pos += wc[w][bk];
+            const uint32_t r = fpos[j];
+            items[3ull * pos] = h;
+            items[3ull * pos + 1] = (uint32_t)j;
+            items[3ull * pos + 2] = r + 1 < nall ? starts_by_rank[r + 1] : nwork;
+        }
+        __syncthreads();
+        if (threadIdx.x <= NHASH) {
+            uint32_t add = 0;
+            for (uint32_t w = 0; w < nw; w++) add += wc[w][threadIdx.x];
+            run[threadIdx.x] += add;
+        }
+        __syncthreads();
     }
 }
 
@@ -458,6 +667,8 @@ ga_tc_tuning default_tuning() {
     t.batches = 8;
     t.load_shift = 2;
     t.hash_cap = 4096;
+    t.flat_chunk = 0;     // the flat kernel is a tuner option (it wins on some degree mixes, not on all)
+    t.flat_block = 512;
     return t;
 }
 
@@ -465,7 +676,11 @@ bool tuning_valid(const ga_tc_tuning& t) {
     for (int i = 0; i < NHASH; i++)
         if (t.lanes[i] != 4 && t.lanes[i] != 8 && t.lanes[i] != 16 && t.lanes[i] != 32 && t.lanes[i] != 64) return false;
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
-           t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096;
+           t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
+           t.flat_block <= 1024 && t.flat_block % 64 == 0 &&
+           (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
+                                  (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
+                                   t.flat_chunk / t.flat_block == 16)));
 }
 
 PlanShape plan_shape(const ga_tc_tuning& t) {
@@ -477,6 +692,25 @@ PlanShape plan_shape(const ga_tc_tuning& t) {
 
 }  // namespace
 
+// Cuts the light entries' units into chunks of tune.flat_chunk (nothing when the flat kernel is off).
+static int flat_rechunk(ga_ctx* ctx, ga_graph* g) {
+    TcPlan& P = g->plan;
+    ga_free(P.cstart);
+    P.cstart = nullptr;
+    P.nch = 0;
+    P.chunk_used = (uint32_t)P.tune.flat_chunk;
+    if (!P.nlight || !P.tune.flat_chunk || !P.frec) return GA_OK;
+    const uint32_t chunk = (uint32_t)P.tune.flat_chunk;
+    const unsigned long long nch = (P.utot + chunk - 1) / chunk;
+    if (nch >= 0xFFFFFFFFull) return ga_fail(ctx, GA_ERR_ARG, "too many flat chunks");
+    GA_TRY(ga_malloc(ctx, (void**)&P.cstart, (nch + 1) * 4));
+    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.nlight, chunk, P.cstart);
+    GA_TRY(ga_launch_check(ctx, "k_plan_chunks"));
+    GA_TRY(ga_fill_u32(ctx, P.cstart + nch, 1, P.nlight));
+    P.nch = (uint32_t)nch;
+    return GA_OK;
+}
+
 static int build_plan(ga_ctx* ctx, ga_graph* g) {
     TcPlan& P = g->plan;
     if (!P.tune.block) P.tune = default_tuning();
@@ -486,51 +720,60 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     ga_free(P.wps);
     ga_free(P.wlen);
     ga_free(P.items);
-    P.wcol = P.wedge = P.wps = P.wlen = P.items = nullptr;
+    ga_free(P.wh);
+    ga_free(P.wu);
+    ga_free(P.cstart);
+    ga_free(P.frec);
+    P.wcol = P.wedge = P.wps = P.wlen = P.items = P.wh = P.cstart = P.frec = nullptr;
+    P.wu = nullptr;
     for (int b = 0; b < NHASH; b++) P.nitems[b] = 0;
     P.nhuge = 0;
-    P.nwork = 0;
-    DevBuf keys, keys2, vals, vals2, hstart_b, wh_b, wt_b, wsum_b, flag_b, fpos_b, starts_b, bc_b;
+    P.nwork = P.nlight = P.nch = P.chunk_used = 0;
+    P.utot = 0;
+    DevBuf keys, keys2, vals, vals2, hstart_b, wt_b, wsum_b, flag_b, fpos_b, starts_b, bc_b, bcount_b;
     GA_TRY(ga_malloc(ctx, &bc_b.p, 128));
-    uint32_t* bc = (uint32_t*)bc_b.p;   // [0..6) bucket counts, [8..14) cursors, [16..24) offsets, [31] nwork
+    uint32_t* bc = (uint32_t*)bc_b.p;   // [30] work entries, [31] light work entries
     GA_TRY(ga_fill_u32(ctx, bc, 32, 0));
-    uint32_t nwork = 0;
+    uint32_t nwork = 0, nlight = 0;
     uint32_t* sorted_vals = nullptr;
     if (m) {
         GA_TRY(ga_malloc(ctx, &keys.p, (size_t)m * 8));
         GA_TRY(ga_malloc(ctx, &keys2.p, (size_t)m * 8));
         GA_TRY(ga_malloc(ctx, &vals.p, (size_t)m * 4));
         GA_TRY(ga_malloc(ctx, &vals2.p, (size_t)m * 4));
-        k_plan_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, m, (unsigned long long*)keys.p,
-                                                                     (uint32_t*)vals.p, bc + 31);
-        GA_TRY(ga_launch_check(ctx, "k_plan_keys"));
         int nb = 1;
         while (nb < 32 && (1ull << nb) < (uint64_t)n) nb++;
+        const bool split = 33 + nb <= 64;   // room for the light/heavy bit above the vertex id
+        k_plan_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, m, nb, split,
+                                                                     (unsigned long long*)keys.p, (uint32_t*)vals.p, bc + 30);
+        GA_TRY(ga_launch_check(ctx, "k_plan_keys"));
         unsigned long long* ko = nullptr;
         GA_TRY(ga_sort_pairs_u64(ctx, (unsigned long long*)keys.p, (unsigned long long*)keys2.p, (uint32_t*)vals.p, (uint32_t*)vals2.p, m,
-                                 32 + nb, &ko, &sorted_vals));
-        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, bc + 31, 4, hipMemcpyDeviceToHost, ctx->stream));
+                                 32 + nb + (split ? 1 : 0), &ko, &sorted_vals));
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, bc + 30, 8, hipMemcpyDeviceToHost, ctx->stream));
         GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
         nwork = (uint32_t)(ctx->hstat[0] & 0xFFFFFFFFu);
+        nlight = split ? (uint32_t)(ctx->hstat[0] >> 32) : 0u;
     }
     GA_TRY(ga_malloc(ctx, (void**)&P.wcol, std::max<size_t>(nwork, 1) * 4));
     GA_TRY(ga_malloc(ctx, (void**)&P.wedge, std::max<size_t>(nwork, 1) * 4));
     GA_TRY(ga_malloc(ctx, (void**)&P.wps, std::max<size_t>(nwork, 1) * 4));
     GA_TRY(ga_malloc(ctx, (void**)&P.wlen, std::max<size_t>(nwork, 1) * 4));
+    GA_TRY(ga_malloc(ctx, (void**)&P.wh, std::max<size_t>(nwork, 1) * 4));
     P.nwork = nwork;
+    P.nlight = nlight;
     if (nwork == 0) {
         GA_TRY(ga_malloc(ctx, (void**)&P.items, 12));
         P.ready = true;
         return GA_OK;
     }
     GA_TRY(ga_malloc(ctx, &hstart_b.p, ((size_t)n + 1) * 4));
-    GA_TRY(ga_malloc(ctx, &wh_b.p, (size_t)nwork * 4));
     GA_TRY(ga_malloc(ctx, &wt_b.p, ((size_t)nwork + 1) * 8));
     GA_TRY(ga_malloc(ctx, &wsum_b.p, ((size_t)nwork + 1) * 8));
     GA_TRY(ga_malloc(ctx, &flag_b.p, ((size_t)nwork + 1) * 4));
     GA_TRY(ga_malloc(ctx, &fpos_b.p, ((size_t)nwork + 1) * 4));
     uint32_t* hstart = (uint32_t*)hstart_b.p;
-    uint32_t* wh = (uint32_t*)wh_b.p;
+    uint32_t* wh = P.wh;
     unsigned long long* wt = (unsigned long long*)wt_b.p;
     unsigned long long* wsum = (unsigned long long*)wsum_b.p;
     uint32_t* flag = (uint32_t*)flag_b.p;
@@ -548,35 +791,55 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     GA_TRY(ga_launch_check(ctx, "k_plan_flags"));
     GA_TRY(ga_fill_u32(ctx, flag + nwork, 1, 0));
     GA_TRY(ga_exclusive_scan_u32(ctx, flag, fpos, (size_t)nwork + 1));
-    k_plan_item_count<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, wh, g->rowptr, nwork, cap, bc);
+    const uint32_t nblk = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(1, ((uint64_t)nwork + 8191) / 8192));
+    const uint32_t span = (uint32_t)((((uint64_t)nwork + nblk - 1) / nblk + 255) / 256 * 256);
+    const size_t nbc = (size_t)(NHASH + 1) * nblk;
+    GA_TRY(ga_malloc(ctx, &bcount_b.p, (nbc + 1) * 8));
+    uint32_t* bcount = (uint32_t*)bcount_b.p;
+    uint32_t* boff = bcount + nbc + 1;
+    k_plan_item_count<<<nblk, 256, 0, ctx->stream>>>(flag, wh, g->rowptr, nwork, span, cap, bcount);
     GA_TRY(ga_launch_check(ctx, "k_plan_item_count"));
-    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, bc, 32, hipMemcpyDeviceToHost, ctx->stream));
-    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat + 4, fpos + nwork, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_TRY(ga_fill_u32(ctx, bcount + nbc, 1, 0));
+    GA_TRY(ga_exclusive_scan_u32(ctx, bcount, boff, nbc + 1));
+    for (int b = 0; b <= NHASH + 1; b++)
+        GA_HIP(ctx, hipMemcpyAsync((uint32_t*)ctx->hstat + b, boff + (size_t)b * nblk, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync((uint32_t*)ctx->hstat + 8, fpos + nwork, 4, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     uint32_t cnt[NHASH + 1];
     const uint32_t* hc = (const uint32_t*)ctx->hstat;
-    for (int b = 0; b <= NHASH; b++) cnt[b] = hc[b];
-    const uint32_t nall = (uint32_t)(ctx->hstat[4] & 0xFFFFFFFFu);
-    uint32_t off[8] = {0};
-    for (int b = 0; b <= NHASH; b++) off[b + 1] = off[b] + cnt[b];
-    if (off[NHASH + 1] != nall) return ga_fail(ctx, GA_ERR_STATE, "count plan: %u items bucketed, %u expected", off[NHASH + 1], nall);
+    for (int b = 0; b <= NHASH; b++) cnt[b] = hc[b + 1] - hc[b];
+    const uint32_t nall = hc[8];
+    if (hc[NHASH + 1] != nall) return ga_fail(ctx, GA_ERR_STATE, "count plan: %u items bucketed, %u expected", hc[NHASH + 1], nall);
     GA_TRY(ga_malloc(ctx, &starts_b.p, (size_t)nall * 4));
     GA_TRY(ga_malloc(ctx, (void**)&P.items, (size_t)nall * 12));
     k_plan_rank<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, fpos, nwork, (uint32_t*)starts_b.p);
     GA_TRY(ga_launch_check(ctx, "k_plan_rank"));
-    GA_HIP(ctx, hipMemcpyAsync(bc + 16, off, 8 * 4, hipMemcpyHostToDevice, ctx->stream));
-    k_plan_items<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, cap,
-                                                                      bc + 16, bc + 8, P.items);
+    k_plan_items<<<nblk, 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, span, cap, boff, P.items);
     GA_TRY(ga_launch_check(ctx, "k_plan_items"));
+    if (nlight) {   // flat units of the light entries (wt is free again: the item flags are done)
+        k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, wh, P.wlen, nlight, wt);
+        GA_TRY(ga_launch_check(ctx, "k_plan_units"));
+        GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nlight, 1, 0));
+        GA_TRY(ga_malloc(ctx, (void**)&P.wu, ((size_t)nlight + 1) * 8));
+        GA_TRY(ga_exclusive_scan_u64(ctx, (const uint64_t*)wt, (uint64_t*)P.wu, (size_t)nlight + 1));
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, P.wu + nlight, 8, hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        P.utot = ctx->hstat[0];
+        if (P.utot < FLAT_UNIT_LIMIT) {
+            GA_TRY(ga_malloc(ctx, (void**)&P.frec, (size_t)nlight * 16));
+            k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, wh, P.wps, P.wu, hstart, nlight, (uint4*)P.frec);
+            GA_TRY(ga_launch_check(ctx, "k_plan_frec"));
+        }
+    }
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int b = 0; b < NHASH; b++) P.nitems[b] = cnt[b];
     P.nhuge = cnt[NHASH];
     P.ready = true;
-    return GA_OK;
+    return flat_rechunk(ctx, g);
 }
 
 int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g) {
-    if (g->plan.ready) return GA_OK;
+    if (g->plan.ready) return g->plan.chunk_used == (uint32_t)g->plan.tune.flat_chunk ? GA_OK : flat_rechunk(ctx, g);
     return build_plan(ctx, g);
 }
 
@@ -610,13 +873,45 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     return ga_launch_check(ctx, "k_tc_hash");
 }
 
+// Flat kernel on the light rows; their lane-group buckets (d+(h) <= 128: buckets 0-2) are then skipped.
+static bool flat_on(const ga_graph* g) {
+    const TcPlan& P = g->plan;
+    return P.nch && P.chunk_used && P.tune.hash_cap >= (int)FLAT_MAX;
+}
+
+static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, const uint32_t* done) {
+    const TcPlan& P = g->plan;
+    FlatArgs f{};
+    f.rowptr = g->rowptr;
+    f.col = g->col;
+    f.wh = P.wh;
+    f.frec = (const uint4*)P.frec;
+    f.cstart = P.cstart;
+    f.nch = P.nch;
+    f.nlight = P.nlight;
+    f.utot = (uint32_t)P.utot;
+    f.total = total;
+    f.done = done;
+    const size_t shmem = flat_shmem(P.chunk_used);
+    if (shmem > ctx->lds_max) return ga_fail(ctx, GA_ERR_ARG, "flat chunk %u needs %zu bytes of LDS", P.chunk_used, shmem);
+    const uint32_t block = (uint32_t)P.tune.flat_block;
+    const uint32_t grid = std::min<uint32_t>(P.nch, (uint32_t)std::max(ctx->cus, 1) * 16u);
+    switch (P.chunk_used / block) {
+        case 4: k_tc_flat<4><<<grid, block, shmem, ctx->stream>>>(f); break;
+        case 8: k_tc_flat<8><<<grid, block, shmem, ctx->stream>>>(f); break;
+        case 16: k_tc_flat<16><<<grid, block, shmem, ctx->stream>>>(f); break;
+        default: return ga_fail(ctx, GA_ERR_ARG, "flat chunk / block must be 4, 8 or 16");
+    }
+    return ga_launch_check(ctx, "k_tc_flat");
+}
+
 template <int FLAGS>
-static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a) {
+static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a, int first_bucket = 0) {
     const TcPlan& P = g->plan;
     uint32_t off = 0;
     for (int b = 0; b < NHASH; b++) {
         const uint32_t* items = P.items + 3ull * off;
-        const uint32_t ni = P.nitems[b];
+        const uint32_t ni = b < first_bucket ? 0u : P.nitems[b];
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
         int rc = GA_OK;
         switch (P.tune.lanes[b]) {
@@ -627,7 +922,7 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a) {
             default: rc = launch_hash_bucket<64, true, FLAGS>(ctx, g, items, ni, bound, a); break;
         }
         GA_TRY(rc);
-        off += ni;
+        off += P.nitems[b];
     }
     return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 3ull * off, P.nhuge, 0, a);
 }
@@ -669,6 +964,10 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         a.done = done;
         a.gof = gof;
         a.gcnt = gcnt;
+        if (flags == 0 && flat_on(g)) {
+            GA_TRY(launch_flat(ctx, g, total, done));
+            return launch_hash<0>(ctx, g, a, 3);
+        }
         switch (flags) {
             case 0: return launch_hash<0>(ctx, g, a);
             case F_TRI: return launch_hash<F_TRI>(ctx, g, a);
@@ -770,6 +1069,8 @@ extern "C" int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out) {
 extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t) {
     if (!ctx || !g || !t) return GA_ERR_ARG;
     if (!tuning_valid(*t)) return ga_fail(ctx, GA_ERR_ARG, "invalid tuning");
+    if (t->flat_chunk && flat_shmem((uint32_t)t->flat_chunk) > ctx->lds_max)
+        return ga_fail(ctx, GA_ERR_ARG, "flat chunk %d does not fit the workgroup's LDS", t->flat_chunk);
     bool replan = !g->plan.tune.block || g->plan.tune.work_chunk != t->work_chunk || g->plan.tune.hash_cap != t->hash_cap ||
                   g->plan.tune.batches != t->batches;
     for (int b = 0; b < NHASH; b++) replan = replan || g->plan.tune.lanes[b] != t->lanes[b];
@@ -807,6 +1108,7 @@ extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* b
     const int lane_opts[5] = {4, 8, 16, 32, 64};
     for (int b = 0; b < NHASH; b++) {
         if (!g->plan.nitems[b]) continue;
+        if (b < 3 && flat_on(g)) continue;   // light rows go to the flat kernel
         for (int li = 0; li < 5; li++) {
             ga_tc_tuning t = bt;
             t.lanes[b] = lane_opts[li];
@@ -822,6 +1124,18 @@ extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* b
     for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
     const int chunks[3] = {2048, 8192, 32768};
     for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
+    if (g->plan.nlight) {
+        const int fsh[7][2] = {{0, 256}, {2048, 256}, {4096, 512}, {4096, 256}, {8192, 512}, {4096, 1024}, {2048, 512}};
+        for (int ci = 0; ci < 7; ci++) {
+            ga_tc_tuning t = bt;
+            t.flat_chunk = fsh[ci][0];
+            t.flat_block = fsh[ci][1];
+            if ((t.flat_chunk == bt.flat_chunk && t.flat_block == bt.flat_block) ||
+                (t.flat_chunk && flat_shmem((uint32_t)t.flat_chunk) > ctx->lds_max))
+                continue;
+            GA_TRY(trial(t));
+        }
+    }
     GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
     GA_TRY(ga_tc_ensure_plan(ctx, g));
     if (best) *best = bt;

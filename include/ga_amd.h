@@ -89,6 +89,10 @@ typedef struct ga_tc_tuning {
     int32_t batches;       /* lane-group batches (lanes probe lists each) per work item before a row is cut */
     int32_t load_shift;    /* hash table slots = next_pow2(d+(h)) << load_shift (1..3) */
     int32_t hash_cap;      /* largest d+(h) kept in an LDS table (<= 4096); larger rows use binary search in HBM */
+    int32_t flat_chunk;    /* plain counts: rows with d+(h) <= 128 are counted by the flat kernel in workgroup chunks of
+                              this many units (table slots, one per work entry, probe elements); 0 = lane groups only.
+                              A multiple of flat_block in [flat_block, 8192]; needs hash_cap >= 128 */
+    int32_t flat_block;    /* threads per workgroup of the flat kernel (64..1024, multiple of 64) */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
 int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t);

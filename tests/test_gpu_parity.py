@@ -164,7 +164,8 @@ def test_hub_graphs(ctx):
 
 
 def test_tuning_shapes(ctx):
-    """Every lane-group width, block size, load factor and work chunk gives the same exact count; the tuner too."""
+    """Every lane-group width, block size, load factor, work chunk and flat chunk/block gives the same exact count;
+    the tuner too."""
     from graph_analyzer_amd import gen
     raw = gen.chung_lu(30000, 300000, 2.05, 0.0, seed=4)
     og = oracle.OGraph.from_pairs(raw)
@@ -173,22 +174,49 @@ def test_tuning_shapes(ctx):
     _, sup_ref = og.edge_support()
     g = ga.DeviceGraph.from_pairs(ctx, raw)
     base = g.get_tuning()
-    for lanes in ([4, 4, 8, 16, 32], [8, 16, 32, 64, 64], [16, 32, 64, 64, 64], [64, 64, 64, 64, 64], [4, 8, 16, 32, 64]):
+    flats = ((0, 64), (2048, 256), (256, 64), (4096, 512), (4096, 1024))
+    for li, lanes in enumerate(([4, 4, 8, 16, 32], [8, 16, 32, 64, 64], [16, 32, 64, 64, 64], [64, 64, 64, 64, 64], [4, 8, 16, 32, 64])):
         for block in (64, 256, 1024):
             for chunk, shift, batches in ((64, 1, 1), (4096, 2, 8), (1 << 20, 3, 1024)):
                 t = ga.TcTuning()
                 t.lanes[:] = lanes
                 t.block, t.work_chunk, t.load_shift, t.batches = block, chunk, shift, batches
+                t.flat_chunk, t.flat_block = flats[(li + block) % len(flats)]
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)
                     assert g.count(ga.TC_HASH)[0] == T, t.as_dict()
         assert np.array_equal(g.vertex_participation()[1], tri_ref), lanes
         assert np.array_equal(g.edge_support()[1], sup_ref), lanes
+    for fc, fb in flats + ((512, 128), (8192, 512), (1024, 64), (16384, 1024)):
+        t = g.get_tuning()
+        t.hash_cap, t.flat_chunk, t.flat_block = 4096, fc, fb
+        try:
+            g.set_tuning(t)
+        except ga.GAError as e:   # a chunk whose LDS does not fit this device is refused up front
+            assert e.code == ga.GA_ERR_ARG and fc >= 16384, (fc, fb)
+            continue
+        assert g.count(ga.TC_HASH)[0] == T, t.as_dict()
     g.set_tuning(base)
     best, ms = g.tune()
     assert g.count(ga.TC_HASH)[0] == T and ms > 0
     g.close()
+
+
+def test_flat_kernel_edge_cases(ctx):
+    """Light rows only, heavy rows only (K_200: d+(h) up to 199) and a mix; chunks smaller than one row's units."""
+    from graph_analyzer_amd import gen
+    for raw in (np.array(clique(12), np.uint32), np.array(clique(200), np.uint32), gen.powerlaw_small(),
+                np.array([[0, 1], [1, 2], [0, 2]], np.uint32)):
+        og = oracle.OGraph.from_pairs(raw)
+        T = og.triangles()
+        g = ga.DeviceGraph.from_pairs(ctx, raw)
+        for fc, fb in ((256, 64), (512, 64), (2048, 256), (0, 64), (1024, 64)):
+            t = g.get_tuning()
+            t.flat_chunk, t.flat_block = fc, fb
+            g.set_tuning(t)
+            assert g.count(ga.TC_HASH)[0] == T, (g.n, fc, fb)
+        g.close()
 
 
 def test_peel_errors(ctx):

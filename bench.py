@@ -1,17 +1,27 @@
-"""Benchmark: exact triangle counting on one MI355X (oriented edges/s), with the HBM roofline of the counting kernels
-and the reference's CPU counter timed on this machine's host cores.
+"""Benchmark: exact triangle counting on one MI355X (oriented edges/s and the HBM roofline of the count), plus the
+4,096-graph batch (graphs/s, one launch set for all graphs vs one count per graph), with the reference's CPU counter
+timed on this machine's host cores.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat21|powerlaw_communities|powerlaw_small]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw_communities|rmat21|powerlaw_small]
+                    [--batch-graphs 4096] [--no-cpu] [--no-tune]
 
-A step is one full exact count T = sum over oriented edges (u,v) of |N+(u) ∩ N+(v)| of the resident graph (CSR already
-in HBM). The graph is generated from a fixed seed, built on the GPU (de-dup, orientation, sort; timed separately) and
-checked byte for byte against the CPU oracle's CSR, and every step's count must equal the CPU count; otherwise the
-script exits 1 without a result line.
+Count step: one full exact count T = sum over oriented edges (u,v) of |N+(u) ∩ N+(v)| of the resident graph (CSR
+already in HBM). The graph is generated from a fixed seed, built on the GPU (de-dup, orientation, sort; timed
+separately) and checked byte for byte against the CPU oracle's CSR, and every step's count must equal the CPU count;
+otherwise the script exits 1 without a result line.
 
 Roofline: algorithmic bytes per count B = 4(n+1) + 4m + 4 * sum over oriented edges of min(d+(u), d+(v)) (read the CSR
 once and, per edge, every element of the shorter out-list once); achieved = B / mean device time of one count (HIP
-events on the library's stream); peak = 8 TB/s HBM3E. traffic = measured HBM bytes per count from the committed
-rocprofv3 counter summary when one exists for the workload (else null).
+events on the library's stream around the count's kernels); peak = 8 TB/s HBM3E. traffic = HBM bytes per count
+measured with rocprofv3 counters (FETCH_SIZE + WRITE_SIZE of the count kernels, scaled by the same counters on a plain
+copy of known size, --copy-probe) when profiles/tc_hash_<workload>_traffic.json holds them, else null.
+
+Batch: the 4,096-graph mix (n log-uniform in [1k, 200k], mean degree uniform in [4, 64], half Erdos-Renyi, half
+Chung-Lu) resident as one disjoint union; graphs/s = graphs / wall time of one per-graph count of every graph, batched
+(one launch set) and one count per graph (same kernels, each graph its own launches and readback; the baseline is the
+faster of the library's default tuning and the batch's tuning). Both must give identical per-graph counts; a sample of
+graphs is also checked against the CPU oracle. Peel: batch peeling at eps = 1/10, all graphs' rounds in shared launches
+vs one fused peel per graph.
 """
 from __future__ import annotations
 
@@ -39,7 +49,8 @@ def make_workload(name):
         return gen.rmat21(), {"workload": "rmat_scale21_ef8", "generator": "R-MAT (0.57,0.19,0.19,0.05), 16,777,216 samples, seed 2102"}
     if name == "powerlaw_communities":
         return gen.powerlaw_communities(), {"workload": "chung_lu_4M_communities",
-                                            "generator": "Chung-Lu n=4.0M exponent 2.3 + 20k planted communities (10-200, p=0.3), seed 2103"}
+                                            "generator": "Chung-Lu n=4.0M exponent 2.3, 15.2M samples + 20k planted communities "
+                                                         "(10-200 vertices, p=0.3), seed 2103"}
     if name == "powerlaw_small":
         return gen.powerlaw_small(), {"workload": "chung_lu_8298", "generator": "Chung-Lu n=8,298 m=100,762 exponent 2.1, seed 2101"}
     raise SystemExit(f"unknown workload {name}")
@@ -61,9 +72,10 @@ def load_traffic(workload):
     return d.get("hbm_bytes_per_count")
 
 
-def cpu_baseline(pairs, T_gpu, budget_s):
+def cpu_baseline(pairs, T_gpu):
     """The reference's cpu_triangles_serial (compiled from its sources into oracle/_ref) when present, else the oracle
-    restatement; run on the whole graph, serially (1 thread). Also checks the GPU count."""
+    restatement; run on the whole graph, serially (1 thread), and with OpenMP on all host cores. Also checks the GPU
+    count."""
     import oracle
     t0 = time.perf_counter()
     if oracle.ref_available():
@@ -78,15 +90,78 @@ def cpu_baseline(pairs, T_gpu, budget_s):
     t0 = time.perf_counter()
     T_omp = g.triangles_omp(0)
     t_omp = time.perf_counter() - t0
-    import oracle as _o
-    threads = _o.olib().orc_omp_max_threads()
+    threads = oracle.olib().orc_omp_max_threads()
     if T_omp != T_gpu:
         raise SystemExit(f"MISMATCH: GPU count {T_gpu} != CPU count {T_omp}")
     t0 = time.perf_counter()
     T_ser = g.triangles()
     t_ser = time.perf_counter() - t0
-    assert T_ser == T_omp
+    if T_ser != T_omp:
+        raise SystemExit(f"MISMATCH: serial CPU count {T_ser} != OpenMP count {T_omp}")
     return {"kind": kind, "t_serial_s": t_ser, "t_omp_s": t_omp, "omp_threads": threads, "m": g.m}
+
+
+def run_batch(ctx, count, steps, tune):
+    """Batched vs one-count-per-graph on the batch mix; returns the result object or exits on any mismatch."""
+    import graph_analyzer_amd as ga
+    import oracle
+    from graph_analyzer_amd import gen
+    t0 = time.perf_counter()
+    specs = gen.batch_mix_specs(count)
+    bp = gen.BatchPairs(specs)
+    log(f"batch: generated {count} graphs, {bp.pairs.shape[0]} raw pairs in {time.perf_counter() - t0:.1f} s")
+    b = ga.Batch(ctx, flat=bp.pairs, offsets=bp.offsets)
+    del bp
+    log(f"batch: union n={b.total_n} m={b.total_m} built in {b.build_ms:.0f} ms")
+    # per-graph baseline with the library's default tuning (the first pass builds the per-graph plans)
+    Tp_def, _ = b.count_all(per_graph=True)
+    per_default = min(b.count_all(per_graph=True)[1] for _ in range(2))
+    tune_info = None
+    if tune:
+        t, tms = b.tune()
+        tune_info = t.as_dict()
+        log(f"batch: tuned {tune_info} ({tms:.2f} ms)")
+    T0, _ = b.count_all()
+    times_b = []
+    for _ in range(steps):
+        T, ms = b.count_all()
+        if not np.array_equal(T, T0):
+            raise SystemExit("MISMATCH: batched per-graph counts changed between steps")
+        times_b.append(ms)
+    per = [per_default]   # and with the batch's tuning (inherited by every graph); the faster one is the baseline
+    for _ in range(2):
+        Tp, ms = b.count_all(per_graph=True)
+        per.append(ms)
+    if not (np.array_equal(Tp, T0) and np.array_equal(Tp_def, T0)):
+        raise SystemExit("MISMATCH: per-graph launches disagree with the batched counts")
+    st, rr, pms = b.peel_all(1, 10, T_full=T0)
+    st2, rr2, pms2 = b.peel_all(1, 10, T_full=T0, per_graph=True)
+    if not (np.array_equal(st, st2) and np.array_equal(rr, rr2)):
+        raise SystemExit("MISMATCH: batched peel disagrees with per-graph peel")
+    # CPU oracle on a sample of graphs (smallest, largest, and a spread in between)
+    order = np.argsort(b.m)
+    sample = sorted(set(int(order[k]) for k in np.linspace(0, count - 1, 8).astype(int)))
+    for i in sample:
+        r, c, _ = b.graph(i)
+        og = oracle.OGraph.from_csr(r, c)
+        bn, bd, br, rounds, rr_o = og.batch_peel(1, 10, omp=True)
+        if og.triangles_omp(0) != int(T0[i]) or (int(st[i, 0]), int(st[i, 1]), int(st[i, 2]), int(st[i, 3])) != (bn, bd, br, rounds):
+            raise SystemExit(f"MISMATCH: batch graph {i} differs from the CPU oracle")
+        v0 = int(b.voff[i])
+        if not np.array_equal(rr[v0: v0 + int(b.n[i])], rr_o):
+            raise SystemExit(f"MISMATCH: batch graph {i} removal rounds differ from the CPU oracle")
+    t_b, t_p = float(np.mean(times_b)), float(min(per))
+    res = {"workload": f"batch_mix_{count}", "graphs": count, "total_n": int(b.total_n), "total_m": int(b.total_m),
+           "triangles": int(T0.sum()), "tuning": tune_info,
+           "count_graphs_per_s": count / (t_b * 1e-3), "count_ms": t_b,
+           "per_graph_count_graphs_per_s": count / (t_p * 1e-3), "per_graph_count_ms": t_p, "count_speedup": t_p / t_b,
+           "per_graph_count_ms_default_tuning": per_default, "per_graph_count_ms_batch_tuning": float(min(per[1:])),
+           "count_edges_per_s": b.total_m / (t_b * 1e-3),
+           "peel_eps": "1/10", "peel_graphs_per_s": count / (pms * 1e-3), "peel_ms": pms,
+           "per_graph_peel_graphs_per_s": count / (pms2 * 1e-3), "per_graph_peel_ms": pms2, "peel_speedup": pms2 / pms,
+           "oracle_checked_graphs": sample}
+    b.close()
+    return res
 
 
 def main():
@@ -94,12 +169,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="rmat21")
+    ap.add_argument("--workload", default="powerlaw_communities")
     ap.add_argument("--variant", default="hash")
+    ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batch comparison (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tuning-from", metavar="JSON",
+                    help="apply a recorded count tuning (no tuner): a result line of this script or a JSON object with a 'tuning' key")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="also run one plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+                    help="also run one plain device copy of MIB MiB (k_copy_probe; calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single MI355X)", file=sys.stderr)
@@ -128,7 +206,23 @@ def main():
     variant = ga.TC_VARIANTS[args.variant]
     prep_ms = g.prepare() if variant == ga.TC_HASH else 0.0
     tune_info = None
-    if variant == ga.TC_HASH and not args.no_tune:
+    if variant == ga.TC_HASH and args.tuning_from:
+        with open(args.tuning_from) as f:
+            text = f.read().strip()
+        try:
+            rec = json.loads(text)["tuning"]
+        except (ValueError, KeyError):
+            rec = json.loads(text.splitlines()[-1])["config"]["tuning"]
+        t = g.get_tuning()
+        for k, v in rec.items():
+            if k == "lanes":
+                t.lanes[:] = v
+            else:
+                setattr(t, k, v)
+        g.set_tuning(t)
+        tune_info = t.as_dict()
+        log(f"tuning from {args.tuning_from}: {tune_info}")
+    elif variant == ga.TC_HASH and not args.no_tune:
         t, tms = g.tune()
         tune_info = t.as_dict()
         log(f"tuned: {tune_info} best {tms:.3f} ms")
@@ -150,28 +244,33 @@ def main():
     value = g.m / (wall / args.steps)
     achieved = B_alg / (mean_dev * 1e-3) / 1e9
     log(f"T={T0} ms/step={ms_per_step:.4f} device mean {mean_dev:.4f} ms (min {min(dev_ms):.4f})")
+    if args.copy_probe:
+        cms = ctx.copy_probe(args.copy_probe << 20)
+        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written in {cms:.3f} ms")
 
     cpu = None
     if not args.no_cpu:
-        cb = cpu_baseline(pairs, T0, 30)
+        cb = cpu_baseline(pairs, T0)
         cpu = {"value": cb["m"] / cb["t_serial_s"], "unit": "oriented edges/s", "cores": 1, "kind": cb["kind"],
-               "sample": f"whole graph ({cb['m']} oriented edges), cpu_triangles_serial, 1 thread, single run; "
-                         f"OpenMP {cb['omp_threads']} threads: {cb['m'] / cb['t_omp_s']:.4g} edges/s",
+               "sample": f"whole graph ({cb['m']} oriented edges), cpu_triangles_serial, 1 thread, single run "
+                         f"({cb['t_serial_s']:.2f} s); OpenMP on {cb['omp_threads']} threads: "
+                         f"{cb['m'] / cb['t_omp_s']:.4g} edges/s ({cb['t_omp_s']:.2f} s)",
                "omp_value": cb["m"] / cb["t_omp_s"], "omp_cores": cb["omp_threads"]}
     else:
         ref_T = og.triangles_omp(0)
         if ref_T != T0:
             raise SystemExit(f"MISMATCH: GPU count {T0} != CPU count {ref_T}")
-    if args.copy_probe:
-        import torch
-        x = torch.ones(args.copy_probe << 18, dtype=torch.int32, device="cuda")
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written")
+    del og
     traffic = load_traffic(cfg["workload"])
+    n, m, max_out = g.n, g.m, g.max_out_degree
+    batch = None
+    if args.batch_graphs:
+        g.close()
+        batch = run_batch(ctx, args.batch_graphs, max(3, min(args.steps, 10)), not args.no_tune)
+        log(f"batch: {batch['count_graphs_per_s']:.0f} graphs/s batched vs {batch['per_graph_count_graphs_per_s']:.0f} per graph "
+            f"({batch['count_speedup']:.2f}x); peel {batch['peel_speedup']:.2f}x")
     out = {
-        "metric": "oriented edges/s (exact triangle count, one resident graph)",
+        "metric": "oriented edges/s (exact triangle count, one resident graph); batched graphs/s vs per-graph launch",
         "value": value,
         "unit": "oriented edges/s",
         "n_gpus": 1,
@@ -183,12 +282,13 @@ def main():
         "vs_baseline": None,
         "dtype": "u32 ids, u64 counts",
         "data": "synthetic (seeded generator)",
-        "config": {**cfg, "n": g.n, "m": g.m, "max_out_degree": g.max_out_degree, "triangles": T0, "variant": ga.variant_name(variant),
+        "config": {**cfg, "n": n, "m": m, "max_out_degree": max_out, "triangles": T0, "variant": ga.variant_name(variant),
                    "tuning": tune_info, "build_ms": g.build_ms, "plan_ms": prep_ms},
         "triangles_per_s": T0 / (wall / args.steps),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": B_alg, "sum_min_outdeg": summin, "kernel_ms_mean": mean_dev},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
             "ga_ctx_sync": (ctypes.c_int, [_vp]),
             "ga_ctx_device_info": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                   ctypes.POINTER(ctypes.c_int), _u64p]),
+            "ga_ctx_copy_probe": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_from_pairs": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_vp),
                                                    ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_from_csr": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.POINTER(_vp)]),
@@ -153,6 +154,12 @@ class Context:
         cus, wave, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
         self.check(lib().ga_ctx_device_info(self._h, name, 256, ctypes.byref(cus), ctypes.byref(wave), ctypes.byref(hbm)), "device_info")
         return {"name": name.value.decode(), "cus": cus.value, "wave": wave.value, "hbm_bytes": hbm.value}
+
+    def copy_probe(self, nbytes: int) -> float:
+        """One plain device copy of nbytes (k_copy_probe); returns its device ms."""
+        ms = ctypes.c_float()
+        self.check(lib().ga_ctx_copy_probe(self._h, nbytes, ctypes.byref(ms)), "copy_probe")
+        return ms.value
 
     def close(self):
         if self._h:
@@ -477,6 +484,7 @@ def _bind_batch(L):
         "ga_batch_download_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, _u32p]),
         "ga_batch_count": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_count_per_graph": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
+        "ga_batch_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_peel": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_peel_per_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p,
                                                    ctypes.POINTER(ctypes.c_float)]),
@@ -491,18 +499,25 @@ def _bind_batch(L):
 class Batch:
     """Many graphs resident together (ga_batch): per-graph counts and batch peeling with shared launches."""
 
-    def __init__(self, ctx: Context, graphs: Sequence[np.ndarray]):
+    def __init__(self, ctx: Context, graphs: Sequence[np.ndarray] = (), flat: Optional[np.ndarray] = None,
+                 offsets: Optional[np.ndarray] = None):
+        """graphs: one raw pair array per graph; or flat (k, 2) uint32 pairs of all graphs with offsets (count + 1
+        pair offsets), passed without a copy."""
         L = lib()
         _bind_batch(L)
         self.ctx = ctx
-        sizes = [int(np.asarray(p).reshape(-1, 2).shape[0]) for p in graphs]
-        off = np.zeros(len(graphs) + 1, np.uint64)
-        off[1:] = np.cumsum(sizes, dtype=np.uint64)
-        flat = np.concatenate([np.ascontiguousarray(p, dtype=np.uint32).reshape(-1) for p in graphs]) if graphs else np.zeros(0, np.uint32)
+        if flat is None:
+            sizes = [int(np.asarray(p).reshape(-1, 2).shape[0]) for p in graphs]
+            off = np.zeros(len(graphs) + 1, np.uint64)
+            off[1:] = np.cumsum(sizes, dtype=np.uint64)
+            flat = np.concatenate([np.ascontiguousarray(p, dtype=np.uint32).reshape(-1) for p in graphs]) if graphs else np.zeros(0, np.uint32)
+        else:
+            off = np.ascontiguousarray(offsets, dtype=np.uint64)
         flat = np.ascontiguousarray(flat, dtype=np.uint32)
+        count = len(off) - 1
         self._h = _vp()
         ms = ctypes.c_float()
-        ctx.check(L.ga_batch_create(ctx.handle, len(graphs), _ptr(off, _u64p), _ptr(flat, _u32p) if flat.size else None,
+        ctx.check(L.ga_batch_create(ctx.handle, count, _ptr(off, _u64p), _ptr(flat, _u32p) if flat.size else None,
                                     ctypes.byref(self._h), ctypes.byref(ms)), "batch_create")
         self.build_ms = ms.value
         c, tn, tm = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint64()
@@ -530,6 +545,13 @@ class Batch:
         f = lib().ga_batch_count_per_graph if per_graph else lib().ga_batch_count
         self.ctx.check(f(self.ctx.handle, self._h, _ptr(T, _u64p), ctypes.byref(ms)), "batch_count")
         return T, ms.value
+
+    def tune(self):
+        """Autotunes the union's count; returns (tuning, best ms)."""
+        t = TcTuning()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_batch_tune(self.ctx.handle, self._h, ctypes.byref(t), ctypes.byref(ms)), "batch_tune")
+        return t, ms.value
 
     def peel_all(self, num=1, den=10, T_full=None, per_graph: bool = False, removed_round: bool = True):
         stats = np.zeros(4 * self.count, np.uint64)

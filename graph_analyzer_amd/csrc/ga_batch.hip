@@ -52,14 +52,33 @@ __device__ __forceinline__ uint32_t graph_of_pair(const unsigned long long* __re
     return lo;
 }
 
+// Per-graph max id over non-loop pairs and "has an edge" flags. A graph's pairs are contiguous, so a wave almost
+// always covers one graph: then one atomic per wave, else one per lane.
 __global__ void k_batch_maxid(const uint32_t* __restrict__ pairs, unsigned long long k, const unsigned long long* __restrict__ off,
                               uint32_t count, uint32_t* __restrict__ mx, uint32_t* __restrict__ any) {
-    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (unsigned long long)gridDim.x * blockDim.x) {
-        const uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
-        if (a == b) continue;
-        const uint32_t g = graph_of_pair(off, count, i);
-        atomicMax(&mx[g], max(a, b));
-        if (!any[g]) atomicOr(&any[g], 1u);
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < k; base += stride) {   // wave-uniform loop
+        const unsigned long long i = base + threadIdx.x;
+        uint32_t g = 0xFFFFFFFFu, v = 0;
+        bool has = false;
+        if (i < k) {
+            const uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+            g = graph_of_pair(off, count, i);
+            has = a != b;
+            v = has ? max(a, b) : 0u;
+        }
+        const uint32_t g0 = __shfl(g, 0, 64);
+        if (__all(g == g0 || g == 0xFFFFFFFFu)) {
+            const uint32_t wm = wave_max_u32(v);
+            const bool wa = __any(has);
+            if (lane_id() == 0 && wa) {
+                atomicMax(&mx[g0], wm);
+                atomicOr(&any[g0], 1u);
+            }
+        } else if (has) {
+            atomicMax(&mx[g], v);
+            atomicOr(&any[g], 1u);
+        }
     }
 }
 
@@ -312,6 +331,8 @@ extern "C" int ga_batch_create(ga_ctx* ctx, uint32_t count, const uint64_t* pair
     float ms = 0;
     if ((rc = ga_build_from_device_pairs(ctx, (const uint32_t*)dp.p, k, (int64_t)b->total_n, false, &b->u, &ms))) return fail_b(rc);
     b->total_m = b->u->m;
+    ga_free(dp.p);   // the raw pairs are no longer needed: free them before the count plan is built
+    dp.p = nullptr;
     if (ga_malloc(ctx, (void**)&b->gof, (b->total_n + 1) * 4)) return fail_b(GA_ERR_OOM);
     k_batch_gof<<<std::min<uint32_t>(count, 65535), 256, 0, ctx->stream>>>((const unsigned long long*)dvoff.p, count, b->gof);
     if ((rc = ga_launch_check(ctx, "k_batch_gof"))) return fail_b(rc);
@@ -396,6 +417,13 @@ extern "C" int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms) 
     GA_HIP(ctx, hipMemcpyAsync(T, gc, (size_t)b->count * 8, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ms) *ms = (float)(wall_ms() - t0);
+    return GA_OK;
+}
+
+extern "C" int ga_batch_tune(ga_ctx* ctx, ga_batch* b, ga_tc_tuning* best, float* best_ms) {
+    if (!ctx || !b) return GA_ERR_ARG;
+    GA_TRY(ga_tc_tune(ctx, b->u, best, best_ms));
+    for (ga_graph* g : b->slices) GA_TRY(ga_tc_set_tuning(ctx, g, &b->u->plan.tune));
     return GA_OK;
 }
 

@@ -31,7 +31,11 @@ struct CoreDev {
 
 enum { CORE_ERR_T = 1, CORE_ERR_BIG = 2, CORE_ERR_STATS = 3 };
 
+struct CoreFusedMem;
+static void cf_release(CoreFusedMem* m);
+
 struct ga_core {
+    CoreFusedMem* fm = nullptr;    // device buffers of the fused loop (first ga_core_run; kept for later runs)
     ga_ctx* ctx = nullptr;
     ga_graph* g = nullptr;
     int incremental = 1;
@@ -234,6 +238,7 @@ extern "C" void ga_core_destroy(ga_core* c) {
     ga_free(c->tri);
     ga_free(c->frontier);
     ga_free(c->st);
+    cf_release(c->fm);
     delete c;
 }
 
@@ -741,6 +746,8 @@ struct CoreFusedMem {
     }
 };
 
+static void cf_release(CoreFusedMem* m) { delete m; }
+
 static int cf_hubs(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     const size_t n = g->n;
     void* idb = nullptr;
@@ -808,9 +815,14 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     if (T_full != UINT64_MAX && T_full >= (1ull << 32))
         return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
     GA_HIP(ctx, hipSetDevice(ctx->device));
-    CoreFusedMem M;
-    GA_TRY(cf_alloc(ctx, g, M));
-    GA_TRY(cf_hubs(ctx, g, M));
+    if (!c->fm) {   // allocated once per core object: later runs allocate and free nothing
+        CoreFusedMem* fm = new CoreFusedMem;
+        int rc = cf_alloc(ctx, g, *fm);
+        if (!rc) rc = cf_hubs(ctx, g, *fm);
+        if (rc) { delete fm; return rc; }
+        c->fm = fm;
+    }
+    CoreFusedMem& M = *c->fm;
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_core_reset_state(c));
     k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, T_full);

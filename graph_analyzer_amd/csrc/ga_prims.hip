@@ -110,6 +110,30 @@ extern "C" int ga_ctx_device_info(ga_ctx* ctx, char* name, int len, int* cus, in
     return GA_OK;
 }
 
+__global__ void k_copy_probe(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+extern "C" int ga_ctx_copy_probe(ga_ctx* ctx, uint64_t bytes, float* ms) {
+    if (!ctx || bytes < 16) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    const uint64_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    GA_TRY(ga_malloc(ctx, &a, n * 16));
+    if (ga_malloc(ctx, &b, n * 16)) { ga_free(a); return GA_ERR_OOM; }
+    int rc = ga_fill_u32(ctx, (uint32_t*)a, n * 4, 1u);
+    if (!rc && hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
+    if (!rc) {
+        k_copy_probe<<<grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 16u), 256, 0, ctx->stream>>>((const uint4*)a, (uint4*)b, n);
+        rc = ga_launch_check(ctx, "k_copy_probe");
+    }
+    if (!rc && (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess)) rc = GA_ERR_HIP;
+    if (!rc && ms) (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    ga_free(a);
+    ga_free(b);
+    return rc;
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // sorts and scans (rocPRIM device primitives, temporary storage from the context scratch)
 // ---------------------------------------------------------------------------------------------------------------

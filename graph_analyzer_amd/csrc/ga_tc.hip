@@ -692,6 +692,33 @@ PlanShape plan_shape(const ga_tc_tuning& t) {
 
 }  // namespace
 
+// Flat units and entry records of the light entries (built the first time the flat kernel is switched on).
+static int flat_build_units(ga_ctx* ctx, ga_graph* g) {
+    TcPlan& P = g->plan;
+    const uint32_t nlight = P.nlight;
+    DevBuf hstart_b, wt_b;
+    GA_TRY(ga_malloc(ctx, &hstart_b.p, ((size_t)g->n + 1) * 4));
+    GA_TRY(ga_malloc(ctx, &wt_b.p, ((size_t)nlight + 1) * 8));
+    uint32_t* hstart = (uint32_t*)hstart_b.p;
+    unsigned long long* wt = (unsigned long long*)wt_b.p;
+    k_plan_rowstart<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wh, nlight, hstart);
+    GA_TRY(ga_launch_check(ctx, "k_plan_rowstart"));
+    k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, P.wh, P.wlen, nlight, wt);
+    GA_TRY(ga_launch_check(ctx, "k_plan_units"));
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nlight, 1, 0));
+    GA_TRY(ga_malloc(ctx, (void**)&P.wu, ((size_t)nlight + 1) * 8));
+    GA_TRY(ga_exclusive_scan_u64(ctx, (const uint64_t*)wt, (uint64_t*)P.wu, (size_t)nlight + 1));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, P.wu + nlight, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    P.utot = ctx->hstat[0];
+    if (P.utot >= FLAT_UNIT_LIMIT) return GA_OK;   // too many units for 32-bit offsets: lane groups only
+    GA_TRY(ga_malloc(ctx, (void**)&P.frec, (size_t)nlight * 16));
+    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, P.wh, P.wps, P.wu, hstart, nlight, (uint4*)P.frec);
+    GA_TRY(ga_launch_check(ctx, "k_plan_frec"));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}
+
 // Cuts the light entries' units into chunks of tune.flat_chunk (nothing when the flat kernel is off).
 static int flat_rechunk(ga_ctx* ctx, ga_graph* g) {
     TcPlan& P = g->plan;
@@ -699,10 +726,11 @@ static int flat_rechunk(ga_ctx* ctx, ga_graph* g) {
     P.cstart = nullptr;
     P.nch = 0;
     P.chunk_used = (uint32_t)P.tune.flat_chunk;
-    if (!P.nlight || !P.tune.flat_chunk || !P.frec) return GA_OK;
+    if (!P.nlight || !P.tune.flat_chunk) return GA_OK;
+    if (!P.wu) GA_TRY(flat_build_units(ctx, g));
+    if (!P.frec) return GA_OK;
     const uint32_t chunk = (uint32_t)P.tune.flat_chunk;
     const unsigned long long nch = (P.utot + chunk - 1) / chunk;
-    if (nch >= 0xFFFFFFFFull) return ga_fail(ctx, GA_ERR_ARG, "too many flat chunks");
     GA_TRY(ga_malloc(ctx, (void**)&P.cstart, (nch + 1) * 4));
     k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.nlight, chunk, P.cstart);
     GA_TRY(ga_launch_check(ctx, "k_plan_chunks"));
@@ -816,21 +844,6 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     GA_TRY(ga_launch_check(ctx, "k_plan_rank"));
     k_plan_items<<<nblk, 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, span, cap, boff, P.items);
     GA_TRY(ga_launch_check(ctx, "k_plan_items"));
-    if (nlight) {   // flat units of the light entries (wt is free again: the item flags are done)
-        k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, wh, P.wlen, nlight, wt);
-        GA_TRY(ga_launch_check(ctx, "k_plan_units"));
-        GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nlight, 1, 0));
-        GA_TRY(ga_malloc(ctx, (void**)&P.wu, ((size_t)nlight + 1) * 8));
-        GA_TRY(ga_exclusive_scan_u64(ctx, (const uint64_t*)wt, (uint64_t*)P.wu, (size_t)nlight + 1));
-        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, P.wu + nlight, 8, hipMemcpyDeviceToHost, ctx->stream));
-        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        P.utot = ctx->hstat[0];
-        if (P.utot < FLAT_UNIT_LIMIT) {
-            GA_TRY(ga_malloc(ctx, (void**)&P.frec, (size_t)nlight * 16));
-            k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, wh, P.wps, P.wu, hstart, nlight, (uint4*)P.frec);
-            GA_TRY(ga_launch_check(ctx, "k_plan_frec"));
-        }
-    }
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int b = 0; b < NHASH; b++) P.nitems[b] = cnt[b];
     P.nhuge = cnt[NHASH];

@@ -60,6 +60,41 @@ inline uint32_t sample_cdf(const std::vector<uint64_t>& cdf, uint64_t r) {
     return (uint32_t)std::min(i, cdf.size() - 1);
 }
 
+// Walker/Vose alias table for w_i = (i + 1 + i0)^(-1/(gamma-1)): draw i uniformly, keep it when the second draw is
+// below prob[i] (64-bit fixed point), else take alias[i]. O(1) per endpoint.
+struct Alias {
+    std::vector<uint64_t> prob;
+    std::vector<uint32_t> alias;
+};
+
+Alias alias_table(uint32_t n, double gamma, double i0) {
+    std::vector<double> q(n);
+    const double ex = -1.0 / (gamma - 1.0);
+    double W = 0;
+    for (uint32_t i = 0; i < n; i++) { q[i] = std::pow((double)i + 1.0 + i0, ex); W += q[i]; }
+    for (uint32_t i = 0; i < n; i++) q[i] = q[i] * n / W;
+    Alias a;
+    a.prob.assign(n, UINT64_MAX);
+    a.alias.resize(n);
+    std::vector<uint32_t> small, large;
+    for (uint32_t i = 0; i < n; i++) (q[i] < 1.0 ? small : large).push_back(i);
+    for (uint32_t i = 0; i < n; i++) a.alias[i] = i;
+    while (!small.empty() && !large.empty()) {
+        const uint32_t s = small.back(), l = large.back();
+        small.pop_back();
+        a.prob[s] = (uint64_t)(q[s] * 18446744073709549568.0);
+        a.alias[s] = l;
+        q[l] = (q[l] + q[s]) - 1.0;
+        if (q[l] < 1.0) { large.pop_back(); small.push_back(l); }
+    }
+    return a;   // leftovers keep probability one
+}
+
+inline uint32_t sample_alias(const Alias& a, Rng& r) {
+    const uint32_t i = r.below((uint32_t)a.prob.size());
+    return r.next() < a.prob[i] ? i : a.alias[i];
+}
+
 std::vector<uint32_t> id_permutation(uint32_t n, uint64_t seed) {
     std::vector<uint32_t> p(n);
     for (uint32_t i = 0; i < n; i++) p[i] = i;
@@ -103,6 +138,27 @@ int ga_gen_chung_lu(uint32_t n, uint64_t samples, double gamma, double i0, uint6
         const uint64_t b = (uint64_t)c * CHUNK, e = std::min(samples, b + CHUNK);
         for (uint64_t i = b; i < e; i++) {
             uint32_t a = sample_cdf(cdf, r.next()), d = sample_cdf(cdf, r.next());
+            if (permute) { a = perm[a]; d = perm[d]; }
+            out->pairs[2 * i] = a;
+            out->pairs[2 * i + 1] = d;
+        }
+    }
+    return 0;
+}
+
+// Chung-Lu as above, endpoints drawn with an alias table (a different stream of graphs from ga_gen_chung_lu).
+int ga_gen_chung_lu_alias(uint32_t n, uint64_t samples, double gamma, double i0, uint64_t seed, int permute, ga_edges* out) {
+    if (n < 2 || gamma <= 1.0 || alloc_edges(out, samples, n)) return -1;
+    const Alias at = alias_table(n, gamma, i0);
+    std::vector<uint32_t> perm;
+    if (permute) perm = id_permutation(n, seed);
+    const uint64_t nchunks = (samples + CHUNK - 1) / CHUNK;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (long long c = 0; c < (long long)nchunks; c++) {
+        Rng r(stream_seed(seed, 5, (uint64_t)c));
+        const uint64_t b = (uint64_t)c * CHUNK, e = std::min(samples, b + CHUNK);
+        for (uint64_t i = b; i < e; i++) {
+            uint32_t a = sample_alias(at, r), d = sample_alias(at, r);
             if (permute) { a = perm[a]; d = perm[d]; }
             out->pairs[2 * i] = a;
             out->pairs[2 * i + 1] = d;
@@ -189,6 +245,30 @@ int ga_gen_er(uint32_t n, uint64_t samples, uint64_t seed, ga_edges* out) {
         for (uint64_t i = b; i < e; i++) { out->pairs[2 * i] = r.below(n); out->pairs[2 * i + 1] = r.below(n); }
     }
     return 0;
+}
+
+// Many graphs into one buffer: graph g gets `samples[g]` pairs generated exactly as ga_gen_er (kind 0),
+// ga_gen_chung_lu (kind 1) or ga_gen_chung_lu_alias (kind 2), ids permuted, would generate them alone; offsets[g] = first pair of graph g
+// (count + 1 entries). i0[g] < 0 means ga_gen_chung_lu_offset(n, samples, gamma, -i0[g]). Graphs are spread over
+// the OpenMP threads (inner loops run serially inside).
+int ga_gen_batch(uint32_t count, const uint32_t* n, const uint64_t* samples, const int32_t* kind, const double* gamma,
+                 const double* i0, const uint64_t* seed, uint64_t* offsets, ga_edges* out) {
+    offsets[0] = 0;
+    for (uint32_t g = 0; g < count; g++) offsets[g + 1] = offsets[g] + samples[g];
+    if (alloc_edges(out, offsets[count], 0)) return -1;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+    for (long long g = 0; g < (long long)count; g++) {
+        ga_edges one{};
+        const double off = i0[g] < 0 ? ga_gen_chung_lu_offset(n[g], samples[g], gamma[g], -i0[g]) : i0[g];
+        const int rc = kind[g] == 0   ? ga_gen_er(n[g], samples[g], seed[g], &one)
+                       : kind[g] == 1 ? ga_gen_chung_lu(n[g], samples[g], gamma[g], off, seed[g], 1, &one)
+                                      : ga_gen_chung_lu_alias(n[g], samples[g], gamma[g], off, seed[g], 1, &one);
+        if (rc) { bad++; continue; }
+        memcpy(out->pairs + 2 * offsets[g], one.pairs, samples[g] * 2 * sizeof(uint32_t));
+        ga_gen_free(&one);
+    }
+    return bad ? -1 : 0;
 }
 
 // Planted communities: `count` groups of distinct random ids with log-uniform sizes in [smin, smax]; every pair inside

@@ -35,9 +35,13 @@ def _gen():
         L.ga_gen_chung_lu_offset.restype = ctypes.c_double
         L.ga_gen_rmat.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, E]
         L.ga_gen_er.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, E]
+        L.ga_gen_chung_lu_alias.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, E]
         L.ga_gen_communities.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, E]
         L.ga_gen_free.argtypes = [E]
         L.ga_gen_free.restype = None
+        P = ctypes.POINTER
+        L.ga_gen_batch.argtypes = [ctypes.c_uint32, P(ctypes.c_uint32), P(ctypes.c_uint64), P(ctypes.c_int32), P(ctypes.c_double),
+                                   P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64), E]
         _gl = L
     return _gl
 
@@ -106,8 +110,8 @@ def powerlaw_communities(seed: int = 2103, n: int = 4_000_000, background: int =
 
 
 def batch_mix_specs(count: int = 4096, seed: int = 2104, nmin: int = 1000, nmax: int = 200_000):
-    """Per-graph parameters of the mix: n log-uniform in [nmin, nmax], mean degree uniform in [4, 64], half
-    Erdos-Renyi and half Chung-Lu with exponent uniform in [2.0, 2.5]."""
+    """Per-graph parameters of the mix: n log-uniform in [nmin, nmax], mean degree uniform in [4, 64] (n * deg / 2
+    sampled pairs), half Erdos-Renyi and half Chung-Lu (alias-table sampling) with exponent uniform in [2.0, 2.5]."""
     rng = np.random.default_rng(seed)
     specs = []
     for i in range(count):
@@ -119,11 +123,55 @@ def batch_mix_specs(count: int = 4096, seed: int = 2104, nmin: int = 1000, nmax:
     return specs
 
 
+def _cl_dmax(spec) -> float:
+    # cap the heaviest expected degree at 4 sqrt(n * deg) to keep counts within uint32 tri[] counters
+    n, deg = spec["n"], spec["deg"]
+    return min(n / 2.0, 4.0 * math.sqrt(n * deg))
+
+
+def _cl_offset(spec) -> float:
+    n, deg = spec["n"], spec["deg"]
+    return chung_lu_offset(n, int(n * deg / 2), spec["gamma"], _cl_dmax(spec))
+
+
 def batch_graph(spec) -> np.ndarray:
     n, deg = spec["n"], spec["deg"]
     samples = int(n * deg / 2)
     if spec["kind"] == "er":
         return erdos_renyi(n, samples, spec["seed"])
-    # cap the heaviest expected degree at sqrt(n * deg) * 4 to keep counts within uint32 tri[] counters
-    i0 = chung_lu_offset(n, samples, spec["gamma"], min(n / 2.0, 4.0 * math.sqrt(n * deg)))
-    return chung_lu(n, samples, spec["gamma"], i0, spec["seed"])
+    return _take(_gen().ga_gen_chung_lu_alias, n, samples, spec["gamma"], _cl_offset(spec), spec["seed"], 1)
+
+
+class BatchPairs:
+    """All graphs of a batch mix in one generator-owned buffer (no copy): pairs is a (k, 2) uint32 view, offsets the
+    count + 1 pair offsets. Keep this object alive while pairs is in use."""
+
+    def __init__(self, specs):
+        c = len(specs)
+        n = np.array([s["n"] for s in specs], np.uint32)
+        samples = np.array([int(s["n"] * s["deg"] / 2) for s in specs], np.uint64)
+        kind = np.array([0 if s["kind"] == "er" else 2 for s in specs], np.int32)
+        gamma = np.array([s["gamma"] for s in specs], np.float64)
+        i0 = np.array([0.0 if s["kind"] == "er" else -_cl_dmax(s) for s in specs], np.float64)   # solved in C
+        seed = np.array([s["seed"] for s in specs], np.uint64)
+        self.offsets = np.zeros(c + 1, np.uint64)
+        self._e = _Edges()
+        P = ctypes.POINTER
+        rc = _gen().ga_gen_batch(c, n.ctypes.data_as(P(ctypes.c_uint32)), samples.ctypes.data_as(P(ctypes.c_uint64)),
+                                 kind.ctypes.data_as(P(ctypes.c_int32)), gamma.ctypes.data_as(P(ctypes.c_double)),
+                                 i0.ctypes.data_as(P(ctypes.c_double)), seed.ctypes.data_as(P(ctypes.c_uint64)),
+                                 self.offsets.ctypes.data_as(P(ctypes.c_uint64)), ctypes.byref(self._e))
+        if rc != 0:
+            raise ValueError(f"batch generator failed with {rc}")
+        k = int(self._e.count)
+        self.pairs = np.ctypeslib.as_array(self._e.pairs, shape=(max(k, 1) * 2,))[: 2 * k].reshape(-1, 2)
+
+    def graph(self, i) -> np.ndarray:
+        return self.pairs[int(self.offsets[i]): int(self.offsets[i + 1])]
+
+    def __del__(self):
+        try:
+            self.pairs = None
+            _gen().ga_gen_free(ctypes.byref(self._e))
+        except Exception:
+            pass

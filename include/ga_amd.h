@@ -38,6 +38,10 @@ const char* ga_ctx_last_error(const ga_ctx* ctx);
 int ga_ctx_sync(ga_ctx* ctx);
 /* name (NUL-terminated, truncated to len), compute units, warp (wave) size, total HBM bytes */
 int ga_ctx_device_info(ga_ctx* ctx, char* name, int len, int* cus, int* wave, uint64_t* hbm_bytes);
+/* Copies `bytes` (rounded down to 16) between two fresh device buffers with one plain copy kernel (k_copy_probe):
+ * the known-size reference that scales hardware traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE).
+ * No counterpart in the reference (measurement support). */
+int ga_ctx_copy_probe(ga_ctx* ctx, uint64_t bytes, float* ms);
 
 /* ---- graphs (graph.h:19-96, densest.h:162-171) -------------------------------------------------------------- */
 /* n < 0: load_graph semantics on a raw pair list (graph.h:71-96 after the text parse): self loops dropped, pairs
@@ -153,6 +157,8 @@ int ga_batch_download_graph(ga_ctx* ctx, ga_batch* b, uint32_t i, uint32_t* rowp
 int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
 /* The same counts with one ga_tc_count per graph (the per-graph-launch baseline). */
 int ga_batch_count_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
+/* Autotunes the union's count (ga_tc_tune on the resident union); the per-graph baseline inherits the result. */
+int ga_batch_tune(ga_ctx* ctx, ga_batch* b, ga_tc_tuning* best, float* best_ms);
 /* Batch peeling of every graph with shared launches per round. stats: 4 per graph {best_num, best_den, best_round,
  * rounds}; removed_round: total_n entries, graph i at its vertex offset (or NULL); T_full: per-graph counts for the
  * round-0 check (or NULL). */

@@ -3,7 +3,7 @@
 timed on this machine's host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw_communities|rmat21|powerlaw_small]
-                    [--batch-graphs 4096] [--no-cpu] [--no-tune]
+                    [--batch-graphs 4096] [--serve-seconds 1.0] [--serve-rate 2000] [--no-cpu] [--no-tune]
 
 Count step: one full exact count T = sum over oriented edges (u,v) of |N+(u) ∩ N+(v)| of the resident graph (CSR
 already in HBM). The graph is generated from a fixed seed, built on the GPU (de-dup, orientation, sort; timed
@@ -22,6 +22,11 @@ Chung-Lu) resident as one disjoint union; graphs/s = graphs / wall time of one p
 faster of the library's default tuning and the batch's tuning). Both must give identical per-graph counts; a sample of
 graphs is also checked against the CPU oracle. Peel: batch peeling at eps = 1/10, all graphs' rounds in shared launches
 vs one fused peel per graph.
+
+Serving: Poisson arrivals (2,000 req/s for 1 s by default) of count / peel (eps 1/20) / core requests, 95% on a pool
+of 256 graphs of the batch size law and 5% on two R-MAT scale-20 graphs, all resident; p50 / p99 latency (arrival to
+result) and sustained throughput of the continuous-batching server vs the plain one-request-at-a-time form on the same
+arrivals, whose results must be identical.
 """
 from __future__ import annotations
 
@@ -164,6 +169,30 @@ def run_batch(ctx, count, steps, tune):
     return res
 
 
+def run_serving(rate, duration):
+    """Continuous batching (16 worker streams, count batching, footprint admission) vs the plain form (one request at
+    a time, one count per launch) on the same Poisson arrivals; every result must be identical in both."""
+    from graph_analyzer_amd import serve
+    source, reqs = serve.mixed_workload(rate=rate, duration=duration, seed=2106, pool=256)
+    keys = sorted({r.key for r in reqs})
+    out = {"workload": f"poisson_{rate:g}rps_{duration:g}s", "offered_rps": rate, "requests": len(reqs),
+           "mix": "count / peel eps 1/20 / core uniform; 95% on 256 graphs of the batch size law, 5% on 2 R-MAT scale-20 graphs"}
+    for name, slots, cb in (("continuous", 16, 64), ("plain", 1, 1)):
+        s = serve.Server(source, slots=slots, count_batch=cb, states_per_graph=4 if slots > 1 else 1)
+        t0 = time.perf_counter()
+        s.preload(keys)
+        log(f"serving ({name}): {len(keys)} graphs resident in {time.perf_counter() - t0:.1f} s, {s.used / 1e9:.1f} GB")
+        res = s.run(reqs)
+        s.close()
+        out[name] = serve.summarize(res)
+        out[name]["values"] = [r.value for r in res]
+        log(f"serving ({name}): {out[name]['throughput_rps']:.0f} req/s, p50 {out[name]['p50_ms']:.1f} ms, p99 {out[name]['p99_ms']:.1f} ms")
+    if out["continuous"].pop("values") != out["plain"].pop("values"):
+        raise SystemExit("MISMATCH: a serving result differs between the continuous and the plain (alone) runs")
+    out["results_identical_alone_and_in_mix"] = True
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +201,8 @@ def main():
     ap.add_argument("--workload", default="powerlaw_communities")
     ap.add_argument("--variant", default="hash")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batch comparison (0: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of Poisson arrivals in the serving comparison (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered requests per second in the serving comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tuning-from", metavar="JSON",
@@ -269,6 +300,9 @@ def main():
         batch = run_batch(ctx, args.batch_graphs, max(3, min(args.steps, 10)), not args.no_tune)
         log(f"batch: {batch['count_graphs_per_s']:.0f} graphs/s batched vs {batch['per_graph_count_graphs_per_s']:.0f} per graph "
             f"({batch['count_speedup']:.2f}x); peel {batch['peel_speedup']:.2f}x")
+    serving = None
+    if args.serve_seconds > 0:
+        serving = run_serving(args.serve_rate, args.serve_seconds)
     out = {
         "metric": "oriented edges/s (exact triangle count, one resident graph); batched graphs/s vs per-graph launch",
         "value": value,
@@ -289,6 +323,7 @@ def main():
                      "traffic": traffic, "algorithmic_bytes": B_alg, "sum_min_outdeg": summin, "kernel_ms_mean": mean_dev},
         "cpu_baseline": cpu,
         "batch": batch,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -84,6 +84,11 @@ def lib() -> ctypes.CDLL:
             "ga_graph_edge_order": (ctypes.c_int, [_vp, _vp, _u32p]),
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_tc_count_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_int, _u64p,
+                                                ctypes.POINTER(ctypes.c_float)]),
+            "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
+            "ga_peel_set_ctx": (ctypes.c_int, [_vp, _vp]),
+            "ga_core_set_ctx": (ctypes.c_int, [_vp, _vp]),
             "ga_tc_prepare": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
             "ga_tc_set_tuning": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning)]),
@@ -154,6 +159,20 @@ class Context:
         cus, wave, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
         self.check(lib().ga_ctx_device_info(self._h, name, 256, ctypes.byref(cus), ctypes.byref(wave), ctypes.byref(hbm)), "device_info")
         return {"name": name.value.decode(), "cus": cus.value, "wave": wave.value, "hbm_bytes": hbm.value}
+
+    def mem_info(self):
+        """(free, total) device bytes."""
+        f, t = ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(lib().ga_ctx_mem_info(self._h, ctypes.byref(f), ctypes.byref(t)), "mem_info")
+        return f.value, t.value
+
+    def count_many(self, graphs, variant: int = 0):
+        """Exact counts of several resident graphs, launched back to back on this context's stream, one readback."""
+        hs = (_vp * max(len(graphs), 1))(*[g.handle for g in graphs])
+        T = np.zeros(max(len(graphs), 1), np.uint64)
+        ms = ctypes.c_float()
+        self.check(lib().ga_tc_count_many(self._h, hs, len(graphs), variant, _ptr(T, _u64p), ctypes.byref(ms)), "tc_count_many")
+        return T[: len(graphs)], ms.value
 
     def copy_probe(self, nbytes: int) -> float:
         """One plain device copy of nbytes (k_copy_probe); returns its device ms."""
@@ -328,6 +347,11 @@ class Peel:
         self.ctx.check(lib().ga_peel_create(self.ctx.handle, g.handle, ctypes.byref(self._h)), "peel_create")
         self.last_ms = 0.0
 
+    def set_ctx(self, ctx: Context):
+        """Runs this (idle) state on another context's stream from now on."""
+        ctx.check(lib().ga_peel_set_ctx(self._h, ctx.handle), "peel_set_ctx")
+        self.ctx = ctx
+
     # backend concept (densest.h:265-278)
     def reset_state(self):
         self.ctx.check(lib().ga_peel_reset_state(self._h), "peel_reset_state")
@@ -404,6 +428,11 @@ class Core:
         self._h = _vp()
         self.ctx.check(lib().ga_core_create(self.ctx.handle, g.handle, 1 if incremental else 0, ctypes.byref(self._h)), "core_create")
         self.last_ms = 0.0
+
+    def set_ctx(self, ctx: Context):
+        """Runs this (idle) state on another context's stream from now on."""
+        ctx.check(lib().ga_core_set_ctx(self._h, ctx.handle), "core_set_ctx")
+        self.ctx = ctx
 
     def _stats(self):
         T, r, k = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()

@@ -225,8 +225,16 @@ extern "C" int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core
         k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, ~0ull);
         rc = ga_launch_check(ctx, "k_core_init");
     }
+    // finished before returning: the state may next run on another context's stream (ga_core_set_ctx)
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ga_fail(ctx, GA_ERR_HIP, "core state initialisation");
     if (rc) { ga_core_destroy(c); return rc; }
     *out = c;
+    return GA_OK;
+}
+
+extern "C" int ga_core_set_ctx(ga_core* c, ga_ctx* ctx) {
+    if (!c || !ctx || ctx->device != c->ctx->device) return GA_ERR_ARG;
+    c->ctx = ctx;   // every call on the state ends with a stream sync, so an idle state has nothing in flight
     return GA_OK;
 }
 

@@ -23,6 +23,8 @@ struct ga_ctx {
     uint64_t* dsmall = nullptr;         // 64 device words for counters of one call (never reallocated)
     void* scratch = nullptr;            // grow-only device scratch for library temporaries (sort / scan storage)
     size_t scratch_bytes = 0;
+    uint64_t* dmany = nullptr;          // grow-only per-graph counts of ga_tc_count_many
+    size_t dmany_n = 0;
     std::string err;
 };
 

@@ -143,8 +143,15 @@ extern "C" int ga_peel_create(ga_ctx* ctx, ga_graph* g, ga_peel** out) {
         return GA_ERR_OOM;
     }
     int rc = ga_tc_ensure_plan(ctx, g);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ga_fail(ctx, GA_ERR_HIP, "peel state initialisation");
     if (rc) { ga_peel_destroy(p); return rc; }
     *out = p;
+    return GA_OK;
+}
+
+extern "C" int ga_peel_set_ctx(ga_peel* p, ga_ctx* ctx) {
+    if (!p || !ctx || ctx->device != p->ctx->device) return GA_ERR_ARG;
+    p->ctx = ctx;   // every call on the state ends with a stream sync, so an idle state has nothing in flight
     return GA_OK;
 }
 

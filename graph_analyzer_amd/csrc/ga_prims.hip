@@ -83,6 +83,7 @@ extern "C" void ga_ctx_destroy(ga_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     ga_free(c->scratch);
+    ga_free(c->dmany);
     ga_free(c->dsmall);
     if (c->hstat) (void)hipHostFree(c->hstat);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -92,6 +93,16 @@ extern "C" void ga_ctx_destroy(ga_ctx* c) {
 }
 
 extern "C" const char* ga_ctx_last_error(const ga_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!ctx) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    GA_HIP(ctx, hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return GA_OK;
+}
 
 extern "C" int ga_ctx_sync(ga_ctx* ctx) {
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));

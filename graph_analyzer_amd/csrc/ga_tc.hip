@@ -1061,6 +1061,35 @@ extern "C" int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, f
     return GA_OK;
 }
 
+extern "C" int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, int variant, uint64_t* T, float* ms) {
+    if (!ctx || (count && (!graphs || !T))) return GA_ERR_ARG;
+    if (variant < 0 || variant >= GA_TC_NVARIANTS) return ga_fail(ctx, GA_ERR_ARG, "unknown variant %d", variant);
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    for (uint32_t i = 0; i < count; i++) {
+        if (!graphs[i]) return ga_fail(ctx, GA_ERR_ARG, "graph %u is NULL", i);
+        if (variant == GA_TC_AUTO || variant == GA_TC_HASH) GA_TRY(ga_tc_ensure_plan(ctx, graphs[i]));
+        if (variant == GA_TC_SORTED) GA_TRY(ga_graph_ensure_perm(ctx, graphs[i]));
+    }
+    if (count > ctx->dmany_n) {
+        ga_free(ctx->dmany);
+        ctx->dmany = nullptr;
+        ctx->dmany_n = 0;
+        const size_t want = std::max<size_t>(count, 256);
+        GA_TRY(ga_malloc(ctx, (void**)&ctx->dmany, want * 8));
+        ctx->dmany_n = want;
+    }
+    if (!count) return GA_OK;
+    GA_TRY(ga_fill_u64(ctx, ctx->dmany, count, 0));
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint32_t i = 0; i < count; i++)
+        GA_TRY(ga_tc_launch(ctx, graphs[i], variant, (unsigned long long*)ctx->dmany + i, nullptr, nullptr, nullptr, nullptr));
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(T, ctx->dmany, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return GA_OK;
+}
+
 extern "C" int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32_t* nitems6) {
     if (!ctx || !g) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));

@@ -42,6 +42,8 @@ int ga_ctx_device_info(ga_ctx* ctx, char* name, int len, int* cus, int* wave, ui
  * the known-size reference that scales hardware traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE).
  * No counterpart in the reference (measurement support). */
 int ga_ctx_copy_probe(ga_ctx* ctx, uint64_t bytes, float* ms);
+/* Free and total device memory (hipMemGetInfo): the serving loop's admission budget. */
+int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* ---- graphs (graph.h:19-96, densest.h:162-171) -------------------------------------------------------------- */
 /* n < 0: load_graph semantics on a raw pair list (graph.h:71-96 after the text parse): self loops dropped, pairs
@@ -82,6 +84,9 @@ const char* ga_tc_variant_name(int variant);
  * counting launch(es), measured with HIP events on the context stream. Preparation of the count plan (GA_TC_HASH,
  * GA_TC_SORTED) happens on first use and is excluded; call ga_tc_prepare to time it. */
 int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, float* ms);
+/* Counts `count` resident graphs back to back on ctx's stream with one readback at the end (T[i] for graphs[i]);
+ * plans are built first where missing. The serving loop's batched count. No counterpart in the reference. */
+int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, int variant, uint64_t* T, float* ms);
 /* Builds the device-side count plan (hash-side work lists, degree buckets, work order); ms = its device time. */
 int ga_tc_prepare(ga_ctx* ctx, ga_graph* g, float* ms);
 /* Lane-group width / hash capacity / block shape per degree bucket (the autotuner's output, see ga_tc_tune). */
@@ -114,6 +119,9 @@ int ga_tc_edge_support(ga_ctx* ctx, ga_graph* g, int variant, uint32_t* sup, uin
 
 /* ---- batch peeling: the backend concept of batch_peel_loop (densest.h:265-303, gpu_densest.cu:70-126) ------- */
 int ga_peel_create(ga_ctx* ctx, ga_graph* g, ga_peel** out);
+/* Moves an idle peel state (no call in flight; creation and every run return with their work finished) to another
+ * context: a serving worker runs it on its own stream. */
+int ga_peel_set_ctx(ga_peel* p, ga_ctx* ctx);
 void ga_peel_destroy(ga_peel* p);
 int ga_peel_reset_state(ga_peel* p);                 /* alive = 1, removed_round = UINT32_MAX */
 int ga_peel_begin_round(ga_peel* p);                 /* tri = 0, counters = 0 (fill kernels, no memset) */
@@ -128,6 +136,8 @@ int ga_peel_run(ga_peel* p, uint64_t T_full, uint64_t num, uint64_t den, uint64_
 
 /* ---- (k,triangle)-core: backends of core_decomp_loop / core_decomp_loop_inc (dsg.h:665-751, gpu_dsg.cu:88-199) - */
 int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core** out);
+/* Moves an idle core state to another context (its stream); same rule as ga_peel_set_ctx. */
+int ga_core_set_ctx(ga_core* c, ga_ctx* ctx);
 void ga_core_destroy(ga_core* c);
 int ga_core_reset_state(ga_core* c);                 /* alive/state = 1, core = UINT32_MAX */
 int ga_core_begin_round(ga_core* c);

@@ -71,3 +71,22 @@ def test_powerlaw_small_shape():
     r, c, s = g.arrays()
     deg = np.bincount(np.concatenate([c, s]), minlength=g.n)
     assert 500 < deg.max() < 2000
+
+
+def test_serving_workload_and_summary():
+    """The serving workload is seeded and shaped as specified; the summary's percentiles are computed per kind."""
+    from graph_analyzer_amd import serve
+    src, reqs = serve.mixed_workload(rate=2000.0, duration=1.0, seed=3, pool=8, rmat_graphs=2, rmat_scale=10)
+    src2, reqs2 = serve.mixed_workload(rate=2000.0, duration=1.0, seed=3, pool=8, rmat_graphs=2, rmat_scale=10)
+    assert [(r.kind, r.key, r.arrival) for r in reqs] == [(r.kind, r.key, r.arrival) for r in reqs2]
+    assert 1800 < len(reqs) < 2200                                   # Poisson(2000) over 1 s
+    share = np.mean([r.key[0] == "rmat" for r in reqs])
+    assert 0.02 < share < 0.09
+    assert {r.kind for r in reqs} == set(serve.KINDS) and all(r.den == 20 for r in reqs)
+    assert np.all(np.diff([r.arrival for r in reqs]) >= 0)
+    res = [serve.Result(r.rid, r.kind, r.key, (0,), r.arrival, r.arrival, r.arrival + 0.001 * (1 + r.rid % 10), 0, 1)
+           for r in reqs]
+    s = serve.summarize(res)
+    assert s["requests"] == len(reqs) and 1.0 <= s["p50_ms"] <= s["p99_ms"] <= 10.0 + 1e-9
+    assert set(s["by_kind"]) == set(serve.KINDS)
+    assert np.array_equal(src(("mix", 3)), src2(("mix", 3)))

@@ -1,0 +1,77 @@
+"""Runs one stage of the path on a named synthetic workload and prints its device time and result.
+
+    python -m graph_analyzer_amd count|peel|core|support [--workload rmat21|powerlaw_communities|powerlaw_small]
+                                 [--repeat R] [--num 1 --den 10]
+    python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
+
+Meant for profiling a single stage (rocprofv3 -- python3 -m graph_analyzer_amd core --workload rmat21) and for quick
+checks on a GPU host; results are not compared with a CPU oracle here (the tests and bench.py do that).
+"""
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+import numpy as np
+
+import graph_analyzer_amd as ga
+from graph_analyzer_amd import gen
+
+WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communities, "powerlaw_small": gen.powerlaw_small}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "batch"])
+    ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
+    ap.add_argument("--repeat", type=int, default=3)
+    ap.add_argument("--num", type=int, default=1)
+    ap.add_argument("--den", type=int, default=10)
+    ap.add_argument("--graphs", type=int, default=4096)
+    args = ap.parse_args(argv)
+    ctx = ga.Context(0)
+    if args.stage == "batch":
+        bp = gen.BatchPairs(gen.batch_mix_specs(args.graphs))
+        b = ga.Batch(ctx, flat=bp.pairs, offsets=bp.offsets)
+        del bp
+        print(f"batch {args.graphs} graphs n={b.total_n} m={b.total_m} build {b.build_ms:.1f} ms", flush=True)
+        for _ in range(args.repeat):
+            T, ms = b.count_all()
+            print(f"batched count: {ms:.3f} ms, T total {int(T.sum())}", flush=True)
+        st, _, ms = b.peel_all(args.num, args.den, T_full=T, removed_round=False)
+        print(f"batched peel {args.num}/{args.den}: {ms:.3f} ms, rounds max {int(st[:, 3].max())}", flush=True)
+        b.close()
+        return 0
+    t0 = time.perf_counter()
+    g = ga.DeviceGraph.from_pairs(ctx, WORKLOADS[args.workload]())
+    print(f"{args.workload}: n={g.n} m={g.m} max_out={g.max_out_degree} build {g.build_ms:.2f} ms "
+          f"(generated and built in {time.perf_counter() - t0:.1f} s)", flush=True)
+    T, ms = g.count(ga.TC_HASH)
+    print(f"count: T={T} {ms:.3f} ms (first, includes nothing but the count)", flush=True)
+    for _ in range(args.repeat):
+        if args.stage == "count":
+            T, ms = g.count(ga.TC_HASH)
+            print(f"count: T={T} {ms:.3f} ms", flush=True)
+        elif args.stage == "support":
+            t0 = time.perf_counter()
+            Ts, sup = g.edge_support(ga.TC_HASH)
+            print(f"support: sum={int(sup.astype(np.uint64).sum())} (= 3T: {int(sup.astype(np.uint64).sum()) == 3 * T}) "
+                  f"{(time.perf_counter() - t0) * 1e3:.1f} ms wall", flush=True)
+        elif args.stage == "peel":
+            p = ga.Peel(g)
+            r = p.run(args.num, args.den, T_full=T)
+            print(f"peel {args.num}/{args.den}: best {r.best_num}/{r.best_den} round {r.best_round} of {r.rounds}, "
+                  f"{p.last_ms:.3f} ms device", flush=True)
+            p.close()
+        else:
+            c = ga.Core(g, True)
+            core, rounds, kmax = c.run(T_full=T)
+            print(f"core: rounds {rounds} kmax {kmax} {c.last_ms:.3f} ms device", flush=True)
+            c.close()
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

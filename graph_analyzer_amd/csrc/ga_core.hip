@@ -354,6 +354,8 @@ struct CoreFusedDev {
     uint32_t nfv;         // frontier vertices this round
     uint32_t nent;        // frontier entries this round
     uint32_t tcursor;     // words of the round's table arena in use
+    uint32_t min_next;    // min tri over the alive vertices after this round (the next round's level candidate)
+    uint32_t ticket;      // workgroups of k_cf_tail that have finished this round
     unsigned long long adj_cursor;   // compaction cursor in the new adjacency buffer
 };
 
@@ -389,6 +391,8 @@ __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32
         f->nfv = 0;
         f->nent = 0;
         f->tcursor = 0;
+        f->min_next = 0xFFFFFFFFu;
+        f->ticket = 0;
         f->adj_cursor = 0;
     }
 }
@@ -469,13 +473,6 @@ __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restric
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
-}
-
-// Clears the round's table arena (its size is only known on the device).
-__global__ void k_cf_tclear(CoreBufs B, const CoreFusedDev* __restrict__ f, const CoreDev* __restrict__ st) {
-    if (st->done) return;
-    const uint32_t used = f->tcursor;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < used; i += gridDim.x * blockDim.x) B.tables[i] = GA_EMPTY;
 }
 
 // Each entry inserts its chunk of cN(u) (alive members only) into u's table.
@@ -651,93 +648,101 @@ __global__ void k_cf_hub_bits(const uint32_t* __restrict__ urow, const uint32_t*
     }
 }
 
-__global__ void k_cf_retire(CoreBufs B, const CoreFusedDev* __restrict__ f, uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
+// Round tail, one launch: retire the frontier, clear the table arena this round used, take the next round's min over
+// the alive list and, when the alive list halves, compact the list and the adjacency into the other buffers (alive =
+// state 1; frontier members read as 2 or 0, both excluded). The last workgroup to finish (atomic ticket, no waiting)
+// applies the round's end rules (dsg.h:744-747) and switches to the compacted buffers.
+__global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ state, const uint32_t* __restrict__ tri, CoreDev* st) {
+    __shared__ uint32_t last;
+    __shared__ uint32_t bmin;
     if (st->done) return;
-    const uint32_t nfv = f->nfv;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nfv; i += gridDim.x * blockDim.x) state[B.fvert[i]] = 0;
-}
-
-// Round epilogue (dsg.h:744-747 rules) and the compaction decision.
-__global__ void k_cf_end(CoreFusedDev* f, CoreDev* st) {
-    if (st->done) return;
-    const unsigned long long removed = f->nfv, k = st->k, nalive = st->nalive;
+    if (threadIdx.x == 0) bmin = 0xFFFFFFFFu;
+    const uint32_t nfv = f->nfv, used = f->tcursor, na = f->na;
+    const unsigned long long nalive = st->nalive;
+    const bool compact = nfv < nalive && 2ull * (nalive - nfv) <= na;   // the same decision in every workgroup
+    const uint32_t lane = lane_id();
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nthr = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = tid; i < nfv; i += nthr) state[B.fvert[i]] = 0;
+    for (uint64_t i = tid; i < used; i += nthr) B.tables[i] = GA_EMPTY;
+    const uint32_t* __restrict__ al = B.alist[f->alist_cur];
+    uint32_t* __restrict__ out = B.alist[f->alist_cur ^ 1u];
+    const uint64_t wave0 = tid / 64, nwaves = nthr / 64;
+    uint32_t mn = 0xFFFFFFFFu;
+    for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
+        const uint64_t i = base + lane;
+        uint32_t v = 0;
+        const bool keep = i < na && state[(v = al[i])] == 1;
+        if (keep) mn = min(mn, tri[v]);
+        if (compact) {
+            const uint64_t mask = __ballot(keep);
+            if (!mask) continue;
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+            uint32_t pos = 0;
+            if (lane == leader) pos = atomicAdd(&f->na_new, (uint32_t)__popcll(mask));
+            pos = __shfl(pos, leader, 64);
+            if (keep) out[pos + mask_rank(mask)] = v;
+        }
+    }
+    mn = wave_min_u32(mn);
+    __syncthreads();   // bmin initialised
+    if (lane == 0 && mn != 0xFFFFFFFFu) atomicMin(&bmin, mn);
+    if (compact) {   // one wave per surviving vertex copies its alive neighbours, in order, into the next buffer
+        const uint32_t cur = f->adj_cur, nxt = cur == 1 ? 2u : 1u;
+        const uint32_t* __restrict__ src = B.adj[cur];
+        uint32_t* __restrict__ dst = B.adjw[nxt];
+        for (uint64_t i = wave0; i < na; i += nwaves) {
+            const uint32_t v = al[i];
+            if (state[v] != 1) continue;   // wave-uniform
+            const uint32_t b = B.cbeg[v], e = B.cend[v];
+            uint32_t cnt = 0;
+            for (uint32_t p = b + lane; p < e; p += 64) cnt += state[src[p]] == 1;
+            cnt = (uint32_t)wave_sum_u64(cnt);
+            unsigned long long at = 0;
+            if (lane == 0 && cnt) at = atomicAdd(&f->adj_cursor, (unsigned long long)cnt);
+            at = __shfl(at, 0, 64);
+            uint32_t w = (uint32_t)at;
+            for (uint32_t p0 = b; p0 < e; p0 += 64) {   // wave-uniform, keeps the sorted order
+                const uint32_t p = p0 + lane;
+                const bool k2 = p < e && state[src[p]] == 1;
+                const uint64_t mask = __ballot(k2);
+                if (k2) dst[w + mask_rank(mask)] = src[p];
+                w += (uint32_t)__popcll(mask);
+            }
+            if (lane == 0) {
+                B.cbeg[v] = (uint32_t)at;
+                B.cend[v] = (uint32_t)at + cnt;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (bmin != 0xFFFFFFFFu) atomicMin(&f->min_next, bmin);   // one global atomic per workgroup
+        __threadfence();
+        last = atomicAdd(&f->ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    const unsigned long long removed = nfv, k = st->k;
     st->removed = removed;
+    f->ticket = 0;
     if (removed == 0 || removed > nalive || k < st->k_prev) { st->error = CORE_ERR_STATS; st->done = 1; return; }
     st->nalive = nalive - removed;
     st->k_prev = k;
     st->rounds++;
-    st->min_tri = 0xFFFFFFFFu;
+    st->min_tri = atomicAdd(&f->min_next, 0u);
+    f->min_next = 0xFFFFFFFFu;
     f->nfv = 0;
     f->nent = 0;
     f->tcursor = 0;
-    if (st->nalive == 0) { st->done = 1; return; }
-    f->compact = 2ull * st->nalive <= f->na ? 1u : 0u;
+    if (compact) {
+        f->na = atomicAdd(&f->na_new, 0u);
+        f->alist_cur ^= 1u;
+        f->adj_cur = f->adj_cur == 1 ? 2u : 1u;
+    }
     f->na_new = 0;
     f->adj_cursor = 0;
-}
-
-// Compaction 1/2: alive entries of alist go to the other alist buffer.
-__global__ void k_cf_compact_list(CoreBufs B, CoreFusedDev* f, const uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
-    if (st->done || !f->compact) return;
-    const uint32_t* __restrict__ al = B.alist[f->alist_cur];
-    uint32_t* __restrict__ out = B.alist[f->alist_cur ^ 1u];
-    const uint32_t na = f->na;
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
-    for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {
-        const uint64_t i = base + lane_id();
-        uint32_t v = 0;
-        const bool keep = i < na && state[(v = al[i])] == 1;
-        const uint64_t mask = __ballot(keep);
-        if (!mask) continue;
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-        uint32_t pos = 0;
-        if (lane_id() == leader) pos = atomicAdd(&f->na_new, (uint32_t)__popcll(mask));
-        pos = __shfl(pos, leader, 64);
-        if (keep) out[pos + mask_rank(mask)] = v;
-    }
-}
-
-// Compaction 2/2: one wave per surviving vertex copies its alive neighbours (in order) into the next adjacency buffer.
-__global__ void k_cf_compact_adj(CoreBufs B, CoreFusedDev* f, const uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
-    if (st->done || !f->compact) return;
-    const uint32_t* __restrict__ al = B.alist[f->alist_cur ^ 1u];
-    const uint32_t na = f->na_new;
-    const uint32_t cur = f->adj_cur, nxt = cur == 1 ? 2u : 1u;
-    const uint32_t* __restrict__ src = B.adj[cur];
-    uint32_t* __restrict__ dst = B.adjw[nxt];
-    const uint32_t lane = lane_id();
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
-    for (uint64_t i = wave0; i < na; i += nwaves) {
-        const uint32_t v = al[i];
-        const uint32_t b = B.cbeg[v], e = B.cend[v];
-        uint32_t cnt = 0;
-        for (uint32_t p = b + lane; p < e; p += 64) cnt += state[src[p]] == 1;
-        cnt = (uint32_t)wave_sum_u64(cnt);
-        unsigned long long base = 0;
-        if (lane == 0 && cnt) base = atomicAdd(&f->adj_cursor, (unsigned long long)cnt);
-        base = __shfl(base, 0, 64);
-        uint32_t w = (uint32_t)base;
-        for (uint32_t p0 = b; p0 < e; p0 += 64) {   // wave-uniform, keeps the sorted order
-            const uint32_t p = p0 + lane;
-            const bool keep = p < e && state[src[p]] == 1;
-            const uint64_t mask = __ballot(keep);
-            if (keep) dst[w + mask_rank(mask)] = src[p];
-            w += (uint32_t)__popcll(mask);
-        }
-        if (lane == 0) {
-            B.cbeg[v] = (uint32_t)base;
-            B.cend[v] = (uint32_t)base + cnt;
-        }
-    }
-}
-
-// Switch to the compacted buffers (runs after both compaction kernels).
-__global__ void k_cf_swap(CoreFusedDev* f, const CoreDev* __restrict__ st) {
-    if (st->done || !f->compact) return;
-    f->na = f->na_new;
-    f->alist_cur ^= 1u;
-    f->adj_cur = f->adj_cur == 1 ? 2u : 1u;
-    f->compact = 0;
+    if (st->nalive == 0) st->done = 1;
 }
 
 struct CoreFusedMem {
@@ -747,6 +752,7 @@ struct CoreFusedMem {
     void* block_bm = nullptr;
     CoreBufs B{};
     CoreFusedDev* f = nullptr;
+    size_t arena_words = 0;
     ~CoreFusedMem() {
         ga_free(block);
         ga_free(hubs);
@@ -809,6 +815,7 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     M.B.toff = p; p += n;
     M.B.tbits = p; p += n;
     M.B.tables = p; p += arena;
+    M.arena_words = arena;
     p = (uint32_t*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
     M.f = (CoreFusedDev*)p;
     return GA_OK;
@@ -837,25 +844,31 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     GA_TRY(ga_launch_check(ctx, "k_core_init"));
     k_cf_init<<<cgrid(ctx, g->n), 256, 0, ctx->stream>>>(M.f, M.B.alist[0], M.B.cbeg, M.B.cend, g->urow, g->n);
     GA_TRY(ga_launch_check(ctx, "k_cf_init"));
+    GA_TRY(ga_fill_u32(ctx, M.B.tables, M.arena_words, GA_EMPTY));   // each round's tail clears what it used
     if (g->n) {
         GA_TRY(init_tri_async(c));
         k_core_check_T<<<1, 1, 0, ctx->stream>>>(c->st);
         GA_TRY(ga_launch_check(ctx, "k_core_check_T"));
     }
     int block_rounds = 8;
-    const unsigned gv = cgrid(ctx, g->n), gu = ugrid(ctx);
+    // grids follow the alive count (read back after every block of rounds): small graphs and late rounds launch
+    // few workgroups, which leaves the device to other streams' work
+    unsigned gv = cgrid(ctx, g->n), gu = ugrid(ctx), gt = 0;
+    auto size_grids = [&](uint64_t alive) {
+        gv = cgrid(ctx, std::max<uint64_t>(alive, 1));
+        gu = std::min<unsigned>(ugrid(ctx), std::max<unsigned>(4u, (unsigned)((8 * alive + 255) / 256)));
+        gt = std::min<unsigned>(gv, (unsigned)std::max(ctx->cus, 1) * 2u);   // few workgroups: few tickets
+    };
+    size_grids(g->n);
+    // round 0's level candidate; later rounds get theirs from the previous round's tail
+    k_cf_min<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
+    GA_TRY(ga_launch_check(ctx, "k_cf_min"));
     for (uint64_t guard = 0;; guard++) {
-        for (int r = 0; r < block_rounds; r++) {
-            k_cf_min<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
+        for (int r = 0; r < block_rounds; r++) {   // 4 launches per round
             k_cf_find<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->core, c->st);
-            k_cf_tclear<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->st);
             k_cf_tinsert<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
             k_cf_update<<<gu * 4, 64, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
-            k_cf_retire<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
-            k_cf_end<<<1, 1, 0, ctx->stream>>>(M.f, c->st);
-            k_cf_compact_list<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
-            k_cf_compact_adj<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
-            k_cf_swap<<<1, 1, 0, ctx->stream>>>(M.f, c->st);
+            k_cf_tail<<<gt, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
         }
         GA_TRY(ga_launch_check(ctx, "core round"));
         GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, c->st, sizeof(CoreDev), hipMemcpyDeviceToHost, ctx->stream));
@@ -864,6 +877,7 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
         if (h->done) break;
         if (guard > (uint64_t)g->n + 16) return ga_fail(ctx, GA_ERR_STATE, "core decomposition did not finish");
         block_rounds = std::min(block_rounds * 2, 256);
+        size_grids(2 * h->nalive);   // the alive list is at most about twice the alive count (compaction at half)
     }
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     GA_HIP(ctx, hipEventSynchronize(ctx->ev1));

@@ -16,6 +16,10 @@
 
 enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4, F_GRAPH = 8 };
 
+#ifndef GA_TC_U8_FROM
+#define GA_TC_U8_FROM 32u   // lane-group phase length from which each lane keeps 8 loads in flight (else 4)
+#endif
+
 #define GA_EMPTY 0xFFFFFFFFu
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -220,8 +224,10 @@ __global__ void k_tc_hash(const TcArgs a) {
 #pragma unroll
             for (uint32_t S = G; S >= 1; S >>= 1) {
                 // entries [start, end) have length in (S/2, S] (for S = G: every length above G/2); each list is read
-                // by W = S/U lanes (at least 1), every lane holding up to U loads in flight
-                constexpr uint32_t U = 4;
+                // by W = S/U lanes (at least 1), every lane holding up to U loads in flight: 8 for lists longer than
+                // GA_TC_U8_FROM / 2 (more lists in flight at once), 4 for shorter ones
+                constexpr uint32_t UMAX = 8;
+                const uint32_t U = S >= GA_TC_U8_FROM ? 8u : 4u;
                 const uint32_t W = S >= U ? S / U : 1u;
                 const uint32_t end = (uint32_t)__popcll(group_bits<G>(__ballot(lk > S / 2)));
                 for (uint32_t k0 = start; k0 < end; k0 += G / W) {
@@ -230,12 +236,12 @@ __global__ void k_tc_hash(const TcArgs a) {
                     if (k < end) {
                         const uint32_t kps = stg[k], klen = stg[G + k];
                         for (uint32_t off = r; off < klen; off += U * W) {
-                            uint32_t x[U];
+                            uint32_t x[UMAX];
 #pragma unroll
-                            for (uint32_t u = 0; u < U; u++) x[u] = off + u * W < klen ? col[kps + off + u * W] : 0u;
+                            for (uint32_t u = 0; u < UMAX; u++) x[u] = (u < U && off + u * W < klen) ? col[kps + off + u * W] : 0u;
 #pragma unroll
-                            for (uint32_t u = 0; u < U; u++) {
-                                if (off + u * W >= klen) break;
+                            for (uint32_t u = 0; u < UMAX; u++) {
+                                if (u >= U || off + u * W >= klen) break;
                                 const uint32_t addr = kps + off + u * W;
                                 bool hit = false;
                                 uint32_t hk = 0;
@@ -339,12 +345,15 @@ struct FlatArgs {
     const uint32_t* wh;
     const uint4* frec;
     const uint32_t* cstart;
+    const uint32_t* chs;     // rowptr[h] of the row holding each chunk's first unit
     uint32_t nch, nlight, utot;
     unsigned long long* total;
     const uint32_t* done;
 };
 
-static size_t flat_shmem(uint32_t chunk) { return (size_t)(chunk + FLAT_TMAX) * 4 + (size_t)chunk * 2; }
+static size_t flat_shmem(uint32_t chunk) {
+    return (size_t)(chunk + FLAT_TMAX) * 4 + (size_t)chunk * 2 + (size_t)(chunk / 2 + 2) * 16;   // table, owners, records
+}
 
 __device__ __forceinline__ void flat_insert(uint32_t* tab, uint32_t tbits, uint32_t x) {
     const uint32_t tsz = 1u << tbits;
@@ -357,37 +366,70 @@ __device__ __forceinline__ void flat_insert(uint32_t* tab, uint32_t tbits, uint3
 
 template <int UPT>
 __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
+    static_assert(UPT == 4 || UPT == 8 || UPT == 16, "4, 8 or 16 units per thread");
+    constexpr int PF = 2;   // entry records per thread fetched one chunk ahead
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t wtot[16];
     if (a.done && *a.done) return;
     const uint32_t T = blockDim.x, CH = UPT * T, tid = threadIdx.x;
     uint32_t* tab = lds;                                // CH + FLAT_TMAX slots ([CH, CH + FLAT_TMAX) = carried row)
-    uint16_t* own = (uint16_t*)(tab + CH + FLAT_TMAX);  // unit -> entry
+    uint16_t* own = (uint16_t*)(tab + CH + FLAT_TMAX);  // unit -> entry (16-byte aligned: CH + FLAT_TMAX is a multiple of 4)
+    uint4* own4 = (uint4*)own;                          // 8 owner slots per uint4
+    uint4* tab4 = (uint4*)tab;
+    uint4* erec = (uint4*)(own + CH);                   // the chunk's entry records (at most CH / 2 + 2)
     const uint32_t lane = lane_id(), wv = tid / 64;
     uint64_t local = 0;
-    for (uint32_t c = blockIdx.x; c < a.nch; c += gridDim.x) {
+    // next chunk's entry range, carried-row start and first PF records per thread, loaded a chunk ahead
+    uint32_t c = blockIdx.x, nj0 = 0, nne = 0, nhs = 0;
+    uint4 pre[PF];
+    auto fetch = [&](uint32_t cc) {
+        nj0 = a.cstart[cc];
+        nne = min(a.cstart[cc + 1] + 1, a.nlight) - nj0;
+        nhs = a.chs[cc];
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const uint32_t i = tid + q * T;
+            pre[q] = i < nne ? a.frec[nj0 + i] : make_uint4(0, 0, 0, 0);
+        }
+    };
+    if (c < a.nch) fetch(c);
+    for (; c < a.nch; c += gridDim.x) {
         const uint32_t u0 = c * CH;
         const uint32_t clen = min(CH, a.utot - u0);
-        const uint32_t j0 = a.cstart[c];
-        const uint32_t ne = min(a.cstart[c + 1] + 1, a.nlight) - j0;
+        const uint32_t j0 = nj0, ne = nne, hs0 = nhs;
+        for (uint32_t w = tid; w < (CH + FLAT_TMAX) / 4; w += T) tab4[w] = make_uint4(GA_EMPTY, GA_EMPTY, GA_EMPTY, GA_EMPTY);
+        for (uint32_t w = tid; w < CH / 8; w += T) own4[w] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < PF; q++)
+            if (tid + q * T < ne) erec[tid + q * T] = pre[q];
+        for (uint32_t i = tid + PF * T; i < ne; i += T) erec[i] = a.frec[j0 + i];   // chunks with many short entries
+        __syncthreads();
         // the row of entry j0: carried when its table begins before this chunk
-        const uint4 r0 = a.frec[j0];
+        const uint4 r0 = erec[0];
         const bool first0 = (r0.w >> 16) & 1u;
         const uint32_t tu0 = first0 ? r0.x : r0.x - r0.y;
         const bool carried = tu0 < u0;
-        const uint32_t hs0 = carried ? (first0 ? r0.y : a.rowptr[a.wh[j0]]) : 0u;
-        for (uint32_t u = tid; u < CH + FLAT_TMAX; u += T) tab[u] = GA_EMPTY;
-        for (uint32_t u = tid; u < CH; u += T) own[u] = 0;
-        __syncthreads();
         for (uint32_t i = tid + 1; i < ne; i += T) {
-            const uint32_t U = a.frec[j0 + i].x - u0;   // > 0: entry j0 holds unit u0
+            const uint32_t U = erec[i].x - u0;   // > 0: entry j0 holds unit u0
             if (U < clen) own[U] = 1;
         }
         __syncthreads();
-        {   // inclusive prefix sum of the marks: UPT contiguous units per thread
+        {   // inclusive prefix sum of the marks, UPT contiguous units per thread read as 16-byte words
+            uint32_t m[UPT];
+#pragma unroll
+            for (int q = 0; q < UPT / 8 + (UPT == 4 ? 1 : 0); q++) {
+                if (UPT == 4) {
+                    const uint2 v = ((const uint2*)own)[tid];
+                    m[0] = v.x & 0xFFFFu; m[1] = v.x >> 16; m[2] = v.y & 0xFFFFu; m[3] = v.y >> 16;
+                } else {
+                    const uint4 v = own4[tid * (UPT / 8) + q];
+                    m[8 * q + 0] = v.x & 0xFFFFu; m[8 * q + 1] = v.x >> 16; m[8 * q + 2] = v.y & 0xFFFFu; m[8 * q + 3] = v.y >> 16;
+                    m[8 * q + 4] = v.z & 0xFFFFu; m[8 * q + 5] = v.z >> 16; m[8 * q + 6] = v.w & 0xFFFFu; m[8 * q + 7] = v.w >> 16;
+                }
+            }
             uint32_t run = 0;
 #pragma unroll
-            for (int k = 0; k < UPT; k++) run += own[tid * UPT + k];
+            for (int k = 0; k < UPT; k++) run += m[k];
             uint32_t inc = run;
 #pragma unroll
             for (uint32_t o = 1; o < 64; o <<= 1) {
@@ -399,37 +441,51 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
             uint32_t acc = inc - run;
             for (uint32_t w = 0; w < wv; w++) acc += wtot[w];
 #pragma unroll
-            for (int k = 0; k < UPT; k++) {
-                acc += own[tid * UPT + k];
-                own[tid * UPT + k] = (uint16_t)acc;
+            for (int k = 0; k < UPT; k++) { acc += m[k]; m[k] = acc; }
+#pragma unroll
+            for (int q = 0; q < UPT / 8 + (UPT == 4 ? 1 : 0); q++) {
+                if (UPT == 4) {
+                    ((uint2*)own)[tid] = make_uint2(m[0] | (m[1] << 16), m[2] | (m[3] << 16));
+                } else {
+                    own4[tid * (UPT / 8) + q] = make_uint4(m[8 * q] | (m[8 * q + 1] << 16), m[8 * q + 2] | (m[8 * q + 3] << 16),
+                                                           m[8 * q + 4] | (m[8 * q + 5] << 16), m[8 * q + 6] | (m[8 * q + 7] << 16));
+                }
             }
         }
         __syncthreads();
-        // classify this thread's units and issue their loads
-        uint32_t x[UPT], meta[UPT];   // meta: kind (bit 31 insert, bit 30 probe) | bits << 16 | table base
+        // classify this thread's units: owner indices and entry records from LDS, then one element load per unit (col[0]
+        // for units that neither insert nor probe), all loads of a phase in flight together, selects instead of branches
+        uint32_t x[UPT], meta[UPT];   // meta: bit 31 insert, bit 30 probe | bits << 16 | table base
+        {
+            uint32_t ix[UPT];
 #pragma unroll
-        for (int k = 0; k < UPT; k++) {
-            const uint32_t u = k * T + tid;
-            meta[k] = 0;
-            x[k] = 0;
-            if (u >= clen) continue;
-            const uint4 r = a.frec[j0 + own[u]];
-            const uint32_t tb = r.w & 31u, dh = (r.w >> 8) & 0xFFu;
-            const bool first = (r.w >> 16) & 1u;
-            const uint32_t U = r.x - u0;   // unsigned: wraps for the entry that began before the chunk
-            const uint32_t q = u - U;
-            const uint32_t t = first ? (1u << tb) : 0u;
-            if (q < t) {   // table unit of a row whose first entry is in this chunk
-                if (q < dh && t <= clen && U <= clen - t) {   // the whole table lies in this chunk
-                    x[k] = a.col[r.y + q];
-                    meta[k] = 0x80000000u | (tb << 16) | U;
-                }
-            } else if (q > t) {   // element unit
-                x[k] = a.col[r.z + (q - t - 1)];
-                const uint32_t tu = first ? r.x : r.x - r.y;
-                const uint32_t base = tu >= u0 ? tu - u0 : CH;
-                meta[k] = 0x40000000u | (tb << 16) | base;
+            for (int k = 0; k < UPT; k++) {
+                const uint32_t u = k * T + tid;
+                ix[k] = own[u < clen ? u : 0u];
             }
+            uint4 r[UPT];
+#pragma unroll
+            for (int k = 0; k < UPT; k++) r[k] = erec[ix[k]];
+            uint32_t addr[UPT];
+#pragma unroll
+            for (int k = 0; k < UPT; k++) {
+                const uint32_t u = k * T + tid;
+                const bool valid = u < clen;
+                const uint32_t tb = r[k].w & 31u, dh = (r[k].w >> 8) & 0xFFu;
+                const bool first = (r[k].w >> 16) & 1u;
+                const uint32_t U = r[k].x - u0;   // unsigned: wraps for the entry that began before the chunk
+                const uint32_t q = u - U;
+                const uint32_t t = first ? (1u << tb) : 0u;
+                const bool ins = valid & (q < t) & (q < dh) & (t <= clen) & (U <= clen - t);   // whole table in the chunk
+                const bool prb = valid & (q > t);
+                const uint32_t tu = first ? r[k].x : r[k].x - r[k].y;
+                // table base: the insert target, the probed row's table (here or the carried region), 0 otherwise
+                const uint32_t base = ins ? U : (prb ? (tu >= u0 ? tu - u0 : CH) : 0u);
+                addr[k] = ins ? r[k].y + q : (prb ? r[k].z + (q - t - 1) : 0u);
+                meta[k] = (ins ? 0x80000000u : 0u) | (prb ? 0x40000000u : 0u) | (tb << 16) | (base & 0xFFFFu);
+            }
+#pragma unroll
+            for (int k = 0; k < UPT; k++) x[k] = a.col[addr[k]];
         }
         if (carried) {
             const uint32_t tb = r0.w & 31u, dh = (r0.w >> 8) & 0xFFu;
@@ -438,19 +494,26 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
 #pragma unroll
         for (int k = 0; k < UPT; k++)
             if (meta[k] & 0x80000000u) flat_insert(tab + (meta[k] & 0xFFFFu), (meta[k] >> 16) & 31u, x[k]);
+        if (c + gridDim.x < a.nch) fetch(c + gridDim.x);   // lands while this chunk probes
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < UPT; k++) {
-            if (!(meta[k] & 0x40000000u)) continue;
-            const uint32_t tb = (meta[k] >> 16) & 31u, tsz = 1u << tb;
+            const uint32_t tb = (meta[k] >> 16) & 31u, tm = (1u << tb) - 1u;
             const uint32_t* tt = tab + (meta[k] & 0xFFFFu);
-            uint32_t s = hash_slot(x[k], tb);
-            for (uint32_t probe = 0; probe < tsz; probe++) {
-                const uint32_t v = tt[s];
-                if (v == x[k]) { local++; break; }
-                if (v == GA_EMPTY) break;
-                s = (s + 1) & (tsz - 1);
+            const uint32_t s0 = hash_slot(x[k], tb);
+            // two slots unrolled (tables are at most a quarter full), the rare longer chain in a loop
+            const uint32_t v0 = (meta[k] & 0x40000000u) ? tt[s0] : GA_EMPTY;
+            bool hit = v0 == x[k] && v0 != GA_EMPTY;
+            if (v0 != GA_EMPTY && !hit) {
+                uint32_t s = (s0 + 1) & tm;
+                for (uint32_t probe = 1; probe <= tm; probe++) {
+                    const uint32_t v = tt[s];
+                    if (v == x[k]) { hit = true; break; }
+                    if (v == GA_EMPTY) break;
+                    s = (s + 1) & tm;
+                }
             }
+            local += hit ? 1u : 0u;
         }
         __syncthreads();
     }
@@ -529,11 +592,15 @@ __global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t*
     }
 }
 
-// cstart[c] = the light entry whose units contain unit c * chunk.
-__global__ void k_plan_chunks(const unsigned long long* __restrict__ wu, uint32_t nlight, uint32_t chunk, uint32_t* __restrict__ cstart) {
+// cstart[c] = the light entry whose units contain unit c * chunk; chs[c] = rowptr[h] of that entry's row.
+__global__ void k_plan_chunks(const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ rowptr,
+                              uint32_t nlight, uint32_t chunk, uint32_t* __restrict__ cstart, uint32_t* __restrict__ chs) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
         const unsigned long long a = wu[j], b = wu[j + 1];
-        for (unsigned long long c = (a + chunk - 1) / chunk; c * chunk < b; c++) cstart[c] = j;
+        for (unsigned long long c = (a + chunk - 1) / chunk; c * chunk < b; c++) {
+            cstart[c] = j;
+            chs[c] = rowptr[wh[j]];
+        }
     }
 }
 
@@ -731,8 +798,9 @@ static int flat_rechunk(ga_ctx* ctx, ga_graph* g) {
     if (!P.frec) return GA_OK;
     const uint32_t chunk = (uint32_t)P.tune.flat_chunk;
     const unsigned long long nch = (P.utot + chunk - 1) / chunk;
-    GA_TRY(ga_malloc(ctx, (void**)&P.cstart, (nch + 1) * 4));
-    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.nlight, chunk, P.cstart);
+    GA_TRY(ga_malloc(ctx, (void**)&P.cstart, (nch + 1) * 8));   // cstart (nch + 1) then chs (nch)
+    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.wh, g->rowptr, P.nlight, chunk, P.cstart,
+                                                                          P.cstart + nch + 1);
     GA_TRY(ga_launch_check(ctx, "k_plan_chunks"));
     GA_TRY(ga_fill_u32(ctx, P.cstart + nch, 1, P.nlight));
     P.nch = (uint32_t)nch;
@@ -900,6 +968,7 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, cons
     f.wh = P.wh;
     f.frec = (const uint4*)P.frec;
     f.cstart = P.cstart;
+    f.chs = P.cstart + P.nch + 1;
     f.nch = P.nch;
     f.nlight = P.nlight;
     f.utot = (uint32_t)P.utot;
@@ -1167,8 +1236,9 @@ extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* b
     const int chunks[3] = {2048, 8192, 32768};
     for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
     if (g->plan.nlight) {
-        const int fsh[7][2] = {{0, 256}, {2048, 256}, {4096, 512}, {4096, 256}, {8192, 512}, {4096, 1024}, {2048, 512}};
-        for (int ci = 0; ci < 7; ci++) {
+        // 4 or 8 units per thread (16 keeps too many records and loads in registers)
+        const int fsh[6][2] = {{0, 256}, {1024, 256}, {2048, 512}, {2048, 256}, {4096, 512}, {4096, 1024}};
+        for (int ci = 0; ci < 6; ci++) {
             ga_tc_tuning t = bt;
             t.flat_chunk = fsh[ci][0];
             t.flat_block = fsh[ci][1];

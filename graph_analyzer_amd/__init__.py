@@ -88,6 +88,7 @@ def lib() -> ctypes.CDLL:
                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
             "ga_peel_set_ctx": (ctypes.c_int, [_vp, _vp]),
+            "ga_core_run_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_core_set_ctx": (ctypes.c_int, [_vp, _vp]),
             "ga_tc_prepare": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
@@ -523,6 +524,20 @@ def _bind_batch(L):
         f.restype = res
         f.argtypes = args
     L._batch_bound = True
+
+
+def core_many(ctx: "Context", cores: Sequence["Core"], T_full: Optional[Sequence[Optional[int]]] = None):
+    """Incremental core decompositions of several resident graphs in one launch (ga_core_run_many: one workgroup per
+    graph, all rounds inside). Returns ([(core, rounds, kmax) per state], device ms)."""
+    hs = (_vp * max(len(cores), 1))(*[c._h for c in cores])
+    tf = None
+    if T_full is not None:
+        tf = np.array([(1 << 64) - 1 if t is None else int(t) for t in T_full], np.uint64)
+    stats = np.zeros(max(2 * len(cores), 2), np.uint32)
+    ms = ctypes.c_float()
+    ctx.check(lib().ga_core_run_many(ctx.handle, hs, len(cores), None if tf is None else _ptr(tf, _u64p), _ptr(stats, _u32p),
+                                     ctypes.byref(ms)), "core_run_many")
+    return [(c.core(), int(stats[2 * i]), int(stats[2 * i + 1])) for i, c in enumerate(cores)], ms.value
 
 
 class Batch:

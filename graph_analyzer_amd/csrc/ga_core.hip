@@ -36,6 +36,7 @@ static void cf_release(CoreFusedMem* m);
 
 struct ga_core {
     CoreFusedMem* fm = nullptr;    // device buffers of the fused loop (first ga_core_run; kept for later runs)
+    uint32_t* blk = nullptr;       // two alive lists of ga_core_run_many (first use; kept)
     ga_ctx* ctx = nullptr;
     ga_graph* g = nullptr;
     int incremental = 1;
@@ -120,8 +121,37 @@ __global__ void k_core_find(const uint32_t* __restrict__ tri, uint8_t* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
 }
 
-// Triangles {u, v, w} through frontier vertex u with w > v: subtract once, by the smallest-id frontier member
-// (kernels.cuh:574-618). One wave per frontier vertex, lanes over N(u).
+// Triangles {u, v, w} through frontier vertex u, v = ucol[p] in N(u), w > v: subtract once, by the smallest-id
+// frontier member (kernels.cuh:574-618). Walks the shorter of N(u)-after-v and N(v)-after-v and binary-searches the
+// other, so hubs on either side stay cheap.
+__device__ __forceinline__ void core_pair(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol, const uint8_t* __restrict__ state,
+                                          uint32_t* __restrict__ tri, uint32_t u, uint32_t p, uint32_t ue) {
+    const uint32_t v = ucol[p];
+    const uint8_t sv = state[v];
+    if (sv == 0) return;
+    const uint32_t a = p + 1;                                                // N(u) after v: all > v
+    const uint32_t vb = urow[v], ve = urow[v + 1];
+    const uint32_t b = upper_bound_u32(ucol, vb, ve, v);                     // N(v) after v
+    const bool a_short = (ue - a) <= (ve - b);
+    uint32_t s = a_short ? a : b;
+    const uint32_t se = a_short ? ue : ve;
+    uint32_t l = a_short ? b : a;
+    const uint32_t le = a_short ? ve : ue;
+    for (; s < se && l < le; s++) {
+        const uint32_t w = ucol[s];
+        l = lower_bound_u32(ucol, l, le, w);
+        if (l >= le) break;
+        if (ucol[l] != w) continue;
+        const uint8_t sw = state[w];
+        if (sw == 0) continue;
+        const bool smaller_frontier = (sv == 2 && v < u) || (sw == 2 && w < u);
+        if (smaller_frontier) continue;
+        if (sv == 1) atomicSub(&tri[v], 1u);
+        if (sw == 1) atomicSub(&tri[w], 1u);
+    }
+}
+
+// One wave per frontier vertex, lanes over N(u).
 template <bool FUSED>
 __global__ void k_core_update(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol, const uint8_t* __restrict__ state,
                               const uint32_t* __restrict__ frontier, CoreDev* st, uint32_t* __restrict__ tri) {
@@ -133,31 +163,7 @@ __global__ void k_core_update(const uint32_t* __restrict__ urow, const uint32_t*
     for (uint64_t i = wave0; i < F; i += nwaves) {
         const uint32_t u = frontier[i];
         const uint32_t ub = urow[u], ue = urow[u + 1];
-        for (uint32_t p = ub + lane; p < ue; p += 64) {
-            const uint32_t v = ucol[p];
-            const uint8_t sv = state[v];
-            if (sv == 0) continue;
-            uint32_t a = p + 1;                                              // N(u) after v: all > v
-            const uint32_t vb = urow[v], ve = urow[v + 1];
-            uint32_t b = upper_bound_u32(ucol, vb, ve, v);                    // N(v) after v
-            const bool a_short = (ue - a) <= (ve - b);
-            uint32_t s = a_short ? a : b;
-            const uint32_t se = a_short ? ue : ve;
-            uint32_t l = a_short ? b : a;
-            const uint32_t le = a_short ? ve : ue;
-            for (; s < se && l < le; s++) {
-                const uint32_t w = ucol[s];
-                l = lower_bound_u32(ucol, l, le, w);
-                if (l >= le) break;
-                if (ucol[l] != w) continue;
-                const uint8_t sw = state[w];
-                if (sw == 0) continue;
-                const bool smaller_frontier = (sv == 2 && v < u) || (sw == 2 && w < u);
-                if (smaller_frontier) continue;
-                if (sv == 1) atomicSub(&tri[v], 1u);
-                if (sw == 1) atomicSub(&tri[w], 1u);
-            }
-        }
+        for (uint32_t p = ub + lane; p < ue; p += 64) core_pair(urow, ucol, state, tri, u, p, ue);
     }
 }
 
@@ -246,6 +252,7 @@ extern "C" void ga_core_destroy(ga_core* c) {
     ga_free(c->tri);
     ga_free(c->frontier);
     ga_free(c->st);
+    ga_free(c->blk);
     cf_release(c->fm);
     delete c;
 }
@@ -892,6 +899,168 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     if (stats) {
         stats[0] = h->rounds;
         stats[1] = (uint32_t)h->k_prev;
+    }
+    return GA_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// many small graphs in one launch: one workgroup runs a whole incremental decomposition
+// ---------------------------------------------------------------------------------------------------------------
+// The rounds are the same level-synchronous rounds as above (k = max(k, min alive tri); frontier = alive with
+// tri <= k; subtract the frontier's triangles with the owner rule; retire), separated by workgroup barriers instead
+// of kernel boundaries, so a graph costs no launches per round and a launch carries as many graphs as it has
+// workgroups. Meant for graphs whose rounds are small (a workgroup walks the alive list every round).
+struct CoreBlockDesc {
+    const uint32_t* urow;
+    const uint32_t* ucol;
+    uint8_t* state;           // all 1 on entry
+    uint32_t* core;
+    uint32_t* tri;            // participation counts on entry
+    uint32_t* front;          // n words
+    uint32_t* alist;          // 2n words
+    const CoreDev* st;        // T of the initial count
+    uint32_t* out;            // 4 words: rounds, kmax, error, T (low 32 bits are enough: T < 2^32 is checked)
+    uint32_t n;
+};
+
+__global__ void __launch_bounds__(1024) k_core_block(const CoreBlockDesc* __restrict__ descs) {
+    const CoreBlockDesc d = descs[blockIdx.x];
+    __shared__ uint32_t s_nf;
+    __shared__ uint32_t wred[16];
+    const uint32_t tid = threadIdx.x, T = blockDim.x, lane = lane_id(), wv = tid / 64, nw = T / 64;
+    const uint32_t* __restrict__ urow = d.urow;
+    const uint32_t* __restrict__ ucol = d.ucol;
+    uint8_t* __restrict__ state = d.state;
+    uint32_t* __restrict__ tri = d.tri;
+    uint32_t* al = d.alist;
+    uint32_t* al2 = d.alist + d.n;
+    uint32_t na = d.n, nalive = d.n, rounds = 0, k_prev = 0, err = 0;
+    for (uint32_t i = tid; i < na; i += T) al[i] = i;
+    __syncthreads();
+    while (nalive > 0) {   // uniform: every thread holds the same na / nalive
+        uint32_t mn = 0xFFFFFFFFu;
+        for (uint32_t i = tid; i < na; i += T) {
+            const uint32_t v = al[i];
+            if (state[v] == 1) mn = min(mn, tri[v]);
+        }
+        mn = wave_min_u32(mn);
+        if (lane == 0) wred[wv] = mn;
+        if (tid == 0) s_nf = 0;
+        __syncthreads();
+        uint32_t bm = 0xFFFFFFFFu;
+        for (uint32_t w = 0; w < nw; w++) bm = min(bm, wred[w]);
+        const uint32_t k = max(k_prev, bm);
+        for (uint32_t base = wv * 64; base < na; base += T) {   // wave-uniform loop
+            const uint32_t i = base + lane;
+            uint32_t v = 0;
+            const bool join = i < na && state[(v = al[i])] == 1 && tri[v] <= k;
+            if (join) {
+                state[v] = 2;
+                d.core[v] = k;
+            }
+            const uint64_t mask = __ballot(join);
+            if (!mask) continue;
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+            uint32_t pos = 0;
+            if (lane == leader) pos = atomicAdd(&s_nf, (uint32_t)__popcll(mask));
+            pos = __shfl(pos, leader, 64);
+            if (join) d.front[pos + mask_rank(mask)] = v;
+        }
+        __syncthreads();
+        const uint32_t nf = s_nf;
+        if (nf == 0 || nf > nalive) { err = 1; break; }   // uniform; cannot happen: a vertex holds the minimum
+        for (uint32_t fi = wv; fi < nf; fi += nw) {
+            const uint32_t u = d.front[fi];
+            if (tri[u] == 0) continue;   // in no alive triangle: nothing to subtract
+            const uint32_t ub = urow[u], ue = urow[u + 1];
+            for (uint32_t p = ub + lane; p < ue; p += 64) core_pair(urow, ucol, state, tri, u, p, ue);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < nf; i += T) state[d.front[i]] = 0;
+        nalive -= nf;
+        k_prev = k;
+        rounds++;
+        __syncthreads();
+        if (nalive > 0 && 2 * nalive <= na) {   // the alive list has halved: compact it, in order
+            uint32_t total = 0;
+            for (uint32_t t0 = 0; t0 < na; t0 += T) {   // uniform loop
+                const uint32_t i = t0 + tid;
+                uint32_t v = 0;
+                const bool keep = i < na && state[(v = al[i])] == 1;
+                const uint64_t mask = __ballot(keep);
+                if (lane == 0) wred[wv] = (uint32_t)__popcll(mask);
+                __syncthreads();
+                uint32_t off = total, add = 0;
+                for (uint32_t w = 0; w < nw; w++) {
+                    if (w < wv) off += wred[w];
+                    add += wred[w];
+                }
+                if (keep) al2[off + mask_rank(mask)] = v;
+                total += add;
+                __syncthreads();
+            }
+            uint32_t* t = al;
+            al = al2;
+            al2 = t;
+            na = total;
+        }
+    }
+    if (tid == 0) {
+        d.out[0] = rounds;
+        d.out[1] = k_prev;
+        d.out[2] = err;
+        d.out[3] = (uint32_t)d.st->T;
+    }
+}
+
+extern "C" int ga_core_run_many(ga_ctx* ctx, ga_core* const* cores, uint32_t count, const uint64_t* T_full, uint32_t* stats, float* ms) {
+    if (!ctx || (count && (!cores || !stats))) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    if (!count) return GA_OK;
+    for (uint32_t i = 0; i < count; i++) {
+        ga_core* c = cores[i];
+        if (!c || !c->incremental || c->ctx->device != ctx->device) return ga_fail(ctx, GA_ERR_ARG, "core %u: not an incremental state of this device", i);
+        if (T_full && T_full[i] != UINT64_MAX && T_full[i] >= (1ull << 32))
+            return ga_fail(ctx, GA_ERR_ARITH, "core %u: triangle count >= 2^32: uint32 tri[] counters could overflow", i);
+        if (!c->blk) GA_TRY(ga_malloc(ctx, (void**)&c->blk, (2ull * c->g->n + 4) * 4));
+        GA_TRY(ga_tc_ensure_plan(ctx, c->g));
+    }
+    const size_t desc_bytes = ((size_t)count * sizeof(CoreBlockDesc) + 255) & ~(size_t)255;
+    const size_t need = desc_bytes + (size_t)count * 16;
+    if (need > ctx->dcore_bytes) {
+        ga_free(ctx->dcore);
+        ctx->dcore = nullptr;
+        ctx->dcore_bytes = 0;
+        GA_TRY(ga_malloc(ctx, &ctx->dcore, need * 2));
+        ctx->dcore_bytes = need * 2;
+    }
+    std::vector<CoreBlockDesc> h(count);
+    uint32_t* dout = (uint32_t*)((char*)ctx->dcore + desc_bytes);
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint32_t i = 0; i < count; i++) {
+        ga_core* c = cores[i];
+        const uint32_t n = c->g->n;
+        GA_TRY(ga_fill_u8(ctx, c->state, n, 1));
+        GA_TRY(ga_fill_u32(ctx, c->core, n, UINT32_MAX));
+        k_core_begin<false><<<cgrid(ctx, n), 256, 0, ctx->stream>>>(c->st, c->tri, n, 1);
+        GA_TRY(ga_launch_check(ctx, "k_core_begin"));
+        GA_TRY(ga_tc_launch(ctx, c->g, c->variant, &c->st->T, c->state, c->tri, nullptr, nullptr));
+        h[i] = CoreBlockDesc{c->g->urow, c->g->ucol, c->state, c->core, c->tri, c->frontier, c->blk, c->st, dout + 4 * i, n};
+    }
+    GA_HIP(ctx, hipMemcpyAsync(ctx->dcore, h.data(), (size_t)count * sizeof(CoreBlockDesc), hipMemcpyHostToDevice, ctx->stream));
+    k_core_block<<<count, 1024, 0, ctx->stream>>>((const CoreBlockDesc*)ctx->dcore);
+    GA_TRY(ga_launch_check(ctx, "k_core_block"));
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<uint32_t> out((size_t)count * 4);
+    GA_HIP(ctx, hipMemcpyAsync(out.data(), dout, (size_t)count * 16, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    for (uint32_t i = 0; i < count; i++) {
+        if (out[4 * i + 2]) return ga_fail(ctx, GA_ERR_STATE, "core %u: bad round on the device (empty frontier)", i);
+        if (T_full && T_full[i] != UINT64_MAX && (uint64_t)out[4 * i + 3] != T_full[i])
+            return ga_fail(ctx, GA_ERR_STATE, "core %u: device triangle count differs from the given count", i);
+        stats[2 * i] = out[4 * i];
+        stats[2 * i + 1] = out[4 * i + 1];
     }
     return GA_OK;
 }

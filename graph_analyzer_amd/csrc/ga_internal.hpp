@@ -25,6 +25,8 @@ struct ga_ctx {
     size_t scratch_bytes = 0;
     uint64_t* dmany = nullptr;          // grow-only per-graph counts of ga_tc_count_many
     size_t dmany_n = 0;
+    void* dcore = nullptr;              // grow-only descriptors and results of ga_core_run_many
+    size_t dcore_bytes = 0;
     std::string err;
 };
 

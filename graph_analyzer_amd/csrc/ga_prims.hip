@@ -84,6 +84,7 @@ extern "C" void ga_ctx_destroy(ga_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     ga_free(c->scratch);
     ga_free(c->dmany);
+    ga_free(c->dcore);
     ga_free(c->dsmall);
     if (c->hstat) (void)hipHostFree(c->hstat);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -14,7 +14,9 @@ Shape on one MI355X:
     readback (ga_tc_count_many, up to `count_batch` graphs);
   * stateful requests (peel, core) take one of up to `states_per_graph` state objects of their graph; states are
     created once and moved between worker streams (ga_peel_set_ctx / ga_core_set_ctx), so the steady state allocates
-    and frees no device memory.
+    and frees no device memory;
+  * core requests on small graphs (at most `core_batch_max_m` edges) are gathered like counts into one
+    ga_core_run_many launch, one workgroup per graph running all of its rounds.
 `slots=1, count_batch=1` is the plain form: one request at a time, one count per launch, in arrival order.
 """
 from __future__ import annotations
@@ -22,6 +24,7 @@ from __future__ import annotations
 import collections
 import dataclasses
 import hashlib
+import itertools
 import math
 import queue
 import threading
@@ -30,7 +33,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import TC_HASH, Context, Core, DeviceGraph, GAError, Peel
+from . import TC_HASH, Context, Core, DeviceGraph, GAError, Peel, core_many
 
 KINDS = ("count", "peel", "core")
 
@@ -103,12 +106,14 @@ class _Resident:
 
 class Server:
     def __init__(self, source: Callable[[object], np.ndarray], slots: int = 16, count_batch: int = 64,
-                 states_per_graph: int = 4, budget_bytes: Optional[int] = None, device: int = 0):
+                 states_per_graph: int = 4, budget_bytes: Optional[int] = None, device: int = 0,
+                 core_batch_max_m: int = 65536):
         if slots < 1 or count_batch < 1 or states_per_graph < 1:
             raise ValueError("slots, count_batch and states_per_graph must be >= 1")
         self.source = source
         self.slots = slots
         self.count_batch = count_batch
+        self.core_batch_max_m = core_batch_max_m if count_batch > 1 else 0
         self.states_per_graph = states_per_graph
         self.loader = Context(device)
         self.ctxs = [Context(device) for _ in range(slots)]
@@ -188,9 +193,8 @@ class Server:
     # ---- scheduling ----------------------------------------------------------------------------------------
     def _admit(self, pending: collections.deque, scan: int = 512):
         """Earliest admissible pending request (and, for a count, every other pending count up to count_batch)."""
-        for idx, req in enumerate(pending):
-            if idx >= scan:
-                break
+        # a snapshot: the deque is only changed right before returning, or restored to the same order
+        for idx, req in enumerate(list(itertools.islice(pending, scan))):
             r = self.load(req.key)
             if r is None:
                 continue
@@ -207,6 +211,24 @@ class Server:
                 for _, rq in group:
                     rq.inflight += 1
                 return ("count", group, None)
+            if req.kind == "core" and r.m <= self.core_batch_max_m:
+                # small graphs: every pending core request on a small graph with a free state goes into one
+                # ga_core_run_many launch (one workgroup per graph)
+                group, states, keep = [], [], collections.deque()
+                while pending and len(group) < self.count_batch:
+                    q = pending.popleft()
+                    rq = self.graphs.get(q.key) if q.kind == "core" else None
+                    st = self._take_state(rq, "core") if rq is not None and rq.m <= self.core_batch_max_m else None
+                    if st is not None:
+                        group.append((q, rq))
+                        states.append(st)
+                        rq.inflight += 1
+                    else:
+                        keep.append(q)
+                pending.extendleft(reversed(keep))
+                if group:
+                    return ("core_many", group, states)
+                continue
             state = self._take_state(r, req.kind)
             if state is None:
                 continue
@@ -222,6 +244,11 @@ class Server:
         if kind == "count":
             T, _ = ctx.count_many([r.dg for _, r in group], TC_HASH)
             vals = [(int(t),) for t in T]
+        elif kind == "core_many":
+            for st in state:
+                st.set_ctx(ctx)
+            res, _ = core_many(ctx, state, [r.T for _, r in group])
+            vals = [(rounds, kmax, digest(core)) for core, rounds, kmax in res]
         elif kind == "peel":
             (req, r), = group
             state.set_ctx(ctx)
@@ -284,7 +311,10 @@ class Server:
                     for _, r in group:
                         r.inflight -= 1
                         r.last = time.perf_counter()
-                    if state is not None:
+                    if kind == "core_many":
+                        for (_, r), st in zip(group, state):
+                            r.idle["core"].append(st)
+                    elif state is not None:
                         group[0][1].idle[kind].append(state)
                     if err is not None:
                         error = err

@@ -138,6 +138,12 @@ int ga_peel_run(ga_peel* p, uint64_t T_full, uint64_t num, uint64_t den, uint64_
 int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core** out);
 /* Moves an idle core state to another context (its stream); same rule as ga_peel_set_ctx. */
 int ga_core_set_ctx(ga_core* c, ga_ctx* ctx);
+/* Incremental decompositions of `count` graphs in one launch, one workgroup per graph running all its rounds
+ * (core_decomp_loop_inc rules, dsg.h:719-751); for small graphs, where per-round launches would dominate. Each state
+ * must be incremental; T_full (count entries, or NULL; UINT64_MAX = unchecked) is checked against the device count;
+ * stats gets rounds and kmax per graph; core numbers stay in each state (ga_core_download). No counterpart in the
+ * reference (it runs one graph per process). */
+int ga_core_run_many(ga_ctx* ctx, ga_core* const* cores, uint32_t count, const uint64_t* T_full, uint32_t* stats, float* ms);
 void ga_core_destroy(ga_core* c);
 int ga_core_reset_state(ga_core* c);                 /* alive/state = 1, core = UINT32_MAX */
 int ga_core_begin_round(ga_core* c);

@@ -238,3 +238,36 @@ def test_powerlaw_small_workload(ctx):
     """The 8,298-vertex / 100,762-edge power-law workload end to end."""
     from graph_analyzer_amd import gen
     check_graph(ctx, gen.powerlaw_small(), eps=[(1, 10), (1, 100), (1, 20)], tag="powerlaw_small")
+
+
+def test_core_many_small_graphs(ctx):
+    """ga_core_run_many (one workgroup per graph, all rounds in one launch) equals the oracle and the fused loop on a
+    mix of graphs: fixtures, cliques, random graphs with duplicates and loops, power-law graphs, degenerate ones."""
+    from graph_analyzer_amd import gen
+    rng = np.random.default_rng(5)
+    raws = [np.array(clique(k), np.uint32) for k in (3, 9, 40)]
+    raws += [np.zeros((0, 2), np.uint32), np.array([[2, 2]], np.uint32), np.array([[0, 1]], np.uint32)]
+    raws += [random_graph(rng, int(rng.integers(20, 2500)), float(rng.uniform(0.002, 0.1)), dup=0.2, loops=2) for _ in range(10)]
+    raws += [gen.chung_lu(int(rng.integers(2000, 30000)), int(rng.integers(20000, 200000)), 2.1, 0.0, seed=31 + i) for i in range(4)]
+    with open(GOLDEN) as f:
+        raws += [np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2) for rec in json.load(f)["graphs"][:8]]
+    graphs = [ga.DeviceGraph.from_pairs(ctx, r) for r in raws]
+    Ts = [g.count(ga.TC_HASH)[0] for g in graphs]
+    cores = [ga.Core(g, True) for g in graphs]
+    for rep in range(2):   # twice on the same states: nothing may leak from one run to the next
+        res, ms = ga.core_many(ctx, cores, Ts)
+        for raw, g, (core, rounds, kmax) in zip(raws, graphs, res):
+            og = oracle.OGraph.from_pairs(raw)
+            oc, okmax = og.core()
+            _, orounds, _ = og.core_levels()
+            assert np.array_equal(core, oc) and kmax == okmax and rounds == orounds, (g.n, g.m, rep)
+    fused = [g.core_decomposition(T_full=T) for g, T in zip(graphs, Ts)]
+    for (core, rounds, kmax), (fc, fr, fk) in zip(res, fused):
+        assert np.array_equal(core, fc) and (rounds, kmax) == (fr, fk)
+    with pytest.raises(ga.GAError) as ei:   # a wrong given count is caught on the device
+        ga.core_many(ctx, cores[:2], [Ts[0], Ts[1] + 1])
+    assert ei.value.code == ga.GA_ERR_STATE
+    for c in cores:
+        c.close()
+    for g in graphs:
+        g.close()

@@ -1,0 +1,59 @@
+"""Full-size parity: the R-MAT scale-21 graph and the 34.7M-edge power-law graph, each result bit-exact against the
+CPU oracle (OpenMP where the oracle has it). Each test runs well under a minute on a GPU host."""
+import numpy as np
+import pytest
+
+import graph_analyzer_amd as ga
+import oracle
+from graph_analyzer_amd import gen
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+_cache = {}
+
+
+def _graph(ctx, name):
+    if name not in _cache:
+        raw = gen.rmat21() if name == "rmat21" else gen.powerlaw_communities()
+        og = oracle.OGraph.from_pairs(raw)
+        _cache[name] = (raw, og, og.triangles_omp(0))
+    raw, og, T = _cache[name]
+    g = ga.DeviceGraph.from_pairs(ctx, raw)
+    r, c, s = g.download()
+    orr, oc, os_ = og.arrays()
+    assert np.array_equal(r, orr) and np.array_equal(c, oc) and np.array_equal(s, os_), f"{name}: CSR differs"
+    assert g.count(ga.TC_HASH)[0] == T and g.count(ga.TC_MERGE)[0] == T
+    return g, og, T
+
+
+@pytest.mark.parametrize("name", ["rmat21", "powerlaw_communities"])
+def test_full_size_peel(ctx, name):
+    """Batch peeling at eps 1/10 and 1/100 (the R-MAT configuration's two epsilons)."""
+    g, og, T = _graph(ctx, name)
+    for num, den in ((1, 10), (1, 100)):
+        bn, bd, br, rounds, rr = og.batch_peel(num, den, omp=True)
+        pr = g.batch_peel(num, den, T_full=T)
+        assert (pr.best_num, pr.best_den, pr.best_round, pr.rounds) == (bn, bd, br, rounds), (name, num, den)
+        assert np.array_equal(pr.removed_round, rr), (name, num, den)
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["rmat21", "powerlaw_communities"])
+def test_full_size_support_and_participation(ctx, name):
+    g, og, T = _graph(ctx, name)
+    Ts, sup = og.edge_support()
+    Tg, supg = g.edge_support(ga.TC_HASH)
+    assert Tg == Ts == T and np.array_equal(supg, sup)
+    _, tri = og.participation(omp=True)
+    Tv, trig = g.vertex_participation(None, ga.TC_HASH)
+    assert Tv == T and np.array_equal(trig, tri)
+    g.close()
+
+
+def test_full_size_core(ctx):
+    """(k,triangle)-core numbers of the 34.7M-edge graph against Alg 3 (dsg.h:116-166)."""
+    g, og, T = _graph(ctx, "powerlaw_communities")
+    core, kmax = og.core()
+    cg, rounds, kg = g.core_decomposition(T_full=T)
+    assert kg == kmax and np.array_equal(cg, core)
+    g.close()

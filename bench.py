@@ -133,12 +133,28 @@ def run_batch(ctx, count, steps, tune):
         if not np.array_equal(T, T0):
             raise SystemExit("MISMATCH: batched per-graph counts changed between steps")
         times_b.append(ms)
-    per = [per_default]   # and with the batch's tuning (inherited by every graph); the faster one is the baseline
-    for _ in range(2):
-        Tp, ms = b.count_all(per_graph=True)
-        per.append(ms)
-    if not (np.array_equal(Tp, T0) and np.array_equal(Tp_def, T0)):
+    # Per-graph baseline: the fastest of the library's default tuning, the batch's tuning (inherited by every graph),
+    # and each of those with the flat kernel in small chunks (one launch for all light rows of a small graph, the
+    # per-graph form's best shapes on this mix)
+    tuned = b.get_tuning()
+    per_by = {"default": per_default}
+    cands = [("batch", tuned)]
+    for name, base_t in (("default", ga.default_tuning()), ("batch", tuned)):
+        for fc, fb in ((512, 64), (1024, 128)):
+            t2 = ga.TcTuning.from_dict(base_t.as_dict())
+            t2.flat_chunk, t2.flat_block = fc, fb
+            cands.append((f"{name}+flat{fc}/{fb}", t2))
+    for name, t2 in cands:
+        b.set_tuning(t2)
+        Tp, _ = b.count_all(per_graph=True)
+        if not np.array_equal(Tp, T0):
+            raise SystemExit(f"MISMATCH: per-graph launches ({name}) disagree with the batched counts")
+        per_by[name] = min(b.count_all(per_graph=True)[1] for _ in range(2))
+    b.set_tuning(tuned)
+    if not np.array_equal(Tp_def, T0):
         raise SystemExit("MISMATCH: per-graph launches disagree with the batched counts")
+    per = list(per_by.values())
+    log("batch: per-graph count ms " + ", ".join(f"{k} {v:.1f}" for k, v in per_by.items()))
     st, rr, pms = b.peel_all(1, 10, T_full=T0)
     st2, rr2, pms2 = b.peel_all(1, 10, T_full=T0, per_graph=True)
     if not (np.array_equal(st, st2) and np.array_equal(rr, rr2)):
@@ -160,7 +176,7 @@ def run_batch(ctx, count, steps, tune):
            "triangles": int(T0.sum()), "tuning": tune_info,
            "count_graphs_per_s": count / (t_b * 1e-3), "count_ms": t_b,
            "per_graph_count_graphs_per_s": count / (t_p * 1e-3), "per_graph_count_ms": t_p, "count_speedup": t_p / t_b,
-           "per_graph_count_ms_default_tuning": per_default, "per_graph_count_ms_batch_tuning": float(min(per[1:])),
+           "per_graph_count_ms_by_tuning": per_by, "per_graph_count_baseline": min(per_by, key=per_by.get),
            "count_edges_per_s": b.total_m / (t_b * 1e-3),
            "peel_eps": "1/10", "peel_graphs_per_s": count / (pms * 1e-3), "peel_ms": pms,
            "per_graph_peel_graphs_per_s": count / (pms2 * 1e-3), "per_graph_peel_ms": pms2, "peel_speedup": pms2 / pms,

@@ -48,12 +48,24 @@ class GAError(RuntimeError):
 class TcTuning(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_int32 * 5), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
                 ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32),
-                ("flat_chunk", ctypes.c_int32), ("flat_block", ctypes.c_int32)]
+                ("flat_chunk", ctypes.c_int32), ("flat_block", ctypes.c_int32),
+                ("item_order", ctypes.c_int32)]
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k, v in d.items():
+            if k == "lanes":
+                t.lanes[:] = v
+            else:
+                setattr(t, k, v)
+        return t
 
     def as_dict(self):
         return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "batches": self.batches,
                 "load_shift": self.load_shift, "hash_cap": self.hash_cap, "flat_chunk": self.flat_chunk,
-                "flat_block": self.flat_block}
+                "flat_block": self.flat_block,
+                "item_order": self.item_order}
 
 
 _lib = None
@@ -93,6 +105,7 @@ def lib() -> ctypes.CDLL:
             "ga_tc_prepare": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
             "ga_tc_set_tuning": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning)]),
+            "ga_tc_default_tuning": (ctypes.c_int, [ctypes.POINTER(TcTuning)]),
             "ga_tc_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_plan_info": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
             "ga_tc_vertex": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u8p, _u32p, _u64p]),
@@ -499,6 +512,13 @@ class Core:
             pass
 
 
+def default_tuning() -> TcTuning:
+    """The library's default count tuning."""
+    t = TcTuning()
+    lib().ga_tc_default_tuning(ctypes.byref(t))
+    return t
+
+
 def variant_name(v: int) -> str:
     return lib().ga_tc_variant_name(v).decode()
 
@@ -515,6 +535,8 @@ def _bind_batch(L):
         "ga_batch_count": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_count_per_graph": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
+        "ga_batch_set_tuning": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning)]),
+        "ga_batch_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
         "ga_batch_peel": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_peel_per_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p,
                                                    ctypes.POINTER(ctypes.c_float)]),
@@ -596,6 +618,15 @@ class Batch:
         ms = ctypes.c_float()
         self.ctx.check(lib().ga_batch_tune(self.ctx.handle, self._h, ctypes.byref(t), ctypes.byref(ms)), "batch_tune")
         return t, ms.value
+
+    def get_tuning(self) -> TcTuning:
+        t = TcTuning()
+        lib().ga_batch_get_tuning(self._h, ctypes.byref(t))
+        return t
+
+    def set_tuning(self, t: TcTuning):
+        """Count tuning of the union and of the per-graph baseline."""
+        self.ctx.check(lib().ga_batch_set_tuning(self.ctx.handle, self._h, ctypes.byref(t)), "batch_set_tuning")
 
     def peel_all(self, num=1, den=10, T_full=None, per_graph: bool = False, removed_round: bool = True):
         stats = np.zeros(4 * self.count, np.uint64)

@@ -350,6 +350,9 @@ extern "C" int ga_batch_create(ga_ctx* ctx, uint32_t count, const uint64_t* pair
         hipMemcpyAsync(b->gmax.data(), dgmax.p, (size_t)count * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         return fail_b(ga_fail(ctx, GA_ERR_HIP, "batch offsets download failed"));
+    b->u->plan.group = b->gof;   // shape-ordered items stay graph-major (cache locality per graph)
+    b->u->plan.group_bits = 1;
+    while (b->u->plan.group_bits < 31 && (1u << b->u->plan.group_bits) < count) b->u->plan.group_bits++;
     if ((rc = ga_tc_ensure_plan(ctx, b->u))) return fail_b(rc);
     if (build_ms) *build_ms = (float)(wall_ms() - t0);
     *out = b;
@@ -420,9 +423,24 @@ extern "C" int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms) 
     return GA_OK;
 }
 
+extern "C" int ga_batch_set_tuning(ga_ctx* ctx, ga_batch* b, const ga_tc_tuning* t) {
+    if (!ctx || !b || !t) return GA_ERR_ARG;
+    GA_TRY(ga_tc_set_tuning(ctx, b->u, t));
+    for (ga_graph* g : b->slices) GA_TRY(ga_tc_set_tuning(ctx, g, t));
+    return GA_OK;
+}
+
+extern "C" int ga_batch_get_tuning(const ga_batch* b, ga_tc_tuning* out) {
+    if (!b || !out) return GA_ERR_ARG;
+    return ga_tc_get_tuning(b->u, out);
+}
+
 extern "C" int ga_batch_tune(ga_ctx* ctx, ga_batch* b, ga_tc_tuning* best, float* best_ms) {
     if (!ctx || !b) return GA_ERR_ARG;
-    GA_TRY(ga_tc_tune(ctx, b->u, best, best_ms));
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf cnt;
+    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count + 1) * 8));
+    GA_TRY(ga_tc_tune_counts(ctx, b->u, b->gof, (unsigned long long*)cnt.p, b->count, best, best_ms));   // what ga_batch_count runs
     for (ga_graph* g : b->slices) GA_TRY(ga_tc_set_tuning(ctx, g, &b->u->plan.tune));
     return GA_OK;
 }

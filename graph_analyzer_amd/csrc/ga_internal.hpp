@@ -49,6 +49,8 @@ struct TcPlan {
     uint32_t nch = 0, chunk_used = 0;    // flat chunks and the chunk size they were cut with
     unsigned long long utot = 0;         // flat units of all light entries
     ga_tc_tuning tune = {};             // block == 0: defaults not applied yet
+    const uint32_t* group = nullptr;     // locality group of each vertex (not owned; a disjoint union's graph ids): items
+    uint32_t group_bits = 0;             //   are shape-ordered within each group, groups ascending
     uint32_t* perm = nullptr;            // GA_TC_SORTED work order (ga_graph_edge_order), lazily
 };
 
@@ -99,6 +101,10 @@ int ga_build_from_device_pairs(ga_ctx* ctx, const uint32_t* dpairs, uint64_t k, 
 int ga_graph_ensure_undirected(ga_ctx* ctx, ga_graph* g);
 int ga_graph_ensure_perm(ga_ctx* ctx, ga_graph* g);
 int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g);
+// ga_tc_tune over the count form a caller runs: plain (gcnt NULL) or per graph of a disjoint union (gof, gcnt of
+// ngraphs + 1 device words), as ga_batch_count launches it.
+int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, ga_tc_tuning* best,
+                      float* best_ms);
 // One count of `variant` into *d_total (zeroed by the caller). Optional: alive mask, per-vertex tri, per-edge sup, a device
 // `done` flag that turns the launch into a no-op, and per-graph counts gcnt[gof[h]] for a disjoint union of graphs.
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* d_alive, uint32_t* d_tri,

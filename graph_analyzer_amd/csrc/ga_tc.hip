@@ -349,6 +349,8 @@ struct FlatArgs {
     uint32_t nch, nlight, utot;
     unsigned long long* total;
     const uint32_t* done;
+    const uint32_t* gof;           // GRAPH: graph of each vertex and per-graph counts (a disjoint union of graphs)
+    unsigned long long* gcnt;
 };
 
 static size_t flat_shmem(uint32_t chunk) {
@@ -364,7 +366,7 @@ __device__ __forceinline__ void flat_insert(uint32_t* tab, uint32_t tbits, uint3
     }
 }
 
-template <int UPT>
+template <int UPT, bool GRAPH>
 __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
     static_assert(UPT == 4 || UPT == 8 || UPT == 16, "4, 8 or 16 units per thread");
     constexpr int PF = 2;   // entry records per thread fetched one chunk ahead
@@ -495,6 +497,14 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
         for (int k = 0; k < UPT; k++)
             if (meta[k] & 0x80000000u) flat_insert(tab + (meta[k] & 0xFFFFu), (meta[k] >> 16) & 31u, x[k]);
         if (c + gridDim.x < a.nch) fetch(c + gridDim.x);   // lands while this chunk probes
+        // GRAPH: light entries are in vertex order and each graph holds a contiguous id range, so a chunk lies in one
+        // graph except where a graph ends inside it; those chunks credit each hit to its entry's graph
+        uint32_t g0 = 0, cc = 0;
+        bool split = false;
+        if (GRAPH) {
+            g0 = a.gof[a.wh[j0]];
+            split = a.gof[a.wh[j0 + ne - 1]] != g0;
+        }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < UPT; k++) {
@@ -514,8 +524,21 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
                 }
             }
             local += hit ? 1u : 0u;
+            if (GRAPH) {
+                if (!split) cc += hit ? 1u : 0u;
+                else if (hit) atomicAdd(&a.gcnt[a.gof[a.wh[j0 + own[k * T + tid]]]], 1ull);
+            }
+        }
+        if (GRAPH) {   // one atomic per chunk: wave sums in wtot (free since the prefix sum), thread 0 adds them
+            cc = (uint32_t)wave_sum_u64(cc);
+            if (lane == 0) wtot[wv] = cc;
         }
         __syncthreads();
+        if (GRAPH && tid == 0 && !split) {   // wtot is rewritten only after the next chunk's first two barriers
+            unsigned long long s = 0;
+            for (uint32_t w = 0; w < T / 64; w++) s += wtot[w];
+            if (s) atomicAdd(&a.gcnt[g0], s);
+        }
     }
     local = wave_sum_u64(local);
     if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
@@ -712,6 +735,35 @@ __global__ void k_plan_items(const uint32_t* __restrict__ flag, const uint32_t* 
     }
 }
 
+// Shape key of each item: bucket (kept major) | locality group of h (a union's graph id; none for one graph) |
+// entries | longest probe list (the item's first) | d+(h), the last three capped at 10 bits. Consecutive items go to
+// the lane groups of one wavefront; equal shapes give those groups loops of equal trip counts (groups of one
+// wavefront otherwise wait, masked off, for the longest of their loops). Group-major order keeps the items in flight
+// on a few graphs, whose lists then stay in cache.
+__global__ void k_plan_item_shape(const uint32_t* __restrict__ items, uint32_t nall, const uint32_t* __restrict__ rowptr,
+                                  const uint32_t* __restrict__ wlen, uint32_t cap, const uint32_t* __restrict__ group, uint32_t gbits,
+                                  unsigned long long* __restrict__ key, uint32_t* __restrict__ val) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) {
+        const uint32_t h = items[3ull * i], b = items[3ull * i + 1], e = items[3ull * i + 2];
+        const uint32_t dh = rowptr[h + 1] - rowptr[h];
+        const unsigned long long bk = bucket_of(dh, cap);
+        const unsigned long long gr = group ? group[h] : 0u;
+        key[i] = (bk << (30 + gbits)) | (gr << 30) | ((unsigned long long)min(e - b, 1023u) << 20) |
+                 ((unsigned long long)min(wlen[b], 1023u) << 10) | min(dh, 1023u);
+        val[i] = i;
+    }
+}
+
+__global__ void k_plan_item_permute(const uint32_t* __restrict__ items, const uint32_t* __restrict__ order, uint32_t nall,
+                                    uint32_t* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) {
+        const uint32_t s = order[i];
+        out[3ull * i] = items[3ull * s];
+        out[3ull * i + 1] = items[3ull * s + 1];
+        out[3ull * i + 2] = items[3ull * s + 2];
+    }
+}
+
 __global__ void k_plan_rank(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ fpos, uint32_t nwork, uint32_t* __restrict__ starts) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x)
         if (flag[j]) starts[fpos[j]] = j;
@@ -736,6 +788,7 @@ ga_tc_tuning default_tuning() {
     t.hash_cap = 4096;
     t.flat_chunk = 0;     // the flat kernel is a tuner option (it wins on some degree mixes, not on all)
     t.flat_block = 512;
+    t.item_order = 1;
     return t;
 }
 
@@ -744,7 +797,7 @@ bool tuning_valid(const ga_tc_tuning& t) {
         if (t.lanes[i] != 4 && t.lanes[i] != 8 && t.lanes[i] != 16 && t.lanes[i] != 32 && t.lanes[i] != 64) return false;
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
-           t.flat_block <= 1024 && t.flat_block % 64 == 0 &&
+           t.flat_block <= 1024 && t.flat_block % 64 == 0 && (t.item_order == 0 || t.item_order == 1) &&
            (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
                                   (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
                                    t.flat_chunk / t.flat_block == 16)));
@@ -913,6 +966,28 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     k_plan_items<<<nblk, 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, span, cap, boff, P.items);
     GA_TRY(ga_launch_check(ctx, "k_plan_items"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (P.tune.item_order && nall > 1) {   // shape order within each bucket
+        DevBuf k1, k2, v1, v2, it2;
+        GA_TRY(ga_malloc(ctx, &k1.p, (size_t)nall * 8));
+        GA_TRY(ga_malloc(ctx, &k2.p, (size_t)nall * 8));
+        GA_TRY(ga_malloc(ctx, &v1.p, (size_t)nall * 4));
+        GA_TRY(ga_malloc(ctx, &v2.p, (size_t)nall * 4));
+        GA_TRY(ga_malloc(ctx, &it2.p, (size_t)nall * 12));
+        const uint32_t gbits = P.group ? P.group_bits : 0u;
+        k_plan_item_shape<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, nall, g->rowptr, P.wlen, cap, P.group, gbits,
+                                                                              (unsigned long long*)k1.p, (uint32_t*)v1.p);
+        GA_TRY(ga_launch_check(ctx, "k_plan_item_shape"));
+        unsigned long long* ks = nullptr;
+        uint32_t* vs = nullptr;
+        GA_TRY(ga_sort_pairs_u64(ctx, (unsigned long long*)k1.p, (unsigned long long*)k2.p, (uint32_t*)v1.p, (uint32_t*)v2.p, nall,
+                                 33 + (int)gbits, &ks, &vs));
+        k_plan_item_permute<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, vs, nall, (uint32_t*)it2.p);
+        GA_TRY(ga_launch_check(ctx, "k_plan_item_permute"));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        uint32_t* row_order = P.items;   // freed with it2
+        P.items = (uint32_t*)it2.p;
+        it2.p = row_order;
+    }
     for (int b = 0; b < NHASH; b++) P.nitems[b] = cnt[b];
     P.nhuge = cnt[NHASH];
     P.ready = true;
@@ -960,7 +1035,8 @@ static bool flat_on(const ga_graph* g) {
     return P.nch && P.chunk_used && P.tune.hash_cap >= (int)FLAT_MAX;
 }
 
-static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, const uint32_t* done) {
+static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, const uint32_t* done, const uint32_t* gof,
+                       unsigned long long* gcnt) {
     const TcPlan& P = g->plan;
     FlatArgs f{};
     f.rowptr = g->rowptr;
@@ -974,15 +1050,22 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, cons
     f.utot = (uint32_t)P.utot;
     f.total = total;
     f.done = done;
+    f.gof = gof;
+    f.gcnt = gcnt;
     const size_t shmem = flat_shmem(P.chunk_used);
     if (shmem > ctx->lds_max) return ga_fail(ctx, GA_ERR_ARG, "flat chunk %u needs %zu bytes of LDS", P.chunk_used, shmem);
     const uint32_t block = (uint32_t)P.tune.flat_block;
     const uint32_t grid = std::min<uint32_t>(P.nch, (uint32_t)std::max(ctx->cus, 1) * 16u);
-    switch (P.chunk_used / block) {
-        case 4: k_tc_flat<4><<<grid, block, shmem, ctx->stream>>>(f); break;
-        case 8: k_tc_flat<8><<<grid, block, shmem, ctx->stream>>>(f); break;
-        case 16: k_tc_flat<16><<<grid, block, shmem, ctx->stream>>>(f); break;
-        default: return ga_fail(ctx, GA_ERR_ARG, "flat chunk / block must be 4, 8 or 16");
+    const int upt = (int)(P.chunk_used / block);
+    if (upt != 4 && upt != 8 && upt != 16) return ga_fail(ctx, GA_ERR_ARG, "flat chunk / block must be 4, 8 or 16");
+    if (gcnt) {
+        if (upt == 4) k_tc_flat<4, true><<<grid, block, shmem, ctx->stream>>>(f);
+        else if (upt == 8) k_tc_flat<8, true><<<grid, block, shmem, ctx->stream>>>(f);
+        else k_tc_flat<16, true><<<grid, block, shmem, ctx->stream>>>(f);
+    } else {
+        if (upt == 4) k_tc_flat<4, false><<<grid, block, shmem, ctx->stream>>>(f);
+        else if (upt == 8) k_tc_flat<8, false><<<grid, block, shmem, ctx->stream>>>(f);
+        else k_tc_flat<16, false><<<grid, block, shmem, ctx->stream>>>(f);
     }
     return ga_launch_check(ctx, "k_tc_flat");
 }
@@ -1046,9 +1129,9 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         a.done = done;
         a.gof = gof;
         a.gcnt = gcnt;
-        if (flags == 0 && flat_on(g)) {
-            GA_TRY(launch_flat(ctx, g, total, done));
-            return launch_hash<0>(ctx, g, a, 3);
+        if ((flags == 0 || flags == F_GRAPH) && flat_on(g)) {
+            GA_TRY(launch_flat(ctx, g, total, done, gof, gcnt));
+            return flags ? launch_hash<F_GRAPH>(ctx, g, a, 3) : launch_hash<0>(ctx, g, a, 3);
         }
         switch (flags) {
             case 0: return launch_hash<0>(ctx, g, a);
@@ -1171,6 +1254,12 @@ extern "C" int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32
     return GA_OK;
 }
 
+extern "C" int ga_tc_default_tuning(ga_tc_tuning* out) {
+    if (!out) return GA_ERR_ARG;
+    *out = default_tuning();
+    return GA_OK;
+}
+
 extern "C" int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out) {
     if (!g || !out) return GA_ERR_ARG;
     *out = g->plan.tune.block ? g->plan.tune : default_tuning();
@@ -1183,21 +1272,42 @@ extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t)
     if (t->flat_chunk && flat_shmem((uint32_t)t->flat_chunk) > ctx->lds_max)
         return ga_fail(ctx, GA_ERR_ARG, "flat chunk %d does not fit the workgroup's LDS", t->flat_chunk);
     bool replan = !g->plan.tune.block || g->plan.tune.work_chunk != t->work_chunk || g->plan.tune.hash_cap != t->hash_cap ||
-                  g->plan.tune.batches != t->batches;
+                  g->plan.tune.batches != t->batches || g->plan.tune.item_order != t->item_order;
     for (int b = 0; b < NHASH; b++) replan = replan || g->plan.tune.lanes[b] != t->lanes[b];
     g->plan.tune = *t;
     if (replan) g->plan.ready = false;
     return GA_OK;
 }
 
+// One timed hash count: plain, or per graph of a disjoint union (gof: graph of each vertex; gcnt: ngraphs + 1 device
+// words, the last one the total) as ga_batch_count runs it.
+static int timed_count(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, uint64_t* T, float* ms) {
+    if (!gcnt) return ga_tc_count(ctx, g, GA_TC_HASH, T, ms);
+    GA_TRY(ga_tc_ensure_plan(ctx, g));
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)gcnt, (size_t)ngraphs + 1, 0));
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, gcnt + ngraphs, nullptr, nullptr, nullptr, nullptr, gof, gcnt));
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, gcnt + ngraphs, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *T = ctx->hstat[0];
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return GA_OK;
+}
+
 extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* best_ms) {
+    return ga_tc_tune_counts(ctx, g, nullptr, nullptr, 0, best, best_ms);
+}
+
+int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, ga_tc_tuning* best,
+                      float* best_ms) {
     if (!ctx || !g) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
     ga_tc_tuning base = g->plan.tune.block ? g->plan.tune : default_tuning();
     GA_TRY(ga_tc_set_tuning(ctx, g, &base));
     uint64_t T0 = 0;
     float ms = 0;
-    GA_TRY(ga_tc_count(ctx, g, GA_TC_HASH, &T0, &ms));
+    GA_TRY(timed_count(ctx, g, gof, gcnt, ngraphs, &T0, &ms));
     ga_tc_tuning bt = base;
     float bms = 1e30f;
     auto trial = [&](const ga_tc_tuning& t) -> int {
@@ -1206,7 +1316,7 @@ extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* b
         for (int r = 0; r < 3; r++) {
             uint64_t T = 0;
             float x = 0;
-            GA_TRY(ga_tc_count(ctx, g, GA_TC_HASH, &T, &x));
+            GA_TRY(timed_count(ctx, g, gof, gcnt, ngraphs, &T, &x));
             if (T != T0) return ga_fail(ctx, GA_ERR_STATE, "tuning trial changed the count (%llu vs %llu)", (unsigned long long)T,
                                         (unsigned long long)T0);
             best_here = std::min(best_here, x);
@@ -1214,8 +1324,13 @@ extern "C" int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* b
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
-    // coordinate search: lanes per bucket, then block, load shift, batches and work chunk
+    // coordinate search: item order, lanes per bucket, then block, load shift, batches and work chunk
     GA_TRY(trial(base));
+    {
+        ga_tc_tuning t = bt;
+        t.item_order = !bt.item_order;
+        GA_TRY(trial(t));
+    }
     const int lane_opts[5] = {4, 8, 16, 32, 64};
     for (int b = 0; b < NHASH; b++) {
         if (!g->plan.nitems[b]) continue;

@@ -102,8 +102,12 @@ typedef struct ga_tc_tuning {
                               this many units (table slots, one per work entry, probe elements); 0 = lane groups only.
                               A multiple of flat_block in [flat_block, 8192]; needs hash_cap >= 128 */
     int32_t flat_block;    /* threads per workgroup of the flat kernel (64..1024, multiple of 64) */
+    int32_t item_order;    /* 1: work items of a bucket ordered by shape (entries, longest probe list, d+(h)), so the
+                              lane groups of one wavefront run loops of equal length; 0: in row order */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
+/* The library's default tuning (what a graph uses until ga_tc_set_tuning or ga_tc_tune). */
+int ga_tc_default_tuning(ga_tc_tuning* out);
 int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t);
 /* Count plan statistics: work entries (edges with min(d+) > 0), items per bucket (5 hashed + 1 binary-search). */
 int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32_t* nitems6);
@@ -175,6 +179,9 @@ int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
 int ga_batch_count_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
 /* Autotunes the union's count (ga_tc_tune on the resident union); the per-graph baseline inherits the result. */
 int ga_batch_tune(ga_ctx* ctx, ga_batch* b, ga_tc_tuning* best, float* best_ms);
+/* Sets the union's count tuning; the per-graph baseline uses the same record (ga_tc_set_tuning on each graph). */
+int ga_batch_set_tuning(ga_ctx* ctx, ga_batch* b, const ga_tc_tuning* t);
+int ga_batch_get_tuning(const ga_batch* b, ga_tc_tuning* out);
 /* Batch peeling of every graph with shared launches per round. stats: 4 per graph {best_num, best_den, best_round,
  * rounds}; removed_round: total_n entries, graph i at its vertex offset (or NULL); T_full: per-graph counts for the
  * round-0 check (or NULL). */

@@ -82,3 +82,26 @@ def test_batch_peel_errors(ctx):
         b.peel_all(1, 10, T_full=np.array([20, 11], np.uint64))   # graph 1 has 10 triangles
     assert ei.value.code == ga.GA_ERR_STATE and "graph 1" in str(ei.value)
     b.close()
+
+
+def test_batch_tunings_flat_per_graph(ctx):
+    """Per-graph counts of the flat kernel (chunks that straddle graph boundaries credit each hit to its own graph) and
+    of every lane-group shape equal the oracle; the batch tuner times the per-graph launch and keeps results exact."""
+    graphs = mix(64, 11, nmax=1500)
+    ref = np.array([oracle.OGraph.from_pairs(p).triangles() for p in graphs], np.uint64)
+    b = ga.Batch(ctx, graphs)
+    for lanes, fc, fb, order in (([8, 16, 32, 64, 64], 256, 64, 1), ([4, 8, 32, 32, 64], 1024, 256, 0),
+                                 ([4, 4, 16, 64, 64], 2048, 256, 1), ([8, 8, 64, 64, 64], 4096, 1024, 1),
+                                 ([16, 32, 64, 64, 64], 0, 512, 0)):
+        t = ga.TcTuning()
+        t.lanes[:] = lanes
+        t.block, t.work_chunk, t.batches, t.load_shift, t.hash_cap = 256, 2048, 4, 2, 4096
+        t.flat_chunk, t.flat_block, t.item_order = fc, fb, order
+        b.set_tuning(t)
+        T, _ = b.count_all()
+        assert np.array_equal(T, ref), (lanes, fc, fb, order)
+        Tp, _ = b.count_all(per_graph=True)
+        assert np.array_equal(Tp, ref), (lanes, fc, fb, order)
+    best, ms = b.tune()
+    assert ms > 0 and np.array_equal(b.count_all()[0], ref)
+    b.close()

@@ -164,8 +164,8 @@ def test_hub_graphs(ctx):
 
 
 def test_tuning_shapes(ctx):
-    """Every lane-group width, block size, load factor, work chunk and flat chunk/block gives the same exact count;
-    the tuner too."""
+    """Every lane-group width, block size, load factor, work chunk, flat chunk/block and item order gives the same exact
+    count; the tuner too."""
     from graph_analyzer_amd import gen
     raw = gen.chung_lu(30000, 300000, 2.05, 0.0, seed=4)
     og = oracle.OGraph.from_pairs(raw)
@@ -182,12 +182,17 @@ def test_tuning_shapes(ctx):
                 t.lanes[:] = lanes
                 t.block, t.work_chunk, t.load_shift, t.batches = block, chunk, shift, batches
                 t.flat_chunk, t.flat_block = flats[(li + block) % len(flats)]
+                t.item_order = (li + shift) % 2
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)
                     assert g.count(ga.TC_HASH)[0] == T, t.as_dict()
-        assert np.array_equal(g.vertex_participation()[1], tri_ref), lanes
-        assert np.array_equal(g.edge_support()[1], sup_ref), lanes
+        for order in (0, 1):
+            t = g.get_tuning()
+            t.item_order = order
+            g.set_tuning(t)
+            assert np.array_equal(g.vertex_participation()[1], tri_ref), (lanes, order)
+            assert np.array_equal(g.edge_support()[1], sup_ref), (lanes, order)
     for fc, fb in flats + ((512, 128), (8192, 512), (1024, 64), (16384, 1024)):
         t = g.get_tuning()
         t.hash_cap, t.flat_chunk, t.flat_block = 4096, fc, fb

@@ -35,7 +35,7 @@ struct TcPlan {
     bool ready = false;
     uint32_t nitems[5] = {0, 0, 0, 0, 0};   // work items per hashed bucket
     uint32_t nhuge = 0;                  // work items whose d+(h) is too large for an LDS table (binary search)
-    uint32_t* items = nullptr;           // 3 words per item (h, begin, end) into wcol, bucket-major
+    uint32_t* items = nullptr;           // 4 words per item {h, begin, rowptr[h], entries | d+(h) << 17}, bucket-major
     uint32_t* wcol = nullptr;            // probe-side vertex per work entry (rows of the hashed vertex, longest first)
     uint32_t* wedge = nullptr;           // oriented edge id of each work entry
     uint32_t* wps = nullptr;             // rowptr[p] of each work entry (start of the probe list)
@@ -49,6 +49,7 @@ struct TcPlan {
     uint32_t nch = 0, chunk_used = 0;    // flat chunks and the chunk size they were cut with
     unsigned long long utot = 0;         // flat units of all light entries
     ga_tc_tuning tune = {};             // block == 0: defaults not applied yet
+    ga_tc_tuning built = {};            // the tuning the current arrays were built with (plan parameters only matter)
     const uint32_t* group = nullptr;     // locality group of each vertex (not owned; a disjoint union's graph ids): items
     uint32_t group_bits = 0;             //   are shape-ordered within each group, groups ascending
     uint32_t* perm = nullptr;            // GA_TC_SORTED work order (ga_graph_edge_order), lazily

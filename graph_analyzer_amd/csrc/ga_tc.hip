@@ -118,6 +118,12 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t wave_mask) {
     return (wave_mask >> goff) & ((1ull << G) - 1ull);
 }
 
+// Work item record (16 bytes): {h, first work entry, rowptr[h], entries | d+(h) << ITEM_DSHIFT}. Entries per item are
+// at most lanes x batches <= 65536; d+(h) is exact up to 32767 (every hashed bucket; the binary-search bucket reads
+// rowptr instead).
+#define ITEM_DSHIFT 17u
+#define ITEM_NMASK 0x1FFFFu
+
 struct TcArgs {
     const uint32_t* items;
     uint32_t nitems;
@@ -155,9 +161,20 @@ __global__ void k_tc_hash(const TcArgs a) {
     uint32_t* stg = tab + tab_words;         // batch staging: ps[G] | len[G] | p[G] | work index[G]
     uint64_t local = 0;
     for (uint32_t it = blockIdx.x * gpb + gib; it < a.nitems; it += gridDim.x * gpb) {
-        const uint32_t h = a.items[3 * it], b = a.items[3 * it + 1], e = a.items[3 * it + 2];
+        const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
+        const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK);
         if ((FLAGS & F_ALIVE) && !alive[h]) continue;
-        const uint32_t hs = rowptr[h], dh = rowptr[h + 1] - hs;
+        const uint32_t dh = HASHED ? im.w >> ITEM_DSHIFT : rowptr[h + 1] - hs;
+        // batch metadata of the first batch, issued before the table is built (later batches: one batch ahead)
+        uint32_t n_p = 0, n_ps = 0, n_len = 0;
+        {
+            const uint32_t j = b + sub;
+            if (j < e) {
+                n_ps = a.wps[j];
+                n_len = a.wlen[j];
+                if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[j];
+            }
+        }
         uint32_t tbits = 0, tsz = 1;
         if (HASHED) {
             // ceil(log2(dh)) + shift, at least 2: never more than the bucket's slots (bucket_slots uses the same rule)
@@ -185,16 +202,6 @@ __global__ void k_tc_hash(const TcArgs a) {
             group_sync();
         }
         uint32_t hcount = 0;
-        // batch metadata of the first batch; later batches are fetched one batch ahead
-        uint32_t n_p = 0, n_ps = 0, n_len = 0;
-        {
-            const uint32_t j = b + sub;
-            if (j < e) {
-                n_ps = a.wps[j];
-                n_len = a.wlen[j];
-                if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[j];
-            }
-        }
         for (uint32_t base = b; base < e; base += G) {
             const uint32_t j = base + sub;
             const uint32_t p = n_p, ps = n_ps;
@@ -721,9 +728,9 @@ __global__ void k_plan_items(const uint32_t* __restrict__ flag, const uint32_t* 
             uint32_t pos = run[bk] + rank;
             for (uint32_t w = 0; w < wv; w++) pos += wc[w][bk];
             const uint32_t r = fpos[j];
-            items[3ull * pos] = h;
-            items[3ull * pos + 1] = (uint32_t)j;
-            items[3ull * pos + 2] = r + 1 < nall ? starts_by_rank[r + 1] : nwork;
+            const uint32_t e = r + 1 < nall ? starts_by_rank[r + 1] : nwork;
+            const uint32_t hs = rowptr[h], dh = rowptr[h + 1] - hs;
+            ((uint4*)items)[pos] = make_uint4(h, (uint32_t)j, hs, (e - (uint32_t)j) | (min(dh, 32767u) << ITEM_DSHIFT));
         }
         __syncthreads();
         if (threadIdx.x <= NHASH) {
@@ -744,7 +751,8 @@ __global__ void k_plan_item_shape(const uint32_t* __restrict__ items, uint32_t n
                                   const uint32_t* __restrict__ wlen, uint32_t cap, const uint32_t* __restrict__ group, uint32_t gbits,
                                   unsigned long long* __restrict__ key, uint32_t* __restrict__ val) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) {
-        const uint32_t h = items[3ull * i], b = items[3ull * i + 1], e = items[3ull * i + 2];
+        const uint4 im = ((const uint4*)items)[i];
+        const uint32_t h = im.x, b = im.y, e = im.y + (im.w & ITEM_NMASK);
         const uint32_t dh = rowptr[h + 1] - rowptr[h];
         const unsigned long long bk = bucket_of(dh, cap);
         const unsigned long long gr = group ? group[h] : 0u;
@@ -757,10 +765,7 @@ __global__ void k_plan_item_shape(const uint32_t* __restrict__ items, uint32_t n
 __global__ void k_plan_item_permute(const uint32_t* __restrict__ items, const uint32_t* __restrict__ order, uint32_t nall,
                                     uint32_t* __restrict__ out) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) {
-        const uint32_t s = order[i];
-        out[3ull * i] = items[3ull * s];
-        out[3ull * i + 1] = items[3ull * s + 1];
-        out[3ull * i + 2] = items[3ull * s + 2];
+        ((uint4*)out)[i] = ((const uint4*)items)[order[i]];
     }
 }
 
@@ -912,7 +917,8 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     P.nwork = nwork;
     P.nlight = nlight;
     if (nwork == 0) {
-        GA_TRY(ga_malloc(ctx, (void**)&P.items, 12));
+        GA_TRY(ga_malloc(ctx, (void**)&P.items, 16));
+        P.built = P.tune;
         P.ready = true;
         return GA_OK;
     }
@@ -960,19 +966,20 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     const uint32_t nall = hc[8];
     if (hc[NHASH + 1] != nall) return ga_fail(ctx, GA_ERR_STATE, "count plan: %u items bucketed, %u expected", hc[NHASH + 1], nall);
     GA_TRY(ga_malloc(ctx, &starts_b.p, (size_t)nall * 4));
-    GA_TRY(ga_malloc(ctx, (void**)&P.items, (size_t)nall * 12));
+    GA_TRY(ga_malloc(ctx, (void**)&P.items, (size_t)nall * 16));
     k_plan_rank<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, fpos, nwork, (uint32_t*)starts_b.p);
     GA_TRY(ga_launch_check(ctx, "k_plan_rank"));
     k_plan_items<<<nblk, 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, span, cap, boff, P.items);
     GA_TRY(ga_launch_check(ctx, "k_plan_items"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    P.built = P.tune;
     if (P.tune.item_order && nall > 1) {   // shape order within each bucket
         DevBuf k1, k2, v1, v2, it2;
         GA_TRY(ga_malloc(ctx, &k1.p, (size_t)nall * 8));
         GA_TRY(ga_malloc(ctx, &k2.p, (size_t)nall * 8));
         GA_TRY(ga_malloc(ctx, &v1.p, (size_t)nall * 4));
         GA_TRY(ga_malloc(ctx, &v2.p, (size_t)nall * 4));
-        GA_TRY(ga_malloc(ctx, &it2.p, (size_t)nall * 12));
+        GA_TRY(ga_malloc(ctx, &it2.p, (size_t)nall * 16));
         const uint32_t gbits = P.group ? P.group_bits : 0u;
         k_plan_item_shape<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, nall, g->rowptr, P.wlen, cap, P.group, gbits,
                                                                               (unsigned long long*)k1.p, (uint32_t*)v1.p);
@@ -1075,7 +1082,7 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a, int first_bucket = 0)
     const TcPlan& P = g->plan;
     uint32_t off = 0;
     for (int b = 0; b < NHASH; b++) {
-        const uint32_t* items = P.items + 3ull * off;
+        const uint32_t* items = P.items + 4ull * off;
         const uint32_t ni = b < first_bucket ? 0u : P.nitems[b];
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
         int rc = GA_OK;
@@ -1089,7 +1096,7 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a, int first_bucket = 0)
         GA_TRY(rc);
         off += P.nitems[b];
     }
-    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 3ull * off, P.nhuge, 0, a);
+    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 4ull * off, P.nhuge, 0, a);
 }
 
 template <int FLAGS>
@@ -1271,11 +1278,14 @@ extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t)
     if (!tuning_valid(*t)) return ga_fail(ctx, GA_ERR_ARG, "invalid tuning");
     if (t->flat_chunk && flat_shmem((uint32_t)t->flat_chunk) > ctx->lds_max)
         return ga_fail(ctx, GA_ERR_ARG, "flat chunk %d does not fit the workgroup's LDS", t->flat_chunk);
-    bool replan = !g->plan.tune.block || g->plan.tune.work_chunk != t->work_chunk || g->plan.tune.hash_cap != t->hash_cap ||
-                  g->plan.tune.batches != t->batches || g->plan.tune.item_order != t->item_order;
-    for (int b = 0; b < NHASH; b++) replan = replan || g->plan.tune.lanes[b] != t->lanes[b];
+    // the arrays are rebuilt only when a plan parameter differs from the ones they were built with (a tuning that comes
+    // back after a trial reuses them)
+    const ga_tc_tuning& b0 = g->plan.built;
+    bool same = g->plan.items && b0.work_chunk == t->work_chunk && b0.hash_cap == t->hash_cap && b0.batches == t->batches &&
+                b0.item_order == t->item_order;
+    for (int b = 0; b < NHASH; b++) same = same && b0.lanes[b] == t->lanes[b];
     g->plan.tune = *t;
-    if (replan) g->plan.ready = false;
+    g->plan.ready = same;
     return GA_OK;
 }
 

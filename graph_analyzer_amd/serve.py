@@ -76,7 +76,7 @@ def csr_bytes(n: int, m: int) -> int:
 
 
 def plan_bytes(m: int) -> int:
-    return 20 * m + 4 * m + 4096          # work entries (5 words) + items (at most one per entry, 3 words, usually far fewer)
+    return 20 * m + 4 * m + 4096          # work entries (5 words) + items (4 words each, about one per row)
 
 
 def peel_bytes(n: int) -> int:

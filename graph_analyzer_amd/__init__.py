@@ -86,6 +86,7 @@ def lib() -> ctypes.CDLL:
             "ga_ctx_device_info": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                   ctypes.POINTER(ctypes.c_int), _u64p]),
             "ga_ctx_copy_probe": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
+            "ga_ctx_gather_probe": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_uint32, _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_from_pairs": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(_vp),
                                                    ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_from_csr": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, ctypes.POINTER(_vp)]),
@@ -193,6 +194,13 @@ class Context:
         ms = ctypes.c_float()
         self.check(lib().ga_ctx_copy_probe(self._h, nbytes, ctypes.byref(ms)), "copy_probe")
         return ms.value
+
+    def gather_probe(self, nbytes: int, seg_bytes: int):
+        """Random reads of seg_bytes-long segments from an nbytes buffer (k_gather_probe); returns (segments, ms)."""
+        segs = ctypes.c_uint64()
+        ms = ctypes.c_float()
+        self.check(lib().ga_ctx_gather_probe(self._h, nbytes, seg_bytes, ctypes.byref(segs), ctypes.byref(ms)), "gather_probe")
+        return segs.value, ms.value
 
     def close(self):
         if self._h:

@@ -3,6 +3,7 @@
     python -m graph_analyzer_amd count|peel|core|support [--workload rmat21|powerlaw_communities|powerlaw_small]
                                  [--repeat R] [--num 1 --den 10]
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
+    python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
 
 Meant for profiling a single stage (rocprofv3 -- python3 -m graph_analyzer_amd core --workload rmat21) and for quick
 checks on a GPU host; results are not compared with a CPU oracle here (the tests and bench.py do that).
@@ -23,7 +24,7 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
-    ap.add_argument("stage", choices=["count", "peel", "core", "support", "batch"])
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "batch", "probe"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--num", type=int, default=1)
@@ -31,6 +32,20 @@ def main(argv=None):
     ap.add_argument("--graphs", type=int, default=4096)
     args = ap.parse_args(argv)
     ctx = ga.Context(0)
+    if args.stage == "probe":
+        for _ in range(2):
+            ms = ctx.copy_probe(1 << 30)
+            print(f"copy 1 GiB: {ms:.3f} ms = {2 * (1 << 30) / ms / 1e9:.2f} TB/s read + write", flush=True)
+        for mib in (139, 2048):
+            for seg in (32, 64, 128, 256):
+                best = None
+                for _ in range(3):
+                    n, ms = ctx.gather_probe(mib << 20, seg)
+                    best = ms if best is None else min(best, ms)
+                lines = n * max(seg // 128, 1)
+                print(f"gather {mib} MiB, {seg}-byte segments: {best:.3f} ms, {n / best / 1e6:.1f} G segments/s, "
+                      f"{lines / best / 1e6:.1f} G 128-byte lines/s, {n * seg / best / 1e9:.2f} TB/s useful", flush=True)
+        return 0
     if args.stage == "batch":
         bp = gen.BatchPairs(gen.batch_mix_specs(args.graphs))
         b = ga.Batch(ctx, flat=bp.pairs, offsets=bp.offsets)

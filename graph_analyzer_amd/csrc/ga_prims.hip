@@ -146,6 +146,62 @@ extern "C" int ga_ctx_copy_probe(ga_ctx* ctx, uint64_t bytes, float* ms) {
     return rc;
 }
 
+// Random-segment gather: every group of G lanes reads 8 independent random G-word segments per round (one u32 per
+// lane, consecutive lanes on consecutive words), the access pattern of the count's short probe lists.
+__device__ __forceinline__ uint32_t probe_mix(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+template <int G>
+__global__ void k_gather_probe(const uint32_t* __restrict__ a, uint32_t nseg, uint32_t rounds, unsigned long long* __restrict__ sink) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x % G, grp = tid / G;
+    uint32_t acc = 0;
+    for (uint32_t r = 0; r < rounds; r++) {
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t i = 0; i < 8; i++) {
+            const uint32_t seg = probe_mix(grp * 0x9E3779B9u + r * 131u + i * 7919u) % nseg;   // < nseg
+            v[i] = a[(size_t)seg * G + lane];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < 8; i++) acc += v[i];
+    }
+    if (acc == 0x12345678u) atomicAdd(sink, 1ull);   // keeps the loads live
+}
+
+extern "C" int ga_ctx_gather_probe(ga_ctx* ctx, uint64_t bytes, uint32_t seg_bytes, uint64_t* segments, float* ms) {
+    if (!ctx || (seg_bytes != 32 && seg_bytes != 64 && seg_bytes != 128 && seg_bytes != 256) || bytes < 4096 ||
+        bytes / seg_bytes >= 0xFFFFFFFFull)
+        return ga_fail(ctx, GA_ERR_ARG, "gather probe: segments of 32, 64, 128 or 256 bytes in 4 KiB .. 2^32 segments");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t nseg = (uint32_t)(bytes / seg_bytes), G = seg_bytes / 4, rounds = 32, block = 256;
+    const uint32_t grid = (uint32_t)std::max(ctx->cus, 1) * 32u;
+    void* a = nullptr;
+    GA_TRY(ga_malloc(ctx, &a, (size_t)nseg * seg_bytes + 64));
+    unsigned long long* sink = (unsigned long long*)((char*)a + (size_t)nseg * seg_bytes);
+    int rc = ga_fill_u32(ctx, (uint32_t*)a, (size_t)nseg * G + 16, 1u);
+    if (!rc && hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
+    if (!rc) {
+        switch (G) {
+            case 8: k_gather_probe<8><<<grid, block, 0, ctx->stream>>>((const uint32_t*)a, nseg, rounds, sink); break;
+            case 16: k_gather_probe<16><<<grid, block, 0, ctx->stream>>>((const uint32_t*)a, nseg, rounds, sink); break;
+            case 32: k_gather_probe<32><<<grid, block, 0, ctx->stream>>>((const uint32_t*)a, nseg, rounds, sink); break;
+            default: k_gather_probe<64><<<grid, block, 0, ctx->stream>>>((const uint32_t*)a, nseg, rounds, sink); break;
+        }
+        rc = ga_launch_check(ctx, "k_gather_probe");
+    }
+    if (!rc && (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess)) rc = GA_ERR_HIP;
+    if (!rc && ms) (void)hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    if (segments) *segments = (uint64_t)grid * block / G * rounds * 8;
+    ga_free(a);
+    return rc;
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // sorts and scans (rocPRIM device primitives, temporary storage from the context scratch)
 // ---------------------------------------------------------------------------------------------------------------

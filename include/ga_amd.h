@@ -42,6 +42,11 @@ int ga_ctx_device_info(ga_ctx* ctx, char* name, int len, int* cus, int* wave, ui
  * the known-size reference that scales hardware traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE).
  * No counterpart in the reference (measurement support). */
 int ga_ctx_copy_probe(ga_ctx* ctx, uint64_t bytes, float* ms);
+/* Random-segment gather (k_gather_probe): *segments reads of seg_bytes (32, 64, 128 or 256), each from a random
+ * aligned offset in a fresh buffer of `bytes`, 8 independent reads in flight per lane group; ms = kernel time. The
+ * measured ceiling of the count's access pattern (short probe lists at random places). No counterpart in the
+ * reference (measurement support). */
+int ga_ctx_gather_probe(ga_ctx* ctx, uint64_t bytes, uint32_t seg_bytes, uint64_t* segments, float* ms);
 /* Free and total device memory (hipMemGetInfo): the serving loop's admission budget. */
 int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 

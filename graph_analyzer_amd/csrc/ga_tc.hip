@@ -110,6 +110,60 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t x, uint32_t tbits) {
     return (x * 2654435769u) >> (32u - tbits);
 }
 
+// LDS hash tables of the lane-group kernel: 2^bbits buckets of 4 slots (16 bytes). A key lives in the first bucket
+// of its probe sequence (its hash bucket, then the next ones cyclically) that had a free slot when it was inserted.
+// A lookup reads whole buckets and stops at the first one with a free slot, so one 16-byte read decides almost every
+// lookup without a loop (at most a quarter of the slots are used at load_shift 2).
+__device__ __forceinline__ uint32_t key_bucket(uint32_t x, uint32_t bbits) {
+    return bbits ? (x * 2654435769u) >> (32u - bbits) : 0u;
+}
+
+// Inserts a key that is not in the table; returns its slot. Ends: a table has at least twice as many slots as keys,
+// and a failed compare-and-swap means another lane has just filled that slot.
+__device__ __forceinline__ uint32_t bucket_insert(uint32_t* tab, uint32_t bbits, uint32_t x) {
+    const uint32_t nbm = (1u << bbits) - 1u;
+    uint32_t bk = key_bucket(x, bbits);
+    for (;;) {
+        const uint4 t = ((const uint4*)tab)[bk];
+        const uint32_t j = t.x == GA_EMPTY ? 0u : t.y == GA_EMPTY ? 1u : t.z == GA_EMPTY ? 2u : t.w == GA_EMPTY ? 3u : 4u;
+        if (j == 4u) {
+            bk = (bk + 1u) & nbm;
+            continue;
+        }
+        if (atomicCAS(&tab[4u * bk + j], GA_EMPTY, x) == GA_EMPTY) return 4u * bk + j;
+    }
+}
+
+// Slot of x, or GA_EMPTY when x is not in the table. The loop runs only past a full bucket without x.
+__device__ __forceinline__ uint32_t bucket_find(const uint32_t* tab, uint32_t bbits, uint32_t x) {
+    const uint32_t nbm = (1u << bbits) - 1u;
+    uint32_t bk = key_bucket(x, bbits);
+    uint4 t = ((const uint4*)tab)[bk];
+    uint32_t j = t.x == x ? 0u : t.y == x ? 1u : t.z == x ? 2u : t.w == x ? 3u : 4u;
+    bool full = t.x != GA_EMPTY && t.y != GA_EMPTY && t.z != GA_EMPTY && t.w != GA_EMPTY;
+    if (j == 4u && full) {
+        for (uint32_t step = 0; step < nbm && j == 4u && full; step++) {
+            bk = (bk + 1u) & nbm;
+            t = ((const uint4*)tab)[bk];
+            j = t.x == x ? 0u : t.y == x ? 1u : t.z == x ? 2u : t.w == x ? 3u : 4u;
+            full = t.x != GA_EMPTY && t.y != GA_EMPTY && t.z != GA_EMPTY && t.w != GA_EMPTY;
+        }
+    }
+    return j < 4u ? 4u * bk + j : GA_EMPTY;
+}
+
+// Membership past a full first bucket (rare): walks the buckets after bk until x or a bucket with a free slot.
+__device__ __noinline__ uint32_t bucket_has_after(const uint32_t* tab, uint32_t bbits, uint32_t x, uint32_t bk) {
+    const uint32_t nbm = (1u << bbits) - 1u;
+    for (uint32_t step = 0; step < nbm; step++) {
+        bk = (bk + 1u) & nbm;
+        const uint4 t = ((const uint4*)tab)[bk];
+        if (t.x == x || t.y == x || t.z == x || t.w == x) return 1u;
+        if (t.x == GA_EMPTY || t.y == GA_EMPTY || t.z == GA_EMPTY || t.w == GA_EMPTY) return 0u;
+    }
+    return 0u;
+}
+
 // This lane group's bits of a wave ballot, shifted down to bit 0.
 template <int G>
 __device__ __forceinline__ uint64_t group_bits(uint64_t wave_mask) {
@@ -181,23 +235,16 @@ __global__ void k_tc_hash(const TcArgs a) {
             tbits = (dh > 1 ? 32u - (uint32_t)__clz(dh - 1) : 0u) + a.load_shift;
             if (tbits < 2) tbits = 2;
             tsz = 1u << tbits;
-            for (uint32_t s = sub; s < tsz; s += G) {
-                tab[s] = GA_EMPTY;
-                if (FLAGS & F_TRI) tix[s] = 0;
+            for (uint32_t w = sub; w < tsz / 4u; w += G) {   // 16-byte stores (tab and tix are 16-byte aligned)
+                ((uint4*)tab)[w] = make_uint4(GA_EMPTY, GA_EMPTY, GA_EMPTY, GA_EMPTY);
+                if (FLAGS & F_TRI) ((uint4*)tix)[w] = make_uint4(0, 0, 0, 0);
             }
             group_sync();
             for (uint32_t k = sub; k < dh; k += G) {
                 const uint32_t x = col[hs + k];
                 if ((FLAGS & F_ALIVE) && !alive[x]) continue;
-                uint32_t s = hash_slot(x, tbits);
-                for (uint32_t probe = 0; probe < tsz; probe++) {
-                    const uint32_t old = atomicCAS(&tab[s], GA_EMPTY, x);
-                    if (old == GA_EMPTY) {
-                        if (FLAGS & F_SUP) tix[s] = k;
-                        break;
-                    }
-                    s = (s + 1) & (tsz - 1);
-                }
+                const uint32_t s = bucket_insert(tab, tbits - 2u, x);
+                if (FLAGS & F_SUP) tix[s] = k;
             }
             group_sync();
         }
@@ -243,29 +290,52 @@ __global__ void k_tc_hash(const TcArgs a) {
                     if (k < end) {
                         const uint32_t kps = stg[k], klen = stg[G + k];
                         for (uint32_t off = r; off < klen; off += U * W) {
+                            // every slot loads (a slot past the list reloads element `off`, same line, and is masked):
+                            // no branch per load or per probe
                             uint32_t x[UMAX];
 #pragma unroll
-                            for (uint32_t u = 0; u < UMAX; u++) x[u] = (u < U && off + u * W < klen) ? col[kps + off + u * W] : 0u;
+                            for (uint32_t u = 0; u < UMAX; u++)
+                                if (u < U) x[u] = col[kps + (off + u * W < klen ? off + u * W : off)];
+                            if (HASHED && !(FLAGS & (F_TRI | F_SUP))) {
+                                // plain counts need only membership: every bucket read issued first, then vector
+                                // tests (x present: min of the four xors is 0; bucket full: max of its slots is not
+                                // GA_EMPTY), and the walk past a full bucket only for the lanes that need it
+                                const uint32_t bb = tbits - 2u;
+                                uint4 tb[UMAX];
+#pragma unroll
+                                for (uint32_t u = 0; u < UMAX; u++)
+                                    if (u < U) tb[u] = ((const uint4*)tab)[key_bucket(x[u], bb)];
+                                uint32_t slow = 0;
+#pragma unroll
+                                for (uint32_t u = 0; u < UMAX; u++)
+                                    if (u < U) {
+                                        const uint32_t xv = x[u];
+                                        const uint32_t d = min(min(tb[u].x ^ xv, tb[u].y ^ xv), min(tb[u].z ^ xv, tb[u].w ^ xv));
+                                        const uint32_t mx = max(max(tb[u].x, tb[u].y), max(tb[u].z, tb[u].w));
+                                        const uint32_t v = off + u * W < klen ? 1u : 0u;
+                                        c += (d == 0u ? 1u : 0u) & v;
+                                        slow |= ((d != 0u && mx != GA_EMPTY) ? v : 0u) << u;
+                                    }
+                                if (slow) {
+#pragma unroll
+                                    for (uint32_t u = 0; u < UMAX; u++)
+                                        if (u < U && ((slow >> u) & 1u)) c += bucket_has_after(tab, bb, x[u], key_bucket(x[u], bb));
+                                }
+                                continue;
+                            }
 #pragma unroll
                             for (uint32_t u = 0; u < UMAX; u++) {
-                                if (u >= U || off + u * W >= klen) break;
+                                if (u >= U) break;
+                                const bool valid = off + u * W < klen;
                                 const uint32_t addr = kps + off + u * W;
                                 bool hit = false;
                                 uint32_t hk = 0;
                                 if (HASHED) {
-                                    uint32_t s = hash_slot(x[u], tbits);
-                                    for (uint32_t probe = 0; probe < tsz; probe++) {
-                                        const uint32_t t = tab[s];
-                                        if (t == x[u]) {
-                                        hit = true;
-                                        if (FLAGS & F_SUP) hk = tix[s];
-                                        if (FLAGS & F_TRI) atomicAdd(&tix[s], 1u);   // x's hits gather in LDS
-                                        break;
-                                    }
-                                        if (t == GA_EMPTY) break;
-                                        s = (s + 1) & (tsz - 1);
-                                    }
-                                } else {
+                                    const uint32_t s = bucket_find(tab, tbits - 2u, x[u]);
+                                    hit = valid && s != GA_EMPTY;
+                                    if (hit && (FLAGS & F_SUP)) hk = tix[s];
+                                    if (hit && (FLAGS & F_TRI)) atomicAdd(&tix[s], 1u);   // x's hits gather in LDS
+                                } else if (valid) {
                                     const uint32_t q = lower_bound_u32(col, hs, hs + dh, x[u]);
                                     hit = q < hs + dh && col[q] == x[u] && (!(FLAGS & F_ALIVE) || alive[x[u]]);
                                     hk = q - hs;

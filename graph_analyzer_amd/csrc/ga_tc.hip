@@ -134,19 +134,25 @@ __device__ __forceinline__ uint32_t bucket_insert(uint32_t* tab, uint32_t bbits,
     }
 }
 
+// Position of x in a bucket (keys are unique, so at most one slot matches), 4 when absent: arithmetic, no branch.
+__device__ __forceinline__ uint32_t bucket_pos(const uint4 t, uint32_t x) {
+    const uint32_t m0 = t.x == x ? 1u : 0u, m1 = t.y == x ? 1u : 0u, m2 = t.z == x ? 1u : 0u, m3 = t.w == x ? 1u : 0u;
+    return (m0 | m1 | m2 | m3) ? m1 + 2u * m2 + 3u * m3 : 4u;
+}
+
 // Slot of x, or GA_EMPTY when x is not in the table. The loop runs only past a full bucket without x.
 __device__ __forceinline__ uint32_t bucket_find(const uint32_t* tab, uint32_t bbits, uint32_t x) {
     const uint32_t nbm = (1u << bbits) - 1u;
     uint32_t bk = key_bucket(x, bbits);
     uint4 t = ((const uint4*)tab)[bk];
-    uint32_t j = t.x == x ? 0u : t.y == x ? 1u : t.z == x ? 2u : t.w == x ? 3u : 4u;
-    bool full = t.x != GA_EMPTY && t.y != GA_EMPTY && t.z != GA_EMPTY && t.w != GA_EMPTY;
+    uint32_t j = bucket_pos(t, x);
+    bool full = max(max(t.x, t.y), max(t.z, t.w)) != GA_EMPTY;
     if (j == 4u && full) {
         for (uint32_t step = 0; step < nbm && j == 4u && full; step++) {
             bk = (bk + 1u) & nbm;
             t = ((const uint4*)tab)[bk];
-            j = t.x == x ? 0u : t.y == x ? 1u : t.z == x ? 2u : t.w == x ? 3u : 4u;
-            full = t.x != GA_EMPTY && t.y != GA_EMPTY && t.z != GA_EMPTY && t.w != GA_EMPTY;
+            j = bucket_pos(t, x);
+            full = max(max(t.x, t.y), max(t.z, t.w)) != GA_EMPTY;
         }
     }
     return j < 4u ? 4u * bk + j : GA_EMPTY;
@@ -279,9 +285,9 @@ __global__ void k_tc_hash(const TcArgs a) {
             for (uint32_t S = G; S >= 1; S >>= 1) {
                 // entries [start, end) have length in (S/2, S] (for S = G: every length above G/2); each list is read
                 // by W = S/U lanes (at least 1), every lane holding up to U loads in flight: 8 for lists longer than
-                // GA_TC_U8_FROM / 2 (more lists in flight at once), 4 for shorter ones
+                // GA_TC_U8_FROM / 2 (more lists in flight at once), 4 for shorter ones, S for lists of 1-3 elements
                 constexpr uint32_t UMAX = 8;
-                const uint32_t U = S >= GA_TC_U8_FROM ? 8u : 4u;
+                const uint32_t U = S >= GA_TC_U8_FROM ? 8u : S >= 4u ? 4u : S;   // lists of 1-3 elements: S slots
                 const uint32_t W = S >= U ? S / U : 1u;
                 const uint32_t end = (uint32_t)__popcll(group_bits<G>(__ballot(lk > S / 2)));
                 for (uint32_t k0 = start; k0 < end; k0 += G / W) {
@@ -301,21 +307,25 @@ __global__ void k_tc_hash(const TcArgs a) {
                                 // tests (x present: min of the four xors is 0; bucket full: max of its slots is not
                                 // GA_EMPTY), and the walk past a full bucket only for the lanes that need it
                                 const uint32_t bb = tbits - 2u;
-                                uint4 tb[UMAX];
-#pragma unroll
-                                for (uint32_t u = 0; u < UMAX; u++)
-                                    if (u < U) tb[u] = ((const uint4*)tab)[key_bucket(x[u], bb)];
                                 uint32_t slow = 0;
 #pragma unroll
-                                for (uint32_t u = 0; u < UMAX; u++)
-                                    if (u < U) {
-                                        const uint32_t xv = x[u];
-                                        const uint32_t d = min(min(tb[u].x ^ xv, tb[u].y ^ xv), min(tb[u].z ^ xv, tb[u].w ^ xv));
-                                        const uint32_t mx = max(max(tb[u].x, tb[u].y), max(tb[u].z, tb[u].w));
-                                        const uint32_t v = off + u * W < klen ? 1u : 0u;
-                                        c += (d == 0u ? 1u : 0u) & v;
-                                        slow |= ((d != 0u && mx != GA_EMPTY) ? v : 0u) << u;
-                                    }
+                                for (uint32_t u0 = 0; u0 < UMAX; u0 += 4) {   // four bucket reads in flight at a time
+                                    if (u0 >= U) break;
+                                    uint4 tb[4];
+#pragma unroll
+                                    for (uint32_t q = 0; q < 4; q++)
+                                        if (u0 + q < U) tb[q] = ((const uint4*)tab)[key_bucket(x[u0 + q], bb)];
+#pragma unroll
+                                    for (uint32_t q = 0; q < 4; q++)
+                                        if (u0 + q < U) {
+                                            const uint32_t u = u0 + q, xv = x[u];
+                                            const uint32_t d = min(min(tb[q].x ^ xv, tb[q].y ^ xv), min(tb[q].z ^ xv, tb[q].w ^ xv));
+                                            const uint32_t mx = max(max(tb[q].x, tb[q].y), max(tb[q].z, tb[q].w));
+                                            const uint32_t v = off + u * W < klen ? 1u : 0u;
+                                            c += (d == 0u ? 1u : 0u) & v;
+                                            slow |= ((d != 0u && mx != GA_EMPTY) ? v : 0u) << u;
+                                        }
+                                }
                                 if (slow) {
 #pragma unroll
                                     for (uint32_t u = 0; u < UMAX; u++)
@@ -1091,7 +1101,9 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     if (!nitems) return GA_OK;
     const TcPlan& P = g->plan;
     uint32_t block = (uint32_t)P.tune.block;
-    const uint32_t ts = HASHED ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), (uint32_t)P.tune.load_shift) : 0;
+    // tables with per-slot counters (participation, supports) take twice the LDS: at most load shift 2 there
+    const uint32_t shift = (FLAGS & (F_TRI | F_SUP)) ? std::min<uint32_t>((uint32_t)P.tune.load_shift, 2u) : (uint32_t)P.tune.load_shift;
+    const uint32_t ts = HASHED ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), shift) : 0;
     const uint32_t per_group = (ts * ((FLAGS & (F_SUP | F_TRI)) ? 2u : 1u) + 4u * G) * 4u;
     while (block > 64 && (size_t)(block / G) * per_group > 64 * 1024) block >>= 1;
     const size_t shmem = (size_t)(block / G) * per_group;
@@ -1101,7 +1113,7 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     a.items = items;
     a.nitems = nitems;
     a.ts_max = ts;
-    a.load_shift = (uint32_t)P.tune.load_shift;
+    a.load_shift = shift;
     k_tc_hash<G, HASHED, FLAGS><<<grid, block, shmem, ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_hash");
 }

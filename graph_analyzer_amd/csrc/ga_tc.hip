@@ -398,6 +398,66 @@ __global__ void k_tc_hash(const TcArgs a) {
     if (lane_id() == 0 && local) atomicAdd(a.total, (unsigned long long)local);
 }
 
+// Rows of d+(h) <= 8 (bucket 0), plain and per-graph counts, with lanes[0] = 1: one lane per item. N+(h) sits in 8
+// registers, padded with GA_EMPTY (no vertex id equals it), and every probe element is compared with all of them: no
+// table, no lane-group synchronisation, 64 items in flight per wavefront. Each entry's metadata is fetched one entry
+// ahead. The item loop is wavefront-uniform so that per-graph counts can be summed per wavefront.
+template <int FLAGS>
+__global__ void k_tc_tiny(const TcArgs a) {
+    static_assert(FLAGS == 0 || FLAGS == F_GRAPH, "plain and per-graph counts only");
+    if (a.done && *a.done) return;
+    const uint32_t* __restrict__ col = a.col;
+    const uint32_t lane = lane_id();
+    uint64_t local = 0;
+    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < a.nitems; base += gridDim.x * blockDim.x) {
+        const uint32_t it = base + lane;
+        uint32_t c = 0, gr = 0;
+        if (it < a.nitems) {
+            const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
+            const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK), dh = im.w >> ITEM_DSHIFT;   // 1..8
+            uint32_t nps = a.wps[b], nlen = a.wlen[b];   // every item has an entry; every list has 1..d+(h) elements
+            uint32_t hr[8];
+#pragma unroll
+            for (uint32_t k = 0; k < 8; k++) {
+                const uint32_t v = col[hs + min(k, dh - 1u)];
+                hr[k] = k < dh ? v : GA_EMPTY;
+            }
+            if (FLAGS & F_GRAPH) gr = a.gof[h];
+            for (uint32_t j = b; j < e; j++) {
+                const uint32_t ps = nps, len = nlen;
+                if (j + 1u < e) {
+                    nps = a.wps[j + 1u];
+                    nlen = a.wlen[j + 1u];
+                }
+                uint32_t x[8];
+#pragma unroll
+                for (uint32_t t = 0; t < 8; t++) x[t] = col[ps + min(t, len - 1u)];
+#pragma unroll
+                for (uint32_t t = 0; t < 8; t++) {
+                    const bool m = (x[t] == hr[0]) | (x[t] == hr[1]) | (x[t] == hr[2]) | (x[t] == hr[3]) | (x[t] == hr[4]) |
+                                   (x[t] == hr[5]) | (x[t] == hr[6]) | (x[t] == hr[7]);
+                    c += (m && t < len) ? 1u : 0u;
+                }
+            }
+        }
+        local += c;
+        if (FLAGS & F_GRAPH) {   // one atomic per graph present in the wavefront (items are graph-major: usually one)
+            uint64_t todo = __ballot(c != 0);
+            while (todo) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)todo) - 1u;
+                const uint32_t gl = __shfl(gr, (int)leader, 64);
+                const bool mine = c != 0 && gr == gl;
+                const uint64_t same = __ballot(mine);
+                const uint64_t s = wave_sum_u64(mine ? c : 0u);
+                if (lane == leader) atomicAdd(&a.gcnt[gl], (unsigned long long)s);
+                todo &= ~same;
+            }
+        }
+    }
+    local = wave_sum_u64(local);
+    if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // flat kernel for light rows (d+(h) <= FLAT_MAX), plain counts
 // ---------------------------------------------------------------------------------------------------------------
@@ -879,7 +939,8 @@ ga_tc_tuning default_tuning() {
 
 bool tuning_valid(const ga_tc_tuning& t) {
     for (int i = 0; i < NHASH; i++)
-        if (t.lanes[i] != 4 && t.lanes[i] != 8 && t.lanes[i] != 16 && t.lanes[i] != 32 && t.lanes[i] != 64) return false;
+        if (t.lanes[i] != 4 && t.lanes[i] != 8 && t.lanes[i] != 16 && t.lanes[i] != 32 && t.lanes[i] != 64 && !(i == 0 && t.lanes[i] == 1))
+            return false;
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
            t.flat_block <= 1024 && t.flat_block % 64 == 0 && (t.item_order == 0 || t.item_order == 1) &&
@@ -1118,6 +1179,16 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     return ga_launch_check(ctx, "k_tc_hash");
 }
 
+template <int FLAGS>
+static int launch_tiny(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, TcArgs a) {
+    if (!nitems) return GA_OK;
+    const uint32_t block = (uint32_t)g->plan.tune.block;
+    a.items = items;
+    a.nitems = nitems;
+    k_tc_tiny<FLAGS><<<grid_for(nitems, block, (unsigned)std::max(ctx->cus, 1) * 64u), block, 0, ctx->stream>>>(a);
+    return ga_launch_check(ctx, "k_tc_tiny");
+}
+
 // Flat kernel on the light rows; their lane-group buckets (d+(h) <= 128: buckets 0-2) are then skipped.
 static bool flat_on(const ga_graph* g) {
     const TcPlan& P = g->plan;
@@ -1169,6 +1240,10 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a, int first_bucket = 0)
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
         int rc = GA_OK;
         switch (P.tune.lanes[b]) {
+            case 1:   // bucket 0 only: one lane per item for plain and per-graph counts, 4-lane groups otherwise
+                if constexpr (FLAGS == 0 || FLAGS == F_GRAPH) rc = launch_tiny<FLAGS>(ctx, g, items, ni, a);
+                else rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, items, ni, bound, a);
+                break;
             case 4: rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, items, ni, bound, a); break;
             case 8: rc = launch_hash_bucket<8, true, FLAGS>(ctx, g, items, ni, bound, a); break;
             case 16: rc = launch_hash_bucket<16, true, FLAGS>(ctx, g, items, ni, bound, a); break;
@@ -1423,11 +1498,11 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         t.item_order = !bt.item_order;
         GA_TRY(trial(t));
     }
-    const int lane_opts[5] = {4, 8, 16, 32, 64};
+    const int lane_opts[6] = {4, 8, 16, 32, 64, 1};   // 1: bucket 0 only (one lane per item)
     for (int b = 0; b < NHASH; b++) {
         if (!g->plan.nitems[b]) continue;
         if (b < 3 && flat_on(g)) continue;   // light rows go to the flat kernel
-        for (int li = 0; li < 5; li++) {
+        for (int li = 0; li < (b == 0 ? 6 : 5); li++) {
             ga_tc_tuning t = bt;
             t.lanes[b] = lane_opts[li];
             if (t.lanes[b] == bt.lanes[b]) continue;

@@ -97,7 +97,8 @@ int ga_tc_prepare(ga_ctx* ctx, ga_graph* g, float* ms);
 /* Lane-group width / hash capacity / block shape per degree bucket (the autotuner's output, see ga_tc_tune). */
 typedef struct ga_tc_tuning {
     int32_t lanes[5];      /* lanes per work item for the hashed buckets d+(h) <= 8, 32, 128, 512, hash_cap
-                              (each one of 4, 8, 16, 32, 64) */
+                              (each one of 4, 8, 16, 32, 64; bucket 0 also 1: one lane per item with N+(h) in
+                              registers, for plain and per-graph counts) */
     int32_t block;         /* threads per workgroup (64..1024, multiple of 64) */
     int32_t work_chunk;    /* probe elements per work item before a row is cut */
     int32_t batches;       /* lane-group batches (lanes probe lists each) per work item before a row is cut */

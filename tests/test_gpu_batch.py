@@ -92,7 +92,8 @@ def test_batch_tunings_flat_per_graph(ctx):
     b = ga.Batch(ctx, graphs)
     for lanes, fc, fb, order in (([8, 16, 32, 64, 64], 256, 64, 1), ([4, 8, 32, 32, 64], 1024, 256, 0),
                                  ([4, 4, 16, 64, 64], 2048, 256, 1), ([8, 8, 64, 64, 64], 4096, 1024, 1),
-                                 ([16, 32, 64, 64, 64], 0, 512, 0)):
+                                 ([16, 32, 64, 64, 64], 0, 512, 0), ([1, 16, 32, 64, 64], 0, 256, 1),
+                                 ([1, 8, 32, 64, 64], 2048, 256, 1)):
         t = ga.TcTuning()
         t.lanes[:] = lanes
         t.block, t.work_chunk, t.batches, t.load_shift, t.hash_cap = 256, 2048, 4, 2, 4096

@@ -175,7 +175,8 @@ def test_tuning_shapes(ctx):
     g = ga.DeviceGraph.from_pairs(ctx, raw)
     base = g.get_tuning()
     flats = ((0, 64), (2048, 256), (256, 64), (4096, 512), (4096, 1024))
-    for li, lanes in enumerate(([4, 4, 8, 16, 32], [8, 16, 32, 64, 64], [16, 32, 64, 64, 64], [64, 64, 64, 64, 64], [4, 8, 16, 32, 64])):
+    for li, lanes in enumerate(([4, 4, 8, 16, 32], [8, 16, 32, 64, 64], [16, 32, 64, 64, 64], [64, 64, 64, 64, 64], [4, 8, 16, 32, 64],
+                                [1, 8, 32, 64, 64], [1, 16, 16, 32, 64])):   # 1: one lane per item for d+(h) <= 8
         for block in (64, 256, 1024):
             for chunk, shift, batches in ((64, 1, 1), (4096, 2, 8), (1 << 20, 3, 1024)):
                 t = ga.TcTuning()

@@ -921,7 +921,7 @@ struct DevBuf {
 
 ga_tc_tuning default_tuning() {
     ga_tc_tuning t;
-    t.lanes[0] = 8;
+    t.lanes[0] = 1;   // one lane per item for d+(h) <= 8 (4-lane groups in participation and support launches)
     t.lanes[1] = 16;
     t.lanes[2] = 32;
     t.lanes[3] = 64;

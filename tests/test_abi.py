@@ -90,3 +90,18 @@ def test_serving_workload_and_summary():
     assert s["requests"] == len(reqs) and 1.0 <= s["p50_ms"] <= s["p99_ms"] <= 10.0 + 1e-9
     assert set(s["by_kind"]) == set(serve.KINDS)
     assert np.array_equal(src(("mix", 3)), src2(("mix", 3)))
+
+
+def test_default_tuning_and_argument_checks_on_host():
+    """Host-only entry points: the default tuning record, and argument checks that reject before any device call."""
+    import graph_analyzer_amd as ga
+    t = ga.default_tuning()
+    d = t.as_dict()
+    assert d["lanes"][0] == 1 and all(x in (4, 8, 16, 32, 64) for x in d["lanes"][1:]), d
+    assert d["item_order"] == 1 and d["block"] % 64 == 0 and 1 <= d["load_shift"] <= 3
+    assert ga.TcTuning.from_dict(d).as_dict() == d
+    L = ga.lib()
+    for ctx, nbytes, seg in ((None, 1 << 20, 64), (None, 1 << 20, 48), (None, 16, 64)):
+        assert L.ga_ctx_gather_probe(ctx, nbytes, seg, None, None) == ga.GA_ERR_ARG
+    assert L.ga_ctx_copy_probe(None, 1 << 20, None) == ga.GA_ERR_ARG
+    assert L.ga_tc_default_tuning(None) == ga.GA_ERR_ARG

@@ -106,3 +106,15 @@ def test_batch_tunings_flat_per_graph(ctx):
     best, ms = b.tune()
     assert ms > 0 and np.array_equal(b.count_all()[0], ref)
     b.close()
+
+
+def test_device_probes(ctx):
+    """The measurement probes run, report what they did, and reject bad sizes."""
+    ms = ctx.copy_probe(64 << 20)
+    assert ms > 0
+    cus = ctx.device_info()["cus"]
+    for seg in (32, 64, 128, 256):
+        n, ms = ctx.gather_probe(64 << 20, seg)
+        assert ms > 0 and n == cus * 32 * 256 // (seg // 4) * 32 * 8, (seg, n, ms)
+    with pytest.raises(ga.GAError):
+        ctx.gather_probe(64 << 20, 96)

@@ -375,6 +375,7 @@ struct CoreBufs {
     uint32_t* adjw[3];
     uint32_t* cbeg;
     uint32_t* cend;
+    uint32_t* cgt;            // per vertex: position in its list of the first neighbour with a larger id (cN(v) after v)
     uint32_t* fvert;
     uint32_t* ent;            // 2 words per entry: u, chunk
     uint32_t* toff;           // per vertex: offset of its table in the arena (frontier vertices of this round)
@@ -383,11 +384,12 @@ struct CoreBufs {
 };
 
 __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32_t* __restrict__ cbeg, uint32_t* __restrict__ cend,
-                          const uint32_t* __restrict__ urow, uint32_t n) {
+                          uint32_t* __restrict__ cgt, const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol, uint32_t n) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
         alist0[v] = v;
         cbeg[v] = urow[v];
         cend[v] = urow[v + 1];
+        cgt[v] = upper_bound_u32(ucol, urow[v], urow[v + 1], v);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         f->na = n;
@@ -539,9 +541,8 @@ __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev
                 v = adj[p];
                 sv = state[v];
                 if (sv) {
-                    const uint32_t vb = B.cbeg[v], ve = B.cend[v];
-                    lo = upper_bound_u32(adj, vb, ve, v);        // cN(v) after v
-                    hi = ve;
+                    lo = B.cgt[v];                                // cN(v) after v
+                    hi = B.cend[v];
                     hub = B.hubid[v];
                     const uint32_t a = ue - (p + 1), bl = hi - lo;
                     if (bl <= a) { ps = lo; len = bl; mode = 0; }   // stream cN(v) after v, test in N(u)
@@ -707,17 +708,20 @@ __global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ sta
             unsigned long long at = 0;
             if (lane == 0 && cnt) at = atomicAdd(&f->adj_cursor, (unsigned long long)cnt);
             at = __shfl(at, 0, 64);
-            uint32_t w = (uint32_t)at;
+            uint32_t w = (uint32_t)at, below = 0;   // below: kept neighbours with a smaller id than v
             for (uint32_t p0 = b; p0 < e; p0 += 64) {   // wave-uniform, keeps the sorted order
                 const uint32_t p = p0 + lane;
-                const bool k2 = p < e && state[src[p]] == 1;
+                const uint32_t x = p < e ? src[p] : 0u;
+                const bool k2 = p < e && state[x] == 1;
                 const uint64_t mask = __ballot(k2);
-                if (k2) dst[w + mask_rank(mask)] = src[p];
+                if (k2) dst[w + mask_rank(mask)] = x;
                 w += (uint32_t)__popcll(mask);
+                below += (uint32_t)__popcll(__ballot(k2 && x < v));
             }
             if (lane == 0) {
                 B.cbeg[v] = (uint32_t)at;
                 B.cend[v] = (uint32_t)at + cnt;
+                B.cgt[v] = (uint32_t)at + below;
             }
         }
     }
@@ -804,7 +808,7 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     const size_t n = g->n, m2 = 2ull * g->m;
     const size_t nent = n + m2 / CORE_CHUNK + 16;
     const size_t arena = 4 * std::max<size_t>(m2, 1) + 4 * n + 64;   // tables: at most 4 slots per listed neighbour (+4 per vertex)
-    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 2 * n + n + 2 * nent + 2 * n + arena + 64 + 16;
+    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 3 * n + n + 2 * nent + 2 * n + arena + 64 + 16;
     if (arena >= 0xFFFFFFFFull || nent >= 0x7FFFFFFFull)
         return ga_fail(ctx, GA_ERR_ARG, "graph too large for the fused core decomposition (32-bit table offsets)");
     GA_TRY(ga_malloc(ctx, &M.block, words * 4));
@@ -817,6 +821,7 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     M.B.adjw[2] = p; M.B.adj[2] = p; p += std::max<size_t>(m2, 1);
     M.B.cbeg = p; p += n;
     M.B.cend = p; p += n;
+    M.B.cgt = p; p += n;
     M.B.fvert = p; p += n;
     M.B.ent = p; p += 2 * nent;
     M.B.toff = p; p += n;
@@ -849,7 +854,7 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     GA_TRY(ga_core_reset_state(c));
     k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, T_full);
     GA_TRY(ga_launch_check(ctx, "k_core_init"));
-    k_cf_init<<<cgrid(ctx, g->n), 256, 0, ctx->stream>>>(M.f, M.B.alist[0], M.B.cbeg, M.B.cend, g->urow, g->n);
+    k_cf_init<<<cgrid(ctx, g->n), 256, 0, ctx->stream>>>(M.f, M.B.alist[0], M.B.cbeg, M.B.cend, M.B.cgt, g->urow, g->ucol, g->n);
     GA_TRY(ga_launch_check(ctx, "k_cf_init"));
     GA_TRY(ga_fill_u32(ctx, M.B.tables, M.arena_words, GA_EMPTY));   // each round's tail clears what it used
     if (g->n) {

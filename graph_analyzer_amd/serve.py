@@ -87,7 +87,7 @@ def core_bytes(n: int, m: int) -> int:
     m2 = max(2 * m, 1)
     nent = n + m2 // 8 + 16
     arena = 4 * m2 + 4 * n + 64
-    words = 2 * n + 2 * m2 + 2 * n + n + 2 * nent + 2 * n + arena + 80
+    words = 2 * n + 2 * m2 + 3 * n + n + 2 * nent + 2 * n + arena + 80
     return 4 * (n + 1) + 4 * m2 + 13 * n + 4 * words + 4096
 
 

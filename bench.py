@@ -23,6 +23,10 @@ faster of the library's default tuning and the batch's tuning). Both must give i
 graphs is also checked against the CPU oracle. Peel: batch peeling at eps = 1/10, all graphs' rounds in shared launches
 vs one fused peel per graph.
 
+Extras (same workload): load_graph with the text parse on the GPU and the binary CSR file, and prune_to_core at
+ceil(kmax-core density) on the device core numbers, each checked against the oracle and timed beside the reference's
+own CPU code on one host core.
+
 Serving: Poisson arrivals (2,000 req/s for 1 s by default) of count / peel (eps 1/20) / core requests, 95% on a pool
 of 256 graphs of the batch size law and 5% on two R-MAT scale-20 graphs, all resident; p50 / p99 latency (arrival to
 result) and sustained throughput of the continuous-batching server vs the plain one-request-at-a-time form on the same
@@ -104,6 +108,81 @@ def cpu_baseline(pairs, T_gpu):
     if T_ser != T_omp:
         raise SystemExit(f"MISMATCH: serial CPU count {T_ser} != OpenMP count {T_omp}")
     return {"kind": kind, "t_serial_s": t_ser, "t_omp_s": t_omp, "omp_threads": threads, "m": g.m}
+
+
+def run_extras(ctx, g, pairs, cfg, csr, T0):
+    """The rows next to the count on the same workload, each checked against the CPU oracle and timed beside the
+    reference's own CPU code (oracle/_ref; the oracle restatement when that is absent), 1 thread:
+    GPU text ingest (load_graph, graph.h:69-96) and the binary CSR file; GPU prune_to_core (dsg.h:636-645) at
+    ceil(kmax-core density) on the device core numbers."""
+    import tempfile
+
+    import graph_analyzer_amd as ga
+    import oracle
+    from graph_analyzer_amd import gen
+    ref = oracle.ref_available()
+    out = {"cpu_kind": "reference" if ref else "port", "cpu_cores": 1}
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "g.txt")
+        gen.write_text(pairs, path, cfg["workload"])
+        size = os.path.getsize(path)
+        walls, devs = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            t = ga.DeviceGraph.load_text(ctx, path)
+            walls.append((time.perf_counter() - t0) * 1e3)
+            devs.append(t.build_ms)
+            if len(walls) == 1 and not all(np.array_equal(a, b) for a, b in zip(t.download(), csr)):
+                raise SystemExit("MISMATCH: GPU text ingest differs from the oracle's CSR")
+            if len(walls) < 3:
+                t.close()
+        t0 = time.perf_counter()
+        rg = oracle.RGraph.load_text(path) if ref else oracle.OGraph.load_text(path)
+        t_cpu = time.perf_counter() - t0
+        if not all(np.array_equal(a, b) for a, b in zip(rg.arrays(), csr)):
+            raise SystemExit("MISMATCH: CPU load_graph differs from the oracle's CSR")
+        del rg
+        bpath = os.path.join(td, "g.gacsr")
+        t.save(bpath)
+        t.close()
+        bwalls, bdevs = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            h = ga.DeviceGraph.load(ctx, bpath)
+            bwalls.append((time.perf_counter() - t0) * 1e3)
+            bdevs.append(h.build_ms)
+            h.close()
+        out["ingest"] = {"text_bytes": size, "lines": int(len(pairs)) + 1,
+                         "gpu_parse_build_ms": min(devs), "gpu_wall_ms": min(walls),
+                         "gpu_text_gb_per_s": size / (min(devs) * 1e-3) / 1e9,
+                         "cpu_load_graph_s": t_cpu, "speedup_wall": t_cpu * 1e3 / min(walls),
+                         "binary_bytes": os.path.getsize(bpath), "binary_load_wall_ms": min(bwalls), "binary_load_device_ms": min(bdevs)}
+    log(f"ingest: {size / 1e6:.0f} MB text, GPU parse+build {min(devs):.1f} ms device / {min(walls):.1f} ms wall "
+        f"(file read + upload); CPU load_graph {t_cpu:.2f} s; binary load {min(bwalls):.1f} ms wall")
+    c = ga.Core(g, True)
+    core, rounds, kmax = c.run(T_full=T0)
+    T_k, _ = g.vertex_participation((core >= kmax).astype(np.uint8))
+    verts = int((core >= kmax).sum())
+    k = (T_k + verts - 1) // verts if verts else 0
+    pms = []
+    for _ in range(3):
+        sub, old, cout = c.prune(k)
+        pms.append(sub.build_ms)
+        if len(pms) < 3:
+            sub.close()
+    cg = oracle.RGraph.build(g.n, np.stack([csr[2], csr[1]], 1)) if ref else oracle.OGraph.from_csr(csr[0], csr[1])
+    t0 = time.perf_counter()
+    osub, oold, ocout = cg.prune_to_core(core, k)
+    t_cpu = time.perf_counter() - t0
+    if not (np.array_equal(old, oold) and np.array_equal(cout, ocout) and
+            all(np.array_equal(a, b) for a, b in zip(sub.download(), osub.arrays()))):
+        raise SystemExit("MISMATCH: GPU prune_to_core differs from the CPU's")
+    out["prune"] = {"k": int(k), "kmax": int(kmax), "core_rounds": int(rounds), "core_ms": c.last_ms, "n_in": g.n, "m_in": g.m,
+                    "n_out": sub.n, "m_out": sub.m, "gpu_ms": min(pms), "cpu_prune_to_core_s": t_cpu}
+    log(f"prune: k={k} n {g.n} -> {sub.n}, m {g.m} -> {sub.m}: GPU {min(pms):.2f} ms, CPU prune_to_core {t_cpu:.2f} s")
+    sub.close()
+    c.close()
+    return out
 
 
 def run_batch(ctx, count, steps, tune):
@@ -221,6 +300,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered requests per second in the serving comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the ingest and prune measurements")
     ap.add_argument("--tuning-from", metavar="JSON",
                     help="apply a recorded count tuning (no tuner): a result line of this script or a JSON object with a 'tuning' key")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -310,6 +390,9 @@ def main():
     del og
     traffic = load_traffic(cfg["workload"])
     n, m, max_out = g.n, g.m, g.max_out_degree
+    extras = None
+    if not args.no_extras:
+        extras = run_extras(ctx, g, pairs, cfg, (rowptr, col, src), T0)
     batch = None
     if args.batch_graphs:
         g.close()
@@ -338,6 +421,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": B_alg, "sum_min_outdeg": summin, "kernel_ms_mean": mean_dev},
         "cpu_baseline": cpu,
+        "extras": extras,
         "batch": batch,
         "serving": serving,
     }

@@ -95,6 +95,14 @@ def lib() -> ctypes.CDLL:
             "ga_graph_download": (ctypes.c_int, [_vp, _vp, _u32p, _u32p, _u32p]),
             "ga_graph_undirected": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
             "ga_graph_edge_order": (ctypes.c_int, [_vp, _vp, _u32p]),
+            "ga_graph_from_text": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]),
+            "ga_graph_load_text": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]),
+            "ga_graph_save": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p]),
+            "ga_graph_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]),
+            "ga_graph_induced": (ctypes.c_int, [_vp, _vp, _u8p, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_graph_prune_to_core": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p,
+                                                      ctypes.POINTER(ctypes.c_float)]),
+            "ga_core_prune": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_count_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_int, _u64p,
@@ -257,6 +265,34 @@ class DeviceGraph:
         ctx.check(lib().ga_graph_from_csr(ctx.handle, r.size - 1, c.size, _ptr(r, _u32p), _ptr(c, _u32p), ctypes.byref(h)), "graph_from_csr")
         return cls(ctx, h)
 
+    @classmethod
+    def from_text(cls, ctx: Context, text: bytes) -> "DeviceGraph":
+        """load_graph on an in-memory SNAP edge list, parsed on the GPU."""
+        h = _vp()
+        ms = ctypes.c_float()
+        ctx.check(lib().ga_graph_from_text(ctx.handle, text, len(text), ctypes.byref(h), ctypes.byref(ms)), "graph_from_text")
+        return cls(ctx, h, ms.value)
+
+    @classmethod
+    def load_text(cls, ctx: Context, path: str) -> "DeviceGraph":
+        """load_graph (graph.h:71-96) with the text parse on the GPU."""
+        h = _vp()
+        ms = ctypes.c_float()
+        ctx.check(lib().ga_graph_load_text(ctx.handle, os.fsencode(path), ctypes.byref(h), ctypes.byref(ms)), "graph_load_text")
+        return cls(ctx, h, ms.value)
+
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "DeviceGraph":
+        """A binary CSR file written by save(), checked on the device."""
+        h = _vp()
+        ms = ctypes.c_float()
+        ctx.check(lib().ga_graph_load(ctx.handle, os.fsencode(path), ctypes.byref(h), ctypes.byref(ms)), "graph_load")
+        return cls(ctx, h, ms.value)
+
+    def save(self, path: str):
+        """Writes the binary CSR file (GACSR001: 64-byte header, rowptr, col)."""
+        self.ctx.check(lib().ga_graph_save(self.ctx.handle, self._h, os.fsencode(path)), "graph_save")
+
     @property
     def handle(self):
         return self._h
@@ -278,6 +314,34 @@ class DeviceGraph:
         perm = np.zeros(max(self.m, 1), np.uint32)
         self.ctx.check(lib().ga_graph_edge_order(self.ctx.handle, self._h, _ptr(perm, _u32p)), "edge_order")
         return perm[: self.m]
+
+    # ---- subgraphs (dsg.h:103-114, 636-645) ----
+    def induced(self, keep: np.ndarray):
+        """induced_subgraph on {v : keep[v] != 0}: returns (DeviceGraph, old_id); new ids follow the old ids' order."""
+        k = np.ascontiguousarray(keep, dtype=np.uint8)
+        if k.size != self.n:
+            raise ValueError(f"keep has {k.size} entries, the graph {self.n} vertices")
+        old = np.zeros(max(self.n, 1), np.uint32)
+        h = _vp()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_graph_induced(self.ctx.handle, self._h, _ptr(k, _u8p) if k.size else None, ctypes.byref(h),
+                                              _ptr(old, _u32p), ctypes.byref(ms)), "graph_induced")
+        sub = DeviceGraph(self.ctx, h, ms.value)
+        return sub, old[: sub.n]
+
+    def prune_to_core(self, core: np.ndarray, k: int):
+        """prune_to_core: the subgraph on {v : core[v] >= k}; returns (DeviceGraph, old_id, core_out)."""
+        c = np.ascontiguousarray(core, dtype=np.uint32)
+        if c.size != self.n:
+            raise ValueError(f"core has {c.size} entries, the graph {self.n} vertices")
+        old = np.zeros(max(self.n, 1), np.uint32)
+        cout = np.zeros(max(self.n, 1), np.uint32)
+        h = _vp()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_graph_prune_to_core(self.ctx.handle, self._h, _ptr(c, _u32p) if c.size else None, int(k), ctypes.byref(h),
+                                                    _ptr(old, _u32p), _ptr(cout, _u32p), ctypes.byref(ms)), "graph_prune_to_core")
+        sub = DeviceGraph(self.ctx, h, ms.value)
+        return sub, old[: sub.n], cout[: sub.n]
 
     # ---- triangle counting ----
     def count(self, variant: int = TC_AUTO):
@@ -465,6 +529,19 @@ class Core:
         c = np.zeros(max(self.g.n, 1), np.uint32)
         self.ctx.check(lib().ga_core_download(self._h, _ptr(c, _u32p)), "core_download")
         return c[: self.g.n]
+
+    def prune(self, k: int):
+        """prune_to_core on the core numbers this (finished) state holds on the device: the subgraph on
+        {v : core[v] >= k}; returns (DeviceGraph, old_id, core_out)."""
+        n = self.g.n
+        old = np.zeros(max(n, 1), np.uint32)
+        cout = np.zeros(max(n, 1), np.uint32)
+        h = _vp()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_core_prune(self._h, int(k), ctypes.byref(h), _ptr(old, _u32p), _ptr(cout, _u32p), ctypes.byref(ms)),
+                       "core_prune")
+        sub = DeviceGraph(self.ctx, h, ms.value)
+        return sub, old[: sub.n], cout[: sub.n]
 
     def run(self, T_full: Optional[int] = None):
         stats = np.zeros(2, np.uint32)

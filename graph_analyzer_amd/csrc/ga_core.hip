@@ -331,6 +331,11 @@ extern "C" int ga_core_run_retire(ga_core* c) {
     return ga_launch_check(ctx, "k_core_retire");
 }
 
+extern "C" int ga_core_prune(ga_core* c, uint64_t k, ga_graph** out, uint32_t* old_id, uint32_t* core_out, float* ms) {
+    if (!c || !out) return GA_ERR_ARG;
+    return ga_prune_core_dev(c->ctx, c->g, c->core, k, out, old_id, core_out, ms);
+}
+
 extern "C" int ga_core_download(ga_core* c, uint32_t* core) {
     ga_ctx* ctx = c->ctx;
     if (c->g->n) GA_HIP(ctx, hipMemcpyAsync(core, c->core, (size_t)c->g->n * 4, hipMemcpyDeviceToHost, ctx->stream));

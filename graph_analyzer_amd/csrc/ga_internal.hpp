@@ -100,6 +100,9 @@ int ga_max_u32(ga_ctx* ctx, const uint32_t* p, size_t count, uint32_t* out_host)
 int ga_build_from_device_pairs(ga_ctx* ctx, const uint32_t* dpairs, uint64_t k, int64_t n_in, bool strict, ga_graph** out,
                                float* build_ms);
 int ga_graph_ensure_undirected(ga_ctx* ctx, ga_graph* g);
+// prune_to_core on device-resident core numbers (ga_sub.hip): keep core_dev[v] >= k.
+int ga_prune_core_dev(ga_ctx* ctx, const ga_graph* g, const uint32_t* core_dev, uint64_t k, ga_graph** out, uint32_t* old_id,
+                      uint32_t* core_out, float* ms);
 int ga_graph_ensure_perm(ga_ctx* ctx, ga_graph* g);
 int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g);
 // ga_tc_tune over the count form a caller runs: plain (gcnt NULL) or per graph of a disjoint union (gof, gcnt of

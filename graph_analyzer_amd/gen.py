@@ -10,6 +10,8 @@ import ctypes
 import math
 import os
 
+from typing import Optional
+
 import numpy as np
 
 from . import GEN_PATH
@@ -37,6 +39,7 @@ def _gen():
         L.ga_gen_er.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, E]
         L.ga_gen_chung_lu_alias.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, E]
         L.ga_gen_communities.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, E]
+        L.ga_gen_write_text.argtypes = [E, ctypes.c_char_p, ctypes.c_char_p]
         L.ga_gen_free.argtypes = [E]
         L.ga_gen_free.restype = None
         P = ctypes.POINTER
@@ -58,6 +61,17 @@ def _take(fn, *args) -> np.ndarray:
         return a.reshape(-1, 2)
     finally:
         _gen().ga_gen_free(ctypes.byref(e))
+
+
+def write_text(pairs: np.ndarray, path: str, comment: Optional[str] = None):
+    """SNAP-style text file, one "a<TAB>b" line per pair, after an optional '#' comment line."""
+    p = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1)
+    e = _Edges()
+    e.pairs = p.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    e.count = p.size // 2
+    e.n = 0
+    if _gen().ga_gen_write_text(ctypes.byref(e), os.fsencode(path), comment.encode() if comment else None) != 0:
+        raise OSError(f"cannot write {path}")
 
 
 def chung_lu(n: int, samples: int, gamma: float, i0: float = 0.0, seed: int = 1, permute: bool = True) -> np.ndarray:

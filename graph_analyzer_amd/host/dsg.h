@@ -1,7 +1,8 @@
 // dsg.h (graph_analyzer_amd) - drop-in for the (k,triangle)-core part of the reference's dsg.h API (dsg.h:34-58,
 // 647-751): Dens and its exact comparisons, dsg_row, CoreLoopResult, the backend-templated level loops
 // core_decomp_loop / core_decomp_loop_inc, the GPU backends of gpu_dsg.cu:88-199 over the C ABI, a fused gpu_core_run,
-// and core_level_density on the device. The paper pipeline (Greedy++, Exact, CoreExact) is not part of this library.
+// core_level_density, and induced_subgraph / prune_to_core (dsg.h:103-114, 636-645) on the device. The paper pipeline
+// (Greedy++, Exact, CoreExact) is not part of this library.
 #pragma once
 #include "densest.h"
 
@@ -152,7 +153,64 @@ struct GpuCoreBackendInc {
         out.assign(n, 0);
         gaa::check(ctx, ga_core_download(c, n ? out.data() : nullptr), "ga_core_download");
     }
+    // prune_to_core on the core numbers of the last finished loop, without downloading them (ga_core_prune).
+    gaa::DeviceGraph prune(uint64_t k, std::vector<uint32_t>& old_id, std::vector<uint32_t>& core_out, float* ms = nullptr) {
+        ga_graph* h = nullptr;
+        old_id.assign(std::max<uint32_t>(n, 1), 0);
+        core_out.assign(std::max<uint32_t>(n, 1), 0);
+        float t = 0;
+        gaa::check(ctx, ga_core_prune(c, k, &h, old_id.data(), core_out.data(), &t), "ga_core_prune");
+        gaa::DeviceGraph d(ctx, h);
+        old_id.resize(d.n());
+        core_out.resize(d.n());
+        if (ms) *ms = t;
+        return d;
+    }
 };
+
+// induced_subgraph (dsg.h:103-114) of a resident graph: the subgraph on {v : keep[v] != 0}, new ids in old-id order,
+// oriented by the degrees inside the subgraph; built and kept in HBM (ga_graph_induced).
+inline gaa::DeviceGraph induced_subgraph_device(const gaa::DeviceGraph& g, const std::vector<uint8_t>& keep, std::vector<uint32_t>& old_id,
+                                                float* ms = nullptr) {
+    if (keep.size() != g.n()) gaa::die(g.ctx(), GA_ERR_ARG, "induced_subgraph: keep must have n entries");
+    ga_graph* h = nullptr;
+    old_id.assign(std::max<uint32_t>(g.n(), 1), 0);
+    float t = 0;
+    gaa::check(g.ctx(), ga_graph_induced(g.ctx(), g.handle(), g.n() ? keep.data() : nullptr, &h, old_id.data(), &t), "ga_graph_induced");
+    gaa::DeviceGraph d(g.ctx(), h);
+    old_id.resize(d.n());
+    if (ms) *ms = t;
+    return d;
+}
+
+// prune_to_core (dsg.h:636-645) of a resident graph with host core numbers (ga_graph_prune_to_core).
+inline gaa::DeviceGraph prune_to_core_device(const gaa::DeviceGraph& g, const std::vector<uint32_t>& core, uint64_t k,
+                                             std::vector<uint32_t>& old_id, std::vector<uint32_t>& core_out, float* ms = nullptr) {
+    if (core.size() != g.n()) gaa::die(g.ctx(), GA_ERR_ARG, "prune_to_core: core must have n entries");
+    ga_graph* h = nullptr;
+    old_id.assign(std::max<uint32_t>(g.n(), 1), 0);
+    core_out.assign(std::max<uint32_t>(g.n(), 1), 0);
+    float t = 0;
+    gaa::check(g.ctx(), ga_graph_prune_to_core(g.ctx(), g.handle(), g.n() ? core.data() : nullptr, k, &h, old_id.data(), core_out.data(), &t),
+               "ga_graph_prune_to_core");
+    gaa::DeviceGraph d(g.ctx(), h);
+    old_id.resize(d.n());
+    core_out.resize(d.n());
+    if (ms) *ms = t;
+    return d;
+}
+
+// The reference's host signatures (dsg.h:103, 636): upload, build on the device, download.
+inline void induced_subgraph(const Graph& g, const std::vector<uint8_t>& keep, Graph& out, std::vector<uint32_t>& old_id) {
+    gaa::DeviceGraph d = gaa::DeviceGraph::upload(gaa::default_ctx(), g);
+    induced_subgraph_device(d, keep, old_id).download(out);
+}
+
+inline void prune_to_core(const Graph& g, const std::vector<uint32_t>& core, uint64_t k, Graph& out, std::vector<uint32_t>& old_id,
+                          std::vector<uint32_t>& core_out) {
+    gaa::DeviceGraph d = gaa::DeviceGraph::upload(gaa::default_ctx(), g);
+    prune_to_core_device(d, core, k, old_id, core_out).download(out);
+}
 
 // Whole incremental decomposition with the level loop on the device (ga_core_run).
 inline CoreLoopResult gpu_core_run(const gaa::DeviceGraph& g, uint64_t T_full, std::vector<uint32_t>& core, float* ms = nullptr) {

@@ -3,7 +3,8 @@
 // Same names and argument meaning: struct Graph, now_ms, build_graph_from_edges, load_graph, build_edge_order.
 // Differences a caller sees:
 //   * build_graph_from_edges / load_graph do de-duplication, (degree, id) orientation and row sorting on the GPU
-//     (ga_graph_from_pairs); the text parse of load_graph stays on the host. The resulting arrays are byte-identical.
+//     (ga_graph_from_pairs); load_graph parses the text on the GPU too (ga_graph_load_text). The arrays are
+//     byte-identical to the reference's.
 //   * the CPU counters cpu_triangles_serial / cpu_triangles_omp are not part of this library (they are the CPU
 //     baseline); gpu_triangles() counts on the device.
 //   * runtime errors print the HIP message and exit(1), like the reference's CUDA_CHECK (gpu_tc.cu:29-37).
@@ -169,22 +170,38 @@ inline void build_graph_from_edges(uint32_t n, const std::vector<std::pair<uint3
     d.download(g);
 }
 
-// load_graph (graph.h:71-96): text parse on the host, canonicalise / dedup / orient on the GPU.
+// load_graph (graph.h:71-96): text parse, canonicalisation, de-duplication and orientation on the GPU. A file that
+// cannot be opened prints "cannot open" and returns false, as in the reference.
+inline bool load_graph_device(const char* path, gaa::DeviceGraph& out, ga_ctx* ctx = nullptr) {
+    FILE* f = fopen(path, "r");
+    if (!f) { fprintf(stderr, "cannot open %s\n", path); return false; }
+    fclose(f);
+    ctx = ctx ? ctx : gaa::default_ctx();
+    ga_graph* h = nullptr;
+    float ms = 0;
+    gaa::check(ctx, ga_graph_load_text(ctx, path, &h, &ms), "ga_graph_load_text");
+    out = gaa::DeviceGraph(ctx, h);
+    return true;
+}
+
 inline bool load_graph(const char* path, Graph& g) {
-    std::vector<uint32_t> pairs;
-    if (!gaa::parse_edge_file(path, pairs)) return false;
-    ga_ctx* ctx = gaa::default_ctx();
-    gaa::DeviceGraph d = gaa::DeviceGraph::from_pairs(ctx, pairs.data(), pairs.size() / 2, -1);
+    gaa::DeviceGraph d;
+    if (!load_graph_device(path, d)) return false;
     d.download(g);
     return true;
 }
 
-// load_graph straight into HBM (no host copy of the CSR).
-inline bool load_graph_device(const char* path, gaa::DeviceGraph& out, ga_ctx* ctx = nullptr) {
-    std::vector<uint32_t> pairs;
-    if (!gaa::parse_edge_file(path, pairs)) return false;
-    out = gaa::DeviceGraph::from_pairs(ctx ? ctx : gaa::default_ctx(), pairs.data(), pairs.size() / 2, -1);
-    return true;
+// The library's binary CSR file (ga_graph_save / ga_graph_load): no text parse and no build on admission; the device
+// checks that the arrays are a valid (degree, id)-oriented CSR.
+inline void save_graph_binary(const gaa::DeviceGraph& g, const char* path) {
+    gaa::check(g.ctx(), ga_graph_save(g.ctx(), g.handle(), path), "ga_graph_save");
+}
+
+inline gaa::DeviceGraph load_graph_binary(const char* path, ga_ctx* ctx = nullptr) {
+    ctx = ctx ? ctx : gaa::default_ctx();
+    ga_graph* h = nullptr;
+    gaa::check(ctx, ga_graph_load(ctx, path, &h, nullptr), "ga_graph_load");
+    return gaa::DeviceGraph(ctx, h);
 }
 
 // build_edge_order (graph.h:156-171), computed on the device.

@@ -71,6 +71,35 @@ int ga_graph_undirected(ga_ctx* ctx, ga_graph* g, uint32_t* urow, uint32_t* ucol
  * graph.h:156-171), computed on the device; perm has m entries. */
 int ga_graph_edge_order(ga_ctx* ctx, ga_graph* g, uint32_t* perm);
 
+/* load_graph (graph.h:71-96) with the text parse on the GPU as well: `bytes` holds a SNAP edge list (len bytes, host
+ * memory). Lines are taken as the reference's fgets(line, 512) loop sees them (physical lines cut into chunks of at most
+ * 511 bytes), chunks starting with '#', '%' or '\n' or without two base-10 numbers (strtoul rules: leading white space,
+ * optional sign, saturation at ULONG_MAX) are skipped, a == b is dropped, ids are cast to uint32; then the n < 0 build of
+ * ga_graph_from_pairs. Equal to the host parse followed by ga_graph_from_pairs on every input. ms (may be NULL): device
+ * time from the first parse kernel to the finished CSR (the upload of the text excluded). */
+int ga_graph_from_text(ga_ctx* ctx, const char* bytes, uint64_t len, ga_graph** out, float* ms);
+/* ga_graph_from_text on the whole content of the file at `path` (read into pinned host memory, uploaded once). */
+int ga_graph_load_text(ga_ctx* ctx, const char* path, ga_graph** out, float* ms);
+/* Binary CSR file of this library (no reference counterpart; the fast admission path next to the text loader): a 64-byte
+ * header {magic "GACSR001", n, m, max_out_degree as u64, 32 reserved bytes}, rowptr u32[n+1], col u32[m]. */
+int ga_graph_save(ga_ctx* ctx, const ga_graph* g, const char* path);
+/* Loads a ga_graph_save file. The device checks that the arrays are exactly what the builder makes for their edge set
+ * (rowptr from 0 to m and never decreasing, col < n, rows strictly ascending, every edge oriented by the (degree, id)
+ * rule of graph.h:43-45, max_out_degree as stored); anything else returns GA_ERR_ARG. ms (may be NULL): device time of
+ * upload, checks and src[] derivation. */
+int ga_graph_load(ga_ctx* ctx, const char* path, ga_graph** out, float* ms);
+
+/* induced_subgraph (dsg.h:103-114) on the GPU: the subgraph on {v : keep[v] != 0} with new ids in old-id order, rebuilt
+ * with build_graph_from_edges semantics (orientation by the degrees inside the subgraph, rows sorted, src[],
+ * max_out_degree). keep: n bytes. old_id (may be NULL): capacity n, gets old_id[new] for the n' = ga_graph_info(*out)
+ * kept vertices. ms (may be NULL): device time from the first kernel to the finished CSR. */
+int ga_graph_induced(ga_ctx* ctx, const ga_graph* g, const uint8_t* keep, ga_graph** out, uint32_t* old_id, float* ms);
+/* prune_to_core (dsg.h:636-645) on the GPU: ga_graph_induced on {v : core[v] >= k}; core (host, n entries) are the
+ * graph's core numbers; core_out (may be NULL, capacity n) gets core[old_id[i]], which are the pruned graph's own core
+ * numbers (the containment argument of dsg.h:629-635). */
+int ga_graph_prune_to_core(ga_ctx* ctx, const ga_graph* g, const uint32_t* core, uint64_t k, ga_graph** out, uint32_t* old_id,
+                           uint32_t* core_out, float* ms);
+
 /* ---- triangle counting (kernels.cuh:52-358, gpu_tc.cu:128-228) --------------------------------------------- */
 enum {
     GA_TC_AUTO = 0,        /* the tuned default (currently GA_TC_HASH) */
@@ -166,6 +195,10 @@ int ga_core_run_find(ga_core* c, uint64_t k);        /* incremental only */
 int ga_core_run_update(ga_core* c);                  /* incremental only */
 int ga_core_run_retire(ga_core* c);                  /* incremental only */
 int ga_core_download(ga_core* c, uint32_t* core);
+/* prune_to_core (dsg.h:636-645) on the core numbers a finished decomposition holds on the device (no host round trip of
+ * core[]; gpu_dsg.cu:318-321 downloads them and prunes on the host). Arguments as ga_graph_prune_to_core; the new graph
+ * belongs to the state's context. */
+int ga_core_prune(ga_core* c, uint64_t k, ga_graph** out, uint32_t* old_id, uint32_t* core_out, float* ms);
 /* Whole decomposition with the level loop on the device; stats = {rounds, kmax}. T_full as in ga_peel_run. */
 int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms);
 

@@ -58,6 +58,7 @@ def olib():
             "orc_core_level_density": (None, [_vp, _u32p, ctypes.c_uint32, _u64p, _u64p]),
             "orc_edge_order": (None, [_vp, _u32p]),
             "orc_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
+            "orc_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
             "orc_brute_participation": (ctypes.c_uint64, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u8p, _u32p]),
             "orc_brute_triangles": (ctypes.c_uint64, [ctypes.c_uint32, _u32p, ctypes.c_uint64]),
             "orc_brute_densest": (None, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u64p, _u64p]),
@@ -99,6 +100,8 @@ def rlib():
             "ref_edge_order": (None, [_vp, _u32p]),
             "ref_undirected": (None, [_vp, _u32p, _u32p]),
             "ref_peel_arith_ok": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
+            "ref_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
+            "ref_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -174,6 +177,23 @@ class OGraph:
         c = np.zeros(max(2 * self.m, 1), np.uint32)
         olib().orc_undirected(self._h, _p(r, _u32p), _p(c, _u32p))
         return r, c[: 2 * self.m]
+
+    def induced(self, keep):
+        """induced_subgraph (dsg.h:103-114): (OGraph, old_id)."""
+        k = np.ascontiguousarray(keep, dtype=np.uint8)
+        old = np.zeros(max(self.n, 1), np.uint32)
+        n2 = ctypes.c_uint32()
+        h = olib().orc_induced_subgraph(self._h, _p(k, _u8p), _p(old, _u32p), ctypes.byref(n2))
+        return OGraph(h), old[: n2.value]
+
+    def prune_to_core(self, core, k):
+        """prune_to_core (dsg.h:636-645): (OGraph, old_id, core_out)."""
+        c = np.ascontiguousarray(core, dtype=np.uint32)
+        old = np.zeros(max(self.n, 1), np.uint32)
+        cout = np.zeros(max(self.n, 1), np.uint32)
+        n2 = ctypes.c_uint32()
+        h = olib().orc_prune_to_core(self._h, _p(c, _u32p), int(k), _p(old, _u32p), _p(cout, _u32p), ctypes.byref(n2))
+        return OGraph(h), old[: n2.value], cout[: n2.value]
 
     def core(self):
         c = np.zeros(max(self.n, 1), np.uint32)
@@ -262,6 +282,21 @@ class RGraph:
         if not rlib().ref_batch_peel(self._h, num, den, 1 if omp else 0, _p(st, _u64p), _p(rr, _u32p)):
             raise ValueError("reference batch peel failed")
         return int(st[0]), int(st[1]), int(st[2]), int(st[3]), rr[: self.n]
+
+    def induced(self, keep):
+        k = np.ascontiguousarray(keep, dtype=np.uint8)
+        old = np.zeros(max(self.n, 1), np.uint32)
+        n2 = ctypes.c_uint32()
+        h = rlib().ref_induced_subgraph(self._h, _p(k, _u8p), _p(old, _u32p), ctypes.byref(n2))
+        return RGraph(h), old[: n2.value]
+
+    def prune_to_core(self, core, k):
+        c = np.ascontiguousarray(core, dtype=np.uint32)
+        old = np.zeros(max(self.n, 1), np.uint32)
+        cout = np.zeros(max(self.n, 1), np.uint32)
+        n2 = ctypes.c_uint32()
+        h = rlib().ref_prune_to_core(self._h, _p(c, _u32p), int(k), _p(old, _u32p), _p(cout, _u32p), ctypes.byref(n2))
+        return RGraph(h), old[: n2.value], cout[: n2.value]
 
     def core(self):
         c = np.zeros(max(self.n, 1), np.uint32)

@@ -454,6 +454,16 @@ inline void induced_subgraph(const Graph& g, const std::vector<uint8_t>& keep, G
     build_graph_from_edges((uint32_t)old_id.size(), e, out);
 }
 
+// prune_to_core (dsg.h:636-645): the induced subgraph on {v : core[v] >= k}; core_out[i] = core[old_id[i]].
+inline void prune_to_core(const Graph& g, const std::vector<uint32_t>& core, uint64_t k, Graph& out, std::vector<uint32_t>& old_id,
+                          std::vector<uint32_t>& core_out) {
+    std::vector<uint8_t> keep(g.n);
+    for (uint32_t v = 0; v < g.n; v++) keep[v] = (uint64_t)core[v] >= k;
+    induced_subgraph(g, keep, out, old_id);
+    core_out.resize(old_id.size());
+    for (size_t i = 0; i < old_id.size(); i++) core_out[i] = core[old_id[i]];
+}
+
 // ---- brute-force oracles (independent of the CSR algorithms above) ----
 
 // Bit-set adjacency: triangle count and per-vertex counts of the alive-induced subgraph, O(n^2 * n/64).

@@ -151,6 +151,17 @@ orc_graph* orc_induced_subgraph(const orc_graph* h, const uint8_t* keep, uint32_
     return o;
 }
 
+// prune_to_core on caller-given core numbers (n entries); old_id and core_out have capacity n.
+orc_graph* orc_prune_to_core(const orc_graph* h, const uint32_t* core, uint64_t k, uint32_t* old_id, uint32_t* core_out, uint32_t* n_out) {
+    orc_graph* o = new orc_graph;
+    std::vector<uint32_t> c(core, core + h->g.n), ids, co;
+    prune_to_core(h->g, c, k, o->g, ids, co);
+    std::copy(ids.begin(), ids.end(), old_id);
+    std::copy(co.begin(), co.end(), core_out);
+    *n_out = (uint32_t)ids.size();
+    return o;
+}
+
 // Brute force on an undirected unique loop-free pair list.
 uint64_t orc_brute_participation(uint32_t n, const uint32_t* pairs, uint64_t k, const uint8_t* alive, uint32_t* tri) {
     EdgeList e(k);

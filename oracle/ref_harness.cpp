@@ -93,6 +93,28 @@ void ref_undirected(const ref_graph* h, uint32_t* urow, uint32_t* ucol) {
     std::copy(c.begin(), c.end(), ucol);
 }
 
+// Reference induced_subgraph (dsg.h:103-114); old_id has capacity n.
+ref_graph* ref_induced_subgraph(const ref_graph* h, const uint8_t* keep, uint32_t* old_id, uint32_t* n_out) {
+    ref_graph* o = new ref_graph;
+    std::vector<uint8_t> k(keep, keep + h->g.n);
+    std::vector<uint32_t> ids;
+    induced_subgraph(h->g, k, o->g, ids);
+    std::copy(ids.begin(), ids.end(), old_id);
+    *n_out = (uint32_t)ids.size();
+    return o;
+}
+
+// Reference prune_to_core (dsg.h:636-645); old_id and core_out have capacity n.
+ref_graph* ref_prune_to_core(const ref_graph* h, const uint32_t* core, uint64_t k, uint32_t* old_id, uint32_t* core_out, uint32_t* n_out) {
+    ref_graph* o = new ref_graph;
+    std::vector<uint32_t> c(core, core + h->g.n), ids, co;
+    prune_to_core(h->g, c, k, o->g, ids, co);
+    std::copy(ids.begin(), ids.end(), old_id);
+    std::copy(co.begin(), co.end(), core_out);
+    *n_out = (uint32_t)ids.size();
+    return o;
+}
+
 int ref_peel_arith_ok(uint32_t n, uint64_t T0, uint64_t num, uint64_t den) { return peel_arith_ok(n, T0, num, den) ? 1 : 0; }
 
 }  // extern "C"

@@ -75,3 +75,26 @@ def random_graph(rng, n, p, dup=0.0, loops=0):
     if len(pairs):
         pairs = pairs[rng.permutation(len(pairs))]
     return pairs.reshape(-1, 2)
+
+
+# SNAP text corner cases for load_graph's fgets(512) + strtoul loop (graph.h:71-89); shared with the GPU ingest tests.
+# Ids that cast to 0xFFFFFFFF (a '-1', or any overflow, which strtoul saturates to ULONG_MAX) are not among them: the
+# reference computes n = max_id + 1 in uint32, which wraps to 0 and is undefined there; the GPU loader rejects them.
+# Cast ids are paired with a larger plain id, because the reference takes max_id from the larger endpoint before the
+# cast, which can leave a vertex id >= n otherwise.
+TEXT_CASES = {
+    "empty": b"",
+    "comments only": b"# a\n% b\n\n",
+    "no final newline": b"1 2\n2 3\n3 1",
+    "crlf and tabs": b"1\t2\r\n2 \t 3\r\n\r\n3\t1\r\n",
+    "leading space, signs": b"  4 5\n+6 +7\n\t8\t9\n+4 -0\n - 3\n-18446744073709551615 3\n",
+    "one number, junk": b"7\n8 x\nx 8\n9 10 11\n12 13junk\n#14 15\n %16 17\n",
+    "loops and duplicates": b"5 5\n1 2\n2 1\n1 2\n",
+    "ids beyond 2^32 (cast)": b"4294967297 3\n4294967300 2\n18446744069414584321 5\n0 9\n",
+    "nul bytes": b"1 2\n3\x00 4\n5 \x006\n\x007 8\n9 10\n",
+    "long line split by fgets": b"1 2" + b" " * 600 + b"3 4\n5 6\n",
+    "number cut at byte 511": b" " * 509 + b"12 34\n1 2\n",
+    "pair cut at byte 511": b" " * 505 + b"21 43 65 87\n",
+    "line of exactly 511 and 512 bytes": b"1 2" + b" " * 507 + b"\n" + b"3 4" + b" " * 508 + b"\n9 8\n",
+    "very long lines": (b"7 9 " + b"0" * 1500 + b"11 12 \n") * 3 + b"#" + b" " * 700 + b"\n",
+}

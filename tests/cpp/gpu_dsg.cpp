@@ -2,9 +2,12 @@
 //   usage: gpu_dsg graph.txt [--repeats N] [--with-recount] [--no-burnin] [--verbose]
 //   rows impl,graph,n,m,triangles,vertices,density,time_ms,h2d_ms,total_ms,rounds,kmax for cpu_core_decomp (Alg 3),
 //   gpu_core_decomp (incremental host loop, one sync per round), gpu_core_decomp_fused (ga_core_run) and, with
-//   --with-recount, gpu_core_decomp_recount. Rows = the kmax-core. Printed only if every GPU core array equals
-//   CPU Alg 3 element-wise and the round counts of all GPU designs agree; exit 1 otherwise.
-// The paper pipeline after the decomposition (greedy, pruning, CoreExact) is not part of this build.
+//   --with-recount, gpu_core_decomp_recount. Rows = the kmax-core. Then prune_kcore (gpu_dsg.cu:312-328): lower bound =
+//   the kmax-core density, keep core >= ceil(lb) on the device core numbers (ga_core_prune), row n,m = the pruned graph,
+//   rounds = ceil(lb). Printed only if every GPU core array equals CPU Alg 3 element-wise, the round counts of all GPU
+//   designs agree, and the pruned graph equals the oracle's prune_to_core with its own core numbers equal to the
+//   restricted ones; exit 1 otherwise.
+// The paper pipeline after pruning (greedy, CoreExact) is not part of this build.
 // Integration test: links the CPU oracle as the checker and as the CPU row.
 #include "../../graph_analyzer_amd/host/dsg.h"
 #include "../../oracle/oracle.hpp"
@@ -99,10 +102,44 @@ int main(int argc, char** argv) {
         fprintf(stderr, "MISMATCH round counts: incremental %u, fused %u, recount %u\n", inc.rounds, fused_rounds, rc_rounds);
         ok = false;
     }
+    // ---- lower bound and pruning (gpu_dsg.cu:312-328); the lower bound is the kmax-core density (no CPU greedy here) ----
+    uint64_t kprune = 0, T_pruned = 0;
+    double t_prune = 1e300;
+    Graph gp;
+    if (ok) {
+        kprune = kverts ? (ktri + kverts - 1) / kverts : 0;
+        std::vector<uint32_t> old_id, core_p;
+        for (int i = -1; i < repeats; i++) {
+            gaa::check(ctx, ga_ctx_sync(ctx), "sync");
+            const double ts = now_ms();
+            gaa::DeviceGraph dp = bi.prune(kprune, old_id, core_p);
+            const double ms = now_ms() - ts;
+            if (i >= 0) t_prune = std::min(t_prune, ms);
+            if (i < 0) {
+                dp.download(gp);
+                T_pruned = gaa::gpu_triangles(dp);
+            }
+        }
+        orc::Graph op;
+        std::vector<uint32_t> oold, ocore;
+        orc::prune_to_core(og, core_cpu, kprune, op, oold, ocore);
+        std::vector<uint32_t> own;
+        if (gp.n != op.n || gp.m != op.m || gp.rowptr != op.rowptr || gp.col != op.col || gp.src != op.src || old_id != oold || core_p != ocore) {
+            fprintf(stderr, "MISMATCH: GPU prune_to_core differs from the oracle's\n");
+            ok = false;
+        } else if (!orc::core_decomposition(op, own) || own != core_p || T_pruned != orc::triangles_serial(op)) {
+            fprintf(stderr, "MISMATCH: restricted core numbers != core numbers of the pruned graph (or its count differs)\n");
+            ok = false;
+        } else {
+            fprintf(stderr, "lower bound %llu/%llu -> keep core >= %llu: pruned graph n=%u m=%u (of n=%u m=%u)\n", (unsigned long long)ktri,
+                    (unsigned long long)kverts, (unsigned long long)kprune, gp.n, gp.m, og.n, og.m);
+        }
+    }
     if (!ok) { fprintf(stderr, "VERIFICATION FAILED - no timing rows printed\n"); return 1; }
     rows.push_back(dsg_row("gpu_core_decomp", path, og.n, og.m, ktri, kverts, best_inc, 0, inc.rounds, kmax));
     rows.push_back(dsg_row("gpu_core_decomp_fused", path, og.n, og.m, ktri, kverts, best_fused, 0, fused_rounds, kmax));
     if (recount) rows.push_back(dsg_row("gpu_core_decomp_recount", path, og.n, og.m, ktri, kverts, best_rc, 0, rc_rounds, kmax));
+    rows.push_back(dsg_row("prune_kcore", path, gp.n, gp.m, T_pruned, gp.n, t_prune, 0, kprune, kmax));
     for (auto& r : rows) puts(r.c_str());
     fflush(stdout);
     fprintf(stderr, "GPU core numbers verified element-wise against CPU Alg 3 (kmax %u, %u rounds)\n", kmax, inc.rounds);

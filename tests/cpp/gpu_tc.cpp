@@ -2,7 +2,8 @@
 //   usage: gpu_tc <edge_list.txt> [repeats] [--no-burnin]
 //   CSV rows impl,graph,n,m,triangles,kernel_ms,h2d_ms,gpu_total_ms, one per variant, printed only when every launch
 //   of every variant equals the serial CPU count; exit 1 otherwise, exit 2 on usage errors.
-// The graph is built on the GPU from the text file's pairs and must be byte-identical to the CPU oracle's CSR.
+// The graph is loaded on the GPU from the text file (load_graph: parse, de-duplication, orientation, sort all on the
+// device) and must be byte-identical to the CPU oracle's CSR built from the host parse of the same file.
 // kernel_ms = best of `repeats` device-timed launches after one untimed warm-up; h2d_ms = upload of rowptr + col
 // (the CSR a caller would hand over); gpu_total_ms = kernel_ms + h2d_ms (+ plan build for the planned variants).
 // Integration test: links the CPU oracle as the checker.
@@ -30,9 +31,11 @@ int main(int argc, char** argv) {
 
     std::vector<uint32_t> pairs;
     if (!gaa::parse_edge_file(path, pairs)) return 1;
-    gaa::DeviceGraph dg = gaa::DeviceGraph::from_pairs(ctx, pairs.data(), pairs.size() / 2, -1);
-    fprintf(stderr, "GPU build n=%u m=%u max_out_deg=%u in %.3f ms (dedup + orientation + sort)\n", dg.n(), dg.m(), dg.max_out_degree(),
-            dg.build_ms());
+    gaa::DeviceGraph dg;
+    const double l0 = now_ms();
+    if (!load_graph_device(path, dg, ctx)) return 1;
+    fprintf(stderr, "GPU load_graph n=%u m=%u max_out_deg=%u in %.3f ms (file read, upload, parse, dedup, orientation, sort)\n", dg.n(),
+            dg.m(), dg.max_out_degree(), now_ms() - l0);
     orc::Graph og;
     orc::graph_from_pairs(pairs.data(), pairs.size() / 2, og);
     Graph g;

@@ -7,7 +7,8 @@ Inputs: the literal zoo of the reference's tiny suite, and seeded random graphs 
 (Erdos-Renyi with 30% duplicate reversed edges and a self loop, gen_test_graph.py; preferential attachment,
 gen_powerlaw.py), written as SNAP text so the reference's own load_graph parses them. Outputs per graph: the oriented
 CSR, T, per-vertex and per-edge counts, batch-peel results for five eps values, Alg-3 core numbers with kmax and the
-kmax-core density, and the v9 edge order.
+kmax-core density, the v9 edge order, and prune_to_core (induced subgraph on core >= k) at k in {0, 1, ceil(kmax/2), kmax,
+kmax + 1}.
 """
 import json
 import os
@@ -58,6 +59,12 @@ def record(name, raw_pairs, tmpdir):
     for num, den in EPS:
         bn, bd, br, rounds, rr = g.batch_peel(num, den)
         peel[f"{num}/{den}"] = {"best_num": bn, "best_den": bd, "best_round": br, "rounds": rounds, "removed_round": rr.tolist()}
+    prune = []
+    for k in sorted({0, 1, (kmax + 1) // 2, kmax, kmax + 1}):
+        pg, old, cout = g.prune_to_core(core, k)
+        pr, pc, _ = pg.arrays()
+        prune.append({"k": k, "n": pg.n, "m": pg.m, "max_out_degree": pg.max_out_degree, "rowptr": pr.tolist(), "col": pc.tolist(),
+                      "old_id": old.tolist(), "core_out": cout.tolist()})
     return {
         "name": name,
         "raw_pairs": np.asarray(raw_pairs, dtype=np.uint32).tolist(),
@@ -67,6 +74,7 @@ def record(name, raw_pairs, tmpdir):
         "core": core.tolist(), "kmax": kmax, "kmax_core": [kt, kv],
         "edge_order": g.edge_order().tolist(),
         "peel": peel,
+        "prune": prune,
     }
 
 

@@ -1,0 +1,80 @@
+"""The C++ drivers (tests/cpp: gpu_tc, gpu_densest, gpu_dsg) end to end on the GPU: they parse a SNAP text file with the
+reference's loader rules, run the drop-in host API over the C ABI, verify every result against the linked CPU oracle and
+print the reference's CSV rows (gpu_tc.cu:14-21, gpu_densest.cu:16-25, gpu_dsg.cu:27-46) only when all of it matches."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from graph_analyzer_amd import gen
+
+BIN = os.path.join(os.path.dirname(__file__), "cpp", "bin")
+
+
+def run(args, timeout=240):
+    r = subprocess.run([os.path.join(BIN, args[0]), *args[1:]], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=timeout)
+    return r.returncode, r.stdout, r.stderr
+
+
+@pytest.fixture(scope="module")
+def graph_file(tmp_path_factory):
+    raw = gen.powerlaw_small()
+    path = tmp_path_factory.mktemp("drv") / "powerlaw_small.txt"
+    with open(path, "w") as f:
+        f.write("# seeded power-law graph\n% comment\n\n")
+        np.savetxt(f, raw, fmt="%d", delimiter="\t")
+    return str(path), oracle.OGraph.load_text(str(path))
+
+
+def rows(out):
+    return [line.split(",") for line in out.strip().splitlines()]
+
+
+def test_driver_usage_errors():
+    """Exit 2 and no rows on usage errors (checked before any device work, so this runs without a GPU)."""
+    for args in (["gpu_tc"], ["gpu_densest"], ["gpu_densest", "g.txt", "1"], ["gpu_dsg"], ["gpu_dsg", "a.txt", "b.txt"]):
+        rc, out, _ = run(args)
+        assert rc == 2 and out == "", args
+
+
+@pytest.mark.gpu
+def test_gpu_tc_driver(graph_file):
+    path, og = graph_file
+    rc, out, err = run(["gpu_tc", path, "2", "--no-burnin"])
+    assert rc == 0, err[-2000:]
+    rs = rows(out)
+    assert len(rs) >= 7 and all(len(r) == 8 for r in rs)
+    T = og.triangles()
+    for r in rs:
+        assert r[1] == path and (int(r[2]), int(r[3]), int(r[4])) == (og.n, og.m, T), r
+
+
+@pytest.mark.gpu
+def test_gpu_densest_driver(graph_file):
+    path, og = graph_file
+    rc, out, err = run(["gpu_densest", path, "1", "20", "--no-burnin", "--repeats", "2"])
+    assert rc == 0, err[-2000:]
+    bn, bd, br, rounds, _ = og.batch_peel(1, 20)
+    rs = rows(out)
+    assert {r[0] for r in rs} >= {"gpu_batch_peel", "gpu_batch_peel_fused"}
+    for r in rs:
+        assert (int(r[4]), int(r[5]), int(r[7])) == (bn, bd, rounds), r
+
+
+@pytest.mark.gpu
+def test_gpu_dsg_driver_with_prune(graph_file):
+    path, og = graph_file
+    rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "2", "--with-recount"])
+    assert rc == 0, err[-2000:]
+    rs = {r[0]: r for r in rows(out)}
+    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "prune_kcore"}
+    core, kmax = og.core()
+    kt, kv = og.core_level_density(core, kmax)
+    for name in ("cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused"):
+        assert (int(rs[name][4]), int(rs[name][5]), int(rs[name][11])) == (kt, kv, kmax)
+    k = (kt + kv - 1) // kv
+    sub, _, _ = og.prune_to_core(core, k)
+    p = rs["prune_kcore"]
+    assert (int(p[2]), int(p[3]), int(p[4]), int(p[10]), int(p[11])) == (sub.n, sub.m, sub.triangles(), k, kmax)

@@ -50,10 +50,44 @@ def test_full_size_support_and_participation(ctx, name):
     g.close()
 
 
+@pytest.mark.parametrize("name", ["rmat21", "powerlaw_communities"])
+def test_full_size_ingest(ctx, name, tmp_path):
+    """load_graph with the text parse on the GPU, and the binary CSR file, on the full-size graphs."""
+    g, og, T = _graph(ctx, name)
+    path = str(tmp_path / "g.txt")
+    gen.write_text(_cache[name][0], path, name)
+    t = ga.DeviceGraph.load_text(ctx, path)
+    for a, b in zip(t.download(), og.arrays()):
+        assert np.array_equal(a, b)
+    print(f"{name}: text parse + build {t.build_ms:.2f} ms", flush=True)
+    bpath = str(tmp_path / "g.gacsr")
+    t.save(bpath)
+    h = ga.DeviceGraph.load(ctx, bpath)
+    for a, b in zip(h.download(), og.arrays()):
+        assert np.array_equal(a, b)
+    assert h.count(ga.TC_HASH)[0] == T
+    print(f"{name}: binary load + checks {h.build_ms:.2f} ms", flush=True)
+    for x in (t, h, g):
+        x.close()
+
+
 def test_full_size_core(ctx):
     """(k,triangle)-core numbers of the 34.7M-edge graph against Alg 3 (dsg.h:116-166)."""
     g, og, T = _graph(ctx, "powerlaw_communities")
     core, kmax = og.core()
-    cg, rounds, kg = g.core_decomposition(T_full=T)
+    c = ga.Core(g, True)
+    cg, rounds, kg = c.run(T_full=T)
     assert kg == kmax and np.array_equal(cg, core)
+    # prune_to_core at ceil(kmax-core density) on the device core numbers (gpu_dsg.cu:318-328)
+    kt, kv = og.core_level_density(core, kmax)
+    k = (kt + kv - 1) // kv
+    sub, old, cout = c.prune(k)
+    osub, oold, ocout = og.prune_to_core(core, k)
+    assert np.array_equal(old, oold) and np.array_equal(cout, ocout)
+    r, col, src = sub.download()
+    orr, oc, os_ = osub.arrays()
+    assert np.array_equal(r, orr) and np.array_equal(col, oc) and np.array_equal(src, os_)
+    print(f"prune at k={k}: n {g.n} -> {sub.n}, m {g.m} -> {sub.m} in {sub.build_ms:.2f} ms", flush=True)
+    sub.close()
+    c.close()
     g.close()

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import clique, random_graph, zoo
+from conftest import TEXT_CASES, clique, random_graph, zoo
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
 
@@ -60,6 +60,20 @@ def test_text_loader_matches_reference(tmp_path):
         assert np.array_equal(a, b)
 
 
+def test_text_loader_corner_cases_match_reference(tmp_path):
+    """The oracle's loader equals the reference's own load_graph on every text corner case the GPU ingest is tested on."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    for name, text in TEXT_CASES.items():
+        path = tmp_path / "g.txt"
+        path.write_bytes(text)
+        o = oracle.OGraph.load_text(str(path))
+        r = oracle.RGraph.load_text(str(path))
+        assert (o.n, o.m) == (r.n, r.m), name
+        for a, b in zip(o.arrays(), r.arrays()):
+            assert np.array_equal(a, b), name
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_oracle_equals_reference_build(seed):
     """Fresh seeded graphs (not in the fixtures): every oracle function equals the reference build."""
@@ -89,6 +103,76 @@ def test_oracle_equals_reference_build(seed):
     assert np.array_equal(o.edge_order(), r.edge_order())
     for a, b in zip(o.undirected(), r.undirected()):
         assert np.array_equal(a, b)
+
+
+def test_golden_prune(golden):
+    """prune_to_core of the oracle equals the reference's on every fixture graph at k in {0, 1, ceil(kmax/2), kmax, kmax+1}."""
+    cases = 0
+    for rec in golden["graphs"]:
+        raw = np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2)
+        g = oracle.OGraph.from_pairs(raw)
+        core = np.array(rec["core"], dtype=np.uint32)
+        for p in rec["prune"]:
+            pg, old, cout = g.prune_to_core(core, p["k"])
+            r, c, _ = pg.arrays()
+            assert (pg.n, pg.m, pg.max_out_degree) == (p["n"], p["m"], p["max_out_degree"]), (rec["name"], p["k"])
+            assert r.tolist() == p["rowptr"] and c.tolist() == p["col"], (rec["name"], p["k"])
+            assert old.tolist() == p["old_id"] and cout.tolist() == p["core_out"], (rec["name"], p["k"])
+            cases += 1
+    assert cases >= 120
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_induced_and_prune_equal_reference_build(seed):
+    """Fresh seeded graphs and random keep masks: induced_subgraph and prune_to_core equal the reference build."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    rng = np.random.default_rng(2000 + seed)
+    n = int(rng.integers(80, 600))
+    raw = random_graph(rng, n, float(rng.uniform(0.02, 0.15)), dup=0.2, loops=1)
+    o = oracle.OGraph.from_pairs(raw)
+    r = oracle.RGraph.build(o.n, oracle.canonical_unique(raw))
+    for p in (1.0, 0.7, 0.3, 0.0):
+        keep = (rng.random(o.n) < p).astype(np.uint8)
+        (og, oo), (rg, ro) = o.induced(keep), r.induced(keep)
+        assert np.array_equal(oo, ro) and (og.n, og.m) == (rg.n, rg.m)
+        for a, b in zip(og.arrays(), rg.arrays()):
+            assert np.array_equal(a, b)
+    core, kmax = o.core()
+    for k in (0, 1, kmax // 3, kmax, kmax + 1):
+        op, oo, oc = o.prune_to_core(core, k)
+        rp, ro, rc = r.prune_to_core(core, k)
+        assert np.array_equal(oo, ro) and np.array_equal(oc, rc)
+        for a, b in zip(op.arrays(), rp.arrays()):
+            assert np.array_equal(a, b)
+
+
+def test_prune_invariants():
+    """What the mathematics fixes: keeping every vertex rebuilds the same CSR; the induced subgraph's T is the alive-masked
+    count of the full graph (count_triangles_in, dsg.h:90-94); the pruned graph's own core numbers equal the restricted
+    ones (the containment argument of dsg.h:629-635, checked by gpu_dsg.cu:325-328); new ids ascend with old ids."""
+    rng = np.random.default_rng(77)
+    for trial in range(6):
+        n = int(rng.integers(100, 500))
+        o = oracle.OGraph.from_pairs(random_graph(rng, n, float(rng.uniform(0.03, 0.12))))
+        full, old = o.induced(np.ones(o.n, np.uint8))
+        assert np.array_equal(old, np.arange(o.n))
+        for a, b in zip(full.arrays(), o.arrays()):
+            assert np.array_equal(a, b)
+        keep = (rng.random(o.n) < 0.6).astype(np.uint8)
+        sub, old = o.induced(keep)
+        assert np.array_equal(old, np.flatnonzero(keep)) and np.all(np.diff(old.astype(np.int64)) > 0)
+        assert sub.triangles() == o.participation(keep)[0]
+        core, kmax = o.core()
+        for k in (1, max(kmax // 2, 1), kmax):
+            pg, old, cout = o.prune_to_core(core, k)
+            assert np.array_equal(old, np.flatnonzero(core >= k))
+            assert np.all(cout >= k)
+            own, _ = pg.core()
+            assert np.array_equal(own, cout), (trial, k)
+    # the empty selection is a graph with no vertices
+    e, old = o.induced(np.zeros(o.n, np.uint8))
+    assert (e.n, e.m, old.size) == (0, 0, 0)
 
 
 def test_closed_forms():

@@ -1,6 +1,6 @@
 """Runs one stage of the path on a named synthetic workload and prints its device time and result.
 
-    python -m graph_analyzer_amd count|peel|core|support [--workload rmat21|powerlaw_communities|powerlaw_small]
+    python -m graph_analyzer_amd count|peel|core|support|ingest|prune [--workload rmat21|powerlaw_communities|powerlaw_small]
                                  [--repeat R] [--num 1 --den 10]
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
     python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
@@ -24,7 +24,7 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
-    ap.add_argument("stage", choices=["count", "peel", "core", "support", "batch", "probe"])
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "ingest", "prune", "batch", "probe"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--num", type=int, default=1)
@@ -58,6 +58,24 @@ def main(argv=None):
         print(f"batched peel {args.num}/{args.den}: {ms:.3f} ms, rounds max {int(st[:, 3].max())}", flush=True)
         b.close()
         return 0
+    if args.stage == "ingest":   # load_graph with the text parse on the GPU, then the binary CSR file
+        import os
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            path, bpath = os.path.join(td, "g.txt"), os.path.join(td, "g.gacsr")
+            gen.write_text(WORKLOADS[args.workload](), path, args.workload)
+            print(f"{args.workload}: {os.path.getsize(path)} bytes of text", flush=True)
+            for _ in range(args.repeat):
+                t0 = time.perf_counter()
+                g = ga.DeviceGraph.load_text(ctx, path)
+                print(f"load_text: n={g.n} m={g.m} {g.build_ms:.3f} ms device, {(time.perf_counter() - t0) * 1e3:.1f} ms wall", flush=True)
+                g.save(bpath)
+                g.close()
+                t0 = time.perf_counter()
+                h = ga.DeviceGraph.load(ctx, bpath)
+                print(f"binary load: {h.build_ms:.3f} ms device, {(time.perf_counter() - t0) * 1e3:.1f} ms wall", flush=True)
+                h.close()
+        return 0
     t0 = time.perf_counter()
     g = ga.DeviceGraph.from_pairs(ctx, WORKLOADS[args.workload]())
     print(f"{args.workload}: n={g.n} m={g.m} max_out={g.max_out_degree} build {g.build_ms:.2f} ms "
@@ -79,6 +97,16 @@ def main(argv=None):
             print(f"peel {args.num}/{args.den}: best {r.best_num}/{r.best_den} round {r.best_round} of {r.rounds}, "
                   f"{p.last_ms:.3f} ms device", flush=True)
             p.close()
+        elif args.stage == "prune":   # prune_to_core at ceil(kmax-core density) on the device core numbers
+            c = ga.Core(g, True)
+            core, rounds, kmax = c.run(T_full=T)
+            Tk, _ = g.vertex_participation((core >= kmax).astype(np.uint8))
+            verts = int((core >= kmax).sum())
+            k = (Tk + verts - 1) // verts if verts else 0
+            sub, _, _ = c.prune(k)
+            print(f"prune k={k}: n {g.n} -> {sub.n}, m {g.m} -> {sub.m}, {sub.build_ms:.3f} ms device", flush=True)
+            sub.close()
+            c.close()
         else:
             c = ga.Core(g, True)
             core, rounds, kmax = c.run(T_full=T)

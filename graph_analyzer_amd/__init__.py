@@ -102,6 +102,10 @@ def lib() -> ctypes.CDLL:
             "ga_graph_induced": (ctypes.c_int, [_vp, _vp, _u8p, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_prune_to_core": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p,
                                                       ctypes.POINTER(ctypes.c_float)]),
+            "ga_truss_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+            "ga_truss_destroy": (None, [_vp]),
+            "ga_truss_run": (ctypes.c_int, [_vp, _u32p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_truss_download": (ctypes.c_int, [_vp, _u32p]),
             "ga_core_prune": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
@@ -401,6 +405,14 @@ class DeviceGraph:
         finally:
             p.close()
 
+    def truss_decomposition(self):
+        """k-truss number of every oriented edge; returns (truss, rounds, kmax)."""
+        t = Truss(self)
+        try:
+            return t.run()
+        finally:
+            t.close()
+
     def core_decomposition(self, incremental: bool = True, fused: bool = True, T_full: Optional[int] = None):
         """(k,triangle)-core numbers; returns (core, rounds, kmax)."""
         c = Core(self, incremental)
@@ -414,6 +426,38 @@ class DeviceGraph:
     def close(self):
         if self._h:
             lib().ga_graph_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Truss:
+    """k-truss decomposition state (ga_truss): edge-support peeling with the level rule of the core loops."""
+
+    def __init__(self, g: DeviceGraph):
+        self.g = g
+        self.ctx = g.ctx
+        self._h = _vp()
+        self.ctx.check(lib().ga_truss_create(self.ctx.handle, g.handle, ctypes.byref(self._h)), "truss_create")
+        self.last_ms = 0.0
+
+    def run(self):
+        """Returns (truss per oriented edge, rounds, kmax)."""
+        stats = np.zeros(2, np.uint32)
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_truss_run(self._h, _ptr(stats, _u32p), ctypes.byref(ms)), "truss_run")
+        self.last_ms = ms.value
+        t = np.zeros(max(self.g.m, 1), np.uint32)
+        self.ctx.check(lib().ga_truss_download(self._h, _ptr(t, _u32p)), "truss_download")
+        return t[: self.g.m], int(stats[0]), int(stats[1])
+
+    def close(self):
+        if self._h:
+            lib().ga_truss_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

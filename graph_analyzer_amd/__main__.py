@@ -1,6 +1,6 @@
 """Runs one stage of the path on a named synthetic workload and prints its device time and result.
 
-    python -m graph_analyzer_amd count|peel|core|support|ingest|prune [--workload rmat21|powerlaw_communities|powerlaw_small]
+    python -m graph_analyzer_amd count|peel|core|support|truss|ingest|prune [--workload rmat21|powerlaw_communities|powerlaw_small]
                                  [--repeat R] [--num 1 --den 10]
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
     python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
@@ -24,7 +24,7 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
-    ap.add_argument("stage", choices=["count", "peel", "core", "support", "ingest", "prune", "batch", "probe"])
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "batch", "probe"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--num", type=int, default=1)
@@ -97,6 +97,11 @@ def main(argv=None):
             print(f"peel {args.num}/{args.den}: best {r.best_num}/{r.best_den} round {r.best_round} of {r.rounds}, "
                   f"{p.last_ms:.3f} ms device", flush=True)
             p.close()
+        elif args.stage == "truss":
+            tr = ga.Truss(g)
+            t, rounds, kmax = tr.run()
+            print(f"truss: rounds {rounds} kmax {kmax} {tr.last_ms:.3f} ms device", flush=True)
+            tr.close()
         elif args.stage == "prune":   # prune_to_core at ceil(kmax-core density) on the device core numbers
             c = ga.Core(g, True)
             core, rounds, kmax = c.run(T_full=T)

@@ -1,0 +1,416 @@
+// ga_truss.hip - k-truss decomposition on the GPU (edge-support peeling). The reference has no k-truss; it extends the
+// per-edge triangle supports (dsg.h:62-94) the way the (k,triangle)-core extends per-vertex participation, and uses the
+// same level rule as the core loops (dsg.h:694-751): level = max(level, least support of a present edge); every
+// present edge whose support is at most the level leaves in this round with truss = level + 2.
+//
+// Set-up: per-edge supports from the hash count (F_SUP); an undirected adjacency whose entries carry the oriented edge
+// id (keys (u << nb | w) sorted with the edge id as value, rows by counts + scan).
+// One round, all on the device, over a list of the present edges (all edges until the first compaction):
+//   k_tr_find     present edges with support <= level -> state 2, appended to the frontier with one atomic per wave;
+//                 the least support of the other present edges is kept (one atomicMin per wave)
+//   k_tr_find2    only when that frontier is empty: the level rises to that least support and the edges at it form
+//                 the frontier (a second pass over the list only when the level changes)
+//   k_tr_update   one wave per frontier edge (u, v): walks the shorter of N(u), N(v) and binary-searches each w in the
+//                 other; for a triangle whose two other edges are present: both present and not in the frontier ->
+//                 both lose one; one of them in the frontier -> the other loses one, decremented only by the frontier
+//                 edge of smaller id; both in the frontier -> nothing (the triangle is gone with all of them)
+//   k_tr_retire   frontier edges -> state 0, truss = level + 2
+//   k_tr_compact  when the present edges are at most half the list: the present ones are copied to the other list
+//   k_tr_end      present count, level, round count, list switch, sanity (empty frontier, more removed than present),
+//                 done flag
+// Rounds are enqueued in blocks that double up to 64 with one readback per block; every kernel returns at once when
+// done is set, so the rounds past the end cost only their launches.
+#include "ga_internal.hpp"
+
+namespace {
+
+struct DevBuf {   // scoped device allocation
+    void* p = nullptr;
+    ~DevBuf() { ga_free(p); }
+};
+
+inline int bits_for(uint32_t n) {
+    int b = 1;
+    while (b < 32 && ((uint64_t)1 << b) < (uint64_t)n) b++;
+    return b;
+}
+
+}  // namespace
+
+struct TrussDev {
+    unsigned long long present;   // edges still present
+    uint32_t level;               // support level (truss = level + 2)
+    uint32_t min_rest;            // least support of the present edges above the level (this round)
+    uint32_t nf;                  // frontier size at the current level (k_tr_find)
+    uint32_t nf2;                 // frontier size after a level rise (k_tr_find2)
+    uint32_t rounds;
+    uint32_t done;
+    uint32_t error;               // 1: empty frontier with edges present, 2: frontier larger than the present set,
+                                  // 3: a compacted list that does not hold exactly the present edges
+    uint32_t np;                  // length of the present list
+    uint32_t ident;               // 1: the present list is the identity (no compaction yet)
+    uint32_t cur;                 // which of the two list buffers is current
+    uint32_t np_next;             // length of the list being compacted into
+    uint32_t compact;             // 1: this round compacts
+};
+
+// level of the round: the current one if it has a frontier, else the least support above it
+__device__ __forceinline__ uint32_t round_level(const TrussDev* st) { return st->nf ? st->level : st->min_rest; }
+
+struct ga_truss {
+    ga_ctx* ctx = nullptr;
+    ga_graph* g = nullptr;
+    uint32_t* sup = nullptr;      // current support per oriented edge
+    uint32_t* truss = nullptr;
+    uint32_t* frontier = nullptr;
+    uint32_t* plist[2] = {nullptr, nullptr};   // lists of present edges (compacted when half are gone)
+    uint8_t* state = nullptr;     // 0 removed, 1 present, 2 frontier of this round
+    uint32_t* urow = nullptr;     // undirected adjacency with edge ids
+    uint32_t* uadj = nullptr;
+    uint32_t* ueid = nullptr;
+    TrussDev* st = nullptr;
+    uint32_t kmax = 0, rounds = 0;
+};
+
+namespace {
+
+__global__ void k_tr_keys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col, uint32_t m, int nb,
+                          unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = src[e], v = col[e];
+        keys[2 * e] = ((unsigned long long)u << nb) | v;
+        keys[2 * e + 1] = ((unsigned long long)v << nb) | u;
+        vals[2 * e] = (uint32_t)e;
+        vals[2 * e + 1] = (uint32_t)e;
+    }
+}
+
+__global__ void k_tr_unpack(const unsigned long long* __restrict__ keys, uint64_t cnt, int nb, uint32_t* __restrict__ adj,
+                            uint32_t* __restrict__ rowcnt) {
+    const unsigned long long mask = (1ull << nb) - 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long x = keys[i];
+        adj[i] = (uint32_t)(x & mask);
+        atomicAdd(&rowcnt[(uint32_t)(x >> nb)], 1u);
+    }
+}
+
+__global__ void k_tr_init(TrussDev* st, uint8_t* __restrict__ state, uint32_t m) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) state[e] = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->present = m;
+        st->level = 0;
+        st->min_rest = 0xFFFFFFFFu;
+        st->nf = 0;
+        st->nf2 = 0;
+        st->rounds = 0;
+        st->done = m == 0;
+        st->error = 0;
+        st->np = m;
+        st->ident = 1;
+        st->cur = 0;
+        st->np_next = 0;
+        st->compact = 0;
+    }
+}
+
+struct Lists {
+    uint32_t* p[2];
+};
+
+// Frontier at the current level, and the least support of the present edges above it.
+__global__ void k_tr_find(const uint32_t* __restrict__ sup, uint8_t* __restrict__ state, Lists L, TrussDev* st,
+                          uint32_t* __restrict__ frontier) {
+    if (st->done) return;
+    const uint32_t level = st->level, np = st->np;
+    const uint32_t* list = L.p[st->cur];
+    const bool ident = st->ident;
+    uint32_t mn = 0xFFFFFFFFu;
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t base = wave0 * 64; base < np; base += nwaves * 64) {   // wave-uniform loop
+        const uint32_t i = base + lane_id();
+        bool f = false;
+        uint32_t e = 0;
+        if (i < np) {
+            e = ident ? i : list[i];
+            if (state[e] == 1) {
+                const uint32_t s = sup[e];
+                f = s <= level;
+                if (!f) mn = min(mn, s);
+            }
+        }
+        const uint64_t mask = __ballot(f);
+        if (!mask) continue;
+        uint32_t at = 0;
+        if (lane_id() == 0) at = atomicAdd(&st->nf, (uint32_t)__popcll(mask));
+        at = __shfl(at, 0, 64);
+        if (f) {
+            state[e] = 2;
+            frontier[at + mask_rank(mask)] = e;
+        }
+    }
+    mn = wave_min_u32(mn);
+    if (lane_id() == 0 && mn != 0xFFFFFFFFu) atomicMin(&st->min_rest, mn);
+}
+
+// Only when k_tr_find found nothing at the current level: the level rises to the least support, whose edges leave.
+__global__ void k_tr_find2(const uint32_t* __restrict__ sup, uint8_t* __restrict__ state, Lists L, TrussDev* st,
+                           uint32_t* __restrict__ frontier) {
+    if (st->done || st->nf) return;
+    const uint32_t level = st->min_rest, np = st->np;
+    const uint32_t* list = L.p[st->cur];
+    const bool ident = st->ident;
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t base = wave0 * 64; base < np; base += nwaves * 64) {   // wave-uniform loop
+        const uint32_t i = base + lane_id();
+        bool f = false;
+        uint32_t e = 0;
+        if (i < np) {
+            e = ident ? i : list[i];
+            f = state[e] == 1 && sup[e] <= level;
+        }
+        const uint64_t mask = __ballot(f);
+        if (!mask) continue;
+        uint32_t at = 0;
+        if (lane_id() == 0) at = atomicAdd(&st->nf2, (uint32_t)__popcll(mask));
+        at = __shfl(at, 0, 64);
+        if (f) {
+            state[e] = 2;
+            frontier[at + mask_rank(mask)] = e;
+        }
+    }
+}
+
+// One wave per frontier edge; frontier entries are taken wave by wave over a fixed grid.
+__global__ void __launch_bounds__(256) k_tr_update(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                                                   const uint32_t* __restrict__ urow, const uint32_t* __restrict__ uadj,
+                                                   const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state,
+                                                   uint32_t* __restrict__ sup, const uint32_t* __restrict__ frontier, const TrussDev* st) {
+    if (st->done) return;
+    const uint32_t nf = st->nf + st->nf2;
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t i = wave0; i < nf; i += nwaves) {
+        const uint32_t e = frontier[i];
+        uint32_t a = src[e], b = col[e];
+        uint32_t ab = urow[a], ae = urow[a + 1], bb = urow[b], be = urow[b + 1];
+        if (ae - ab > be - bb) {   // stream the shorter list, search the longer
+            uint32_t t = ab; ab = bb; bb = t;
+            t = ae; ae = be; be = t;
+        }
+        for (uint32_t p = ab + lane_id(); p < ae; p += 64) {
+            const uint32_t ea = ueid[p];
+            const uint32_t sa = state[ea];
+            if (sa == 0 || ea == e) continue;
+            const uint32_t w = uadj[p];
+            const uint32_t q = lower_bound_u32(uadj, bb, be, w);
+            if (q == be || uadj[q] != w) continue;
+            const uint32_t eb = ueid[q];
+            const uint32_t sb = state[eb];
+            if (sb == 0) continue;
+            if (sa == 1 && sb == 1) {
+                atomicSub(&sup[ea], 1u);
+                atomicSub(&sup[eb], 1u);
+            } else if (sa == 2 && sb == 1) {
+                if (e < ea) atomicSub(&sup[eb], 1u);
+            } else if (sa == 1 && sb == 2) {
+                if (e < eb) atomicSub(&sup[ea], 1u);
+            }
+        }
+    }
+}
+
+__global__ void k_tr_retire(uint8_t* __restrict__ state, uint32_t* __restrict__ truss, const uint32_t* __restrict__ frontier,
+                            TrussDev* st) {
+    if (st->done) return;
+    const uint32_t nf = st->nf + st->nf2, t = round_level(st) + 2u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
+        const uint32_t e = frontier[i];
+        state[e] = 0;
+        truss[e] = t;
+    }
+    // compact this round when what stays present is at most half the list (decided identically by every thread)
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->compact = (st->present - nf) * 2 <= st->np && st->present > nf;
+}
+
+// Copies the present edges of the current list to the other one (after retire, so frontier edges are gone).
+__global__ void k_tr_compact(const uint8_t* __restrict__ state, Lists L, TrussDev* st) {
+    if (st->done || !st->compact) return;
+    const uint32_t np = st->np;
+    const uint32_t* list = L.p[st->cur];
+    uint32_t* out = L.p[st->cur ^ 1u];
+    const bool ident = st->ident;
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t base = wave0 * 64; base < np; base += nwaves * 64) {   // wave-uniform loop
+        const uint32_t i = base + lane_id();
+        uint32_t e = 0;
+        bool k = false;
+        if (i < np) {
+            e = ident ? i : list[i];
+            k = state[e] == 1;
+        }
+        const uint64_t mask = __ballot(k);
+        if (!mask) continue;
+        uint32_t at = 0;
+        if (lane_id() == 0) at = atomicAdd(&st->np_next, (uint32_t)__popcll(mask));
+        at = __shfl(at, 0, 64);
+        if (k) out[at + mask_rank(mask)] = e;
+    }
+}
+
+__global__ void k_tr_end(TrussDev* st) {
+    if (st->done) return;
+    const uint32_t nf = st->nf + st->nf2;
+    if (nf == 0 || nf > st->present) {
+        st->error = nf == 0 ? 1u : 2u;
+        st->done = 1;
+        return;
+    }
+    if (st->compact && st->np_next != st->present - nf) {
+        st->error = 3;
+        st->done = 1;
+        return;
+    }
+    st->level = round_level(st);
+    st->present -= nf;
+    st->rounds++;
+    st->nf = 0;
+    st->nf2 = 0;
+    st->min_rest = 0xFFFFFFFFu;
+    if (st->compact) {
+        st->np = st->np_next;
+        st->np_next = 0;
+        st->cur ^= 1u;
+        st->ident = 0;
+        st->compact = 0;
+    }
+    if (st->present == 0) st->done = 1;
+}
+
+}  // namespace
+
+extern "C" int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out) {
+    if (!out) return ctx ? ga_fail(ctx, GA_ERR_ARG, "out is NULL") : GA_ERR_ARG;
+    *out = nullptr;
+    if (!ctx || !g) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    ga_truss* t = new ga_truss;
+    t->ctx = ctx;
+    t->g = g;
+    const size_t m = g->m, n = g->n;
+    if (ga_malloc(ctx, (void**)&t->sup, (m + 1) * 4) || ga_malloc(ctx, (void**)&t->truss, (m + 1) * 4) ||
+        ga_malloc(ctx, (void**)&t->frontier, (m + 1) * 4) || ga_malloc(ctx, (void**)&t->state, m + 16) ||
+        ga_malloc(ctx, (void**)&t->plist[0], (m + 1) * 4) || ga_malloc(ctx, (void**)&t->plist[1], (m + 1) * 4) ||
+        ga_malloc(ctx, (void**)&t->urow, (n + 1) * 4) || ga_malloc(ctx, (void**)&t->uadj, (2 * m + 1) * 4) ||
+        ga_malloc(ctx, (void**)&t->ueid, (2 * m + 1) * 4) || ga_malloc(ctx, (void**)&t->st, sizeof(TrussDev))) {
+        ga_truss_destroy(t);
+        return GA_ERR_OOM;
+    }
+    // undirected adjacency with edge ids
+    int rc = GA_OK;
+    const uint64_t cnt = 2ull * m;
+    const int nb = bits_for(g->n);
+    DevBuf k0, k1, v1, rc_cnt;
+    if (!rc && (ga_malloc(ctx, &k0.p, std::max<uint64_t>(cnt, 1) * 8) || ga_malloc(ctx, &k1.p, std::max<uint64_t>(cnt, 1) * 8) ||
+                ga_malloc(ctx, &v1.p, std::max<uint64_t>(cnt, 1) * 4) || ga_malloc(ctx, &rc_cnt.p, (n + 1) * 4)))
+        rc = GA_ERR_OOM;
+    if (!rc) rc = ga_fill_u32(ctx, (uint32_t*)rc_cnt.p, n + 1, 0);
+    if (!rc && m) {
+        k_tr_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->src, g->col, g->m, nb, (unsigned long long*)k0.p, t->ueid);
+        rc = ga_launch_check(ctx, "k_tr_keys");
+        unsigned long long* ko = nullptr;
+        uint32_t* vo = nullptr;
+        if (!rc) rc = ga_sort_pairs_u64(ctx, (unsigned long long*)k0.p, (unsigned long long*)k1.p, t->ueid, (uint32_t*)v1.p, cnt, 2 * nb, &ko, &vo);
+        if (!rc && vo != t->ueid && hipMemcpyAsync(t->ueid, vo, cnt * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            rc = ga_fail(ctx, GA_ERR_HIP, "edge id copy failed");
+        if (!rc) {
+            k_tr_unpack<<<grid_for(cnt, 256, 8192), 256, 0, ctx->stream>>>(ko, cnt, nb, t->uadj, (uint32_t*)rc_cnt.p);
+            rc = ga_launch_check(ctx, "k_tr_unpack");
+        }
+    }
+    if (!rc) rc = ga_exclusive_scan_u32(ctx, (const uint32_t*)rc_cnt.p, t->urow, n + 1);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ga_fail(ctx, GA_ERR_HIP, "truss set-up failed");
+    if (rc) {
+        ga_truss_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return GA_OK;
+}
+
+extern "C" void ga_truss_destroy(ga_truss* t) {
+    if (!t) return;
+    if (t->ctx) {
+        (void)hipSetDevice(t->ctx->device);
+        (void)hipStreamSynchronize(t->ctx->stream);
+    }
+    ga_free(t->sup);
+    ga_free(t->truss);
+    ga_free(t->frontier);
+    ga_free(t->plist[0]);
+    ga_free(t->plist[1]);
+    ga_free(t->state);
+    ga_free(t->urow);
+    ga_free(t->uadj);
+    ga_free(t->ueid);
+    ga_free(t->st);
+    delete t;
+}
+
+extern "C" int ga_truss_run(ga_truss* t, uint32_t* stats, float* ms) {
+    if (!t) return GA_ERR_ARG;
+    ga_ctx* ctx = t->ctx;
+    ga_graph* g = t->g;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t m = g->m;
+    DevBuf d;
+    GA_TRY(ga_malloc(ctx, &d.p, 8));
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_TRY(ga_fill_u32(ctx, t->sup, m, 0));
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)d.p, 1, 0));
+    if (m) GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, (unsigned long long*)d.p, nullptr, nullptr, t->sup, nullptr));
+    k_tr_init<<<grid_for(std::max<uint32_t>(m, 1), 256, 8192), 256, 0, ctx->stream>>>(t->st, t->state, m);
+    GA_TRY(ga_launch_check(ctx, "k_tr_init"));
+    const unsigned scan_grid = grid_for(std::max<uint32_t>(m, 1), 256, 4u * (unsigned)std::max(ctx->cus, 1));
+    const unsigned upd_grid = 4096;   // 16K waves take frontier edges in turn
+    const Lists lists = {{t->plist[0], t->plist[1]}};
+    int block_rounds = 1;
+    for (uint64_t guard = 0;; guard++) {
+        for (int r = 0; r < block_rounds; r++) {
+            k_tr_find<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, t->st, t->frontier);
+            k_tr_find2<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, t->st, t->frontier);
+            k_tr_update<<<upd_grid, 256, 0, ctx->stream>>>(g->src, g->col, t->urow, t->uadj, t->ueid, t->state, t->sup, t->frontier, t->st);
+            k_tr_retire<<<1024, 256, 0, ctx->stream>>>(t->state, t->truss, t->frontier, t->st);
+            k_tr_compact<<<scan_grid, 256, 0, ctx->stream>>>(t->state, lists, t->st);
+            k_tr_end<<<1, 1, 0, ctx->stream>>>(t->st);
+        }
+        GA_TRY(ga_launch_check(ctx, "truss round"));
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, t->st, sizeof(TrussDev), hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const TrussDev* h = (const TrussDev*)ctx->hstat;
+        if (h->error) return ga_fail(ctx, GA_ERR_STATE, "truss round %u: %s", h->rounds,
+                                     h->error == 1 ? "empty frontier with edges present"
+                                     : h->error == 2 ? "frontier larger than the present set" : "compacted list size differs from the present count");
+        if (h->done) break;
+        if (guard > (uint64_t)m + 2) return ga_fail(ctx, GA_ERR_STATE, "truss decomposition did not finish");
+        block_rounds = std::min(block_rounds * 2, 64);
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    const TrussDev* h = (const TrussDev*)ctx->hstat;
+    t->rounds = h->rounds;
+    t->kmax = m ? h->level + 2 : 0;
+    if (stats) {
+        stats[0] = t->rounds;
+        stats[1] = t->kmax;
+    }
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return GA_OK;
+}
+
+extern "C" int ga_truss_download(ga_truss* t, uint32_t* truss) {
+    if (!t || (t->g->m && !truss)) return GA_ERR_ARG;
+    ga_ctx* ctx = t->ctx;
+    if (t->g->m) GA_HIP(ctx, hipMemcpyAsync(truss, t->truss, (size_t)t->g->m * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}

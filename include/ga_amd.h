@@ -29,6 +29,7 @@ typedef struct ga_graph ga_graph;   /* oriented CSR resident in HBM (+ optional 
 typedef struct ga_peel ga_peel;     /* batch-peeling state of one graph */
 typedef struct ga_core ga_core;     /* (k,triangle)-core state of one graph */
 typedef struct ga_batch ga_batch;   /* many graphs resident together, counted / peeled in shared launches */
+typedef struct ga_truss ga_truss;   /* k-truss decomposition state of one graph */
 
 /* ---- context ------------------------------------------------------------------------------------------------ */
 /* Replaces the driver set-up of gpu_tc.cu:83-85 (device properties) and the implicit default stream. */
@@ -201,6 +202,19 @@ int ga_core_download(ga_core* c, uint32_t* core);
 int ga_core_prune(ga_core* c, uint64_t k, ga_graph** out, uint32_t* old_id, uint32_t* core_out, float* ms);
 /* Whole decomposition with the level loop on the device; stats = {rounds, kmax}. T_full as in ga_peel_run. */
 int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms);
+
+/* ---- k-truss decomposition (no reference function: it extends the per-edge supports of dsg.h:62-94 as the
+ *      (k,triangle)-core extends per-vertex participation, with the same level rule as dsg.h:694-751) --------------- */
+/* Allocates the state and builds the undirected adjacency with oriented edge ids on the device. */
+int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out);
+void ga_truss_destroy(ga_truss* t);
+/* Whole decomposition on the device: supports by the hash count, then rounds of level = max(level, least present
+ * support), removal of every present edge with support <= level (truss = level + 2) and support decrements for the
+ * triangles they close; rounds enqueued in blocks with one readback per block. stats = {rounds, kmax} (kmax 0 when
+ * m = 0); ms: device time including the support count. */
+int ga_truss_run(ga_truss* t, uint32_t* stats, float* ms);
+/* truss[e] for every oriented edge e (index into col), m entries; valid after ga_truss_run. */
+int ga_truss_download(ga_truss* t, uint32_t* truss);
 
 /* ---- batching: many graphs resident together (no reference counterpart: the reference runs one graph per process;
  *      per-graph semantics are exactly load_graph + the calls above, graph by graph) ----------------------------- */

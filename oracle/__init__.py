@@ -59,6 +59,8 @@ def olib():
             "orc_edge_order": (None, [_vp, _u32p]),
             "orc_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
             "orc_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
+            "orc_truss": (ctypes.c_uint32, [_vp, _u32p]),
+            "orc_brute_truss": (None, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u32p]),
             "orc_brute_participation": (ctypes.c_uint64, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u8p, _u32p]),
             "orc_brute_triangles": (ctypes.c_uint64, [ctypes.c_uint32, _u32p, ctypes.c_uint64]),
             "orc_brute_densest": (None, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u64p, _u64p]),
@@ -177,6 +179,12 @@ class OGraph:
         c = np.zeros(max(2 * self.m, 1), np.uint32)
         olib().orc_undirected(self._h, _p(r, _u32p), _p(c, _u32p))
         return r, c[: 2 * self.m]
+
+    def truss(self):
+        """k-truss number of every oriented edge (index into col) and kmax (Wang & Cheng peeling)."""
+        t = np.zeros(max(self.m, 1), np.uint32)
+        kmax = olib().orc_truss(self._h, _p(t, _u32p))
+        return t[: self.m], int(kmax)
 
     def induced(self, keep):
         """induced_subgraph (dsg.h:103-114): (OGraph, old_id)."""
@@ -367,6 +375,14 @@ def brute_core(n: int, unique_pairs: np.ndarray) -> np.ndarray:
     c = np.zeros(max(n, 1), np.uint32)
     olib().orc_brute_core(n, _p(p, _u32p), p.size // 2, _p(c, _u32p))
     return c[:n]
+
+
+def brute_truss(n: int, unique_pairs: np.ndarray) -> np.ndarray:
+    """k-truss number of each undirected pair by the definition (small graphs)."""
+    p = np.ascontiguousarray(unique_pairs, dtype=np.uint32).reshape(-1)
+    t = np.zeros(max(p.size // 2, 1), np.uint32)
+    olib().orc_brute_truss(n, _p(p, _u32p), p.size // 2, _p(t, _u32p))
+    return t[: p.size // 2]
 
 
 def peel_arith_ok(n, T0, num, den) -> bool:

@@ -464,6 +464,63 @@ inline void prune_to_core(const Graph& g, const std::vector<uint32_t>& core, uin
     for (size_t i = 0; i < old_id.size(); i++) core_out[i] = core[old_id[i]];
 }
 
+// ---- k-truss decomposition (no reference function: it extends the per-edge supports of dsg.h:62-94) ----
+// truss[e] for every oriented edge e (an index into g.col) is the largest k such that e lies in the k-truss, the
+// maximal subgraph in which every edge closes at least k - 2 triangles; every edge is in the 2-truss. Computed by the
+// min-support-first peeling of Wang & Cheng, "Truss decomposition in massive networks" (PVLDB 5(9), 2012, Alg. 2):
+// take an edge of least current support s, truss = max(level, s) + 2 with level raised to s, remove it, and take one
+// from the support of the two other edges of every triangle it closes whose edges are all still present.
+// Returns kmax (0 for a graph without edges).
+inline uint32_t truss_decomposition(const Graph& g, std::vector<uint32_t>& truss) {
+    const uint32_t n = g.n, m = g.m;
+    std::vector<uint32_t> urow((size_t)n + 1, 0), uadj(2ull * m), ueid(2ull * m);
+    for (uint32_t e = 0; e < m; e++) { urow[g.src[e] + 1]++; urow[g.col[e] + 1]++; }
+    for (uint32_t i = 0; i < n; i++) urow[i + 1] += urow[i];
+    {
+        std::vector<std::pair<uint32_t, uint32_t>> tmp(2ull * m);
+        std::vector<uint32_t> cur(urow.begin(), urow.end() - 1);
+        for (uint32_t e = 0; e < m; e++) {
+            tmp[cur[g.src[e]]++] = {g.col[e], e};
+            tmp[cur[g.col[e]]++] = {g.src[e], e};
+        }
+        for (uint32_t u = 0; u < n; u++) std::sort(tmp.begin() + urow[u], tmp.begin() + urow[u + 1]);
+        for (size_t i = 0; i < tmp.size(); i++) { uadj[i] = tmp[i].first; ueid[i] = tmp[i].second; }
+    }
+    std::vector<uint32_t> sup;
+    edge_support(g, sup);
+    truss.assign(m, 0);
+    std::vector<uint8_t> gone(m, 0);
+    typedef std::pair<uint32_t, uint32_t> Item;   // (support, edge)
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+    for (uint32_t e = 0; e < m; e++) heap.push({sup[e], e});
+    uint32_t level = 0, kmax = 0;
+    while (!heap.empty()) {
+        const Item it = heap.top();
+        heap.pop();
+        const uint32_t e = it.second;
+        if (gone[e] || it.first != sup[e]) continue;   // removed, or a stale entry
+        level = std::max(level, sup[e]);
+        truss[e] = level + 2;
+        kmax = std::max(kmax, truss[e]);
+        gone[e] = 1;
+        uint32_t a = g.src[e], b = g.col[e];
+        if (urow[a + 1] - urow[a] > urow[b + 1] - urow[b]) std::swap(a, b);   // walk the shorter list
+        for (uint32_t p = urow[a]; p < urow[a + 1]; p++) {
+            const uint32_t w = uadj[p], ea = ueid[p];
+            if (gone[ea]) continue;
+            const uint32_t* lo = uadj.data() + urow[b];
+            const uint32_t* hi = uadj.data() + urow[b + 1];
+            const uint32_t* q = std::lower_bound(lo, hi, w);
+            if (q == hi || *q != w) continue;
+            const uint32_t eb = ueid[q - uadj.data()];
+            if (gone[eb]) continue;
+            heap.push({--sup[ea], ea});
+            heap.push({--sup[eb], eb});
+        }
+    }
+    return kmax;
+}
+
 // ---- brute-force oracles (independent of the CSR algorithms above) ----
 
 // Bit-set adjacency: triangle count and per-vertex counts of the alive-induced subgraph, O(n^2 * n/64).
@@ -544,6 +601,32 @@ inline void brute_core(uint32_t n, const EdgeList& undirected, std::vector<uint3
         uint32_t mn = UINT32_MAX;
         for (uint32_t v = 0; v < n; v++) if (S >> v & 1) mn = std::min(mn, d[v]);
         for (uint32_t v = 0; v < n; v++) if (S >> v & 1) core[v] = std::max(core[v], mn);
+    }
+}
+
+// k-truss by its definition: for k = 3, 4, ... keep removing edges that close fewer than k - 2 triangles among the
+// edges still kept until none is left to remove; truss of an undirected pair = the largest k it survives (2 at least).
+// Supports are recounted from the adjacency bits each pass; for small n.
+inline void brute_truss(uint32_t n, const EdgeList& e, std::vector<uint32_t>& truss) {
+    truss.assign(e.size(), 2);
+    std::vector<uint8_t> in(e.size(), 1);
+    for (uint32_t k = 3;; k++) {
+        bool changed = true;
+        while (changed) {
+            changed = false;
+            EdgeList kept;
+            for (size_t i = 0; i < e.size(); i++) if (in[i]) kept.push_back(e[i]);
+            Brute b(n, kept);
+            for (size_t i = 0; i < e.size(); i++) {
+                if (!in[i]) continue;
+                uint32_t s = 0;
+                for (uint32_t w = 0; w < n; w++) s += b.has(e[i].first, w) && b.has(e[i].second, w);
+                if (s + 2 < k) { in[i] = 0; changed = true; }
+            }
+        }
+        bool any = false;
+        for (size_t i = 0; i < e.size(); i++) if (in[i]) { truss[i] = k; any = true; }
+        if (!any) return;
     }
 }
 

@@ -162,6 +162,23 @@ orc_graph* orc_prune_to_core(const orc_graph* h, const uint32_t* core, uint64_t 
     return o;
 }
 
+// k-truss numbers per oriented edge (m entries); returns kmax.
+uint32_t orc_truss(const orc_graph* h, uint32_t* truss) {
+    std::vector<uint32_t> t;
+    const uint32_t kmax = truss_decomposition(h->g, t);
+    std::copy(t.begin(), t.end(), truss);
+    return kmax;
+}
+
+// k-truss by definition on an undirected unique loop-free pair list (truss per pair).
+void orc_brute_truss(uint32_t n, const uint32_t* pairs, uint64_t k, uint32_t* truss) {
+    EdgeList e(k);
+    for (uint64_t i = 0; i < k; i++) e[i] = {pairs[2 * i], pairs[2 * i + 1]};
+    std::vector<uint32_t> t;
+    brute_truss(n, e, t);
+    std::copy(t.begin(), t.end(), truss);
+}
+
 // Brute force on an undirected unique loop-free pair list.
 uint64_t orc_brute_participation(uint32_t n, const uint32_t* pairs, uint64_t k, const uint8_t* alive, uint32_t* tri) {
     EdgeList e(k);

@@ -175,6 +175,63 @@ def test_prune_invariants():
     assert (e.n, e.m, old.size) == (0, 0, 0)
 
 
+def _truss_by_pair(g):
+    t, kmax = g.truss()
+    r, c, s = g.arrays()
+    return {(int(min(a, b)), int(max(a, b))): int(x) for a, b, x in zip(s, c, t)}, kmax
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_truss_brute_force(seed):
+    """k-truss peeling equals the definition (repeated removal of edges with too few triangles per level)."""
+    rng = np.random.default_rng(900 + seed)
+    n = int(rng.integers(6, 36))
+    raw = random_graph(rng, n, float(rng.uniform(0.1, 0.7)))
+    if seed % 3 == 0:   # plant a clique so that high levels occur
+        k = int(rng.integers(4, min(n, 9)))
+        raw = np.concatenate([raw, np.array(clique(k), np.uint32)])
+    g = oracle.OGraph.from_pairs(raw)
+    uniq = oracle.canonical_unique(raw)
+    bt = oracle.brute_truss(g.n, uniq)
+    want = {(int(a), int(b)): int(x) for (a, b), x in zip(uniq, bt)}
+    got, kmax = _truss_by_pair(g)
+    assert got == want
+    assert kmax == (max(want.values()) if want else 0)
+
+
+def test_truss_closed_forms_and_invariants():
+    """K_k: every edge k; trees and cycles (no triangle): 2; wheels W_n (n >= 5): 3; two K_k sharing an edge: k for
+    all, including the shared edge; truss <= support + 2; every edge of the j-truss closes >= j - 2 triangles in it."""
+    for k in range(3, 12):
+        t, kmax = oracle.OGraph.from_pairs(np.array(clique(k), np.uint32)).truss()
+        assert kmax == k and np.all(t == k)
+    tree = np.array([(i, (i - 1) // 2) for i in range(1, 40)], np.uint32)
+    cyc = np.array([(i, (i + 1) % 30) for i in range(30)], np.uint32)
+    for raw in (tree, cyc):
+        t, kmax = oracle.OGraph.from_pairs(raw).truss()
+        assert kmax == 2 and np.all(t == 2)
+    for n in range(5, 12):
+        wheel = np.array([e for i in range(1, n) for e in ((0, i), (i, i % (n - 1) + 1))], np.uint32)
+        t, kmax = oracle.OGraph.from_pairs(wheel).truss()
+        assert kmax == 3 and np.all(t == 3), n
+    for k in (4, 6, 9):
+        two = sorted(set(clique(k)) | set(clique(k, k - 2)))
+        t, kmax = oracle.OGraph.from_pairs(np.array(two, np.uint32)).truss()
+        assert kmax == k and np.all(t == k)
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        g = oracle.OGraph.from_pairs(random_graph(rng, 300, 0.08))
+        t, kmax = g.truss()
+        _, sup = g.edge_support()
+        assert np.all(t <= sup + 2) and np.all(t >= 2)
+        r, c, s = g.arrays()
+        for j in range(3, kmax + 1):
+            keep = t >= j
+            sub = oracle.OGraph.from_pairs(np.stack([s[keep], c[keep]], 1), g.n)
+            _, ss = sub.edge_support()
+            assert ss.size == 0 or ss.min() + 2 >= j, j
+
+
 def test_closed_forms():
     for k in range(3, 16):
         g = oracle.OGraph.from_pairs(np.array(clique(k), np.uint32))

@@ -5,21 +5,24 @@
 //
 // Set-up: per-edge supports from the hash count (F_SUP); an undirected adjacency whose entries carry the oriented edge
 // id (keys (u << nb | w) sorted with the edge id as value, rows by counts + scan).
-// One round, all on the device, over a list of the present edges (all edges until the first compaction):
-//   k_tr_find     present edges with support <= level -> state 2, appended to the frontier with one atomic per wave;
-//                 the least support of the other present edges is kept (one atomicMin per wave)
-//   k_tr_find2    only when that frontier is empty: the level rises to that least support and the edges at it form
-//                 the frontier (a second pass over the list only when the level changes)
-//   k_tr_update   one wave per frontier edge (u, v): walks the shorter of N(u), N(v) and binary-searches each w in the
+// The frontier of a round at an unchanged level is exactly the set of edges whose support a decrement of the round
+// before took from level + 1 to level (supports fall by one at a time and atomicSub returns the old value, so each
+// such edge is appended once, by that decrement). Only when that set is empty does the level rise, and only then is the
+// list of present edges scanned: for the least support, then for the edges at it.
+// One round, all on the device:
+//   k_tr_min      level rise only: least support over the present list (one atomicMin per wave)
+//   k_tr_find     level rise only: present edges at that support -> state 2, appended to the frontier
+//   k_tr_update   one wave per frontier edge (u, v): walks the shorter of N(u), N(v) (chunks past the first 256
+//                 elements go to a queue that k_tr_update_chunks spreads over waves) and binary-searches each w in the
 //                 other; for a triangle whose two other edges are present: both present and not in the frontier ->
 //                 both lose one; one of them in the frontier -> the other loses one, decremented only by the frontier
-//                 edge of smaller id; both in the frontier -> nothing (the triangle is gone with all of them)
-//   k_tr_retire   frontier edges -> state 0, truss = level + 2
-//   k_tr_compact  when the present edges are at most half the list: the present ones are copied to the other list
-//   k_tr_end      present count, level, round count, list switch, sanity (empty frontier, more removed than present),
-//                 done flag
+//                 edge of smaller id; both in the frontier -> nothing (the triangle is gone with all of them). A
+//                 decrement from level + 1 appends the edge to the next round's frontier
+//   k_tr_retire   frontier edges -> state 0, truss = level + 2; next frontier -> state 2
+//   k_tr_compact  when the present edges are at most half the list: they are copied to the other list
+//   k_tr_end      present count, level, round count, buffer switches, sanity, done flag
 // Rounds are enqueued in blocks that double up to 64 with one readback per block; every kernel returns at once when
-// done is set, so the rounds past the end cost only their launches.
+// done is set (and the two scans unless the level rises), so rounds past the end cost only their launches.
 #include "ga_internal.hpp"
 
 namespace {
@@ -38,11 +41,13 @@ inline int bits_for(uint32_t n) {
 }  // namespace
 
 struct TrussDev {
-    unsigned long long present;   // edges still present
-    uint32_t level;               // support level (truss = level + 2)
-    uint32_t min_rest;            // least support of the present edges above the level (this round)
-    uint32_t nf;                  // frontier size at the current level (k_tr_find)
-    uint32_t nf2;                 // frontier size after a level rise (k_tr_find2)
+    unsigned long long present;   // edges still present (state 1 or 2)
+    uint32_t level;               // support level of the last round (truss = level + 2)
+    uint32_t min_rest;            // least support of the present edges (level-rise rounds)
+    uint32_t nf;                  // frontier size of this round
+    uint32_t nnext;               // next round's frontier, appended by this round's decrements
+    uint32_t rise;                // 1: this round raises the level (the next frontier came out empty)
+    uint32_t fcur;                // which frontier buffer holds this round's frontier
     uint32_t rounds;
     uint32_t done;
     uint32_t error;               // 1: empty frontier with edges present, 2: frontier larger than the present set,
@@ -52,18 +57,20 @@ struct TrussDev {
     uint32_t cur;                 // which of the two list buffers is current
     uint32_t np_next;             // length of the list being compacted into
     uint32_t compact;             // 1: this round compacts
+    unsigned long long nchunk;    // queued chunks of long shorter lists (this round)
 };
 
-// level of the round: the current one if it has a frontier, else the least support above it
-__device__ __forceinline__ uint32_t round_level(const TrussDev* st) { return st->nf ? st->level : st->min_rest; }
+// support level of this round: the least present support when the level rises, else the last round's
+__device__ __forceinline__ uint32_t round_level(const TrussDev* st) { return st->rise ? st->min_rest : st->level; }
 
 struct ga_truss {
     ga_ctx* ctx = nullptr;
     ga_graph* g = nullptr;
     uint32_t* sup = nullptr;      // current support per oriented edge
     uint32_t* truss = nullptr;
-    uint32_t* frontier = nullptr;
+    uint32_t* frontier[2] = {nullptr, nullptr};   // this round's frontier and the next one (swapped each round)
     uint32_t* plist[2] = {nullptr, nullptr};   // lists of present edges (compacted when half are gone)
+    uint2* chunks = nullptr;                   // queue of (frontier edge, start) chunks of long shorter lists
     uint8_t* state = nullptr;     // 0 removed, 1 present, 2 frontier of this round
     uint32_t* urow = nullptr;     // undirected adjacency with edge ids
     uint32_t* uadj = nullptr;
@@ -102,7 +109,9 @@ __global__ void k_tr_init(TrussDev* st, uint8_t* __restrict__ state, uint32_t m)
         st->level = 0;
         st->min_rest = 0xFFFFFFFFu;
         st->nf = 0;
-        st->nf2 = 0;
+        st->nnext = 0;
+        st->rise = 1;
+        st->fcur = 0;
         st->rounds = 0;
         st->done = m == 0;
         st->error = 0;
@@ -111,54 +120,35 @@ __global__ void k_tr_init(TrussDev* st, uint8_t* __restrict__ state, uint32_t m)
         st->cur = 0;
         st->np_next = 0;
         st->compact = 0;
+        st->nchunk = 0;
     }
 }
 
-struct Lists {
+struct Bufs {
     uint32_t* p[2];
 };
 
-// Frontier at the current level, and the least support of the present edges above it.
-__global__ void k_tr_find(const uint32_t* __restrict__ sup, uint8_t* __restrict__ state, Lists L, TrussDev* st,
-                          uint32_t* __restrict__ frontier) {
-    if (st->done) return;
-    const uint32_t level = st->level, np = st->np;
+// Level rise: least support over the present edges of the list.
+__global__ void k_tr_min(const uint32_t* __restrict__ sup, const uint8_t* __restrict__ state, Bufs L, TrussDev* st) {
+    if (st->done || !st->rise) return;
+    const uint32_t np = st->np;
     const uint32_t* list = L.p[st->cur];
     const bool ident = st->ident;
     uint32_t mn = 0xFFFFFFFFu;
-    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
-    for (uint32_t base = wave0 * 64; base < np; base += nwaves * 64) {   // wave-uniform loop
-        const uint32_t i = base + lane_id();
-        bool f = false;
-        uint32_t e = 0;
-        if (i < np) {
-            e = ident ? i : list[i];
-            if (state[e] == 1) {
-                const uint32_t s = sup[e];
-                f = s <= level;
-                if (!f) mn = min(mn, s);
-            }
-        }
-        const uint64_t mask = __ballot(f);
-        if (!mask) continue;
-        uint32_t at = 0;
-        if (lane_id() == 0) at = atomicAdd(&st->nf, (uint32_t)__popcll(mask));
-        at = __shfl(at, 0, 64);
-        if (f) {
-            state[e] = 2;
-            frontier[at + mask_rank(mask)] = e;
-        }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+        const uint32_t e = ident ? i : list[i];
+        if (state[e] == 1) mn = min(mn, sup[e]);
     }
     mn = wave_min_u32(mn);
     if (lane_id() == 0 && mn != 0xFFFFFFFFu) atomicMin(&st->min_rest, mn);
 }
 
-// Only when k_tr_find found nothing at the current level: the level rises to the least support, whose edges leave.
-__global__ void k_tr_find2(const uint32_t* __restrict__ sup, uint8_t* __restrict__ state, Lists L, TrussDev* st,
-                           uint32_t* __restrict__ frontier) {
-    if (st->done || st->nf) return;
+// Level rise: the present edges at the least support form the frontier.
+__global__ void k_tr_find(const uint32_t* __restrict__ sup, uint8_t* __restrict__ state, Bufs L, Bufs F, TrussDev* st) {
+    if (st->done || !st->rise) return;
     const uint32_t level = st->min_rest, np = st->np;
     const uint32_t* list = L.p[st->cur];
+    uint32_t* frontier = F.p[st->fcur];
     const bool ident = st->ident;
     const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
     for (uint32_t base = wave0 * 64; base < np; base += nwaves * 64) {   // wave-uniform loop
@@ -172,7 +162,7 @@ __global__ void k_tr_find2(const uint32_t* __restrict__ sup, uint8_t* __restrict
         const uint64_t mask = __ballot(f);
         if (!mask) continue;
         uint32_t at = 0;
-        if (lane_id() == 0) at = atomicAdd(&st->nf2, (uint32_t)__popcll(mask));
+        if (lane_id() == 0) at = atomicAdd(&st->nf, (uint32_t)__popcll(mask));
         at = __shfl(at, 0, 64);
         if (f) {
             state[e] = 2;
@@ -181,59 +171,118 @@ __global__ void k_tr_find2(const uint32_t* __restrict__ sup, uint8_t* __restrict
     }
 }
 
-// One wave per frontier edge; frontier entries are taken wave by wave over a fixed grid.
-__global__ void __launch_bounds__(256) k_tr_update(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
-                                                   const uint32_t* __restrict__ urow, const uint32_t* __restrict__ uadj,
-                                                   const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state,
-                                                   uint32_t* __restrict__ sup, const uint32_t* __restrict__ frontier, const TrussDev* st) {
-    if (st->done) return;
-    const uint32_t nf = st->nf + st->nf2;
-    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
-    for (uint32_t i = wave0; i < nf; i += nwaves) {
-        const uint32_t e = frontier[i];
-        uint32_t a = src[e], b = col[e];
-        uint32_t ab = urow[a], ae = urow[a + 1], bb = urow[b], be = urow[b + 1];
-        if (ae - ab > be - bb) {   // stream the shorter list, search the longer
-            uint32_t t = ab; ab = bb; bb = t;
-            t = ae; ae = be; be = t;
-        }
-        for (uint32_t p = ab + lane_id(); p < ae; p += 64) {
-            const uint32_t ea = ueid[p];
-            const uint32_t sa = state[ea];
-            if (sa == 0 || ea == e) continue;
-            const uint32_t w = uadj[p];
-            const uint32_t q = lower_bound_u32(uadj, bb, be, w);
-            if (q == be || uadj[q] != w) continue;
-            const uint32_t eb = ueid[q];
-            const uint32_t sb = state[eb];
-            if (sb == 0) continue;
-            if (sa == 1 && sb == 1) {
-                atomicSub(&sup[ea], 1u);
-                atomicSub(&sup[eb], 1u);
-            } else if (sa == 2 && sb == 1) {
-                if (e < ea) atomicSub(&sup[eb], 1u);
-            } else if (sa == 1 && sb == 2) {
-                if (e < eb) atomicSub(&sup[ea], 1u);
-            }
+// One support decrement; the one from level + 1 puts the edge in the next frontier.
+__device__ __forceinline__ void tr_dec(uint32_t* __restrict__ sup, uint32_t x, uint32_t level, uint32_t* __restrict__ next, TrussDev* st) {
+    const uint32_t old = atomicSub(&sup[x], 1u);
+    if (old == level + 1u) next[atomicAdd(&st->nnext, 1u)] = x;
+}
+
+// The triangles that frontier edge e closes through the part [p0, p1) of its shorter list [ab, ae), the longer one being
+// [bb, be); one wave, lanes over consecutive elements.
+__device__ __forceinline__ void tr_walk(uint32_t e, uint32_t p0, uint32_t p1, uint32_t bb, uint32_t be, const uint32_t* __restrict__ uadj,
+                                        const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state, uint32_t* __restrict__ sup,
+                                        uint32_t level, uint32_t* __restrict__ next, TrussDev* st) {
+    for (uint32_t p = p0 + lane_id(); p < p1; p += 64) {
+        const uint32_t ea = ueid[p];
+        const uint32_t sa = state[ea];
+        if (sa == 0 || ea == e) continue;
+        const uint32_t w = uadj[p];
+        const uint32_t q = lower_bound_u32(uadj, bb, be, w);
+        if (q == be || uadj[q] != w) continue;
+        const uint32_t eb = ueid[q];
+        const uint32_t sb = state[eb];
+        if (sb == 0) continue;
+        if (sa == 1 && sb == 1) {
+            tr_dec(sup, ea, level, next, st);
+            tr_dec(sup, eb, level, next, st);
+        } else if (sa == 2 && sb == 1) {
+            if (e < ea) tr_dec(sup, eb, level, next, st);
+        } else if (sa == 1 && sb == 2) {
+            if (e < eb) tr_dec(sup, ea, level, next, st);
         }
     }
 }
 
-__global__ void k_tr_retire(uint8_t* __restrict__ state, uint32_t* __restrict__ truss, const uint32_t* __restrict__ frontier,
-                            TrussDev* st) {
+__device__ __forceinline__ void tr_lists(uint32_t e, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                                         const uint32_t* __restrict__ urow, uint32_t& ab, uint32_t& ae, uint32_t& bb, uint32_t& be) {
+    const uint32_t a = src[e], b = col[e];
+    ab = urow[a]; ae = urow[a + 1]; bb = urow[b]; be = urow[b + 1];
+    if (ae - ab > be - bb) {   // stream the shorter list, search the longer
+        uint32_t t = ab; ab = bb; bb = t;
+        t = ae; ae = be; be = t;
+    }
+}
+
+constexpr uint32_t TR_CHUNK = 256;   // shorter-list elements one wave walks per work unit
+
+// One wave per frontier edge; frontier entries are taken wave by wave over a fixed grid. A shorter list longer than
+// TR_CHUNK: the wave walks the first chunk and queues the others (edge, start) for k_tr_update_chunks, so the long
+// lists of a few frontier edges spread over many waves. A full queue: the wave walks the rest itself.
+__global__ void __launch_bounds__(256) k_tr_update(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                                                   const uint32_t* __restrict__ urow, const uint32_t* __restrict__ uadj,
+                                                   const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state,
+                                                   uint32_t* __restrict__ sup, Bufs F, uint2* __restrict__ chunks, uint32_t chunk_cap,
+                                                   TrussDev* st) {
     if (st->done) return;
-    const uint32_t nf = st->nf + st->nf2, t = round_level(st) + 2u;
+    const uint32_t nf = st->nf, level = round_level(st);
+    const uint32_t* frontier = F.p[st->fcur];
+    uint32_t* next = F.p[st->fcur ^ 1u];
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t i = wave0; i < nf; i += nwaves) {
+        const uint32_t e = frontier[i];
+        uint32_t ab, ae, bb, be;
+        tr_lists(e, src, col, urow, ab, ae, bb, be);
+        uint32_t end = ae;
+        if (ae - ab > TR_CHUNK) {
+            const uint32_t extra = (ae - ab - 1) / TR_CHUNK;   // chunks after the first
+            unsigned long long at = 0;
+            if (lane_id() == 0) at = atomicAdd(&st->nchunk, (unsigned long long)extra);
+            at = __shfl(at, 0, 64);
+            if (at + extra <= chunk_cap) {
+                for (uint32_t c = lane_id(); c < extra; c += 64) chunks[at + c] = make_uint2(e, ab + (c + 1) * TR_CHUNK);
+                end = ab + TR_CHUNK;
+            }
+        }
+        tr_walk(e, ab, end, bb, be, uadj, ueid, state, sup, level, next, st);
+    }
+}
+
+// One wave per queued chunk of a long shorter list.
+__global__ void __launch_bounds__(256) k_tr_update_chunks(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                                                          const uint32_t* __restrict__ urow, const uint32_t* __restrict__ uadj,
+                                                          const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state,
+                                                          uint32_t* __restrict__ sup, Bufs F, const uint2* __restrict__ chunks,
+                                                          uint32_t chunk_cap, TrussDev* st) {
+    if (st->done) return;
+    const uint32_t nc = (uint32_t)min(st->nchunk, (unsigned long long)chunk_cap), level = round_level(st);
+    uint32_t* next = F.p[st->fcur ^ 1u];
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
+    for (uint32_t i = wave0; i < nc; i += nwaves) {
+        const uint2 c = chunks[i];
+        uint32_t ab, ae, bb, be;
+        tr_lists(c.x, src, col, urow, ab, ae, bb, be);
+        tr_walk(c.x, c.y, min(c.y + TR_CHUNK, ae), bb, be, uadj, ueid, state, sup, level, next, st);
+    }
+}
+
+// Retires this round's frontier and marks the next one (after the update, so the owner rule saw this round's states).
+__global__ void k_tr_retire(uint8_t* __restrict__ state, uint32_t* __restrict__ truss, Bufs F, TrussDev* st) {
+    if (st->done) return;
+    const uint32_t nf = st->nf, nn = st->nnext, t = round_level(st) + 2u;
+    const uint32_t* frontier = F.p[st->fcur];
+    const uint32_t* next = F.p[st->fcur ^ 1u];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x) {
         const uint32_t e = frontier[i];
         state[e] = 0;
         truss[e] = t;
     }
-    // compact this round when what stays present is at most half the list (decided identically by every thread)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nn; i += gridDim.x * blockDim.x) state[next[i]] = 2;
+    // compact this round when what stays present is at most half the list
     if (blockIdx.x == 0 && threadIdx.x == 0) st->compact = (st->present - nf) * 2 <= st->np && st->present > nf;
 }
 
-// Copies the present edges of the current list to the other one (after retire, so frontier edges are gone).
-__global__ void k_tr_compact(const uint8_t* __restrict__ state, Lists L, TrussDev* st) {
+// Copies the present edges (state 1 or 2) of the current list to the other one.
+__global__ void k_tr_compact(const uint8_t* __restrict__ state, Bufs L, TrussDev* st) {
     if (st->done || !st->compact) return;
     const uint32_t np = st->np;
     const uint32_t* list = L.p[st->cur];
@@ -246,7 +295,7 @@ __global__ void k_tr_compact(const uint8_t* __restrict__ state, Lists L, TrussDe
         bool k = false;
         if (i < np) {
             e = ident ? i : list[i];
-            k = state[e] == 1;
+            k = state[e] != 0;
         }
         const uint64_t mask = __ballot(k);
         if (!mask) continue;
@@ -259,7 +308,7 @@ __global__ void k_tr_compact(const uint8_t* __restrict__ state, Lists L, TrussDe
 
 __global__ void k_tr_end(TrussDev* st) {
     if (st->done) return;
-    const uint32_t nf = st->nf + st->nf2;
+    const uint32_t nf = st->nf;
     if (nf == 0 || nf > st->present) {
         st->error = nf == 0 ? 1u : 2u;
         st->done = 1;
@@ -273,9 +322,12 @@ __global__ void k_tr_end(TrussDev* st) {
     st->level = round_level(st);
     st->present -= nf;
     st->rounds++;
-    st->nf = 0;
-    st->nf2 = 0;
+    st->nf = st->nnext;
+    st->nnext = 0;
+    st->fcur ^= 1u;
+    st->rise = st->nf == 0;
     st->min_rest = 0xFFFFFFFFu;
+    st->nchunk = 0;
     if (st->compact) {
         st->np = st->np_next;
         st->np_next = 0;
@@ -298,7 +350,8 @@ extern "C" int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out) {
     t->g = g;
     const size_t m = g->m, n = g->n;
     if (ga_malloc(ctx, (void**)&t->sup, (m + 1) * 4) || ga_malloc(ctx, (void**)&t->truss, (m + 1) * 4) ||
-        ga_malloc(ctx, (void**)&t->frontier, (m + 1) * 4) || ga_malloc(ctx, (void**)&t->state, m + 16) ||
+        ga_malloc(ctx, (void**)&t->frontier[0], (m + 1) * 4) || ga_malloc(ctx, (void**)&t->frontier[1], (m + 1) * 4) ||
+        ga_malloc(ctx, (void**)&t->state, m + 16) || ga_malloc(ctx, (void**)&t->chunks, (m / 4 + 1024) * 8) ||
         ga_malloc(ctx, (void**)&t->plist[0], (m + 1) * 4) || ga_malloc(ctx, (void**)&t->plist[1], (m + 1) * 4) ||
         ga_malloc(ctx, (void**)&t->urow, (n + 1) * 4) || ga_malloc(ctx, (void**)&t->uadj, (2 * m + 1) * 4) ||
         ga_malloc(ctx, (void**)&t->ueid, (2 * m + 1) * 4) || ga_malloc(ctx, (void**)&t->st, sizeof(TrussDev))) {
@@ -345,7 +398,9 @@ extern "C" void ga_truss_destroy(ga_truss* t) {
     }
     ga_free(t->sup);
     ga_free(t->truss);
-    ga_free(t->frontier);
+    ga_free(t->chunks);
+    ga_free(t->frontier[0]);
+    ga_free(t->frontier[1]);
     ga_free(t->plist[0]);
     ga_free(t->plist[1]);
     ga_free(t->state);
@@ -372,14 +427,19 @@ extern "C" int ga_truss_run(ga_truss* t, uint32_t* stats, float* ms) {
     GA_TRY(ga_launch_check(ctx, "k_tr_init"));
     const unsigned scan_grid = grid_for(std::max<uint32_t>(m, 1), 256, 4u * (unsigned)std::max(ctx->cus, 1));
     const unsigned upd_grid = 4096;   // 16K waves take frontier edges in turn
-    const Lists lists = {{t->plist[0], t->plist[1]}};
+    const Bufs lists = {{t->plist[0], t->plist[1]}};
+    const Bufs fr = {{t->frontier[0], t->frontier[1]}};
+    const uint32_t chunk_cap = m / 4 + 1024;
     int block_rounds = 1;
     for (uint64_t guard = 0;; guard++) {
         for (int r = 0; r < block_rounds; r++) {
-            k_tr_find<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, t->st, t->frontier);
-            k_tr_find2<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, t->st, t->frontier);
-            k_tr_update<<<upd_grid, 256, 0, ctx->stream>>>(g->src, g->col, t->urow, t->uadj, t->ueid, t->state, t->sup, t->frontier, t->st);
-            k_tr_retire<<<1024, 256, 0, ctx->stream>>>(t->state, t->truss, t->frontier, t->st);
+            k_tr_min<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, t->st);
+            k_tr_find<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, fr, t->st);
+            k_tr_update<<<upd_grid, 256, 0, ctx->stream>>>(g->src, g->col, t->urow, t->uadj, t->ueid, t->state, t->sup, fr, t->chunks,
+                                                           chunk_cap, t->st);
+            k_tr_update_chunks<<<upd_grid, 256, 0, ctx->stream>>>(g->src, g->col, t->urow, t->uadj, t->ueid, t->state, t->sup, fr, t->chunks,
+                                                                  chunk_cap, t->st);
+            k_tr_retire<<<1024, 256, 0, ctx->stream>>>(t->state, t->truss, fr, t->st);
             k_tr_compact<<<scan_grid, 256, 0, ctx->stream>>>(t->state, lists, t->st);
             k_tr_end<<<1, 1, 0, ctx->stream>>>(t->st);
         }

@@ -71,6 +71,27 @@ def test_full_size_ingest(ctx, name, tmp_path):
         x.close()
 
 
+def test_full_size_truss(ctx):
+    """k-truss of the 34.7M-edge graph through properties (the oracle's exact peeling takes minutes at this size; exact
+    parity runs on graphs of up to about a million edges in tests/test_gpu_truss.py): kmax equals the oracle's result
+    for this seeded graph, 2 <= truss <= support + 2 for every edge, and at several levels k every edge of the edges
+    with truss >= k closes at least k - 2 triangles among them (the defining property, recounted on the device)."""
+    g, og, T = _graph(ctx, "powerlaw_communities")
+    t, rounds, kmax = g.truss_decomposition()
+    assert kmax == 83   # oracle.OGraph.truss() on this graph (one host core, about 260 s)
+    Ts, sup = g.edge_support(ga.TC_HASH)
+    assert Ts == T and t.min() >= 2 and np.all(t.astype(np.int64) <= sup.astype(np.int64) + 2)
+    r, c, s = g.download()
+    for k in sorted({3, 4, int(np.median(t)) + 1, kmax // 2, kmax - 1, kmax}):
+        keep = t >= k
+        h = ga.DeviceGraph.from_pairs(ctx, np.stack([s[keep], c[keep]], 1), n=g.n)
+        _, hs = h.edge_support(ga.TC_HASH)
+        assert h.m == int(keep.sum()) and int(hs.min()) + 2 >= k, k
+        h.close()
+    print(f"truss: {rounds} rounds, kmax {kmax}", flush=True)
+    g.close()
+
+
 def test_full_size_core(ctx):
     """(k,triangle)-core numbers of the 34.7M-edge graph against Alg 3 (dsg.h:116-166)."""
     g, og, T = _graph(ctx, "powerlaw_communities")

@@ -65,3 +65,9 @@ def test_truss_powerlaw(ctx):
     check(ctx, gen.chung_lu(30000, 300000, 2.1, 0.0, seed=51), "chung_lu 30K")
     raw = np.concatenate([gen.chung_lu(80000, 600000, 2.3, 5.0, seed=52), gen.communities(80000, 400, 10, 60, 0.3, seed=53)])
     check(ctx, raw, "chung_lu 80K + communities")
+
+
+def test_truss_rmat(ctx):
+    """R-MAT scale 16 (477K edges, hubs of high degree: long shorter-lists split into chunks over waves)."""
+    from graph_analyzer_amd import gen
+    assert check(ctx, gen.rmat(16, 8 << 16, 0.57, 0.19, 0.19, 7), "rmat16") == 77

@@ -114,7 +114,8 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
     """The rows next to the count on the same workload, each checked against the CPU oracle and timed beside the
     reference's own CPU code (oracle/_ref; the oracle restatement when that is absent), 1 thread:
     GPU text ingest (load_graph, graph.h:69-96) and the binary CSR file; GPU prune_to_core (dsg.h:636-645) at
-    ceil(kmax-core density) on the device core numbers."""
+    ceil(kmax-core density) on the device core numbers; the k-truss decomposition (GPU time, checked against the
+    per-edge supports)."""
     import tempfile
 
     import graph_analyzer_amd as ga
@@ -159,6 +160,7 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
                          "binary_bytes": os.path.getsize(bpath), "binary_load_wall_ms": min(bwalls), "binary_load_device_ms": min(bdevs)}
     log(f"ingest: {size / 1e6:.0f} MB text, GPU parse+build {min(devs):.1f} ms device / {min(walls):.1f} ms wall "
         f"(file read + upload); CPU load_graph {t_cpu:.2f} s; binary load {min(bwalls):.1f} ms wall")
+    _, sup_full = g.edge_support(ga.TC_HASH)
     c = ga.Core(g, True)
     core, rounds, kmax = c.run(T_full=T0)
     T_k, _ = g.vertex_participation((core >= kmax).astype(np.uint8))
@@ -182,6 +184,18 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
     log(f"prune: k={k} n {g.n} -> {sub.n}, m {g.m} -> {sub.m}: GPU {min(pms):.2f} ms, CPU prune_to_core {t_cpu:.2f} s")
     sub.close()
     c.close()
+    # k-truss: device time of the whole decomposition (support count included); best of 2. No CPU leg here: the
+    # oracle's sequential peeling needs minutes on this graph.
+    tr = ga.Truss(g)
+    tms = []
+    for _ in range(2):
+        t, rounds, kmax_t = tr.run()
+        tms.append(tr.last_ms)
+    tr.close()
+    if t.min() < 2 or np.any(t.astype(np.int64) > np.asarray(sup_full, np.int64) + 2):
+        raise SystemExit("MISMATCH: a truss number outside [2, support + 2]")
+    out["truss"] = {"gpu_ms": min(tms), "rounds": rounds, "kmax": kmax_t, "edges_per_s": g.m / (min(tms) * 1e-3)}
+    log(f"truss: {rounds} rounds, kmax {kmax_t}, GPU {min(tms):.1f} ms")
     return out
 
 

@@ -4,7 +4,9 @@
 // present edge whose support is at most the level leaves in this round with truss = level + 2.
 //
 // Set-up: per-edge supports from the hash count (F_SUP); an undirected adjacency whose entries carry the oriented edge
-// id (keys (u << nb | w) sorted with the edge id as value, rows by counts + scan).
+// id (keys (u << nb | w) sorted with the edge id as value, rows by counts + scan); for every vertex of degree > 32 an
+// open-addressing membership table {neighbour -> position in the adjacency} with load <= 1/2, so that testing a
+// neighbour of the longer list costs about one or two loads instead of a binary search.
 // The frontier of a round at an unchanged level is exactly the set of edges whose support a decrement of the round
 // before took from level + 1 to level (supports fall by one at a time and atomicSub returns the old value, so each
 // such edge is appended once, by that decrement). Only when that set is empty does the level rise, and only then is the
@@ -75,6 +77,9 @@ struct ga_truss {
     uint32_t* urow = nullptr;     // undirected adjacency with edge ids
     uint32_t* uadj = nullptr;
     uint32_t* ueid = nullptr;
+    unsigned long long* toff = nullptr;   // per vertex: offset of its membership table (vertices of degree > 32)
+    uint8_t* tbits = nullptr;             // per vertex: log2 of its table size, 0 = no table (binary search)
+    uint2* tables = nullptr;              // {neighbour, position in uadj}; key 0xFFFFFFFF = empty
     TrussDev* st = nullptr;
     uint32_t kmax = 0, rounds = 0;
 };
@@ -99,6 +104,43 @@ __global__ void k_tr_unpack(const unsigned long long* __restrict__ keys, uint64_
         const unsigned long long x = keys[i];
         adj[i] = (uint32_t)(x & mask);
         atomicAdd(&rowcnt[(uint32_t)(x >> nb)], 1u);
+    }
+}
+
+constexpr uint32_t TR_HASH_MIN = 32;   // a vertex with more neighbours gets a membership table
+constexpr uint32_t TR_EMPTY = 0xFFFFFFFFu;   // empty table slot (no vertex id: ids are < n <= 2^32 - 1)
+
+__device__ __forceinline__ uint32_t tr_hash(uint32_t w, uint32_t bits) { return (w * 2654435769u) >> (32u - bits); }
+
+// Table size per vertex: the power of two >= 2 x degree (load <= 1/2) for degrees in (32, 2^30], else none.
+__global__ void k_tr_tsize(const uint32_t* __restrict__ urow, uint32_t n, unsigned long long* __restrict__ tsz, uint8_t* __restrict__ tbits) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= n; v += gridDim.x * blockDim.x) {
+        const uint32_t d = v < n ? urow[v + 1] - urow[v] : 0u;
+        uint32_t b = 0;
+        if (d > TR_HASH_MIN && d <= (1u << 30)) b = 64u - (uint32_t)__clzll(2ull * d - 1ull);
+        tsz[v] = b ? (1ull << b) : 0ull;
+        if (v < n) tbits[v] = (uint8_t)b;
+    }
+}
+
+// Inserts every adjacency entry of a vertex with a table; the sorted keys still carry the row (u << nb | w).
+__global__ void k_tr_tinsert(const unsigned long long* __restrict__ keys, uint64_t cnt, int nb, const unsigned long long* __restrict__ toff,
+                             const uint8_t* __restrict__ tbits, uint2* __restrict__ tab) {
+    const unsigned long long mask = (1ull << nb) - 1ull;
+    for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long x = keys[p];
+        const uint32_t u = (uint32_t)(x >> nb), w = (uint32_t)(x & mask);
+        const uint32_t b = tbits[u];
+        if (!b) continue;
+        const unsigned long long base = toff[u], smask = (1ull << b) - 1ull;
+        unsigned long long sl = tr_hash(w, b);
+        for (;;) {   // at most half the slots are taken, so an empty one is always reached
+            if (atomicCAS(&tab[base + sl].x, TR_EMPTY, w) == TR_EMPTY) {
+                tab[base + sl].y = (uint32_t)p;
+                break;
+            }
+            sl = (sl + 1ull) & smask;
+        }
     }
 }
 
@@ -177,18 +219,45 @@ __device__ __forceinline__ void tr_dec(uint32_t* __restrict__ sup, uint32_t x, u
     if (old == level + 1u) next[atomicAdd(&st->nnext, 1u)] = x;
 }
 
+// Position of w in the longer list [bb, be) of vertex lv, or be: its membership table when it has one, else a binary
+// search.
+struct TrTables {
+    const unsigned long long* toff;
+    const uint8_t* tbits;
+    const uint2* tab;
+};
+
+__device__ __forceinline__ uint32_t tr_find(uint32_t w, uint32_t bb, uint32_t be, uint32_t tb, unsigned long long base,
+                                            const uint2* __restrict__ tab, const uint32_t* __restrict__ uadj) {
+    if (tb) {
+        const unsigned long long smask = (1ull << tb) - 1ull;
+        unsigned long long sl = tr_hash(w, tb);
+        for (;;) {   // at most half the slots are taken: an empty slot ends every miss
+            const uint2 t = tab[base + sl];
+            if (t.x == w) return t.y;
+            if (t.x == TR_EMPTY) return be;
+            sl = (sl + 1ull) & smask;
+        }
+    }
+    const uint32_t q = lower_bound_u32(uadj, bb, be, w);
+    return (q < be && uadj[q] == w) ? q : be;
+}
+
 // The triangles that frontier edge e closes through the part [p0, p1) of its shorter list [ab, ae), the longer one being
-// [bb, be); one wave, lanes over consecutive elements.
-__device__ __forceinline__ void tr_walk(uint32_t e, uint32_t p0, uint32_t p1, uint32_t bb, uint32_t be, const uint32_t* __restrict__ uadj,
-                                        const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state, uint32_t* __restrict__ sup,
-                                        uint32_t level, uint32_t* __restrict__ next, TrussDev* st) {
+// [bb, be) of vertex lv; one wave, lanes over consecutive elements.
+__device__ __forceinline__ void tr_walk(uint32_t e, uint32_t p0, uint32_t p1, uint32_t bb, uint32_t be, uint32_t lv, TrTables T,
+                                        const uint32_t* __restrict__ uadj, const uint32_t* __restrict__ ueid,
+                                        const uint8_t* __restrict__ state, uint32_t* __restrict__ sup, uint32_t level,
+                                        uint32_t* __restrict__ next, TrussDev* st) {
+    const uint32_t tb = T.tbits[lv];
+    const unsigned long long base = tb ? T.toff[lv] : 0ull;
     for (uint32_t p = p0 + lane_id(); p < p1; p += 64) {
         const uint32_t ea = ueid[p];
         const uint32_t sa = state[ea];
         if (sa == 0 || ea == e) continue;
         const uint32_t w = uadj[p];
-        const uint32_t q = lower_bound_u32(uadj, bb, be, w);
-        if (q == be || uadj[q] != w) continue;
+        const uint32_t q = tr_find(w, bb, be, tb, base, T.tab, uadj);
+        if (q == be) continue;
         const uint32_t eb = ueid[q];
         const uint32_t sb = state[eb];
         if (sb == 0) continue;
@@ -204,14 +273,18 @@ __device__ __forceinline__ void tr_walk(uint32_t e, uint32_t p0, uint32_t p1, ui
 }
 
 __device__ __forceinline__ void tr_lists(uint32_t e, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
-                                         const uint32_t* __restrict__ urow, uint32_t& ab, uint32_t& ae, uint32_t& bb, uint32_t& be) {
+                                         const uint32_t* __restrict__ urow, uint32_t& ab, uint32_t& ae, uint32_t& bb, uint32_t& be,
+                                         uint32_t& lv) {
     const uint32_t a = src[e], b = col[e];
     ab = urow[a]; ae = urow[a + 1]; bb = urow[b]; be = urow[b + 1];
+    lv = b;
     if (ae - ab > be - bb) {   // stream the shorter list, search the longer
         uint32_t t = ab; ab = bb; bb = t;
         t = ae; ae = be; be = t;
+        lv = a;
     }
 }
+
 
 constexpr uint32_t TR_CHUNK = 256;   // shorter-list elements one wave walks per work unit
 
@@ -222,7 +295,7 @@ __global__ void __launch_bounds__(256) k_tr_update(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ urow, const uint32_t* __restrict__ uadj,
                                                    const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state,
                                                    uint32_t* __restrict__ sup, Bufs F, uint2* __restrict__ chunks, uint32_t chunk_cap,
-                                                   TrussDev* st) {
+                                                   TrTables T, TrussDev* st) {
     if (st->done) return;
     const uint32_t nf = st->nf, level = round_level(st);
     const uint32_t* frontier = F.p[st->fcur];
@@ -230,8 +303,8 @@ __global__ void __launch_bounds__(256) k_tr_update(const uint32_t* __restrict__ 
     const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
     for (uint32_t i = wave0; i < nf; i += nwaves) {
         const uint32_t e = frontier[i];
-        uint32_t ab, ae, bb, be;
-        tr_lists(e, src, col, urow, ab, ae, bb, be);
+        uint32_t ab, ae, bb, be, lv;
+        tr_lists(e, src, col, urow, ab, ae, bb, be, lv);
         uint32_t end = ae;
         if (ae - ab > TR_CHUNK) {
             const uint32_t extra = (ae - ab - 1) / TR_CHUNK;   // chunks after the first
@@ -243,7 +316,7 @@ __global__ void __launch_bounds__(256) k_tr_update(const uint32_t* __restrict__ 
                 end = ab + TR_CHUNK;
             }
         }
-        tr_walk(e, ab, end, bb, be, uadj, ueid, state, sup, level, next, st);
+        tr_walk(e, ab, end, bb, be, lv, T, uadj, ueid, state, sup, level, next, st);
     }
 }
 
@@ -252,16 +325,16 @@ __global__ void __launch_bounds__(256) k_tr_update_chunks(const uint32_t* __rest
                                                           const uint32_t* __restrict__ urow, const uint32_t* __restrict__ uadj,
                                                           const uint32_t* __restrict__ ueid, const uint8_t* __restrict__ state,
                                                           uint32_t* __restrict__ sup, Bufs F, const uint2* __restrict__ chunks,
-                                                          uint32_t chunk_cap, TrussDev* st) {
+                                                          uint32_t chunk_cap, TrTables T, TrussDev* st) {
     if (st->done) return;
     const uint32_t nc = (uint32_t)min(st->nchunk, (unsigned long long)chunk_cap), level = round_level(st);
     uint32_t* next = F.p[st->fcur ^ 1u];
     const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = (gridDim.x * blockDim.x) / 64;
     for (uint32_t i = wave0; i < nc; i += nwaves) {
         const uint2 c = chunks[i];
-        uint32_t ab, ae, bb, be;
-        tr_lists(c.x, src, col, urow, ab, ae, bb, be);
-        tr_walk(c.x, c.y, min(c.y + TR_CHUNK, ae), bb, be, uadj, ueid, state, sup, level, next, st);
+        uint32_t ab, ae, bb, be, lv;
+        tr_lists(c.x, src, col, urow, ab, ae, bb, be, lv);
+        tr_walk(c.x, c.y, min(c.y + TR_CHUNK, ae), bb, be, lv, T, uadj, ueid, state, sup, level, next, st);
     }
 }
 
@@ -344,6 +417,8 @@ extern "C" int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out) {
     if (!out) return ctx ? ga_fail(ctx, GA_ERR_ARG, "out is NULL") : GA_ERR_ARG;
     *out = nullptr;
     if (!ctx || !g) return GA_ERR_ARG;
+    if (2ull * g->m >= 0xFFFFFFFFull)
+        return ga_fail(ctx, GA_ERR_ARG, "graph too large for the k-truss decomposition (2m must fit 32-bit adjacency positions)");
     GA_HIP(ctx, hipSetDevice(ctx->device));
     ga_truss* t = new ga_truss;
     t->ctx = ctx;
@@ -363,6 +438,7 @@ extern "C" int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out) {
     const uint64_t cnt = 2ull * m;
     const int nb = bits_for(g->n);
     DevBuf k0, k1, v1, rc_cnt;
+    unsigned long long* ko = nullptr;
     if (!rc && (ga_malloc(ctx, &k0.p, std::max<uint64_t>(cnt, 1) * 8) || ga_malloc(ctx, &k1.p, std::max<uint64_t>(cnt, 1) * 8) ||
                 ga_malloc(ctx, &v1.p, std::max<uint64_t>(cnt, 1) * 4) || ga_malloc(ctx, &rc_cnt.p, (n + 1) * 4)))
         rc = GA_ERR_OOM;
@@ -370,7 +446,6 @@ extern "C" int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out) {
     if (!rc && m) {
         k_tr_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->src, g->col, g->m, nb, (unsigned long long*)k0.p, t->ueid);
         rc = ga_launch_check(ctx, "k_tr_keys");
-        unsigned long long* ko = nullptr;
         uint32_t* vo = nullptr;
         if (!rc) rc = ga_sort_pairs_u64(ctx, (unsigned long long*)k0.p, (unsigned long long*)k1.p, t->ueid, (uint32_t*)v1.p, cnt, 2 * nb, &ko, &vo);
         if (!rc && vo != t->ueid && hipMemcpyAsync(t->ueid, vo, cnt * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
@@ -381,6 +456,26 @@ extern "C" int ga_truss_create(ga_ctx* ctx, ga_graph* g, ga_truss** out) {
         }
     }
     if (!rc) rc = ga_exclusive_scan_u32(ctx, (const uint32_t*)rc_cnt.p, t->urow, n + 1);
+    // membership tables of the vertices with more than TR_HASH_MIN neighbours
+    DevBuf tsz;
+    if (!rc && (ga_malloc(ctx, &tsz.p, (n + 1) * 8) || ga_malloc(ctx, (void**)&t->toff, (n + 1) * 8) ||
+                ga_malloc(ctx, (void**)&t->tbits, n + 16)))
+        rc = GA_ERR_OOM;
+    if (!rc) {
+        k_tr_tsize<<<grid_for(n + 1, 256, 8192), 256, 0, ctx->stream>>>(t->urow, g->n, (unsigned long long*)tsz.p, t->tbits);
+        rc = ga_launch_check(ctx, "k_tr_tsize");
+    }
+    if (!rc) rc = ga_exclusive_scan_u64(ctx, (const uint64_t*)tsz.p, (uint64_t*)t->toff, n + 1);
+    if (!rc && hipMemcpyAsync(ctx->hstat, t->toff + n, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        rc = ga_fail(ctx, GA_ERR_HIP, "table size readback failed");
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ga_fail(ctx, GA_ERR_HIP, "truss set-up failed");
+    const uint64_t tentries = rc ? 0 : ctx->hstat[0];
+    if (!rc && ga_malloc(ctx, (void**)&t->tables, std::max<uint64_t>(tentries, 1) * 8)) rc = GA_ERR_OOM;
+    if (!rc) rc = ga_fill_u64(ctx, (uint64_t*)t->tables, std::max<uint64_t>(tentries, 1), ~0ull);
+    if (!rc && m && tentries) {
+        k_tr_tinsert<<<grid_for(cnt, 256, 8192), 256, 0, ctx->stream>>>(ko, cnt, nb, t->toff, t->tbits, t->tables);
+        rc = ga_launch_check(ctx, "k_tr_tinsert");
+    }
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = ga_fail(ctx, GA_ERR_HIP, "truss set-up failed");
     if (rc) {
         ga_truss_destroy(t);
@@ -407,6 +502,9 @@ extern "C" void ga_truss_destroy(ga_truss* t) {
     ga_free(t->urow);
     ga_free(t->uadj);
     ga_free(t->ueid);
+    ga_free(t->toff);
+    ga_free(t->tbits);
+    ga_free(t->tables);
     ga_free(t->st);
     delete t;
 }
@@ -430,15 +528,16 @@ extern "C" int ga_truss_run(ga_truss* t, uint32_t* stats, float* ms) {
     const Bufs lists = {{t->plist[0], t->plist[1]}};
     const Bufs fr = {{t->frontier[0], t->frontier[1]}};
     const uint32_t chunk_cap = m / 4 + 1024;
+    const TrTables tabs = {t->toff, t->tbits, t->tables};
     int block_rounds = 1;
     for (uint64_t guard = 0;; guard++) {
         for (int r = 0; r < block_rounds; r++) {
             k_tr_min<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, t->st);
             k_tr_find<<<scan_grid, 256, 0, ctx->stream>>>(t->sup, t->state, lists, fr, t->st);
             k_tr_update<<<upd_grid, 256, 0, ctx->stream>>>(g->src, g->col, t->urow, t->uadj, t->ueid, t->state, t->sup, fr, t->chunks,
-                                                           chunk_cap, t->st);
+                                                           chunk_cap, tabs, t->st);
             k_tr_update_chunks<<<upd_grid, 256, 0, ctx->stream>>>(g->src, g->col, t->urow, t->uadj, t->ueid, t->state, t->sup, fr, t->chunks,
-                                                                  chunk_cap, t->st);
+                                                                  chunk_cap, tabs, t->st);
             k_tr_retire<<<1024, 256, 0, ctx->stream>>>(t->state, t->truss, fr, t->st);
             k_tr_compact<<<scan_grid, 256, 0, ctx->stream>>>(t->state, lists, t->st);
             k_tr_end<<<1, 1, 0, ctx->stream>>>(t->st);

@@ -1,6 +1,7 @@
 // ga_io.hip - graph ingest on the GPU: the SNAP text parse of load_graph (graph.h:69-96) and a binary CSR file format.
 //
-// Text. The host reads the file into pinned memory and uploads it once; the parse runs on the device with the exact
+// Text. The host streams the file through two 64 MiB pinned buffers into device memory (reading one chunk while the
+// previous one is copied); the parse runs on the device with the exact
 // rules of the reference's loop over fgets(line, 512) + strtoul(base 10):
 //   * a physical line [s, e) (e just past its '\n', or the end of the file) reaches the loop as fgets chunks of at most
 //     511 bytes: [s, s+511), [s+511, s+1022), ..., each handled as a line of its own;
@@ -198,24 +199,75 @@ struct HostPinned {   // scoped pinned host allocation
 
 }  // namespace
 
-extern "C" int ga_graph_from_text(ga_ctx* ctx, const char* bytes, uint64_t len, ga_graph** out, float* ms) {
-    if (!out) return ctx ? ga_fail(ctx, GA_ERR_ARG, "out is NULL") : GA_ERR_ARG;
-    *out = nullptr;
-    if (!ctx) return GA_ERR_ARG;
-    if (len && !bytes) return ga_fail(ctx, GA_ERR_ARG, "bytes is NULL");
-    GA_HIP(ctx, hipSetDevice(ctx->device));
+// Streams len bytes of f (from its current position) to the device through two 64 MiB pinned buffers, reading one
+// chunk while the previous one is copied: bytes [0, len1) go to dst1, the rest to dst2. Captures the 4-byte words at
+// the file offsets in `at` (multiples of 4, so no word straddles two chunks) and the last byte.
+struct Capture {
+    uint64_t at;
+    uint32_t* out;
+};
+
+static int stream_file(ga_ctx* ctx, FILE* f, const char* path, uint64_t len, uint8_t* dst1, uint64_t len1, uint8_t* dst2, Capture* cap,
+                       int ncap, char* last) {
+    constexpr size_t STAGE = 64u << 20;
+    HostPinned hb[2];
+    hipEvent_t done[2] = {nullptr, nullptr};
+    struct Evs {
+        hipEvent_t* e;
+        ~Evs() { for (int i = 0; i < 2; i++) if (e[i]) (void)hipEventDestroy(e[i]); }
+    } evs{done};
+    struct Drain {   // runs before the staging buffers are freed, on every return: no copy out of them is in flight
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{ctx->stream};
+    if (!len) return GA_OK;
+    for (int i = 0; i < 2; i++) {
+        if (hipHostMalloc(&hb[i].p, STAGE) != hipSuccess) return ga_fail(ctx, GA_ERR_OOM, "pinned staging failed");
+        GA_HIP(ctx, hipEventCreate(&done[i]));
+    }
+    uint64_t pos = 0;
+    for (int i = 0; pos < len; i ^= 1) {
+        GA_HIP(ctx, hipEventSynchronize(done[i]));   // the copy out of this buffer (two chunks back) has finished
+        const size_t want = (size_t)std::min<uint64_t>(STAGE, len - pos);
+        const uint8_t* hp = (const uint8_t*)hb[i].p;
+        if (fread(hb[i].p, 1, want, f) != want) return ga_fail(ctx, GA_ERR_ARG, "short read of %s", path);
+        for (int c = 0; c < ncap; c++)
+            if (cap[c].at >= pos && cap[c].at + 4 <= pos + want) std::memcpy(cap[c].out, hp + (cap[c].at - pos), 4);
+        const uint64_t end = pos + want;
+        if (pos < len1) {
+            const uint64_t k = std::min<uint64_t>(end, len1) - pos;
+            GA_HIP(ctx, hipMemcpyAsync(dst1 + pos, hp, k, hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (end > len1) {
+            const uint64_t from = std::max<uint64_t>(pos, len1);
+            GA_HIP(ctx, hipMemcpyAsync(dst2 + (from - len1), hp + (from - pos), end - from, hipMemcpyHostToDevice, ctx->stream));
+        }
+        GA_HIP(ctx, hipEventRecord(done[i], ctx->stream));
+        if (last) *last = (char)hp[want - 1];
+        pos = end;
+    }
+    return GA_OK;
+}
+
+// The device text buffer: nvec 16-byte vectors, the last one zeroed first so the padding after the text is zero.
+static int text_buffer(ga_ctx* ctx, uint64_t len, DevBuf& db) {
+    const uint64_t nvec = (len + 15) / 16;
+    GA_TRY(ga_malloc(ctx, &db.p, std::max<uint64_t>(nvec, 1) * 16));
+    if (nvec) GA_TRY(ga_fill_u32(ctx, (uint32_t*)((uint8_t*)db.p + (nvec - 1) * 16), 4, 0));
+    return GA_OK;
+}
+
+// Parse and build from the text in db (len bytes, zero-padded to 16); `tail`: the last byte is not '\n' (a final line
+// without newline). Frees the text buffer once the pairs are made. ms: device time from the first parse kernel.
+static int text_to_graph(ga_ctx* ctx, DevBuf& db, uint64_t len, bool tail, ga_graph** out, float* ms) {
     const uint64_t nvec = (len + 15) / 16;
     const uint64_t ntiles = std::max<uint64_t>((nvec + 255) / 256, 1);
     if (ntiles > 0xFFFFFFFFull) return ga_fail(ctx, GA_ERR_ARG, "text too large");
-    DevBuf db, tcnt, toff, nlb, pb, curb;
-    GA_TRY(ga_malloc(ctx, &db.p, std::max<uint64_t>(nvec, 1) * 16));
+    DevBuf tcnt, toff, nlb, pb, curb;
     GA_TRY(ga_malloc(ctx, &tcnt.p, (ntiles + 1) * 4));
     GA_TRY(ga_malloc(ctx, &toff.p, (ntiles + 1) * 4));
     GA_TRY(ga_malloc(ctx, &curb.p, 64));
     unsigned long long* cursor = (unsigned long long*)curb.p;
-    // zero padding after the text (at most 15 bytes), then the text itself
-    if (nvec) GA_TRY(ga_fill_u32(ctx, (uint32_t*)((uint8_t*)db.p + (nvec - 1) * 16), 4, 0));
-    if (len) GA_HIP(ctx, hipMemcpyAsync(db.p, bytes, len, hipMemcpyHostToDevice, ctx->stream));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)cursor, 1, 0));
     GA_TRY(ga_fill_u32(ctx, (uint32_t*)tcnt.p, ntiles + 1, 0));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -240,9 +292,6 @@ extern "C" int ga_graph_from_text(ga_ctx* ctx, const char* bytes, uint64_t len, 
         k_nl_write<<<(unsigned)ntiles, 256, 0, ctx->stream>>>((const uint4*)db.p, nvec, (const uint32_t*)toff.p, (unsigned long long*)nlb.p);
         GA_TRY(ga_launch_check(ctx, "k_nl_write"));
     }
-    // the last byte decides whether a final line without '\n' exists
-    bool tail = false;
-    if (len) tail = bytes[len - 1] != '\n';
     const uint64_t lines = nnl + (tail ? 1 : 0);
     const uint64_t extra = len / FGETS_MAX + 1;   // bound on the further chunks of long lines
     GA_TRY(ga_malloc(ctx, &pb.p, std::max<uint64_t>(lines + extra, 1) * 8));
@@ -268,23 +317,40 @@ extern "C" int ga_graph_from_text(ga_ctx* ctx, const char* bytes, uint64_t len, 
     return GA_OK;
 }
 
+extern "C" int ga_graph_from_text(ga_ctx* ctx, const char* bytes, uint64_t len, ga_graph** out, float* ms) {
+    if (!out) return ctx ? ga_fail(ctx, GA_ERR_ARG, "out is NULL") : GA_ERR_ARG;
+    *out = nullptr;
+    if (!ctx) return GA_ERR_ARG;
+    if (len && !bytes) return ga_fail(ctx, GA_ERR_ARG, "bytes is NULL");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf db;
+    GA_TRY(text_buffer(ctx, len, db));
+    if (len) GA_HIP(ctx, hipMemcpyAsync(db.p, bytes, len, hipMemcpyHostToDevice, ctx->stream));
+    return text_to_graph(ctx, db, len, len && bytes[len - 1] != '\n', out, ms);
+}
+
+// The file is streamed to the device (stream_file): the host never holds the whole text, and reading overlaps the
+// copies.
 extern "C" int ga_graph_load_text(ga_ctx* ctx, const char* path, ga_graph** out, float* ms) {
     if (!out) return ctx ? ga_fail(ctx, GA_ERR_ARG, "out is NULL") : GA_ERR_ARG;
     *out = nullptr;
     if (!ctx || !path) return GA_ERR_ARG;
     FILE* f = fopen(path, "rb");
     if (!f) return ga_fail(ctx, GA_ERR_ARG, "cannot open %s", path);
-    fseek(f, 0, SEEK_END);
+    struct File {
+        FILE* f;
+        ~File() { fclose(f); }
+    } file{f};
+    if (fseek(f, 0, SEEK_END) != 0) return ga_fail(ctx, GA_ERR_ARG, "cannot size %s", path);
     const long sz = ftell(f);
-    fseek(f, 0, SEEK_SET);
-    if (sz < 0) { fclose(f); return ga_fail(ctx, GA_ERR_ARG, "cannot size %s", path); }
-    HostPinned hb;
+    if (sz < 0 || fseek(f, 0, SEEK_SET) != 0) return ga_fail(ctx, GA_ERR_ARG, "cannot size %s", path);
+    const uint64_t len = (uint64_t)sz;
     GA_HIP(ctx, hipSetDevice(ctx->device));
-    if (hipHostMalloc(&hb.p, std::max<size_t>((size_t)sz, 1)) != hipSuccess) { fclose(f); return ga_fail(ctx, GA_ERR_OOM, "pinned staging failed"); }
-    const size_t got = sz ? fread(hb.p, 1, (size_t)sz, f) : 0;
-    fclose(f);
-    if (got != (size_t)sz) return ga_fail(ctx, GA_ERR_ARG, "short read of %s", path);
-    return ga_graph_from_text(ctx, (const char*)hb.p, (uint64_t)sz, out, ms);
+    DevBuf db;
+    GA_TRY(text_buffer(ctx, len, db));
+    char last = '\n';
+    GA_TRY(stream_file(ctx, f, path, len, (uint8_t*)db.p, len, nullptr, nullptr, 0, &last));
+    return text_to_graph(ctx, db, len, len && last != '\n', out, ms);
 }
 
 extern "C" int ga_graph_save(ga_ctx* ctx, const ga_graph* g, const char* path) {
@@ -324,27 +390,26 @@ extern "C" int ga_graph_load(ga_ctx* ctx, const char* path, ga_graph** out, floa
         return ga_fail(ctx, GA_ERR_ARG, "%s: header sizes do not match the file (n %llu, m %llu, %ld bytes)", path,
                        (unsigned long long)h.n, (unsigned long long)h.m, sz);
     }
-    fseek(f, sizeof h, SEEK_SET);
     const uint32_t n = (uint32_t)h.n, m = (uint32_t)h.m;
-    const size_t words = (size_t)n + 1 + m;
-    HostPinned hb;
     GA_HIP(ctx, hipSetDevice(ctx->device));
-    if (hipHostMalloc(&hb.p, words * 4) != hipSuccess) { fclose(f); return ga_fail(ctx, GA_ERR_OOM, "pinned staging failed"); }
-    const size_t got = fread(hb.p, 4, words, f);
-    fclose(f);
-    if (got != words) return ga_fail(ctx, GA_ERR_ARG, "short read of %s", path);
-    const uint32_t* hr = (const uint32_t*)hb.p;
-    if (hr[0] != 0 || hr[n] != m) return ga_fail(ctx, GA_ERR_ARG, "%s: rowptr must start at 0 and end at m", path);
     ga_graph* g = new ga_graph;
     g->ctx = ctx;
     g->n = n;
     g->m = m;
     auto fail_g = [&](int rc) { ga_graph_destroy(g); return rc; };
+    auto fail_gf = [&](int rc) { fclose(f); return fail_g(rc); };
     DevBuf degb, outb, badb;
     if (ga_malloc(ctx, (void**)&g->rowptr, ((size_t)n + 1) * 4) || ga_malloc(ctx, (void**)&g->col, std::max<size_t>(m, 1) * 4) ||
         ga_malloc(ctx, (void**)&g->src, std::max<size_t>(m, 1) * 4) || ga_malloc(ctx, &degb.p, ((size_t)n + 1) * 4) ||
         ga_malloc(ctx, &outb.p, ((size_t)n + 1) * 4) || ga_malloc(ctx, &badb.p, 64))
-        return fail_g(GA_ERR_OOM);
+        return fail_gf(GA_ERR_OOM);
+    if (fseek(f, sizeof h, SEEK_SET) != 0) return fail_gf(ga_fail(ctx, GA_ERR_ARG, "cannot read %s", path));
+    uint32_t r0 = 1, rn = 0;
+    Capture cap[2] = {{0, &r0}, {4ull * n, &rn}};
+    int src = stream_file(ctx, f, path, 4ull * (n + 1) + 4ull * m, (uint8_t*)g->rowptr, 4ull * (n + 1), (uint8_t*)g->col, cap, 2, nullptr);
+    fclose(f);
+    if (src) return fail_g(src);
+    if (r0 != 0 || rn != m) return fail_g(ga_fail(ctx, GA_ERR_ARG, "%s: rowptr must start at 0 and end at m", path));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess) return fail_g(ga_fail(ctx, GA_ERR_HIP, "hipEventCreate failed"));
     if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail_g(ga_fail(ctx, GA_ERR_HIP, "hipEventCreate failed")); }
@@ -355,10 +420,7 @@ extern "C" int ga_graph_load(ga_ctx* ctx, const char* path, ga_graph** out, floa
     uint32_t* deg = (uint32_t*)degb.p;
     uint32_t* bad = (uint32_t*)badb.p;
     int rc = GA_OK;
-    if (hipEventRecord(e0, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(g->rowptr, hr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-        (m && hipMemcpyAsync(g->col, hr + n + 1, (size_t)m * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
-        return fail_g(ga_fail(ctx, GA_ERR_HIP, "upload failed"));
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) return fail_g(ga_fail(ctx, GA_ERR_HIP, "event record failed"));
     if ((rc = ga_fill_u32(ctx, deg, (size_t)n + 1, 0)) || (rc = ga_fill_u32(ctx, bad, 2, 0)) || (rc = ga_fill_u32(ctx, (uint32_t*)outb.p, (size_t)n + 1, 0)))
         return fail_g(rc);
     k_csr_check_rows<<<grid_for(n, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->col, n, m, deg, bad);

@@ -79,15 +79,17 @@ int ga_graph_edge_order(ga_ctx* ctx, ga_graph* g, uint32_t* perm);
  * ga_graph_from_pairs. Equal to the host parse followed by ga_graph_from_pairs on every input. ms (may be NULL): device
  * time from the first parse kernel to the finished CSR (the upload of the text excluded). */
 int ga_graph_from_text(ga_ctx* ctx, const char* bytes, uint64_t len, ga_graph** out, float* ms);
-/* ga_graph_from_text on the whole content of the file at `path` (read into pinned host memory, uploaded once). */
+/* ga_graph_from_text on the whole content of the file at `path`, streamed to the device through two pinned 64 MiB
+ * buffers (the host never holds the whole file). */
 int ga_graph_load_text(ga_ctx* ctx, const char* path, ga_graph** out, float* ms);
 /* Binary CSR file of this library (no reference counterpart; the fast admission path next to the text loader): a 64-byte
  * header {magic "GACSR001", n, m, max_out_degree as u64, 32 reserved bytes}, rowptr u32[n+1], col u32[m]. */
 int ga_graph_save(ga_ctx* ctx, const ga_graph* g, const char* path);
 /* Loads a ga_graph_save file. The device checks that the arrays are exactly what the builder makes for their edge set
  * (rowptr from 0 to m and never decreasing, col < n, rows strictly ascending, every edge oriented by the (degree, id)
- * rule of graph.h:43-45, max_out_degree as stored); anything else returns GA_ERR_ARG. ms (may be NULL): device time of
- * upload, checks and src[] derivation. */
+ * rule of graph.h:43-45, max_out_degree as stored); anything else returns GA_ERR_ARG. The file is streamed to the
+ * device through two pinned buffers. ms (may be NULL): device time of the checks and the src[] derivation (the upload,
+ * overlapped with the file read, is not included). */
 int ga_graph_load(ga_ctx* ctx, const char* path, ga_graph** out, float* ms);
 
 /* induced_subgraph (dsg.h:103-114) on the GPU: the subgraph on {v : keep[v] != 0} with new ids in old-id order, rebuilt

@@ -88,6 +88,15 @@ def test_binary_round_trip(ctx, tmp_path):
             assert h.count(ga.TC_HASH)[0] == g.count(ga.TC_HASH)[0]
         g.close()
         h.close()
+    # a graph with no vertices (an empty induced subgraph) round-trips too
+    g = ga.DeviceGraph.from_pairs(ctx, gen.powerlaw_small())
+    e, _ = g.induced(np.zeros(g.n, np.uint8))
+    path = str(tmp_path / "empty.gacsr")
+    e.save(path)
+    h = ga.DeviceGraph.load(ctx, path)
+    assert (h.n, h.m, h.max_out_degree) == (0, 0, 0) and h.download()[0].tolist() == [0]
+    for x in (e, h, g):
+        x.close()
 
 
 def test_binary_rejects_malformed(ctx, tmp_path):

@@ -7,7 +7,7 @@
 //   rounds = ceil(lb). Printed only if every GPU core array equals CPU Alg 3 element-wise, the round counts of all GPU
 //   designs agree, and the pruned graph equals the oracle's prune_to_core with its own core numbers equal to the
 //   restricted ones; exit 1 otherwise.
-// The paper pipeline after pruning (greedy, CoreExact) is not part of this build.
+// The paper pipeline after pruning (greedy, CoreExact) is not part of this library.
 // Integration test: links the CPU oracle as the checker and as the CPU row.
 #include "../../graph_analyzer_amd/host/dsg.h"
 #include "../../oracle/oracle.hpp"

@@ -1,7 +1,8 @@
 // dsg.h (graph_analyzer_amd) - drop-in for the (k,triangle)-core part of the reference's dsg.h API (dsg.h:34-58,
 // 647-751): Dens and its exact comparisons, dsg_row, CoreLoopResult, the backend-templated level loops
 // core_decomp_loop / core_decomp_loop_inc, the GPU backends of gpu_dsg.cu:88-199 over the C ABI, a fused gpu_core_run,
-// core_level_density, and induced_subgraph / prune_to_core (dsg.h:103-114, 636-645) on the device. The paper pipeline
+// core_level_density, induced_subgraph / prune_to_core (dsg.h:103-114, 636-645) and the k-truss decomposition on the
+// device. The paper pipeline
 // (Greedy++, Exact, CoreExact) is not part of this library.
 #pragma once
 #include "densest.h"
@@ -231,6 +232,29 @@ inline CoreLoopResult gpu_core_run(const gaa::DeviceGraph& g, uint64_t T_full, s
     }
     ga_core_destroy(c);
     if (ms) *ms = t;
+    return r;
+}
+
+// k-truss number of every oriented edge (index into col) on the device (ga_truss_*); kmax is 0 for a graph without
+// edges. No reference function: it extends the per-edge supports of for_each_triangle (dsg.h:62-94).
+inline CoreLoopResult gpu_truss_decomposition(const gaa::DeviceGraph& g, std::vector<uint32_t>& truss, float* ms = nullptr) {
+    CoreLoopResult r;
+    ga_truss* t = nullptr;
+    gaa::check(g.ctx(), ga_truss_create(g.ctx(), g.handle(), &t), "ga_truss_create");
+    uint32_t st[2] = {0, 0};
+    float tm = 0;
+    const int rc = ga_truss_run(t, st, &tm);
+    if (rc != GA_OK) {
+        r.ok = false;
+        r.error = ga_ctx_last_error(g.ctx());
+    } else {
+        r.rounds = st[0];
+        r.kmax = st[1];
+        truss.assign(g.m(), 0);
+        gaa::check(g.ctx(), ga_truss_download(t, g.m() ? truss.data() : nullptr), "ga_truss_download");
+    }
+    ga_truss_destroy(t);
+    if (ms) *ms = tm;
     return r;
 }
 

@@ -1,6 +1,7 @@
-"""The C++ drivers (tests/cpp: gpu_tc, gpu_densest, gpu_dsg) end to end on the GPU: they parse a SNAP text file with the
-reference's loader rules, run the drop-in host API over the C ABI, verify every result against the linked CPU oracle and
-print the reference's CSV rows (gpu_tc.cu:14-21, gpu_densest.cu:16-25, gpu_dsg.cu:27-46) only when all of it matches."""
+"""The C++ drivers (tests/cpp: gpu_tc, gpu_densest, gpu_dsg, gpu_truss) end to end on the GPU: they parse a SNAP text
+file with the reference's loader rules, run the drop-in host API over the C ABI, verify every result against the linked
+CPU oracle and print the reference's CSV rows (gpu_tc.cu:14-21, gpu_densest.cu:16-25, gpu_dsg.cu:27-46; gpu_truss has
+no reference driver) only when all of it matches."""
 import os
 import subprocess
 
@@ -34,7 +35,7 @@ def rows(out):
 
 def test_driver_usage_errors():
     """Exit 2 and no rows on usage errors (checked before any device work, so this runs without a GPU)."""
-    for args in (["gpu_tc"], ["gpu_densest"], ["gpu_densest", "g.txt", "1"], ["gpu_dsg"], ["gpu_dsg", "a.txt", "b.txt"]):
+    for args in (["gpu_tc"], ["gpu_densest"], ["gpu_densest", "g.txt", "1"], ["gpu_dsg"], ["gpu_dsg", "a.txt", "b.txt"], ["gpu_truss"]):
         rc, out, _ = run(args)
         assert rc == 2 and out == "", args
 
@@ -78,3 +79,13 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     sub, _, _ = og.prune_to_core(core, k)
     p = rs["prune_kcore"]
     assert (int(p[2]), int(p[3]), int(p[4]), int(p[10]), int(p[11])) == (sub.n, sub.m, sub.triangles(), k, kmax)
+
+
+@pytest.mark.gpu
+def test_gpu_truss_driver(graph_file):
+    path, og = graph_file
+    rc, out, err = run(["gpu_truss", path, "--repeats", "1"])
+    assert rc == 0, err[-2000:]
+    r = rows(out)
+    _, kmax = og.truss()
+    assert len(r) == 1 and r[0][0] == "gpu_truss" and (int(r[0][2]), int(r[0][3]), int(r[0][4])) == (og.n, og.m, kmax)

@@ -237,9 +237,13 @@ __global__ void k_tc_hash(const TcArgs a) {
         }
         uint32_t tbits = 0, tsz = 1;
         if (HASHED) {
-            // ceil(log2(dh)) + shift, at least 2: never more than the bucket's slots (bucket_slots uses the same rule)
+            // ceil(log2(dh)) + shift, at least 2, at most the group's reserved slots: the reservation holds the bucket's
+            // largest row at load shift 2 (table at most a quarter full), so the largest rows of a bucket get shift 2
+            // and the smaller ones the tuned shift
             tbits = (dh > 1 ? 32u - (uint32_t)__clz(dh - 1) : 0u) + a.load_shift;
             if (tbits < 2) tbits = 2;
+            const uint32_t tcap = 31u - (uint32_t)__clz(a.ts_max);
+            if (tbits > tcap) tbits = tcap;
             tsz = 1u << tbits;
             for (uint32_t w = sub; w < tsz / 4u; w += G) {   // 16-byte stores (tab and tix are 16-byte aligned)
                 ((uint4*)tab)[w] = make_uint4(GA_EMPTY, GA_EMPTY, GA_EMPTY, GA_EMPTY);
@@ -1164,7 +1168,9 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     uint32_t block = (uint32_t)P.tune.block;
     // tables with per-slot counters (participation, supports) take twice the LDS: at most load shift 2 there
     const uint32_t shift = (FLAGS & (F_TRI | F_SUP)) ? std::min<uint32_t>((uint32_t)P.tune.load_shift, 2u) : (uint32_t)P.tune.load_shift;
-    const uint32_t ts = HASHED ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), shift) : 0;
+    // LDS reserved per lane group: the bucket's largest row at load shift <= 2 (a quarter full at most). Reserving for
+    // shift 3 doubled the tables of the 32- and 64-lane buckets and held them to 4 waves per SIMD by LDS.
+    const uint32_t ts = HASHED ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), std::min<uint32_t>(shift, 2u)) : 0;
     const uint32_t per_group = (ts * ((FLAGS & (F_SUP | F_TRI)) ? 2u : 1u) + 4u * G) * 4u;
     while (block > 64 && (size_t)(block / G) * per_group > 64 * 1024) block >>= 1;
     const size_t shmem = (size_t)(block / G) * per_group;

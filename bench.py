@@ -114,8 +114,8 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
     """The rows next to the count on the same workload, each checked against the CPU oracle and timed beside the
     reference's own CPU code (oracle/_ref; the oracle restatement when that is absent), 1 thread:
     GPU text ingest (load_graph, graph.h:69-96) and the binary CSR file; GPU prune_to_core (dsg.h:636-645) at
-    ceil(kmax-core density) on the device core numbers; the k-truss decomposition (GPU time, checked against the
-    per-edge supports)."""
+    ceil(kmax-core density) on the device core numbers; Greedy++ (dsg.h:189-248) with 10 passes on the pruned graph;
+    the k-truss decomposition (GPU time, checked against the per-edge supports)."""
     import tempfile
 
     import graph_analyzer_amd as ga
@@ -182,6 +182,20 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
     out["prune"] = {"k": int(k), "kmax": int(kmax), "core_rounds": int(rounds), "core_ms": c.last_ms, "n_in": g.n, "m_in": g.m,
                     "n_out": sub.n, "m_out": sub.m, "gpu_ms": min(pms), "cpu_prune_to_core_s": t_cpu}
     log(f"prune: k={k} n {g.n} -> {sub.n}, m {g.m} -> {sub.m}: GPU {min(pms):.2f} ms, CPU prune_to_core {t_cpu:.2f} s")
+    # Greedy++ with 10 passes on the pruned graph (gpu_dsg.cu:344-356): device time, best of 3, against the CPU's
+    passes = 10
+    if sub.n <= ctx.greedy_max_vertices():
+        gms = []
+        for _ in range(3):
+            gt, gv, gs, gpp, ms = sub.greedy_pp(passes)
+            gms.append(ms)
+        t0 = time.perf_counter()
+        ct, cv, cs, cpp = osub.greedy_pp(passes)
+        t_g = time.perf_counter() - t0
+        if (gt, gv, gpp) != (ct, cv, cpp) or not np.array_equal(gs, cs):
+            raise SystemExit("MISMATCH: GPU greedy_pp differs from the CPU's")
+        out["greedy_pp"] = {"passes": passes, "n": sub.n, "m": sub.m, "tri": gt, "verts": gv, "gpu_ms": min(gms), "cpu_greedy_pp_s": t_g}
+        log(f"greedy++: {passes} passes on the pruned graph: {gt}/{gv}, GPU {min(gms):.2f} ms, CPU greedy_pp {t_g:.2f} s")
     sub.close()
     c.close()
     # k-truss: device time of the whole decomposition (support count included); best of 2. No CPU leg here: the

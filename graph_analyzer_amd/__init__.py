@@ -106,6 +106,8 @@ def lib() -> ctypes.CDLL:
             "ga_truss_destroy": (None, [_vp]),
             "ga_truss_run": (ctypes.c_int, [_vp, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_truss_download": (ctypes.c_int, [_vp, _u32p]),
+            "ga_greedy_pp": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u64p, _u64p, _u64p, _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_greedy_max_vertices": (ctypes.c_uint32, [_vp]),
             "ga_core_prune": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
@@ -186,6 +188,10 @@ class Context:
         cus, wave, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
         self.check(lib().ga_ctx_device_info(self._h, name, 256, ctypes.byref(cus), ctypes.byref(wave), ctypes.byref(hbm)), "device_info")
         return {"name": name.value.decode(), "cus": cus.value, "wave": wave.value, "hbm_bytes": hbm.value}
+
+    def greedy_max_vertices(self) -> int:
+        """The largest graph (vertices) DeviceGraph.greedy_pp takes on this device: one workgroup's LDS holds its state."""
+        return int(lib().ga_greedy_max_vertices(self._h))
 
     def mem_info(self):
         """(free, total) device bytes."""
@@ -404,6 +410,20 @@ class DeviceGraph:
             return p.run(num, den, T_full) if fused else p.run_rounds(num, den, T_full)
         finally:
             p.close()
+
+    def greedy_pp(self, passes: int = 1):
+        """Greedy++ for the triangle-density densest subgraph (passes = 1: Charikar's Greedy). Returns (tri, verts, set,
+        per_pass, ms): the best density tri / verts found (0 / 0 without triangles), its vertices ascending, and the best
+        (tri, verts) after every pass. The graph may have at most `ctx.greedy_max_vertices()` vertices."""
+        passes = int(passes)
+        t, v = ctypes.c_uint64(), ctypes.c_uint64()
+        pp = np.zeros(2 * max(passes, 1), np.uint64)
+        st = np.zeros(max(self.n, 1), np.uint32)
+        cnt = ctypes.c_uint32()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_greedy_pp(self.ctx.handle, self._h, passes, ctypes.byref(t), ctypes.byref(v), _ptr(pp, _u64p),
+                                          _ptr(st, _u32p), ctypes.byref(cnt), ctypes.byref(ms)), "greedy_pp")
+        return t.value, v.value, st[: cnt.value], [(int(a), int(b)) for a, b in pp[: 2 * passes].reshape(-1, 2)], ms.value
 
     def truss_decomposition(self):
         """k-truss number of every oriented edge; returns (truss, rounds, kmax)."""

@@ -1,9 +1,9 @@
 // dsg.h (graph_analyzer_amd) - drop-in for the (k,triangle)-core part of the reference's dsg.h API (dsg.h:34-58,
 // 647-751): Dens and its exact comparisons, dsg_row, CoreLoopResult, the backend-templated level loops
 // core_decomp_loop / core_decomp_loop_inc, the GPU backends of gpu_dsg.cu:88-199 over the C ABI, a fused gpu_core_run,
-// core_level_density, induced_subgraph / prune_to_core (dsg.h:103-114, 636-645) and the k-truss decomposition on the
-// device. The paper pipeline
-// (Greedy++, Exact, CoreExact) is not part of this library.
+// core_level_density, induced_subgraph / prune_to_core (dsg.h:103-114, 636-645), Greedy++ / Greedy (dsg.h:189-250) and
+// the k-truss decomposition on the device. The max-flow part of the paper pipeline (Exact, CoreExact) is not part of
+// this library.
 #pragma once
 #include "densest.h"
 
@@ -257,6 +257,47 @@ inline CoreLoopResult gpu_truss_decomposition(const gaa::DeviceGraph& g, std::ve
     if (ms) *ms = tm;
     return r;
 }
+
+// Result record of the paper algorithms (dsg.h:50-60); this library fills it from Greedy++.
+struct DsgResult {
+    uint64_t tri = 0, verts = 0;          // best subgraph: triangles / vertices (0/0 = graph without triangles)
+    std::vector<uint32_t> set;            // its vertices, ascending
+    std::vector<Dens> per_pass;           // best density after pass 1, 2, ..., T
+    uint32_t flow_calls = 0;              // max-flow fields of the reference record: always 0 here
+    uint32_t components_searched = 0;
+    uint32_t shrinks = 0;
+    bool ok = true;
+    std::string error;
+    Dens dens() const { return Dens{tri, verts}; }
+};
+
+// Greedy++ with `passes` passes on a resident graph (ga_greedy_pp): one workgroup runs every pass with the per-vertex
+// state in LDS, so the graph must be small (ga_greedy_max_vertices; the pruned graph of gpu_dsg.cu:344-356 is).
+// A graph that is too large, or a failed self-check, gives ok = false and the library's message in error.
+inline DsgResult greedy_pp_device(const gaa::DeviceGraph& g, uint32_t passes, float* ms = nullptr) {
+    DsgResult r;
+    std::vector<uint64_t> pp(2 * (size_t)std::max<uint32_t>(passes, 1));
+    r.set.resize(std::max<uint32_t>(g.n(), 1));
+    uint32_t cnt = 0;
+    float t = 0;
+    const int rc = ga_greedy_pp(g.ctx(), g.handle(), passes, &r.tri, &r.verts, pp.data(), r.set.data(), &cnt, &t);
+    if (rc != GA_OK) {
+        r.ok = false;
+        r.error = ga_ctx_last_error(g.ctx());
+        r.set.clear();
+        return r;
+    }
+    r.set.resize(cnt);
+    for (uint32_t i = 0; i < passes; i++) r.per_pass.push_back(Dens{pp[2 * i], pp[2 * i + 1]});
+    if (ms) *ms = t;
+    return r;
+}
+
+// The reference's host signatures (dsg.h:189, 250): upload, run on the device.
+inline DsgResult greedy_pp(const Graph& g, uint32_t passes) {
+    return greedy_pp_device(gaa::DeviceGraph::upload(gaa::default_ctx(), g), passes);
+}
+inline DsgResult greedy(const Graph& g) { return greedy_pp(g, 1); }
 
 // Triangles and vertices of {v : core[v] >= k} (dsg.h:169-174), counted on the device.
 inline Dens core_level_density(const gaa::DeviceGraph& g, const std::vector<uint32_t>& core, uint32_t k) {

@@ -60,6 +60,7 @@ def olib():
             "orc_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
             "orc_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
             "orc_truss": (ctypes.c_uint32, [_vp, _u32p]),
+            "orc_greedy_pp": (None, [_vp, ctypes.c_uint32, _u64p, _u32p, _u64p]),
             "orc_brute_truss": (None, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u32p]),
             "orc_brute_participation": (ctypes.c_uint64, [ctypes.c_uint32, _u32p, ctypes.c_uint64, _u8p, _u32p]),
             "orc_brute_triangles": (ctypes.c_uint64, [ctypes.c_uint32, _u32p, ctypes.c_uint64]),
@@ -104,6 +105,7 @@ def rlib():
             "ref_peel_arith_ok": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
             "ref_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
             "ref_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
+            "ref_greedy_pp": (None, [_vp, ctypes.c_uint32, _u64p, _u32p, _u64p]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -111,6 +113,17 @@ def rlib():
             f.argtypes = a
         _rl = L
     return _rl
+
+
+
+def _greedy(fn, h, n, passes):
+    out = np.zeros(4, np.uint64)
+    st = np.zeros(max(n, 1), np.uint32)
+    pp = np.zeros(2 * passes, np.uint64)
+    fn(h, int(passes), _p(out, _u64p), _p(st, _u32p), _p(pp, _u64p))
+    if not out[3]:
+        raise RuntimeError("greedy_pp: triangle bookkeeping inconsistent")
+    return int(out[0]), int(out[1]), st[: int(out[2])], [(int(a), int(b)) for a, b in pp.reshape(-1, 2)]
 
 
 class OGraph:
@@ -185,6 +198,10 @@ class OGraph:
         t = np.zeros(max(self.m, 1), np.uint32)
         kmax = olib().orc_truss(self._h, _p(t, _u32p))
         return t[: self.m], int(kmax)
+
+    def greedy_pp(self, passes=1):
+        """Greedy++ (dsg.h:189-250): (tri, verts, set ascending, per_pass [(tri, verts)] after each pass)."""
+        return _greedy(olib().orc_greedy_pp, self._h, self.n, passes)
 
     def induced(self, keep):
         """induced_subgraph (dsg.h:103-114): (OGraph, old_id)."""
@@ -297,6 +314,9 @@ class RGraph:
         n2 = ctypes.c_uint32()
         h = rlib().ref_induced_subgraph(self._h, _p(k, _u8p), _p(old, _u32p), ctypes.byref(n2))
         return RGraph(h), old[: n2.value]
+
+    def greedy_pp(self, passes=1):
+        return _greedy(rlib().ref_greedy_pp, self._h, self.n, passes)
 
     def prune_to_core(self, core, k):
         c = np.ascontiguousarray(core, dtype=np.uint32)

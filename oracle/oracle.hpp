@@ -14,6 +14,8 @@
 //   participation / peel  brute force on random alive masks, u128 threshold oracle, exhaustive densest (n <= 16)
 //   core numbers          exhaustive definition (n <= 12), fixpoint definition, closed form K_k, the reference build
 //   per-edge support      brute force, sum = 3T, K_k -> k-2 on every edge
+//   greedy_pp             the reference build, exhaustive optimum (n <= 16: never above it, never below a third of it
+//                         in pass 1), closed forms (K_k, disjoint cliques), the returned set's own triangle count
 #pragma once
 #include <algorithm>
 #include <chrono>
@@ -628,6 +630,79 @@ inline void brute_truss(uint32_t n, const EdgeList& e, std::vector<uint32_t>& tr
         for (size_t i = 0; i < e.size(); i++) if (in[i]) { truss[i] = k; any = true; }
         if (!any) return;
     }
+}
+
+// Exact density comparison a/ad > b/bd with 0/0 read as 0 (dsg.h:37-46).
+inline bool dens_gt(uint64_t an, uint64_t ad, uint64_t bn, uint64_t bd) {
+    if (!ad) { an = 0; ad = 1; }
+    if (!bd) { bn = 0; bd = 1; }
+    return (u128)an * bd > (u128)bn * ad;
+}
+
+struct GreedyResult {
+    uint64_t tri = 0, verts = 0;                            // best density found (0/0: no triangle)
+    std::vector<uint32_t> set;                              // its vertices, ascending
+    std::vector<std::pair<uint64_t, uint64_t>> per_pass;    // best (tri, verts) after each pass
+    bool ok = true;
+};
+
+// Greedy++ with triangles as hyperedges (Flowless Alg 2; dsg.h:176-250 greedy_pp, greedy = one pass). Each pass starts
+// from the whole graph and deletes, one at a time, the alive vertex with the smallest (load + tri_H, id), adding tri_H
+// to its load; the current subgraph H is compared with the best before every deletion and replaces it only when
+// strictly denser. This restatement finds the minimum by a scan over the alive vertices (the reference keeps a lazy
+// heap; both take the same vertex) and the triangles through v as the oriented edges a -> b between two alive
+// neighbours of v.
+inline GreedyResult greedy_pp(const Graph& g, uint32_t passes) {
+    GreedyResult r;
+    std::vector<uint32_t> urow, ucol;
+    undirected_csr(g, urow, ucol);
+    std::vector<uint64_t> tri0, cur;
+    const uint64_t T0 = triangle_degrees(g, tri0);
+    std::vector<uint64_t> load(g.n, 0);
+    std::vector<uint8_t> alive, nb(g.n, 0);
+    std::vector<uint32_t> order, best_order;
+    size_t best_cut = 0;
+    for (uint32_t p = 0; p < passes; p++) {
+        cur = tri0;
+        alive.assign(g.n, 1);
+        order.clear();
+        uint64_t T = T0;
+        bool improved = false;
+        size_t cut = 0;
+        for (uint32_t left = g.n; left > 0; left--) {
+            if (dens_gt(T, left, r.tri, r.verts)) { r.tri = T; r.verts = left; improved = true; cut = order.size(); }
+            uint32_t v = UINT32_MAX;
+            for (uint32_t x = 0; x < g.n; x++)
+                if (alive[x] && (v == UINT32_MAX || load[x] + cur[x] < load[v] + cur[v])) v = x;
+            for (uint32_t j = urow[v]; j < urow[v + 1]; j++) nb[ucol[j]] = alive[ucol[j]];
+            uint64_t found = 0;
+            for (uint32_t j = urow[v]; j < urow[v + 1]; j++) {
+                const uint32_t a = ucol[j];
+                if (!nb[a]) continue;
+                for (uint32_t e = g.rowptr[a]; e < g.rowptr[a + 1]; e++) {
+                    const uint32_t b = g.col[e];
+                    if (!nb[b]) continue;
+                    if (!cur[a] || !cur[b]) { r.ok = false; return r; }
+                    cur[a]--;
+                    cur[b]--;
+                    found++;
+                }
+            }
+            for (uint32_t j = urow[v]; j < urow[v + 1]; j++) nb[ucol[j]] = 0;
+            if (found != cur[v]) { r.ok = false; return r; }
+            load[v] += cur[v];
+            alive[v] = 0;
+            T -= cur[v];
+            order.push_back(v);
+        }
+        if (improved) { best_order = order; best_cut = cut; }
+        r.per_pass.push_back({r.tri, r.verts});
+    }
+    if (r.verts) {
+        r.set.assign(best_order.begin() + best_cut, best_order.end());
+        std::sort(r.set.begin(), r.set.end());
+    }
+    return r;
 }
 
 // Fixpoint definition: the k-core is what remains after repeatedly deleting vertices with fewer than k triangles

@@ -170,6 +170,17 @@ uint32_t orc_truss(const orc_graph* h, uint32_t* truss) {
     return kmax;
 }
 
+// Greedy++ (passes >= 1): out = {tri, verts, set_size, ok}; set (capacity n), per_pass (2 * passes).
+void orc_greedy_pp(const orc_graph* h, uint32_t passes, uint64_t* out, uint32_t* set, uint64_t* per_pass) {
+    const GreedyResult r = greedy_pp(h->g, passes);
+    out[0] = r.tri;
+    out[1] = r.verts;
+    out[2] = r.set.size();
+    out[3] = r.ok ? 1 : 0;
+    std::copy(r.set.begin(), r.set.end(), set);
+    for (size_t i = 0; i < r.per_pass.size(); i++) { per_pass[2 * i] = r.per_pass[i].first; per_pass[2 * i + 1] = r.per_pass[i].second; }
+}
+
 // k-truss by definition on an undirected unique loop-free pair list (truss per pair).
 void orc_brute_truss(uint32_t n, const uint32_t* pairs, uint64_t k, uint32_t* truss) {
     EdgeList e(k);

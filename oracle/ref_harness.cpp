@@ -115,6 +115,17 @@ ref_graph* ref_prune_to_core(const ref_graph* h, const uint32_t* core, uint64_t 
     return o;
 }
 
+// Reference greedy_pp (dsg.h:189-248): out = {tri, verts, set_size, ok}; set (capacity n), per_pass (2 * passes).
+void ref_greedy_pp(const ref_graph* h, uint32_t passes, uint64_t* out, uint32_t* set, uint64_t* per_pass) {
+    const DsgResult r = greedy_pp(h->g, passes);
+    out[0] = r.tri;
+    out[1] = r.verts;
+    out[2] = r.set.size();
+    out[3] = r.ok ? 1 : 0;
+    std::copy(r.set.begin(), r.set.end(), set);
+    for (size_t i = 0; i < r.per_pass.size(); i++) { per_pass[2 * i] = r.per_pass[i].num; per_pass[2 * i + 1] = r.per_pass[i].den; }
+}
+
 int ref_peel_arith_ok(uint32_t n, uint64_t T0, uint64_t num, uint64_t den) { return peel_arith_ok(n, T0, num, den) ? 1 : 0; }
 
 }  // extern "C"

@@ -1,30 +1,39 @@
 // gpu_dsg.cpp - (k,triangle)-core driver with the core part of the reference's contract (gpu_dsg.cu:1-46, 201-316):
-//   usage: gpu_dsg graph.txt [--repeats N] [--with-recount] [--no-burnin] [--verbose]
+//   usage: gpu_dsg graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedypp] [--verbose]
 //   rows impl,graph,n,m,triangles,vertices,density,time_ms,h2d_ms,total_ms,rounds,kmax for cpu_core_decomp (Alg 3),
 //   gpu_core_decomp (incremental host loop, one sync per round), gpu_core_decomp_fused (ga_core_run) and, with
 //   --with-recount, gpu_core_decomp_recount. Rows = the kmax-core. Then prune_kcore (gpu_dsg.cu:312-328): lower bound =
 //   the kmax-core density, keep core >= ceil(lb) on the device core numbers (ga_core_prune), row n,m = the pruned graph,
 //   rounds = ceil(lb). Printed only if every GPU core array equals CPU Alg 3 element-wise, the round counts of all GPU
 //   designs agree, and the pruned graph equals the oracle's prune_to_core with its own core numbers equal to the
-//   restricted ones; exit 1 otherwise.
-// The paper pipeline after pruning (greedy, CoreExact) is not part of this library.
+//   restricted ones; exit 1 otherwise. Then Greedy++ with T passes on the pruned graph (gpu_dsg.cu:344-356):
+//   cpu_greedy_pp_pruned (the oracle) and gpu_greedy_pp_pruned (ga_greedy_pp), rounds = T; the GPU result (density, set
+//   and the best after every pass) must equal the oracle's. --no-greedypp skips it, and so does a pruned graph larger
+//   than ga_greedy_max_vertices (said on stderr).
+// The max-flow part of the pipeline (CoreExact) is not part of this library.
 // Integration test: links the CPU oracle as the checker and as the CPU row.
 #include "../../graph_analyzer_amd/host/dsg.h"
 #include "../../oracle/oracle.hpp"
 
 int main(int argc, char** argv) {
     std::vector<const char*> pos;
-    bool recount = false, burnin = true, verbose = false;
+    bool recount = false, burnin = true, verbose = false, gpp = true;
     int repeats = 5;
+    uint32_t T_pass = 10;
     for (int i = 1; i < argc; i++) {
         if (strcmp(argv[i], "--with-recount") == 0) recount = true;
         else if (strcmp(argv[i], "--no-burnin") == 0) burnin = false;
         else if (strcmp(argv[i], "--verbose") == 0) verbose = true;
+        else if (strcmp(argv[i], "--no-greedypp") == 0) gpp = false;
+        else if (strcmp(argv[i], "--T") == 0 && i + 1 < argc) T_pass = (uint32_t)std::max(1, atoi(argv[++i]));
         else if (strcmp(argv[i], "--repeats") == 0 && i + 1 < argc) repeats = std::max(1, atoi(argv[++i]));
         else if (argv[i][0] == '-') {}
         else pos.push_back(argv[i]);
     }
-    if (pos.size() != 1) { fprintf(stderr, "usage: %s graph.txt [--repeats N] [--with-recount] [--no-burnin] [--verbose]\n", argv[0]); return 2; }
+    if (pos.size() != 1) {
+        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedypp] [--verbose]\n", argv[0]);
+        return 2;
+    }
     const char* path = pos[0];
     ga_ctx* ctx = gaa::default_ctx();
     std::vector<uint32_t> pairs;
@@ -135,11 +144,46 @@ int main(int argc, char** argv) {
                     (unsigned long long)kverts, (unsigned long long)kprune, gp.n, gp.m, og.n, og.m);
         }
     }
+    // ---- Greedy++ on the pruned graph (gpu_dsg.cu:344-356) ----
+    DsgResult gpr;
+    double t_gpp = 1e300, t_gpp_cpu = 0;
+    bool have_gpp = false;
+    if (ok && gpp && gp.n > ga_greedy_max_vertices(ctx)) {
+        fprintf(stderr, "pruned graph n=%u exceeds greedy_pp's %u vertices: no Greedy++ rows\n", gp.n, ga_greedy_max_vertices(ctx));
+    } else if (ok && gpp) {
+        gaa::DeviceGraph dgp = gaa::DeviceGraph::upload(ctx, gp);
+        for (int i = -1; i < repeats && ok; i++) {
+            float ms = 0;
+            gpr = greedy_pp_device(dgp, T_pass, &ms);
+            if (!gpr.ok) { fprintf(stderr, "greedy_pp failed: %s\n", gpr.error.c_str()); ok = false; }
+            if (i >= 0) t_gpp = std::min(t_gpp, (double)ms);
+        }
+        orc::Graph op;
+        std::vector<uint32_t> oold, ocore;
+        orc::prune_to_core(og, core_cpu, kprune, op, oold, ocore);
+        const double tc = now_ms();
+        const orc::GreedyResult cg = orc::greedy_pp(op, T_pass);
+        t_gpp_cpu = now_ms() - tc;
+        bool same = ok && cg.ok && cg.tri == gpr.tri && cg.verts == gpr.verts && cg.set == gpr.set && cg.per_pass.size() == gpr.per_pass.size();
+        for (size_t i = 0; same && i < cg.per_pass.size(); i++)
+            same = cg.per_pass[i].first == gpr.per_pass[i].num && cg.per_pass[i].second == gpr.per_pass[i].den;
+        if (!same) {
+            fprintf(stderr, "MISMATCH: GPU greedy_pp differs from the oracle's\n");
+            ok = false;
+        } else {
+            have_gpp = true;
+            fprintf(stderr, "greedy++ (%u passes) on the pruned graph: %llu/%llu\n", T_pass, (unsigned long long)gpr.tri, (unsigned long long)gpr.verts);
+        }
+    }
     if (!ok) { fprintf(stderr, "VERIFICATION FAILED - no timing rows printed\n"); return 1; }
     rows.push_back(dsg_row("gpu_core_decomp", path, og.n, og.m, ktri, kverts, best_inc, 0, inc.rounds, kmax));
     rows.push_back(dsg_row("gpu_core_decomp_fused", path, og.n, og.m, ktri, kverts, best_fused, 0, fused_rounds, kmax));
     if (recount) rows.push_back(dsg_row("gpu_core_decomp_recount", path, og.n, og.m, ktri, kverts, best_rc, 0, rc_rounds, kmax));
     rows.push_back(dsg_row("prune_kcore", path, gp.n, gp.m, T_pruned, gp.n, t_prune, 0, kprune, kmax));
+    if (have_gpp) {
+        rows.push_back(dsg_row("cpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp_cpu, 0, T_pass, 0));
+        rows.push_back(dsg_row("gpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp, 0, T_pass, 0));
+    }
     for (auto& r : rows) puts(r.c_str());
     fflush(stdout);
     fprintf(stderr, "GPU core numbers verified element-wise against CPU Alg 3 (kmax %u, %u rounds)\n", kmax, inc.rounds);

@@ -7,8 +7,8 @@ Inputs: the literal zoo of the reference's tiny suite, and seeded random graphs 
 (Erdos-Renyi with 30% duplicate reversed edges and a self loop, gen_test_graph.py; preferential attachment,
 gen_powerlaw.py), written as SNAP text so the reference's own load_graph parses them. Outputs per graph: the oriented
 CSR, T, per-vertex and per-edge counts, batch-peel results for five eps values, Alg-3 core numbers with kmax and the
-kmax-core density, the v9 edge order, and prune_to_core (induced subgraph on core >= k) at k in {0, 1, ceil(kmax/2), kmax,
-kmax + 1}.
+kmax-core density, the v9 edge order, prune_to_core (induced subgraph on core >= k) at k in {0, 1, ceil(kmax/2), kmax,
+kmax + 1}, and greedy_pp with 1, 3 and 10 passes (best density, its vertex set, the best after every pass).
 """
 import json
 import os
@@ -65,6 +65,10 @@ def record(name, raw_pairs, tmpdir):
         pr, pc, _ = pg.arrays()
         prune.append({"k": k, "n": pg.n, "m": pg.m, "max_out_degree": pg.max_out_degree, "rowptr": pr.tolist(), "col": pc.tolist(),
                       "old_id": old.tolist(), "core_out": cout.tolist()})
+    greedy = {}
+    for passes in (1, 3, 10):
+        gt, gv, gs, gpp = g.greedy_pp(passes)
+        greedy[str(passes)] = {"tri": gt, "verts": gv, "set": gs.tolist(), "per_pass": gpp}
     return {
         "name": name,
         "raw_pairs": np.asarray(raw_pairs, dtype=np.uint32).tolist(),
@@ -75,6 +79,7 @@ def record(name, raw_pairs, tmpdir):
         "edge_order": g.edge_order().tolist(),
         "peel": peel,
         "prune": prune,
+        "greedy_pp": greedy,
     }
 
 

@@ -35,7 +35,8 @@ def rows(out):
 
 def test_driver_usage_errors():
     """Exit 2 and no rows on usage errors (checked before any device work, so this runs without a GPU)."""
-    for args in (["gpu_tc"], ["gpu_densest"], ["gpu_densest", "g.txt", "1"], ["gpu_dsg"], ["gpu_dsg", "a.txt", "b.txt"], ["gpu_truss"]):
+    for args in (["gpu_tc"], ["gpu_densest"], ["gpu_densest", "g.txt", "1"], ["gpu_dsg"], ["gpu_dsg", "a.txt", "b.txt"],
+                 ["gpu_dsg", "--T", "3"], ["gpu_truss"]):
         rc, out, _ = run(args)
         assert rc == 2 and out == "", args
 
@@ -70,7 +71,8 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "2", "--with-recount"])
     assert rc == 0, err[-2000:]
     rs = {r[0]: r for r in rows(out)}
-    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "prune_kcore"}
+    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "prune_kcore",
+                       "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"}
     core, kmax = og.core()
     kt, kv = og.core_level_density(core, kmax)
     for name in ("cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused"):
@@ -79,6 +81,10 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     sub, _, _ = og.prune_to_core(core, k)
     p = rs["prune_kcore"]
     assert (int(p[2]), int(p[3]), int(p[4]), int(p[10]), int(p[11])) == (sub.n, sub.m, sub.triangles(), k, kmax)
+    gt, gv, _, _ = sub.greedy_pp(10)
+    for name in ("cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"):
+        r = rs[name]
+        assert (int(r[2]), int(r[3]), int(r[4]), int(r[5]), int(r[10])) == (sub.n, sub.m, gt, gv, 10), r
 
 
 @pytest.mark.gpu

@@ -1,0 +1,136 @@
+"""GPU parity of Greedy++ (ga_greedy_pp, dsg.h:189-250): the best density, its vertex set and the best after every pass
+equal the reference's on the committed fixtures and the CPU oracle's (itself pinned against the reference build, the
+exhaustive optimum and closed forms in tests/test_oracle.py) on random graphs around the bitmap word size, power-law
+graphs, pruned graphs of the core pipeline, the largest graph one workgroup takes, and degenerate inputs."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import graph_analyzer_amd as ga
+import oracle
+from conftest import clique, random_graph
+from graph_analyzer_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+
+
+def same(got, exp, tag):
+    t, v, s, pp, _ = got
+    et, ev, es, epp = exp
+    assert (t, v) == (et, ev), tag
+    assert np.array_equal(s, np.asarray(es, np.uint32)), tag
+    assert [tuple(x) for x in pp] == [tuple(x) for x in epp], tag
+
+
+def check(ctx, raw, passes, tag):
+    og = oracle.OGraph.from_pairs(raw)
+    g = ga.DeviceGraph.from_pairs(ctx, raw)
+    for p in passes:
+        exp = og.greedy_pp(p)
+        same(g.greedy_pp(p), exp, (tag, p))
+    g.close()
+
+
+def test_greedy_fixtures(ctx):
+    print(f"greedy_pp takes up to {ctx.greedy_max_vertices()} vertices", flush=True)
+    cases = 0
+    with open(GOLDEN) as f:
+        for rec in json.load(f)["graphs"]:
+            raw = np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2)
+            g = ga.DeviceGraph.from_pairs(ctx, raw)
+            for passes, e in rec["greedy_pp"].items():
+                same(g.greedy_pp(int(passes)), (e["tri"], e["verts"], e["set"], e["per_pass"]), (rec["name"], passes))
+                cases += 1
+            g.close()
+    assert cases >= 90
+
+
+def test_greedy_closed_forms(ctx):
+    for k in (3, 4, 9, 40):
+        g = ga.DeviceGraph.from_pairs(ctx, np.array(clique(k), np.uint32))
+        t, v, s, pp, _ = g.greedy_pp(3)
+        assert (t, v, s.tolist(), pp) == (math.comb(k, 3), k, list(range(k)), [(math.comb(k, 3), k)] * 3)
+        g.close()
+    g = ga.DeviceGraph.from_pairs(ctx, np.array(clique(4) + clique(5, 4), np.uint32))
+    assert g.greedy_pp(2)[:2] == (10, 5) and g.greedy_pp(2)[2].tolist() == [4, 5, 6, 7, 8]
+    g.close()
+    star = np.array([(0, i) for i in range(1, 70)], np.uint32)
+    g = ga.DeviceGraph.from_pairs(ctx, star)
+    t, v, s, pp, _ = g.greedy_pp(4)
+    assert (t, v, len(s), pp) == (0, 0, 0, [(0, 0)] * 4)
+    # a graph with no vertices (an empty induced subgraph)
+    e, _ = g.induced(np.zeros(g.n, np.uint8))
+    t, v, s, pp, _ = e.greedy_pp(2)
+    assert (t, v, len(s), pp) == (0, 0, 0, [(0, 0)] * 2)
+    e.close()
+    g.close()
+
+
+@pytest.mark.parametrize("n", [31, 32, 33, 1023, 1024, 1025, 2049])
+def test_greedy_random_word_boundaries(ctx, n):
+    """n around multiples of 32 (bitmap words) and 1,024 (lanes per workgroup); the last id is in an edge so n holds."""
+    rng = np.random.default_rng(900 + n)
+    p = min(0.5, 12.0 / n) if n > 64 else 0.4
+    raw = np.concatenate([random_graph(rng, n, p, dup=0.2, loops=1), np.array([[0, n - 1], [1, n - 1], [0, 1]], np.uint32)])
+    check(ctx, raw, (1, 2, 7), f"n={n}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_greedy_random(ctx, seed):
+    rng = np.random.default_rng(4200 + seed)
+    n = int(rng.integers(50, 3000))
+    raw = random_graph(rng, n, float(rng.uniform(2.0, 40.0) / n), dup=0.3, loops=3)
+    check(ctx, raw, (1, 3, 10), seed)
+
+
+def test_greedy_powerlaw(ctx):
+    check(ctx, gen.powerlaw_small(), (1, 10), "powerlaw_small")
+
+
+def test_greedy_on_pruned_graphs(ctx):
+    """The pipeline of gpu_dsg.cu:318-356: core numbers, prune at ceil(kmax-core density), Greedy++ on what is left."""
+    rng = np.random.default_rng(77)
+    for trial in range(3):
+        raw = np.concatenate([gen.powerlaw_small(), random_graph(rng, 5000, 0.002)])
+        if trial:
+            base = 9000 + 40 * trial
+            raw = np.concatenate([raw, np.array(clique(30 + 10 * trial, base), np.uint32)])
+        g = ga.DeviceGraph.from_pairs(ctx, raw)
+        c = ga.Core(g, True)
+        core, _, kmax = c.run()
+        T_k, _ = g.vertex_participation((core >= kmax).astype(np.uint8))
+        verts = int((core >= kmax).sum())
+        k = (T_k + verts - 1) // verts
+        sub, _, _ = c.prune(k)
+        og = oracle.OGraph.from_pairs(raw)
+        osub, _, _ = og.prune_to_core(core, k)
+        for p in (1, 10):
+            same(sub.greedy_pp(p), osub.greedy_pp(p), (trial, p))
+        sub.close()
+        c.close()
+        g.close()
+
+
+def test_greedy_size_limit_and_arguments(ctx):
+    cap = ctx.greedy_max_vertices()
+    assert cap >= 4096
+    rng = np.random.default_rng(5)
+    raw = np.concatenate([random_graph(rng, cap, 6.0 / cap), np.array([[0, cap - 1], [1, cap - 1], [0, 1]], np.uint32)])
+    check(ctx, raw, (2,), "n = limit")
+    big = np.concatenate([raw, np.array([[2, cap]], np.uint32)])
+    g = ga.DeviceGraph.from_pairs(ctx, big)
+    assert g.n == cap + 1
+    with pytest.raises(ga.GAError) as ei:
+        g.greedy_pp(1)
+    assert ei.value.code == ga.GA_ERR_ARG
+    g.close()
+    g = ga.DeviceGraph.from_pairs(ctx, np.array(clique(5), np.uint32))
+    with pytest.raises(ga.GAError) as ei:
+        g.greedy_pp(0)
+    assert ei.value.code == ga.GA_ERR_ARG
+    g.close()

@@ -343,3 +343,81 @@ def test_planted_mistake_is_caught():
     gf = oracle.OGraph.from_csr(fr, flipped_dst[order])
     assert gf.triangles() == g.triangles()                        # T is invariant ...
     assert not np.array_equal(gf.arrays()[0], r)                  # ... but the CSR is not the reference's
+
+
+def test_golden_greedy(golden):
+    """greedy_pp of the oracle equals the reference's with 1, 3 and 10 passes on every fixture graph: best density, its
+    vertex set and the best after every pass."""
+    cases = 0
+    for rec in golden["graphs"]:
+        g = oracle.OGraph.from_pairs(np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2))
+        for passes, exp in rec["greedy_pp"].items():
+            t, v, s, pp = g.greedy_pp(int(passes))
+            assert (t, v, s.tolist()) == (exp["tri"], exp["verts"], exp["set"]), (rec["name"], passes)
+            assert [list(x) for x in pp] == exp["per_pass"], (rec["name"], passes)
+            cases += 1
+    assert cases >= 90
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_greedy_equal_reference_build(seed):
+    """Fresh seeded graphs, 1, 2 and 7 passes: the oracle's greedy_pp equals the reference build's."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    rng = np.random.default_rng(3100 + seed)
+    for _ in range(8):
+        n = int(rng.integers(5, 400))
+        raw = random_graph(rng, n, float(rng.uniform(0.02, 0.4)), dup=0.2, loops=1)
+        o = oracle.OGraph.from_pairs(raw)
+        r = oracle.RGraph.build(o.n, oracle.canonical_unique(raw))
+        for passes in (1, 2, 7):
+            a, b = o.greedy_pp(passes), r.greedy_pp(passes)
+            assert a[:2] == b[:2] and np.array_equal(a[2], b[2]) and a[3] == b[3], (seed, n, passes)
+
+
+def _set_triangles(uniq, s):
+    keep = np.zeros(int(uniq.max()) + 1 if len(uniq) else 1, bool)
+    keep[s[s < len(keep)]] = True
+    sub = uniq[keep[uniq[:, 0]] & keep[uniq[:, 1]]] if len(uniq) else uniq
+    return oracle.brute_triangles(len(keep), sub)
+
+
+def test_greedy_against_exhaustive_optimum():
+    """n <= 14, exhaustive optimum by brute force: Greedy++ is never denser than it; one pass (Charikar's peeling) is at
+    least a third of it, the approximation bound of peeling for triangle density; the best never falls from pass to
+    pass and the last pass's best is the result; the returned set holds exactly `tri` triangles on `verts` vertices."""
+    rng = np.random.default_rng(515)
+    for _ in range(60):
+        n = int(rng.integers(3, 15))
+        uniq = oracle.canonical_unique(random_graph(rng, n, float(rng.uniform(0.2, 0.95))))
+        if not len(uniq):
+            continue
+        o = oracle.OGraph.from_pairs(uniq)
+        on, od = oracle.brute_densest(o.n, uniq)
+        t1, v1, s1, _ = o.greedy_pp(1)
+        t, v, s, pp = o.greedy_pp(30)
+        assert t * max(od, 1) <= on * max(v, 1)
+        assert 3 * t1 * max(od, 1) >= on * max(v1, 1)
+        dens = [a / b if b else 0.0 for a, b in pp]
+        assert all(x <= y for x, y in zip(dens, dens[1:])) and pp[-1] == (t, v)
+        for tt, vv, ss in ((t1, v1, s1), (t, v, s)):
+            assert len(ss) == vv and _set_triangles(uniq, ss) == tt
+            assert np.all(np.diff(ss.astype(np.int64)) > 0)
+
+
+def test_greedy_closed_forms():
+    """K_k: the whole clique, C(k,3)/k; disjoint K5 and K4: the K5 (10/5 beats 4/4 and the union's 14/9); graphs
+    without a triangle and the empty graph: 0/0 and no set."""
+    for k in (3, 4, 7, 12):
+        o = oracle.OGraph.from_pairs(np.array(clique(k), np.uint32))
+        for passes in (1, 5):
+            t, v, s, pp = o.greedy_pp(passes)
+            assert (t, v, s.tolist()) == (math.comb(k, 3), k, list(range(k)))
+            assert pp == [(math.comb(k, 3), k)] * passes
+    o = oracle.OGraph.from_pairs(np.array(clique(4) + clique(5, 4), np.uint32))
+    t, v, s, _ = o.greedy_pp(3)
+    assert (t, v, s.tolist()) == (10, 5, [4, 5, 6, 7, 8])
+    for pairs in ([(0, 1), (1, 2), (2, 3), (3, 0)], [(0, i) for i in range(1, 9)], []):
+        o = oracle.OGraph.from_pairs(np.array(pairs, np.uint32).reshape(-1, 2))
+        t, v, s, pp = o.greedy_pp(4)
+        assert (t, v, len(s)) == (0, 0, 0) and pp == [(0, 0)] * 4

@@ -161,6 +161,139 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
     }
 }
 
+// Dense form for graphs whose adjacency bitmap fits LDS (n <= about 1,100 with 160 KB): adj[x] is a row of W = n / 32
+// words, padded to an odd stride so that lanes reading consecutive rows hit different banks. Lane t owns the vertices
+// t and t + 1024 and keeps their load, tri_H and alive flag in registers. One deletion:
+//   argmin over the owned vertices, a wave reduction and a 16-entry reduction through LDS;
+//   each owner of an alive neighbour x of v counts c = |N(x) & N(v) & H| with W popcounts and drops tri_H(x) by c:
+//   every triangle {v, x, b} of H is counted once at x and once at b, so the counts sum to 2 tri_H(v);
+//   the owner of v retires it; lane 0 writes the order and clears v's bit of the shared alive row.
+// Two barriers per deletion; nothing but the order leaves the CU.
+__global__ __launch_bounds__(GR_BLOCK) void k_greedy_dense(uint32_t n, uint32_t m, uint32_t wpad, const uint32_t* __restrict__ src,
+                                                           const uint32_t* __restrict__ col, const uint32_t* __restrict__ tri0,
+                                                           const unsigned long long* __restrict__ dT, uint32_t passes,
+                                                           uint32_t* __restrict__ orders, unsigned long long* __restrict__ per_pass,
+                                                           GreedyOut* __restrict__ out) {
+    extern __shared__ __align__(16) uint32_t adj[];
+    uint32_t* alive_w = adj + (size_t)n * wpad;
+    __shared__ unsigned long long red_key[GR_WAVES];
+    __shared__ uint32_t red_id[GR_WAVES], red_found[GR_WAVES];
+    __shared__ uint32_t s_cv, s_bad;
+
+    const uint32_t tid = threadIdx.x, lane = tid & (GA_WAVE - 1), wave = tid / GA_WAVE;
+    const uint32_t W = (n + 31) >> 5;
+    for (uint32_t i = tid; i < n * wpad; i += GR_BLOCK) adj[i] = 0;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (uint32_t e = tid; e < m; e += GR_BLOCK) {
+        const uint32_t a = src[e], b = col[e];
+        atomicOr(&adj[a * wpad + (b >> 5)], 1u << (b & 31));
+        atomicOr(&adj[b * wpad + (a >> 5)], 1u << (a & 31));
+    }
+    const uint64_t T0 = *dT;
+    unsigned long long load[2] = {0, 0};
+    uint32_t cur[2] = {0, 0};
+    bool alive[2] = {false, false};
+    uint64_t bn = 0, bd = 0;
+    uint32_t best_pass = UINT32_MAX, best_cut = 0, err = 0;
+    for (uint32_t p = 0; p < passes && !err; p++) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const uint32_t x = tid + k * GR_BLOCK;
+            alive[k] = x < n;
+            cur[k] = x < n ? tri0[x] : 0;
+        }
+        for (uint32_t i = tid; i < W; i += GR_BLOCK) alive_w[i] = (i + 1) * 32 <= n ? ~0u : ((1u << (n & 31)) - 1u);
+        __syncthreads();
+        uint32_t* order = orders + (size_t)p * n;
+        uint64_t T = T0;
+        bool improved = false;
+        uint32_t cut = 0;
+        for (uint32_t step = 0; step < n; step++) {
+            const uint32_t left = n - step;
+            if (dens_gt(T, left, bn, bd)) {
+                bn = T;
+                bd = left;
+                improved = true;
+                cut = step;
+            }
+            unsigned long long bk = ~0ull;
+            uint32_t bv = UINT32_MAX;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const unsigned long long key = load[k] + cur[k];
+                if (alive[k] && key < bk) { bk = key; bv = tid + k * GR_BLOCK; }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long ok = __shfl_xor(bk, o, GA_WAVE);
+                const uint32_t ov = __shfl_xor(bv, o, GA_WAVE);
+                if (ok < bk || (ok == bk && ov < bv)) { bk = ok; bv = ov; }
+            }
+            if (lane == 0) { red_key[wave] = bk; red_id[wave] = bv; }
+            __syncthreads();
+            bk = red_key[0];
+            bv = red_id[0];
+            for (int w = 1; w < GR_WAVES; w++) {
+                const unsigned long long ok = red_key[w];
+                const uint32_t ov = red_id[w];
+                if (ok < bk || (ok == bk && ov < bv)) { bk = ok; bv = ov; }
+            }
+            const uint32_t v = bv;   // left > 0: some lane owns an alive vertex, so v < n
+            const uint32_t* av = adj + v * wpad;
+            uint32_t found = 0;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const uint32_t x = tid + k * GR_BLOCK;
+                if (x == v) s_cv = cur[k];
+                if (alive[k] && ((av[x >> 5] >> (x & 31)) & 1u)) {
+                    const uint32_t* ax = adj + x * wpad;
+                    uint32_t c = 0;
+                    for (uint32_t i = 0; i < W; i++) c += __popc(ax[i] & av[i] & alive_w[i]);
+                    if (c > cur[k]) s_bad = 1;
+                    cur[k] -= min(c, cur[k]);
+                    found += c;
+                }
+            }
+            found = (uint32_t)wave_sum_u64(found);
+            if (lane == 0) red_found[wave] = found;
+            __syncthreads();
+            uint64_t total = 0;
+            for (int w = 0; w < GR_WAVES; w++) total += red_found[w];
+            const uint32_t cv = s_cv;
+            if (s_bad || total != 2ull * cv) err = 1;   // read after the barrier: the same in every lane
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                if (tid + k * GR_BLOCK == v) {
+                    alive[k] = false;
+                    load[k] += cv;
+                }
+            if (tid == 0) {
+                order[step] = v;
+                alive_w[v >> 5] &= ~(1u << (v & 31));
+            }
+            T -= cv;
+            if (err) break;
+        }
+        if (improved) {
+            best_pass = p;
+            best_cut = cut;
+        }
+        if (tid == 0) {
+            per_pass[2 * p] = bn;
+            per_pass[2 * p + 1] = bd;
+        }
+        __syncthreads();   // the alive row is rewritten by the next pass
+    }
+    if (tid == 0) {
+        out->bn = bn;
+        out->bd = bd;
+        out->best_pass = best_pass;
+        out->best_cut = best_cut;
+        out->error = err;
+    }
+}
+
 __global__ void k_greedy_flags(const uint32_t* __restrict__ order, uint32_t n, const GreedyOut* __restrict__ out,
                                uint32_t* __restrict__ flags) {
     const uint32_t cut = out->best_pass == UINT32_MAX ? n : out->best_cut;
@@ -181,11 +314,23 @@ struct DevBuf {
 
 size_t greedy_lds(uint32_t n) { return (size_t)n * 12 + (size_t)((n + 31) / 32) * 8; }
 
+uint32_t dense_stride(uint32_t n) { return ((n + 31) / 32) | 1u; }
+size_t dense_lds(uint32_t n) { return ((size_t)n + 1) * dense_stride(n) * 4; }
+
+size_t lds_room(const ga_ctx* ctx) { return ctx->lds_max > 1024 ? ctx->lds_max - 1024 : 0; }   // static LDS stays below 1 KB
+
+// The largest n the dense form takes: its bitmap fits LDS and each lane owns at most two vertices.
+uint32_t dense_max(const ga_ctx* ctx) {
+    uint32_t n = 2 * GR_BLOCK;
+    while (n && dense_lds(n) > lds_room(ctx)) n--;
+    return n;
+}
+
 }  // namespace
 
 extern "C" uint32_t ga_greedy_max_vertices(ga_ctx* ctx) {
     if (!ctx) return 0;
-    const size_t room = ctx->lds_max > 1024 ? ctx->lds_max - 1024 : 0;   // static LDS of the kernel stays below 1 KB
+    const size_t room = lds_room(ctx);
     uint32_t n = (uint32_t)(room / 12);
     while (n && greedy_lds(n) > room) n--;
     return n;
@@ -222,7 +367,12 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     GA_TRY(ga_fill_u32(ctx, (uint32_t*)dflags.p, (size_t)n + 1, 0));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)dout.p, sizeof(GreedyOut) / 8, 0));
     if (g->m) GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, (unsigned long long*)dT.p, nullptr, (uint32_t*)dtri.p, nullptr, nullptr));
-    if (n) {
+    if (n && n <= dense_max(ctx)) {
+        k_greedy_dense<<<1, GR_BLOCK, dense_lds(n), ctx->stream>>>(n, g->m, dense_stride(n), g->src, g->col, (const uint32_t*)dtri.p,
+                                                                 (const unsigned long long*)dT.p, passes, (uint32_t*)dorders.p,
+                                                                 (unsigned long long*)dpass.p, (GreedyOut*)dout.p);
+        GA_TRY(ga_launch_check(ctx, "k_greedy_dense"));
+    } else if (n) {
         k_greedy_pp<<<1, GR_BLOCK, greedy_lds(n), ctx->stream>>>(n, g->rowptr, g->col, g->urow, g->ucol, (const uint32_t*)dtri.p,
                                                                (const unsigned long long*)dT.p, passes, (uint32_t*)dorders.p, (unsigned long long*)dpass.p,
                                                                (GreedyOut*)dout.p);

@@ -71,9 +71,11 @@ def test_greedy_closed_forms(ctx):
     g.close()
 
 
-@pytest.mark.parametrize("n", [31, 32, 33, 1023, 1024, 1025, 2049])
+@pytest.mark.parametrize("n", [31, 32, 33, 1023, 1024, 1025, 1120, 1121, 2049])
 def test_greedy_random_word_boundaries(ctx, n):
-    """n around multiples of 32 (bitmap words) and 1,024 (lanes per workgroup); the last id is in an edge so n holds."""
+    """n around multiples of 32 (bitmap words) and 1,024 (lanes per workgroup), and at the largest graph whose adjacency
+    bitmap fits 160 KB of LDS (1,120 vertices: the dense form) and one more (the sparse form); the last id is in an edge so
+    n holds."""
     rng = np.random.default_rng(900 + n)
     p = min(0.5, 12.0 / n) if n > 64 else 0.4
     raw = np.concatenate([random_graph(rng, n, p, dup=0.2, loops=1), np.array([[0, n - 1], [1, n - 1], [0, 1]], np.uint32)])
@@ -86,6 +88,13 @@ def test_greedy_random(ctx, seed):
     n = int(rng.integers(50, 3000))
     raw = random_graph(rng, n, float(rng.uniform(2.0, 40.0) / n), dup=0.3, loops=3)
     check(ctx, raw, (1, 3, 10), seed)
+
+
+@pytest.mark.parametrize("n,p", [(700, 0.4), (1100, 0.15)])
+def test_greedy_dense_random(ctx, n, p):
+    """Dense graphs like the pruned ones (hundreds of neighbours per vertex, millions of triangles)."""
+    rng = np.random.default_rng(n)
+    check(ctx, random_graph(rng, n, p), (1, 5), (n, p))
 
 
 def test_greedy_powerlaw(ctx):

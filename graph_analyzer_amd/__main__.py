@@ -1,7 +1,8 @@
 """Runs one stage of the path on a named synthetic workload and prints its device time and result.
 
-    python -m graph_analyzer_amd count|peel|core|support|truss|ingest|prune [--workload rmat21|powerlaw_communities|powerlaw_small]
-                                 [--repeat R] [--num 1 --den 10]
+    python -m graph_analyzer_amd count|peel|core|support|truss|ingest|prune|greedy
+                                 [--workload rmat21|powerlaw_communities|powerlaw_small] [--repeat R] [--num 1 --den 10]
+                                 [--passes 10]
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
     python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
 
@@ -24,12 +25,13 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
-    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "batch", "probe"])
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "greedy", "batch", "probe"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--num", type=int, default=1)
     ap.add_argument("--den", type=int, default=10)
     ap.add_argument("--graphs", type=int, default=4096)
+    ap.add_argument("--passes", type=int, default=10)
     args = ap.parse_args(argv)
     ctx = ga.Context(0)
     if args.stage == "probe":
@@ -82,8 +84,20 @@ def main(argv=None):
           f"(generated and built in {time.perf_counter() - t0:.1f} s)", flush=True)
     T, ms = g.count(ga.TC_HASH)
     print(f"count: T={T} {ms:.3f} ms (first, includes nothing but the count)", flush=True)
+    sub = None
+    if args.stage == "greedy":   # Greedy++ on the graph prune_to_core leaves at ceil(kmax-core density)
+        c = ga.Core(g, True)
+        core, _, kmax = c.run(T_full=T)
+        Tk, _ = g.vertex_participation((core >= kmax).astype(np.uint8))
+        verts = int((core >= kmax).sum())
+        sub, _, _ = c.prune((Tk + verts - 1) // verts if verts else 0)
+        c.close()
+        print(f"greedy on the pruned graph: n={sub.n} m={sub.m}", flush=True)
     for _ in range(args.repeat):
-        if args.stage == "count":
+        if args.stage == "greedy":
+            t, v, s, _, ms = sub.greedy_pp(args.passes)
+            print(f"greedy++ {args.passes} passes: {t}/{v} ({len(s)} vertices) {ms:.3f} ms device", flush=True)
+        elif args.stage == "count":
             T, ms = g.count(ga.TC_HASH)
             print(f"count: T={T} {ms:.3f} ms", flush=True)
         elif args.stage == "support":
@@ -117,6 +131,8 @@ def main(argv=None):
             core, rounds, kmax = c.run(T_full=T)
             print(f"core: rounds {rounds} kmax {kmax} {c.last_ms:.3f} ms device", flush=True)
             c.close()
+    if sub is not None:
+        sub.close()
     g.close()
     return 0
 

@@ -24,7 +24,7 @@ struct GreedyOut {
     unsigned long long bn, bd;   // best density found: triangles / vertices (0/0 = none)
     uint32_t best_pass;          // pass whose deletion order holds the best subgraph (UINT32_MAX = none)
     uint32_t best_cut;           // the best subgraph = that pass's order[best_cut .. n)
-    uint32_t error;              // 1 = triangle bookkeeping inconsistent
+    uint32_t error;              // 1 = triangle bookkeeping inconsistent, 2 = no alive vertex found
     uint32_t pad;
 };
 
@@ -34,6 +34,43 @@ __device__ __forceinline__ bool dens_gt(uint64_t an, uint64_t ad, uint64_t bn, u
     const uint64_t lh = __umul64hi(an, bd), ll = an * bd;
     const uint64_t rh = __umul64hi(bn, ad), rl = bn * ad;
     return lh > rh || (lh == rh && ll > rl);
+}
+
+// Cross-lane moves inside rows of 16 lanes with DPP (VALU latency, no LDS round trip): quad_perm [1,0,3,2] and
+// [2,3,0,1] swap neighbours at distance 1 and 2, row_half_mirror and row_mirror reflect 8- and 16-lane halves. After the
+// four steps every lane of a row holds the row's result; readlane combines the four rows.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp64(unsigned long long x) {
+    return ((unsigned long long)dpp32<CTRL>((uint32_t)(x >> 32)) << 32) | dpp32<CTRL>((uint32_t)x);
+}
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long x, int l) {
+    return ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+}
+
+// Minimum over the wavefront, the same in every lane.
+__device__ __forceinline__ unsigned long long wave_min_u64_dpp(unsigned long long x) {
+    x = min(x, dpp64<0xB1>(x));
+    x = min(x, dpp64<0x4E>(x));
+    x = min(x, dpp64<0x141>(x));
+    x = min(x, dpp64<0x140>(x));
+    return min(min(readlane64(x, 0), readlane64(x, 16)), min(readlane64(x, 32), readlane64(x, 48)));
+}
+
+// Sum over the wavefront, the same in every lane.
+__device__ __forceinline__ uint32_t wave_sum_u32_dpp(uint32_t x) {
+    x += dpp32<0xB1>(x);
+    x += dpp32<0x4E>(x);
+    x += dpp32<0x141>(x);
+    x += dpp32<0x140>(x);
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 0) + (uint32_t)__builtin_amdgcn_readlane((int)x, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)x, 32) + (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
 }
 
 __device__ __forceinline__ bool bit(const uint32_t* b, uint32_t x) { return (b[x >> 5] >> (x & 31)) & 1u; }
@@ -105,6 +142,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
                 if (ok < bk || (ok == bk && ov < bv)) { bk = ok; bv = ov; }
             }
             const uint32_t v = bv;              // left > 0, so an alive vertex exists and v < n
+            if (v >= n) { err = 2; break; }     // uniform: every lane reduced the same values
             const uint32_t cv = cur[v];
             const uint32_t ub = urow[v], ue = urow[v + 1];
             for (uint32_t j = ub + tid; j < ue; j += GR_BLOCK) {
@@ -161,55 +199,94 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
     }
 }
 
-// Dense form for graphs whose adjacency bitmap fits LDS (n <= about 1,100 with 160 KB): adj[x] is a row of W = n / 32
-// words, padded to an odd stride so that lanes reading consecutive rows hit different banks. Lane t owns the vertices
-// t and t + 1024 and keeps their load, tri_H and alive flag in registers. One deletion:
-//   argmin over the owned vertices, a wave reduction and a 16-entry reduction through LDS;
-//   each owner of an alive neighbour x of v counts c = |N(x) & N(v) & H| with W popcounts and drops tri_H(x) by c:
-//   every triangle {v, x, b} of H is counted once at x and once at b, so the counts sum to 2 tri_H(v);
-//   the owner of v retires it; lane 0 writes the order and clears v's bit of the shared alive row.
-// Two barriers per deletion; nothing but the order leaves the CU.
-__global__ __launch_bounds__(GR_BLOCK) void k_greedy_dense(uint32_t n, uint32_t m, uint32_t wpad, const uint32_t* __restrict__ src,
+// Dense form for graphs whose adjacency bitmap fits LDS (n <= 1,120 with 160 KB): adj[x] is a row of W = n / 32 words,
+// padded to an odd stride so that lanes reading consecutive rows hit different banks. A workgroup of DB lanes runs it;
+// lane t owns the vertices t + k * DB (k < OWN) and keeps their load, tri_H and alive flag in registers. 1,024 lanes
+// owning two vertices each beat 256 lanes owning five (14.4 vs 24.2 ms for 10 passes on a 729-vertex graph): the
+// neighbours' popcount rows are the serial part, and more lanes split them finer. A vertex's key is (load + tri_H) << 11 | id, so one
+// 64-bit minimum is the (key, id) order; a key is at most passes * C(n-1, 2) + C(n-1, 2) < 2^32 * 2^20, so it fits the
+// 53 bits above the id.
+// One deletion, one barrier:
+//   v = the minimum published for this deletion (one LDS word, the same for every lane);
+//   each owner of an alive neighbour x of v counts c = |N(x) & N(v) & H| with W popcounts and drops tri_H(x) by c
+//   (every triangle {v, x, b} of H is counted at x and at b, so the counts sum to 2 tri_H(v)); the owner of v retires
+//   it and publishes tri_H(v);
+//   every wave takes the minimum of its lanes' new keys (DPP) and merges it into the next deletion's slot with one LDS
+//   atomicMin; wave sums of the counts go into this deletion's counter;
+//   after the barrier, the next deletion first checks this one (counts = 2 tri_H(v), no count above tri_H).
+// The deletion order stays in registers (lane t keeps the deletions t + k * DB) and is written once per pass:
+// a global store inside the loop would make every barrier wait for it.
+// The minimum slots and counters rotate over three and the published tri_H over two, so each is reset one deletion
+// before it is written and never written while a slower lane may still read it.
+template <int DB, int OWN>
+__global__ __launch_bounds__(DB) void k_greedy_dense(uint32_t n, uint32_t m, uint32_t wpad, const uint32_t* __restrict__ src,
                                                            const uint32_t* __restrict__ col, const uint32_t* __restrict__ tri0,
                                                            const unsigned long long* __restrict__ dT, uint32_t passes,
                                                            uint32_t* __restrict__ orders, unsigned long long* __restrict__ per_pass,
                                                            GreedyOut* __restrict__ out) {
-    extern __shared__ __align__(16) uint32_t adj[];
-    uint32_t* alive_w = adj + (size_t)n * wpad;
-    __shared__ unsigned long long red_key[GR_WAVES];
-    __shared__ uint32_t red_id[GR_WAVES], red_found[GR_WAVES];
-    __shared__ uint32_t s_cv, s_bad;
+    extern __shared__ __align__(16) uint32_t adj[];                  // n rows of wpad words
+    uint32_t* alive_w = adj + (size_t)n * wpad;                      // the alive row
+    __shared__ unsigned long long s_min[3];
+    __shared__ uint32_t s_cv[2], s_found[3], s_bad;
+    constexpr uint32_t IDB = 11;
+    constexpr unsigned long long IDM = (1ull << IDB) - 1;
 
-    const uint32_t tid = threadIdx.x, lane = tid & (GA_WAVE - 1), wave = tid / GA_WAVE;
+    const uint32_t tid = threadIdx.x, lane = tid & (GA_WAVE - 1);
     const uint32_t W = (n + 31) >> 5;
-    for (uint32_t i = tid; i < n * wpad; i += GR_BLOCK) adj[i] = 0;
+    for (uint32_t i = tid; i < n * wpad; i += DB) adj[i] = 0;
     if (tid == 0) s_bad = 0;
     __syncthreads();
-    for (uint32_t e = tid; e < m; e += GR_BLOCK) {
+    for (uint32_t e = tid; e < m; e += DB) {
         const uint32_t a = src[e], b = col[e];
         atomicOr(&adj[a * wpad + (b >> 5)], 1u << (b & 31));
         atomicOr(&adj[b * wpad + (a >> 5)], 1u << (a & 31));
     }
     const uint64_t T0 = *dT;
-    unsigned long long load[2] = {0, 0};
-    uint32_t cur[2] = {0, 0};
-    bool alive[2] = {false, false};
+    unsigned long long load[OWN];
+    uint32_t cur[OWN], ord[OWN];
+    bool alive[OWN];
+#pragma unroll
+    for (int k = 0; k < OWN; k++) {
+        load[k] = 0;
+        cur[k] = ord[k] = 0;
+        alive[k] = false;
+    }
     uint64_t bn = 0, bd = 0;
     uint32_t best_pass = UINT32_MAX, best_cut = 0, err = 0;
     for (uint32_t p = 0; p < passes && !err; p++) {
+        for (uint32_t i = tid; i < W; i += DB) alive_w[i] = (i + 1) * 32 <= n ? ~0u : ((1u << (n & 31)) - 1u);
+        if (tid < 3) {
+            s_found[tid] = 0;
+            s_min[tid] = ~0ull;
+        }
+        __syncthreads();
+        unsigned long long mine = ~0ull;
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const uint32_t x = tid + k * GR_BLOCK;
+        for (int k = 0; k < OWN; k++) {
+            const uint32_t x = tid + k * DB;
             alive[k] = x < n;
             cur[k] = x < n ? tri0[x] : 0;
+            if (x < n) mine = min(mine, ((load[k] + cur[k]) << IDB) | x);
         }
-        for (uint32_t i = tid; i < W; i += GR_BLOCK) alive_w[i] = (i + 1) * 32 <= n ? ~0u : ((1u << (n & 31)) - 1u);
+        mine = wave_min_u64_dpp(mine);
+        if (lane == 0) atomicMin(&s_min[0], mine);
         __syncthreads();
         uint32_t* order = orders + (size_t)p * n;
         uint64_t T = T0;
         bool improved = false;
         uint32_t cut = 0;
+        uint32_t done = 0;   // deletions of this pass
         for (uint32_t step = 0; step < n; step++) {
+            if (step) {   // the previous deletion, read after the barrier: the same in every lane
+                const uint32_t cv = s_cv[(step - 1) & 1];
+                if (s_bad || s_found[(step - 1) % 3] != 2ull * cv) err = 1;
+                T -= cv;
+                if (err) break;
+            }
+            if (tid == 0) {
+                s_found[(step + 1) % 3] = 0;
+                s_min[(step + 2) % 3] = ~0ull;
+            }
             const uint32_t left = n - step;
             if (dens_gt(T, left, bn, bd)) {
                 bn = T;
@@ -217,63 +294,54 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_dense(uint32_t n, uint32_t 
                 improved = true;
                 cut = step;
             }
-            unsigned long long bk = ~0ull;
-            uint32_t bv = UINT32_MAX;
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const unsigned long long key = load[k] + cur[k];
-                if (alive[k] && key < bk) { bk = key; bv = tid + k * GR_BLOCK; }
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const unsigned long long ok = __shfl_xor(bk, o, GA_WAVE);
-                const uint32_t ov = __shfl_xor(bv, o, GA_WAVE);
-                if (ok < bk || (ok == bk && ov < bv)) { bk = ok; bv = ov; }
-            }
-            if (lane == 0) { red_key[wave] = bk; red_id[wave] = bv; }
-            __syncthreads();
-            bk = red_key[0];
-            bv = red_id[0];
-            for (int w = 1; w < GR_WAVES; w++) {
-                const unsigned long long ok = red_key[w];
-                const uint32_t ov = red_id[w];
-                if (ok < bk || (ok == bk && ov < bv)) { bk = ok; bv = ov; }
-            }
-            const uint32_t v = bv;   // left > 0: some lane owns an alive vertex, so v < n
+            const uint32_t v = (uint32_t)(s_min[step % 3] & IDM);   // left > 0: an alive key was published, so v < n
+            if (v >= n) { err = 2; break; }                           // uniform: every lane read the same slot
             const uint32_t* av = adj + v * wpad;
             uint32_t found = 0;
+            mine = ~0ull;
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const uint32_t x = tid + k * GR_BLOCK;
-                if (x == v) s_cv = cur[k];
-                if (alive[k] && ((av[x >> 5] >> (x & 31)) & 1u)) {
+            for (int k = 0; k < OWN; k++) {
+                const uint32_t x = tid + k * DB;
+                if (!alive[k]) continue;
+                if (x == v) {
+                    s_cv[step & 1] = cur[k];
+                    alive[k] = false;
+                    load[k] += cur[k];
+                    alive_w[v >> 5] &= ~(1u << (v & 31));   // bit v is clear in av, so no count of this step reads it
+                    continue;
+                }
+                if ((av[x >> 5] >> (x & 31)) & 1u) {
                     const uint32_t* ax = adj + x * wpad;
                     uint32_t c = 0;
+#pragma unroll 8
                     for (uint32_t i = 0; i < W; i++) c += __popc(ax[i] & av[i] & alive_w[i]);
                     if (c > cur[k]) s_bad = 1;
                     cur[k] -= min(c, cur[k]);
                     found += c;
                 }
+                mine = min(mine, ((load[k] + cur[k]) << IDB) | x);
             }
-            found = (uint32_t)wave_sum_u64(found);
-            if (lane == 0) red_found[wave] = found;
-            __syncthreads();
-            uint64_t total = 0;
-            for (int w = 0; w < GR_WAVES; w++) total += red_found[w];
-            const uint32_t cv = s_cv;
-            if (s_bad || total != 2ull * cv) err = 1;   // read after the barrier: the same in every lane
+            mine = wave_min_u64_dpp(mine);
+            found = wave_sum_u32_dpp(found);
+            if (lane == 0) {
+                if (mine != ~0ull) atomicMin(&s_min[(step + 1) % 3], mine);
+                if (found) atomicAdd(&s_found[step % 3], found);
+            }
+            if (step % DB == tid) {
 #pragma unroll
-            for (int k = 0; k < 2; k++)
-                if (tid + k * GR_BLOCK == v) {
-                    alive[k] = false;
-                    load[k] += cv;
-                }
-            if (tid == 0) {
-                order[step] = v;
-                alive_w[v >> 5] &= ~(1u << (v & 31));
+                for (int k = 0; k < OWN; k++)
+                    if (step / DB == (uint32_t)k) ord[k] = v;
             }
+            done = step + 1;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int k = 0; k < OWN; k++)
+            if (tid + k * DB < done) order[tid + k * DB] = ord[k];
+        if (!err && n) {
+            const uint32_t cv = s_cv[(n - 1) & 1];
+            if (s_bad || s_found[(n - 1) % 3] != 2ull * cv) err = 1;
             T -= cv;
-            if (err) break;
         }
         if (improved) {
             best_pass = p;
@@ -283,7 +351,7 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_dense(uint32_t n, uint32_t 
             per_pass[2 * p] = bn;
             per_pass[2 * p + 1] = bd;
         }
-        __syncthreads();   // the alive row is rewritten by the next pass
+        __syncthreads();   // the alive row, the slots and the counters are reset by the next pass
     }
     if (tid == 0) {
         out->bn = bn;
@@ -298,7 +366,7 @@ __global__ void k_greedy_flags(const uint32_t* __restrict__ order, uint32_t n, c
                                uint32_t* __restrict__ flags) {
     const uint32_t cut = out->best_pass == UINT32_MAX ? n : out->best_cut;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        if (i >= cut) flags[order[i]] = 1;
+        if (i >= cut && order[i] < n) flags[order[i]] = 1;
 }
 
 __global__ void k_greedy_set(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, uint32_t n,
@@ -319,9 +387,11 @@ size_t dense_lds(uint32_t n) { return ((size_t)n + 1) * dense_stride(n) * 4; }
 
 size_t lds_room(const ga_ctx* ctx) { return ctx->lds_max > 1024 ? ctx->lds_max - 1024 : 0; }   // static LDS stays below 1 KB
 
-// The largest n the dense form takes: its bitmap fits LDS and each lane owns at most two vertices.
+constexpr int DENSE_BLOCK = 1024, DENSE_OWN = 2;
+
+// The largest n the dense form takes: its bitmap fits LDS and each lane owns at most DENSE_OWN vertices.
 uint32_t dense_max(const ga_ctx* ctx) {
-    uint32_t n = 2 * GR_BLOCK;
+    uint32_t n = DENSE_BLOCK * DENSE_OWN;
     while (n && dense_lds(n) > lds_room(ctx)) n--;
     return n;
 }
@@ -368,7 +438,7 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)dout.p, sizeof(GreedyOut) / 8, 0));
     if (g->m) GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, (unsigned long long*)dT.p, nullptr, (uint32_t*)dtri.p, nullptr, nullptr));
     if (n && n <= dense_max(ctx)) {
-        k_greedy_dense<<<1, GR_BLOCK, dense_lds(n), ctx->stream>>>(n, g->m, dense_stride(n), g->src, g->col, (const uint32_t*)dtri.p,
+        k_greedy_dense<DENSE_BLOCK, DENSE_OWN><<<1, DENSE_BLOCK, dense_lds(n), ctx->stream>>>(n, g->m, dense_stride(n), g->src, g->col, (const uint32_t*)dtri.p,
                                                                  (const unsigned long long*)dT.p, passes, (uint32_t*)dorders.p,
                                                                  (unsigned long long*)dpass.p, (GreedyOut*)dout.p);
         GA_TRY(ga_launch_check(ctx, "k_greedy_dense"));
@@ -382,7 +452,9 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     GA_HIP(ctx, hipMemcpyAsync(&h, dout.p, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (!n) h.best_pass = UINT32_MAX;
-    if (h.error) return ga_fail(ctx, GA_ERR_STATE, "greedy_pp: triangle bookkeeping inconsistent");
+    if (h.error)
+        return ga_fail(ctx, GA_ERR_STATE, "greedy_pp: %s",
+                       h.error == 1 ? "triangle bookkeeping inconsistent" : "no alive vertex found for a deletion");
     uint32_t count = 0;
     if (h.best_pass != UINT32_MAX && h.bd) {
         const uint32_t* order = (const uint32_t*)dorders.p + (size_t)h.best_pass * n;

@@ -685,6 +685,7 @@ def _bind_batch(L):
         "ga_batch_count_per_graph": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_set_tuning": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning)]),
+        "ga_batch_set_peel_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
         "ga_batch_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
         "ga_batch_peel": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
         "ga_batch_peel_per_graph": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _u64p, _u64p, _u32p,
@@ -776,6 +777,11 @@ class Batch:
     def set_tuning(self, t: TcTuning):
         """Count tuning of the union and of the per-graph baseline."""
         self.ctx.check(lib().ga_batch_set_tuning(self.ctx.handle, self._h, ctypes.byref(t)), "batch_set_tuning")
+
+    def set_peel_graph(self, rounds: int):
+        """Peel rounds captured into one HIP graph and replayed by peel_all (default 8; 0 enqueues every round)."""
+        if lib().ga_batch_set_peel_graph(self._h, int(rounds)) != GA_OK:
+            raise GAError(GA_ERR_ARG, f"peel graph rounds must be in [0, 64], got {rounds}")
 
     def peel_all(self, num=1, den=10, T_full=None, per_graph: bool = False, removed_round: bool = True):
         stats = np.zeros(4 * self.count, np.uint64)

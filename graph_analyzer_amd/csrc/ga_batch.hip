@@ -37,6 +37,7 @@ struct ga_batch {
     uint32_t* rr = nullptr;
     void* pstate = nullptr;
     BatchPeelDev pd{};
+    int graph_rounds = 8;             // peel rounds captured into one HIP graph (0: enqueue every round directly)
 };
 
 namespace {
@@ -224,6 +225,15 @@ __global__ void k_bpeel_end(BatchPeelDev d, const unsigned long long* __restrict
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { ga_free(p); }
+};
+
+struct GraphExec {   // scoped captured graph and its executable
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 double wall_ms() {
@@ -497,21 +507,39 @@ extern "C" int ga_batch_peel(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t de
     k_bpeel_init<<<grid_for(b->count, 256), 256, 0, ctx->stream>>>(d, (const unsigned long long*)dvoff.p, b->count);
     GA_TRY(ga_launch_check(ctx, "k_bpeel_init"));
     const unsigned gv = grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 8u);
-    int block_rounds = 4;
+    // One round: begin, the participation count (its bucket launches), mark, end. No launch takes a host value that
+    // changes between rounds, and a finished batch turns every launch into a no-op through d.all_done, so rounds can be
+    // enqueued ahead of the readback, and a block of them can be captured once and replayed.
+    auto round = [&]() -> int {
+        k_bpeel_begin<<<gv, 256, 0, ctx->stream>>>(d, b->tri, n, b->count);
+        GA_TRY(ga_tc_launch(ctx, b->u, GA_TC_HASH, (unsigned long long*)scratch.p, b->alive, b->tri, nullptr, d.all_done, b->gof, d.T));
+        k_bpeel_mark<<<gv, 256, 0, ctx->stream>>>(d, b->tri, b->alive, b->rr, b->gof, n, num, den);
+        k_bpeel_end<<<grid_for(b->count, 256), 256, 0, ctx->stream>>>(d, (const unsigned long long*)dvoff.p,
+                                                                      (const unsigned long long*)dtf.p, b->count, num, den);
+        return ga_launch_check(ctx, "batched peel round");
+    };
+    GraphExec ge;
+    if (b->graph_rounds > 0) {
+        GA_TRY(ga_tc_ensure_plan(ctx, b->u));   // built (and allocated) before the capture, never inside it
+        GA_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        int rc = GA_OK;
+        for (int r = 0; r < b->graph_rounds && !rc; r++) rc = round();
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &ge.graph);
+        if (rc) return rc;
+        if (ce != hipSuccess) return ga_fail(ctx, GA_ERR_HIP, "capture of the peel rounds failed: %s", hipGetErrorString(ce));
+        GA_HIP(ctx, hipGraphInstantiate(&ge.exec, ge.graph, nullptr, nullptr, 0));
+    }
+    int block = 1;   // rounds (direct) or graph launches (captured) per readback, doubling
     for (uint64_t guard = 0;; guard++) {
-        for (int r = 0; r < block_rounds; r++) {
-            k_bpeel_begin<<<gv, 256, 0, ctx->stream>>>(d, b->tri, n, b->count);
-            GA_TRY(ga_tc_launch(ctx, b->u, GA_TC_HASH, (unsigned long long*)scratch.p, b->alive, b->tri, nullptr, d.all_done, b->gof, d.T));
-            k_bpeel_mark<<<gv, 256, 0, ctx->stream>>>(d, b->tri, b->alive, b->rr, b->gof, n, num, den);
-            k_bpeel_end<<<grid_for(b->count, 256), 256, 0, ctx->stream>>>(d, (const unsigned long long*)dvoff.p,
-                                                                          (const unsigned long long*)dtf.p, b->count, num, den);
+        for (int r = 0; r < block; r++) {
+            if (ge.exec) GA_HIP(ctx, hipGraphLaunch(ge.exec, ctx->stream));
+            else GA_TRY(round());
         }
-        GA_TRY(ga_launch_check(ctx, "batched peel round"));
         GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, d.active, 8, hipMemcpyDeviceToHost, ctx->stream));
         GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (((const uint32_t*)ctx->hstat)[1]) break;
         if (guard > n + 16) return ga_fail(ctx, GA_ERR_STATE, "batched peel did not finish");
-        block_rounds = std::min(block_rounds * 2, 64);
+        block = std::min(block * 2, ge.exec ? 8 : 64);
     }
     std::vector<unsigned long long> bn(b->count), bd(b->count);
     std::vector<uint32_t> br(b->count), rounds(b->count), err(b->count);
@@ -530,6 +558,12 @@ extern "C" int ga_batch_peel(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t de
         stats[4 * g + 3] = rounds[g];
     }
     if (removed_round && n) GA_HIP(ctx, hipMemcpy(removed_round, b->rr, n * 4, hipMemcpyDeviceToHost));
+    return GA_OK;
+}
+
+extern "C" int ga_batch_set_peel_graph(ga_batch* b, int rounds) {
+    if (!b || rounds < 0 || rounds > 64) return GA_ERR_ARG;
+    b->graph_rounds = rounds;
     return GA_OK;
 }
 

@@ -257,6 +257,11 @@ int ga_batch_get_tuning(const ga_batch* b, ga_tc_tuning* out);
  * round-0 check (or NULL). */
 int ga_batch_peel(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t den, const uint64_t* T_full, uint64_t* stats,
                   uint32_t* removed_round, float* ms);
+/* Peel rounds per captured HIP graph in ga_batch_peel (default 8): the first call's block of rounds is captured once
+ * and replayed until every graph is done, with one readback per 1, 2, 4, 8, 8, ... replays. 0 enqueues every round
+ * directly (one readback per 1, 2, 4, ..., 64 rounds). Results are identical either way. No counterpart in the
+ * reference, which syncs once per round (densest.h:279-303). */
+int ga_batch_set_peel_graph(ga_batch* b, int rounds);
 /* The same with one ga_peel_run per graph (the per-graph-launch baseline). */
 int ga_batch_peel_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t den, const uint64_t* T_full, uint64_t* stats,
                             uint32_t* removed_round, float* ms);

@@ -72,6 +72,32 @@ def test_alone_equals_in_mix(ctx):
     b.close()
 
 
+def test_peel_rounds_captured_equal_direct(ctx):
+    """The batched peel with its rounds captured into a HIP graph and replayed (1, 3, 8 and 64 rounds per graph) gives
+    the same stats and removal rounds as every round enqueued directly, call after call, with eager counts between the
+    calls; the direct form itself equals the oracle."""
+    graphs = mix(32, 11)
+    b = ga.Batch(ctx, graphs)
+    T, _ = b.count_all()
+    b.set_peel_graph(0)
+    want = {eps: b.peel_all(*eps, T_full=T)[:2] for eps in ((1, 10), (1, 2), (0, 1))}
+    for i in (0, 3, 7, 20):
+        og = oracle.OGraph.from_pairs(graphs[i])
+        bn, bd, br, rounds, orr_ = og.batch_peel(1, 10)
+        assert tuple(int(x) for x in want[(1, 10)][0][i]) == (bn, bd, br, rounds), i
+    for rounds in (1, 3, 8, 64):
+        b.set_peel_graph(rounds)
+        for eps, (st0, rr0) in want.items():
+            for rep in range(2):
+                st, rr, _ = b.peel_all(*eps, T_full=T)
+                assert np.array_equal(st, st0) and np.array_equal(rr, rr0), (rounds, eps, rep)
+                T2, _ = b.count_all()
+                assert np.array_equal(T2, T)
+    with pytest.raises(ga.GAError):
+        b.set_peel_graph(65)
+    b.close()
+
+
 def test_batch_peel_errors(ctx):
     graphs = [np.array(clique(6), np.uint32), np.array(clique(5), np.uint32)]
     b = ga.Batch(ctx, graphs)

@@ -335,7 +335,7 @@ def main():
                     help="also run one plain device copy of MIB MiB (k_copy_probe; calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single MI355X)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library runs on a single MI355X; sharding is a design, not code)", file=sys.stderr)
         sys.exit(2)
 
     import graph_analyzer_amd as ga

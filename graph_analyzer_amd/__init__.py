@@ -108,6 +108,8 @@ def lib() -> ctypes.CDLL:
             "ga_truss_download": (ctypes.c_int, [_vp, _u32p]),
             "ga_greedy_pp": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u64p, _u64p, _u64p, _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_greedy_max_vertices": (ctypes.c_uint32, [_vp]),
+            "ga_greedy_pp_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_uint32, _u64p, _u8p,
+                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_core_prune": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
@@ -206,6 +208,22 @@ class Context:
         ms = ctypes.c_float()
         self.check(lib().ga_tc_count_many(self._h, hs, len(graphs), variant, _ptr(T, _u64p), ctypes.byref(ms)), "tc_count_many")
         return T[: len(graphs)], ms.value
+
+    def greedy_pp_many(self, graphs, passes: int = 1):
+        """Greedy++ on several resident graphs, one workgroup per graph in one launch where they fit the LDS-bitmap form.
+        Returns (stats, sets, ms): stats[i] = (tri, verts) of graph i's best density, sets[i] its vertices ascending."""
+        hs = (_vp * max(len(graphs), 1))(*[g.handle for g in graphs])
+        st = np.zeros(2 * max(len(graphs), 1), np.uint64)
+        total = sum(g.n for g in graphs)
+        flags = np.zeros(max(total, 1), np.uint8)
+        ms = ctypes.c_float()
+        self.check(lib().ga_greedy_pp_many(self._h, hs, len(graphs), int(passes), _ptr(st, _u64p), _ptr(flags, _u8p),
+                                           ctypes.byref(ms)), "greedy_pp_many")
+        sets, off = [], 0
+        for g in graphs:
+            sets.append(np.flatnonzero(flags[off: off + g.n]).astype(np.uint32))
+            off += g.n
+        return st[: 2 * len(graphs)].reshape(-1, 2), sets, ms.value
 
     def copy_probe(self, nbytes: int) -> float:
         """One plain device copy of nbytes (k_copy_probe); returns its device ms."""

@@ -4,6 +4,7 @@
                                  [--workload rmat21|powerlaw_communities|powerlaw_small] [--repeat R] [--num 1 --den 10]
                                  [--passes 10]
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
+    python -m graph_analyzer_amd greedy_many [--graphs 256] [--passes 10]   (seeded dense graphs of 100-1,000 vertices)
     python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
 
 Meant for profiling a single stage (rocprofv3 -- python3 -m graph_analyzer_amd core --workload rmat21) and for quick
@@ -25,7 +26,8 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
-    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "greedy", "batch", "probe"])
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "greedy", "greedy_many", "batch",
+                                      "probe"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--num", type=int, default=1)
@@ -47,6 +49,24 @@ def main(argv=None):
                 lines = n * max(seg // 128, 1)
                 print(f"gather {mib} MiB, {seg}-byte segments: {best:.3f} ms, {n / best / 1e6:.1f} G segments/s, "
                       f"{lines / best / 1e6:.1f} G 128-byte lines/s, {n * seg / best / 1e9:.2f} TB/s useful", flush=True)
+        return 0
+    if args.stage == "greedy_many":   # Greedy++ on many small dense graphs: one launch vs one call per graph
+        rng = np.random.default_rng(2024)
+        graphs = []
+        for _ in range(args.graphs):
+            n = int(np.exp(rng.uniform(np.log(100), np.log(1000))))
+            p = float(rng.uniform(0.05, 0.4))
+            a, b = np.triu_indices(n, 1)
+            keep = rng.random(a.size) < p
+            graphs.append(ga.DeviceGraph.from_pairs(ctx, np.stack([a[keep], b[keep]], 1).astype(np.uint32)))
+        print(f"greedy_many: {len(graphs)} graphs, n {sum(g.n for g in graphs)}, m {sum(g.m for g in graphs)}", flush=True)
+        for _ in range(args.repeat):
+            st, _, ms = ctx.greedy_pp_many(graphs, args.passes)
+            one = sum(g.greedy_pp(args.passes)[4] for g in graphs)
+            print(f"greedy++ {args.passes} passes: one launch {ms:.2f} ms, one call per graph {one:.2f} ms (device), "
+                  f"{one / ms:.1f}x", flush=True)
+        for g in graphs:
+            g.close()
         return 0
     if args.stage == "batch":
         bp = gen.BatchPairs(gen.batch_mix_specs(args.graphs))

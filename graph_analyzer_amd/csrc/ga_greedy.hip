@@ -218,12 +218,29 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
 // a global store inside the loop would make every barrier wait for it.
 // The minimum slots and counters rotate over three and the published tri_H over two, so each is reset one deletion
 // before it is written and never written while a slower lane may still read it.
+// One graph of a dense launch: workgroup b runs jobs[b].
+struct GreedyJob {
+    uint32_t n, m, wpad, pad;
+    const uint32_t* src;
+    const uint32_t* col;
+    const uint32_t* tri0;
+    const unsigned long long* dT;
+    uint32_t* orders;                 // passes * n
+    unsigned long long* per_pass;     // 2 * passes
+    GreedyOut* out;
+};
+
 template <int DB, int OWN>
-__global__ __launch_bounds__(DB) void k_greedy_dense(uint32_t n, uint32_t m, uint32_t wpad, const uint32_t* __restrict__ src,
-                                                           const uint32_t* __restrict__ col, const uint32_t* __restrict__ tri0,
-                                                           const unsigned long long* __restrict__ dT, uint32_t passes,
-                                                           uint32_t* __restrict__ orders, unsigned long long* __restrict__ per_pass,
-                                                           GreedyOut* __restrict__ out) {
+__global__ __launch_bounds__(DB) void k_greedy_dense(const GreedyJob* __restrict__ jobs, uint32_t passes) {
+    const GreedyJob& job = jobs[blockIdx.x];
+    const uint32_t n = job.n, m = job.m, wpad = job.wpad;
+    const uint32_t* __restrict__ src = job.src;
+    const uint32_t* __restrict__ col = job.col;
+    const uint32_t* __restrict__ tri0 = job.tri0;
+    uint32_t* __restrict__ orders = job.orders;
+    unsigned long long* __restrict__ per_pass = job.per_pass;
+    GreedyOut* __restrict__ out = job.out;
+    const unsigned long long* __restrict__ dT = job.dT;
     extern __shared__ __align__(16) uint32_t adj[];                  // n rows of wpad words
     uint32_t* alive_w = adj + (size_t)n * wpad;                      // the alive row
     __shared__ unsigned long long s_min[3];
@@ -375,6 +392,19 @@ __global__ void k_greedy_set(const uint32_t* __restrict__ flags, const uint32_t*
         if (flags[v]) set[pos[v]] = v;
 }
 
+// In-set flags of many graphs: graph y (blockIdx.y + base) marks order[cut .. n) of its best pass at its offset.
+__global__ void k_greedy_flags_many(const GreedyJob* __restrict__ jobs, const unsigned long long* __restrict__ voff, uint32_t base,
+                                    uint8_t* __restrict__ flags) {
+    const uint32_t gi = base + blockIdx.y;
+    const GreedyJob& j = jobs[gi];
+    const GreedyOut* o = j.out;
+    if (o->best_pass == UINT32_MAX || o->bd == 0) return;
+    const uint32_t* order = j.orders + (size_t)o->best_pass * j.n;
+    uint8_t* f = flags + voff[gi];
+    for (uint32_t i = o->best_cut + blockIdx.x * blockDim.x + threadIdx.x; i < j.n; i += gridDim.x * blockDim.x)
+        if (order[i] < j.n) f[order[i]] = 1;
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { ga_free(p); }
@@ -421,7 +451,7 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
                        (unsigned long long)passes * n);
     GA_HIP(ctx, hipSetDevice(ctx->device));
     GA_TRY(ga_graph_ensure_undirected(ctx, g));
-    DevBuf dtri, dorders, dpass, dout, dflags, dpos, dset, dT;
+    DevBuf dtri, dorders, dpass, dout, dflags, dpos, dset, dT, djob;
     GA_TRY(ga_malloc(ctx, &dtri.p, (size_t)std::max(n, 1u) * 4));
     GA_TRY(ga_malloc(ctx, &dorders.p, (size_t)std::max<uint64_t>((uint64_t)passes * n, 1) * 4));
     GA_TRY(ga_malloc(ctx, &dpass.p, (size_t)passes * 16));
@@ -438,9 +468,11 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)dout.p, sizeof(GreedyOut) / 8, 0));
     if (g->m) GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, (unsigned long long*)dT.p, nullptr, (uint32_t*)dtri.p, nullptr, nullptr));
     if (n && n <= dense_max(ctx)) {
-        k_greedy_dense<DENSE_BLOCK, DENSE_OWN><<<1, DENSE_BLOCK, dense_lds(n), ctx->stream>>>(n, g->m, dense_stride(n), g->src, g->col, (const uint32_t*)dtri.p,
-                                                                 (const unsigned long long*)dT.p, passes, (uint32_t*)dorders.p,
-                                                                 (unsigned long long*)dpass.p, (GreedyOut*)dout.p);
+        const GreedyJob job = {n, g->m, dense_stride(n), 0, g->src, g->col, (const uint32_t*)dtri.p, (const unsigned long long*)dT.p,
+                               (uint32_t*)dorders.p, (unsigned long long*)dpass.p, (GreedyOut*)dout.p};
+        GA_TRY(ga_malloc(ctx, &djob.p, sizeof job));
+        GA_HIP(ctx, hipMemcpyAsync(djob.p, &job, sizeof job, hipMemcpyHostToDevice, ctx->stream));
+        k_greedy_dense<DENSE_BLOCK, DENSE_OWN><<<1, DENSE_BLOCK, dense_lds(n), ctx->stream>>>((const GreedyJob*)djob.p, passes);
         GA_TRY(ga_launch_check(ctx, "k_greedy_dense"));
     } else if (n) {
         k_greedy_pp<<<1, GR_BLOCK, greedy_lds(n), ctx->stream>>>(n, g->rowptr, g->col, g->urow, g->ucol, (const uint32_t*)dtri.p,
@@ -475,6 +507,115 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     if (tri) *tri = h.bd ? h.bn : 0;
     if (verts) *verts = h.bd;
     if (set_size) *set_size = count;
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    return GA_OK;
+}
+
+// Greedy++ on many graphs: one workgroup per graph in one launch for the graphs the dense form takes, the CSR form one
+// graph at a time for the others.
+extern "C" int ga_greedy_pp_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, uint32_t passes, uint64_t* stats,
+                                 uint8_t* in_set, float* ms) {
+    if (!ctx) return GA_ERR_ARG;
+    if (count && (!graphs || !stats)) return ga_fail(ctx, GA_ERR_ARG, "graphs and stats are required");
+    if (passes == 0) return ga_fail(ctx, GA_ERR_ARG, "greedy_pp needs at least one pass");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t dmax = dense_max(ctx), cap = ga_greedy_max_vertices(ctx);
+    std::vector<unsigned long long> voff((size_t)count + 1, 0);
+    std::vector<size_t> slab_off((size_t)count + 1, 0);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    for (uint32_t i = 0; i < count; i++) {
+        const ga_graph* g = graphs[i];
+        if (!g || g->ctx != ctx) return ga_fail(ctx, GA_ERR_ARG, "graph %u is NULL or belongs to another context", i);
+        if (g->n > cap) return ga_fail(ctx, GA_ERR_ARG, "graph %u: n = %u exceeds greedy_pp's %u vertices", i, g->n, cap);
+        if ((uint64_t)passes * g->n > (1ull << 28))
+            return ga_fail(ctx, GA_ERR_ARG, "graph %u: passes * n = %llu exceeds 2^28", i, (unsigned long long)passes * g->n);
+        voff[i + 1] = voff[i] + g->n;
+        // per graph: tri0 (n), T (u64), orders (passes * n), per_pass (2 * passes u64); the results sit in one array
+        slab_off[i + 1] = slab_off[i] + al((size_t)g->n * 4) + al(8) + al((size_t)passes * g->n * 4) + al((size_t)passes * 16);
+    }
+    if (!count) return GA_OK;
+    DevBuf slab, djobs, dvoff, dflags, douts;
+    GA_TRY(ga_malloc(ctx, &slab.p, std::max<size_t>(slab_off[count], 256)));
+    GA_TRY(ga_malloc(ctx, &douts.p, (size_t)count * sizeof(GreedyOut)));
+    GA_TRY(ga_malloc(ctx, &djobs.p, (size_t)count * sizeof(GreedyJob)));
+    GA_TRY(ga_malloc(ctx, &dvoff.p, ((size_t)count + 1) * 8));
+    if (in_set) GA_TRY(ga_malloc(ctx, &dflags.p, std::max<size_t>(voff[count], 1)));
+    std::vector<GreedyJob> jobs(count);
+    std::vector<uint32_t> dense_ids, sparse_ids;
+    size_t lds = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        ga_graph* g = graphs[i];
+        char* base = (char*)slab.p + slab_off[i];
+        GreedyJob& j = jobs[i];
+        j.n = g->n;
+        j.m = g->m;
+        j.wpad = dense_stride(g->n);
+        j.pad = 0;
+        j.src = g->src;
+        j.col = g->col;
+        j.tri0 = (const uint32_t*)base;
+        base += al((size_t)g->n * 4);
+        j.dT = (const unsigned long long*)base;
+        base += al(8);
+        j.orders = (uint32_t*)base;
+        base += al((size_t)passes * g->n * 4);
+        j.per_pass = (unsigned long long*)base;
+        j.out = (GreedyOut*)douts.p + i;
+        if (g->n && g->n <= dmax) {
+            dense_ids.push_back(i);
+            lds = std::max(lds, dense_lds(g->n));
+        } else if (g->n) {
+            sparse_ids.push_back(i);
+            GA_TRY(ga_graph_ensure_undirected(ctx, g));
+        }
+    }
+    std::vector<GreedyJob> djv;
+    for (uint32_t i : dense_ids) djv.push_back(jobs[i]);
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_TRY(ga_fill_u8(ctx, (uint8_t*)slab.p, slab_off[count], 0));
+    GA_TRY(ga_fill_u8(ctx, (uint8_t*)douts.p, (size_t)count * sizeof(GreedyOut), 0));
+    GA_HIP(ctx, hipMemcpyAsync(djobs.p, jobs.data(), (size_t)count * sizeof(GreedyJob), hipMemcpyHostToDevice, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(dvoff.p, voff.data(), ((size_t)count + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+    for (uint32_t i = 0; i < count; i++)   // per-vertex triangles and T of every graph
+        if (graphs[i]->m)
+            GA_TRY(ga_tc_launch(ctx, graphs[i], GA_TC_HASH, (unsigned long long*)jobs[i].dT, nullptr, (uint32_t*)jobs[i].tri0, nullptr,
+                                nullptr));
+    DevBuf ddense;
+    if (!djv.empty()) {
+        GA_TRY(ga_malloc(ctx, &ddense.p, djv.size() * sizeof(GreedyJob)));
+        GA_HIP(ctx, hipMemcpyAsync(ddense.p, djv.data(), djv.size() * sizeof(GreedyJob), hipMemcpyHostToDevice, ctx->stream));
+        k_greedy_dense<DENSE_BLOCK, DENSE_OWN><<<(unsigned)djv.size(), DENSE_BLOCK, lds, ctx->stream>>>((const GreedyJob*)ddense.p, passes);
+        GA_TRY(ga_launch_check(ctx, "k_greedy_dense (many)"));
+    }
+    for (uint32_t i : sparse_ids) {
+        const ga_graph* g = graphs[i];
+        const GreedyJob& j = jobs[i];
+        k_greedy_pp<<<1, GR_BLOCK, greedy_lds(g->n), ctx->stream>>>(g->n, g->rowptr, g->col, g->urow, g->ucol, j.tri0, j.dT, passes,
+                                                                  j.orders, j.per_pass, j.out);
+        GA_TRY(ga_launch_check(ctx, "k_greedy_pp (many)"));
+    }
+    if (in_set) {
+        GA_TRY(ga_fill_u8(ctx, (uint8_t*)dflags.p, std::max<size_t>(voff[count], 1), 0));
+        for (uint32_t b0 = 0; b0 < count; b0 += 65535u) {
+            const dim3 grid(4, std::min<uint32_t>(65535u, count - b0));
+            k_greedy_flags_many<<<grid, 256, 0, ctx->stream>>>((const GreedyJob*)djobs.p, (const unsigned long long*)dvoff.p, b0,
+                                                              (uint8_t*)dflags.p);
+        }
+        GA_TRY(ga_launch_check(ctx, "k_greedy_flags_many"));
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    std::vector<GreedyOut> outs(count);
+    GA_HIP(ctx, hipMemcpy(outs.data(), douts.p, (size_t)count * sizeof(GreedyOut), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < count; i++) {
+        const GreedyOut& h = outs[i];
+        if (h.error)
+            return ga_fail(ctx, GA_ERR_STATE, "greedy_pp graph %u: %s", i,
+                           h.error == 1 ? "triangle bookkeeping inconsistent" : "no alive vertex found for a deletion");
+        stats[2 * i] = h.bd ? h.bn : 0;
+        stats[2 * i + 1] = h.bd;
+    }
+    if (in_set && voff[count]) GA_HIP(ctx, hipMemcpy(in_set, dflags.p, voff[count], hipMemcpyDeviceToHost));
     if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
     return GA_OK;
 }

@@ -232,6 +232,14 @@ int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t* tri, uint6
                  uint32_t* set_size, float* ms);
 /* The largest vertex count ga_greedy_pp takes on this device (no counterpart in the reference, whose CPU code has none). */
 uint32_t ga_greedy_max_vertices(ga_ctx* ctx);
+/* Greedy++ with `passes` passes on each of `count` graphs of this context, graph by graph equal to ga_greedy_pp: one
+ * workgroup per graph in one launch for graphs of up to 1,120 vertices (the LDS-bitmap form on MI355X), one launch
+ * per graph for larger ones (each at most ga_greedy_max_vertices). stats: 2 per graph, {tri, verts} of the best density
+ * (0 / 0 without triangles); in_set (may be NULL): n_0 + ... + n_{count-1} bytes, graph i's vertices at offset
+ * n_0 + ... + n_{i-1}, 1 for the vertices of its best set. ms: device time of the whole call. No counterpart in the
+ * reference, which runs one graph per process. */
+int ga_greedy_pp_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, uint32_t passes, uint64_t* stats, uint8_t* in_set,
+                      float* ms);
 
 /* ---- batching: many graphs resident together (no reference counterpart: the reference runs one graph per process;
  *      per-graph semantics are exactly load_graph + the calls above, graph by graph) ----------------------------- */

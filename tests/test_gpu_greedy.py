@@ -143,3 +143,30 @@ def test_greedy_size_limit_and_arguments(ctx):
         g.greedy_pp(0)
     assert ei.value.code == ga.GA_ERR_ARG
     g.close()
+
+
+def test_greedy_many_equals_one_by_one(ctx):
+    """ga_greedy_pp_many: every graph's best density and set equal ga_greedy_pp's and the oracle's, for a mix of dense
+    graphs (one launch), graphs above the LDS-bitmap size (their own launches), triangle-free and empty graphs; twice in a
+    row with a count in between."""
+    rng = np.random.default_rng(31)
+    raws = []
+    for i in range(40):
+        n = int(rng.integers(5, 900))
+        raws.append(random_graph(rng, n, float(rng.uniform(0.02, 0.5)), dup=0.1, loops=1))
+    raws[3] = np.array(clique(12), np.uint32)
+    raws[5] = np.array([(0, i) for i in range(1, 40)], np.uint32)            # no triangle
+    raws[7] = np.zeros((0, 2), np.uint32)                                    # no edge: n = 1 after the build
+    raws[9] = np.concatenate([random_graph(rng, 1500, 0.01), np.array([[0, 1499]], np.uint32)])   # CSR form
+    graphs = [ga.DeviceGraph.from_pairs(ctx, r) for r in raws]
+    for rep in range(2):
+        st, sets, ms = ctx.greedy_pp_many(graphs, 4)
+        for i, (g, r) in enumerate(zip(graphs, raws)):
+            t, v, s, _, _ = g.greedy_pp(4)
+            assert (int(st[i][0]), int(st[i][1])) == (t, v) and np.array_equal(sets[i], s), (rep, i)
+            if i % 5 == 0:
+                ot, ov, os_, _ = oracle.OGraph.from_pairs(r).greedy_pp(4)
+                assert (t, v) == (ot, ov) and np.array_equal(s, os_), (rep, i)
+        graphs[0].count()
+    for g in graphs:
+        g.close()

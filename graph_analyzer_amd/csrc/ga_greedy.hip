@@ -3,16 +3,11 @@
 //
 // The reference runs it on the graph that prune_to_core leaves (gpu_dsg.cu:344-356): hundreds to a few thousand
 // vertices. Each pass deletes the vertices one at a time, so one workgroup of 1,024 lanes runs a whole Greedy++ (all
-// passes, one launch) with the per-vertex state in LDS:
-//   load[v]  u64  the Greedy++ load l_v (in place: a deleted vertex is not read again in the pass, dsg.h:184-185)
-//   cur[v]   u32  tri_H(v), the triangles of the current subgraph H through v (<= C(n-1, 2) < 2^32 at these sizes)
-//   alive    bitmap of H
-//   nbr      bitmap of the alive neighbours of the vertex being deleted
-// One deletion: argmin of (load + cur, id) over H (each lane scans n / 1024 vertices, then a wave and a workgroup
-// reduction), the nbr bitmap from v's undirected row, then every triangle {v, a, b} of H once: wave w takes v's
-// neighbours a = w, w + 16, ... and its lanes read a's oriented out-row; b is a hit when its nbr bit is set, and
-// cur[a], cur[b] drop by one with LDS atomics. The triangles found must equal cur[v] (the reference's self-check).
-// The adjacency (rowptr/col and the undirected urow/ucol) stays in HBM and L2.
+// passes, one launch) with the graph's state on the CU. Two forms:
+//   k_greedy_dense  (n <= 1,120 with 160 KB of LDS) the adjacency as a bitmap in LDS, per-vertex state in registers,
+//                   one barrier per deletion; also many graphs in one launch, one workgroup each
+//   k_greedy_pp     (the CSR form, up to about 13,000 vertices) per-vertex state in LDS, the adjacency in HBM
+// Every pass's deletion order is kept; the best set is that pass's order from the cut on.
 #include "ga_internal.hpp"
 
 namespace {
@@ -75,6 +70,13 @@ __device__ __forceinline__ uint32_t wave_sum_u32_dpp(uint32_t x) {
 
 __device__ __forceinline__ bool bit(const uint32_t* b, uint32_t x) { return (b[x >> 5] >> (x & 31)) & 1u; }
 
+// CSR form. LDS holds load[v] (u64, updated in place: a deleted vertex is not read again in the pass, dsg.h:184-185),
+// tri_H[v] (u32, at most C(n-1, 2) < 2^32 at these sizes), the alive bitmap of H and the bitmap of the alive neighbours
+// of the vertex being deleted. One deletion: argmin of (load + tri_H, id) over H (each lane scans n / 1024 vertices,
+// then a wave and a workgroup reduction), the neighbour bitmap from v's undirected row, then every triangle {v, a, b}
+// of H once: wave w takes v's neighbours a = w, w + 16, ... and its lanes read a's oriented out-row; b is a hit when
+// its bit is set, and tri_H[a], tri_H[b] drop by one with LDS atomics. The triangles found must equal tri_H[v] (the
+// reference's self-check).
 __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
                                                         const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol,
                                                         const uint32_t* __restrict__ tri0, const unsigned long long* __restrict__ dT,
@@ -203,9 +205,9 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
 // padded to an odd stride so that lanes reading consecutive rows hit different banks. A workgroup of DB lanes runs it;
 // lane t owns the vertices t + k * DB (k < OWN) and keeps their load, tri_H and alive flag in registers. 1,024 lanes
 // owning two vertices each beat 256 lanes owning five (14.4 vs 24.2 ms for 10 passes on a 729-vertex graph): the
-// neighbours' popcount rows are the serial part, and more lanes split them finer. A vertex's key is (load + tri_H) << 11 | id, so one
-// 64-bit minimum is the (key, id) order; a key is at most passes * C(n-1, 2) + C(n-1, 2) < 2^32 * 2^20, so it fits the
-// 53 bits above the id.
+// neighbours' popcount rows are the serial part, and more lanes split them finer. A vertex's key is
+// (load + tri_H) << 11 | id, so one 64-bit minimum is the (key, id) order; a key is at most
+// passes * C(n-1, 2) + C(n-1, 2) < 2^32 * 2^20, so it fits the 53 bits above the id.
 // Before the first pass each owner takes tri(x) from the job's per-vertex count or, for a job without one (many graphs
 // in one launch), counts it from the bitmap (popcount of N(x) & N(y) over x's neighbours y, halved); T = the sum / 3.
 // One graph is faster with the separate count (0.19 vs 0.5 ms on a 729-vertex graph: the owners of hubs are the
@@ -218,10 +220,11 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
 //   every wave takes the minimum of its lanes' new keys (DPP) and merges it into the next deletion's slot with one LDS
 //   atomicMin; wave sums of the counts go into this deletion's counter;
 //   after the barrier, the next deletion first checks this one (counts = 2 tri_H(v), no count above tri_H).
-// The deletion order stays in registers (lane t keeps the deletions t + k * DB) and is written once per pass:
-// a global store inside the loop would make every barrier wait for it.
+// The deletion order stays in registers (lane t keeps the deletions t + k * DB) and is written once per pass, so the
+// loop touches no global memory (one global store per deletion measured the same).
 // The minimum slots and counters rotate over three and the published tri_H over two, so each is reset one deletion
 // before it is written and never written while a slower lane may still read it.
+
 // One graph of a dense launch: workgroup b runs jobs[b].
 struct GreedyJob {
     uint32_t n, m, wpad, pad;

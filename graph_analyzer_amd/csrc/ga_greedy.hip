@@ -214,12 +214,14 @@ __global__ __launch_bounds__(GR_BLOCK) void k_greedy_pp(uint32_t n, const uint32
 // serial part), many graphs without it (one launch instead of a few count launches per graph).
 // One deletion, one barrier:
 //   v = the minimum published for this deletion (one LDS word, the same for every lane);
-//   each owner of an alive neighbour x of v counts c = |N(x) & N(v) & H| with W popcounts and drops tri_H(x) by c
-//   (every triangle {v, x, b} of H is counted at x and at b, so the counts sum to 2 tri_H(v)); the owner of v retires
-//   it and publishes tri_H(v);
+//   each owner of an alive neighbour x of v counts c = |N(x) & N(v)| with W popcounts (rows hold alive neighbours only,
+//   so this is the triangles {v, x, b} of H) and drops tri_H(x) by c (every such triangle is counted at x and at b, so
+//   the counts sum to 2 tri_H(v)), then clears v from its own row x; the owner of v retires it and publishes tri_H(v);
 //   every wave takes the minimum of its lanes' new keys (DPP) and merges it into the next deletion's slot with one LDS
 //   atomicMin; wave sums of the counts go into this deletion's counter;
 //   after the barrier, the next deletion first checks this one (counts = 2 tri_H(v), no count above tri_H).
+// Rows only lose bits inside a pass (v's own row is not read again once v is deleted), so every pass after the first
+// rebuilds them from the CSR (m atomics, a few percent of a pass).
 // The deletion order stays in registers (lane t keeps the deletions t + k * DB) and is written once per pass, so the
 // loop touches no global memory (one global store per deletion measured the same).
 // The minimum slots and counters rotate over three and the published tri_H over two, so each is reset one deletion
@@ -247,7 +249,6 @@ __global__ __launch_bounds__(DB) void k_greedy_dense(const GreedyJob* __restrict
     unsigned long long* __restrict__ per_pass = job.per_pass;
     GreedyOut* __restrict__ out = job.out;
     extern __shared__ __align__(16) uint32_t adj[];                  // n rows of wpad words
-    uint32_t* alive_w = adj + (size_t)n * wpad;                      // the alive row
     __shared__ unsigned long long s_min[3];
     __shared__ uint32_t s_cv[2], s_found[3], s_bad;
     __shared__ unsigned long long s_T0;
@@ -303,7 +304,15 @@ __global__ __launch_bounds__(DB) void k_greedy_dense(const GreedyJob* __restrict
     uint64_t bn = 0, bd = 0;
     uint32_t best_pass = UINT32_MAX, best_cut = 0, err = 0;
     for (uint32_t p = 0; p < passes && !err; p++) {
-        for (uint32_t i = tid; i < W; i += DB) alive_w[i] = (i + 1) * 32 <= n ? ~0u : ((1u << (n & 31)) - 1u);
+        if (p) {   // the rows of the previous pass lost every deleted vertex: rebuild them from the CSR
+            for (uint32_t i = tid; i < n * wpad; i += DB) adj[i] = 0;
+            __syncthreads();
+            for (uint32_t e = tid; e < m; e += DB) {
+                const uint32_t a = src[e], b = col[e];
+                atomicOr(&adj[a * wpad + (b >> 5)], 1u << (b & 31));
+                atomicOr(&adj[b * wpad + (a >> 5)], 1u << (a & 31));
+            }
+        }
         if (tid < 3) {
             s_found[tid] = 0;
             s_min[tid] = ~0ull;
@@ -356,14 +365,14 @@ __global__ __launch_bounds__(DB) void k_greedy_dense(const GreedyJob* __restrict
                     s_cv[step & 1] = cur[k];
                     alive[k] = false;
                     load[k] += cur[k];
-                    alive_w[v >> 5] &= ~(1u << (v & 31));   // bit v is clear in av, so no count of this step reads it
                     continue;
                 }
                 if ((av[x >> 5] >> (x & 31)) & 1u) {
-                    const uint32_t* ax = adj + x * wpad;
+                    uint32_t* ax = adj + x * wpad;
                     uint32_t c = 0;
 #pragma unroll 8
-                    for (uint32_t i = 0; i < W; i++) c += __popc(ax[i] & av[i] & alive_w[i]);
+                    for (uint32_t i = 0; i < W; i++) c += __popc(ax[i] & av[i]);
+                    ax[v >> 5] &= ~(1u << (v & 31));   // x's own row drops v; no other lane reads it this step
                     if (c > cur[k]) s_bad = 1;
                     cur[k] -= min(c, cur[k]);
                     found += c;
@@ -400,7 +409,7 @@ __global__ __launch_bounds__(DB) void k_greedy_dense(const GreedyJob* __restrict
             per_pass[2 * p] = bn;
             per_pass[2 * p + 1] = bd;
         }
-        __syncthreads();   // the alive row, the slots and the counters are reset by the next pass
+        __syncthreads();   // the rows, the slots and the counters are reset by the next pass
     }
     if (tid == 0) {
         out->bn = bn;

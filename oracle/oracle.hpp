@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <queue>
+#include <set>
 #include <string>
 #include <utility>
 #include <vector>
@@ -649,33 +650,38 @@ struct GreedyResult {
 // Greedy++ with triangles as hyperedges (Flowless Alg 2; dsg.h:176-250 greedy_pp, greedy = one pass). Each pass starts
 // from the whole graph and deletes, one at a time, the alive vertex with the smallest (load + tri_H, id), adding tri_H
 // to its load; the current subgraph H is compared with the best before every deletion and replaces it only when
-// strictly denser. This restatement finds the minimum by a scan over the alive vertices (the reference keeps a lazy
-// heap; both take the same vertex) and the triangles through v as the oriented edges a -> b between two alive
-// neighbours of v.
+// strictly denser. This restatement keeps the alive vertices in an ordered set of (load + tri_H, id), re-keying the
+// vertices a deletion touched once it is done (the reference keeps a lazy heap; both take the same vertex), and finds
+// the triangles through v as the oriented edges a -> b between two alive neighbours of v.
 inline GreedyResult greedy_pp(const Graph& g, uint32_t passes) {
     GreedyResult r;
     std::vector<uint32_t> urow, ucol;
     undirected_csr(g, urow, ucol);
     std::vector<uint64_t> tri0, cur;
     const uint64_t T0 = triangle_degrees(g, tri0);
-    std::vector<uint64_t> load(g.n, 0);
-    std::vector<uint8_t> alive, nb(g.n, 0);
-    std::vector<uint32_t> order, best_order;
+    std::vector<uint64_t> load(g.n, 0), oldkey(g.n, 0);
+    std::vector<uint8_t> alive, nb(g.n, 0), touched(g.n, 0);
+    std::vector<uint32_t> order, best_order, tl;
+    std::set<std::pair<uint64_t, uint32_t>> keys;   // (load + tri_H, id) of the alive vertices
     size_t best_cut = 0;
     for (uint32_t p = 0; p < passes; p++) {
         cur = tri0;
         alive.assign(g.n, 1);
         order.clear();
+        keys.clear();
+        for (uint32_t x = 0; x < g.n; x++) keys.emplace(load[x] + cur[x], x);
         uint64_t T = T0;
         bool improved = false;
         size_t cut = 0;
         for (uint32_t left = g.n; left > 0; left--) {
             if (dens_gt(T, left, r.tri, r.verts)) { r.tri = T; r.verts = left; improved = true; cut = order.size(); }
-            uint32_t v = UINT32_MAX;
-            for (uint32_t x = 0; x < g.n; x++)
-                if (alive[x] && (v == UINT32_MAX || load[x] + cur[x] < load[v] + cur[v])) v = x;
+            const uint32_t v = keys.begin()->second;
+            keys.erase(keys.begin());
             for (uint32_t j = urow[v]; j < urow[v + 1]; j++) nb[ucol[j]] = alive[ucol[j]];
             uint64_t found = 0;
+            auto touch = [&](uint32_t x) {
+                if (!touched[x]) { touched[x] = 1; oldkey[x] = load[x] + cur[x]; tl.push_back(x); }
+            };
             for (uint32_t j = urow[v]; j < urow[v + 1]; j++) {
                 const uint32_t a = ucol[j];
                 if (!nb[a]) continue;
@@ -683,12 +689,20 @@ inline GreedyResult greedy_pp(const Graph& g, uint32_t passes) {
                     const uint32_t b = g.col[e];
                     if (!nb[b]) continue;
                     if (!cur[a] || !cur[b]) { r.ok = false; return r; }
+                    touch(a);
+                    touch(b);
                     cur[a]--;
                     cur[b]--;
                     found++;
                 }
             }
             for (uint32_t j = urow[v]; j < urow[v + 1]; j++) nb[ucol[j]] = 0;
+            for (uint32_t x : tl) {
+                keys.erase({oldkey[x], x});
+                keys.emplace(load[x] + cur[x], x);
+                touched[x] = 0;
+            }
+            tl.clear();
             if (found != cur[v]) { r.ok = false; return r; }
             load[v] += cur[v];
             alive[v] = 0;

@@ -1,10 +1,11 @@
 // gpu_dsg.cpp - (k,triangle)-core driver with the core part of the reference's contract (gpu_dsg.cu:1-46, 201-316):
-//   usage: gpu_dsg graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedypp] [--verbose]
+//   usage: gpu_dsg graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedy] [--no-greedypp] [--verbose]
 //   rows impl,graph,n,m,triangles,vertices,density,time_ms,h2d_ms,total_ms,rounds,kmax for cpu_core_decomp (Alg 3),
 //   gpu_core_decomp (incremental host loop, one sync per round), gpu_core_decomp_fused (ga_core_run) and, with
-//   --with-recount, gpu_core_decomp_recount. Rows = the kmax-core. Then prune_kcore (gpu_dsg.cu:312-328): lower bound =
-//   the kmax-core density, keep core >= ceil(lb) on the device core numbers (ga_core_prune), row n,m = the pruned graph,
-//   rounds = ceil(lb). Printed only if every GPU core array equals CPU Alg 3 element-wise, the round counts of all GPU
+//   --with-recount, gpu_core_decomp_recount. Rows = the kmax-core. Then the lower bound (gpu_dsg.cu:318-330): the
+//   kmax-core density, raised to Greedy's on the whole graph when that is denser (cpu_greedy, the oracle's one-pass
+//   greedy_pp as the reference's CPU row; --no-greedy skips it, as on graphs with huge hubs), and prune_kcore: keep
+//   core >= ceil(lb) on the device core numbers (ga_core_prune), row n,m = the pruned graph, rounds = ceil(lb). Printed only if every GPU core array equals CPU Alg 3 element-wise, the round counts of all GPU
 //   designs agree, and the pruned graph equals the oracle's prune_to_core with its own core numbers equal to the
 //   restricted ones; exit 1 otherwise. Then Greedy++ with T passes on the pruned graph (gpu_dsg.cu:344-356):
 //   cpu_greedy_pp_pruned (the oracle) and gpu_greedy_pp_pruned (ga_greedy_pp), rounds = T; the GPU result (density, set
@@ -17,7 +18,7 @@
 
 int main(int argc, char** argv) {
     std::vector<const char*> pos;
-    bool recount = false, burnin = true, verbose = false, gpp = true;
+    bool recount = false, burnin = true, verbose = false, gpp = true, greedy_on = true;
     int repeats = 5;
     uint32_t T_pass = 10;
     for (int i = 1; i < argc; i++) {
@@ -25,13 +26,15 @@ int main(int argc, char** argv) {
         else if (strcmp(argv[i], "--no-burnin") == 0) burnin = false;
         else if (strcmp(argv[i], "--verbose") == 0) verbose = true;
         else if (strcmp(argv[i], "--no-greedypp") == 0) gpp = false;
+        else if (strcmp(argv[i], "--no-greedy") == 0) greedy_on = false;
         else if (strcmp(argv[i], "--T") == 0 && i + 1 < argc) T_pass = (uint32_t)std::max(1, atoi(argv[++i]));
         else if (strcmp(argv[i], "--repeats") == 0 && i + 1 < argc) repeats = std::max(1, atoi(argv[++i]));
         else if (argv[i][0] == '-') {}
         else pos.push_back(argv[i]);
     }
     if (pos.size() != 1) {
-        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedypp] [--verbose]\n", argv[0]);
+        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedy] [--no-greedypp] [--verbose]\n",
+                argv[0]);
         return 2;
     }
     const char* path = pos[0];
@@ -111,12 +114,21 @@ int main(int argc, char** argv) {
         fprintf(stderr, "MISMATCH round counts: incremental %u, fused %u, recount %u\n", inc.rounds, fused_rounds, rc_rounds);
         ok = false;
     }
-    // ---- lower bound and pruning (gpu_dsg.cu:312-328); the lower bound is the kmax-core density (no CPU greedy here) ----
+    // ---- lower bound and pruning (gpu_dsg.cu:318-330) ----
     uint64_t kprune = 0, T_pruned = 0;
-    double t_prune = 1e300;
+    double t_prune = 1e300, t_greedy = 0;
+    Dens lb{ktri, kverts};
+    orc::GreedyResult cgr;
+    if (ok && greedy_on) {
+        const double tg = now_ms();
+        cgr = orc::greedy_pp(og, 1);
+        t_greedy = now_ms() - tg;
+        if (!cgr.ok) { fprintf(stderr, "greedy failed: triangle bookkeeping inconsistent\n"); return 1; }
+        if (dens_gt(Dens{cgr.tri, cgr.verts}, lb)) lb = Dens{cgr.tri, cgr.verts};
+    }
     Graph gp;
     if (ok) {
-        kprune = kverts ? (ktri + kverts - 1) / kverts : 0;
+        kprune = dens_ceil(lb);
         std::vector<uint32_t> old_id, core_p;
         for (int i = -1; i < repeats; i++) {
             gaa::check(ctx, ga_ctx_sync(ctx), "sync");
@@ -140,8 +152,8 @@ int main(int argc, char** argv) {
             fprintf(stderr, "MISMATCH: restricted core numbers != core numbers of the pruned graph (or its count differs)\n");
             ok = false;
         } else {
-            fprintf(stderr, "lower bound %llu/%llu -> keep core >= %llu: pruned graph n=%u m=%u (of n=%u m=%u)\n", (unsigned long long)ktri,
-                    (unsigned long long)kverts, (unsigned long long)kprune, gp.n, gp.m, og.n, og.m);
+            fprintf(stderr, "lower bound %llu/%llu -> keep core >= %llu: pruned graph n=%u m=%u (of n=%u m=%u)\n", (unsigned long long)lb.num,
+                    (unsigned long long)lb.den, (unsigned long long)kprune, gp.n, gp.m, og.n, og.m);
         }
     }
     // ---- Greedy++ on the pruned graph (gpu_dsg.cu:344-356) ----
@@ -179,6 +191,7 @@ int main(int argc, char** argv) {
     rows.push_back(dsg_row("gpu_core_decomp", path, og.n, og.m, ktri, kverts, best_inc, 0, inc.rounds, kmax));
     rows.push_back(dsg_row("gpu_core_decomp_fused", path, og.n, og.m, ktri, kverts, best_fused, 0, fused_rounds, kmax));
     if (recount) rows.push_back(dsg_row("gpu_core_decomp_recount", path, og.n, og.m, ktri, kverts, best_rc, 0, rc_rounds, kmax));
+    if (greedy_on) rows.push_back(dsg_row("cpu_greedy", path, og.n, og.m, cgr.tri, cgr.verts, t_greedy, 0, og.n, 0));
     rows.push_back(dsg_row("prune_kcore", path, gp.n, gp.m, T_pruned, gp.n, t_prune, 0, kprune, kmax));
     if (have_gpp) {
         rows.push_back(dsg_row("cpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp_cpu, 0, T_pass, 0));

@@ -71,12 +71,16 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "2", "--with-recount"])
     assert rc == 0, err[-2000:]
     rs = {r[0]: r for r in rows(out)}
-    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "prune_kcore",
-                       "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"}
+    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "cpu_greedy",
+                       "prune_kcore", "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"}
     core, kmax = og.core()
     kt, kv = og.core_level_density(core, kmax)
     for name in ("cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused"):
         assert (int(rs[name][4]), int(rs[name][5]), int(rs[name][11])) == (kt, kv, kmax)
+    gt, gv, _, _ = og.greedy_pp(1)   # the lower bound is the denser of the kmax-core and Greedy (gpu_dsg.cu:318-330)
+    assert (int(rs["cpu_greedy"][4]), int(rs["cpu_greedy"][5])) == (gt, gv)
+    if gt * kv > kt * gv:
+        kt, kv = gt, gv
     k = (kt + kv - 1) // kv
     sub, _, _ = og.prune_to_core(core, k)
     p = rs["prune_kcore"]
@@ -85,6 +89,19 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     for name in ("cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"):
         r = rs[name]
         assert (int(r[2]), int(r[3]), int(r[4]), int(r[5]), int(r[10])) == (sub.n, sub.m, gt, gv, 10), r
+
+
+@pytest.mark.gpu
+def test_gpu_dsg_driver_no_greedy(graph_file):
+    """--no-greedy: no cpu_greedy row, and the lower bound is the kmax-core density alone."""
+    path, og = graph_file
+    rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "1", "--no-greedy", "--no-greedypp"])
+    assert rc == 0, err[-2000:]
+    rs = {r[0]: r for r in rows(out)}
+    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "prune_kcore"}
+    core, kmax = og.core()
+    kt, kv = og.core_level_density(core, kmax)
+    assert int(rs["prune_kcore"][10]) == (kt + kv - 1) // kv
 
 
 @pytest.mark.gpu

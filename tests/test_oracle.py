@@ -375,6 +375,20 @@ def test_greedy_equal_reference_build(seed):
             assert a[:2] == b[:2] and np.array_equal(a[2], b[2]) and a[3] == b[3], (seed, n, passes)
 
 
+def test_greedy_equal_reference_build_powerlaw():
+    """The 8,298-vertex power-law fixture graph (thousands of re-keyed vertices per pass): the oracle's greedy_pp equals
+    the reference build's at 1 and 3 passes."""
+    if not oracle.ref_available():
+        pytest.skip("reference build absent")
+    from graph_analyzer_amd import gen
+    raw = gen.powerlaw_small()
+    o = oracle.OGraph.from_pairs(raw)
+    r = oracle.RGraph.build(o.n, oracle.canonical_unique(raw))
+    for passes in (1, 3):
+        a, b = o.greedy_pp(passes), r.greedy_pp(passes)
+        assert a[:2] == b[:2] and np.array_equal(a[2], b[2]) and a[3] == b[3], passes
+
+
 def _set_triangles(uniq, s):
     keep = np.zeros(int(uniq.max()) + 1 if len(uniq) else 1, bool)
     keep[s[s < len(keep)]] = True

@@ -435,3 +435,59 @@ def test_greedy_closed_forms():
         o = oracle.OGraph.from_pairs(np.array(pairs, np.uint32).reshape(-1, 2))
         t, v, s, pp = o.greedy_pp(4)
         assert (t, v, len(s)) == (0, 0, 0) and pp == [(0, 0)] * 4
+
+
+def _greedy_model(n, pairs, passes, ge=False, use_load=True):
+    """A small Python model of Greedy++ whose two rules can be broken on purpose: `ge` replaces the strict improvement
+    test by >= (ties would keep the later, smaller subgraph), `use_load=False` drops the load term (every pass repeats
+    Charikar's Greedy)."""
+    adj = [set() for _ in range(n)]
+    for a, b in pairs:
+        adj[a].add(b)
+        adj[b].add(a)
+    tri0 = [sum(1 for i, a in enumerate(sorted(adj[v])) for b in sorted(adj[v])[i + 1:] if b in adj[a]) for v in range(n)]
+    T0 = sum(tri0) // 3
+    load = [0] * n
+    bn = bd = 0
+    best, per = None, []
+    for _ in range(passes):
+        cur, alive, order, T, improved, cut = tri0[:], [True] * n, [], T0, False, 0
+        for left in range(n, 0, -1):
+            a_, b_ = (bn, bd) if bd else (0, 1)
+            if ((T * b_ >= a_ * left) if ge else (T * b_ > a_ * left)) and not (ge and T == 0):
+                bn, bd, improved, cut = T, left, True, len(order)
+            v = min((x for x in range(n) if alive[x]), key=lambda x: ((load[x] if use_load else 0) + cur[x], x))
+            nb = [a for a in adj[v] if alive[a]]
+            for i, a in enumerate(nb):
+                for b in nb[i + 1:]:
+                    if b in adj[a]:
+                        cur[a] -= 1
+                        cur[b] -= 1
+            load[v] += cur[v]
+            alive[v] = False
+            T -= cur[v]
+            order.append(v)
+        if improved:
+            best = (order[:], cut)
+        per.append([bn, bd])
+    return bn, bd, (sorted(best[0][best[1]:]) if bd else []), per
+
+
+def test_greedy_fixtures_catch_broken_rules(golden):
+    """The golden Greedy++ vectors are sensitive: a faithful model reproduces them on every fixture graph of at most 300
+    vertices, while the same model with the tie rule flipped, or with the load term dropped, disagrees on at least one.
+    So the golden comparison of the oracle would catch either mistake."""
+    ge_differs = load_differs = 0
+    for rec in golden["graphs"]:
+        if rec["n"] > 300:
+            continue
+        g = oracle.OGraph.from_pairs(np.array(rec["raw_pairs"], dtype=np.uint32).reshape(-1, 2))
+        _, c, s = g.arrays()
+        pairs = list(zip(s.tolist(), c.tolist()))
+        e = rec["greedy_pp"]["3"]
+        want = (e["tri"], e["verts"], e["set"], e["per_pass"])
+        assert _greedy_model(g.n, pairs, 3) == want, rec["name"]
+        ge_differs += _greedy_model(g.n, pairs, 3, ge=True)[:3] != want[:3]
+        load_differs += _greedy_model(g.n, pairs, 3, use_load=False) != want
+    assert ge_differs >= 1 and load_differs >= 1
+

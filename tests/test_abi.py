@@ -105,3 +105,12 @@ def test_default_tuning_and_argument_checks_on_host():
         assert L.ga_ctx_gather_probe(ctx, nbytes, seg, None, None) == ga.GA_ERR_ARG
     assert L.ga_ctx_copy_probe(None, 1 << 20, None) == ga.GA_ERR_ARG
     assert L.ga_tc_default_tuning(None) == ga.GA_ERR_ARG
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2, a message and no result line (checked before any device
+    work, so this runs without a GPU)."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 2 and r.stdout == "" and "--gpus 2 is not supported" in r.stderr

@@ -20,7 +20,7 @@ GEN_PATH = os.path.join(_HERE, "libga_gen.so")
 GA_OK, GA_ERR_ARG, GA_ERR_HIP, GA_ERR_OOM, GA_ERR_ARITH, GA_ERR_STATE = range(6)
 
 # triangle-count variants (ga_amd.h)
-TC_AUTO, TC_MERGE, TC_GROUP4, TC_GROUP8, TC_GROUP16, TC_GROUP64, TC_HASH, TC_SORTED = range(8)
+TC_AUTO, TC_MERGE, TC_GROUP4, TC_GROUP8, TC_GROUP16, TC_GROUP64, TC_HASH, TC_SORTED, TC_NOSRC, TC_VERTEX = range(10)
 TC_VARIANTS = {
     "merge": TC_MERGE,
     "group4": TC_GROUP4,
@@ -29,6 +29,8 @@ TC_VARIANTS = {
     "group64": TC_GROUP64,
     "hash": TC_HASH,
     "sorted": TC_SORTED,
+    "nosrc": TC_NOSRC,
+    "vertex": TC_VERTEX,
 }
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)

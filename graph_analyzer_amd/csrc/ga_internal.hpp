@@ -114,6 +114,9 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* d_alive, uint32_t* d_tri,
                  uint32_t* d_sup, const uint32_t* d_done, const uint32_t* gof = nullptr, unsigned long long* gcnt = nullptr);
 
+// GA_TC_NOSRC / GA_TC_VERTEX count into *total (zeroed by the caller): ga_tc_simple.hip.
+int ga_tc_launch_simple(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* total);
+
 inline unsigned grid_for(uint64_t threads, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (threads + block - 1) / block;
     if (b < 1) b = 1;

@@ -1330,6 +1330,7 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         }
     }
     if (flags) return ga_fail(ctx, GA_ERR_ARG, "variant %d only counts", variant);
+    if (variant == GA_TC_NOSRC || variant == GA_TC_VERTEX) return ga_tc_launch_simple(ctx, g, variant, total);
     switch (variant) {
         case GA_TC_GROUP4: return launch_group<4>(ctx, g, total);
         case GA_TC_GROUP8: return launch_group<8>(ctx, g, total);
@@ -1349,6 +1350,8 @@ extern "C" const char* ga_tc_variant_name(int variant) {
         case GA_TC_GROUP64: return "group64_bsearch";
         case GA_TC_HASH: return "hash_lds_bucketed";
         case GA_TC_SORTED: return "merge_work_sorted";
+        case GA_TC_NOSRC: return "group4_rowptr_src";
+        case GA_TC_VERTEX: return "vertex_merge";
         default: return "unknown";
     }
 }

@@ -114,7 +114,13 @@ enum {
     GA_TC_HASH = 6,        /* longer list of each edge in an LDS hash set, shorter list streamed by lane groups
                               sized per degree bucket (roles of v3/v8 generalised, v4/v9 work ordering on device) */
     GA_TC_SORTED = 7,      /* one lane per edge in the device-built work order of ga_graph_edge_order (role of v9) */
-    GA_TC_NVARIANTS = 8
+    GA_TC_NOSRC = 8,       /* 4 lanes per edge without src[]: each edge's source recovered from rowptr, one binary search
+                              per wavefront range and then a 64-entry rowptr window in the lanes (role of v7,
+                              tc_edge_find_src, kernels.cuh:219-248) */
+    GA_TC_VERTEX = 9,      /* vertex-centric without src[]: one lane merges the out-edges of a vertex of out-degree <= 16,
+                              a whole wavefront those of a larger one (role of v6, tc_vertex_per_thread,
+                              kernels.cuh:195-217) */
+    GA_TC_NVARIANTS = 10
 };
 const char* ga_tc_variant_name(int variant);
 /* T = sum over oriented edges (u,v) of |N+(u) ∩ N+(v)| (graph.h:110-121). ms (may be NULL): device time of the

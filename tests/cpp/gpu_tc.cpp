@@ -59,7 +59,8 @@ int main(int argc, char** argv) {
         do { (void)gaa::gpu_triangles(up, GA_TC_MERGE); } while (now_ms() - t0 < 1500.0);
     }
     bool ok = true;
-    const int variants[] = {GA_TC_MERGE, GA_TC_GROUP4, GA_TC_GROUP8, GA_TC_GROUP16, GA_TC_GROUP64, GA_TC_SORTED, GA_TC_HASH};
+    const int variants[] = {GA_TC_MERGE, GA_TC_GROUP4, GA_TC_GROUP8, GA_TC_GROUP16, GA_TC_GROUP64, GA_TC_VERTEX, GA_TC_NOSRC, GA_TC_SORTED,
+                            GA_TC_HASH};
     for (int v : variants) {
         float best = 1e30f;
         uint64_t T = 0;

@@ -26,6 +26,31 @@ def _graph(ctx, name):
     return g, og, T
 
 
+ALL_VARIANTS = [ga.TC_MERGE, ga.TC_GROUP4, ga.TC_GROUP8, ga.TC_GROUP16, ga.TC_GROUP64, ga.TC_HASH, ga.TC_SORTED, ga.TC_NOSRC,
+                ga.TC_VERTEX]
+
+
+def test_full_size_rmat21_every_variant(ctx):
+    """The R-MAT configuration asks for every intersection variant bit-exact: each one counts the full R-MAT scale-21
+    graph to the oracle's T."""
+    g, og, T = _graph(ctx, "rmat21")
+    for v in ALL_VARIANTS:
+        Tv, ms = g.count(v)
+        print(f"rmat21 {ga.variant_name(v)}: {ms:.2f} ms", flush=True)
+        assert Tv == T, ga.variant_name(v)
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["rmat21", "powerlaw_communities"])
+def test_full_size_undirected_csr(ctx, name):
+    """build_undirected_csr (densest.h:162-171) on the device, byte for byte against the oracle's."""
+    g, og, T = _graph(ctx, name)
+    ur, uc = g.undirected()
+    our, ouc = og.undirected()
+    assert np.array_equal(ur, our) and np.array_equal(uc, ouc)
+    g.close()
+
+
 @pytest.mark.parametrize("name", ["rmat21", "powerlaw_communities"])
 def test_full_size_peel(ctx, name):
     """Batch peeling at eps 1/10 and 1/100 (the R-MAT configuration's two epsilons)."""

@@ -17,7 +17,8 @@ from conftest import clique, random_graph
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
-ALL_VARIANTS = [ga.TC_MERGE, ga.TC_GROUP4, ga.TC_GROUP8, ga.TC_GROUP16, ga.TC_GROUP64, ga.TC_HASH, ga.TC_SORTED]
+ALL_VARIANTS = [ga.TC_MERGE, ga.TC_GROUP4, ga.TC_GROUP8, ga.TC_GROUP16, ga.TC_GROUP64, ga.TC_HASH, ga.TC_SORTED, ga.TC_NOSRC,
+                ga.TC_VERTEX]
 EPS = [(1, 10), (1, 2), (1, 1), (3, 1), (0, 1)]
 
 
@@ -74,6 +75,9 @@ def test_golden_fixtures(ctx):
         assert g.vertex_participation()[1].tolist() == rec["tri"]
         assert g.edge_support()[1].tolist() == rec["sup"]
         assert g.edge_order().tolist() == rec["edge_order"]
+        ur, uc = g.undirected()   # build_undirected_csr (densest.h:162-171), against the oracle's
+        our, ouc = oracle.OGraph.from_pairs(raw).undirected()
+        assert np.array_equal(ur, our) and np.array_equal(uc, ouc), rec["name"]
         for key, p in rec["peel"].items():
             num, den = map(int, key.split("/"))
             pr = g.batch_peel(num, den, T_full=rec["T"])

@@ -115,6 +115,7 @@ def lib() -> ctypes.CDLL:
             "ga_core_prune": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_tc_count_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_uint32, _vp]),
             "ga_tc_count_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_int, _u64p,
                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),

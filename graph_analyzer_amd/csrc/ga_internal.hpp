@@ -116,6 +116,8 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
 
 // GA_TC_NOSRC / GA_TC_VERTEX count into *total (zeroed by the caller): ga_tc_simple.hip.
 int ga_tc_launch_simple(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* total);
+// Looks up the device address of ga_noalloc.hip's static scratch on ctx's device (once per device).
+int ga_noalloc_init(ga_ctx* ctx);
 
 inline unsigned grid_for(uint64_t threads, unsigned block, unsigned cap = 1u << 20) {
     uint64_t b = (threads + block - 1) / block;

@@ -74,6 +74,10 @@ extern "C" int ga_ctx_create(int device, ga_ctx** out) {
         ga_ctx_destroy(c);
         return GA_ERR_HIP;
     }
+    if (ga_noalloc_init(c) != GA_OK) {   // this device's address of the allocation-free count's static scratch
+        ga_ctx_destroy(c);
+        return GA_ERR_HIP;
+    }
     *out = c;
     return GA_OK;
 }

@@ -130,6 +130,16 @@ int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, float* ms);
 /* Counts `count` resident graphs back to back on ctx's stream with one readback at the end (T[i] for graphs[i]);
  * plans are built first where missing. The serving loop's batched count. No counterpart in the reference. */
 int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, int variant, uint64_t* T, float* ms);
+/* ga_tc_count for GA_TC_NOSRC or GA_TC_VERTEX that allocates nothing. No counterpart in the reference: it is for callers
+ * that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid allocation). The
+ * count is enqueued on ctx's stream and the call returns without waiting; *d_T, a device word the caller owns, holds T
+ * once the stream has reached that point (the caller synchronises). The per-workgroup partial sums and a completion
+ * ticket live in a static device buffer of the library (GA_NOALLOC_MAX_BLOCKS + 1 words per device, held until the
+ * process ends), shared by every call on that device: calls on one device that overlap in time race, so enqueue them
+ * on one stream or serialise them. blocks = workgroups (0 = the library's choice); more than GA_NOALLOC_MAX_BLOCKS is
+ * GA_ERR_ARG. Other variants (they need src[] or a count plan) are GA_ERR_ARG. */
+#define GA_NOALLOC_MAX_BLOCKS 4096u
+int ga_tc_count_noalloc(ga_ctx* ctx, const ga_graph* g, int variant, uint32_t blocks, uint64_t* d_T);
 /* Builds the device-side count plan (hash-side work lists, degree buckets, work order); ms = its device time. */
 int ga_tc_prepare(ga_ctx* ctx, ga_graph* g, float* ms);
 /* Lane-group width / hash capacity / block shape per degree bucket (the autotuner's output, see ga_tc_tune). */

@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
             "ga_graph_download": (ctypes.c_int, [_vp, _vp, _u32p, _u32p, _u32p]),
             "ga_graph_undirected": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
             "ga_graph_edge_order": (ctypes.c_int, [_vp, _vp, _u32p]),
+            "ga_graph_tasks": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint64, _u64p]),
             "ga_graph_from_text": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_load_text": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_float)]),
             "ga_graph_save": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p]),
@@ -116,6 +117,7 @@ def lib() -> ctypes.CDLL:
             "ga_tc_variant_name": (ctypes.c_char_p, [ctypes.c_int]),
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_count_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_uint32, _vp]),
+            "ga_batch_work": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_count_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_int, _u64p,
                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -345,6 +347,17 @@ class DeviceGraph:
         perm = np.zeros(max(self.m, 1), np.uint32)
         self.ctx.check(lib().ga_graph_edge_order(self.ctx.handle, self._h, _ptr(perm, _u32p)), "edge_order")
         return perm[: self.m]
+
+    def build_tasks(self, chunk: int = 128):
+        """build_tasks (graph.h:138-150) on the device: (t_edge, t_start), every edge's min(d+u, d+v) in chunks."""
+        nt = ctypes.c_uint64()
+        self.ctx.check(lib().ga_graph_tasks(self.ctx.handle, self._h, chunk, None, None, 0, ctypes.byref(nt)), "graph_tasks")
+        k = nt.value
+        te, ts = np.zeros(max(k, 1), np.uint32), np.zeros(max(k, 1), np.uint32)
+        if k:
+            self.ctx.check(lib().ga_graph_tasks(self.ctx.handle, self._h, chunk, _ptr(te, _u32p), _ptr(ts, _u32p), k,
+                                                ctypes.byref(nt)), "graph_tasks")
+        return te[:k], ts[:k]
 
     # ---- subgraphs (dsg.h:103-114, 636-645) ----
     def induced(self, keep: np.ndarray):
@@ -782,6 +795,13 @@ class Batch:
         f = lib().ga_batch_count_per_graph if per_graph else lib().ga_batch_count
         self.ctx.check(f(self.ctx.handle, self._h, _ptr(T, _u64p), ctypes.byref(ms)), "batch_count")
         return T, ms.value
+
+    def work(self):
+        """(sum over every graph's oriented edges of min(d+u, d+v), device ms of the last batched count)."""
+        sm = ctypes.c_uint64()
+        ms = ctypes.c_float()
+        self.ctx.check(lib().ga_batch_work(self.ctx.handle, self._h, ctypes.byref(sm), ctypes.byref(ms)), "batch_work")
+        return sm.value, ms.value
 
     def tune(self):
         """Autotunes the union's count; returns (tuning, best ms)."""

@@ -428,3 +428,59 @@ extern "C" int ga_graph_edge_order(ga_ctx* ctx, ga_graph* g, uint32_t* perm) {
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GA_OK;
 }
+
+// ---- build_tasks (graph.h:138-150) on the device: every oriented edge's work min(d+u, d+v) cut into chunks ----
+// Tasks of edge e: ceil(min(d+u, d+v) / chunk), in edge order, each (e, chunk * i).
+__global__ void k_task_counts(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                              uint32_t m, uint32_t chunk, unsigned long long* __restrict__ cnt) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const uint32_t u = src[e], v = col[e];
+        const uint32_t s = min(rowptr[u + 1] - rowptr[u], rowptr[v + 1] - rowptr[v]);
+        cnt[e] = ((uint64_t)s + chunk - 1) / chunk;
+    }
+}
+
+__global__ void k_task_fill(const unsigned long long* __restrict__ off, uint32_t m, uint32_t chunk, uint32_t* __restrict__ t_edge,
+                            uint32_t* __restrict__ t_start) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const unsigned long long b = off[e], k = off[e + 1] - b;
+        for (unsigned long long i = 0; i < k; i++) {
+            t_edge[b + i] = e;
+            t_start[b + i] = (uint32_t)(i * chunk);
+        }
+    }
+}
+
+extern "C" int ga_graph_tasks(ga_ctx* ctx, ga_graph* g, uint32_t chunk, uint32_t* t_edge, uint32_t* t_start, uint64_t capacity,
+                              uint64_t* ntasks) {
+    if (!ctx || !g || !ntasks) return ctx ? ga_fail(ctx, GA_ERR_ARG, "ga_graph_tasks: NULL graph or count") : GA_ERR_ARG;
+    if (chunk == 0) return ga_fail(ctx, GA_ERR_ARG, "ga_graph_tasks: chunk must be at least 1");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    *ntasks = 0;
+    if (g->m == 0) return GA_OK;
+    DevBuf cnt, off;
+    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)g->m + 1) * 8));
+    GA_TRY(ga_malloc(ctx, &off.p, ((size_t)g->m + 1) * 8));
+    k_task_counts<<<grid_for(g->m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, g->m, chunk, (unsigned long long*)cnt.p);
+    GA_TRY(ga_launch_check(ctx, "k_task_counts"));
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)cnt.p + g->m, 1, 0));
+    GA_TRY(ga_exclusive_scan_u64(ctx, (const uint64_t*)cnt.p, (uint64_t*)off.p, (size_t)g->m + 1));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, (uint64_t*)off.p + g->m, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const uint64_t nt = ctx->hstat[0];
+    *ntasks = nt;
+    if (!t_edge && !t_start) return GA_OK;   // size query
+    if (!t_edge || !t_start) return ga_fail(ctx, GA_ERR_ARG, "ga_graph_tasks: t_edge and t_start are both needed");
+    if (capacity < nt) return ga_fail(ctx, GA_ERR_ARG, "ga_graph_tasks: %llu tasks, capacity %llu", (unsigned long long)nt, (unsigned long long)capacity);
+    if (nt == 0) return GA_OK;
+    DevBuf te, ts;
+    GA_TRY(ga_malloc(ctx, &te.p, nt * 4));
+    GA_TRY(ga_malloc(ctx, &ts.p, nt * 4));
+    k_task_fill<<<grid_for(g->m, 256, 8192), 256, 0, ctx->stream>>>((const unsigned long long*)off.p, g->m, chunk, (uint32_t*)te.p,
+                                                                    (uint32_t*)ts.p);
+    GA_TRY(ga_launch_check(ctx, "k_task_fill"));
+    GA_HIP(ctx, hipMemcpyAsync(t_edge, te.p, nt * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(t_start, ts.p, nt * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return GA_OK;
+}

@@ -1,6 +1,7 @@
 // densest.h (graph_analyzer_amd) - drop-in for the reference's densest.h API (densest.h:46-78, 254-303):
-// PeelResult, density_value, peel_arith_ok, peel_improves, densest_row and the backend-templated host loop
-// batch_peel_loop, plus GpuBackend (the gpu_densest.cu:70-126 backend) over the C ABI and a fused gpu_batch_peel.
+// PeelResult, density_value, peel_arith_ok, peel_improves, densest_row, build_undirected_csr (on the device) and the
+// backend-templated host loop batch_peel_loop, plus GpuBackend (the gpu_densest.cu:70-126 backend) over the C ABI and
+// a fused gpu_batch_peel.
 // The CPU algorithms of the reference header (cpu_triangle_participation, cpu_batch_peel, cpu_greedy_peel) are the
 // CPU baseline and are not part of this library.
 #pragma once
@@ -129,4 +130,13 @@ inline PeelResult gpu_batch_peel(const gaa::DeviceGraph& g, uint64_t T_full, uin
     ga_peel_destroy(p);
     if (ms) *ms = t;
     return r;
+}
+
+// build_undirected_csr (densest.h:162-171): both directions of every edge, rows sorted, built on the device.
+inline void build_undirected_csr(const Graph& g, std::vector<uint32_t>& urow, std::vector<uint32_t>& ucol) {
+    ga_ctx* ctx = gaa::default_ctx();
+    gaa::DeviceGraph d = gaa::DeviceGraph::upload(ctx, g);
+    urow.assign((size_t)g.n + 1, 0);
+    ucol.assign(2ull * g.m, 0);
+    gaa::check(ctx, ga_graph_undirected(ctx, d.handle(), urow.data(), g.m ? ucol.data() : nullptr), "ga_graph_undirected");
 }

@@ -309,3 +309,14 @@ inline Dens core_level_density(const gaa::DeviceGraph& g, const std::vector<uint
     gaa::check(g.ctx(), ga_tc_vertex(g.ctx(), g.handle(), GA_TC_AUTO, g.n() ? in.data() : nullptr, tri.data(), &T), "ga_tc_vertex");
     return Dens{T, cnt};
 }
+
+// core_decomposition (dsg.h:116-166): the (k,triangle)-core number of every vertex, computed on the device by the
+// incremental level loop (ga_core_run; core numbers are unique, so they equal Alg 3's). Returns false where the device
+// path cannot hold the counts in u32 (T >= 2^32; the reference fails only when a core number itself exceeds u32).
+inline bool core_decomposition(const Graph& g, std::vector<uint32_t>& core, uint32_t* kmax_out = nullptr) {
+    gaa::DeviceGraph d = gaa::DeviceGraph::upload(gaa::default_ctx(), g);
+    const CoreLoopResult r = gpu_core_run(d, UINT64_MAX, core);
+    if (!r.ok) return false;
+    if (kmax_out) *kmax_out = (uint32_t)r.kmax;
+    return true;
+}

@@ -1,6 +1,7 @@
 // graph.h (graph_analyzer_amd) - drop-in for the reference's graph.h API (graph.h:19-171), backed by the GPU.
 //
-// Same names and argument meaning: struct Graph, now_ms, build_graph_from_edges, load_graph, build_edge_order.
+// Same names and argument meaning: struct Graph, now_ms, build_graph_from_edges, load_graph, build_tasks,
+// build_edge_order.
 // Differences a caller sees:
 //   * build_graph_from_edges / load_graph do de-duplication, (degree, id) orientation and row sorting on the GPU
 //     (ga_graph_from_pairs); load_graph parses the text on the GPU too (ga_graph_load_text). The arrays are
@@ -210,4 +211,16 @@ inline void build_edge_order(const Graph& g, std::vector<uint32_t>& perm) {
     gaa::DeviceGraph d = gaa::DeviceGraph::upload(ctx, g);
     perm.assign(g.m, 0);
     gaa::check(ctx, ga_graph_edge_order(ctx, d.handle(), g.m ? perm.data() : nullptr), "ga_graph_edge_order");
+}
+
+// build_tasks (graph.h:138-150), computed on the device: every oriented edge's work min(d+u, d+v) in chunks of `chunk`
+// elements, task (t_edge[t], t_start[t]) in edge order.
+inline void build_tasks(const Graph& g, uint32_t chunk, std::vector<uint32_t>& t_edge, std::vector<uint32_t>& t_start) {
+    ga_ctx* ctx = gaa::default_ctx();
+    gaa::DeviceGraph d = gaa::DeviceGraph::upload(ctx, g);
+    uint64_t nt = 0;
+    gaa::check(ctx, ga_graph_tasks(ctx, d.handle(), chunk, nullptr, nullptr, 0, &nt), "ga_graph_tasks");
+    t_edge.assign(nt, 0);
+    t_start.assign(nt, 0);
+    if (nt) gaa::check(ctx, ga_graph_tasks(ctx, d.handle(), chunk, t_edge.data(), t_start.data(), nt, &nt), "ga_graph_tasks");
 }

@@ -292,12 +292,15 @@ class Server:
                 while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
                     pending.append(reqs[i])
                     i += 1
+                refused = False   # the last admission attempt found nothing it could start
                 while free and pending:
                     task = self._admit(pending)
                     if task is None:
+                        refused = True
                         break
                     todo[free.pop()].put(task)
-                if pending and len(free) == self.slots and i >= len(reqs) and self._admit(pending) is None:
+                if refused and len(free) == self.slots and i >= len(reqs):
+                    # nothing in flight, nothing left to arrive, and the pending requests cannot start
                     raise GAError(3, f"{len(pending)} requests cannot be admitted: their graphs do not fit the budget")
                 wait = 0.02 if i >= len(reqs) else max(0.0, min(0.02, reqs[i].arrival - (time.perf_counter() - t0)))
                 try:

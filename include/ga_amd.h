@@ -72,6 +72,12 @@ int ga_graph_undirected(ga_ctx* ctx, ga_graph* g, uint32_t* urow, uint32_t* ucol
  * graph.h:156-171), computed on the device; perm has m entries. */
 int ga_graph_edge_order(ga_ctx* ctx, ga_graph* g, uint32_t* perm);
 
+/* build_tasks (graph.h:138-150) on the device: every oriented edge's work s = min(d+(u), d+(v)) cut into chunks of
+ * `chunk` elements, task i of edge e being (t_edge = e, t_start = chunk * i), edges in order. *ntasks gets the number of
+ * tasks; with t_edge and t_start NULL the call only counts, otherwise both need `capacity` >= *ntasks entries
+ * (GA_ERR_ARG if not). chunk = 0 is GA_ERR_ARG. */
+int ga_graph_tasks(ga_ctx* ctx, ga_graph* g, uint32_t chunk, uint32_t* t_edge, uint32_t* t_start, uint64_t capacity, uint64_t* ntasks);
+
 /* load_graph (graph.h:71-96) with the text parse on the GPU as well: `bytes` holds a SNAP edge list (len bytes, host
  * memory). Lines are taken as the reference's fgets(line, 512) loop sees them (physical lines cut into chunks of at most
  * 511 bytes), chunks starting with '#', '%' or '\n' or without two base-10 numbers (strtoul rules: leading white space,
@@ -269,6 +275,11 @@ int ga_batch_graph_info(const ga_batch* b, uint32_t i, uint32_t* n, uint32_t* m,
 int ga_batch_download_graph(ga_ctx* ctx, ga_batch* b, uint32_t i, uint32_t* rowptr, uint32_t* col, uint32_t* src);
 /* Per-graph T (count entries) from one shared launch set; ms = host wall time from launch to result. */
 int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
+/* The batch's algorithmic work and count time, for its roofline: *summin (may be NULL) = sum over every graph's oriented
+ * edges of min(d+(u), d+(v)), computed on the device, so the batch's algorithmic bytes are 4 (total_n + count) +
+ * 4 total_m + 4 *summin; *last_count_device_ms (may be NULL) = device time of the last ga_batch_count's launches (HIP
+ * events on ctx's stream; 0 before the first). No counterpart in the reference (measurement support). */
+int ga_batch_work(ga_ctx* ctx, ga_batch* b, uint64_t* summin, float* last_count_device_ms);
 /* The same counts with one ga_tc_count per graph (the per-graph-launch baseline). */
 int ga_batch_count_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms);
 /* Autotunes the union's count (ga_tc_tune on the resident union); the per-graph baseline inherits the result. */

@@ -57,6 +57,7 @@ def olib():
             "orc_core_fixpoint": (None, [_vp, _u32p]),
             "orc_core_level_density": (None, [_vp, _u32p, ctypes.c_uint32, _u64p, _u64p]),
             "orc_edge_order": (None, [_vp, _u32p]),
+            "orc_build_tasks": (ctypes.c_uint64, [_vp, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint64]),
             "orc_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
             "orc_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
             "orc_truss": (ctypes.c_uint32, [_vp, _u32p]),
@@ -101,6 +102,7 @@ def rlib():
             "ref_core_decomposition": (ctypes.c_int, [_vp, _u32p, _u32p]),
             "ref_core_level_density": (None, [_vp, _u32p, ctypes.c_uint32, _u64p, _u64p]),
             "ref_edge_order": (None, [_vp, _u32p]),
+            "ref_build_tasks": (ctypes.c_uint64, [_vp, ctypes.c_uint32, _u32p, _u32p, ctypes.c_uint64]),
             "ref_undirected": (None, [_vp, _u32p, _u32p]),
             "ref_peel_arith_ok": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
             "ref_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
@@ -249,6 +251,12 @@ class OGraph:
         olib().orc_edge_order(self._h, _p(p, _u32p))
         return p[: self.m]
 
+    def build_tasks(self, chunk=128):
+        k = int(olib().orc_build_tasks(self._h, chunk, None, None, 0))
+        te, ts = np.zeros(max(k, 1), np.uint32), np.zeros(max(k, 1), np.uint32)
+        olib().orc_build_tasks(self._h, chunk, _p(te, _u32p), _p(ts, _u32p), k)
+        return te[:k], ts[:k]
+
     def __del__(self):
         try:
             if self._h:
@@ -343,6 +351,12 @@ class RGraph:
         p = np.zeros(max(self.m, 1), np.uint32)
         rlib().ref_edge_order(self._h, _p(p, _u32p))
         return p[: self.m]
+
+    def build_tasks(self, chunk=128):
+        k = int(rlib().ref_build_tasks(self._h, chunk, None, None, 0))
+        te, ts = np.zeros(max(k, 1), np.uint32), np.zeros(max(k, 1), np.uint32)
+        rlib().ref_build_tasks(self._h, chunk, _p(te, _u32p), _p(ts, _u32p), k)
+        return te[:k], ts[:k]
 
     def undirected(self):
         r = np.zeros(self.n + 1, np.uint32)

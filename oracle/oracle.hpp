@@ -446,6 +446,19 @@ inline void edge_order(const Graph& g, std::vector<uint32_t>& perm) {
     for (uint32_t e = 0; e < g.m; e++) perm[start[bucket(e)]++] = e;
 }
 
+// v4 work split (graph.h:138-150): edge by edge, chunks of `chunk` elements of the smaller out-list, task (e, start).
+inline void build_tasks(const Graph& g, uint32_t chunk, std::vector<uint32_t>& t_edge, std::vector<uint32_t>& t_start) {
+    t_edge.clear();
+    t_start.clear();
+    for (uint32_t e = 0; e < g.m; e++) {
+        const uint32_t s = std::min(outdeg(g, g.src[e]), outdeg(g, g.col[e]));
+        for (uint32_t off = 0; off < s; off += chunk) {
+            t_edge.push_back(e);
+            t_start.push_back(off);
+        }
+    }
+}
+
 // Induced subgraph on keep[] with ids compacted in old-id order, rebuilt by build_graph_from_edges (dsg.h:103-114).
 inline void induced_subgraph(const Graph& g, const std::vector<uint8_t>& keep, Graph& out, std::vector<uint32_t>& old_id) {
     std::vector<uint32_t> nid(g.n, UINT32_MAX);

@@ -140,6 +140,17 @@ void orc_edge_order(const orc_graph* h, uint32_t* perm) {
     std::copy(p.begin(), p.end(), perm);
 }
 
+// build_tasks: returns the task count; writes up to `cap` tasks when the buffers are given.
+uint64_t orc_build_tasks(const orc_graph* h, uint32_t chunk, uint32_t* t_edge, uint32_t* t_start, uint64_t cap) {
+    std::vector<uint32_t> te, ts;
+    build_tasks(h->g, chunk, te, ts);
+    if (t_edge && t_start && cap >= te.size()) {
+        std::copy(te.begin(), te.end(), t_edge);
+        std::copy(ts.begin(), ts.end(), t_start);
+    }
+    return te.size();
+}
+
 // Induced subgraph on keep[]; returns the new graph, old ids written to old_id (capacity n).
 orc_graph* orc_induced_subgraph(const orc_graph* h, const uint8_t* keep, uint32_t* old_id, uint32_t* n_out) {
     orc_graph* o = new orc_graph;

@@ -86,6 +86,16 @@ void ref_edge_order(const ref_graph* h, uint32_t* perm) {
     std::copy(p.begin(), p.end(), perm);
 }
 
+uint64_t ref_build_tasks(const ref_graph* h, uint32_t chunk, uint32_t* t_edge, uint32_t* t_start, uint64_t cap) {
+    std::vector<uint32_t> te, ts;
+    build_tasks(h->g, chunk, te, ts);
+    if (t_edge && t_start && cap >= te.size()) {
+        std::copy(te.begin(), te.end(), t_edge);
+        std::copy(ts.begin(), ts.end(), t_start);
+    }
+    return te.size();
+}
+
 void ref_undirected(const ref_graph* h, uint32_t* urow, uint32_t* ucol) {
     std::vector<uint32_t> r, c;
     build_undirected_csr(h->g, r, c);

@@ -98,3 +98,13 @@ TEXT_CASES = {
     "line of exactly 511 and 512 bytes": b"1 2" + b" " * 507 + b"\n" + b"3 4" + b" " * 508 + b"\n9 8\n",
     "very long lines": (b"7 9 " + b"0" * 1500 + b"11 12 \n") * 3 + b"#" + b" " * 700 + b"\n",
 }
+
+
+# Texts on which this library's loader departs from the reference on purpose, with the result it gives: a pair that
+# equals another only after the cast to uint32 ("1 3" and "4294967297 3", which casts to (3, 1)). The reference sorts
+# and de-duplicates the cast pairs without putting them back in (min, max) order, so it keeps (1, 3) and (3, 1) and
+# hands the builder a repeated edge (m = 3 here, a multigraph); the GPU loader and the oracle canonicalise after the
+# cast and keep one (n, m) = (10, 2).
+DIVERGENT_TEXT_CASES = {
+    "pair equal to another after the cast": (b"1 3\n4294967297 3\n0 9\n", 10, 2),
+}

@@ -112,3 +112,12 @@ def test_gpu_truss_driver(graph_file):
     r = rows(out)
     _, kmax = og.truss()
     assert len(r) == 1 and r[0][0] == "gpu_truss" and (int(r[0][2]), int(r[0][3]), int(r[0][4])) == (og.n, og.m, kmax)
+
+
+@pytest.mark.gpu
+def test_host_api_signatures(graph_file):
+    """build_tasks, build_edge_order, build_undirected_csr and core_decomposition called through the drop-in headers'
+    reference signatures (graph.h:138-171, densest.h:162-171, dsg.h:116-166), each equal to the CPU oracle."""
+    path, og = graph_file
+    rc, out, err = run(["test_host_api", path])
+    assert rc == 0 and "ALL OK" in out, (out[-2000:], err[-2000:])

@@ -158,13 +158,17 @@ def test_greedy_many_equals_one_by_one(ctx):
     raws[5] = np.array([(0, i) for i in range(1, 40)], np.uint32)            # no triangle
     raws[7] = np.zeros((0, 2), np.uint32)                                    # no edge: n = 1 after the build
     raws[9] = np.concatenate([random_graph(rng, 1500, 0.01), np.array([[0, 1499]], np.uint32)])   # CSR form
+    # dense form with a second owned vertex per lane (1,024 < n <= 1,120): its triangles come from the LDS bitmap
+    raws[10] = np.concatenate([random_graph(rng, 1100, 0.1), np.array([[0, 1099]], np.uint32)])
+    raws[15] = np.concatenate([random_graph(rng, 1120, 0.05), np.array([[3, 1119]], np.uint32)])
     graphs = [ga.DeviceGraph.from_pairs(ctx, r) for r in raws]
+    assert graphs[10].n == 1100 and graphs[15].n == 1120
     for rep in range(2):
         st, sets, ms = ctx.greedy_pp_many(graphs, 4)
         for i, (g, r) in enumerate(zip(graphs, raws)):
             t, v, s, _, _ = g.greedy_pp(4)
             assert (int(st[i][0]), int(st[i][1])) == (t, v) and np.array_equal(sets[i], s), (rep, i)
-            if i % 5 == 0:
+            if i % 5 == 0 or i in (10, 15):
                 ot, ov, os_, _ = oracle.OGraph.from_pairs(r).greedy_pp(4)
                 assert (t, v) == (ot, ov) and np.array_equal(s, os_), (rep, i)
         graphs[0].count()

@@ -9,7 +9,7 @@ import pytest
 
 import graph_analyzer_amd as ga
 import oracle
-from conftest import TEXT_CASES, random_graph
+from conftest import DIVERGENT_TEXT_CASES, TEXT_CASES, random_graph
 from graph_analyzer_amd import gen
 
 pytestmark = pytest.mark.gpu
@@ -27,6 +27,18 @@ def test_text_corner_cases(ctx, tmp_path):
         path = tmp_path / "g.txt"
         path.write_bytes(text)
         og = oracle.OGraph.load_text(str(path))
+        same_csr(ga.DeviceGraph.from_text(ctx, text), og, name)
+        same_csr(ga.DeviceGraph.load_text(ctx, str(path)), og, name + " (file)")
+
+
+def test_text_divergent_cases(ctx, tmp_path):
+    """Inputs where the loader departs from the reference on purpose (conftest.DIVERGENT_TEXT_CASES): the GPU gives
+    the oracle's documented result."""
+    for name, (text, n, m) in DIVERGENT_TEXT_CASES.items():
+        path = tmp_path / "g.txt"
+        path.write_bytes(text)
+        og = oracle.OGraph.load_text(str(path))
+        assert (og.n, og.m) == (n, m), name
         same_csr(ga.DeviceGraph.from_text(ctx, text), og, name)
         same_csr(ga.DeviceGraph.load_text(ctx, str(path)), og, name + " (file)")
 

@@ -75,9 +75,13 @@ def test_golden_fixtures(ctx):
         assert g.vertex_participation()[1].tolist() == rec["tri"]
         assert g.edge_support()[1].tolist() == rec["sup"]
         assert g.edge_order().tolist() == rec["edge_order"]
+        og = oracle.OGraph.from_pairs(raw)
         ur, uc = g.undirected()   # build_undirected_csr (densest.h:162-171), against the oracle's
-        our, ouc = oracle.OGraph.from_pairs(raw).undirected()
+        our, ouc = og.undirected()
         assert np.array_equal(ur, our) and np.array_equal(uc, ouc), rec["name"]
+        for chunk in (1, 5, 128):   # build_tasks (graph.h:138-150)
+            for a, b in zip(g.build_tasks(chunk), og.build_tasks(chunk)):
+                assert np.array_equal(a, b), (rec["name"], chunk)
         for key, p in rec["peel"].items():
             num, den = map(int, key.split("/"))
             pr = g.batch_peel(num, den, T_full=rec["T"])

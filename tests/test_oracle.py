@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import TEXT_CASES, clique, random_graph, zoo
+from conftest import DIVERGENT_TEXT_CASES, TEXT_CASES, clique, random_graph, zoo
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
 
@@ -74,6 +74,19 @@ def test_text_loader_corner_cases_match_reference(tmp_path):
             assert np.array_equal(a, b), name
 
 
+def test_loader_divergence_documented(tmp_path):
+    """The one loader input class where this library departs from the reference on purpose: a pair that repeats
+    another only after the uint32 cast is kept once here and twice by the reference."""
+    for name, (text, n, m) in DIVERGENT_TEXT_CASES.items():
+        path = tmp_path / "g.txt"
+        path.write_bytes(text)
+        o = oracle.OGraph.load_text(str(path))
+        assert (o.n, o.m) == (n, m), name
+        if oracle.ref_available():
+            r = oracle.RGraph.load_text(str(path))
+            assert r.n == n and r.m == m + 1, name   # the reference's repeated edge
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_oracle_equals_reference_build(seed):
     """Fresh seeded graphs (not in the fixtures): every oracle function equals the reference build."""
@@ -103,6 +116,9 @@ def test_oracle_equals_reference_build(seed):
     assert np.array_equal(o.edge_order(), r.edge_order())
     for a, b in zip(o.undirected(), r.undirected()):
         assert np.array_equal(a, b)
+    for chunk in (1, 3, 128):   # build_tasks (graph.h:138-150)
+        for a, b in zip(o.build_tasks(chunk), r.build_tasks(chunk)):
+            assert np.array_equal(a, b), chunk
 
 
 def test_golden_prune(golden):

@@ -1,6 +1,7 @@
 """Benchmark: exact triangle counting on one MI355X (oriented edges/s and the HBM roofline of the count), plus the
-4,096-graph batch (graphs/s, one launch set for all graphs vs one count per graph), with the reference's CPU counter
-timed on this machine's host cores.
+4,096-graph batch (graphs/s, one launch set for all graphs vs one count per graph), with the CPU oracle's restatement of
+the reference's CPU code (count, batch peel, Alg 3 core) timed on this machine's host cores (model, nproc and OpenMP
+threads printed).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw_communities|rmat21|powerlaw_small]
                     [--batch-graphs 4096] [--serve-seconds 1.0] [--serve-rate 2000] [--no-cpu] [--no-tune]
@@ -20,12 +21,17 @@ Batch: the 4,096-graph mix (n log-uniform in [1k, 200k], mean degree uniform in 
 Chung-Lu) resident as one disjoint union; graphs/s = graphs / wall time of one per-graph count of every graph, batched
 (one launch set) and one count per graph (same kernels, each graph its own launches and readback; the baseline is the
 faster of the library's default tuning and the batch's tuning). Both must give identical per-graph counts; a sample of
-graphs is also checked against the CPU oracle. Peel: batch peeling at eps = 1/10, all graphs' rounds in shared launches
-vs one fused peel per graph.
+graphs is also checked against the CPU oracle. The batch roofline: B_alg summed over the graphs over the device time of
+the batched launch set. Peel: batch peeling at eps = 1/10, all graphs' rounds in shared launches vs one fused peel per
+graph.
+
+CPU rows (same workload): the count serial and with OpenMP (the cpu_baseline object), batch peeling at eps = 1/10 with
+OpenMP and serially, and serial Alg 3, each beside the GPU time and each checked against the GPU result; the serial
+peel and core run in a background thread while the GPU legs run.
 
 Extras (same workload): load_graph with the text parse on the GPU and the binary CSR file, and prune_to_core at
-ceil(kmax-core density) on the device core numbers, each checked against the oracle and timed beside the reference's
-own CPU code on one host core.
+ceil(kmax-core density) on the device core numbers, each checked against the oracle and timed beside the oracle's CPU
+restatement on one host core.
 
 Serving: Poisson arrivals (2,000 req/s for 1 s by default) of count / peel (eps 1/20) / core requests, 95% on a pool
 of 256 graphs of the batch size law and 5% on two R-MAT scale-20 graphs, all resident; p50 / p99 latency (arrival to
@@ -81,38 +87,97 @@ def load_traffic(workload):
     return d.get("hbm_bytes_per_count")
 
 
-def cpu_baseline(pairs, T_gpu):
-    """The reference's cpu_triangles_serial (compiled from its sources into oracle/_ref) when present, else the oracle
-    restatement; run on the whole graph, serially (1 thread), and with OpenMP on all host cores. Also checks the GPU
-    count."""
+def host_info():
+    """CPU model, logical CPUs of this process and OpenMP's thread count (the CPU baseline's cores)."""
     import oracle
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable, "os_cpu_count": os.cpu_count(), "omp_max_threads": int(oracle.olib().orc_omp_max_threads())}
+
+
+def cpu_baseline(og, T_gpu):
+    """The CPU oracle's restatement of the reference's counters (cpu_triangles_serial / cpu_triangles_omp,
+    graph.h:110-136: one thread, then OpenMP schedule(dynamic,64) on every host core), on the whole graph; each run must
+    reproduce the GPU count. Best of 3 for the OpenMP run (cpu_tc.cpp:15-19); the serial one runs once (8-10 s)."""
+    import oracle
+    threads = int(oracle.olib().orc_omp_max_threads())
+    t_omp = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        T_omp = og.triangles_omp(0)
+        t = time.perf_counter() - t0
+        t_omp = t if t_omp is None else min(t_omp, t)
+        if T_omp != T_gpu:
+            raise SystemExit(f"MISMATCH: GPU count {T_gpu} != CPU count {T_omp}")
     t0 = time.perf_counter()
-    if oracle.ref_available():
-        uniq = oracle.canonical_unique(pairs)
-        n = int(uniq.max()) + 1 if uniq.size else 1
-        g = oracle.RGraph.build(n, uniq)
-        kind = "reference"
-    else:
-        g = oracle.OGraph.from_pairs(pairs)
-        kind = "port"
-    log(f"cpu graph built in {time.perf_counter() - t0:.1f} s ({kind})")
-    t0 = time.perf_counter()
-    T_omp = g.triangles_omp(0)
-    t_omp = time.perf_counter() - t0
-    threads = oracle.olib().orc_omp_max_threads()
-    if T_omp != T_gpu:
-        raise SystemExit(f"MISMATCH: GPU count {T_gpu} != CPU count {T_omp}")
-    t0 = time.perf_counter()
-    T_ser = g.triangles()
+    T_ser = og.triangles()
     t_ser = time.perf_counter() - t0
     if T_ser != T_omp:
         raise SystemExit(f"MISMATCH: serial CPU count {T_ser} != OpenMP count {T_omp}")
-    return {"kind": kind, "t_serial_s": t_ser, "t_omp_s": t_omp, "omp_threads": threads, "m": g.m}
+    return {"kind": "port", "t_serial_s": t_ser, "t_omp_s": t_omp, "omp_threads": threads, "m": og.m}
+
+
+def cpu_peel_core_rows(og, gpu_peel, start_serial):
+    """CPU rows beside the GPU peel (eps = 1/10) and core on the same graph (densest.h:138-160, dsg.h:125-166, as
+    cpu_densest.cpp:27-38 runs them, single runs): batch peeling with OpenMP on every host core now, then, in a
+    background thread (ctypes releases the GIL), serial batch peeling and serial Alg 3 while the GPU legs run; each
+    result must equal the GPU's. Returns (rows, join) where join() waits for the serial rows and adds them."""
+    import threading
+    rows = {}
+    t0 = time.perf_counter()
+    r = og.batch_peel(1, 10, omp=True)
+    rows["peel_omp_s"] = time.perf_counter() - t0
+    if r[:4] != (gpu_peel.best_num, gpu_peel.best_den, gpu_peel.best_round, gpu_peel.rounds) or not np.array_equal(r[4], gpu_peel.removed_round):
+        raise SystemExit("MISMATCH: GPU batch peel differs from the CPU's")
+    err = []
+
+    def serial():
+        try:
+            t1 = time.perf_counter()
+            rs = og.batch_peel(1, 10, omp=False)
+            rows["peel_serial_s"] = time.perf_counter() - t1
+            if rs[:4] != r[:4]:
+                err.append("serial and OpenMP CPU peels differ")
+            t1 = time.perf_counter()
+            core, kmax = og.core()
+            rows["core_serial_s"] = time.perf_counter() - t1
+            rows["core_kmax"] = int(kmax)
+            rows["_core"] = core
+        except Exception as e:   # reported by join()
+            err.append(repr(e))
+
+    th = threading.Thread(target=serial, daemon=True)
+    if start_serial:
+        th.start()
+
+    def join(gpu_core):
+        if not start_serial:
+            return rows
+        th.join()
+        if err:
+            raise SystemExit("MISMATCH: " + "; ".join(err))
+        core = rows.pop("_core")
+        if gpu_core is not None and not np.array_equal(core, gpu_core):
+            raise SystemExit("MISMATCH: GPU core numbers differ from CPU Alg 3")
+        return rows
+
+    return rows, join
 
 
 def run_extras(ctx, g, pairs, cfg, csr, T0):
     """The rows next to the count on the same workload, each checked against the CPU oracle and timed beside the
-    reference's own CPU code (oracle/_ref; the oracle restatement when that is absent), 1 thread:
+    oracle's restatement of the reference's CPU code, 1 thread:
     GPU text ingest (load_graph, graph.h:69-96) and the binary CSR file; GPU prune_to_core (dsg.h:636-645) at
     ceil(kmax-core density) on the device core numbers; Greedy++ (dsg.h:189-248) with 10 passes on the pruned graph;
     the k-truss decomposition (GPU time, checked against the per-edge supports)."""
@@ -121,8 +186,7 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
     import graph_analyzer_amd as ga
     import oracle
     from graph_analyzer_amd import gen
-    ref = oracle.ref_available()
-    out = {"cpu_kind": "reference" if ref else "port", "cpu_cores": 1}
+    out = {"cpu_kind": "port", "cpu_cores": 1}
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "g.txt")
         gen.write_text(pairs, path, cfg["workload"])
@@ -138,7 +202,7 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
             if len(walls) < 3:
                 t.close()
         t0 = time.perf_counter()
-        rg = oracle.RGraph.load_text(path) if ref else oracle.OGraph.load_text(path)
+        rg = oracle.OGraph.load_text(path)
         t_cpu = time.perf_counter() - t0
         if not all(np.array_equal(a, b) for a, b in zip(rg.arrays(), csr)):
             raise SystemExit("MISMATCH: CPU load_graph differs from the oracle's CSR")
@@ -172,7 +236,7 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
         pms.append(sub.build_ms)
         if len(pms) < 3:
             sub.close()
-    cg = oracle.RGraph.build(g.n, np.stack([csr[2], csr[1]], 1)) if ref else oracle.OGraph.from_csr(csr[0], csr[1])
+    cg = oracle.OGraph.from_csr(csr[0], csr[1])
     t0 = time.perf_counter()
     osub, oold, ocout = cg.prune_to_core(core, k)
     t_cpu = time.perf_counter() - t0
@@ -210,7 +274,7 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
         raise SystemExit("MISMATCH: a truss number outside [2, support + 2]")
     out["truss"] = {"gpu_ms": min(tms), "rounds": rounds, "kmax": kmax_t, "edges_per_s": g.m / (min(tms) * 1e-3)}
     log(f"truss: {rounds} rounds, kmax {kmax_t}, GPU {min(tms):.1f} ms")
-    return out
+    return out, core
 
 
 def run_batch(ctx, count, steps, tune):
@@ -234,12 +298,15 @@ def run_batch(ctx, count, steps, tune):
         tune_info = t.as_dict()
         log(f"batch: tuned {tune_info} ({tms:.2f} ms)")
     T0, _ = b.count_all()
-    times_b = []
+    times_b, dev_b = [], []
     for _ in range(steps):
         T, ms = b.count_all()
         if not np.array_equal(T, T0):
             raise SystemExit("MISMATCH: batched per-graph counts changed between steps")
         times_b.append(ms)
+        dev_b.append(b.work()[1])
+    summin, _ = b.work()
+    B_alg = 4 * (int(b.total_n) + count) + 4 * int(b.total_m) + 4 * int(summin)
     # Per-graph baseline: the fastest of the library's default tuning, the batch's tuning (inherited by every graph),
     # and each of those with the flat kernel in small chunks (one launch for all light rows of a small graph, the
     # per-graph form's best shapes on this mix)
@@ -285,6 +352,11 @@ def run_batch(ctx, count, steps, tune):
            "per_graph_count_graphs_per_s": count / (t_p * 1e-3), "per_graph_count_ms": t_p, "count_speedup": t_p / t_b,
            "per_graph_count_ms_by_tuning": per_by, "per_graph_count_baseline": min(per_by, key=per_by.get),
            "count_edges_per_s": b.total_m / (t_b * 1e-3),
+           "roofline": {"bound": "hbm", "algorithmic_bytes": B_alg, "sum_min_outdeg": int(summin),
+                        "count_device_ms_mean": float(np.mean(dev_b)),
+                        "achieved": B_alg / (float(np.mean(dev_b)) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": B_alg / (float(np.mean(dev_b)) * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "note": "B_alg summed over the 4,096 graphs; device time of the batched launch set (HIP events)"},
            "peel_eps": "1/10", "peel_graphs_per_s": count / (pms * 1e-3), "peel_ms": pms,
            "per_graph_peel_graphs_per_s": count / (pms2 * 1e-3), "per_graph_peel_ms": pms2, "peel_speedup": pms2 / pms,
            "oracle_checked_graphs": sample}
@@ -403,30 +475,57 @@ def main():
         cms = ctx.copy_probe(args.copy_probe << 20)
         log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written in {cms:.3f} ms")
 
+    # GPU batch peeling at eps = 1/10 on the same graph (device time of the fused round loop), for the CPU rows
+    pms = []
+    for _ in range(3):
+        p = ga.Peel(g)
+        pr = p.run(1, 10, T_full=T0)
+        pms.append(p.last_ms)
+        p.close()
+    gpu_peel = {"eps": "1/10", "gpu_ms": min(pms), "rounds": pr.rounds, "best": [pr.best_num, pr.best_den]}
+    log(f"peel eps 1/10: {pr.rounds} rounds, best {pr.best_num}/{pr.best_den}, GPU {min(pms):.2f} ms")
     cpu = None
+    join_serial = None
     if not args.no_cpu:
-        cb = cpu_baseline(pairs, T0)
+        hi = host_info()
+        cb = cpu_baseline(og, T0)
+        rows, join_serial = cpu_peel_core_rows(og, pr, start_serial=True)
         cpu = {"value": cb["m"] / cb["t_serial_s"], "unit": "oriented edges/s", "cores": 1, "kind": cb["kind"],
-               "sample": f"whole graph ({cb['m']} oriented edges), cpu_triangles_serial, 1 thread, single run "
-                         f"({cb['t_serial_s']:.2f} s); OpenMP on {cb['omp_threads']} threads: "
-                         f"{cb['m'] / cb['t_omp_s']:.4g} edges/s ({cb['t_omp_s']:.2f} s)",
-               "omp_value": cb["m"] / cb["t_omp_s"], "omp_cores": cb["omp_threads"]}
+               "sample": f"whole graph ({cb['m']} oriented edges): the oracle's restatement of cpu_triangles_serial, 1 thread, "
+                         f"single run ({cb['t_serial_s']:.2f} s); OpenMP schedule(dynamic,64) on {cb['omp_threads']} threads, "
+                         f"best of 3: {cb['m'] / cb['t_omp_s']:.4g} edges/s ({cb['t_omp_s']:.2f} s)",
+               "omp_value": cb["m"] / cb["t_omp_s"], "omp_cores": cb["omp_threads"], "host": hi,
+               "peel_eps_1_10": {"omp_s": rows["peel_omp_s"], "omp_cores": cb["omp_threads"], "gpu_ms": gpu_peel["gpu_ms"]}}
+        log(f"cpu: {hi}; count serial {cb['t_serial_s']:.2f} s, OpenMP {cb['t_omp_s']:.2f} s; peel OpenMP {rows['peel_omp_s']:.2f} s")
     else:
         ref_T = og.triangles_omp(0)
         if ref_T != T0:
             raise SystemExit(f"MISMATCH: GPU count {T0} != CPU count {ref_T}")
-    del og
     traffic = load_traffic(cfg["workload"])
     n, m, max_out = g.n, g.m, g.max_out_degree
     extras = None
+    gpu_core = None
     if not args.no_extras:
-        extras = run_extras(ctx, g, pairs, cfg, (rowptr, col, src), T0)
+        extras, gpu_core = run_extras(ctx, g, pairs, cfg, (rowptr, col, src), T0)
     batch = None
     if args.batch_graphs:
         g.close()
         batch = run_batch(ctx, args.batch_graphs, max(3, min(args.steps, 10)), not args.no_tune)
         log(f"batch: {batch['count_graphs_per_s']:.0f} graphs/s batched vs {batch['per_graph_count_graphs_per_s']:.0f} per graph "
             f"({batch['count_speedup']:.2f}x); peel {batch['peel_speedup']:.2f}x")
+    if join_serial is not None:
+        t0 = time.perf_counter()
+        rows = join_serial(gpu_core)
+        del og
+        cpu["peel_eps_1_10"].update({"serial_s": rows["peel_serial_s"], "cores_serial": 1})
+        cpu["core"] = {"serial_s": rows["core_serial_s"], "kmax": rows["core_kmax"], "cores": 1,
+                       "gpu_ms": extras["prune"]["core_ms"] if extras else None,
+                       "note": "serial Alg 3 (dsg.h:125-166); no OpenMP form exists in the reference"}
+        cpu["serial_rows_note"] = "serial peel and core ran in a background thread while the GPU legs ran"
+        log(f"cpu: peel serial {rows['peel_serial_s']:.1f} s, core serial {rows['core_serial_s']:.1f} s "
+            f"(waited {time.perf_counter() - t0:.1f} s for them)")
+    else:
+        del og
     serving = None
     if args.serve_seconds > 0:
         serving = run_serving(args.serve_rate, args.serve_seconds)
@@ -449,6 +548,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes": B_alg, "sum_min_outdeg": summin, "kernel_ms_mean": mean_dev},
         "cpu_baseline": cpu,
+        "peel": gpu_peel,
         "extras": extras,
         "batch": batch,
         "serving": serving,

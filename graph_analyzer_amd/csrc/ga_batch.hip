@@ -38,6 +38,7 @@ struct ga_batch {
     void* pstate = nullptr;
     BatchPeelDev pd{};
     int graph_rounds = 8;             // peel rounds captured into one HIP graph (0: enqueue every round directly)
+    float count_dev_ms = 0;           // device time of the last ga_batch_count's launches
 };
 
 namespace {
@@ -426,10 +427,45 @@ extern "C" int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms) 
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double t0 = wall_ms();
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)gc, (size_t)b->count + 1, 0));
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_tc_launch(ctx, b->u, GA_TC_HASH, gc + b->count, nullptr, nullptr, nullptr, nullptr, b->gof, gc));
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     GA_HIP(ctx, hipMemcpyAsync(T, gc, (size_t)b->count * 8, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ms) *ms = (float)(wall_ms() - t0);
+    GA_HIP(ctx, hipEventElapsedTime(&b->count_dev_ms, ctx->ev0, ctx->ev1));
+    return GA_OK;
+}
+
+namespace {
+// Sum over oriented edges of min(d+(u), d+(v)): the probe elements of an exact count (one pass over src / col).
+__global__ void k_sum_min_outdeg(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                                 uint64_t m, unsigned long long* __restrict__ out) {
+    uint64_t local = 0;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t u = src[e], v = col[e];
+        local += min(rowptr[u + 1] - rowptr[u], rowptr[v + 1] - rowptr[v]);
+    }
+    local = wave_sum_u64(local);
+    if (lane_id() == 0 && local) atomicAdd(out, (unsigned long long)local);
+}
+}  // namespace
+
+extern "C" int ga_batch_work(ga_ctx* ctx, ga_batch* b, uint64_t* summin, float* last_count_device_ms) {
+    if (!ctx || !b) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    if (summin) {
+        uint64_t* d = ctx->dsmall;
+        GA_TRY(ga_fill_u64(ctx, d, 1, 0));
+        if (b->u->m)
+            k_sum_min_outdeg<<<grid_for(b->u->m, 256, 16384), 256, 0, ctx->stream>>>(b->u->rowptr, b->u->src, b->u->col, b->u->m,
+                                                                                    (unsigned long long*)d);
+        GA_TRY(ga_launch_check(ctx, "k_sum_min_outdeg"));
+        GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        *summin = ctx->hstat[0];
+    }
+    if (last_count_device_ms) *last_count_device_ms = b->count_dev_ms;
     return GA_OK;
 }
 

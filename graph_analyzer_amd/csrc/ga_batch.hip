@@ -364,7 +364,7 @@ extern "C" int ga_batch_create(ga_ctx* ctx, uint32_t count, const uint64_t* pair
     b->u->plan.group = b->gof;   // shape-ordered items stay graph-major (cache locality per graph)
     b->u->plan.group_bits = 1;
     while (b->u->plan.group_bits < 31 && (1u << b->u->plan.group_bits) < count) b->u->plan.group_bits++;
-    if ((rc = ga_tc_ensure_plan(ctx, b->u))) return fail_b(rc);
+    if ((rc = ga_tc_ensure_plan(ctx, b->u, true))) return fail_b(rc);
     if (build_ms) *build_ms = (float)(wall_ms() - t0);
     *out = b;
     return GA_OK;
@@ -496,7 +496,7 @@ extern "C" int ga_batch_count_per_graph(ga_ctx* ctx, ga_batch* b, uint64_t* T, f
     if (!ctx || !b || !T) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
     GA_TRY(ensure_slices(ctx, b));
-    for (uint32_t i = 0; i < b->count; i++) GA_TRY(ga_tc_ensure_plan(ctx, b->slices[i]));
+    for (uint32_t i = 0; i < b->count; i++) GA_TRY(ga_tc_ensure_plan(ctx, b->slices[i], true));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double t0 = wall_ms();
     for (uint32_t i = 0; i < b->count; i++) {

@@ -331,7 +331,11 @@ extern "C" void ga_graph_destroy(ga_graph* g) {
     ga_free(g->src);
     ga_free(g->urow);
     ga_free(g->ucol);
+    ga_free(g->crow);
+    ga_free(g->ccol);
+    ga_free(g->csrc);
     free_plan(g->plan);
+    free_plan(g->cplan);
     delete g;
 }
 

@@ -60,7 +60,12 @@ struct ga_graph {
     uint32_t n = 0, m = 0, max_out = 0;
     uint32_t *rowptr = nullptr, *col = nullptr, *src = nullptr;
     uint32_t *urow = nullptr, *ucol = nullptr;   // undirected CSR, built on demand
-    TcPlan plan;
+    TcPlan plan;                                  // count plan on the CSR's own ids (every flagged launch; counts when
+                                                  //   tune.relabel is 0)
+    // tune.relabel = 1: plain and per-graph counts use a renumbered copy of the oriented CSR (same orientation, new
+    // ids grouping vertices that share triangles) and its own plan; built on demand
+    uint32_t *crow = nullptr, *ccol = nullptr, *csrc = nullptr;
+    TcPlan cplan;
 };
 
 int ga_fail(ga_ctx* ctx, int code, const char* fmt, ...);
@@ -104,7 +109,9 @@ int ga_graph_ensure_undirected(ga_ctx* ctx, ga_graph* g);
 int ga_prune_core_dev(ga_ctx* ctx, const ga_graph* g, const uint32_t* core_dev, uint64_t k, ga_graph** out, uint32_t* old_id,
                       uint32_t* core_out, float* ms);
 int ga_graph_ensure_perm(ga_ctx* ctx, ga_graph* g);
-int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g);
+// Builds the count plan a launch uses: counts_only = the plan of plain and per-graph counts (on the renumbered copy
+// when tune.relabel is set), else the plan on the CSR's own ids (participation, support, alive masks).
+int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g, bool counts_only = false);
 // ga_tc_tune over the count form a caller runs: plain (gcnt NULL) or per graph of a disjoint union (gof, gcnt of
 // ngraphs + 1 device words), as ga_batch_count launches it.
 int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, ga_tc_tuning* best,

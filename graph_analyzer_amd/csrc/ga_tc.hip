@@ -202,7 +202,32 @@ struct TcArgs {
     const uint32_t* done;
     const uint32_t* gof;
     unsigned long long* gcnt;
+    uint32_t xcd;          // items in contiguous chunks per workgroup, each XCD (blockIdx mod 8) one eighth of them
 };
+
+// Items of this workgroup: with a.xcd, the workgroups the hardware places on XCD x (b mod 8 = x) share the x-th
+// contiguous eighth of the bucket's items, grid-strided among them, so each XCD's L2 serves one eighth; else the plain
+// grid stride over all items.
+struct ItemRange {
+    uint64_t first, end, step;
+};
+__device__ __forceinline__ ItemRange item_range(const TcArgs& a, uint32_t step, uint32_t offset) {
+    ItemRange r;
+    if (a.xcd && gridDim.x >= 8u) {   // (a grid of fewer than 8 workgroups would leave eighths without one)
+        // XCD x (workgroups b = x, x + 8, ...) takes the x-th eighth of the items, grid-strided over its workgroups
+        const uint32_t x = blockIdx.x % 8u, j = blockIdx.x / 8u;
+        const uint32_t nx = (gridDim.x - x + 7u) / 8u;   // workgroups on XCD x
+        const uint64_t lo = (uint64_t)a.nitems * x / 8u, hi = (uint64_t)a.nitems * (x + 1u) / 8u;
+        r.first = lo + (uint64_t)j * step + offset;
+        r.end = hi;
+        r.step = (uint64_t)nx * step;
+    } else {
+        r.first = (uint64_t)blockIdx.x * step + offset;
+        r.end = a.nitems;
+        r.step = (uint64_t)gridDim.x * step;
+    }
+    return r;
+}
 
 template <int G, bool HASHED, int FLAGS>
 __global__ void k_tc_hash(const TcArgs a) {
@@ -220,7 +245,8 @@ __global__ void k_tc_hash(const TcArgs a) {
     uint32_t* tix = tab + a.ts_max;          // F_SUP: position of each key in N+(h); F_TRI: hits per key
     uint32_t* stg = tab + tab_words;         // batch staging: ps[G] | len[G] | p[G] | work index[G]
     uint64_t local = 0;
-    for (uint32_t it = blockIdx.x * gpb + gib; it < a.nitems; it += gridDim.x * gpb) {
+    const ItemRange ir = item_range(a, gpb, gib);
+    for (uint64_t it = ir.first; it < ir.end; it += ir.step) {
         const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
         const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK);
         if ((FLAGS & F_ALIVE) && !alive[h]) continue;
@@ -236,12 +262,13 @@ __global__ void k_tc_hash(const TcArgs a) {
             }
         }
         uint32_t tbits = 0, tsz = 1;
+        bool spill = true;   // false: every key of N+(h) is in its home bucket, so a lookup reads that bucket only
         if (HASHED) {
             // ceil(log2(dh)) + shift, at least 2, at most the group's reserved slots: the reservation holds the bucket's
             // largest row at load shift 2 (table at most a quarter full), so the largest rows of a bucket get shift 2
             // and the smaller ones the tuned shift
             tbits = (dh > 1 ? 32u - (uint32_t)__clz(dh - 1) : 0u) + a.load_shift;
-            if (tbits < 2) tbits = 2;
+            if (tbits < 3) tbits = 3;   // at least two buckets (key_bucket's shift stays below 32; ts_max >= 8 slots)
             const uint32_t tcap = 31u - (uint32_t)__clz(a.ts_max);
             if (tbits > tcap) tbits = tcap;
             tsz = 1u << tbits;
@@ -250,12 +277,15 @@ __global__ void k_tc_hash(const TcArgs a) {
                 if (FLAGS & F_TRI) ((uint4*)tix)[w] = make_uint4(0, 0, 0, 0);
             }
             group_sync();
+            uint32_t moved = 0;   // a key that sits past its home bucket (then lookups must walk on)
             for (uint32_t k = sub; k < dh; k += G) {
                 const uint32_t x = col[hs + k];
                 if ((FLAGS & F_ALIVE) && !alive[x]) continue;
                 const uint32_t s = bucket_insert(tab, tbits - 2u, x);
                 if (FLAGS & F_SUP) tix[s] = k;
+                moved |= (s >> 2) != key_bucket(x, tbits - 2u) ? 1u : 0u;
             }
+            spill = group_bits<G>(__ballot(moved != 0)) != 0;
             group_sync();
         }
         uint32_t hcount = 0;
@@ -300,12 +330,31 @@ __global__ void k_tc_hash(const TcArgs a) {
                     if (k < end) {
                         const uint32_t kps = stg[k], klen = stg[G + k];
                         for (uint32_t off = r; off < klen; off += U * W) {
-                            // every slot loads (a slot past the list reloads element `off`, same line, and is masked):
-                            // no branch per load or per probe
+                            // every slot loads (a slot past the list reloads its last element, same line, and is
+                            // masked): no branch per load or per probe; all loads issue before the first use
                             uint32_t x[UMAX];
+                            const uint32_t last = klen - 1u - off;   // slot u is valid while u * W <= last
+                            const uint32_t* __restrict__ lp = col + kps + off;
 #pragma unroll
                             for (uint32_t u = 0; u < UMAX; u++)
-                                if (u < U) x[u] = col[kps + (off + u * W < klen ? off + u * W : off)];
+                                if (u < U) x[u] = lp[min(u * W, last)];
+                            __builtin_amdgcn_sched_barrier(0);
+                            if (HASHED && !(FLAGS & (F_TRI | F_SUP)) && !spill) {
+                                // no key left its home bucket: one bucket read and four compares per element
+                                const uint32_t bb = tbits - 2u;
+                                uint4 tb[UMAX];
+#pragma unroll
+                                for (uint32_t u = 0; u < UMAX; u++)
+                                    if (u < U) tb[u] = ((const uint4*)tab)[(x[u] * 2654435769u) >> (32u - bb)];
+#pragma unroll
+                                for (uint32_t u = 0; u < UMAX; u++)
+                                    if (u < U) {
+                                        const uint32_t xv = x[u];
+                                        const bool hit = (tb[u].x == xv) | (tb[u].y == xv) | (tb[u].z == xv) | (tb[u].w == xv);
+                                        c += (hit && u * W <= last) ? 1u : 0u;
+                                    }
+                                continue;
+                            }
                             if (HASHED && !(FLAGS & (F_TRI | F_SUP))) {
                                 // plain counts need only membership: every bucket read issued first, then vector
                                 // tests (x present: min of the four xors is 0; bucket full: max of its slots is not
@@ -413,10 +462,11 @@ __global__ void k_tc_tiny(const TcArgs a) {
     const uint32_t* __restrict__ col = a.col;
     const uint32_t lane = lane_id();
     uint64_t local = 0;
-    for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < a.nitems; base += gridDim.x * blockDim.x) {
-        const uint32_t it = base + lane;
+    const ItemRange ir = item_range(a, blockDim.x, threadIdx.x & ~63u);
+    for (uint64_t base = ir.first; base < ir.end; base += ir.step) {   // wavefront-uniform
+        const uint64_t it = base + lane;
         uint32_t c = 0, gr = 0;
-        if (it < a.nitems) {
+        if (it < ir.end) {
             const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
             const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK), dh = im.w >> ITEM_DSHIFT;   // 1..8
             uint32_t nps = a.wps[b], nlen = a.wlen[b];   // every item has an entry; every list has 1..d+(h) elements
@@ -886,20 +936,21 @@ __global__ void k_plan_items(const uint32_t* __restrict__ flag, const uint32_t* 
     }
 }
 
-// Shape key of each item: bucket (kept major) | locality group of h (a union's graph id; none for one graph) |
+// Shape key of each item: bucket (kept major) | locality group of h (a union's graph id, or h's window of 2^wshift ids
+// for the windowed order; none otherwise) |
 // entries | longest probe list (the item's first) | d+(h), the last three capped at 10 bits. Consecutive items go to
 // the lane groups of one wavefront; equal shapes give those groups loops of equal trip counts (groups of one
 // wavefront otherwise wait, masked off, for the longest of their loops). Group-major order keeps the items in flight
 // on a few graphs, whose lists then stay in cache.
 __global__ void k_plan_item_shape(const uint32_t* __restrict__ items, uint32_t nall, const uint32_t* __restrict__ rowptr,
                                   const uint32_t* __restrict__ wlen, uint32_t cap, const uint32_t* __restrict__ group, uint32_t gbits,
-                                  unsigned long long* __restrict__ key, uint32_t* __restrict__ val) {
+                                  uint32_t wshift, unsigned long long* __restrict__ key, uint32_t* __restrict__ val) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) {
         const uint4 im = ((const uint4*)items)[i];
         const uint32_t h = im.x, b = im.y, e = im.y + (im.w & ITEM_NMASK);
         const uint32_t dh = rowptr[h + 1] - rowptr[h];
         const unsigned long long bk = bucket_of(dh, cap);
-        const unsigned long long gr = group ? group[h] : 0u;
+        const unsigned long long gr = group ? group[h] : wshift ? (h >> wshift) : 0u;   // locality group, or h's id window
         key[i] = (bk << (30 + gbits)) | (gr << 30) | ((unsigned long long)min(e - b, 1023u) << 20) |
                  ((unsigned long long)min(wlen[b], 1023u) << 10) | min(dh, 1023u);
         val[i] = i;
@@ -938,7 +989,15 @@ ga_tc_tuning default_tuning() {
     t.flat_chunk = 0;     // the flat kernel is a tuner option (it wins on some degree mixes, not on all)
     t.flat_block = 512;
     t.item_order = 1;
+    t.relabel = 0;
     return t;
+}
+
+// The parameters the plan's arrays depend on (the others only change launches).
+bool same_plan_params(const ga_tc_tuning& a, const ga_tc_tuning& b) {
+    bool same = a.work_chunk == b.work_chunk && a.hash_cap == b.hash_cap && a.batches == b.batches && a.item_order == b.item_order;
+    for (int i = 0; i < NHASH; i++) same = same && a.lanes[i] == b.lanes[i];
+    return same;
 }
 
 bool tuning_valid(const ga_tc_tuning& t) {
@@ -947,7 +1006,7 @@ bool tuning_valid(const ga_tc_tuning& t) {
             return false;
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
-           t.flat_block <= 1024 && t.flat_block % 64 == 0 && (t.item_order == 0 || t.item_order == 1) &&
+           t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && (t.relabel == 0 || t.relabel == 1) &&
            (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
                                   (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
                                    t.flat_chunk / t.flat_block == 16)));
@@ -962,9 +1021,21 @@ PlanShape plan_shape(const ga_tc_tuning& t) {
 
 }  // namespace
 
+// The CSR and the plan one launch reads: the graph's own (participation, support, alive masks, and counts without
+// relabel) or the renumbered copy of the counts.
+struct CountView {
+    TcPlan* P;
+    const uint32_t *rowptr, *col, *src;
+};
+static CountView own_view(ga_graph* g) { return CountView{&g->plan, g->rowptr, g->col, g->src}; }
+static bool relabel_on(const ga_graph* g) { return g->plan.tune.block && g->plan.tune.relabel; }
+static CountView count_view(ga_graph* g) {
+    return relabel_on(g) ? CountView{&g->cplan, g->crow, g->ccol, g->csrc} : own_view(g);
+}
+
 // Flat units and entry records of the light entries (built the first time the flat kernel is switched on).
-static int flat_build_units(ga_ctx* ctx, ga_graph* g) {
-    TcPlan& P = g->plan;
+static int flat_build_units(ga_ctx* ctx, ga_graph* g, const CountView& V) {
+    TcPlan& P = *V.P;
     const uint32_t nlight = P.nlight;
     DevBuf hstart_b, wt_b;
     GA_TRY(ga_malloc(ctx, &hstart_b.p, ((size_t)g->n + 1) * 4));
@@ -973,7 +1044,7 @@ static int flat_build_units(ga_ctx* ctx, ga_graph* g) {
     unsigned long long* wt = (unsigned long long*)wt_b.p;
     k_plan_rowstart<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wh, nlight, hstart);
     GA_TRY(ga_launch_check(ctx, "k_plan_rowstart"));
-    k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, P.wh, P.wlen, nlight, wt);
+    k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wlen, nlight, wt);
     GA_TRY(ga_launch_check(ctx, "k_plan_units"));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nlight, 1, 0));
     GA_TRY(ga_malloc(ctx, (void**)&P.wu, ((size_t)nlight + 1) * 8));
@@ -983,26 +1054,26 @@ static int flat_build_units(ga_ctx* ctx, ga_graph* g) {
     P.utot = ctx->hstat[0];
     if (P.utot >= FLAT_UNIT_LIMIT) return GA_OK;   // too many units for 32-bit offsets: lane groups only
     GA_TRY(ga_malloc(ctx, (void**)&P.frec, (size_t)nlight * 16));
-    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, P.wh, P.wps, P.wu, hstart, nlight, (uint4*)P.frec);
+    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wps, P.wu, hstart, nlight, (uint4*)P.frec);
     GA_TRY(ga_launch_check(ctx, "k_plan_frec"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GA_OK;
 }
 
 // Cuts the light entries' units into chunks of tune.flat_chunk (nothing when the flat kernel is off).
-static int flat_rechunk(ga_ctx* ctx, ga_graph* g) {
-    TcPlan& P = g->plan;
+static int flat_rechunk(ga_ctx* ctx, ga_graph* g, const CountView& V) {
+    TcPlan& P = *V.P;
     ga_free(P.cstart);
     P.cstart = nullptr;
     P.nch = 0;
     P.chunk_used = (uint32_t)P.tune.flat_chunk;
     if (!P.nlight || !P.tune.flat_chunk) return GA_OK;
-    if (!P.wu) GA_TRY(flat_build_units(ctx, g));
+    if (!P.wu) GA_TRY(flat_build_units(ctx, g, V));
     if (!P.frec) return GA_OK;
     const uint32_t chunk = (uint32_t)P.tune.flat_chunk;
     const unsigned long long nch = (P.utot + chunk - 1) / chunk;
     GA_TRY(ga_malloc(ctx, (void**)&P.cstart, (nch + 1) * 8));   // cstart (nch + 1) then chs (nch)
-    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.wh, g->rowptr, P.nlight, chunk, P.cstart,
+    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.wh, V.rowptr, P.nlight, chunk, P.cstart,
                                                                           P.cstart + nch + 1);
     GA_TRY(ga_launch_check(ctx, "k_plan_chunks"));
     GA_TRY(ga_fill_u32(ctx, P.cstart + nch, 1, P.nlight));
@@ -1010,8 +1081,8 @@ static int flat_rechunk(ga_ctx* ctx, ga_graph* g) {
     return GA_OK;
 }
 
-static int build_plan(ga_ctx* ctx, ga_graph* g) {
-    TcPlan& P = g->plan;
+static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
+    TcPlan& P = *V.P;
     if (!P.tune.block) P.tune = default_tuning();
     const uint32_t n = g->n, m = g->m;
     ga_free(P.wcol);
@@ -1043,7 +1114,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
         int nb = 1;
         while (nb < 32 && (1ull << nb) < (uint64_t)n) nb++;
         const bool split = 33 + nb <= 64;   // room for the light/heavy bit above the vertex id
-        k_plan_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, m, nb, split,
+        k_plan_keys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, V.src, V.col, m, nb, split,
                                                                      (unsigned long long*)keys.p, (uint32_t*)vals.p, bc + 30);
         GA_TRY(ga_launch_check(ctx, "k_plan_keys"));
         unsigned long long* ko = nullptr;
@@ -1078,7 +1149,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     unsigned long long* wsum = (unsigned long long*)wsum_b.p;
     uint32_t* flag = (uint32_t*)flag_b.p;
     uint32_t* fpos = (uint32_t*)fpos_b.p;
-    k_plan_fill<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(g->rowptr, g->src, g->col, sorted_vals, nwork, P.wcol, P.wedge, P.wps,
+    k_plan_fill<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, V.src, V.col, sorted_vals, nwork, P.wcol, P.wedge, P.wps,
                                                                      P.wlen, wh, wt);
     GA_TRY(ga_launch_check(ctx, "k_plan_fill"));
     k_plan_rowstart<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(wh, nwork, hstart);
@@ -1086,7 +1157,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nwork, 1, 0));
     GA_TRY(ga_exclusive_scan_u64(ctx, (const uint64_t*)wt, (uint64_t*)wsum, (size_t)nwork + 1));
     const uint32_t cap = (uint32_t)P.tune.hash_cap;
-    k_plan_flags<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(hstart, wh, wsum, g->rowptr, nwork, (uint32_t)P.tune.work_chunk, cap,
+    k_plan_flags<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(hstart, wh, wsum, V.rowptr, nwork, (uint32_t)P.tune.work_chunk, cap,
                                                                       plan_shape(P.tune), flag);
     GA_TRY(ga_launch_check(ctx, "k_plan_flags"));
     GA_TRY(ga_fill_u32(ctx, flag + nwork, 1, 0));
@@ -1097,7 +1168,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     GA_TRY(ga_malloc(ctx, &bcount_b.p, (nbc + 1) * 8));
     uint32_t* bcount = (uint32_t*)bcount_b.p;
     uint32_t* boff = bcount + nbc + 1;
-    k_plan_item_count<<<nblk, 256, 0, ctx->stream>>>(flag, wh, g->rowptr, nwork, span, cap, bcount);
+    k_plan_item_count<<<nblk, 256, 0, ctx->stream>>>(flag, wh, V.rowptr, nwork, span, cap, bcount);
     GA_TRY(ga_launch_check(ctx, "k_plan_item_count"));
     GA_TRY(ga_fill_u32(ctx, bcount + nbc, 1, 0));
     GA_TRY(ga_exclusive_scan_u32(ctx, bcount, boff, nbc + 1));
@@ -1114,7 +1185,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     GA_TRY(ga_malloc(ctx, (void**)&P.items, (size_t)nall * 16));
     k_plan_rank<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(flag, fpos, nwork, (uint32_t*)starts_b.p);
     GA_TRY(ga_launch_check(ctx, "k_plan_rank"));
-    k_plan_items<<<nblk, 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, g->rowptr, nwork, nall, span, cap, boff, P.items);
+    k_plan_items<<<nblk, 256, 0, ctx->stream>>>(flag, fpos, (uint32_t*)starts_b.p, wh, V.rowptr, nwork, nall, span, cap, boff, P.items);
     GA_TRY(ga_launch_check(ctx, "k_plan_items"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     P.built = P.tune;
@@ -1125,8 +1196,13 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
         GA_TRY(ga_malloc(ctx, &v1.p, (size_t)nall * 4));
         GA_TRY(ga_malloc(ctx, &v2.p, (size_t)nall * 4));
         GA_TRY(ga_malloc(ctx, &it2.p, (size_t)nall * 16));
-        const uint32_t gbits = P.group ? P.group_bits : 0u;
-        k_plan_item_shape<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, nall, g->rowptr, P.wlen, cap, P.group, gbits,
+        // item_order 2: windows of 2^12 consecutive h ids in id order (locality), shape order inside each window
+        const uint32_t wshift = P.tune.item_order >= 4 ? 16u : P.tune.item_order >= 2 ? 12u : 0u;
+        const uint32_t* group = wshift ? nullptr : P.group;
+        uint32_t gbits = group ? P.group_bits : 0u;
+        if (wshift)
+            while (gbits < 32 && ((uint64_t)1 << gbits) <= ((uint64_t)n >> wshift)) gbits++;
+        k_plan_item_shape<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, nall, V.rowptr, P.wlen, cap, group, gbits, wshift,
                                                                               (unsigned long long*)k1.p, (uint32_t*)v1.p);
         GA_TRY(ga_launch_check(ctx, "k_plan_item_shape"));
         unsigned long long* ks = nullptr;
@@ -1143,28 +1219,41 @@ static int build_plan(ga_ctx* ctx, ga_graph* g) {
     for (int b = 0; b < NHASH; b++) P.nitems[b] = cnt[b];
     P.nhuge = cnt[NHASH];
     P.ready = true;
-    return flat_rechunk(ctx, g);
+    return flat_rechunk(ctx, g, V);
 }
 
-int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g) {
-    if (g->plan.ready) return g->plan.chunk_used == (uint32_t)g->plan.tune.flat_chunk ? GA_OK : flat_rechunk(ctx, g);
-    return build_plan(ctx, g);
+static int ensure_relabel(ga_ctx* ctx, ga_graph* g);
+
+int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g, bool counts_only) {
+    if (!g->plan.tune.block) g->plan.tune = default_tuning();
+    if (counts_only && relabel_on(g)) GA_TRY(ensure_relabel(ctx, g));
+    const CountView V = counts_only ? count_view(g) : own_view(g);
+    TcPlan& P = *V.P;
+    if (&P != &g->plan) {   // the renumbered copy's plan follows the graph's tuning and locality groups
+        P.tune = g->plan.tune;
+        P.group = g->plan.group;
+        P.group_bits = g->plan.group_bits;
+    }
+    if (P.ready && same_plan_params(P.built, P.tune))
+        return P.chunk_used == (uint32_t)P.tune.flat_chunk ? GA_OK : flat_rechunk(ctx, g, V);
+    return build_plan(ctx, g, V);
 }
 
 // table slots for the largest d+(h) a bucket may hold (h in bucket b has d+(h) <= bound):
-// 2^max(2, ceil(log2(bound)) + shift), the kernel's per-item rule at d+(h) = bound
+// 2^max(3, ceil(log2(bound)) + shift), the kernel's per-item rule at d+(h) = bound
 static uint32_t bucket_slots(uint32_t bound, uint32_t shift) {
     uint32_t lg = 0;
     while ((1u << lg) < bound) lg++;
     uint32_t bits = lg + shift;
-    if (bits < 2) bits = 2;
+    if (bits < 3) bits = 3;   // two buckets at least: the kernel's smallest table
     return 1u << bits;
 }
 
 template <int G, bool HASHED, int FLAGS>
-static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, uint32_t bound, TcArgs a) {
+static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const CountView& V, const uint32_t* items, uint32_t nitems, uint32_t bound,
+                              TcArgs a) {
     if (!nitems) return GA_OK;
-    const TcPlan& P = g->plan;
+    const TcPlan& P = *V.P;
     uint32_t block = (uint32_t)P.tune.block;
     // tables with per-slot counters (participation, supports) take twice the LDS: at most load shift 2 there
     const uint32_t shift = (FLAGS & (F_TRI | F_SUP)) ? std::min<uint32_t>((uint32_t)P.tune.load_shift, 2u) : (uint32_t)P.tune.load_shift;
@@ -1175,38 +1264,40 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const uint32_t* items, u
     while (block > 64 && (size_t)(block / G) * per_group > 64 * 1024) block >>= 1;
     const size_t shmem = (size_t)(block / G) * per_group;
     if (HASHED && shmem > ctx->lds_max)   // the tables do not fit one workgroup: binary-search this bucket in HBM
-        return launch_hash_bucket<64, false, FLAGS>(ctx, g, items, nitems, bound, a);
+        return launch_hash_bucket<64, false, FLAGS>(ctx, g, V, items, nitems, bound, a);
     const unsigned grid = grid_for((uint64_t)nitems * G, block, (unsigned)std::max(ctx->cus, 1) * 64u);
     a.items = items;
     a.nitems = nitems;
     a.ts_max = ts;
     a.load_shift = shift;
+    a.xcd = (P.built.item_order == 3 || P.built.item_order == 5) ? 1u : 0u;
     k_tc_hash<G, HASHED, FLAGS><<<grid, block, shmem, ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_hash");
 }
 
 template <int FLAGS>
-static int launch_tiny(ga_ctx* ctx, ga_graph* g, const uint32_t* items, uint32_t nitems, TcArgs a) {
+static int launch_tiny(ga_ctx* ctx, ga_graph* g, const CountView& V, const uint32_t* items, uint32_t nitems, TcArgs a) {
     if (!nitems) return GA_OK;
-    const uint32_t block = (uint32_t)g->plan.tune.block;
+    const uint32_t block = (uint32_t)V.P->tune.block;
     a.items = items;
     a.nitems = nitems;
+    a.xcd = (V.P->built.item_order == 3 || V.P->built.item_order == 5) ? 1u : 0u;
     k_tc_tiny<FLAGS><<<grid_for(nitems, block, (unsigned)std::max(ctx->cus, 1) * 64u), block, 0, ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_tiny");
 }
 
 // Flat kernel on the light rows; their lane-group buckets (d+(h) <= 128: buckets 0-2) are then skipped.
-static bool flat_on(const ga_graph* g) {
-    const TcPlan& P = g->plan;
+static bool flat_on(const CountView& V) {
+    const TcPlan& P = *V.P;
     return P.nch && P.chunk_used && P.tune.hash_cap >= (int)FLAT_MAX;
 }
 
-static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, const uint32_t* done, const uint32_t* gof,
+static int launch_flat(ga_ctx* ctx, ga_graph* g, const CountView& V, unsigned long long* total, const uint32_t* done, const uint32_t* gof,
                        unsigned long long* gcnt) {
-    const TcPlan& P = g->plan;
+    const TcPlan& P = *V.P;
     FlatArgs f{};
-    f.rowptr = g->rowptr;
-    f.col = g->col;
+    f.rowptr = V.rowptr;
+    f.col = V.col;
     f.wh = P.wh;
     f.frec = (const uint4*)P.frec;
     f.cstart = P.cstart;
@@ -1237,8 +1328,8 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, unsigned long long* total, cons
 }
 
 template <int FLAGS>
-static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a, int first_bucket = 0) {
-    const TcPlan& P = g->plan;
+static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, int first_bucket = 0) {
+    const TcPlan& P = *V.P;
     uint32_t off = 0;
     for (int b = 0; b < NHASH; b++) {
         const uint32_t* items = P.items + 4ull * off;
@@ -1247,19 +1338,19 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, TcArgs a, int first_bucket = 0)
         int rc = GA_OK;
         switch (P.tune.lanes[b]) {
             case 1:   // bucket 0 only: one lane per item for plain and per-graph counts, 4-lane groups otherwise
-                if constexpr (FLAGS == 0 || FLAGS == F_GRAPH) rc = launch_tiny<FLAGS>(ctx, g, items, ni, a);
-                else rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, items, ni, bound, a);
+                if constexpr (FLAGS == 0 || FLAGS == F_GRAPH) rc = launch_tiny<FLAGS>(ctx, g, V, items, ni, a);
+                else rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, V, items, ni, bound, a);
                 break;
-            case 4: rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, items, ni, bound, a); break;
-            case 8: rc = launch_hash_bucket<8, true, FLAGS>(ctx, g, items, ni, bound, a); break;
-            case 16: rc = launch_hash_bucket<16, true, FLAGS>(ctx, g, items, ni, bound, a); break;
-            case 32: rc = launch_hash_bucket<32, true, FLAGS>(ctx, g, items, ni, bound, a); break;
-            default: rc = launch_hash_bucket<64, true, FLAGS>(ctx, g, items, ni, bound, a); break;
+            case 4: rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 8: rc = launch_hash_bucket<8, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 16: rc = launch_hash_bucket<16, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 32: rc = launch_hash_bucket<32, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            default: rc = launch_hash_bucket<64, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
         }
         GA_TRY(rc);
         off += P.nitems[b];
     }
-    return launch_hash_bucket<64, false, FLAGS>(ctx, g, P.items + 4ull * off, P.nhuge, 0, a);
+    return launch_hash_bucket<64, false, FLAGS>(ctx, g, V, P.items + 4ull * off, P.nhuge, 0, a);
 }
 
 template <int FLAGS>
@@ -1277,6 +1368,125 @@ static int launch_group(ga_ctx* ctx, ga_graph* g, unsigned long long* total) {
     return ga_launch_check(ctx, "k_tc_group");
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// the renumbered copy of plain and per-graph counts (tune.relabel)
+// ---------------------------------------------------------------------------------------------------------------
+// Vertex v is keyed by lab(v) = the smallest id among v and its neighbours u with sup(u, v) >= 2 (the edge closes two
+// or more triangles), and the new ids follow (lab(v), v). Vertices of a dense cluster mostly share a few keys, so their
+// out-lists, which the count's items read together, land in nearby lines. Every strong neighbour of v lies in v's own
+// graph of a disjoint union and lab(v) <= v, so each graph keeps its id range. The renumbered CSR has the same
+// orientation (edge u -> v becomes perm(u) -> perm(v), rows re-sorted), hence the same count and the same work.
+namespace {
+
+__global__ void k_relabel_init(uint32_t n, uint32_t* __restrict__ lab) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) lab[v] = v;
+}
+
+__global__ void k_relabel_strong(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col, const uint32_t* __restrict__ sup,
+                                 uint32_t m, uint32_t* __restrict__ lab) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x)
+        if (sup[e] >= 2) {
+            const uint32_t u = src[e], v = col[e];
+            atomicMin(&lab[u], v);
+            atomicMin(&lab[v], u);
+        }
+}
+
+__global__ void k_relabel_vkeys(const uint32_t* __restrict__ lab, uint32_t n, unsigned long long* __restrict__ key) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        key[v] = ((unsigned long long)lab[v] << 32) | v;
+}
+
+// perm[old] = new, and the new row length of each new id
+__global__ void k_relabel_perm(const unsigned long long* __restrict__ sorted, const uint32_t* __restrict__ rowptr, uint32_t n,
+                               uint32_t* __restrict__ perm, uint32_t* __restrict__ deg) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t v = (uint32_t)sorted[i];
+        perm[v] = i;
+        deg[i] = rowptr[v + 1] - rowptr[v];
+    }
+}
+
+__global__ void k_relabel_ekeys(const uint32_t* __restrict__ src, const uint32_t* __restrict__ col, const uint32_t* __restrict__ perm,
+                                uint32_t m, unsigned long long* __restrict__ key) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x)
+        key[e] = ((unsigned long long)perm[src[e]] << 32) | perm[col[e]];
+}
+
+__global__ void k_relabel_unpack(const unsigned long long* __restrict__ key, uint32_t m, uint32_t* __restrict__ col,
+                                 uint32_t* __restrict__ src) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const unsigned long long k = key[e];
+        col[e] = (uint32_t)k;
+        src[e] = (uint32_t)(k >> 32);
+    }
+}
+
+}  // namespace
+
+template <int FLAGS>
+static int launch_merge(ga_ctx* ctx, ga_graph* g, const uint32_t* order, unsigned long long* total, const uint8_t* alive, uint32_t* tri,
+                        uint32_t* sup, const uint32_t* done);
+
+static int ensure_relabel(ga_ctx* ctx, ga_graph* g) {
+    if (g->crow) return GA_OK;
+    const uint32_t n = g->n, m = g->m;
+    uint32_t *crow = nullptr, *ccol = nullptr, *csrc = nullptr;
+    auto fail = [&](int rc) {
+        ga_free(crow);
+        ga_free(ccol);
+        ga_free(csrc);
+        return rc;
+    };
+    if (ga_malloc(ctx, (void**)&crow, ((size_t)n + 1) * 4) || ga_malloc(ctx, (void**)&ccol, std::max<size_t>(m, 1) * 4) ||
+        ga_malloc(ctx, (void**)&csrc, std::max<size_t>(m, 1) * 4))
+        return fail(GA_ERR_OOM);
+    int nb = 1;
+    while (nb < 32 && (1ull << nb) < (uint64_t)n) nb++;
+    int rc = GA_OK;
+    {
+        DevBuf sup, lab, cnt;
+        DevBuf vk0, vk1, perm, deg;
+        if ((rc = ga_malloc(ctx, &sup.p, std::max<size_t>(m, 1) * 4)) || (rc = ga_malloc(ctx, &lab.p, ((size_t)n + 1) * 4)) ||
+            (rc = ga_malloc(ctx, &cnt.p, 8)) || (rc = ga_malloc(ctx, &vk0.p, (size_t)n * 8)) || (rc = ga_malloc(ctx, &vk1.p, (size_t)n * 8)) ||
+            (rc = ga_malloc(ctx, &perm.p, (size_t)n * 4)) || (rc = ga_malloc(ctx, &deg.p, ((size_t)n + 1) * 4)))
+            return fail(rc);
+        // supports of every oriented edge (plan-free merge count) and the strong-neighbour key of every vertex
+        if ((rc = ga_fill_u32(ctx, (uint32_t*)sup.p, m, 0)) || (rc = ga_fill_u64(ctx, (uint64_t*)cnt.p, 1, 0))) return fail(rc);
+        if ((rc = launch_merge<F_SUP>(ctx, g, nullptr, (unsigned long long*)cnt.p, nullptr, nullptr, (uint32_t*)sup.p, nullptr))) return fail(rc);
+        k_relabel_init<<<grid_for(n, 256, 8192), 256, 0, ctx->stream>>>(n, (uint32_t*)lab.p);
+        if ((rc = ga_launch_check(ctx, "k_relabel_init"))) return fail(rc);
+        if (m) k_relabel_strong<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->src, g->col, (const uint32_t*)sup.p, m, (uint32_t*)lab.p);
+        if ((rc = ga_launch_check(ctx, "k_relabel_strong"))) return fail(rc);
+        k_relabel_vkeys<<<grid_for(n, 256, 8192), 256, 0, ctx->stream>>>((const uint32_t*)lab.p, n, (unsigned long long*)vk0.p);
+        if ((rc = ga_launch_check(ctx, "k_relabel_vkeys"))) return fail(rc);
+        unsigned long long* vs = nullptr;
+        uint64_t* vsorted = nullptr;
+        if ((rc = ga_sort_u64(ctx, (uint64_t*)vk0.p, (uint64_t*)vk1.p, n, 32 + nb, &vsorted))) return fail(rc);
+        vs = (unsigned long long*)vsorted;
+        k_relabel_perm<<<grid_for(n, 256, 8192), 256, 0, ctx->stream>>>(vs, g->rowptr, n, (uint32_t*)perm.p, (uint32_t*)deg.p);
+        if ((rc = ga_launch_check(ctx, "k_relabel_perm"))) return fail(rc);
+        if ((rc = ga_fill_u32(ctx, (uint32_t*)deg.p + n, 1, 0))) return fail(rc);
+        if ((rc = ga_exclusive_scan_u32(ctx, (const uint32_t*)deg.p, crow, (size_t)n + 1))) return fail(rc);
+        if (m) {
+            DevBuf ek0, ek1;
+            if ((rc = ga_malloc(ctx, &ek0.p, (size_t)m * 8)) || (rc = ga_malloc(ctx, &ek1.p, (size_t)m * 8))) return fail(rc);
+            k_relabel_ekeys<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(g->src, g->col, (const uint32_t*)perm.p, m,
+                                                                            (unsigned long long*)ek0.p);
+            if ((rc = ga_launch_check(ctx, "k_relabel_ekeys"))) return fail(rc);
+            uint64_t* es = nullptr;
+            if ((rc = ga_sort_u64(ctx, (uint64_t*)ek0.p, (uint64_t*)ek1.p, m, 32 + nb, &es))) return fail(rc);
+            k_relabel_unpack<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>((const unsigned long long*)es, m, ccol, csrc);
+            if ((rc = ga_launch_check(ctx, "k_relabel_unpack"))) return fail(rc);
+        }
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ga_fail(ctx, GA_ERR_HIP, "relabel failed"));
+    }
+    g->crow = crow;
+    g->ccol = ccol;
+    g->csrc = csrc;
+    return GA_OK;
+}
+
 // Launches one count of `variant` into *d_total (zeroed by the caller). alive/tri/sup as in the flags.
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* alive, uint32_t* tri, uint32_t* sup,
                  const uint32_t* done, const uint32_t* gof, unsigned long long* gcnt) {
@@ -1284,14 +1494,16 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
     if (variant == GA_TC_AUTO) variant = GA_TC_HASH;
     const int flags = (alive ? F_ALIVE : 0) | (tri ? F_TRI : 0) | (sup ? F_SUP : 0) | (gcnt ? F_GRAPH : 0);
     if (variant == GA_TC_HASH) {
-        GA_TRY(ga_tc_ensure_plan(ctx, g));
+        const bool counts = flags == 0 || flags == F_GRAPH;   // plain and per-graph counts may run on the renumbered copy
+        GA_TRY(ga_tc_ensure_plan(ctx, g, counts));
+        const CountView V = counts ? count_view(g) : own_view(g);
         TcArgs a{};
-        a.rowptr = g->rowptr;
-        a.col = g->col;
-        a.wcol = g->plan.wcol;
-        a.wedge = g->plan.wedge;
-        a.wps = g->plan.wps;
-        a.wlen = g->plan.wlen;
+        a.rowptr = V.rowptr;
+        a.col = V.col;
+        a.wcol = V.P->wcol;
+        a.wedge = V.P->wedge;
+        a.wps = V.P->wps;
+        a.wlen = V.P->wlen;
         a.alive = alive;
         a.tri = tri;
         a.sup = sup;
@@ -1299,17 +1511,17 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         a.done = done;
         a.gof = gof;
         a.gcnt = gcnt;
-        if ((flags == 0 || flags == F_GRAPH) && flat_on(g)) {
-            GA_TRY(launch_flat(ctx, g, total, done, gof, gcnt));
-            return flags ? launch_hash<F_GRAPH>(ctx, g, a, 3) : launch_hash<0>(ctx, g, a, 3);
+        if (counts && flat_on(V)) {
+            GA_TRY(launch_flat(ctx, g, V, total, done, gof, gcnt));
+            return flags ? launch_hash<F_GRAPH>(ctx, g, V, a, 3) : launch_hash<0>(ctx, g, V, a, 3);
         }
         switch (flags) {
-            case 0: return launch_hash<0>(ctx, g, a);
-            case F_TRI: return launch_hash<F_TRI>(ctx, g, a);
-            case F_ALIVE | F_TRI: return launch_hash<F_ALIVE | F_TRI>(ctx, g, a);
-            case F_SUP: return launch_hash<F_SUP>(ctx, g, a);
-            case F_GRAPH: return launch_hash<F_GRAPH>(ctx, g, a);
-            case F_ALIVE | F_TRI | F_GRAPH: return launch_hash<F_ALIVE | F_TRI | F_GRAPH>(ctx, g, a);
+            case 0: return launch_hash<0>(ctx, g, V, a);
+            case F_TRI: return launch_hash<F_TRI>(ctx, g, V, a);
+            case F_ALIVE | F_TRI: return launch_hash<F_ALIVE | F_TRI>(ctx, g, V, a);
+            case F_SUP: return launch_hash<F_SUP>(ctx, g, V, a);
+            case F_GRAPH: return launch_hash<F_GRAPH>(ctx, g, V, a);
+            case F_ALIVE | F_TRI | F_GRAPH: return launch_hash<F_ALIVE | F_TRI | F_GRAPH>(ctx, g, V, a);
             default: return ga_fail(ctx, GA_ERR_ARG, "unsupported flag combination %d for the hash variant", flags);
         }
     }
@@ -1360,8 +1572,9 @@ extern "C" int ga_tc_prepare(ga_ctx* ctx, ga_graph* g, float* ms) {
     if (!ctx || !g) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
     g->plan.ready = false;
+    g->cplan.ready = false;
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    GA_TRY(build_plan(ctx, g));
+    GA_TRY(ga_tc_ensure_plan(ctx, g, true));   // the plan the counts use (with the renumbered copy when relabel is set)
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
@@ -1372,7 +1585,7 @@ extern "C" int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, f
     if (!ctx || !g || !T) return GA_ERR_ARG;
     if (variant < 0 || variant >= GA_TC_NVARIANTS) return ga_fail(ctx, GA_ERR_ARG, "unknown variant %d", variant);
     GA_HIP(ctx, hipSetDevice(ctx->device));
-    if (variant == GA_TC_AUTO || variant == GA_TC_HASH) GA_TRY(ga_tc_ensure_plan(ctx, g));
+    if (variant == GA_TC_AUTO || variant == GA_TC_HASH) GA_TRY(ga_tc_ensure_plan(ctx, g, true));
     if (variant == GA_TC_SORTED) GA_TRY(ga_graph_ensure_perm(ctx, g));
     uint64_t* d = ctx->dsmall;
     GA_TRY(ga_fill_u64(ctx, d, 1, 0));
@@ -1392,7 +1605,7 @@ extern "C" int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t c
     GA_HIP(ctx, hipSetDevice(ctx->device));
     for (uint32_t i = 0; i < count; i++) {
         if (!graphs[i]) return ga_fail(ctx, GA_ERR_ARG, "graph %u is NULL", i);
-        if (variant == GA_TC_AUTO || variant == GA_TC_HASH) GA_TRY(ga_tc_ensure_plan(ctx, graphs[i]));
+        if (variant == GA_TC_AUTO || variant == GA_TC_HASH) GA_TRY(ga_tc_ensure_plan(ctx, graphs[i], true));
         if (variant == GA_TC_SORTED) GA_TRY(ga_graph_ensure_perm(ctx, graphs[i]));
     }
     if (count > ctx->dmany_n) {
@@ -1418,11 +1631,12 @@ extern "C" int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t c
 extern "C" int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32_t* nitems6) {
     if (!ctx || !g) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
-    GA_TRY(ga_tc_ensure_plan(ctx, g));
-    if (nwork) *nwork = g->plan.nwork;
+    GA_TRY(ga_tc_ensure_plan(ctx, g, true));
+    const TcPlan& P = *count_view(g).P;
+    if (nwork) *nwork = P.nwork;
     if (nitems6) {
-        for (int b = 0; b < NHASH; b++) nitems6[b] = g->plan.nitems[b];
-        nitems6[NHASH] = g->plan.nhuge;
+        for (int b = 0; b < NHASH; b++) nitems6[b] = P.nitems[b];
+        nitems6[NHASH] = P.nhuge;
     }
     return GA_OK;
 }
@@ -1444,14 +1658,10 @@ extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t)
     if (!tuning_valid(*t)) return ga_fail(ctx, GA_ERR_ARG, "invalid tuning");
     if (t->flat_chunk && flat_shmem((uint32_t)t->flat_chunk) > ctx->lds_max)
         return ga_fail(ctx, GA_ERR_ARG, "flat chunk %d does not fit the workgroup's LDS", t->flat_chunk);
-    // the arrays are rebuilt only when a plan parameter differs from the ones they were built with (a tuning that comes
-    // back after a trial reuses them)
-    const ga_tc_tuning& b0 = g->plan.built;
-    bool same = g->plan.items && b0.work_chunk == t->work_chunk && b0.hash_cap == t->hash_cap && b0.batches == t->batches &&
-                b0.item_order == t->item_order;
-    for (int b = 0; b < NHASH; b++) same = same && b0.lanes[b] == t->lanes[b];
+    // each plan's arrays are rebuilt, on its next use, only when a plan parameter differs from the ones they were built
+    // with (a tuning that comes back after a trial reuses them)
     g->plan.tune = *t;
-    g->plan.ready = same;
+    g->cplan.tune = *t;
     return GA_OK;
 }
 
@@ -1459,7 +1669,7 @@ extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t)
 // words, the last one the total) as ga_batch_count runs it.
 static int timed_count(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, uint64_t* T, float* ms) {
     if (!gcnt) return ga_tc_count(ctx, g, GA_TC_HASH, T, ms);
-    GA_TRY(ga_tc_ensure_plan(ctx, g));
+    GA_TRY(ga_tc_ensure_plan(ctx, g, true));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)gcnt, (size_t)ngraphs + 1, 0));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, gcnt + ngraphs, nullptr, nullptr, nullptr, nullptr, gof, gcnt));
@@ -1500,17 +1710,23 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
-    // coordinate search: item order, lanes per bucket, then block, load shift, batches and work chunk
+    // coordinate search: relabel and item order, lanes per bucket, then block, load shift, batches and work chunk
     GA_TRY(trial(base));
-    {
-        ga_tc_tuning t = bt;
-        t.item_order = !bt.item_order;
-        GA_TRY(trial(t));
-    }
+    for (int rl = 0; rl <= 1; rl++)
+        for (int io = 0; io <= 2; io++) {
+            ga_tc_tuning t = bt;
+            t.relabel = rl;
+            t.item_order = io;
+            if (t.relabel == bt.relabel && t.item_order == bt.item_order) continue;
+            GA_TRY(trial(t));
+        }
+    GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
+    GA_TRY(ga_tc_ensure_plan(ctx, g, true));
+    const CountView V = count_view(g);
     const int lane_opts[6] = {4, 8, 16, 32, 64, 1};   // 1: bucket 0 only (one lane per item)
     for (int b = 0; b < NHASH; b++) {
-        if (!g->plan.nitems[b]) continue;
-        if (b < 3 && flat_on(g)) continue;   // light rows go to the flat kernel
+        if (!V.P->nitems[b]) continue;
+        if (b < 3 && flat_on(V)) continue;   // light rows go to the flat kernel
         for (int li = 0; li < (b == 0 ? 6 : 5); li++) {
             ga_tc_tuning t = bt;
             t.lanes[b] = lane_opts[li];
@@ -1526,7 +1742,7 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
     for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
     const int chunks[3] = {2048, 8192, 32768};
     for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
-    if (g->plan.nlight) {
+    if (V.P->nlight) {
         // 4 or 8 units per thread (16 keeps too many records and loads in registers)
         const int fsh[6][2] = {{0, 256}, {1024, 256}, {2048, 512}, {2048, 256}, {4096, 512}, {4096, 1024}};
         for (int ci = 0; ci < 6; ci++) {
@@ -1540,7 +1756,7 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         }
     }
     GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
-    GA_TRY(ga_tc_ensure_plan(ctx, g));
+    GA_TRY(ga_tc_ensure_plan(ctx, g, true));
     if (best) *best = bt;
     if (best_ms) *best_ms = bms;
     return GA_OK;

@@ -163,7 +163,14 @@ typedef struct ga_tc_tuning {
                               A multiple of flat_block in [flat_block, 8192]; needs hash_cap >= 128 */
     int32_t flat_block;    /* threads per workgroup of the flat kernel (64..1024, multiple of 64) */
     int32_t item_order;    /* 1: work items of a bucket ordered by shape (entries, longest probe list, d+(h)), so the
-                              lane groups of one wavefront run loops of equal length; 0: in row order */
+                              lane groups of one wavefront run loops of equal length; 0: in row order; 2: by shape
+                              within windows of 4,096 consecutive h ids, windows in id order, each XCD (workgroups
+                              b with the same b mod 8) taking one contiguous eighth of a bucket's items */
+    int32_t relabel;       /* 1: plain and per-graph counts run on a private copy of the oriented CSR whose vertices
+                              are renumbered so that vertices joined by edges in two or more triangles sit together
+                              (each vertex is keyed by the smallest id among itself and such neighbours); the
+                              orientation, and so every count, is unchanged. 0: on the CSR itself. Participation,
+                              support and the per-vertex states always use the CSR's own ids */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
 /* The library's default tuning (what a graph uses until ga_tc_set_tuning or ga_tc_tune). */

@@ -191,17 +191,20 @@ def test_tuning_shapes(ctx):
                 t.lanes[:] = lanes
                 t.block, t.work_chunk, t.load_shift, t.batches = block, chunk, shift, batches
                 t.flat_chunk, t.flat_block = flats[(li + block) % len(flats)]
-                t.item_order = (li + shift) % 2
+                t.item_order = (li + shift) % 3
+                t.relabel = (li + block // 64 + shift) % 2   # the renumbered copy for plain counts
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)
                     assert g.count(ga.TC_HASH)[0] == T, t.as_dict()
-        for order in (0, 1):
+        for order in (0, 1, 2):
             t = g.get_tuning()
             t.item_order = order
+            t.relabel = order % 2   # participation and support stay on the CSR's own ids whatever relabel says
             g.set_tuning(t)
             assert np.array_equal(g.vertex_participation()[1], tri_ref), (lanes, order)
             assert np.array_equal(g.edge_support()[1], sup_ref), (lanes, order)
+            assert g.count(ga.TC_HASH)[0] == T, (lanes, order)
     for fc, fb in flats + ((512, 128), (8192, 512), (1024, 64), (16384, 1024)):
         t = g.get_tuning()
         t.hash_cap, t.flat_chunk, t.flat_block = 4096, fc, fb
@@ -215,6 +218,42 @@ def test_tuning_shapes(ctx):
     best, ms = g.tune()
     assert g.count(ga.TC_HASH)[0] == T and ms > 0
     g.close()
+
+
+def test_relabel_counts_on_every_graph_kind(ctx):
+    """The renumbered copy (relabel = 1) with the windowed, XCD-contiguous item order counts every graph kind exactly,
+    including graphs with isolated vertices, a star, cliques sharing vertices and triangle-free graphs; the batch union
+    keeps each graph's count."""
+    from graph_analyzer_amd import gen
+    rng = np.random.default_rng(77)
+    cases = [gen.chung_lu(20000, 200000, 2.1, 2.0, seed=8), gen.communities(20000, 300, 10, 120, 0.3, seed=9),
+             gen.rmat(14, 1 << 17, seed=10), np.array([(0, i) for i in range(1, 3000)], np.uint32),
+             np.array(clique(40) + clique(40, 39) + clique(30, 5000), np.uint32), random_graph(rng, 700, 0.05, dup=0.2, loops=3)]
+    for raw in cases:
+        og = oracle.OGraph.from_pairs(raw)
+        T = og.triangles()
+        g = ga.DeviceGraph.from_pairs(ctx, raw)
+        for order in (1, 2):
+            for lanes0 in (1, 4):
+                t = g.get_tuning()
+                t.relabel, t.item_order, t.lanes[0] = 1, order, lanes0
+                g.set_tuning(t)
+                assert g.count(ga.TC_HASH)[0] == T, (order, lanes0)
+        t = g.get_tuning()
+        t.flat_chunk, t.flat_block = 2048, 256   # the flat kernel on the renumbered copy
+        g.set_tuning(t)
+        assert g.count(ga.TC_HASH)[0] == T
+        g.close()
+    b = ga.Batch(ctx, cases)
+    Tb, _ = b.count_all()
+    t = b.get_tuning()
+    t.relabel, t.item_order = 1, 2
+    b.set_tuning(t)
+    Tr, _ = b.count_all()
+    Tp, _ = b.count_all(per_graph=True)
+    assert np.array_equal(Tb, Tr) and np.array_equal(Tb, Tp)
+    assert [int(x) for x in Tb] == [oracle.OGraph.from_pairs(r).triangles() for r in cases]
+    b.close()
 
 
 def test_flat_kernel_edge_cases(ctx):

@@ -118,6 +118,8 @@ def lib() -> ctypes.CDLL:
             "ga_tc_count": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_count_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_uint32, _vp]),
             "ga_batch_work": (ctypes.c_int, [_vp, _vp, _u64p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_ctx_barrier_probe": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                                                    ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_count_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_int, _u64p,
                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
@@ -199,6 +201,12 @@ class Context:
     def greedy_max_vertices(self) -> int:
         """The largest graph (vertices) DeviceGraph.greedy_pp takes on this device: one workgroup's LDS holds its state."""
         return int(lib().ga_greedy_max_vertices(self._h))
+
+    def barrier_probe(self, workgroups: int = 256, iters: int = 2000):
+        """(us per device-wide barrier in a cooperative kernel, us per back-to-back launch) at this grid size."""
+        b, l = ctypes.c_float(), ctypes.c_float()
+        self.check(lib().ga_ctx_barrier_probe(self._h, workgroups, iters, ctypes.byref(b), ctypes.byref(l)), "barrier_probe")
+        return b.value, l.value
 
     def mem_info(self):
         """(free, total) device bytes."""

@@ -207,6 +207,112 @@ extern "C" int ga_ctx_gather_probe(ga_ctx* ctx, uint64_t bytes, uint32_t seg_byt
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// barrier probe: what one device-wide barrier inside a cooperative (all-resident) kernel costs, next to what one
+// kernel boundary costs, for the choice between a persistent round loop and one launch per phase
+// ---------------------------------------------------------------------------------------------------------------
+namespace {
+struct ProbeBar {
+    unsigned count[8 * 16];   // per-XCD arrival counters, 64 bytes apart
+    unsigned top, gen, fail, pad;
+};
+
+// Two-level barrier: workgroups arrive on their XCD's counter (blockIdx mod 8), the last of each XCD on the top
+// counter, the last of those bumps the generation. Release before arriving, acquire after leaving (agent scope: L2
+// write-back and invalidate on this part); the wait polls relaxed and gives up after `limit` polls (fail = 1).
+__device__ __forceinline__ bool probe_grid_sync(ProbeBar* b, unsigned& gen, unsigned limit) {
+    __shared__ unsigned ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned g = gen, nb = gridDim.x;
+        const unsigned x = blockIdx.x % 8u, nx = (nb - x + 7u) / 8u, groups = nb < 8u ? nb : 8u;
+        unsigned good = 1;
+        bool last = false;
+        if (atomicAdd(&b->count[x * 16], 1u) + 1u == nx) {
+            __hip_atomic_store(&b->count[x * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = atomicAdd(&b->top, 1u) + 1u == groups;
+        }
+        if (last) {
+            __hip_atomic_store(&b->top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            for (unsigned spins = 0; __hip_atomic_load(&b->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g;) {
+                if (++spins > limit) {
+                    good = 0;
+                    __hip_atomic_store(&b->fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+        gen = g + 1;
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+__global__ void k_barrier_probe(ProbeBar* b, unsigned iters, unsigned limit, unsigned* sink) {
+    unsigned gen = 0;
+    for (unsigned i = 0; i < iters; i++) {
+        if (threadIdx.x == 0) sink[blockIdx.x] += 1u;   // one write per workgroup per phase
+        if (!probe_grid_sync(b, gen, limit)) return;
+    }
+}
+
+__global__ void k_launch_probe(unsigned* sink) {
+    if (threadIdx.x == 0) sink[blockIdx.x] += 1u;
+}
+}  // namespace
+
+extern "C" int ga_ctx_barrier_probe(ga_ctx* ctx, uint32_t workgroups, uint32_t iters, float* us_per_barrier, float* us_per_launch) {
+    if (!ctx) return GA_ERR_ARG;
+    if (workgroups < 1 || workgroups > 65536 || iters < 1 || iters > 100000)
+        return ga_fail(ctx, GA_ERR_ARG, "barrier probe: 1..65536 workgroups, 1..100000 barriers");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    int per_cu = 0;
+    GA_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_barrier_probe, 256, 0));
+    if ((uint64_t)workgroups > (uint64_t)per_cu * (uint64_t)std::max(ctx->cus, 1))
+        return ga_fail(ctx, GA_ERR_ARG, "barrier probe: %u workgroups cannot all be resident (%d per CU)", workgroups, per_cu);
+    void* mem = nullptr;
+    GA_TRY(ga_malloc(ctx, &mem, sizeof(ProbeBar) + (size_t)workgroups * 4));
+    ProbeBar* b = (ProbeBar*)mem;
+    unsigned* sink = (unsigned*)((char*)mem + sizeof(ProbeBar));
+    int rc = ga_fill_u32(ctx, (uint32_t*)mem, sizeof(ProbeBar) / 4 + workgroups, 0);
+    const unsigned limit = 1u << 22;   // about 0.1 s of polling before a workgroup gives up
+    float ms_b = 0, ms_l = 0;
+    if (!rc) {
+        void* args[] = {&b, &iters, (void*)&limit, &sink};
+        if (hipEventRecord(ctx->ev0, ctx->stream) != hipSuccess ||
+            hipLaunchCooperativeKernel((const void*)k_barrier_probe, dim3(workgroups), dim3(256), args, 0, ctx->stream) != hipSuccess ||
+            hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+            hipEventElapsedTime(&ms_b, ctx->ev0, ctx->ev1) != hipSuccess)
+            rc = ga_fail(ctx, GA_ERR_HIP, "barrier probe: cooperative launch failed");
+    }
+    if (!rc) {
+        ProbeBar h;
+        if (hipMemcpy(&h, b, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) rc = GA_ERR_HIP;
+        else if (h.fail || h.gen != iters) rc = ga_fail(ctx, GA_ERR_STATE, "barrier probe: a workgroup gave up waiting");
+    }
+    if (!rc) {
+        (void)hipEventRecord(ctx->ev0, ctx->stream);
+        for (uint32_t i = 0; i < iters && !rc; i++) {
+            k_launch_probe<<<workgroups, 256, 0, ctx->stream>>>(sink);
+            if (hipGetLastError() != hipSuccess) rc = GA_ERR_HIP;
+        }
+        if (!rc && (hipEventRecord(ctx->ev1, ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev1) != hipSuccess ||
+                    hipEventElapsedTime(&ms_l, ctx->ev0, ctx->ev1) != hipSuccess))
+            rc = GA_ERR_HIP;
+    }
+    ga_free(mem);
+    if (rc) return rc;
+    if (us_per_barrier) *us_per_barrier = ms_b * 1e3f / (float)iters;
+    if (us_per_launch) *us_per_launch = ms_l * 1e3f / (float)iters;
+    return GA_OK;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // sorts and scans (rocPRIM device primitives, temporary storage from the context scratch)
 // ---------------------------------------------------------------------------------------------------------------
 int ga_sort_u64(ga_ctx* ctx, uint64_t* keys, uint64_t* tmp, size_t count, int end_bit, uint64_t** result) {

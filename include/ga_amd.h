@@ -48,6 +48,12 @@ int ga_ctx_copy_probe(ga_ctx* ctx, uint64_t bytes, float* ms);
  * measured ceiling of the count's access pattern (short probe lists at random places). No counterpart in the
  * reference (measurement support). */
 int ga_ctx_gather_probe(ga_ctx* ctx, uint64_t bytes, uint32_t seg_bytes, uint64_t* segments, float* ms);
+/* What a device-wide barrier costs inside a cooperative kernel of `workgroups` resident workgroups of 256 threads
+ * (two-level arrival counters, agent-scope release/acquire, bounded wait), and what one launch of an equally shaped
+ * trivial kernel costs back to back on the stream: us_per_barrier, us_per_launch, each averaged over `iters`. The
+ * measurement behind running the core's rounds as separate launches. No counterpart in the reference (measurement
+ * support). */
+int ga_ctx_barrier_probe(ga_ctx* ctx, uint32_t workgroups, uint32_t iters, float* us_per_barrier, float* us_per_launch);
 /* Free and total device memory (hipMemGetInfo): the serving loop's admission budget. */
 int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 

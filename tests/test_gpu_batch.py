@@ -144,3 +144,9 @@ def test_device_probes(ctx):
         assert ms > 0 and n == cus * 32 * 256 // (seg // 4) * 32 * 8, (seg, n, ms)
     with pytest.raises(ga.GAError):
         ctx.gather_probe(64 << 20, 96)
+    for wg in (8, 64, cus):   # a device-wide barrier vs one launch boundary (the persistent-loop trade-off)
+        bar_us, launch_us = ctx.barrier_probe(wg, 500)
+        print(f"barrier probe: {wg} workgroups: {bar_us:.2f} us per barrier, {launch_us:.2f} us per launch", flush=True)
+        assert bar_us > 0 and launch_us > 0
+    with pytest.raises(ga.GAError):
+        ctx.barrier_probe(cus * 1000, 10)   # cannot all be resident

@@ -17,8 +17,6 @@ __device__ __forceinline__ void group_sync_wave() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ uint32_t hash_slot_u32(uint32_t x, uint32_t tbits) { return (x * 2654435769u) >> (32u - tbits); }
-
 #define GA_EMPTY 0xFFFFFFFFu
 
 struct CoreDev {
@@ -365,14 +363,12 @@ struct CoreFusedDev {
     uint32_t na_new;      // compaction cursor for the new alist
     uint32_t nfv;         // frontier vertices this round
     uint32_t nent;        // frontier entries this round
-    uint32_t tcursor;     // words of the round's table arena in use
     uint32_t min_next;    // min tri over the alive vertices after this round (the next round's level candidate)
     uint32_t ticket;      // workgroups of k_cf_tail that have finished this round
     unsigned long long adj_cursor;   // compaction cursor in the new adjacency buffer
 };
 
 struct CoreBufs {
-    const uint32_t* hubid;    // bitmap index of each vertex or GA_EMPTY
     const uint32_t* bitmaps;  // bm_words words per hub
     size_t bm_words;
     uint32_t* alist[2];
@@ -383,9 +379,8 @@ struct CoreBufs {
     uint32_t* cgt;            // per vertex: position in its list of the first neighbour with a larger id (cN(v) after v)
     uint32_t* fvert;
     uint32_t* ent;            // 2 words per entry: u, chunk
-    uint32_t* toff;           // per vertex: offset of its table in the arena (frontier vertices of this round)
-    uint32_t* tbits;          // per vertex: log2 of its table size
-    uint32_t* tables;         // the round's table arena
+    const uint2* mref;        // per vertex: {bitmap index, 1 << 31} for a hub, else {first bucket, log2 of buckets}
+    const uint4* mtab;        // membership tables of the original neighbour lists (4-slot buckets, load <= 1/2)
 };
 
 __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32_t* __restrict__ cbeg, uint32_t* __restrict__ cend,
@@ -404,7 +399,6 @@ __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32
         f->na_new = 0;
         f->nfv = 0;
         f->nent = 0;
-        f->tcursor = 0;
         f->min_next = 0xFFFFFFFFu;
         f->ticket = 0;
         f->adj_cursor = 0;
@@ -425,8 +419,8 @@ __global__ void k_cf_min(CoreBufs B, const CoreFusedDev* __restrict__ f, const u
     if (lane_id() == 0 && mn != 0xFFFFFFFFu) atomicMin(&st->min_tri, mn);
 }
 
-// Frontier = alive v with tri <= k; appends v to fvert, its chunk entries and (if it is in a triangle) a hash table
-// of 2^(ceil(log2 |cN(v)|) + 1) slots in the round's table arena (one atomic per wave for each).
+// Frontier = alive v with tri <= k; appends v to fvert and, if it is in a triangle, its chunk entries (one atomic per
+// wave for each).
 __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restrict__ tri, uint8_t* __restrict__ state,
                           uint32_t* __restrict__ core, CoreDev* st) {
     if (st->done) return;
@@ -439,46 +433,37 @@ __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restric
     for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
         const uint64_t i = base + lane;
         bool join = false;
-        uint32_t v = 0, nc = 0, tb = 0, ts = 0;
+        uint32_t v = 0, nc = 0;
         if (i < na) {
             v = al[i];
             if (state[v] == 1 && (unsigned long long)tri[v] <= k) {
                 join = true;
                 state[v] = 2;
                 core[v] = (uint32_t)k;
-                // a vertex in no alive triangle has nothing to subtract: no entries, no table
+                // a vertex in no alive triangle has nothing to subtract: no entries
                 const uint32_t d = B.cend[v] - B.cbeg[v];
-                if (tri[v] && d) {
-                    nc = (d + CORE_CHUNK - 1) / CORE_CHUNK;
-                    tb = (d > 1 ? 32u - (uint32_t)__clz(d - 1) : 0u) + 1u;
-                    if (tb < 2) tb = 2;
-                    ts = 1u << tb;
-                }
+                if (tri[v] && d) nc = (d + CORE_CHUNK - 1) / CORE_CHUNK;
             }
         }
         const uint64_t mask = __ballot(join);
         if (!mask) continue;
         const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-        uint32_t inc = nc, tinc = ts;   // inclusive scans over the wave
+        uint32_t inc = nc;   // inclusive scan over the wave
 #pragma unroll
         for (uint32_t o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, 64), t2 = __shfl_up(tinc, o, 64);
-            if (lane >= o) { inc += t; tinc += t2; }
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
         }
-        const uint32_t tot = __shfl(inc, 63, 64), ttot = __shfl(tinc, 63, 64);
-        uint32_t vpos = 0, epos = 0, tpos = 0;
+        const uint32_t tot = __shfl(inc, 63, 64);
+        uint32_t vpos = 0, epos = 0;
         if (lane == leader) {
             vpos = atomicAdd(&f->nfv, (uint32_t)__popcll(mask));
             epos = tot ? atomicAdd(&f->nent, tot) : 0u;
-            tpos = ttot ? atomicAdd(&f->tcursor, ttot) : 0u;
         }
         vpos = __shfl(vpos, leader, 64);
         epos = __shfl(epos, leader, 64);
-        tpos = __shfl(tpos, leader, 64);
         if (join) {
             B.fvert[vpos + mask_rank(mask)] = v;
-            B.toff[v] = tpos + tinc - ts;
-            B.tbits[v] = tb;
             const uint32_t e0 = epos + inc - nc;
             for (uint32_t c = 0; c < nc; c++) {
                 B.ent[2 * (e0 + c)] = v;
@@ -489,45 +474,35 @@ __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
 }
 
-// Each entry inserts its chunk of cN(u) (alive members only) into u's table.
-__global__ void k_cf_tinsert(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state, const CoreDev* __restrict__ st) {
-    if (st->done) return;
-    const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
-    const uint32_t nent = f->nent;
-    const uint32_t lane = lane_id();
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
-    for (uint64_t ei = wave0; ei < nent; ei += nwaves) {
-        const uint32_t u = B.ent[2 * ei], chunk = B.ent[2 * ei + 1];
-        const uint32_t pb = B.cbeg[u] + chunk * CORE_CHUNK, pe = min(B.cend[u], pb + CORE_CHUNK);
-        const uint32_t tb = B.tbits[u], tsz = 1u << tb;
-        uint32_t* tab = B.tables + B.toff[u];
-        for (uint32_t p = pb + lane; p < pe; p += 64) {
-            const uint32_t x = adj[p];
-            if (state[x] == 0) continue;
-            uint32_t s = hash_slot_u32(x, tb);
-            for (uint32_t probe = 0; probe < tsz; probe++) {
-                if (atomicCAS(&tab[s], GA_EMPTY, x) == GA_EMPTY) break;
-                s = (s + 1) & (tsz - 1);
-            }
-        }
-    }
-}
-
 // Triangles {u, v, w} (w > v) through entry (u, chunk): subtract once, by the smallest-id frontier member.
 // One wave per entry, 64 neighbours v at a time. For each pair the SHORTER of cN(u) after v and cN(v) after v is
-// streamed and its elements are tested for membership on the other side:
-//   in N(u): the LDS hash set of cN(u) when |cN(u)| <= CORE_HCAP, else u's membership bitmap
-//   in N(v): v's membership bitmap when v has one (original degree > CORE_HCAP), else binary search in cN(v)
+// streamed and its elements are tested for membership in the other endpoint's ORIGINAL neighbour list: the hub
+// bitmap (original degree > CORE_HCAP) or the static 4-slot-bucket table built once per state (cf_members). A member
+// that has since been removed reads state 0 and is skipped, so the static sets answer like sets of alive neighbours.
 // Streaming follows the count kernel: lists compacted in LDS longest first, read by sub-groups sized to their length,
 // 4 loads in flight per lane. A hit w is a triangle {u, v, w}; its alive corners lose one triangle each.
 #define CORE_HCAP 2048u
-#define CORE_TSLOTS (2u * CORE_HCAP)
 
 __device__ __forceinline__ bool bit_test(const uint32_t* __restrict__ bm, uint32_t w) { return (bm[w >> 5] >> (w & 31)) & 1u; }
 
+// w in the original neighbour list of the vertex whose membership reference is (rx, ry)?
+__device__ __forceinline__ bool cf_member(const CoreBufs& B, uint32_t rx, uint32_t ry, uint32_t w) {
+    if (ry >> 31) return bit_test(B.bitmaps + (size_t)rx * B.bm_words, w);
+    const uint32_t lb = ry & 31u, nb = 1u << lb;
+    const uint4* __restrict__ t = B.mtab + rx;
+    uint32_t b = lb ? (w * 2654435769u) >> (32u - lb) : 0u;
+    for (uint32_t i = 0; i < nb; i++) {   // buckets fill front to back: a bucket with a free last slot ends the probe
+        const uint4 q = t[b];
+        if (q.x == w || q.y == w || q.z == w || q.w == w) return true;
+        if (q.w == GA_EMPTY) return false;
+        b = (b + 1) & (nb - 1);
+    }
+    return false;
+}
+
 __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
                                                   uint32_t* __restrict__ tri, const CoreDev* __restrict__ st) {
-    __shared__ uint32_t stg[7 * 64];   // ps | len | v | sv + (mode << 8) | lo | hi | hub
+    __shared__ uint32_t stg[6 * 64];   // ps | len | v | sv + (mode << 8) | ref.x | ref.y
     if (st->done) return;
     const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
     const uint32_t nent = f->nent;
@@ -537,18 +512,17 @@ __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev
         const uint32_t ub = B.cbeg[u], ue = B.cend[u];
         const uint32_t pb = ub + chunk * CORE_CHUNK;
         const uint32_t pe = min(ue, pb + CORE_CHUNK);
-        const uint32_t tbits = B.tbits[u], tsz = 1u << tbits;
-        const uint32_t* __restrict__ tab = B.tables + B.toff[u];
+        const uint2 uref = B.mref[u];
         for (uint32_t base = pb; base < pe; base += 64) {
             const uint32_t p = base + lane;
-            uint32_t v = 0, sv = 0, ps = 0, len = 0, mode = 0, lo = 0, hi = 0, hub = GA_EMPTY;
+            uint32_t v = 0, sv = 0, ps = 0, len = 0, mode = 0;
+            uint2 vref = make_uint2(0u, 0u);
             if (p < pe) {
                 v = adj[p];
                 sv = state[v];
                 if (sv) {
-                    lo = B.cgt[v];                                // cN(v) after v
-                    hi = B.cend[v];
-                    hub = B.hubid[v];
+                    const uint32_t lo = B.cgt[v], hi = B.cend[v];   // cN(v) after v
+                    vref = B.mref[v];
                     const uint32_t a = ue - (p + 1), bl = hi - lo;
                     if (bl <= a) { ps = lo; len = bl; mode = 0; }   // stream cN(v) after v, test in N(u)
                     else { ps = p + 1; len = a; mode = 1; }         // stream cN(u) after v, test in N(v)
@@ -562,9 +536,8 @@ __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev
                 stg[64 + r] = len;
                 stg[128 + r] = v;
                 stg[192 + r] = sv | (mode << 8);
-                stg[256 + r] = lo;
-                stg[320 + r] = hi;
-                stg[384 + r] = hub;
+                stg[256 + r] = vref.x;
+                stg[320 + r] = vref.y;
             }
             group_sync_wave();
             const uint32_t lk = lane < cnt ? stg[64 + lane] : 0u;
@@ -582,9 +555,9 @@ __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev
                         kv = stg[128 + k];
                         const uint32_t sm = stg[192 + k];
                         ksv = sm & 0xFF;
-                        const uint32_t kmode = sm >> 8;
-                        const uint32_t klo = stg[256 + k], khi = stg[320 + k], khub = stg[384 + k];
-                        const uint32_t* __restrict__ vbm = khub != GA_EMPTY ? B.bitmaps + (size_t)khub * B.bm_words : nullptr;
+                        // the other endpoint's membership reference: u's when streaming cN(v), v's when streaming cN(u)
+                        const bool test_u = (sm >> 8) == 0;
+                        const uint32_t rx = test_u ? uref.x : stg[256 + k], ry = test_u ? uref.y : stg[320 + k];
                         for (uint32_t off = r; off < klen; off += U * W) {
                             uint32_t x[U];
 #pragma unroll
@@ -593,24 +566,7 @@ __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev
                             for (uint32_t q = 0; q < U; q++) {
                                 if (off + q * W >= klen) break;
                                 const uint32_t w = x[q];
-                                bool hit = false;
-                                if (kmode == 0) {            // w in N(u)?
-                                    uint32_t s = hash_slot_u32(w, tbits);
-                                    for (uint32_t probe = 0; probe < tsz; probe++) {
-                                        const uint32_t t = tab[s];
-                                        if (t == w) { hit = true; break; }
-                                        if (t == GA_EMPTY) break;
-                                        s = (s + 1) & (tsz - 1);
-                                    }
-                                } else {                      // w in N(v)?  (w > v by construction)
-                                    if (vbm) {
-                                        hit = bit_test(vbm, w);
-                                    } else {
-                                        const uint32_t qi = lower_bound_u32(adj, klo, khi, w);
-                                        hit = qi < khi && adj[qi] == w;
-                                    }
-                                }
-                                if (!hit) continue;
+                                if (!cf_member(B, rx, ry, w)) continue;
                                 const uint8_t sw = state[w];
                                 if (sw == 0) continue;
                                 if ((ksv == 2 && kv < u) || (sw == 2 && w < u)) continue;   // a smaller frontier member owns it
@@ -661,7 +617,52 @@ __global__ void k_cf_hub_bits(const uint32_t* __restrict__ urow, const uint32_t*
     }
 }
 
-// Round tail, one launch: retire the frontier, clear the table arena this round used, take the next round's min over
+// Membership tables of the other vertices (1 <= original degree <= CORE_HCAP): 2^lb buckets of 4 slots with
+// lb = max(ceil(log2 d) - 1, 0), so at most half of the slots are used. mref[v] = {first bucket, lb}; hubs get
+// {bitmap index, 1 << 31}. nbk[v] = buckets of v (exclusive-scanned into the first buckets by the caller).
+__device__ __forceinline__ uint32_t cf_table_log2(uint32_t d) {
+    const uint32_t c = d > 1 ? 32u - (uint32_t)__clz(d - 1) : 0u;   // ceil(log2 d)
+    return c ? c - 1u : 0u;
+}
+
+__global__ void k_cf_table_sizes(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ hubid, uint32_t n, uint32_t* __restrict__ nbk) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v <= n; v += gridDim.x * blockDim.x) {
+        const uint32_t d = v < n ? urow[v + 1] - urow[v] : 0u;
+        nbk[v] = (d == 0 || hubid[v] != GA_EMPTY) ? 0u : 1u << cf_table_log2(d);
+    }
+}
+
+__global__ void k_cf_table_refs(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ hubid, const uint32_t* __restrict__ boff,
+                                uint32_t n, uint2* __restrict__ mref) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const uint32_t h = hubid[v];
+        mref[v] = h != GA_EMPTY ? make_uint2(h, 1u << 31) : make_uint2(boff[v], cf_table_log2(urow[v + 1] - urow[v]));
+    }
+}
+
+// One wave per vertex inserts its original neighbours (no duplicates in the undirected CSR) into its table: slots of a
+// bucket are claimed in order, so a bucket fills front to back; a full bucket passes on to the next one.
+__global__ void k_cf_table_fill(const uint32_t* __restrict__ urow, const uint32_t* __restrict__ ucol, const uint2* __restrict__ mref,
+                                uint32_t n, uint32_t* __restrict__ mtab) {
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    for (uint64_t v = wave0; v < n; v += nwaves) {
+        const uint2 ref = mref[v];
+        if (ref.y >> 31) continue;
+        const uint32_t lb = ref.y, nb = 1u << lb;
+        uint32_t* __restrict__ t = mtab + 4ull * ref.x;
+        for (uint32_t p = urow[v] + lane_id(); p < urow[v + 1]; p += 64) {
+            const uint32_t w = ucol[p];
+            uint32_t b = lb ? (w * 2654435769u) >> (32u - lb) : 0u;
+            bool placed = false;
+            for (uint32_t i = 0; i < nb && !placed; i++) {
+                for (uint32_t q = 0; q < 4 && !placed; q++) placed = atomicCAS(&t[4 * b + q], GA_EMPTY, w) == GA_EMPTY;
+                b = (b + 1) & (nb - 1);
+            }
+        }
+    }
+}
+
+// Round tail, one launch: retire the frontier, take the next round's min over
 // the alive list and, when the alive list halves, compact the list and the adjacency into the other buffers (alive =
 // state 1; frontier members read as 2 or 0, both excluded). The last workgroup to finish (atomic ticket, no waiting)
 // applies the round's end rules (dsg.h:744-747) and switches to the compacted buffers.
@@ -670,13 +671,12 @@ __global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ sta
     __shared__ uint32_t bmin;
     if (st->done) return;
     if (threadIdx.x == 0) bmin = 0xFFFFFFFFu;
-    const uint32_t nfv = f->nfv, used = f->tcursor, na = f->na;
+    const uint32_t nfv = f->nfv, na = f->na;
     const unsigned long long nalive = st->nalive;
     const bool compact = nfv < nalive && 2ull * (nalive - nfv) <= na;   // the same decision in every workgroup
     const uint32_t lane = lane_id();
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nthr = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = tid; i < nfv; i += nthr) state[B.fvert[i]] = 0;
-    for (uint64_t i = tid; i < used; i += nthr) B.tables[i] = GA_EMPTY;
     const uint32_t* __restrict__ al = B.alist[f->alist_cur];
     uint32_t* __restrict__ out = B.alist[f->alist_cur ^ 1u];
     const uint64_t wave0 = tid / 64, nwaves = nthr / 64;
@@ -750,7 +750,6 @@ __global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ sta
     f->min_next = 0xFFFFFFFFu;
     f->nfv = 0;
     f->nent = 0;
-    f->tcursor = 0;
     if (compact) {
         f->na = atomicAdd(&f->na_new, 0u);
         f->alist_cur ^= 1u;
@@ -766,56 +765,72 @@ struct CoreFusedMem {
     void* block = nullptr;
     void* hubs = nullptr;
     void* block_bm = nullptr;
+    void* block_tab = nullptr;
     CoreBufs B{};
     CoreFusedDev* f = nullptr;
-    size_t arena_words = 0;
     ~CoreFusedMem() {
         ga_free(block);
         ga_free(hubs);
         ga_free(block_bm);
+        ga_free(block_tab);
     }
 };
 
 static void cf_release(CoreFusedMem* m) { delete m; }
 
-static int cf_hubs(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
+// The membership structures of the original undirected lists, built once per state: bitmaps for hubs, 4-slot-bucket
+// tables for the other vertices (about 2-4 slots per listed neighbour).
+static int cf_members(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     const size_t n = g->n;
     void* idb = nullptr;
-    GA_TRY(ga_malloc(ctx, &idb, (n + 1) * 4 + 64));
+    // hubid (n+1) | counter (16) | bucket counts (n+1) | first buckets (n+1) | pad | mref (2n, 8-byte aligned)
+    GA_TRY(ga_malloc(ctx, &idb, (5 * n + 24) * 4 + 64));
+    M.hubs = idb;
     uint32_t* hubid = (uint32_t*)idb;
     uint32_t* cnt = hubid + n + 1;
+    uint32_t* nbk = cnt + 16;
+    uint32_t* boff = nbk + n + 1;
+    uint2* mref = (uint2*)(((uintptr_t)(boff + n + 1) + 7) & ~(uintptr_t)7);
     GA_TRY(ga_fill_u32(ctx, cnt, 1, 0));
     k_cf_hub_ids<<<cgrid(ctx, n), 256, 0, ctx->stream>>>(g->urow, g->n, hubid, cnt);
-    int rc = ga_launch_check(ctx, "k_cf_hub_ids");
-    if (!rc && hipMemcpyAsync(ctx->hstat, cnt, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
-    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
-    if (rc) { ga_free(idb); return ga_fail(ctx, rc ? rc : GA_ERR_HIP, "hub detection failed"); }
-    const size_t nh = ctx->hstat[0] & 0xFFFFFFFFu;
+    GA_TRY(ga_launch_check(ctx, "k_cf_hub_ids"));
+    k_cf_table_sizes<<<cgrid(ctx, n + 1), 256, 0, ctx->stream>>>(g->urow, hubid, g->n, nbk);
+    GA_TRY(ga_launch_check(ctx, "k_cf_table_sizes"));
+    GA_TRY(ga_exclusive_scan_u32(ctx, nbk, boff, n + 1));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat + 1, boff + n, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const size_t nh = ctx->hstat[0] & 0xFFFFFFFFu, nbuckets = ctx->hstat[1] & 0xFFFFFFFFu;
+    if (nbuckets > 2ull * g->m) return ga_fail(ctx, GA_ERR_STATE, "membership table sizes exceed the edge count");
     const size_t words = (n + 31) / 32;
-    M.hubs = idb;
-    M.B.hubid = hubid;
     M.B.bm_words = words;
     M.B.bitmaps = nullptr;
     if (nh) {
-        void* bm = nullptr;
-        if (ga_malloc(ctx, &bm, nh * words * 4)) return GA_ERR_OOM;
-        // keep both allocations owned by M: bitmaps live in their own block
-        M.block_bm = bm;
-        GA_TRY(ga_fill_u32(ctx, (uint32_t*)bm, nh * words, 0));
-        k_cf_hub_bits<<<std::min<uint32_t>(g->n, 65535), 256, 0, ctx->stream>>>(g->urow, g->ucol, hubid, g->n, (uint32_t*)bm, words);
+        GA_TRY(ga_malloc(ctx, &M.block_bm, nh * words * 4));
+        GA_TRY(ga_fill_u32(ctx, (uint32_t*)M.block_bm, nh * words, 0));
+        k_cf_hub_bits<<<std::min<uint32_t>(g->n, 65535), 256, 0, ctx->stream>>>(g->urow, g->ucol, hubid, g->n, (uint32_t*)M.block_bm, words);
         GA_TRY(ga_launch_check(ctx, "k_cf_hub_bits"));
-        M.B.bitmaps = (const uint32_t*)bm;
+        M.B.bitmaps = (const uint32_t*)M.block_bm;
     }
+    GA_TRY(ga_malloc(ctx, &M.block_tab, std::max<size_t>(nbuckets, 1) * 16));
+    GA_TRY(ga_fill_u32(ctx, (uint32_t*)M.block_tab, 4 * std::max<size_t>(nbuckets, 1), GA_EMPTY));
+    k_cf_table_refs<<<cgrid(ctx, n), 256, 0, ctx->stream>>>(g->urow, hubid, boff, g->n, mref);
+    GA_TRY(ga_launch_check(ctx, "k_cf_table_refs"));
+    if (n) {
+        k_cf_table_fill<<<grid_for(64ull * n, 256, 65535), 256, 0, ctx->stream>>>(g->urow, g->ucol, mref, g->n, (uint32_t*)M.block_tab);
+        GA_TRY(ga_launch_check(ctx, "k_cf_table_fill"));
+    }
+    M.B.mref = mref;
+    M.B.mtab = (const uint4*)M.block_tab;
     return GA_OK;
 }
 
 static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     const size_t n = g->n, m2 = 2ull * g->m;
     const size_t nent = n + m2 / CORE_CHUNK + 16;
-    const size_t arena = 4 * std::max<size_t>(m2, 1) + 4 * n + 64;   // tables: at most 4 slots per listed neighbour (+4 per vertex)
-    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 3 * n + n + 2 * nent + 2 * n + arena + 64 + 16;
-    if (arena >= 0xFFFFFFFFull || nent >= 0x7FFFFFFFull)
-        return ga_fail(ctx, GA_ERR_ARG, "graph too large for the fused core decomposition (32-bit table offsets)");
+    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 3 * n + n + 2 * nent + 64 + 16;
+    if (m2 >= 0xFFFFFFFFull || nent >= 0x7FFFFFFFull)
+        return ga_fail(ctx, GA_ERR_ARG, "graph too large for the fused core decomposition (32-bit offsets)");
     GA_TRY(ga_malloc(ctx, &M.block, words * 4));
     uint32_t* p = (uint32_t*)M.block;
     M.B.alist[0] = p; p += n;
@@ -829,10 +844,6 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     M.B.cgt = p; p += n;
     M.B.fvert = p; p += n;
     M.B.ent = p; p += 2 * nent;
-    M.B.toff = p; p += n;
-    M.B.tbits = p; p += n;
-    M.B.tables = p; p += arena;
-    M.arena_words = arena;
     p = (uint32_t*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
     M.f = (CoreFusedDev*)p;
     return GA_OK;
@@ -850,7 +861,7 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     if (!c->fm) {   // allocated once per core object: later runs allocate and free nothing
         CoreFusedMem* fm = new CoreFusedMem;
         int rc = cf_alloc(ctx, g, *fm);
-        if (!rc) rc = cf_hubs(ctx, g, *fm);
+        if (!rc) rc = cf_members(ctx, g, *fm);
         if (rc) { delete fm; return rc; }
         c->fm = fm;
     }
@@ -861,7 +872,6 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     GA_TRY(ga_launch_check(ctx, "k_core_init"));
     k_cf_init<<<cgrid(ctx, g->n), 256, 0, ctx->stream>>>(M.f, M.B.alist[0], M.B.cbeg, M.B.cend, M.B.cgt, g->urow, g->ucol, g->n);
     GA_TRY(ga_launch_check(ctx, "k_cf_init"));
-    GA_TRY(ga_fill_u32(ctx, M.B.tables, M.arena_words, GA_EMPTY));   // each round's tail clears what it used
     if (g->n) {
         GA_TRY(init_tri_async(c));
         k_core_check_T<<<1, 1, 0, ctx->stream>>>(c->st);
@@ -881,9 +891,8 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
     k_cf_min<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
     GA_TRY(ga_launch_check(ctx, "k_cf_min"));
     for (uint64_t guard = 0;; guard++) {
-        for (int r = 0; r < block_rounds; r++) {   // 4 launches per round
+        for (int r = 0; r < block_rounds; r++) {   // 3 launches per round
             k_cf_find<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->core, c->st);
-            k_cf_tinsert<<<gu, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->st);
             k_cf_update<<<gu * 4, 64, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
             k_cf_tail<<<gt, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
         }

@@ -86,9 +86,9 @@ def peel_bytes(n: int) -> int:
 def core_bytes(n: int, m: int) -> int:
     m2 = max(2 * m, 1)
     nent = n + m2 // 8 + 16
-    arena = 4 * m2 + 4 * n + 64
-    words = 2 * n + 2 * m2 + 3 * n + n + 2 * nent + 2 * n + arena + 80
-    return 4 * (n + 1) + 4 * m2 + 13 * n + 4 * words + 4096
+    words = 6 * n + 2 * m2 + 2 * nent + 80            # alive lists, adjacency copies, ranges, frontier, entries
+    members = 4 * (5 * n + 24) + 64 + 16 * m2          # hub ids and table references; tables (at most m2 buckets)
+    return 4 * (n + 1) + 4 * m2 + 13 * n + 4 * words + members + 4096
 
 
 class _Resident:

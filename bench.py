@@ -4,7 +4,7 @@ the reference's CPU code (count, batch peel, Alg 3 core) timed on this machine's
 threads printed).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw_communities|rmat21|powerlaw_small]
-                    [--batch-graphs 4096] [--serve-seconds 1.0] [--serve-rate 2000] [--no-cpu] [--no-tune]
+                    [--batch-graphs 4096] [--serve-seconds 1.0] [--serve-rates 250,500,1000,2000] [--no-cpu] [--no-tune]
 
 Count step: one full exact count T = sum over oriented edges (u,v) of |N+(u) ∩ N+(v)| of the resident graph (CSR
 already in HBM). The graph is generated from a fixed seed, built on the GPU (de-dup, orientation, sort; timed
@@ -33,10 +33,10 @@ Extras (same workload): load_graph with the text parse on the GPU and the binary
 ceil(kmax-core density) on the device core numbers, each checked against the oracle and timed beside the oracle's CPU
 restatement on one host core.
 
-Serving: Poisson arrivals (2,000 req/s for 1 s by default) of count / peel (eps 1/20) / core requests, 95% on a pool
-of 256 graphs of the batch size law and 5% on two R-MAT scale-20 graphs, all resident; p50 / p99 latency (arrival to
-result) and sustained throughput of the continuous-batching server vs the plain one-request-at-a-time form on the same
-arrivals, whose results must be identical.
+Serving: a load sweep of Poisson arrivals (250, 500, 1,000 and 2,000 req/s, 1 s each by default) of count / peel
+(eps 1/20) / core requests, 95% on a pool of 256 graphs of the batch size law and 5% on two R-MAT scale-20 graphs, all
+resident; at each rate, p50 / p99 latency (arrival to result) and throughput of the continuous-batching server vs the
+plain one-request-at-a-time form on the same arrivals, whose results must be identical.
 """
 from __future__ import annotations
 
@@ -364,26 +364,38 @@ def run_batch(ctx, count, steps, tune):
     return res
 
 
-def run_serving(rate, duration):
-    """Continuous batching (16 worker streams, count batching, footprint admission) vs the plain form (one request at
-    a time, one count per launch) on the same Poisson arrivals; every result must be identical in both."""
+def run_serving(rates, duration):
+    """Load sweep: at each offered rate, the same Poisson arrivals served by continuous batching (16 worker streams,
+    count batching, shared-launch core batches, footprint admission) and by the plain form (one request at a time, one
+    count per launch, one decomposition at a time); every result must be identical in both forms."""
     from graph_analyzer_amd import serve
-    source, reqs = serve.mixed_workload(rate=rate, duration=duration, seed=2106, pool=256)
-    keys = sorted({r.key for r in reqs})
-    out = {"workload": f"poisson_{rate:g}rps_{duration:g}s", "offered_rps": rate, "requests": len(reqs),
-           "mix": "count / peel eps 1/20 / core uniform; 95% on 256 graphs of the batch size law, 5% on 2 R-MAT scale-20 graphs"}
-    for name, slots, cb in (("continuous", 16, 64), ("plain", 1, 1)):
-        s = serve.Server(source, slots=slots, count_batch=cb, states_per_graph=4 if slots > 1 else 1)
+    source = serve.workload_source(seed=2106, pool=256, rmat_scale=20)
+    sweep = {rate: serve.poisson_requests(rate, duration, seed=2106, pool=256) for rate in rates}
+    keys = sorted({q.key for reqs in sweep.values() for q in reqs})
+    out = {"workload": f"poisson_{duration:g}s_per_rate", "duration_s": duration,
+           "mix": "count / peel eps 1/20 / core uniform; 95% on 256 graphs of the batch size law, 5% on 2 R-MAT scale-20 graphs",
+           "sweep": []}
+    forms = (("continuous", dict(slots=16, count_batch=64, core_batch=1024, states_per_graph=16)),
+             ("plain", dict(slots=1, count_batch=1, core_batch=1, states_per_graph=1)))
+    rows = {rate: {"offered_rps": rate, "requests": len(sweep[rate])} for rate in rates}
+    values = {}
+    for name, kw in forms:
+        s = serve.Server(source, **kw)
         t0 = time.perf_counter()
         s.preload(keys)
         log(f"serving ({name}): {len(keys)} graphs resident in {time.perf_counter() - t0:.1f} s, {s.used / 1e9:.1f} GB")
-        res = s.run(reqs)
+        for rate in rates:
+            res = s.run(sweep[rate])
+            summ = serve.summarize(res)
+            values[(name, rate)] = [r.value for r in res]
+            rows[rate][name] = {k: summ[k] for k in ("throughput_rps", "p50_ms", "p99_ms", "makespan_s", "by_kind")}
+            log(f"serving ({name}) at {rate:g} req/s: {summ['throughput_rps']:.0f} req/s, p50 {summ['p50_ms']:.1f} ms, "
+                f"p99 {summ['p99_ms']:.1f} ms, makespan {summ['makespan_s']:.2f} s")
         s.close()
-        out[name] = serve.summarize(res)
-        out[name]["values"] = [r.value for r in res]
-        log(f"serving ({name}): {out[name]['throughput_rps']:.0f} req/s, p50 {out[name]['p50_ms']:.1f} ms, p99 {out[name]['p99_ms']:.1f} ms")
-    if out["continuous"].pop("values") != out["plain"].pop("values"):
-        raise SystemExit("MISMATCH: a serving result differs between the continuous and the plain (alone) runs")
+    for rate in rates:
+        if values[("continuous", rate)] != values[("plain", rate)]:
+            raise SystemExit(f"MISMATCH: a serving result at {rate:g} req/s differs between the continuous and the plain runs")
+        out["sweep"].append(rows[rate])
     out["results_identical_alone_and_in_mix"] = True
     return out
 
@@ -397,7 +409,7 @@ def main():
     ap.add_argument("--variant", default="hash")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="graphs in the batch comparison (0: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="seconds of Poisson arrivals in the serving comparison (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=2000.0, help="offered requests per second in the serving comparison")
+    ap.add_argument("--serve-rates", default="250,500,1000,2000", help="offered requests per second of the serving load sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the ingest and prune measurements")
@@ -528,7 +540,7 @@ def main():
         del og
     serving = None
     if args.serve_seconds > 0:
-        serving = run_serving(args.serve_rate, args.serve_seconds)
+        serving = run_serving([float(x) for x in args.serve_rates.split(",")], args.serve_seconds)
     out = {
         "metric": "oriented edges/s (exact triangle count, one resident graph); batched graphs/s vs per-graph launch",
         "value": value,

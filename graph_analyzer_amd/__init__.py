@@ -125,6 +125,7 @@ def lib() -> ctypes.CDLL:
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
             "ga_peel_set_ctx": (ctypes.c_int, [_vp, _vp]),
             "ga_core_run_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_core_run_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_core_set_ctx": (ctypes.c_int, [_vp, _vp]),
             "ga_tc_prepare": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(TcTuning)]),
@@ -751,6 +752,24 @@ def core_many(ctx: "Context", cores: Sequence["Core"], T_full: Optional[Sequence
     ms = ctypes.c_float()
     ctx.check(lib().ga_core_run_many(ctx.handle, hs, len(cores), None if tf is None else _ptr(tf, _u64p), _ptr(stats, _u32p),
                                      ctypes.byref(ms)), "core_run_many")
+    return [(c.core(), int(stats[2 * i]), int(stats[2 * i + 1])) for i, c in enumerate(cores)], ms.value
+
+
+def core_batch(ctx: "Context", cores: Sequence["Core"], T_full: Optional[Sequence[Optional[int]]] = None):
+    """Fused incremental core decompositions of several resident graphs in shared launches (ga_core_run_batch: every
+    round's three launches cover all unfinished states). The states are moved to ctx first. Returns
+    ([(core, rounds, kmax) per state], device ms)."""
+    for c in cores:
+        if c.ctx is not ctx:
+            c.set_ctx(ctx)
+    hs = (_vp * max(len(cores), 1))(*[c._h for c in cores])
+    tf = None
+    if T_full is not None:
+        tf = np.array([(1 << 64) - 1 if t is None else int(t) for t in T_full], np.uint64)
+    stats = np.zeros(max(2 * len(cores), 2), np.uint32)
+    ms = ctypes.c_float()
+    ctx.check(lib().ga_core_run_batch(ctx.handle, hs, len(cores), None if tf is None else _ptr(tf, _u64p), _ptr(stats, _u32p),
+                                      ctypes.byref(ms)), "core_run_batch")
     return [(c.core(), int(stats[2 * i]), int(stats[2 * i + 1])) for i, c in enumerate(cores)], ms.value
 
 

@@ -12,6 +12,10 @@
 // ga_core_run keeps k / nalive / rounds on the device (CoreDev) and enqueues rounds in blocks, one readback per block.
 #include "ga_internal.hpp"
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 __device__ __forceinline__ void group_sync_wave() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -421,15 +425,16 @@ __global__ void k_cf_min(CoreBufs B, const CoreFusedDev* __restrict__ f, const u
 
 // Frontier = alive v with tri <= k; appends v to fvert and, if it is in a triangle, its chunk entries (one atomic per
 // wave for each).
-__global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restrict__ tri, uint8_t* __restrict__ state,
-                          uint32_t* __restrict__ core, CoreDev* st) {
+__device__ __forceinline__ void cf_find_body(const CoreBufs& B, CoreFusedDev* f, const uint32_t* __restrict__ tri,
+                                             uint8_t* __restrict__ state, uint32_t* __restrict__ core, CoreDev* st, uint32_t bid,
+                                             uint32_t nb) {
     if (st->done) return;
     const unsigned long long mt = st->min_tri, kp = st->k_prev;
     const unsigned long long k = mt > kp ? mt : kp;
     const uint32_t* __restrict__ al = B.alist[f->alist_cur];
     const uint32_t na = f->na;
     const uint32_t lane = lane_id();
-    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)gridDim.x * blockDim.x) / 64;
+    const uint64_t wave0 = ((uint64_t)bid * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)nb * blockDim.x) / 64;
     for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
         const uint64_t i = base + lane;
         bool join = false;
@@ -471,7 +476,12 @@ __global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restric
             }
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->k = k;
+    if (bid == 0 && threadIdx.x == 0) st->k = k;
+}
+
+__global__ void k_cf_find(CoreBufs B, CoreFusedDev* f, const uint32_t* __restrict__ tri, uint8_t* __restrict__ state,
+                          uint32_t* __restrict__ core, CoreDev* st) {
+    cf_find_body(B, f, tri, state, core, st, blockIdx.x, gridDim.x);
 }
 
 // Triangles {u, v, w} (w > v) through entry (u, chunk): subtract once, by the smallest-id frontier member.
@@ -500,14 +510,15 @@ __device__ __forceinline__ bool cf_member(const CoreBufs& B, uint32_t rx, uint32
     return false;
 }
 
-__global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
-                                                  uint32_t* __restrict__ tri, const CoreDev* __restrict__ st) {
-    __shared__ uint32_t stg[6 * 64];   // ps | len | v | sv + (mode << 8) | ref.x | ref.y
+// stg: 6 * 64 words of LDS (ps | len | v | sv + (mode << 8) | ref.x | ref.y); one 64-lane workgroup
+__device__ __forceinline__ void cf_update_body(const CoreBufs& B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
+                                               uint32_t* __restrict__ tri, const CoreDev* __restrict__ st, uint32_t bid, uint32_t nb,
+                                               uint32_t* __restrict__ stg) {
     if (st->done) return;
     const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
     const uint32_t nent = f->nent;
     const uint32_t lane = threadIdx.x;
-    for (uint32_t ei = blockIdx.x; ei < nent; ei += gridDim.x) {
+    for (uint32_t ei = bid; ei < nent; ei += nb) {
         const uint32_t u = B.ent[2 * ei], chunk = B.ent[2 * ei + 1];
         const uint32_t ub = B.cbeg[u], ue = B.cend[u];
         const uint32_t pb = ub + chunk * CORE_CHUNK;
@@ -585,6 +596,12 @@ __global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev
             group_sync_wave();
         }
     }
+}
+
+__global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
+                                                  uint32_t* __restrict__ tri, const CoreDev* __restrict__ st) {
+    __shared__ uint32_t stg[6 * 64];
+    cf_update_body(B, f, state, tri, st, blockIdx.x, gridDim.x, stg);
 }
 
 // Membership bitmaps of the vertices whose ORIGINAL degree exceeds CORE_HCAP: hubid[v] = index or GA_EMPTY.
@@ -666,16 +683,18 @@ __global__ void k_cf_table_fill(const uint32_t* __restrict__ urow, const uint32_
 // the alive list and, when the alive list halves, compact the list and the adjacency into the other buffers (alive =
 // state 1; frontier members read as 2 or 0, both excluded). The last workgroup to finish (atomic ticket, no waiting)
 // applies the round's end rules (dsg.h:744-747) and switches to the compacted buffers.
-__global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ state, const uint32_t* __restrict__ tri, CoreDev* st) {
-    __shared__ uint32_t last;
-    __shared__ uint32_t bmin;
+// sh: 2 words of LDS (last, bmin)
+__device__ __forceinline__ void cf_tail_body(const CoreBufs& B, CoreFusedDev* f, uint8_t* __restrict__ state, const uint32_t* __restrict__ tri,
+                                             CoreDev* st, uint32_t bid, uint32_t nb, uint32_t* sh) {
+    uint32_t& last = sh[0];
+    uint32_t& bmin = sh[1];
     if (st->done) return;
     if (threadIdx.x == 0) bmin = 0xFFFFFFFFu;
     const uint32_t nfv = f->nfv, na = f->na;
     const unsigned long long nalive = st->nalive;
     const bool compact = nfv < nalive && 2ull * (nalive - nfv) <= na;   // the same decision in every workgroup
     const uint32_t lane = lane_id();
-    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, nthr = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)bid * blockDim.x + threadIdx.x, nthr = (uint64_t)nb * blockDim.x;
     for (uint64_t i = tid; i < nfv; i += nthr) state[B.fvert[i]] = 0;
     const uint32_t* __restrict__ al = B.alist[f->alist_cur];
     uint32_t* __restrict__ out = B.alist[f->alist_cur ^ 1u];
@@ -734,7 +753,7 @@ __global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ sta
     if (threadIdx.x == 0) {
         if (bmin != 0xFFFFFFFFu) atomicMin(&f->min_next, bmin);   // one global atomic per workgroup
         __threadfence();
-        last = atomicAdd(&f->ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
+        last = atomicAdd(&f->ticket, 1u) == nb - 1 ? 1u : 0u;
     }
     __syncthreads();
     if (!last || threadIdx.x != 0) return;
@@ -758,6 +777,11 @@ __global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ sta
     f->na_new = 0;
     f->adj_cursor = 0;
     if (st->nalive == 0) st->done = 1;
+}
+
+__global__ void k_cf_tail(CoreBufs B, CoreFusedDev* f, uint8_t* __restrict__ state, const uint32_t* __restrict__ tri, CoreDev* st) {
+    __shared__ uint32_t sh[2];
+    cf_tail_body(B, f, state, tri, st, blockIdx.x, gridDim.x, sh);
 }
 
 struct CoreFusedMem {
@@ -849,24 +873,23 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     return GA_OK;
 }
 
-// Whole decomposition with the round loop on the device (incremental design with compaction; the recount design is
-// available through the per-round calls).
-extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms) {
+// First use of a state in the fused loop: its buffers and membership structures (kept for later runs).
+static int cf_ensure(ga_core* c) {
+    if (c->fm) return GA_OK;
+    CoreFusedMem* fm = new CoreFusedMem;
+    int rc = cf_alloc(c->ctx, c->g, *fm);
+    if (!rc) rc = cf_members(c->ctx, c->g, *fm);
+    if (rc) { delete fm; return rc; }
+    c->fm = fm;
+    return GA_OK;
+}
+
+// Enqueues the start of a fused decomposition: state reset, the one full participation count, the T check and round
+// 0's level candidate.
+static int cf_begin(ga_core* c, uint64_t T_full) {
     ga_ctx* ctx = c->ctx;
     ga_graph* g = c->g;
-    if (!c->incremental) return ga_fail(ctx, GA_ERR_ARG, "ga_core_run runs the incremental design");
-    if (T_full != UINT64_MAX && T_full >= (1ull << 32))
-        return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
-    GA_HIP(ctx, hipSetDevice(ctx->device));
-    if (!c->fm) {   // allocated once per core object: later runs allocate and free nothing
-        CoreFusedMem* fm = new CoreFusedMem;
-        int rc = cf_alloc(ctx, g, *fm);
-        if (!rc) rc = cf_members(ctx, g, *fm);
-        if (rc) { delete fm; return rc; }
-        c->fm = fm;
-    }
     CoreFusedMem& M = *c->fm;
-    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_core_reset_state(c));
     k_core_init<<<1, 1, 0, ctx->stream>>>(c->st, g->n, T_full);
     GA_TRY(ga_launch_check(ctx, "k_core_init"));
@@ -877,24 +900,58 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
         k_core_check_T<<<1, 1, 0, ctx->stream>>>(c->st);
         GA_TRY(ga_launch_check(ctx, "k_core_check_T"));
     }
+    k_cf_min<<<cgrid(ctx, g->n), 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
+    return ga_launch_check(ctx, "k_cf_min");
+}
+
+// Launch shapes of one decomposition's round for `alive` list entries: find / tail workgroups of 256 threads, update
+// workgroups of 64. Small graphs and late rounds launch few workgroups, which leaves the device to other work.
+struct CfGrid {
+    unsigned v, u, t;
+};
+
+static CfGrid cf_grid(ga_ctx* ctx, uint64_t alive) {
+    CfGrid r;
+    r.v = cgrid(ctx, std::max<uint64_t>(alive, 1));
+    r.u = 4u * std::min<unsigned>(ugrid(ctx), std::max<unsigned>(4u, (unsigned)std::min<uint64_t>((8 * alive + 255) / 256, 1u << 20)));
+    r.t = std::min<unsigned>(r.v, (unsigned)std::max(ctx->cus, 1) * 2u);   // few workgroups: few tickets
+    return r;
+}
+
+static int cf_result(ga_ctx* ctx, const CoreDev* h, uint32_t* stats, const char* who) {
+    switch (h->error) {
+        case 0: break;
+        case CORE_ERR_T: return ga_fail(ctx, GA_ERR_STATE, "%sdevice triangle count differs from the given count", who);
+        case CORE_ERR_BIG: return ga_fail(ctx, GA_ERR_ARITH, "%striangle count >= 2^32: uint32 tri[] counters could overflow", who);
+        default: return ga_fail(ctx, GA_ERR_STATE, "%sbad stats from device (removed 0, removed > alive, or level decreased)", who);
+    }
+    if (stats) {
+        stats[0] = h->rounds;
+        stats[1] = (uint32_t)h->k_prev;
+    }
+    return GA_OK;
+}
+
+// Whole decomposition with the round loop on the device (incremental design with compaction; the recount design is
+// available through the per-round calls).
+extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* ms) {
+    ga_ctx* ctx = c->ctx;
+    ga_graph* g = c->g;
+    if (!c->incremental) return ga_fail(ctx, GA_ERR_ARG, "ga_core_run runs the incremental design");
+    if (T_full != UINT64_MAX && T_full >= (1ull << 32))
+        return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    GA_TRY(cf_ensure(c));   // allocated once per core object: later runs allocate and free nothing
+    CoreFusedMem& M = *c->fm;
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_TRY(cf_begin(c, T_full));
     int block_rounds = 8;
-    // grids follow the alive count (read back after every block of rounds): small graphs and late rounds launch
-    // few workgroups, which leaves the device to other streams' work
-    unsigned gv = cgrid(ctx, g->n), gu = ugrid(ctx), gt = 0;
-    auto size_grids = [&](uint64_t alive) {
-        gv = cgrid(ctx, std::max<uint64_t>(alive, 1));
-        gu = std::min<unsigned>(ugrid(ctx), std::max<unsigned>(4u, (unsigned)((8 * alive + 255) / 256)));
-        gt = std::min<unsigned>(gv, (unsigned)std::max(ctx->cus, 1) * 2u);   // few workgroups: few tickets
-    };
-    size_grids(g->n);
-    // round 0's level candidate; later rounds get theirs from the previous round's tail
-    k_cf_min<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->st);
-    GA_TRY(ga_launch_check(ctx, "k_cf_min"));
+    CfGrid gr = cf_grid(ctx, g->n);
     for (uint64_t guard = 0;; guard++) {
         for (int r = 0; r < block_rounds; r++) {   // 3 launches per round
-            k_cf_find<<<gv, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->core, c->st);
-            k_cf_update<<<gu * 4, 64, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
-            k_cf_tail<<<gt, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
+            k_cf_find<<<gr.v, 256, 0, ctx->stream>>>(M.B, M.f, c->tri, c->state, c->core, c->st);
+            k_cf_update<<<gr.u, 64, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
+            k_cf_tail<<<gr.t, 256, 0, ctx->stream>>>(M.B, M.f, c->state, c->tri, c->st);
         }
         GA_TRY(ga_launch_check(ctx, "core round"));
         GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, c->st, sizeof(CoreDev), hipMemcpyDeviceToHost, ctx->stream));
@@ -903,21 +960,172 @@ extern "C" int ga_core_run(ga_core* c, uint64_t T_full, uint32_t* stats, float* 
         if (h->done) break;
         if (guard > (uint64_t)g->n + 16) return ga_fail(ctx, GA_ERR_STATE, "core decomposition did not finish");
         block_rounds = std::min(block_rounds * 2, 256);
-        size_grids(2 * h->nalive);   // the alive list is at most about twice the alive count (compaction at half)
+        gr = cf_grid(ctx, 2 * h->nalive);   // the alive list is at most about twice the alive count (compaction at half)
     }
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
     if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
-    const CoreDev* h = (const CoreDev*)ctx->hstat;
-    switch (h->error) {
-        case 0: break;
-        case CORE_ERR_T: return ga_fail(ctx, GA_ERR_STATE, "device triangle count differs from the given count");
-        case CORE_ERR_BIG: return ga_fail(ctx, GA_ERR_ARITH, "triangle count >= 2^32: uint32 tri[] counters could overflow");
-        default: return ga_fail(ctx, GA_ERR_STATE, "bad stats from device (removed 0, removed > alive, or level decreased)");
+    return cf_result(ctx, (const CoreDev*)ctx->hstat, stats, "");
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// many decompositions in shared launches: each round's find / update / tail launch covers every unfinished state
+// ---------------------------------------------------------------------------------------------------------------
+// A state owns a contiguous range of each launch's workgroups (base, nblk); map[] gives the state of every
+// workgroup. The kernels are the single-graph ones run on the state's own block range, so every state follows exactly
+// the rounds it follows alone. Ranges follow each state's alive count and are rebuilt after every block of rounds;
+// finished states get none.
+struct CoreJob {
+    CoreBufs B;
+    CoreFusedDev* f;
+    uint8_t* state;
+    uint32_t* tri;
+    uint32_t* core;
+    CoreDev* st;
+    uint32_t base[3], nblk[3];   // find, update, tail
+};
+
+__global__ void k_cfb_find(const CoreJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    const CoreJob* J = jobs + map[blockIdx.x];
+    cf_find_body(J->B, J->f, J->tri, J->state, J->core, J->st, blockIdx.x - J->base[0], J->nblk[0]);
+}
+
+__global__ void __launch_bounds__(64) k_cfb_update(const CoreJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    __shared__ uint32_t stg[6 * 64];
+    const CoreJob* J = jobs + map[blockIdx.x];
+    cf_update_body(J->B, J->f, J->state, J->tri, J->st, blockIdx.x - J->base[1], J->nblk[1], stg);
+}
+
+__global__ void k_cfb_tail(const CoreJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    __shared__ uint32_t sh[2];
+    const CoreJob* J = jobs + map[blockIdx.x];
+    cf_tail_body(J->B, J->f, J->state, J->tri, J->st, blockIdx.x - J->base[2], J->nblk[2], sh);
+}
+
+__global__ void k_cfb_stats(const CoreJob* __restrict__ jobs, uint32_t count, CoreDev* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) out[i] = *jobs[i].st;
+}
+
+extern "C" int ga_core_run_batch(ga_ctx* ctx, ga_core* const* cores, uint32_t count, const uint64_t* T_full, uint32_t* stats, float* ms) {
+    if (!ctx || (count && (!cores || !stats))) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    if (!count) return GA_OK;
+    if (count > (1u << 20)) return ga_fail(ctx, GA_ERR_ARG, "ga_core_run_batch: at most 2^20 states per call");
+    for (uint32_t i = 0; i < count; i++) {
+        ga_core* c = cores[i];
+        if (!c || !c->incremental || c->ctx != ctx)
+            return ga_fail(ctx, GA_ERR_ARG, "core %u: not an incremental state of this context (ga_core_set_ctx moves one)", i);
+        if (T_full && T_full[i] != UINT64_MAX && T_full[i] >= (1ull << 32))
+            return ga_fail(ctx, GA_ERR_ARITH, "core %u: triangle count >= 2^32: uint32 tri[] counters could overflow", i);
     }
-    if (stats) {
-        stats[0] = h->rounds;
-        stats[1] = (uint32_t)h->k_prev;
+    {
+        std::vector<ga_core*> sorted(cores, cores + count);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return ga_fail(ctx, GA_ERR_ARG, "ga_core_run_batch: a state appears twice");
+    }
+    for (uint32_t i = 0; i < count; i++) GA_TRY(cf_ensure(cores[i]));
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint32_t i = 0; i < count; i++) GA_TRY(cf_begin(cores[i], T_full ? T_full[i] : UINT64_MAX));
+
+    // per state: the workgroups it would launch alone (they follow its alive count); when a launch's total exceeds
+    // its budget, every state's share is scaled down by the same factor, so large alive lists keep proportionally more
+    std::vector<uint64_t> alive(count);
+    std::vector<uint8_t> done(count, 0);
+    for (uint32_t i = 0; i < count; i++) alive[i] = cores[i]->g->n;
+    const unsigned cus = (unsigned)std::max(ctx->cus, 1);
+    std::vector<CoreJob> jobs(count);
+    for (uint32_t i = 0; i < count; i++) {
+        ga_core* c = cores[i];
+        jobs[i] = CoreJob{c->fm->B, c->fm->f, c->state, c->tri, c->core, c->st, {0, 0, 0}, {0, 0, 0}};
+    }
+    std::vector<uint32_t> map[3];
+    std::vector<uint8_t> host;
+    size_t jobs_bytes = ((size_t)count * sizeof(CoreJob) + 255) & ~(size_t)255;
+    const size_t stats_bytes = ((size_t)count * sizeof(CoreDev) + 255) & ~(size_t)255;
+    int block_rounds = 8;
+    std::vector<CoreDev> hst(count);
+    const uint64_t budget[3] = {64ull * cus, 256ull * cus, 16ull * cus};   // find / update / tail workgroups per launch
+    const unsigned floor_nb[3] = {1u, 4u, 1u};
+    std::vector<CfGrid> grids(count);
+    for (uint64_t guard = 0;; guard++) {
+        uint32_t active = 0;
+        for (uint32_t i = 0; i < count; i++) active += !done[i];
+        if (!active) break;
+        uint64_t total[3] = {0, 0, 0};
+        for (uint32_t i = 0; i < count; i++) {
+            if (done[i]) continue;
+            grids[i] = cf_grid(ctx, guard ? 2 * alive[i] : alive[i]);
+            total[0] += grids[i].v;
+            total[1] += grids[i].u;
+            total[2] += grids[i].t;
+        }
+        for (auto& m : map) m.clear();
+        for (uint32_t i = 0; i < count; i++) {
+            CoreJob& J = jobs[i];
+            for (int q = 0; q < 3; q++) J.base[q] = J.nblk[q] = 0;
+            if (done[i]) continue;
+            unsigned nb[3] = {grids[i].v, grids[i].u, grids[i].t};
+            for (int q = 0; q < 3; q++) {
+                if (total[q] > budget[q]) nb[q] = std::max(floor_nb[q], (unsigned)((uint64_t)nb[q] * budget[q] / total[q]));
+                J.base[q] = (uint32_t)map[q].size();
+                J.nblk[q] = nb[q];
+                map[q].insert(map[q].end(), nb[q], i);
+            }
+        }
+        const size_t map_words = map[0].size() + map[1].size() + map[2].size();
+        const size_t up_bytes = jobs_bytes + map_words * 4;
+        const size_t stats_off = (up_bytes + 255) & ~(size_t)255;
+        const size_t need = stats_off + stats_bytes;
+        if (need > ctx->dcore_bytes) {   // grow-only: shared with ga_core_run_many, used by one call at a time
+            ga_free(ctx->dcore);
+            ctx->dcore = nullptr;
+            ctx->dcore_bytes = 0;
+            GA_TRY(ga_malloc(ctx, &ctx->dcore, need * 2));
+            ctx->dcore_bytes = need * 2;
+        }
+        host.resize(up_bytes);
+        memcpy(host.data(), jobs.data(), (size_t)count * sizeof(CoreJob));
+        size_t at = jobs_bytes;
+        for (auto& m : map) {
+            memcpy(host.data() + at, m.data(), m.size() * 4);
+            at += m.size() * 4;
+        }
+        char* dbase = (char*)ctx->dcore;
+        const CoreJob* djobs = (const CoreJob*)dbase;
+        const uint32_t* dmap[3];
+        dmap[0] = (const uint32_t*)(dbase + jobs_bytes);
+        dmap[1] = dmap[0] + map[0].size();
+        dmap[2] = dmap[1] + map[1].size();
+        CoreDev* dstats = (CoreDev*)(dbase + stats_off);
+        // the host copy stays untouched until the stream sync below
+        GA_HIP(ctx, hipMemcpyAsync(dbase, host.data(), up_bytes, hipMemcpyHostToDevice, ctx->stream));
+        for (int r = 0; r < block_rounds; r++) {   // 3 launches per round for every unfinished state
+            k_cfb_find<<<(unsigned)map[0].size(), 256, 0, ctx->stream>>>(djobs, dmap[0]);
+            k_cfb_update<<<(unsigned)map[1].size(), 64, 0, ctx->stream>>>(djobs, dmap[1]);
+            k_cfb_tail<<<(unsigned)map[2].size(), 256, 0, ctx->stream>>>(djobs, dmap[2]);
+        }
+        GA_TRY(ga_launch_check(ctx, "batched core round"));
+        k_cfb_stats<<<grid_for(count, 256), 256, 0, ctx->stream>>>(djobs, count, dstats);
+        GA_TRY(ga_launch_check(ctx, "k_cfb_stats"));
+        GA_HIP(ctx, hipMemcpyAsync(hst.data(), dstats, (size_t)count * sizeof(CoreDev), hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        uint64_t nmax = 0;
+        for (uint32_t i = 0; i < count; i++) {
+            nmax = std::max<uint64_t>(nmax, cores[i]->g->n);
+            if (hst[i].done) done[i] = 1;
+            else alive[i] = hst[i].nalive;
+        }
+        if (guard > nmax + 16) return ga_fail(ctx, GA_ERR_STATE, "batched core decomposition did not finish");
+        block_rounds = std::min(block_rounds * 2, 256);
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    for (uint32_t i = 0; i < count; i++) {
+        char who[32];
+        snprintf(who, sizeof who, "core %u: ", i);
+        GA_TRY(cf_result(ctx, &hst[i], stats + 2 * i, who));
     }
     return GA_OK;
 }

@@ -15,9 +15,11 @@ Shape on one MI355X:
   * stateful requests (peel, core) take one of up to `states_per_graph` state objects of their graph; states are
     created once and moved between worker streams (ga_peel_set_ctx / ga_core_set_ctx), so the steady state allocates
     and frees no device memory;
-  * core requests on small graphs (at most `core_batch_max_m` edges) are gathered like counts into one
-    ga_core_run_many launch, one workgroup per graph running all of its rounds.
-`slots=1, count_batch=1` is the plain form: one request at a time, one count per launch, in arrival order.
+  * pending core requests (up to `core_batch` of them, on any graphs, each with its own state) are gathered into one
+    ga_core_run_batch: every round's three launches cover all of them, so the round latency that bounds one
+    decomposition is paid once per batch instead of once per request.
+`slots=1, count_batch=1, core_batch=1` is the plain form: one request at a time, one count per launch, one
+decomposition at a time, in arrival order.
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import TC_HASH, Context, Core, DeviceGraph, GAError, Peel, core_many
+from . import TC_HASH, Context, Core, DeviceGraph, GAError, Peel, core_batch
 
 KINDS = ("count", "peel", "core")
 
@@ -107,13 +109,13 @@ class _Resident:
 class Server:
     def __init__(self, source: Callable[[object], np.ndarray], slots: int = 16, count_batch: int = 64,
                  states_per_graph: int = 4, budget_bytes: Optional[int] = None, device: int = 0,
-                 core_batch_max_m: int = 65536):
-        if slots < 1 or count_batch < 1 or states_per_graph < 1:
-            raise ValueError("slots, count_batch and states_per_graph must be >= 1")
+                 core_batch: int = 1024):
+        if slots < 1 or count_batch < 1 or states_per_graph < 1 or core_batch < 1:
+            raise ValueError("slots, count_batch, core_batch and states_per_graph must be >= 1")
         self.source = source
         self.slots = slots
         self.count_batch = count_batch
-        self.core_batch_max_m = core_batch_max_m if count_batch > 1 else 0
+        self.core_batch = core_batch
         self.states_per_graph = states_per_graph
         self.loader = Context(device)
         self.ctxs = [Context(device) for _ in range(slots)]
@@ -211,14 +213,14 @@ class Server:
                 for _, rq in group:
                     rq.inflight += 1
                 return ("count", group, None)
-            if req.kind == "core" and r.m <= self.core_batch_max_m:
-                # small graphs: every pending core request on a small graph with a free state goes into one
-                # ga_core_run_many launch (one workgroup per graph)
+            if req.kind == "core" and self.core_batch > 1:
+                # every pending core request whose graph is resident and has a free state goes into one
+                # ga_core_run_batch (shared launches for all of their rounds)
                 group, states, keep = [], [], collections.deque()
-                while pending and len(group) < self.count_batch:
+                while pending and len(group) < self.core_batch:
                     q = pending.popleft()
                     rq = self.graphs.get(q.key) if q.kind == "core" else None
-                    st = self._take_state(rq, "core") if rq is not None and rq.m <= self.core_batch_max_m else None
+                    st = self._take_state(rq, "core") if rq is not None else None
                     if st is not None:
                         group.append((q, rq))
                         states.append(st)
@@ -227,7 +229,7 @@ class Server:
                         keep.append(q)
                 pending.extendleft(reversed(keep))
                 if group:
-                    return ("core_many", group, states)
+                    return ("core_batch", group, states)
                 continue
             state = self._take_state(r, req.kind)
             if state is None:
@@ -244,10 +246,8 @@ class Server:
         if kind == "count":
             T, _ = ctx.count_many([r.dg for _, r in group], TC_HASH)
             vals = [(int(t),) for t in T]
-        elif kind == "core_many":
-            for st in state:
-                st.set_ctx(ctx)
-            res, _ = core_many(ctx, state, [r.T for _, r in group])
+        elif kind == "core_batch":
+            res, _ = core_batch(ctx, state, [r.T for _, r in group])
             vals = [(rounds, kmax, digest(core)) for core, rounds, kmax in res]
         elif kind == "peel":
             (req, r), = group
@@ -314,7 +314,7 @@ class Server:
                     for _, r in group:
                         r.inflight -= 1
                         r.last = time.perf_counter()
-                    if kind == "core_many":
+                    if kind == "core_batch":
                         for (_, r), st in zip(group, state):
                             r.idle["core"].append(st)
                     elif state is not None:
@@ -359,13 +359,10 @@ def summarize(results: List[Result]) -> dict:
             "mean_count_batch": float(np.mean([r.batch for r in results if r.kind == "count"] or [0]))}
 
 
-def mixed_workload(rate: float, duration: float, seed: int = 2106, pool: int = 256, rmat_graphs: int = 2,
-                   rmat_scale: int = 20, rmat_share: float = 0.05, num: int = 1, den: int = 20, nmax: int = 200_000):
-    """Poisson arrivals at `rate` req/s for `duration` s of count / peel (eps num/den) / core requests, uniform over the
-    kinds; a request picks one of `pool` graphs of the batch size law (95%) or one of `rmat_graphs` R-MAT graphs of
-    scale `rmat_scale`, edge factor 8 (5%). Returns (source, requests)."""
-    from . import gen
-    specs = gen.batch_mix_specs(pool, seed=seed, nmax=nmax)
+def poisson_requests(rate: float, duration: float, seed: int = 2106, pool: int = 256, rmat_graphs: int = 2,
+                     rmat_share: float = 0.05, num: int = 1, den: int = 20) -> List[Request]:
+    """Poisson arrivals at `rate` req/s for `duration` s, uniform over the kinds; a request picks one of `pool` graphs
+    ("mix", i) with probability 1 - rmat_share, else one of `rmat_graphs` R-MAT graphs ("rmat", j)."""
     rng = np.random.default_rng(seed)
     t, reqs = 0.0, []
     while True:
@@ -375,6 +372,14 @@ def mixed_workload(rate: float, duration: float, seed: int = 2106, pool: int = 2
         kind = KINDS[int(rng.integers(3))]
         key = ("rmat", int(rng.integers(rmat_graphs))) if rng.random() < rmat_share else ("mix", int(rng.integers(pool)))
         reqs.append(Request(len(reqs), kind, key, t, num, den))
+    return reqs
+
+
+def workload_source(seed: int = 2106, pool: int = 256, rmat_scale: int = 20, nmax: int = 200_000):
+    """The graphs requests name: ("mix", i) is graph i of `pool` graphs of the batch size law, ("rmat", j) an R-MAT
+    graph of scale `rmat_scale`, edge factor 8. Each is generated once, on first use."""
+    from . import gen
+    specs = gen.batch_mix_specs(pool, seed=seed, nmax=nmax)
     cache: Dict[object, np.ndarray] = {}
 
     def source(key):
@@ -385,4 +390,13 @@ def mixed_workload(rate: float, duration: float, seed: int = 2106, pool: int = 2
                 cache[key] = gen.rmat(rmat_scale, 8 << rmat_scale, seed=seed * 10 + key[1])
         return cache[key]
 
-    return source, reqs
+    return source
+
+
+def mixed_workload(rate: float, duration: float, seed: int = 2106, pool: int = 256, rmat_graphs: int = 2,
+                   rmat_scale: int = 20, rmat_share: float = 0.05, num: int = 1, den: int = 20, nmax: int = 200_000):
+    """Poisson arrivals at `rate` req/s for `duration` s of count / peel (eps num/den) / core requests, uniform over the
+    kinds; a request picks one of `pool` graphs of the batch size law (95%) or one of `rmat_graphs` R-MAT graphs of
+    scale `rmat_scale`, edge factor 8 (5%). Returns (source, requests)."""
+    source = workload_source(seed, pool, rmat_scale, nmax)
+    return source, poisson_requests(rate, duration, seed, pool, rmat_graphs, rmat_share, num, den)

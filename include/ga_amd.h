@@ -221,6 +221,13 @@ int ga_core_set_ctx(ga_core* c, ga_ctx* ctx);
  * stats gets rounds and kmax per graph; core numbers stay in each state (ga_core_download). No counterpart in the
  * reference (it runs one graph per process). */
 int ga_core_run_many(ga_ctx* ctx, ga_core* const* cores, uint32_t count, const uint64_t* T_full, uint32_t* stats, float* ms);
+/* Fused incremental decompositions of `count` states in shared launches: each round's find / update / tail launch
+ * covers every unfinished state, each on its own range of workgroups, so a state follows exactly the rounds of
+ * ga_core_run and gets the same results; the round latency is paid once for the whole batch. For serving several
+ * requests of any size at once. Every state must be incremental, belong to ctx (ga_core_set_ctx moves one) and appear
+ * once; T_full, stats and errors as ga_core_run_many (an error names the first failing state). No counterpart in the
+ * reference (it runs one graph per process). */
+int ga_core_run_batch(ga_ctx* ctx, ga_core* const* cores, uint32_t count, const uint64_t* T_full, uint32_t* stats, float* ms);
 void ga_core_destroy(ga_core* c);
 int ga_core_reset_state(ga_core* c);                 /* alive/state = 1, core = UINT32_MAX */
 int ga_core_begin_round(ga_core* c);

@@ -324,3 +324,48 @@ def test_core_many_small_graphs(ctx):
         c.close()
     for g in graphs:
         g.close()
+
+
+def test_core_batch_shared_launches(ctx):
+    """ga_core_run_batch (every round's launches shared by all unfinished states) equals the oracle and ga_core_run on
+    graphs of very different sizes and round counts; twice in a row, the second time on a subset in another order, two
+    states of one graph side by side, and with a wrong given count caught for the state it belongs to."""
+    from graph_analyzer_amd import gen
+    rng = np.random.default_rng(11)
+    raws = [np.array(clique(k), np.uint32) for k in (4, 30)]
+    raws += [np.zeros((0, 2), np.uint32), np.array([[0, 1]], np.uint32)]
+    raws += [random_graph(rng, int(rng.integers(50, 3000)), float(rng.uniform(0.002, 0.05)), dup=0.1, loops=1) for _ in range(6)]
+    raws += [gen.chung_lu(int(rng.integers(5000, 60000)), int(rng.integers(50000, 400000)), 2.1, 0.0, seed=71 + i) for i in range(3)]
+    raws += [gen.rmat(14, 8 << 14, seed=5)]
+    graphs = [ga.DeviceGraph.from_pairs(ctx, r) for r in raws]
+    Ts = [g.count(ga.TC_HASH)[0] for g in graphs]
+    want = []
+    for raw in raws:
+        og = oracle.OGraph.from_pairs(raw)
+        oc, okmax = og.core()
+        _, orounds, _ = og.core_levels()
+        want.append((oc, orounds, okmax))
+    cores = [ga.Core(g, True) for g in graphs] + [ga.Core(graphs[-1], True)]   # two states of the R-MAT graph
+    idx = list(range(len(graphs))) + [len(graphs) - 1]
+    res, ms = ga.core_batch(ctx, cores, [Ts[i] for i in idx])
+    print(f"core batch: {len(cores)} states in {ms:.2f} ms; rounds {[r for _, r, _ in res]}", flush=True)
+    for i, (core, rounds, kmax) in zip(idx, res):
+        oc, orounds, okmax = want[i]
+        assert np.array_equal(core, oc) and rounds == orounds and kmax == okmax, (i, graphs[i].n, graphs[i].m)
+    sub = [len(cores) - 1, 10, 3, 0, 7]
+    res2, _ = ga.core_batch(ctx, [cores[j] for j in sub], [Ts[idx[j]] for j in sub])
+    for j, (core, rounds, kmax) in zip(sub, res2):
+        oc, orounds, okmax = want[idx[j]]
+        assert np.array_equal(core, oc) and rounds == orounds and kmax == okmax, ("second call", j)
+    for j in (9, 10):   # equal to the single-graph fused loop
+        fc, fr, fk = graphs[idx[j]].core_decomposition(T_full=Ts[idx[j]])
+        assert np.array_equal(fc, res[j][0]) and (fr, fk) == res[j][1:]
+    with pytest.raises(ga.GAError) as ei:   # a wrong given count is caught on the device, for that state only
+        ga.core_batch(ctx, cores[8:11], [Ts[8], Ts[9] + 1, Ts[10]])
+    assert ei.value.code == ga.GA_ERR_STATE and "core 1:" in str(ei.value)
+    with pytest.raises(ga.GAError):
+        ga.core_batch(ctx, [cores[0], cores[0]], [Ts[0], Ts[0]])
+    for c in cores:
+        c.close()
+    for g in graphs:
+        g.close()

@@ -446,6 +446,234 @@ __global__ void k_greedy_flags_many(const GreedyJob* __restrict__ jobs, const un
         if (order[i] < j.n) f[order[i]] = 1;
 }
 
+// Large form, for graphs whose per-vertex state does not fit LDS (the whole graph of gpu_dsg.cu:318-330 before the
+// pruning). One workgroup still makes the deletions one at a time, in the exact (load + tri_H, id) order, but the state
+// lives in HBM and the minimum comes from a two-level structure instead of a scan of all vertices:
+//   bmin[b]   per block of 64 consecutive ids: min of (key << 6 | offset) over its alive vertices (HBM)
+//   smin[s]   per super-block of 64 blocks (4,096 ids): min of (key << 12 | offset in the super-block) (LDS)
+// A deletion takes the minimum over smin (ties: the lower super-block, so the lower id), enumerates the triangles
+// {v, a, b} of H as the CSR form does (alive-neighbour bitmap of v, oriented rows of v's neighbours), and lowers the
+// minima of every touched vertex with atomicMin (its key only decreases). Then one wave recomputes v's block and
+// super-block. Reads of state that other waves change with atomics go past the L1 cache (agent-scope atomic loads).
+// Each pass first deletes, in id order and at once, the vertices whose key is 0 (no triangle and no load): they touch
+// no other vertex, and as they leave T / |H| only grows, so of their density checks only the last one can record a new
+// best (the check before the next deletion sees a higher density still). Keys must fit 52 bits: T < 2^32 and at most
+// 2^19 passes.
+constexpr uint32_t GB_BLK = 64, GB_SUPV = 4096;
+constexpr unsigned long long GB_NONE = ~0ull;
+
+struct GreedyBig {
+    uint32_t n, passes;
+    const uint32_t* rowptr;
+    const uint32_t* col;
+    const uint32_t* urow;
+    const uint32_t* ucol;
+    const uint32_t* tri0;
+    const unsigned long long* dT;
+    uint32_t* orders;
+    unsigned long long* per_pass;
+    GreedyOut* out;
+    unsigned long long* load;   // n
+    uint32_t* cur;              // n: tri_H
+    uint32_t* alive;            // n
+    uint32_t* nbr;              // (n + 31) / 32: alive neighbours of the vertex being deleted
+    unsigned long long* bmin;   // (n + 63) / 64
+};
+
+__device__ __forceinline__ unsigned long long gb_ld64(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t gb_ld32(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Minimum of block b over its alive vertices other than `skip` (one wave; the same value in every lane).
+__device__ __forceinline__ unsigned long long gb_block_min(const GreedyBig& J, uint32_t b, uint32_t skip) {
+    const uint32_t l = lane_id(), x = b * GB_BLK + l;
+    unsigned long long k = GB_NONE;
+    if (x < J.n && x != skip && gb_ld32(J.alive + x)) k = ((gb_ld64(J.load + x) + gb_ld32(J.cur + x)) << 6) | l;
+    return wave_min_u64_dpp(k);
+}
+
+// Minimum of super-block sb from its blocks' minima; block `fresh_b` reads `fresh` instead of bmin (one wave).
+__device__ __forceinline__ unsigned long long gb_super_min(const GreedyBig& J, uint32_t sb, uint32_t fresh_b, unsigned long long fresh) {
+    const uint32_t l = lane_id(), b = sb * (GB_SUPV / GB_BLK) + l, nb = (J.n + GB_BLK - 1) / GB_BLK;
+    unsigned long long k = GB_NONE;
+    if (b < nb) {
+        const unsigned long long v = b == fresh_b ? fresh : gb_ld64(J.bmin + b);
+        if (v != GB_NONE) k = ((v >> 6) << 12) | (l << 6) | (v & 63u);
+    }
+    return wave_min_u64_dpp(k);
+}
+
+__device__ __forceinline__ void gb_lower(const GreedyBig& J, unsigned long long* smin, uint32_t x, unsigned long long key) {
+    atomicMin(J.bmin + x / GB_BLK, (key << 6) | (x % GB_BLK));
+    atomicMin(smin + x / GB_SUPV, (key << 12) | (x % GB_SUPV));
+}
+
+__global__ __launch_bounds__(GR_BLOCK) void k_greedy_big(const GreedyBig J) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned long long* smin = (unsigned long long*)smem;
+    __shared__ unsigned long long red_key[GR_WAVES], red_found[GR_WAVES];
+    __shared__ uint32_t red_s[GR_WAVES], red_cnt[GR_WAVES + 1];
+    __shared__ uint32_t bad;
+    const uint32_t n = J.n, tid = threadIdx.x, lane = tid & (GA_WAVE - 1), wave = tid / GA_WAVE;
+    const uint32_t nb = (n + GB_BLK - 1) / GB_BLK, ns = (n + GB_SUPV - 1) / GB_SUPV;
+    const uint64_t T0 = *J.dT;
+    for (uint32_t i = tid; i < n; i += GR_BLOCK) J.load[i] = 0;
+    for (uint32_t i = tid; i < (n + 31) / 32; i += GR_BLOCK) J.nbr[i] = 0;
+    if (tid == 0) bad = 0;
+    uint64_t bn = 0, bd = 0;
+    uint32_t best_pass = UINT32_MAX, best_cut = 0, err = 0;
+    for (uint32_t p = 0; p < J.passes && !err; p++) {
+        __threadfence();
+        __syncthreads();
+        uint32_t* order = J.orders + (size_t)p * n;
+        // the vertices of key 0 (no triangle, no load) leave first, in id order; the others start alive
+        uint32_t z = 0;
+        for (uint32_t base = 0; base < n; base += GR_BLOCK) {
+            const uint32_t x = base + tid;
+            bool zero = false;
+            if (x < n) {
+                const uint32_t c = J.tri0[x];
+                zero = c == 0 && gb_ld64(J.load + x) == 0;
+                J.cur[x] = c;
+                J.alive[x] = zero ? 0u : 1u;
+            }
+            const uint64_t m = __ballot(zero);
+            if (lane == 0) red_cnt[wave] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t before = z;
+            for (uint32_t w = 0; w < wave; w++) before += red_cnt[w];
+            if (zero) order[before + mask_rank(m)] = x;
+            for (uint32_t w = 0; w < GR_WAVES; w++) z += red_cnt[w];
+            __syncthreads();
+        }
+        uint64_t T = T0;
+        bool improved = false;
+        uint32_t cut = 0;
+        if (z && dens_gt(T, n - (z - 1), bn, bd)) {   // the last of the bulk's checks (the only one that can matter)
+            bn = T;
+            bd = n - (z - 1);
+            improved = true;
+            cut = z - 1;
+        }
+        __threadfence();
+        __syncthreads();
+        for (uint32_t b = wave; b < nb; b += GR_WAVES) {
+            const unsigned long long k = gb_block_min(J, b, UINT32_MAX);
+            if (lane == 0) __hip_atomic_store(J.bmin + b, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __threadfence();
+        __syncthreads();
+        for (uint32_t sb = wave; sb < ns; sb += GR_WAVES) {
+            const unsigned long long k = gb_super_min(J, sb, UINT32_MAX, 0);
+            if (lane == 0) smin[sb] = k;
+        }
+        __syncthreads();
+        for (uint32_t step = z; step < n; step++) {
+            const uint32_t left = n - step;
+            if (dens_gt(T, left, bn, bd)) {   // H is evaluated before every deletion (dsg.h:208)
+                bn = T;
+                bd = left;
+                improved = true;
+                cut = step;
+            }
+            // the minimum (key, id): key = smin >> 12, then the super-block index; each lane's super-blocks ascend
+            unsigned long long bk = GB_NONE;
+            uint32_t bs = UINT32_MAX;
+            for (uint32_t sb = tid; sb < ns; sb += GR_BLOCK) {
+                const unsigned long long k = smin[sb];
+                if (k != GB_NONE && (bk == GB_NONE || (k >> 12) < (bk >> 12))) { bk = k; bs = sb; }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const unsigned long long ok = __shfl_xor(bk, o, GA_WAVE);
+                const uint32_t os = __shfl_xor(bs, o, GA_WAVE);
+                if (ok != GB_NONE && (bk == GB_NONE || (ok >> 12) < (bk >> 12) || ((ok >> 12) == (bk >> 12) && os < bs))) { bk = ok; bs = os; }
+            }
+            if (lane == 0) { red_key[wave] = bk; red_s[wave] = bs; }
+            __syncthreads();
+            bk = red_key[0];
+            bs = red_s[0];
+            for (int w = 1; w < GR_WAVES; w++) {
+                const unsigned long long ok = red_key[w];
+                const uint32_t os = red_s[w];
+                if (ok != GB_NONE && (bk == GB_NONE || (ok >> 12) < (bk >> 12) || ((ok >> 12) == (bk >> 12) && os < bs))) { bk = ok; bs = os; }
+            }
+            if (bk == GB_NONE) { err = 2; break; }   // left > 0, so an alive vertex exists (uniform)
+            const uint32_t v = bs * GB_SUPV + (uint32_t)(bk & (GB_SUPV - 1));
+            if (v >= n) { err = 2; break; }
+            const uint32_t cv = gb_ld32(J.cur + v);
+            if (tid == 0) {
+                __hip_atomic_store(J.alive + v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                order[step] = v;
+            }
+            const uint32_t ub = J.urow[v], ue = J.urow[v + 1];
+            uint64_t found = 0;
+            if (cv) {   // uniform
+                for (uint32_t j = ub + tid; j < ue; j += GR_BLOCK) {
+                    const uint32_t a = J.ucol[j];
+                    if (gb_ld32(J.alive + a)) atomicOr(&J.nbr[a >> 5], 1u << (a & 31));
+                }
+                __syncthreads();
+                // triangles {v, a, b} of H, each once: from the oriented edge between a and b
+                for (uint32_t j = ub + wave; j < ue; j += GR_WAVES) {
+                    const uint32_t a = J.ucol[j];
+                    if (!((gb_ld32(J.nbr + (a >> 5)) >> (a & 31)) & 1u)) continue;
+                    const uint32_t re = J.rowptr[a + 1];
+                    for (uint32_t e = J.rowptr[a] + lane; e < re; e += GA_WAVE) {
+                        const uint32_t b = J.col[e];
+                        if ((gb_ld32(J.nbr + (b >> 5)) >> (b & 31)) & 1u) {
+                            found++;
+                            const uint32_t oa = atomicSub(J.cur + a, 1u), ob = atomicSub(J.cur + b, 1u);
+                            if (oa == 0 || ob == 0) {
+                                bad = 1;
+                            } else {
+                                gb_lower(J, smin, a, gb_ld64(J.load + a) + oa - 1u);
+                                gb_lower(J, smin, b, gb_ld64(J.load + b) + ob - 1u);
+                            }
+                        }
+                    }
+                }
+                found = wave_sum_u64(found);
+                if (lane == 0) red_found[wave] = found;
+                __syncthreads();
+                found = 0;
+                for (int w = 0; w < GR_WAVES; w++) found += red_found[w];
+                for (uint32_t j = ub + tid; j < ue; j += GR_BLOCK) J.nbr[J.ucol[j] >> 5] = 0;   // only v's neighbours' bits were set
+                if (bad || found != cv) err = 1;   // read after the barrier: the same in every lane
+            }
+            if (tid == 0 && cv)   // l_v of the next pass (dsg.h:234); v is not read again in this pass
+                __hip_atomic_fetch_add(J.load + v, (unsigned long long)cv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            T -= cv;
+            __threadfence();
+            __syncthreads();
+            if (err) break;
+            if (wave == 0) {   // v's block and super-block without v (after every decrement of this deletion)
+                const unsigned long long bm = gb_block_min(J, v / GB_BLK, v);
+                if (lane == 0) __hip_atomic_store(J.bmin + v / GB_BLK, bm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long sm = gb_super_min(J, v / GB_SUPV, v / GB_BLK, bm);
+                if (lane == 0) smin[v / GB_SUPV] = sm;
+            }
+            __syncthreads();
+        }
+        if (improved) {
+            best_pass = p;
+            best_cut = cut;
+        }
+        if (tid == 0) {
+            J.per_pass[2 * p] = bn;
+            J.per_pass[2 * p + 1] = bd;
+        }
+    }
+    if (tid == 0) {
+        J.out->bn = bn;
+        J.out->bd = bd;
+        J.out->best_pass = best_pass;
+        J.out->best_cut = best_cut;
+        J.out->error = err;
+    }
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { ga_free(p); }
@@ -484,9 +712,10 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     if (passes == 0) return ga_fail(ctx, GA_ERR_ARG, "greedy_pp needs at least one pass");
     const uint32_t n = g->n;
     const uint32_t cap = ga_greedy_max_vertices(ctx);
-    if (n > cap)
-        return ga_fail(ctx, GA_ERR_ARG, "greedy_pp runs one workgroup per graph with its state in LDS: n = %u exceeds %u vertices "
-                       "(run it on the graph prune_to_core leaves)", n, cap);
+    const bool big = n > cap;   // state in HBM (k_greedy_big)
+    if (big && ((size_t)(n + GB_SUPV - 1) / GB_SUPV) * 8 > lds_room(ctx))
+        return ga_fail(ctx, GA_ERR_ARG, "greedy_pp: n = %u needs more super-block minima than LDS holds", n);
+    if (big && passes > (1u << 19)) return ga_fail(ctx, GA_ERR_ARG, "greedy_pp on a large graph: at most 2^19 passes (52-bit keys)");
     if ((uint64_t)passes * n > (1ull << 28))
         return ga_fail(ctx, GA_ERR_ARG, "greedy_pp keeps every pass's deletion order: passes * n = %llu exceeds 2^28",
                        (unsigned long long)passes * n);
@@ -508,7 +737,35 @@ extern "C" int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t*
     GA_TRY(ga_fill_u32(ctx, (uint32_t*)dflags.p, (size_t)n + 1, 0));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)dout.p, sizeof(GreedyOut) / 8, 0));
     if (g->m) GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, (unsigned long long*)dT.p, nullptr, (uint32_t*)dtri.p, nullptr, nullptr));
-    if (n && n <= dense_max(ctx)) {
+    DevBuf dbig;
+    if (big) {
+        unsigned long long hT = 0;
+        GA_HIP(ctx, hipMemcpyAsync(&hT, dT.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hT >= (1ull << 32)) return ga_fail(ctx, GA_ERR_ARITH, "greedy_pp on a large graph: T >= 2^32 (u32 tri_H counters)");
+        const size_t nw = ((size_t)n + 31) / 32, nb = ((size_t)n + GB_BLK - 1) / GB_BLK;
+        GA_TRY(ga_malloc(ctx, &dbig.p, (size_t)n * 16 + nw * 4 + nb * 8 + 64));
+        GreedyBig J;
+        J.n = n;
+        J.passes = passes;
+        J.rowptr = g->rowptr;
+        J.col = g->col;
+        J.urow = g->urow;
+        J.ucol = g->ucol;
+        J.tri0 = (const uint32_t*)dtri.p;
+        J.dT = (const unsigned long long*)dT.p;
+        J.orders = (uint32_t*)dorders.p;
+        J.per_pass = (unsigned long long*)dpass.p;
+        J.out = (GreedyOut*)dout.p;
+        J.load = (unsigned long long*)dbig.p;
+        J.bmin = J.load + n;
+        J.cur = (uint32_t*)(J.bmin + nb);
+        J.alive = J.cur + n;
+        J.nbr = J.alive + n;
+        const size_t ns = ((size_t)n + GB_SUPV - 1) / GB_SUPV;
+        k_greedy_big<<<1, GR_BLOCK, ns * 8, ctx->stream>>>(J);
+        GA_TRY(ga_launch_check(ctx, "k_greedy_big"));
+    } else if (n && n <= dense_max(ctx)) {
         const GreedyJob job = {n, g->m, dense_stride(n), 0, g->src, g->col, (const uint32_t*)dtri.p, (const unsigned long long*)dT.p,
                                (uint32_t*)dorders.p, (unsigned long long*)dpass.p, (GreedyOut*)dout.p};
         GA_TRY(ga_malloc(ctx, &djob.p, sizeof job));

@@ -271,9 +271,10 @@ struct DsgResult {
     Dens dens() const { return Dens{tri, verts}; }
 };
 
-// Greedy++ with `passes` passes on a resident graph (ga_greedy_pp): one workgroup runs every pass with the per-vertex
-// state in LDS, so the graph must be small (ga_greedy_max_vertices; the pruned graph of gpu_dsg.cu:344-356 is).
-// A graph that is too large, or a failed self-check, gives ok = false and the library's message in error.
+// Greedy++ with `passes` passes on a resident graph (ga_greedy_pp): one workgroup runs every pass, with the per-vertex
+// state in LDS for graphs of up to ga_greedy_max_vertices (the pruned graph of gpu_dsg.cu:344-356) and in HBM with
+// block minima for larger ones (the whole graph of gpu_dsg.cu:318-330). A failed self-check, or T >= 2^32 on a large
+// graph, gives ok = false and the library's message in error.
 inline DsgResult greedy_pp_device(const gaa::DeviceGraph& g, uint32_t passes, float* ms = nullptr) {
     DsgResult r;
     std::vector<uint64_t> pp(2 * (size_t)std::max<uint32_t>(passes, 1));

@@ -262,17 +262,20 @@ int ga_truss_download(ga_truss* t, uint32_t* truss);
 
 /* ---- Greedy++ for the triangle-density densest subgraph (Flowless Alg 2, triangles as hyperedges) ---------------- */
 /* greedy_pp(g, passes) (dsg.h:189-248) and greedy(g) = passes 1 (dsg.h:250), as gpu_dsg runs Greedy++ on the graph
- * prune_to_core leaves (gpu_dsg.cu:344-356). Each deletion takes the alive vertex with the smallest (load + tri_H, id);
+ * prune_to_core leaves (gpu_dsg.cu:344-356) and on the whole graph (gpu_dsg.cu:318-330). Each deletion takes the alive
+ * vertex with the smallest (load + tri_H, id);
  * H is evaluated before every deletion and replaces the best only when strictly denser (exact cross products).
  * Outputs: *tri / *verts = the best density found (0 / 0 when the graph has no triangle, as DsgResult); per_pass
  * (2 * passes words, may be NULL) = the best (tri, verts) after each pass (DsgResult::per_pass); set (capacity n, may be
- * NULL) = the best subgraph's vertices ascending, *set_size of them (DsgResult::set). One workgroup keeps the per-vertex
- * state in LDS, so n must not exceed ga_greedy_max_vertices(ctx), else GA_ERR_ARG; passes = 0 is GA_ERR_ARG;
- * GA_ERR_STATE when the triangle bookkeeping self-check of dsg.h:230 fails. ms: device time including the per-vertex
- * triangle count. */
+ * NULL) = the best subgraph's vertices ascending, *set_size of them (DsgResult::set). Up to ga_greedy_max_vertices(ctx)
+ * vertices one workgroup keeps the per-vertex state in LDS; above it the state lives in HBM and the minimum comes from
+ * two levels of block minima (then T must be below 2^32, else GA_ERR_ARITH, and passes at most 2^19). passes = 0 is
+ * GA_ERR_ARG; GA_ERR_STATE when the triangle bookkeeping self-check of dsg.h:230 fails. ms: device time including the
+ * per-vertex triangle count. */
 int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t* tri, uint64_t* verts, uint64_t* per_pass, uint32_t* set,
                  uint32_t* set_size, float* ms);
-/* The largest vertex count ga_greedy_pp takes on this device (no counterpart in the reference, whose CPU code has none). */
+/* The largest vertex count ga_greedy_pp keeps in LDS on this device, and the limit of ga_greedy_pp_many (no counterpart
+ * in the reference, whose CPU code has none). */
 uint32_t ga_greedy_max_vertices(ga_ctx* ctx);
 /* Greedy++ with `passes` passes on each of `count` graphs of this context, graph by graph equal to ga_greedy_pp: one
  * workgroup per graph in one launch for graphs of up to 1,120 vertices (the LDS-bitmap form on MI355X), one launch

@@ -4,9 +4,11 @@
 //   build_edge_order(g, perm)                  graph.h:156-171
 //   build_undirected_csr(g, urow, ucol)        densest.h:162-171
 //   core_decomposition(g, core, &kmax)         dsg.h:116-166
+//   greedy(g), greedy_pp(g, passes)            dsg.h:189-250, on a graph above the one-workgroup (LDS) size as well
 // on the graph of the text file given as the argument (loaded with load_graph, graph.h:71-96) and on seeded random
 // graphs with hubs and isolated vertices. Prints ALL OK and exits 0 when every result matches; exits 1 otherwise.
 #include <random>
+#include <set>
 
 #include "../../graph_analyzer_amd/host/dsg.h"
 #include "../../oracle/oracle.hpp"
@@ -74,6 +76,32 @@ int main(int argc, char** argv) {
     Graph empty;
     build_graph_from_edges(4, {}, empty);
     check_graph(empty, "no edges");
+    {   // Greedy++ through the drop-in signatures: a graph of the pruned size and one of 20,000 vertices (the whole-graph
+        // call of gpu_dsg.cu:318-330, above ga_greedy_max_vertices), against the oracle's Greedy++
+        for (uint32_t n : {3000u, 20000u}) {
+            std::set<std::pair<uint32_t, uint32_t>> es;   // build_graph_from_edges takes unique pairs
+            std::uniform_int_distribution<uint32_t> V(0, n - 1);
+            for (uint32_t i = 0; i < 5 * n; i++) {
+                const uint32_t a = V(rng), b = V(rng);
+                if (a != b) es.emplace(std::min(a, b), std::max(a, b));
+            }
+            for (uint32_t a = 0; a < 25; a++)
+                for (uint32_t b = a + 1; b < 25; b++) es.emplace(n - 30 + a, n - 30 + b);   // a planted clique
+            const std::vector<std::pair<uint32_t, uint32_t>> e(es.begin(), es.end());
+            Graph r;
+            build_graph_from_edges(n, e, r);
+            const orc::Graph o = to_oracle(r);
+            const DsgResult g1 = greedy(r);
+            const orc::GreedyResult o1 = orc::greedy_pp(o, 1);
+            CHECK(g1.ok && g1.tri == o1.tri && g1.verts == o1.verts && g1.set == o1.set, "greedy(g) on n %u: %s", n, g1.error.c_str());
+            const DsgResult g3 = greedy_pp(r, 3);
+            const orc::GreedyResult o3 = orc::greedy_pp(o, 3);
+            bool pp_ok = g3.ok && g3.per_pass.size() == o3.per_pass.size();
+            for (size_t i = 0; pp_ok && i < o3.per_pass.size(); i++)
+                pp_ok = g3.per_pass[i].num == o3.per_pass[i].first && g3.per_pass[i].den == o3.per_pass[i].second;
+            CHECK(pp_ok && g3.tri == o3.tri && g3.verts == o3.verts && g3.set == o3.set, "greedy_pp(g, 3) on n %u: %s", n, g3.error.c_str());
+        }
+    }
     if (g_fail) {
         printf("%d failures\n", g_fail);
         return 1;

@@ -131,13 +131,8 @@ def test_greedy_size_limit_and_arguments(ctx):
     rng = np.random.default_rng(5)
     raw = np.concatenate([random_graph(rng, cap, 6.0 / cap), np.array([[0, cap - 1], [1, cap - 1], [0, 1]], np.uint32)])
     check(ctx, raw, (2,), "n = limit")
-    big = np.concatenate([raw, np.array([[2, cap]], np.uint32)])
-    g = ga.DeviceGraph.from_pairs(ctx, big)
-    assert g.n == cap + 1
-    with pytest.raises(ga.GAError) as ei:
-        g.greedy_pp(1)
-    assert ei.value.code == ga.GA_ERR_ARG
-    g.close()
+    big = np.concatenate([raw, np.array([[2, cap]], np.uint32)])   # one vertex more: the HBM form
+    check(ctx, big, (1, 2), "n = limit + 1")
     g = ga.DeviceGraph.from_pairs(ctx, np.array(clique(5), np.uint32))
     with pytest.raises(ga.GAError) as ei:
         g.greedy_pp(0)
@@ -173,4 +168,28 @@ def test_greedy_many_equals_one_by_one(ctx):
                 assert (t, v) == (ot, ov) and np.array_equal(s, os_), (rep, i)
         graphs[0].count()
     for g in graphs:
+        g.close()
+
+
+def test_greedy_large_graphs(ctx):
+    """Graphs above the LDS limit (the whole graph of gpu_dsg.cu:318-330), where the state lives in HBM and the minimum
+    comes from block minima: random graphs, a power-law graph with hubs and many vertices in no triangle (the key-0
+    bulk at the start of each pass), planted cliques (ties of equal keys across blocks and super-blocks), and ids
+    spread past one super-block of 4,096."""
+    cap = ctx.greedy_max_vertices()
+    rng = np.random.default_rng(41)
+    cases = [
+        ("random", np.concatenate([random_graph(rng, cap + 3000, 8.0 / cap), np.array([[0, cap + 2999]], np.uint32)])),
+        ("chung_lu", gen.chung_lu(30000, 150000, 2.1, 0.0, seed=9)),
+        ("cliques", np.concatenate([np.array(clique(20, 4090), np.uint32), np.array(clique(20, 9000), np.uint32),
+                                    np.array(clique(12, 20000), np.uint32), random_graph(rng, 24000, 0.0002)])),
+    ]
+    for tag, raw in cases:
+        g = ga.DeviceGraph.from_pairs(ctx, raw)
+        assert g.n > cap, tag
+        og = oracle.OGraph.from_pairs(raw)
+        for p in (1, 3):
+            got = g.greedy_pp(p)
+            same(got, og.greedy_pp(p), (tag, p))
+        print(f"greedy_pp large {tag}: n {g.n}, m {g.m}, 3 passes in {got[4]:.1f} ms", flush=True)
         g.close()

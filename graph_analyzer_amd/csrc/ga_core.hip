@@ -369,6 +369,8 @@ struct CoreFusedDev {
     uint32_t nent;        // frontier entries this round
     uint32_t min_next;    // min tri over the alive vertices after this round (the next round's level candidate)
     uint32_t ticket;      // workgroups of k_cf_tail that have finished this round
+    uint32_t xcur;        // cross[xcur] holds this round's crossers (read by the find), cross[xcur ^ 1] the next round's
+    uint32_t xcount[2];   // entries of each cross list
     unsigned long long adj_cursor;   // compaction cursor in the new adjacency buffer
 };
 
@@ -382,6 +384,7 @@ struct CoreBufs {
     uint32_t* cend;
     uint32_t* cgt;            // per vertex: position in its list of the first neighbour with a larger id (cN(v) after v)
     uint32_t* fvert;
+    uint32_t* cross[2];       // vertices whose tri fell to the level or below in an update (the next frontier)
     uint32_t* ent;            // 2 words per entry: u, chunk
     const uint2* mref;        // per vertex: {bitmap index, 1 << 31} for a hub, else {first bucket, log2 of buckets}
     const uint4* mtab;        // membership tables of the original neighbour lists (4-slot buckets, load <= 1/2)
@@ -405,6 +408,8 @@ __global__ void k_cf_init(CoreFusedDev* f, uint32_t* __restrict__ alist0, uint32
         f->nent = 0;
         f->min_next = 0xFFFFFFFFu;
         f->ticket = 0;
+        f->xcur = 0;
+        f->xcount[0] = f->xcount[1] = 0;
         f->adj_cursor = 0;
     }
 }
@@ -424,15 +429,18 @@ __global__ void k_cf_min(CoreBufs B, const CoreFusedDev* __restrict__ f, const u
 }
 
 // Frontier = alive v with tri <= k; appends v to fvert and, if it is in a triangle, its chunk entries (one atomic per
-// wave for each).
+// wave for each). After a round whose update pushed some alive tri from above the level to the level or below, those
+// crossers are exactly the next frontier and the level stays (min <= k_prev): the find then reads the cross list
+// instead of the alive list. Otherwise k = the min over the alive vertices, taken by the previous tail.
 __device__ __forceinline__ void cf_find_body(const CoreBufs& B, CoreFusedDev* f, const uint32_t* __restrict__ tri,
                                              uint8_t* __restrict__ state, uint32_t* __restrict__ core, CoreDev* st, uint32_t bid,
                                              uint32_t nb) {
     if (st->done) return;
+    const uint32_t xc = f->xcur, nx = f->xcount[xc];
     const unsigned long long mt = st->min_tri, kp = st->k_prev;
-    const unsigned long long k = mt > kp ? mt : kp;
-    const uint32_t* __restrict__ al = B.alist[f->alist_cur];
-    const uint32_t na = f->na;
+    const unsigned long long k = nx ? kp : (mt > kp ? mt : kp);
+    const uint32_t* __restrict__ al = nx ? B.cross[xc] : B.alist[f->alist_cur];
+    const uint32_t na = nx ? nx : f->na;
     const uint32_t lane = lane_id();
     const uint64_t wave0 = ((uint64_t)bid * blockDim.x + threadIdx.x) / 64, nwaves = ((uint64_t)nb * blockDim.x) / 64;
     for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
@@ -511,11 +519,16 @@ __device__ __forceinline__ bool cf_member(const CoreBufs& B, uint32_t rx, uint32
 }
 
 // stg: 6 * 64 words of LDS (ps | len | v | sv + (mode << 8) | ref.x | ref.y); one 64-lane workgroup
-__device__ __forceinline__ void cf_update_body(const CoreBufs& B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
+__device__ __forceinline__ void cf_update_body(const CoreBufs& B, CoreFusedDev* f, const uint8_t* __restrict__ state,
                                                uint32_t* __restrict__ tri, const CoreDev* __restrict__ st, uint32_t bid, uint32_t nb,
                                                uint32_t* __restrict__ stg) {
     if (st->done) return;
     const uint32_t* __restrict__ adj = B.adj[f->adj_cur];
+    // an alive vertex whose tri falls from above the level to the level or below joins the next round's frontier
+    const unsigned long long lvl = st->k;
+    const uint32_t xw = f->xcur ^ 1u;
+    uint32_t* __restrict__ xlist = B.cross[xw];
+    uint32_t* xcnt = &f->xcount[xw];
     const uint32_t nent = f->nent;
     const uint32_t lane = threadIdx.x;
     for (uint32_t ei = bid; ei < nent; ei += nb) {
@@ -573,22 +586,34 @@ __device__ __forceinline__ void cf_update_body(const CoreBufs& B, const CoreFuse
                             uint32_t x[U];
 #pragma unroll
                             for (uint32_t q = 0; q < U; q++) x[q] = off + q * W < klen ? adj[kps + off + q * W] : 0u;
+                            bool dm[U];
 #pragma unroll
                             for (uint32_t q = 0; q < U; q++) {
-                                if (off + q * W >= klen) break;
+                                dm[q] = false;
+                                if (off + q * W >= klen) continue;
                                 const uint32_t w = x[q];
                                 if (!cf_member(B, rx, ry, w)) continue;
                                 const uint8_t sw = state[w];
                                 if (sw == 0) continue;
                                 if ((ksv == 2 && kv < u) || (sw == 2 && w < u)) continue;   // a smaller frontier member owns it
-                                if (sw == 1) atomicSub(&tri[w], 1u);
+                                dm[q] = sw == 1;
                                 c++;
                             }
+                            uint32_t old[U];
+#pragma unroll
+                            for (uint32_t q = 0; q < U; q++)   // all decrements in flight before the first crossing test
+                                if (dm[q]) old[q] = atomicSub(&tri[x[q]], 1u);
+#pragma unroll
+                            for (uint32_t q = 0; q < U; q++)
+                                if (dm[q] && (unsigned long long)old[q] == lvl + 1) xlist[atomicAdd(xcnt, 1u)] = x[q];
                         }
                     }
 #pragma unroll
                     for (uint32_t o = W / 2; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-                    if (r == 0 && k < end && c && ksv == 1) atomicSub(&tri[kv], c);
+                    if (r == 0 && k < end && c && ksv == 1) {
+                        const uint32_t o = atomicSub(&tri[kv], c);
+                        if ((unsigned long long)o > lvl && (unsigned long long)o <= lvl + c) xlist[atomicAdd(xcnt, 1u)] = kv;
+                    }
                 }
                 start = end;
                 if (S == 1) break;
@@ -598,7 +623,7 @@ __device__ __forceinline__ void cf_update_body(const CoreBufs& B, const CoreFuse
     }
 }
 
-__global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, const CoreFusedDev* __restrict__ f, const uint8_t* __restrict__ state,
+__global__ void __launch_bounds__(64) k_cf_update(CoreBufs B, CoreFusedDev* f, const uint8_t* __restrict__ state,
                                                   uint32_t* __restrict__ tri, const CoreDev* __restrict__ st) {
     __shared__ uint32_t stg[6 * 64];
     cf_update_body(B, f, state, tri, st, blockIdx.x, gridDim.x, stg);
@@ -693,6 +718,8 @@ __device__ __forceinline__ void cf_tail_body(const CoreBufs& B, CoreFusedDev* f,
     const uint32_t nfv = f->nfv, na = f->na;
     const unsigned long long nalive = st->nalive;
     const bool compact = nfv < nalive && 2ull * (nalive - nfv) <= na;   // the same decision in every workgroup
+    // with crossers the next round keeps the level and reads them: no min is needed, so no scan unless compacting
+    const uint32_t sna = (compact || f->xcount[f->xcur ^ 1u] == 0) ? na : 0u;
     const uint32_t lane = lane_id();
     const uint64_t tid = (uint64_t)bid * blockDim.x + threadIdx.x, nthr = (uint64_t)nb * blockDim.x;
     for (uint64_t i = tid; i < nfv; i += nthr) state[B.fvert[i]] = 0;
@@ -700,7 +727,7 @@ __device__ __forceinline__ void cf_tail_body(const CoreBufs& B, CoreFusedDev* f,
     uint32_t* __restrict__ out = B.alist[f->alist_cur ^ 1u];
     const uint64_t wave0 = tid / 64, nwaves = nthr / 64;
     uint32_t mn = 0xFFFFFFFFu;
-    for (uint64_t base = wave0 * 64; base < na; base += nwaves * 64) {   // wave-uniform loop
+    for (uint64_t base = wave0 * 64; base < sna; base += nwaves * 64) {   // wave-uniform loop
         const uint64_t i = base + lane;
         uint32_t v = 0;
         const bool keep = i < na && state[(v = al[i])] == 1;
@@ -769,6 +796,8 @@ __device__ __forceinline__ void cf_tail_body(const CoreBufs& B, CoreFusedDev* f,
     f->min_next = 0xFFFFFFFFu;
     f->nfv = 0;
     f->nent = 0;
+    f->xcount[f->xcur] = 0;   // this round's crossers are consumed; the next round reads the ones its update found
+    f->xcur ^= 1u;
     if (compact) {
         f->na = atomicAdd(&f->na_new, 0u);
         f->alist_cur ^= 1u;
@@ -852,7 +881,7 @@ static int cf_members(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
 static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     const size_t n = g->n, m2 = 2ull * g->m;
     const size_t nent = n + m2 / CORE_CHUNK + 16;
-    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 3 * n + n + 2 * nent + 64 + 16;
+    const size_t words = 2 * n + 2 * std::max<size_t>(m2, 1) + 3 * n + n + 2 * n + 2 * nent + 64 + 16;
     if (m2 >= 0xFFFFFFFFull || nent >= 0x7FFFFFFFull)
         return ga_fail(ctx, GA_ERR_ARG, "graph too large for the fused core decomposition (32-bit offsets)");
     GA_TRY(ga_malloc(ctx, &M.block, words * 4));
@@ -867,6 +896,8 @@ static int cf_alloc(ga_ctx* ctx, ga_graph* g, CoreFusedMem& M) {
     M.B.cend = p; p += n;
     M.B.cgt = p; p += n;
     M.B.fvert = p; p += n;
+    M.B.cross[0] = p; p += n;
+    M.B.cross[1] = p; p += n;
     M.B.ent = p; p += 2 * nent;
     p = (uint32_t*)(((uintptr_t)p + 63) & ~(uintptr_t)63);
     M.f = (CoreFusedDev*)p;

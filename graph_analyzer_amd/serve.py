@@ -88,7 +88,7 @@ def peel_bytes(n: int) -> int:
 def core_bytes(n: int, m: int) -> int:
     m2 = max(2 * m, 1)
     nent = n + m2 // 8 + 16
-    words = 6 * n + 2 * m2 + 2 * nent + 80            # alive lists, adjacency copies, ranges, frontier, entries
+    words = 8 * n + 2 * m2 + 2 * nent + 80            # alive lists, adjacency copies, ranges, frontier, cross lists, entries
     members = 4 * (5 * n + 24) + 64 + 16 * m2          # hub ids and table references; tables (at most m2 buckets)
     return 4 * (n + 1) + 4 * m2 + 13 * n + 4 * words + members + 4096
 

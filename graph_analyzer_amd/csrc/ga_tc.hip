@@ -1750,9 +1750,14 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
-    // coordinate search: relabel and item order, lanes per bucket, then block, load shift, batches and work chunk
+    // coordinate search: relabel and item order, lanes per bucket, then block, load shift, batches and work chunk.
+    // The renumbered copy and its plan need about 60 bytes per edge while they are built (sorts included): it is tried
+    // only when that fits in half of the free device memory (a 2.6B-edge batch union does not).
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const bool relabel_fits = g->crow != nullptr || 60ull * g->m + 16ull * g->n < free_b / 2;
     GA_TRY(trial(base));
-    for (int rl = 0; rl <= 1; rl++)
+    for (int rl = 0; rl <= (relabel_fits ? 1 : 0); rl++)
         for (int io = 0; io <= 2; io++) {
             ga_tc_tuning t = bt;
             t.relabel = rl;

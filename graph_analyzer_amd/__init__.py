@@ -51,7 +51,7 @@ class TcTuning(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_int32 * 5), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
                 ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32),
                 ("flat_chunk", ctypes.c_int32), ("flat_block", ctypes.c_int32),
-                ("item_order", ctypes.c_int32), ("relabel", ctypes.c_int32)]
+                ("item_order", ctypes.c_int32), ("relabel", ctypes.c_int32), ("probe", ctypes.c_int32)]
 
     @classmethod
     def from_dict(cls, d):
@@ -67,7 +67,7 @@ class TcTuning(ctypes.Structure):
         return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "batches": self.batches,
                 "load_shift": self.load_shift, "hash_cap": self.hash_cap, "flat_chunk": self.flat_chunk,
                 "flat_block": self.flat_block,
-                "item_order": self.item_order, "relabel": self.relabel}
+                "item_order": self.item_order, "relabel": self.relabel, "probe": self.probe}
 
 
 _lib = None
@@ -123,6 +123,7 @@ def lib() -> ctypes.CDLL:
             "ga_tc_count_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, ctypes.c_int, _u64p,
                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
+            "ga_ctx_set_grid_cap": (ctypes.c_int, [_vp, ctypes.c_uint32]),
             "ga_peel_set_ctx": (ctypes.c_int, [_vp, _vp]),
             "ga_core_run_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_core_run_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
@@ -208,6 +209,11 @@ class Context:
         b, l = ctypes.c_float(), ctypes.c_float()
         self.check(lib().ga_ctx_barrier_probe(self._h, workgroups, iters, ctypes.byref(b), ctypes.byref(l)), "barrier_probe")
         return b.value, l.value
+
+    def set_grid_cap(self, max_workgroups: int = 0):
+        """Caps every count, participation, support, peel and core launch of this context at max_workgroups workgroups
+        (0: the library's own grids), so that tests can run the grid-stride and item loops many passes each."""
+        self.check(lib().ga_ctx_set_grid_cap(self._h, int(max_workgroups)), "set_grid_cap")
 
     def mem_info(self):
         """(free, total) device bytes."""

@@ -6,6 +6,10 @@
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
     python -m graph_analyzer_amd greedy_many [--graphs 256] [--passes 10]   (seeded dense graphs of 100-1,000 vertices)
     python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
+    python -m graph_analyzer_amd serve [--rate 2000] [--seconds 1] [--pool 256]
+                                 (continuous-batching server on the mixed request workload: throughput and latency)
+    python -m graph_analyzer_amd core_batch [--rate 2000] [--seconds 1] [--pool 256]
+                                 (every core request of that mixed workload in one ga_core_run_batch: device time)
 
 Meant for profiling a single stage (rocprofv3 -- python3 -m graph_analyzer_amd core --workload rmat21) and for quick
 checks on a GPU host; results are not compared with a CPU oracle here (the tests and bench.py do that).
@@ -27,15 +31,48 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
     ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "greedy", "greedy_many", "batch",
-                                      "probe"])
+                                      "probe", "serve", "core_batch"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--num", type=int, default=1)
     ap.add_argument("--den", type=int, default=10)
     ap.add_argument("--graphs", type=int, default=4096)
     ap.add_argument("--passes", type=int, default=10)
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--seconds", type=float, default=1.0)
+    ap.add_argument("--pool", type=int, default=256)
     args = ap.parse_args(argv)
+    if args.stage == "serve":   # one Poisson run of the mixed workload through the continuous-batching server
+        from graph_analyzer_amd import serve
+        source, reqs = serve.mixed_workload(args.rate, args.seconds, pool=args.pool)
+        s = serve.Server(source, slots=16, count_batch=64, core_batch=1024, states_per_graph=16)
+        t0 = time.perf_counter()
+        s.preload(sorted({q.key for q in reqs}))
+        print(f"serve: {len(reqs)} requests at {args.rate:g} req/s, graphs resident in {time.perf_counter() - t0:.1f} s", flush=True)
+        for _ in range(args.repeat):
+            summ = serve.summarize(s.run(reqs))
+            print(f"serve: {summ['throughput_rps']:.0f} req/s, p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms, "
+                  f"makespan {summ['makespan_s']:.2f} s, by kind {summ['by_kind']}", flush=True)
+        s.close()
+        return 0
     ctx = ga.Context(0)
+    if args.stage == "core_batch":   # the serving mix's core requests as one shared-launch batch (the server's core path)
+        from graph_analyzer_amd import serve
+        source, reqs = serve.mixed_workload(args.rate, args.seconds, pool=args.pool)
+        cores = [q for q in reqs if q.kind == "core"]
+        dgs = {k: ga.DeviceGraph.from_pairs(ctx, source(k)) for k in sorted({q.key for q in cores})}
+        Ts = {k: g.count(ga.TC_HASH)[0] for k, g in dgs.items()}
+        states = [ga.Core(dgs[q.key], True) for q in cores]
+        print(f"core_batch: {len(cores)} core requests on {len(dgs)} graphs "
+              f"({sum(1 for q in cores if q.key[0] == 'rmat')} on R-MAT graphs)", flush=True)
+        for _ in range(args.repeat):
+            res, ms = ga.core_batch(ctx, states, [Ts[q.key] for q in cores])
+            print(f"core_batch: {ms:.1f} ms device, rounds max {max(r[1] for r in res)}", flush=True)
+        for st in states:
+            st.close()
+        for g in dgs.values():
+            g.close()
+        return 0
     if args.stage == "probe":
         for _ in range(2):
             ms = ctx.copy_probe(1 << 30)

@@ -542,7 +542,7 @@ extern "C" int ga_batch_peel(ga_ctx* ctx, ga_batch* b, uint64_t num, uint64_t de
     GA_TRY(ga_fill_u32(ctx, d.all_done, 1, 0));
     k_bpeel_init<<<grid_for(b->count, 256), 256, 0, ctx->stream>>>(d, (const unsigned long long*)dvoff.p, b->count);
     GA_TRY(ga_launch_check(ctx, "k_bpeel_init"));
-    const unsigned gv = grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 8u);
+    const unsigned gv = ctx_grid(ctx, grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 8u));
     // One round: begin, the participation count (its bucket launches), mark, end. No launch takes a host value that
     // changes between rounds, and a finished batch turns every launch into a no-op through d.all_done, so rounds can be
     // enqueued ahead of the readback, and a block of them can be captured once and replayed.

@@ -209,8 +209,8 @@ __global__ void k_core_end(CoreDev* st) {
     if (st->nalive == 0) st->done = 1;
 }
 
-static unsigned cgrid(ga_ctx* ctx, uint64_t n) { return grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 8u); }
-static unsigned ugrid(ga_ctx* ctx) { return (unsigned)std::max(ctx->cus, 1) * 8u; }
+static unsigned cgrid(ga_ctx* ctx, uint64_t n) { return ctx_grid(ctx, grid_for(n, 256, (unsigned)std::max(ctx->cus, 1) * 8u)); }
+static unsigned ugrid(ga_ctx* ctx) { return ctx_grid(ctx, (unsigned)std::max(ctx->cus, 1) * 8u); }
 
 extern "C" int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core** out) {
     *out = nullptr;
@@ -944,7 +944,7 @@ struct CfGrid {
 static CfGrid cf_grid(ga_ctx* ctx, uint64_t alive) {
     CfGrid r;
     r.v = cgrid(ctx, std::max<uint64_t>(alive, 1));
-    r.u = 4u * std::min<unsigned>(ugrid(ctx), std::max<unsigned>(4u, (unsigned)std::min<uint64_t>((8 * alive + 255) / 256, 1u << 20)));
+    r.u = ctx_grid(ctx, 4u * std::min<unsigned>(ugrid(ctx), std::max<unsigned>(4u, (unsigned)std::min<uint64_t>((8 * alive + 255) / 256, 1u << 20))));
     r.t = std::min<unsigned>(r.v, (unsigned)std::max(ctx->cus, 1) * 2u);   // few workgroups: few tickets
     return r;
 }

@@ -27,6 +27,7 @@ struct ga_ctx {
     size_t dmany_n = 0;
     void* dcore = nullptr;              // grow-only descriptors and results of ga_core_run_many
     size_t dcore_bytes = 0;
+    uint32_t grid_cap = 0;              // test hook: at most this many workgroups per count / peel / core launch (0: none)
     std::string err;
 };
 
@@ -132,6 +133,11 @@ inline unsigned grid_for(uint64_t threads, unsigned block, unsigned cap = 1u << 
     if (b > cap) b = cap;
     return (unsigned)b;
 }
+
+// The grid of a count, peel or core launch under the context's cap (ga_ctx_set_grid_cap): with a cap of 1, 3 or 7
+// workgroups every grid-stride and item loop runs many passes, the way the reference's emulator runs its kernels on
+// 1-, 3- and 7-block grids (emu_main.cpp:40-57). Every kernel these launches use covers its work for any grid size.
+inline unsigned ctx_grid(const ga_ctx* ctx, unsigned grid) { return ctx->grid_cap && grid > ctx->grid_cap ? ctx->grid_cap : grid; }
 
 // ---------------------------------------------------------------------------------------------------------------
 // device side: wave64 helpers

@@ -126,7 +126,7 @@ __global__ void k_peel_end(PeelDev* st, unsigned long long n) {
     if (st->nalive == 0) st->done = 1;
 }
 
-static unsigned vgrid(ga_ctx* ctx, uint32_t n) { return grid_for(((uint64_t)n + 3) / 4, 256, (unsigned)std::max(ctx->cus, 1) * 8u); }
+static unsigned vgrid(ga_ctx* ctx, uint32_t n) { return ctx_grid(ctx, grid_for(((uint64_t)n + 3) / 4, 256, (unsigned)std::max(ctx->cus, 1) * 8u)); }
 
 extern "C" int ga_peel_create(ga_ctx* ctx, ga_graph* g, ga_peel** out) {
     *out = nullptr;

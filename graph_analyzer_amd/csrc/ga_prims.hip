@@ -109,6 +109,12 @@ extern "C" int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* tota
     return GA_OK;
 }
 
+extern "C" int ga_ctx_set_grid_cap(ga_ctx* ctx, uint32_t max_workgroups) {
+    if (!ctx) return GA_ERR_ARG;
+    ctx->grid_cap = max_workgroups;
+    return GA_OK;
+}
+
 extern "C" int ga_ctx_sync(ga_ctx* ctx) {
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GA_OK;

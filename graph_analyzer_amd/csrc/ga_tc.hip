@@ -14,7 +14,7 @@
 //                     size (the tuning table), and hub rows are split into work-bounded chunks.
 #include "ga_internal.hpp"
 
-enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4, F_GRAPH = 8 };
+enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4, F_GRAPH = 8, F_SLOTS = 16 };   // F_SLOTS: plain / per-graph counts read in lane slots
 
 #ifndef GA_TC_U8_FROM
 #define GA_TC_U8_FROM 32u   // lane-group phase length from which each lane keeps 8 loads in flight (else 4)
@@ -275,6 +275,10 @@ __global__ void k_tc_hash(const TcArgs a) {
     uint32_t* tix = tab + a.ts_max;          // F_SUP: position of each key in N+(h); F_TRI: hits per key
     uint32_t* stg = tab + tab_words;         // batch staging: ps[G] | len[G] | p[G] | work index[G]
     constexpr bool TC = (FLAGS & F_TRI) && !(FLAGS & F_GRAPH);   // participation through the counter cache
+    // plain and per-graph counts read a batch's lists in lane slots of SU elements (below); participation and supports,
+    // which need each hit's slot or list, read them by length class
+    constexpr bool SLOTS = HASHED && (FLAGS & F_SLOTS) && !(FLAGS & (F_TRI | F_SUP | F_ALIVE));
+    constexpr uint32_t SU = 8;
     uint32_t* ckey = lds + (size_t)gpb * (tab_words + 4u * G);
     uint32_t* ccnt = ckey + TRI_CACHE;
     if (TC) {
@@ -334,6 +338,94 @@ __global__ void k_tc_hash(const TcArgs a) {
             const uint32_t p = n_p, ps = n_ps;
             uint32_t len = n_len;
             if ((FLAGS & F_ALIVE) && len && !alive[p]) len = 0;
+            if constexpr (SLOTS) {
+                // Lane slots: list q of the batch gets w_q = ceil(len_q / SU) lanes, lane r of them reading elements
+                // r, r + w_q, ... (consecutive lanes on consecutive addresses), so every list of the batch is in flight
+                // in the same round: ceil(sum w_q / G) rounds per batch instead of one per length class.
+                const uint32_t w = (len + SU - 1u) / SU;
+                uint32_t incl = w;   // inclusive prefix of w over the group
+#pragma unroll
+                for (uint32_t d = 1; d < G; d <<= 1) {
+                    const uint32_t t = __shfl_up(incl, d, G);
+                    incl += sub >= d ? t : 0u;
+                }
+                const uint32_t wt = __shfl(incl, G - 1, G);   // lane slots of the batch
+                stg[sub] = ps;
+                stg[G + sub] = len;
+                stg[2 * G + sub] = incl;
+                {   // prefetch the next batch while this one is streamed
+                    const uint32_t jn = base + G + sub;
+                    n_ps = n_len = 0;
+                    if (jn < e) {
+                        n_ps = a.wps[jn];
+                        n_len = a.wlen[jn];
+                    }
+                }
+                group_sync();
+                for (uint32_t s0 = 0; s0 < wt; s0 += G) {
+                    const uint32_t s = s0 + sub;
+                    uint32_t c = 0;
+                    if (s < wt) {
+                        // the list of slot s: the first q with incl_q > s (incl never decreases over the group)
+                        uint32_t q = 0;
+#pragma unroll
+                        for (uint32_t step = G / 2; step >= 1; step >>= 1)
+                            if (stg[2 * G + q + step - 1u] <= s) q += step;
+                        const uint32_t kps = stg[q], klen = stg[G + q];
+                        const uint32_t kw = (klen + SU - 1u) / SU;
+                        const uint32_t r = s - (stg[2 * G + q] - kw);   // this lane's rank among the list's kw lanes
+                        const uint32_t last = klen - 1u;
+                        const uint32_t* __restrict__ lp = col + kps;
+                        uint32_t x[SU];
+#pragma unroll
+                        for (uint32_t u = 0; u < SU; u++) x[u] = lp[min(r + u * kw, last)];   // past the end: masked
+                        __builtin_amdgcn_sched_barrier(0);
+                        const uint32_t bb = tbits - 2u;
+                        if (!spill) {   // one bucket read and four compares per element (wave masks, as above)
+#pragma unroll
+                            for (uint32_t u0 = 0; u0 < SU; u0 += 4) {
+                                uint4 tb[4];
+#pragma unroll
+                                for (uint32_t t = 0; t < 4; t++) tb[t] = ((const uint4*)tab)[(x[u0 + t] * 2654435769u) >> (32u - bb)];
+#pragma unroll
+                                for (uint32_t t = 0; t < 4; t++) {
+                                    const uint32_t xv = x[u0 + t];
+                                    const uint64_t hm = (__builtin_amdgcn_ballot_w64(tb[t].x == xv) | __builtin_amdgcn_ballot_w64(tb[t].y == xv) |
+                                                         __builtin_amdgcn_ballot_w64(tb[t].z == xv) | __builtin_amdgcn_ballot_w64(tb[t].w == xv)) &
+                                                        __builtin_amdgcn_ballot_w64(r + (u0 + t) * kw <= last);
+                                    c += __builtin_amdgcn_inverse_ballot_w64(hm) ? 1u : 0u;
+                                }
+                            }
+                        } else {   // a key left its home bucket: walk on past a full bucket without x
+                            uint32_t slow = 0;
+#pragma unroll
+                            for (uint32_t u0 = 0; u0 < SU; u0 += 4) {
+                                uint4 tb[4];
+#pragma unroll
+                                for (uint32_t t = 0; t < 4; t++) tb[t] = ((const uint4*)tab)[key_bucket(x[u0 + t], bb)];
+#pragma unroll
+                                for (uint32_t t = 0; t < 4; t++) {
+                                    const uint32_t u = u0 + t, xv = x[u];
+                                    const uint32_t d = min(min(tb[t].x ^ xv, tb[t].y ^ xv), min(tb[t].z ^ xv, tb[t].w ^ xv));
+                                    const uint32_t mx = max(max(tb[t].x, tb[t].y), max(tb[t].z, tb[t].w));
+                                    const uint32_t v = r + u * kw <= last ? 1u : 0u;
+                                    c += (d == 0u ? 1u : 0u) & v;
+                                    slow |= ((d != 0u && mx != GA_EMPTY) ? v : 0u) << u;
+                                }
+                            }
+                            if (slow) {
+#pragma unroll
+                                for (uint32_t u = 0; u < SU; u++)
+                                    if ((slow >> u) & 1u) c += bucket_has_after(tab, bb, x[u], key_bucket(x[u], bb));
+                            }
+                        }
+                    }
+                    local += c;
+                    if (FLAGS & F_GRAPH) hcount += c;
+                }
+                group_sync();   // staging is rewritten by the next batch
+                continue;
+            }
             const uint64_t gm = group_bits<G>(__ballot(len > 0));
             const uint32_t cnt = (uint32_t)__popcll(gm);
             if (len > 0) {   // stable compaction keeps the batch longest first
@@ -380,7 +472,10 @@ __global__ void k_tc_hash(const TcArgs a) {
                                 if (u < U) x[u] = lp[min(u * W, last)];
                             __builtin_amdgcn_sched_barrier(0);
                             if (HASHED && !(FLAGS & (F_TRI | F_SUP)) && !spill) {
-                                // no key left its home bucket: one bucket read and four compares per element
+                                // no key left its home bucket: one bucket read and four compares per element. The
+                                // compares go to wave masks (v_cmp into SGPRs, or-ed on the scalar unit) and come back
+                                // as one per-lane bit: written as an or of four bools, the compiler packs the bits
+                                // into a 4-bit vector with about three times the vector instructions.
                                 const uint32_t bb = tbits - 2u;
                                 uint4 tb[UMAX];
 #pragma unroll
@@ -390,8 +485,10 @@ __global__ void k_tc_hash(const TcArgs a) {
                                 for (uint32_t u = 0; u < UMAX; u++)
                                     if (u < U) {
                                         const uint32_t xv = x[u];
-                                        const bool hit = (tb[u].x == xv) | (tb[u].y == xv) | (tb[u].z == xv) | (tb[u].w == xv);
-                                        c += (hit && u * W <= last) ? 1u : 0u;
+                                        const uint64_t hm = (__builtin_amdgcn_ballot_w64(tb[u].x == xv) | __builtin_amdgcn_ballot_w64(tb[u].y == xv) |
+                                                             __builtin_amdgcn_ballot_w64(tb[u].z == xv) | __builtin_amdgcn_ballot_w64(tb[u].w == xv)) &
+                                                            __builtin_amdgcn_ballot_w64(u * W <= last);
+                                        c += __builtin_amdgcn_inverse_ballot_w64(hm) ? 1u : 0u;
                                     }
                                 continue;
                             }
@@ -1035,6 +1132,7 @@ ga_tc_tuning default_tuning() {
     t.flat_block = 512;
     t.item_order = 1;
     t.relabel = 0;
+    t.probe = 1;
     return t;
 }
 
@@ -1051,7 +1149,7 @@ bool tuning_valid(const ga_tc_tuning& t) {
             return false;
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
-           t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && (t.relabel == 0 || t.relabel == 1) &&
+           t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && (t.relabel == 0 || t.relabel == 1) && (t.probe == 0 || t.probe == 1) &&
            (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
                                   (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
                                    t.flat_chunk / t.flat_block == 16)));
@@ -1314,7 +1412,7 @@ static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const CountView& V, cons
         return launch_hash_bucket<64, false, FLAGS>(ctx, g, V, items, nitems, bound, a);
     // participation: fewer, longer-lived workgroups, so that each counter cache gathers more items before its flush
     // (64 -> 16 per CU: 27.3 -> 23.0 ms for the R-MAT scale-21 peel, 9.5 -> 8.6 ms on the 34.7M-edge graph)
-    const unsigned grid = grid_for((uint64_t)nitems * G, block, (unsigned)std::max(ctx->cus, 1) * (TC ? 16u : 64u));
+    const unsigned grid = ctx_grid(ctx, grid_for((uint64_t)nitems * G, block, (unsigned)std::max(ctx->cus, 1) * (TC ? 16u : 64u)));
     a.items = items;
     a.nitems = nitems;
     a.ts_max = ts;
@@ -1331,7 +1429,7 @@ static int launch_tiny(ga_ctx* ctx, ga_graph* g, const CountView& V, const uint3
     a.items = items;
     a.nitems = nitems;
     a.xcd = (V.P->built.item_order == 3 || V.P->built.item_order == 5) ? 1u : 0u;
-    k_tc_tiny<FLAGS><<<grid_for(nitems, block, (unsigned)std::max(ctx->cus, 1) * 64u), block, 0, ctx->stream>>>(a);
+    k_tc_tiny<FLAGS><<<ctx_grid(ctx, grid_for(nitems, block, (unsigned)std::max(ctx->cus, 1) * 64u)), block, 0, ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_tiny");
 }
 
@@ -1361,7 +1459,7 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, const CountView& V, unsigned lo
     const size_t shmem = flat_shmem(P.chunk_used);
     if (shmem > ctx->lds_max) return ga_fail(ctx, GA_ERR_ARG, "flat chunk %u needs %zu bytes of LDS", P.chunk_used, shmem);
     const uint32_t block = (uint32_t)P.tune.flat_block;
-    const uint32_t grid = std::min<uint32_t>(P.nch, (uint32_t)std::max(ctx->cus, 1) * 16u);
+    const uint32_t grid = ctx_grid(ctx, std::min<uint32_t>(P.nch, (uint32_t)std::max(ctx->cus, 1) * 16u));
     const int upt = (int)(P.chunk_used / block);
     if (upt != 4 && upt != 8 && upt != 16) return ga_fail(ctx, GA_ERR_ARG, "flat chunk / block must be 4, 8 or 16");
     if (gcnt) {
@@ -1374,6 +1472,14 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, const CountView& V, unsigned lo
         else k_tc_flat<16, false><<<grid, block, shmem, ctx->stream>>>(f);
     }
     return ga_launch_check(ctx, "k_tc_flat");
+}
+
+// A lane-group bucket with its LDS table; plain and per-graph counts read their lists in lane slots when tune.probe.
+template <int G, int FLAGS>
+static int launch_hashed(ga_ctx* ctx, ga_graph* g, const CountView& V, const uint32_t* items, uint32_t ni, uint32_t bound, TcArgs a) {
+    if constexpr (FLAGS == 0 || FLAGS == F_GRAPH)
+        if (V.P->tune.probe) return launch_hash_bucket<G, true, FLAGS | F_SLOTS>(ctx, g, V, items, ni, bound, a);
+    return launch_hash_bucket<G, true, FLAGS>(ctx, g, V, items, ni, bound, a);
 }
 
 template <int FLAGS>
@@ -1390,11 +1496,11 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, i
                 if constexpr (FLAGS == 0 || FLAGS == F_GRAPH) rc = launch_tiny<FLAGS>(ctx, g, V, items, ni, a);
                 else rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, V, items, ni, bound, a);
                 break;
-            case 4: rc = launch_hash_bucket<4, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
-            case 8: rc = launch_hash_bucket<8, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
-            case 16: rc = launch_hash_bucket<16, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
-            case 32: rc = launch_hash_bucket<32, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
-            default: rc = launch_hash_bucket<64, true, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 4: rc = launch_hashed<4, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 8: rc = launch_hashed<8, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 16: rc = launch_hashed<16, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            case 32: rc = launch_hashed<32, FLAGS>(ctx, g, V, items, ni, bound, a); break;
+            default: rc = launch_hashed<64, FLAGS>(ctx, g, V, items, ni, bound, a); break;
         }
         GA_TRY(rc);
         off += P.nitems[b];
@@ -1406,14 +1512,14 @@ template <int FLAGS>
 static int launch_merge(ga_ctx* ctx, ga_graph* g, const uint32_t* order, unsigned long long* total, const uint8_t* alive, uint32_t* tri,
                         uint32_t* sup, const uint32_t* done) {
     if (!g->m) return GA_OK;
-    k_tc_merge<FLAGS><<<grid_for(g->m, 256), 256, 0, ctx->stream>>>(g->rowptr, g->col, g->src, order, g->m, alive, tri, sup, total, done);
+    k_tc_merge<FLAGS><<<ctx_grid(ctx, grid_for(g->m, 256)), 256, 0, ctx->stream>>>(g->rowptr, g->col, g->src, order, g->m, alive, tri, sup, total, done);
     return ga_launch_check(ctx, "k_tc_merge");
 }
 
 template <int G>
 static int launch_group(ga_ctx* ctx, ga_graph* g, unsigned long long* total) {
     if (!g->m) return GA_OK;
-    k_tc_group<G><<<grid_for((uint64_t)g->m * G, 256), 256, 0, ctx->stream>>>(g->rowptr, g->col, g->src, g->m, total);
+    k_tc_group<G><<<ctx_grid(ctx, grid_for((uint64_t)g->m * G, 256)), 256, 0, ctx->stream>>>(g->rowptr, g->col, g->src, g->m, total);
     return ga_launch_check(ctx, "k_tc_group");
 }
 
@@ -1748,18 +1854,22 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
     auto trial = [&](const ga_tc_tuning& t) -> int {
         GA_TRY(ga_tc_set_tuning(ctx, g, &t));
         float best_here = 1e30f;
-        for (int r = 0; r < 3; r++) {
+        // r = 0 is an untimed warm-up: after a plan rebuild the device has idled on the host's work, and the first
+        // counts can run slower than the rest (a trial timed on them alone once kept 8-lane groups for d+(h) <= 128 on
+        // the 34.7M-edge graph: 1.58 ms against 1.31 ms)
+        for (int r = 0; r < 4; r++) {
             uint64_t T = 0;
             float x = 0;
             GA_TRY(timed_count(ctx, g, gof, gcnt, ngraphs, &T, &x));
             if (T != T0) return ga_fail(ctx, GA_ERR_STATE, "tuning trial changed the count (%llu vs %llu)", (unsigned long long)T,
                                         (unsigned long long)T0);
-            best_here = std::min(best_here, x);
+            if (r > 0) best_here = std::min(best_here, x);
         }
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
-    // coordinate search: relabel and item order, lanes per bucket, then block, load shift, batches and work chunk.
+    // coordinate search: relabel and item order, the probe form, lanes per bucket, then block, load shift, batches and
+    // work chunk.
     // The renumbered copy and its plan need about 60 bytes per edge while they are built (sorts included): it is tried
     // only when that fits in half of the free device memory (a 2.6B-edge batch union does not).
     size_t free_b = 0, total_b = 0;
@@ -1774,6 +1884,11 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
             if (t.relabel == bt.relabel && t.item_order == bt.item_order) continue;
             GA_TRY(trial(t));
         }
+    {   // how the lane groups read a batch's probe lists: lane slots or length classes
+        ga_tc_tuning t = bt;
+        t.probe = 1 - bt.probe;
+        GA_TRY(trial(t));
+    }
     GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
     GA_TRY(ga_tc_ensure_plan(ctx, g, true));
     const CountView V = count_view(g);

@@ -191,7 +191,7 @@ inline uint32_t nosrc_per_wave(uint32_t m, uint32_t blocks) {
 inline uint32_t default_blocks(const ga_ctx* ctx, const ga_graph* g, int variant) {
     const uint64_t work = variant == GA_TC_NOSRC ? (uint64_t)g->m * 4 : (uint64_t)g->n;
     const uint64_t cap = (uint64_t)std::max(ctx->cus, 1) * 8;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, (work + SIMPLE_BLOCK - 1) / SIMPLE_BLOCK));
+    return ctx_grid(ctx, (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, (work + SIMPLE_BLOCK - 1) / SIMPLE_BLOCK)));
 }
 
 }  // namespace ga_simple

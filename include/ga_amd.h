@@ -54,6 +54,11 @@ int ga_ctx_gather_probe(ga_ctx* ctx, uint64_t bytes, uint32_t seg_bytes, uint64_
  * measurement behind running the core's rounds as separate launches. No counterpart in the reference (measurement
  * support). */
 int ga_ctx_barrier_probe(ga_ctx* ctx, uint32_t workgroups, uint32_t iters, float* us_per_barrier, float* us_per_launch);
+/* Test hook: every count, participation, support, peel and core launch of this context takes at most max_workgroups
+ * workgroups (0: the library's own grids), so that grid-stride and item loops run many passes. Replaces the tiny
+ * grids (1, 3 and 7 blocks) the reference's CPU emulator runs every kernel on (emu_main.cpp:40-57). Results must not
+ * depend on it. */
+int ga_ctx_set_grid_cap(ga_ctx* ctx, uint32_t max_workgroups);
 /* Free and total device memory (hipMemGetInfo): the serving loop's admission budget. */
 int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
@@ -177,6 +182,10 @@ typedef struct ga_tc_tuning {
                               (each vertex is keyed by the smallest id among itself and such neighbours); the
                               orientation, and so every count, is unchanged. 0: on the CSR itself. Participation,
                               support and the per-vertex states always use the CSR's own ids */
+    int32_t probe;         /* plain and per-graph counts, lane-group buckets: 1 = every probe list of a batch gets
+                              ceil(length / 8) lanes and all of them are read in the same round; 0 = the lists are read
+                              by length class (lists of length in (S/2, S] together, one round per class). Participation
+                              and support always read by length class */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
 /* The library's default tuning (what a graph uses until ga_tc_set_tuning or ga_tc_tune). */

@@ -116,14 +116,15 @@ def test_batch_tunings_flat_per_graph(ctx):
     graphs = mix(64, 11, nmax=1500)
     ref = np.array([oracle.OGraph.from_pairs(p).triangles() for p in graphs], np.uint64)
     b = ga.Batch(ctx, graphs)
-    for lanes, fc, fb, order in (([8, 16, 32, 64, 64], 256, 64, 1), ([4, 8, 32, 32, 64], 1024, 256, 0),
-                                 ([4, 4, 16, 64, 64], 2048, 256, 1), ([8, 8, 64, 64, 64], 4096, 1024, 1),
-                                 ([16, 32, 64, 64, 64], 0, 512, 0), ([1, 16, 32, 64, 64], 0, 256, 1),
-                                 ([1, 8, 32, 64, 64], 2048, 256, 1)):
+    for i, (lanes, fc, fb, order) in enumerate((([8, 16, 32, 64, 64], 256, 64, 1), ([4, 8, 32, 32, 64], 1024, 256, 0),
+                                                ([4, 4, 16, 64, 64], 2048, 256, 1), ([8, 8, 64, 64, 64], 4096, 1024, 1),
+                                                ([16, 32, 64, 64, 64], 0, 512, 0), ([1, 16, 32, 64, 64], 0, 256, 1),
+                                                ([1, 8, 32, 64, 64], 2048, 256, 1), ([4, 16, 32, 64, 64], 0, 256, 1))):
         t = ga.TcTuning()
         t.lanes[:] = lanes
         t.block, t.work_chunk, t.batches, t.load_shift, t.hash_cap = 256, 2048, 4, 2, 4096
         t.flat_chunk, t.flat_block, t.item_order = fc, fb, order
+        t.probe = i % 2   # per-graph counts in lane slots or by length class
         b.set_tuning(t)
         T, _ = b.count_all()
         assert np.array_equal(T, ref), (lanes, fc, fb, order)

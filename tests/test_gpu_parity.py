@@ -193,6 +193,7 @@ def test_tuning_shapes(ctx):
                 t.flat_chunk, t.flat_block = flats[(li + block) % len(flats)]
                 t.item_order = (li + shift) % 3
                 t.relabel = (li + block // 64 + shift) % 2   # the renumbered copy for plain counts
+                t.probe = (li + shift + block // 256) % 2   # lane slots or length classes for plain counts
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)

@@ -54,8 +54,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - _T0:6.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def make_workload(name):
@@ -293,6 +296,7 @@ def run_batch(ctx, count, steps, tune):
     Tp_def, _ = b.count_all(per_graph=True)
     per_default = min(b.count_all(per_graph=True)[1] for _ in range(2))
     tune_info = None
+    log(f"batch: per-graph count with the default tuning {per_default:.1f} ms; tuning the batched count")
     if tune:
         t, tms = b.tune()
         tune_info = t.as_dict()
@@ -319,6 +323,7 @@ def run_batch(ctx, count, steps, tune):
             t2.flat_chunk, t2.flat_block = fc, fb
             cands.append((f"{name}+flat{fc}/{fb}", t2))
     for name, t2 in cands:
+        log(f"batch: per-graph count with the {name} tuning")
         b.set_tuning(t2)
         Tp, _ = b.count_all(per_graph=True)
         if not np.array_equal(Tp, T0):

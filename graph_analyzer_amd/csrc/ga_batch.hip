@@ -237,10 +237,6 @@ struct GraphExec {   // scoped captured graph and its executable
     }
 };
 
-double wall_ms() {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 int ensure_slices(ga_ctx* ctx, ga_batch* b) {
     if (!b->slices.empty()) return GA_OK;
     b->slices.assign(b->count, nullptr);

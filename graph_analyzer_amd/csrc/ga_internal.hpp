@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,7 @@ struct TcPlan {
     uint32_t* wps = nullptr;             // rowptr[p] of each work entry (start of the probe list)
     uint32_t* wlen = nullptr;            // d+(p) of each work entry (length of the probe list)
     uint32_t nwork = 0;                  // work entries (edges with min(d+) > 0)
+    bool entries = false;                // wcol/wedge/wps/wlen/wh hold the sorted work entries (they depend on the CSR only)
     uint32_t* wh = nullptr;              // hashed vertex h of each work entry
     uint32_t nlight = 0;                 // entries [0, nlight) have d+(h) <= 128 (the flat kernel's class)
     unsigned long long* wu = nullptr;    // flat unit offset of each light entry (nlight + 1, exclusive prefix)
@@ -115,6 +117,8 @@ int ga_graph_ensure_perm(ga_ctx* ctx, ga_graph* g);
 int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g, bool counts_only = false);
 // ga_tc_tune over the count form a caller runs: plain (gcnt NULL) or per graph of a disjoint union (gof, gcnt of
 // ngraphs + 1 device words), as ga_batch_count launches it.
+// Host wall clock in milliseconds (steady clock).
+double wall_ms();
 int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, ga_tc_tuning* best,
                       float* best_ms);
 // One count of `variant` into *d_total (zeroed by the caller). Optional: alive mask, per-vertex tri, per-edge sup, a device

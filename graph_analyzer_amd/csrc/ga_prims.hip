@@ -3,6 +3,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <chrono>
 #include <cstring>
 
 #include "ga_internal.hpp"
@@ -107,6 +108,10 @@ extern "C" int ga_ctx_mem_info(ga_ctx* ctx, uint64_t* free_bytes, uint64_t* tota
     if (free_bytes) *free_bytes = f;
     if (total_bytes) *total_bytes = t;
     return GA_OK;
+}
+
+double wall_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 extern "C" int ga_ctx_set_grid_cap(ga_ctx* ctx, uint32_t max_workgroups) {

@@ -14,7 +14,10 @@
 //                     size (the tuning table), and hub rows are split into work-bounded chunks.
 #include "ga_internal.hpp"
 
-enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4, F_GRAPH = 8, F_SLOTS = 16 };   // F_SLOTS: plain / per-graph counts read in lane slots
+enum { F_ALIVE = 1, F_TRI = 2, F_SUP = 4, F_GRAPH = 8, F_SLOTS = 16 };
+#ifndef GA_TC_SU
+#define GA_TC_SU 8u   // elements per lane slot of the lane-slot probe
+#endif   // F_SLOTS: plain / per-graph counts read in lane slots
 
 #ifndef GA_TC_U8_FROM
 #define GA_TC_U8_FROM 32u   // lane-group phase length from which each lane keeps 8 loads in flight (else 4)
@@ -278,7 +281,7 @@ __global__ void k_tc_hash(const TcArgs a) {
     // plain and per-graph counts read a batch's lists in lane slots of SU elements (below); participation and supports,
     // which need each hit's slot or list, read them by length class
     constexpr bool SLOTS = HASHED && (FLAGS & F_SLOTS) && !(FLAGS & (F_TRI | F_SUP | F_ALIVE));
-    constexpr uint32_t SU = 8;
+    constexpr uint32_t SU = GA_TC_SU;
     uint32_t* ckey = lds + (size_t)gpb * (tab_words + 4u * G);
     uint32_t* ccnt = ckey + TRI_CACHE;
     if (TC) {
@@ -987,6 +990,11 @@ __global__ void k_plan_fill(const uint32_t* __restrict__ rowptr, const uint32_t*
     }
 }
 
+// Probe work of each entry, from the kept entries (what k_plan_fill writes into wt).
+__global__ void k_plan_work(const uint32_t* __restrict__ wlen, uint32_t nwork, unsigned long long* __restrict__ wt) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) wt[j] = wlen[j];
+}
+
 __global__ void k_plan_rowstart(const uint32_t* __restrict__ wh, uint32_t nwork, uint32_t* __restrict__ hstart) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x)
         if (j == 0 || wh[j - 1] != wh[j]) hstart[wh[j]] = j;
@@ -1123,10 +1131,10 @@ ga_tc_tuning default_tuning() {
     t.lanes[2] = 32;
     t.lanes[3] = 64;
     t.lanes[4] = 64;
-    t.block = 256;
-    t.work_chunk = 8192;
+    t.block = 128;        // block, work chunk and load shift: what the tuner settles on for the 34.7M-edge graph
+    t.work_chunk = 2048;
     t.batches = 8;
-    t.load_shift = 2;
+    t.load_shift = 3;
     t.hash_cap = 4096;
     t.flat_chunk = 0;     // the flat kernel is a tuner option (it wins on some degree mixes, not on all)
     t.flat_block = 512;
@@ -1228,28 +1236,35 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     TcPlan& P = *V.P;
     if (!P.tune.block) P.tune = default_tuning();
     const uint32_t n = g->n, m = g->m;
-    ga_free(P.wcol);
-    ga_free(P.wedge);
-    ga_free(P.wps);
-    ga_free(P.wlen);
+    // The sorted work entries depend on the CSR only: a rebuild for other tuning parameters keeps them and redoes the
+    // item cutting, bucketing and ordering (on a 2.6B-edge batch union the entry sort is most of a plan build).
+    const bool keep = P.entries;
+    if (!keep) {
+        ga_free(P.wcol);
+        ga_free(P.wedge);
+        ga_free(P.wps);
+        ga_free(P.wlen);
+        ga_free(P.wh);
+        P.wcol = P.wedge = P.wps = P.wlen = P.wh = nullptr;
+        P.nwork = P.nlight = 0;
+    }
     ga_free(P.items);
-    ga_free(P.wh);
     ga_free(P.wu);
     ga_free(P.cstart);
     ga_free(P.frec);
-    P.wcol = P.wedge = P.wps = P.wlen = P.items = P.wh = P.cstart = P.frec = nullptr;
+    P.items = P.cstart = P.frec = nullptr;
     P.wu = nullptr;
     for (int b = 0; b < NHASH; b++) P.nitems[b] = 0;
     P.nhuge = 0;
-    P.nwork = P.nlight = P.nch = P.chunk_used = 0;
+    P.nch = P.chunk_used = 0;
     P.utot = 0;
     DevBuf keys, keys2, vals, vals2, hstart_b, wt_b, wsum_b, flag_b, fpos_b, starts_b, bc_b, bcount_b;
     GA_TRY(ga_malloc(ctx, &bc_b.p, 128));
     uint32_t* bc = (uint32_t*)bc_b.p;   // [30] work entries, [31] light work entries
     GA_TRY(ga_fill_u32(ctx, bc, 32, 0));
-    uint32_t nwork = 0, nlight = 0;
+    uint32_t nwork = keep ? P.nwork : 0u, nlight = keep ? P.nlight : 0u;
     uint32_t* sorted_vals = nullptr;
-    if (m) {
+    if (m && !keep) {
         GA_TRY(ga_malloc(ctx, &keys.p, (size_t)m * 8));
         GA_TRY(ga_malloc(ctx, &keys2.p, (size_t)m * 8));
         GA_TRY(ga_malloc(ctx, &vals.p, (size_t)m * 4));
@@ -1268,11 +1283,13 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
         nwork = (uint32_t)(ctx->hstat[0] & 0xFFFFFFFFu);
         nlight = split ? (uint32_t)(ctx->hstat[0] >> 32) : 0u;
     }
-    GA_TRY(ga_malloc(ctx, (void**)&P.wcol, std::max<size_t>(nwork, 1) * 4));
-    GA_TRY(ga_malloc(ctx, (void**)&P.wedge, std::max<size_t>(nwork, 1) * 4));
-    GA_TRY(ga_malloc(ctx, (void**)&P.wps, std::max<size_t>(nwork, 1) * 4));
-    GA_TRY(ga_malloc(ctx, (void**)&P.wlen, std::max<size_t>(nwork, 1) * 4));
-    GA_TRY(ga_malloc(ctx, (void**)&P.wh, std::max<size_t>(nwork, 1) * 4));
+    if (!keep) {
+        GA_TRY(ga_malloc(ctx, (void**)&P.wcol, std::max<size_t>(nwork, 1) * 4));
+        GA_TRY(ga_malloc(ctx, (void**)&P.wedge, std::max<size_t>(nwork, 1) * 4));
+        GA_TRY(ga_malloc(ctx, (void**)&P.wps, std::max<size_t>(nwork, 1) * 4));
+        GA_TRY(ga_malloc(ctx, (void**)&P.wlen, std::max<size_t>(nwork, 1) * 4));
+        GA_TRY(ga_malloc(ctx, (void**)&P.wh, std::max<size_t>(nwork, 1) * 4));
+    }
     P.nwork = nwork;
     P.nlight = nlight;
     if (nwork == 0) {
@@ -1292,9 +1309,15 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     unsigned long long* wsum = (unsigned long long*)wsum_b.p;
     uint32_t* flag = (uint32_t*)flag_b.p;
     uint32_t* fpos = (uint32_t*)fpos_b.p;
-    k_plan_fill<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, V.src, V.col, sorted_vals, nwork, P.wcol, P.wedge, P.wps,
-                                                                     P.wlen, wh, wt);
-    GA_TRY(ga_launch_check(ctx, "k_plan_fill"));
+    if (!keep) {
+        k_plan_fill<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, V.src, V.col, sorted_vals, nwork, P.wcol, P.wedge,
+                                                                         P.wps, P.wlen, wh, wt);
+        GA_TRY(ga_launch_check(ctx, "k_plan_fill"));
+        P.entries = true;
+    } else {
+        k_plan_work<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(P.wlen, nwork, wt);
+        GA_TRY(ga_launch_check(ctx, "k_plan_work"));
+    }
     k_plan_rowstart<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(wh, nwork, hstart);
     GA_TRY(ga_launch_check(ctx, "k_plan_rowstart"));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nwork, 1, 0));
@@ -1865,6 +1888,11 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
                                         (unsigned long long)T0);
             if (r > 0) best_here = std::min(best_here, x);
         }
+#ifdef GA_TUNE_TRACE   // build-time switch: print every trial of the coordinate search
+        fprintf(stderr, "tune trial lanes %d %d %d %d %d block %d chunk %d batches %d shift %d flat %d/%d order %d relabel %d probe %d: %.4f ms\n",
+                t.lanes[0], t.lanes[1], t.lanes[2], t.lanes[3], t.lanes[4], t.block, t.work_chunk, t.batches, t.load_shift, t.flat_chunk,
+                t.flat_block, t.item_order, t.relabel, t.probe, best_here);
+#endif
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
@@ -1876,53 +1904,65 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
     const bool relabel_fits = g->crow != nullptr || 60ull * g->m + 16ull * g->n < free_b / 2;
     GA_TRY(trial(base));
-    for (int rl = 0; rl <= (relabel_fits ? 1 : 0); rl++)
-        for (int io = 0; io <= 2; io++) {
+    // The lane widths that win at the starting block size and load factor need not win at the block size and load
+    // factor found after them (from block 256 and load shift 2, 8-lane groups for d+(h) <= 128 led and were kept at block
+    // 128 and shift 3: 1.58 ms against 1.31 ms on the 34.7M-edge graph). So when the first pass moves the block size or
+    // the load factor, a second pass tries the lane widths, block sizes and load factors again. Every trial that changes
+    // the plan rebuilds it (1-3 s on a 2.6B-edge batch union), so the second pass runs only after a first pass of less
+    // than 20 s.
+    const double t_first = wall_ms();
+    for (int pass = 0; pass < 2; pass++) {
+        const ga_tc_tuning start = bt;
+        for (int rl = 0; pass == 0 && rl <= (relabel_fits ? 1 : 0); rl++)
+            for (int io = 0; io <= 2; io++) {
+                ga_tc_tuning t = bt;
+                t.relabel = rl;
+                t.item_order = io;
+                if (t.relabel == bt.relabel && t.item_order == bt.item_order) continue;
+                GA_TRY(trial(t));
+            }
+        if (pass == 0) {   // how the lane groups read a batch's probe lists: lane slots or length classes
             ga_tc_tuning t = bt;
-            t.relabel = rl;
-            t.item_order = io;
-            if (t.relabel == bt.relabel && t.item_order == bt.item_order) continue;
+            t.probe = 1 - bt.probe;
             GA_TRY(trial(t));
         }
-    {   // how the lane groups read a batch's probe lists: lane slots or length classes
-        ga_tc_tuning t = bt;
-        t.probe = 1 - bt.probe;
-        GA_TRY(trial(t));
-    }
-    GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
-    GA_TRY(ga_tc_ensure_plan(ctx, g, true));
-    const CountView V = count_view(g);
-    const int lane_opts[6] = {4, 8, 16, 32, 64, 1};   // 1: bucket 0 only (one lane per item)
-    for (int b = 0; b < NHASH; b++) {
-        if (!V.P->nitems[b]) continue;
-        if (b < 3 && flat_on(V)) continue;   // light rows go to the flat kernel
-        for (int li = 0; li < (b == 0 ? 6 : 5); li++) {
-            ga_tc_tuning t = bt;
-            t.lanes[b] = lane_opts[li];
-            if (t.lanes[b] == bt.lanes[b]) continue;
-            if (b >= 3 && t.lanes[b] < 16) continue;   // large tables per few lanes would not fit LDS
-            GA_TRY(trial(t));
+        GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
+        GA_TRY(ga_tc_ensure_plan(ctx, g, true));
+        const CountView V = count_view(g);
+        const int lane_opts[6] = {4, 8, 16, 32, 64, 1};   // 1: bucket 0 only (one lane per item)
+        for (int b = 0; b < NHASH; b++) {
+            if (!V.P->nitems[b]) continue;
+            if (b < 3 && flat_on(V)) continue;   // light rows go to the flat kernel
+            for (int li = 0; li < (b == 0 ? 6 : 5); li++) {
+                ga_tc_tuning t = bt;
+                t.lanes[b] = lane_opts[li];
+                if (t.lanes[b] == bt.lanes[b]) continue;
+                if (b >= 3 && t.lanes[b] < 16) continue;   // large tables per few lanes would not fit LDS
+                GA_TRY(trial(t));
+            }
         }
-    }
-    const int blocks[3] = {128, 256, 512};
-    for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.block = blocks[bi]; if (t.block != bt.block) GA_TRY(trial(t)); }
-    for (int ls = 1; ls <= 3; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
-    const int bat[3] = {2, 8, 32};
-    for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
-    const int chunks[3] = {2048, 8192, 32768};
-    for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
-    if (V.P->nlight) {
-        // 4 or 8 units per thread (16 keeps too many records and loads in registers)
-        const int fsh[6][2] = {{0, 256}, {1024, 256}, {2048, 512}, {2048, 256}, {4096, 512}, {4096, 1024}};
-        for (int ci = 0; ci < 6; ci++) {
-            ga_tc_tuning t = bt;
-            t.flat_chunk = fsh[ci][0];
-            t.flat_block = fsh[ci][1];
-            if ((t.flat_chunk == bt.flat_chunk && t.flat_block == bt.flat_block) ||
-                (t.flat_chunk && flat_shmem((uint32_t)t.flat_chunk) > ctx->lds_max))
-                continue;
-            GA_TRY(trial(t));
+        const int blocks[3] = {128, 256, 512};
+        for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.block = blocks[bi]; if (t.block != bt.block) GA_TRY(trial(t)); }
+        for (int ls = 1; ls <= 3; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
+        if (pass == 1) break;
+        const int bat[3] = {2, 8, 32};
+        for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
+        const int chunks[3] = {2048, 8192, 32768};
+        for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
+        if (V.P->nlight) {
+            // 4 or 8 units per thread (16 keeps too many records and loads in registers)
+            const int fsh[6][2] = {{0, 256}, {1024, 256}, {2048, 512}, {2048, 256}, {4096, 512}, {4096, 1024}};
+            for (int ci = 0; ci < 6; ci++) {
+                ga_tc_tuning t = bt;
+                t.flat_chunk = fsh[ci][0];
+                t.flat_block = fsh[ci][1];
+                if ((t.flat_chunk == bt.flat_chunk && t.flat_block == bt.flat_block) ||
+                    (t.flat_chunk && flat_shmem((uint32_t)t.flat_chunk) > ctx->lds_max))
+                    continue;
+                GA_TRY(trial(t));
+            }
         }
+        if ((bt.block == start.block && bt.load_shift == start.load_shift) || wall_ms() - t_first > 20000.0) break;
     }
     GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
     GA_TRY(ga_tc_ensure_plan(ctx, g, true));

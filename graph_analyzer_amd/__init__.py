@@ -222,7 +222,8 @@ class Context:
         return f.value, t.value
 
     def count_many(self, graphs, variant: int = 0):
-        """Exact counts of several resident graphs, launched back to back on this context's stream, one readback."""
+        """Exact counts of several resident graphs on this context's stream with one readback: for the hash variant one
+        launch per bucket shape covers every graph (graphs on the flat kernel, and other variants, get their own)."""
         hs = (_vp * max(len(graphs), 1))(*[g.handle for g in graphs])
         T = np.zeros(max(len(graphs), 1), np.uint64)
         ms = ctypes.c_float()

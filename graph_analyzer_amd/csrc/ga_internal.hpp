@@ -28,6 +28,9 @@ struct ga_ctx {
     size_t dmany_n = 0;
     void* dcore = nullptr;              // grow-only descriptors and results of ga_core_run_many
     size_t dcore_bytes = 0;
+    void* dtcm = nullptr;               // grow-only job tables and workgroup maps of ga_tc_count_many's shared launches
+    size_t dtcm_bytes = 0;
+    std::vector<uint8_t> htcm;          // their host copy: left untouched until the next call (each call ends with a sync)
     uint32_t grid_cap = 0;              // test hook: at most this many workgroups per count / peel / core launch (0: none)
     std::string err;
 };

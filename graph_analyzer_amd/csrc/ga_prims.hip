@@ -90,6 +90,7 @@ extern "C" void ga_ctx_destroy(ga_ctx* c) {
     ga_free(c->scratch);
     ga_free(c->dmany);
     ga_free(c->dcore);
+    ga_free(c->dtcm);
     ga_free(c->dsmall);
     if (c->hstat) (void)hipHostFree(c->hstat);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -244,26 +244,28 @@ struct TcArgs {
 struct ItemRange {
     uint64_t first, end, step;
 };
-__device__ __forceinline__ ItemRange item_range(const TcArgs& a, uint32_t step, uint32_t offset) {
+// bid / nb: this workgroup and the workgroups of the grid that share the items (the whole grid, or in a launch over
+// several graphs the workgroups of this graph).
+__device__ __forceinline__ ItemRange item_range(const TcArgs& a, uint32_t step, uint32_t offset, uint32_t bid, uint32_t nb) {
     ItemRange r;
-    if (a.xcd && gridDim.x >= 8u) {   // (a grid of fewer than 8 workgroups would leave eighths without one)
+    if (a.xcd && nb >= 8u) {   // (a grid of fewer than 8 workgroups would leave eighths without one)
         // XCD x (workgroups b = x, x + 8, ...) takes the x-th eighth of the items, grid-strided over its workgroups
-        const uint32_t x = blockIdx.x % 8u, j = blockIdx.x / 8u;
-        const uint32_t nx = (gridDim.x - x + 7u) / 8u;   // workgroups on XCD x
+        const uint32_t x = bid % 8u, j = bid / 8u;
+        const uint32_t nx = (nb - x + 7u) / 8u;   // workgroups on XCD x
         const uint64_t lo = (uint64_t)a.nitems * x / 8u, hi = (uint64_t)a.nitems * (x + 1u) / 8u;
         r.first = lo + (uint64_t)j * step + offset;
         r.end = hi;
         r.step = (uint64_t)nx * step;
     } else {
-        r.first = (uint64_t)blockIdx.x * step + offset;
+        r.first = (uint64_t)bid * step + offset;
         r.end = a.nitems;
-        r.step = (uint64_t)gridDim.x * step;
+        r.step = (uint64_t)nb * step;
     }
     return r;
 }
 
 template <int G, bool HASHED, int FLAGS>
-__global__ void k_tc_hash(const TcArgs a) {
+__device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint32_t nb) {
     static_assert(G >= 4 && G <= 64 && (64 % G) == 0, "G must divide 64");
     extern __shared__ uint32_t lds[];
     if (a.done && *a.done) return;
@@ -292,7 +294,7 @@ __global__ void k_tc_hash(const TcArgs a) {
         __syncthreads();
     }
     uint64_t local = 0;
-    const ItemRange ir = item_range(a, gpb, gib);
+    const ItemRange ir = item_range(a, gpb, gib, bid, nb);
     for (uint64_t it = ir.first; it < ir.end; it += ir.step) {
         const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
         const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK);
@@ -601,13 +603,13 @@ __global__ void k_tc_hash(const TcArgs a) {
 // table, no lane-group synchronisation, 64 items in flight per wavefront. Each entry's metadata is fetched one entry
 // ahead. The item loop is wavefront-uniform so that per-graph counts can be summed per wavefront.
 template <int FLAGS>
-__global__ void k_tc_tiny(const TcArgs a) {
+__device__ __forceinline__ void tc_tiny_body(const TcArgs& a, uint32_t bid, uint32_t nb) {
     static_assert(FLAGS == 0 || FLAGS == F_GRAPH, "plain and per-graph counts only");
     if (a.done && *a.done) return;
     const uint32_t* __restrict__ col = a.col;
     const uint32_t lane = lane_id();
     uint64_t local = 0;
-    const ItemRange ir = item_range(a, blockDim.x, threadIdx.x & ~63u);
+    const ItemRange ir = item_range(a, blockDim.x, threadIdx.x & ~63u, bid, nb);
     for (uint64_t base = ir.first; base < ir.end; base += ir.step) {   // wavefront-uniform
         const uint64_t it = base + lane;
         uint32_t c = 0, gr = 0;
@@ -655,6 +657,36 @@ __global__ void k_tc_tiny(const TcArgs a) {
     }
     local = wave_sum_u64(local);
     if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
+}
+
+template <int G, bool HASHED, int FLAGS>
+__global__ void k_tc_hash(const TcArgs a) {
+    tc_hash_body<G, HASHED, FLAGS>(a, blockIdx.x, gridDim.x);
+}
+
+template <int FLAGS>
+__global__ void k_tc_tiny(const TcArgs a) {
+    tc_tiny_body<FLAGS>(a, blockIdx.x, gridDim.x);
+}
+
+// Counts of several resident graphs in one launch per bucket shape: job i holds graph i's arguments for the bucket
+// (its items, tables and its own count word) and a range [base, base + nblk) of the launch's workgroups; map[b] is the
+// job of workgroup b. Each graph's workgroups run exactly what its own launch would run on a grid of nblk.
+struct TcJob {
+    TcArgs a;
+    uint32_t base, nblk;
+};
+
+template <int G, bool HASHED, int FLAGS>
+__global__ void k_tc_hash_many(const TcJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    const TcJob& J = jobs[map[blockIdx.x]];
+    tc_hash_body<G, HASHED, FLAGS>(J.a, blockIdx.x - J.base, J.nblk);
+}
+
+template <int FLAGS>
+__global__ void k_tc_tiny_many(const TcJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    const TcJob& J = jobs[map[blockIdx.x]];
+    tc_tiny_body<FLAGS>(J.a, blockIdx.x - J.base, J.nblk);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1415,33 +1447,51 @@ static uint32_t bucket_slots(uint32_t bound, uint32_t shift) {
     return 1u << bits;
 }
 
+// Launch shape of a lane-group bucket (also used for the shared launches of ga_tc_count_many, so that each graph there
+// runs exactly what its own launch runs).
+struct HashShape {
+    uint32_t block, ts, shift;
+    size_t shmem;
+    unsigned grid;
+    bool fits;   // the tables fit one workgroup's LDS (else the bucket binary-searches N+(h) in HBM)
+};
+
+static HashShape hash_shape(const ga_ctx* ctx, const ga_graph* g, const TcPlan& P, uint32_t G, bool hashed, int flags, uint32_t nitems,
+                            uint32_t bound) {
+    HashShape s;
+    s.block = (uint32_t)P.tune.block;
+    const bool counters = (flags & (F_TRI | F_SUP)) != 0;
+    // tables with per-slot counters (participation, supports) take twice the LDS: at most load shift 2 there
+    s.shift = counters ? std::min<uint32_t>((uint32_t)P.tune.load_shift, 2u) : (uint32_t)P.tune.load_shift;
+    // LDS reserved per lane group: the bucket's largest row at load shift <= 2 (a quarter full at most). Reserving for
+    // shift 3 doubled the tables of the 32- and 64-lane buckets and held them to 4 waves per SIMD by LDS.
+    s.ts = hashed ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), std::min<uint32_t>(s.shift, 2u)) : 0;
+    const uint32_t per_group = (s.ts * (counters ? 2u : 1u) + 4u * G) * 4u;
+    const bool tc = (flags & F_TRI) && !(flags & F_GRAPH);
+    const size_t cache_bytes = tc ? 2u * TRI_CACHE * 4u : 0u;   // the workgroup's participation counters
+    while (s.block > 64 && (size_t)(s.block / G) * per_group + cache_bytes > 64 * 1024) s.block >>= 1;
+    s.shmem = (size_t)(s.block / G) * per_group + cache_bytes;
+    s.fits = !hashed || s.shmem <= ctx->lds_max;
+    // participation: fewer, longer-lived workgroups, so that each counter cache gathers more items before its flush
+    // (64 -> 16 per CU: 27.3 -> 23.0 ms for the R-MAT scale-21 peel, 9.5 -> 8.6 ms on the 34.7M-edge graph)
+    s.grid = ctx_grid(ctx, grid_for((uint64_t)nitems * G, s.block, (unsigned)std::max(ctx->cus, 1) * (tc ? 16u : 64u)));
+    return s;
+}
+
 template <int G, bool HASHED, int FLAGS>
 static int launch_hash_bucket(ga_ctx* ctx, ga_graph* g, const CountView& V, const uint32_t* items, uint32_t nitems, uint32_t bound,
                               TcArgs a) {
     if (!nitems) return GA_OK;
     const TcPlan& P = *V.P;
-    uint32_t block = (uint32_t)P.tune.block;
-    // tables with per-slot counters (participation, supports) take twice the LDS: at most load shift 2 there
-    const uint32_t shift = (FLAGS & (F_TRI | F_SUP)) ? std::min<uint32_t>((uint32_t)P.tune.load_shift, 2u) : (uint32_t)P.tune.load_shift;
-    // LDS reserved per lane group: the bucket's largest row at load shift <= 2 (a quarter full at most). Reserving for
-    // shift 3 doubled the tables of the 32- and 64-lane buckets and held them to 4 waves per SIMD by LDS.
-    const uint32_t ts = HASHED ? bucket_slots(std::min(bound, std::max(g->max_out, 1u)), std::min<uint32_t>(shift, 2u)) : 0;
-    const uint32_t per_group = (ts * ((FLAGS & (F_SUP | F_TRI)) ? 2u : 1u) + 4u * G) * 4u;
-    constexpr bool TC = (FLAGS & F_TRI) && !(FLAGS & F_GRAPH);
-    const size_t cache_bytes = TC ? 2u * TRI_CACHE * 4u : 0u;   // the workgroup's participation counters
-    while (block > 64 && (size_t)(block / G) * per_group + cache_bytes > 64 * 1024) block >>= 1;
-    const size_t shmem = (size_t)(block / G) * per_group + cache_bytes;
-    if (HASHED && shmem > ctx->lds_max)   // the tables do not fit one workgroup: binary-search this bucket in HBM
+    const HashShape sh = hash_shape(ctx, g, P, G, HASHED, FLAGS, nitems, bound);
+    if (!sh.fits)   // the tables do not fit one workgroup: binary-search this bucket in HBM
         return launch_hash_bucket<64, false, FLAGS>(ctx, g, V, items, nitems, bound, a);
-    // participation: fewer, longer-lived workgroups, so that each counter cache gathers more items before its flush
-    // (64 -> 16 per CU: 27.3 -> 23.0 ms for the R-MAT scale-21 peel, 9.5 -> 8.6 ms on the 34.7M-edge graph)
-    const unsigned grid = ctx_grid(ctx, grid_for((uint64_t)nitems * G, block, (unsigned)std::max(ctx->cus, 1) * (TC ? 16u : 64u)));
     a.items = items;
     a.nitems = nitems;
-    a.ts_max = ts;
-    a.load_shift = shift;
+    a.ts_max = sh.ts;
+    a.load_shift = sh.shift;
     a.xcd = (P.built.item_order == 3 || P.built.item_order == 5) ? 1u : 0u;
-    k_tc_hash<G, HASHED, FLAGS><<<grid, block, shmem, ctx->stream>>>(a);
+    k_tc_hash<G, HASHED, FLAGS><<<sh.grid, sh.block, sh.shmem, ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_hash");
 }
 
@@ -1777,6 +1827,146 @@ extern "C" int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, f
     return GA_OK;
 }
 
+// ---- shared launches of ga_tc_count_many (hash variant) ----
+// A launch group: graphs whose bucket runs the same kernel (kind, lanes, probe form, block); one launch covers them.
+struct ManyGroup {
+    int kind;    // 0: one lane per item (k_tc_tiny), 1: lane groups with LDS tables, 2: binary search in HBM
+    int G, flags;
+    uint32_t block;
+    size_t shmem = 0;
+    std::vector<TcJob> jobs;
+    std::vector<unsigned> grid;   // each job's own grid
+};
+
+template <int FLAGS>
+static void launch_many_hash(ga_ctx* ctx, int G, bool hashed, unsigned nwg, uint32_t block, size_t shmem, const TcJob* jobs, const uint32_t* map) {
+    if (!hashed) { k_tc_hash_many<64, false, 0><<<nwg, block, shmem, ctx->stream>>>(jobs, map); return; }
+    switch (G) {
+        case 4: k_tc_hash_many<4, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
+        case 8: k_tc_hash_many<8, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
+        case 16: k_tc_hash_many<16, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
+        case 32: k_tc_hash_many<32, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
+        default: k_tc_hash_many<64, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
+    }
+}
+
+// Plain counts of several resident graphs into counts[i] with one launch per bucket shape instead of one launch set per
+// graph. Graphs whose tuning runs the flat kernel are counted by their own launches (solo).
+static int count_many_shared(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, unsigned long long* counts) {
+    std::vector<ManyGroup> groups;
+    std::vector<uint32_t> solo;
+    auto group_of = [&](int kind, int G, int flags, uint32_t block) -> ManyGroup& {
+        for (auto& gr : groups)
+            if (gr.kind == kind && gr.G == G && gr.flags == flags && gr.block == block) return gr;
+        groups.push_back(ManyGroup{kind, G, flags, block});
+        return groups.back();
+    };
+    const unsigned cus = (unsigned)std::max(ctx->cus, 1);
+    for (uint32_t i = 0; i < count; i++) {
+        ga_graph* g = graphs[i];
+        const CountView V = count_view(g);
+        const TcPlan& P = *V.P;
+        if (flat_on(V)) {
+            solo.push_back(i);
+            continue;
+        }
+        TcArgs base{};
+        base.rowptr = V.rowptr;
+        base.col = V.col;
+        base.wcol = P.wcol;
+        base.wedge = P.wedge;
+        base.wps = P.wps;
+        base.wlen = P.wlen;
+        base.total = counts + i;
+        base.xcd = (P.built.item_order == 3 || P.built.item_order == 5) ? 1u : 0u;
+        uint32_t off = 0;
+        for (int b = 0; b <= NHASH; b++) {   // b = NHASH: the binary-search bucket
+            const uint32_t ni = b < NHASH ? P.nitems[b] : P.nhuge;
+            TcArgs a = base;
+            a.items = P.items + 4ull * off;
+            a.nitems = ni;
+            off += ni;
+            if (!ni) continue;
+            if (b < NHASH && P.tune.lanes[b] == 1) {   // bucket 0, one lane per item
+                ManyGroup& gr = group_of(0, 1, 0, (uint32_t)P.tune.block);
+                gr.jobs.push_back(TcJob{a, 0, 0});
+                gr.grid.push_back(ctx_grid(ctx, grid_for(ni, gr.block, cus * 64u)));
+                continue;
+            }
+            int G = b < NHASH ? P.tune.lanes[b] : 64;
+            bool hashed = b < NHASH;
+            const uint32_t bound = b < NHASH ? std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap) : 0u;
+            int flags = hashed && P.tune.probe ? F_SLOTS : 0;
+            HashShape sh = hash_shape(ctx, g, P, (uint32_t)G, hashed, flags, ni, bound);
+            if (!sh.fits) {   // as the graph's own launch: binary search in HBM
+                G = 64;
+                hashed = false;
+                flags = 0;
+                sh = hash_shape(ctx, g, P, 64u, false, 0, ni, bound);
+            }
+            a.ts_max = sh.ts;
+            a.load_shift = sh.shift;
+            ManyGroup& gr = group_of(hashed ? 1 : 2, G, flags, sh.block);
+            gr.shmem = std::max(gr.shmem, sh.shmem);
+            gr.jobs.push_back(TcJob{a, 0, 0});
+            gr.grid.push_back(sh.grid);
+        }
+    }
+    // workgroup ranges: each job its own grid, scaled down together when a launch would exceed 64 workgroups per CU
+    size_t jobs_total = 0, map_total = 0;
+    std::vector<std::vector<uint32_t>> maps(groups.size());
+    for (size_t q = 0; q < groups.size(); q++) {
+        ManyGroup& gr = groups[q];
+        uint64_t sum = 0;
+        for (unsigned x : gr.grid) sum += x;
+        const uint64_t budget = 64ull * cus;
+        for (size_t j = 0; j < gr.jobs.size(); j++) {
+            unsigned nb = gr.grid[j];
+            if (sum > budget) nb = std::max(1u, (unsigned)((uint64_t)nb * budget / sum));
+            gr.jobs[j].base = (uint32_t)maps[q].size();
+            gr.jobs[j].nblk = nb;
+            maps[q].insert(maps[q].end(), nb, (uint32_t)(jobs_total + j));
+        }
+        jobs_total += gr.jobs.size();
+        map_total += maps[q].size();
+    }
+    if (jobs_total) {
+        const size_t jobs_bytes = (jobs_total * sizeof(TcJob) + 255) & ~(size_t)255;
+        const size_t need = jobs_bytes + map_total * 4;
+        if (need > ctx->dtcm_bytes) {   // grow-only, used by one call at a time on this context's stream
+            ga_free(ctx->dtcm);
+            ctx->dtcm = nullptr;
+            ctx->dtcm_bytes = 0;
+            GA_TRY(ga_malloc(ctx, &ctx->dtcm, need * 2));
+            ctx->dtcm_bytes = need * 2;
+        }
+        std::vector<uint8_t>& host = ctx->htcm;
+        host.resize(need);
+        size_t jo = 0, mo = jobs_bytes;
+        for (size_t q = 0; q < groups.size(); q++) {
+            memcpy(host.data() + jo, groups[q].jobs.data(), groups[q].jobs.size() * sizeof(TcJob));
+            jo += groups[q].jobs.size() * sizeof(TcJob);
+            memcpy(host.data() + mo, maps[q].data(), maps[q].size() * 4);
+            mo += maps[q].size() * 4;
+        }
+        // ctx->htcm stays untouched until the stream sync that ends ga_tc_count_many
+        GA_HIP(ctx, hipMemcpyAsync(ctx->dtcm, host.data(), need, hipMemcpyHostToDevice, ctx->stream));
+        const TcJob* djobs = (const TcJob*)ctx->dtcm;
+        const uint32_t* dmap = (const uint32_t*)((const char*)ctx->dtcm + jobs_bytes);
+        for (size_t q = 0; q < groups.size(); q++) {
+            const ManyGroup& gr = groups[q];
+            const unsigned nwg = (unsigned)maps[q].size();
+            if (gr.kind == 0) k_tc_tiny_many<0><<<nwg, gr.block, 0, ctx->stream>>>(djobs, dmap);
+            else if (gr.flags & F_SLOTS) launch_many_hash<F_SLOTS>(ctx, gr.G, gr.kind == 1, nwg, gr.block, gr.shmem, djobs, dmap);
+            else launch_many_hash<0>(ctx, gr.G, gr.kind == 1, nwg, gr.block, gr.shmem, djobs, dmap);
+            GA_TRY(ga_launch_check(ctx, "k_tc_hash_many"));
+            dmap += nwg;
+        }
+    }
+    for (uint32_t i : solo) GA_TRY(ga_tc_launch(ctx, graphs[i], GA_TC_HASH, counts + i, nullptr, nullptr, nullptr, nullptr));
+    return GA_OK;
+}
+
 extern "C" int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, int variant, uint64_t* T, float* ms) {
     if (!ctx || (count && (!graphs || !T))) return GA_ERR_ARG;
     if (variant < 0 || variant >= GA_TC_NVARIANTS) return ga_fail(ctx, GA_ERR_ARG, "unknown variant %d", variant);
@@ -1797,8 +1987,12 @@ extern "C" int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t c
     if (!count) return GA_OK;
     GA_TRY(ga_fill_u64(ctx, ctx->dmany, count, 0));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    for (uint32_t i = 0; i < count; i++)
-        GA_TRY(ga_tc_launch(ctx, graphs[i], variant, (unsigned long long*)ctx->dmany + i, nullptr, nullptr, nullptr, nullptr));
+    if ((variant == GA_TC_AUTO || variant == GA_TC_HASH) && count > 1) {
+        GA_TRY(count_many_shared(ctx, graphs, count, (unsigned long long*)ctx->dmany));
+    } else {
+        for (uint32_t i = 0; i < count; i++)
+            GA_TRY(ga_tc_launch(ctx, graphs[i], variant, (unsigned long long*)ctx->dmany + i, nullptr, nullptr, nullptr, nullptr));
+    }
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     GA_HIP(ctx, hipMemcpyAsync(T, ctx->dmany, (size_t)count * 8, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));

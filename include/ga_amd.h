@@ -144,8 +144,11 @@ const char* ga_tc_variant_name(int variant);
  * counting launch(es), measured with HIP events on the context stream. Preparation of the count plan (GA_TC_HASH,
  * GA_TC_SORTED) happens on first use and is excluded; call ga_tc_prepare to time it. */
 int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, float* ms);
-/* Counts `count` resident graphs back to back on ctx's stream with one readback at the end (T[i] for graphs[i]);
- * plans are built first where missing. The serving loop's batched count. No counterpart in the reference. */
+/* Counts `count` resident graphs on ctx's stream with one readback at the end (T[i] for graphs[i]); plans are built
+ * first where missing. GA_TC_HASH / GA_TC_AUTO with two or more graphs: one launch per bucket shape covers every
+ * graph (a device table of per-graph arguments and a workgroup-to-graph map; each graph runs what its own launch
+ * would); graphs whose tuning uses the flat kernel, and other variants, get their own launches. The serving loop's
+ * batched count. No counterpart in the reference. */
 int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, int variant, uint64_t* T, float* ms);
 /* ga_tc_count for GA_TC_NOSRC or GA_TC_VERTEX that allocates nothing. No counterpart in the reference: it is for callers
  * that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid allocation). The

@@ -151,3 +151,39 @@ def test_device_probes(ctx):
         assert bar_us > 0 and launch_us > 0
     with pytest.raises(ga.GAError):
         ctx.barrier_probe(cus * 1000, 10)   # cannot all be resident
+
+
+def test_count_many_shared_launches(ctx):
+    """ga_tc_count_many over separately resident graphs (one launch per bucket shape for all of them) equals each
+    graph's own count and the oracle: graphs of every size and shape class, mixed tunings (one lane per item, lane
+    groups in both probe forms, the renumbered copy, the binary-search bucket, and the flat kernel, which takes the
+    per-graph path), empty and loop-only graphs, two calls in a row (the second on fewer graphs) and a capped grid."""
+    graphs = mix(40, 21, nmax=4000)
+    graphs.append(np.array(clique(60), np.uint32))
+    ref = [oracle.OGraph.from_pairs(p).triangles() for p in graphs]
+    dgs = [ga.DeviceGraph.from_pairs(ctx, p) for p in graphs]
+    shapes = (([1, 16, 32, 64, 64], 0, 4096, 1, 0), ([4, 8, 16, 32, 64], 0, 4096, 0, 1), ([1, 32, 16, 64, 64], 0, 8, 2, 1),
+              ([8, 16, 32, 64, 64], 1024, 4096, 1, 0))
+    for i, g in enumerate(dgs):
+        if i % 5 == 4:
+            continue   # library defaults
+        lanes, flat, cap, order, probe = shapes[i % 4]
+        t = ga.TcTuning.from_dict(ga.default_tuning().as_dict())
+        t.lanes[:] = lanes
+        t.flat_chunk, t.flat_block = (flat, 256) if flat else (0, 256)
+        t.hash_cap, t.item_order, t.probe, t.relabel = cap, order, probe, i % 3 == 0
+        g.set_tuning(t)
+    own = [g.count(ga.TC_HASH)[0] for g in dgs]
+    assert own == ref
+    for rep, sel in enumerate((list(range(len(dgs))), list(range(0, len(dgs), 3)))):
+        T, ms = ctx.count_many([dgs[i] for i in sel], ga.TC_HASH)
+        assert [int(x) for x in T] == [ref[i] for i in sel], rep
+        assert ms > 0
+    ctx.set_grid_cap(3)
+    try:
+        T, _ = ctx.count_many(dgs, ga.TC_HASH)
+        assert [int(x) for x in T] == ref
+    finally:
+        ctx.set_grid_cap(0)
+    for g in dgs:
+        g.close()

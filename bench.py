@@ -380,8 +380,8 @@ def run_serving(rates, duration):
     out = {"workload": f"poisson_{duration:g}s_per_rate", "duration_s": duration,
            "mix": "count / peel eps 1/20 / core uniform; 95% on 256 graphs of the batch size law, 5% on 2 R-MAT scale-20 graphs",
            "sweep": []}
-    forms = (("continuous", dict(slots=16, count_batch=64, core_batch=1024, states_per_graph=16)),
-             ("plain", dict(slots=1, count_batch=1, core_batch=1, states_per_graph=1)))
+    forms = (("continuous", dict(slots=16, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16)),
+             ("plain", dict(slots=1, count_batch=1, core_batch=1, peel_batch=1, states_per_graph=1)))
     rows = {rate: {"offered_rps": rate, "requests": len(sweep[rate])} for rate in rates}
     values = {}
     for name, kw in forms:

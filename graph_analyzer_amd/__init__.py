@@ -124,6 +124,8 @@ def lib() -> ctypes.CDLL:
                                                 ctypes.POINTER(ctypes.c_float)]),
             "ga_ctx_mem_info": (ctypes.c_int, [_vp, _u64p, _u64p]),
             "ga_ctx_set_grid_cap": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+            "ga_peel_run_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, ctypes.c_uint64, ctypes.c_uint64,
+                                                 _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_peel_set_ctx": (ctypes.c_int, [_vp, _vp]),
             "ga_core_run_many": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
             "ga_core_run_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_uint32, _u64p, _u32p, ctypes.POINTER(ctypes.c_float)]),
@@ -760,6 +762,28 @@ def core_many(ctx: "Context", cores: Sequence["Core"], T_full: Optional[Sequence
     ctx.check(lib().ga_core_run_many(ctx.handle, hs, len(cores), None if tf is None else _ptr(tf, _u64p), _ptr(stats, _u32p),
                                      ctypes.byref(ms)), "core_run_many")
     return [(c.core(), int(stats[2 * i]), int(stats[2 * i + 1])) for i, c in enumerate(cores)], ms.value
+
+
+def peel_batch(ctx: "Context", peels: Sequence["Peel"], num: int = 1, den: int = 10,
+               T_full: Optional[Sequence[Optional[int]]] = None):
+    """Fused peels of several resident graphs in shared launches (ga_peel_run_batch: every round's launches cover all
+    unfinished states). The states are moved to ctx. Returns ([PeelResult per state], device ms)."""
+    for p in peels:
+        if p.ctx is not ctx:
+            p.set_ctx(ctx)
+    hs = (_vp * max(len(peels), 1))(*[p._h for p in peels])
+    tf = None
+    if T_full is not None:
+        tf = np.array([(1 << 64) - 1 if t is None else int(t) for t in T_full], np.uint64)
+    stats = np.zeros(max(4 * len(peels), 4), np.uint64)
+    ms = ctypes.c_float()
+    ctx.check(lib().ga_peel_run_batch(ctx.handle, hs, len(peels), None if tf is None else _ptr(tf, _u64p), num, den,
+                                      _ptr(stats, _u64p), ctypes.byref(ms)), "peel_run_batch")
+    out = [PeelResult(int(stats[4 * i]), int(stats[4 * i + 1]), int(stats[4 * i + 2]), int(stats[4 * i + 3]), p.removed_round())
+           for i, p in enumerate(peels)]
+    for p in peels:
+        p.last_ms = ms.value
+    return out, ms.value
 
 
 def core_batch(ctx: "Context", cores: Sequence["Core"], T_full: Optional[Sequence[Optional[int]]] = None):

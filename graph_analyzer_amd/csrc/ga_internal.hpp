@@ -122,6 +122,14 @@ int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g, bool counts_only = false);
 // ngraphs + 1 device words), as ga_batch_count launches it.
 // Host wall clock in milliseconds (steady clock).
 double wall_ms();
+// Participation counts of several graphs in shared launches (one per bucket shape): graph i with alive[i], tri[i],
+// its triangle total in *total[i] and its device flag done[i]. Built and uploaded once (into the context's buffer of
+// ga_tc_count_many: one such call at a time on a context), then launched any number of times on ctx's stream.
+struct ga_tc_many;
+int ga_tc_participation_many(ga_ctx* ctx, ga_graph* const* graphs, const uint8_t* const* alive, uint32_t* const* tri,
+                             unsigned long long* const* total, const uint32_t* const* done, uint32_t count, ga_tc_many** out);
+int ga_tc_many_launch(ga_ctx* ctx, const ga_tc_many* M);
+void ga_tc_many_free(ga_tc_many* M);
 int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, ga_tc_tuning* best,
                       float* best_ms);
 // One count of `variant` into *d_total (zeroed by the caller). Optional: alive mask, per-vertex tri, per-edge sup, a device

@@ -28,10 +28,12 @@ struct ga_peel {
 };
 
 // tri[v] * nalive * den <= 3 * (den + num) * T  (kernels.cuh:427-459, densest.h:151-154): remove v, record the round.
+// bid / nb: this workgroup and the workgroups sharing the graph (the grid, or the graph's range of a shared launch)
 template <bool FUSED>
-__global__ void k_mark_peel(const uint32_t* __restrict__ tri, uint8_t* __restrict__ alive, uint32_t* __restrict__ removed_round,
-                            uint32_t n, unsigned long long nalive_a, unsigned long long num_a, unsigned long long den_a,
-                            uint32_t round_a, PeelDev* __restrict__ st) {
+__device__ __forceinline__ void mark_peel_body(const uint32_t* __restrict__ tri, uint8_t* __restrict__ alive, uint32_t* __restrict__ removed_round,
+                                               uint32_t n, unsigned long long nalive_a, unsigned long long num_a,
+                                               unsigned long long den_a, uint32_t round_a, PeelDev* __restrict__ st, uint32_t bid,
+                                               uint32_t nb) {
     if (FUSED && st->done) return;
     const unsigned long long nalive = FUSED ? st->nalive : nalive_a;
     const unsigned long long num = FUSED ? st->num : num_a;
@@ -42,7 +44,7 @@ __global__ void k_mark_peel(const uint32_t* __restrict__ tri, uint8_t* __restric
     uint64_t local = 0;
     // four vertices per lane: one 32-bit load of alive[] and one 128-bit load of tri[] when aligned
     const uint32_t nq = n / 4;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+    for (uint32_t q = bid * blockDim.x + threadIdx.x; q < nq; q += nb * blockDim.x) {
         uint32_t a4 = reinterpret_cast<const uint32_t*>(alive)[q];
         if (!a4) continue;
         const uint4 t4 = reinterpret_cast<const uint4*>(tri)[q];
@@ -58,7 +60,7 @@ __global__ void k_mark_peel(const uint32_t* __restrict__ tri, uint8_t* __restric
         }
         if (na4 != a4) reinterpret_cast<uint32_t*>(alive)[q] = na4;
     }
-    for (uint32_t v = nq * 4 + blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    for (uint32_t v = nq * 4 + bid * blockDim.x + threadIdx.x; v < n; v += nb * blockDim.x) {
         if (alive[v] && (unsigned long long)tri[v] * mul <= rhs) {
             alive[v] = 0;
             removed_round[v] = round;
@@ -69,18 +71,31 @@ __global__ void k_mark_peel(const uint32_t* __restrict__ tri, uint8_t* __restric
     if (lane_id() == 0 && local) atomicAdd(&st->removed, (unsigned long long)local);
 }
 
+template <bool FUSED>
+__global__ void k_mark_peel(const uint32_t* __restrict__ tri, uint8_t* __restrict__ alive, uint32_t* __restrict__ removed_round,
+                            uint32_t n, unsigned long long nalive_a, unsigned long long num_a, unsigned long long den_a,
+                            uint32_t round_a, PeelDev* __restrict__ st) {
+    mark_peel_body<FUSED>(tri, alive, removed_round, n, nalive_a, num_a, den_a, round_a, st, blockIdx.x, gridDim.x);
+}
+
 // Zeroes tri[] and the round counters (replaces the two cudaMemset calls of gpu_densest.cu:101-104).
 template <bool FUSED>
-__global__ void k_peel_begin(uint32_t* __restrict__ tri, uint32_t n, PeelDev* __restrict__ st) {
+__device__ __forceinline__ void peel_begin_body(uint32_t* __restrict__ tri, uint32_t n, PeelDev* __restrict__ st, uint32_t bid, uint32_t nb) {
     if (FUSED && st->done) return;
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) tri[v] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (uint32_t v = bid * blockDim.x + threadIdx.x; v < n; v += nb * blockDim.x) tri[v] = 0;
+    if (bid == 0 && threadIdx.x == 0) {
         st->T = 0;
         st->removed = 0;
     }
 }
 
-__global__ void k_peel_init(PeelDev* st, unsigned long long n, unsigned long long T_full, unsigned long long num, unsigned long long den) {
+template <bool FUSED>
+__global__ void k_peel_begin(uint32_t* __restrict__ tri, uint32_t n, PeelDev* __restrict__ st) {
+    peel_begin_body<FUSED>(tri, n, st, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void peel_init_dev(PeelDev* st, unsigned long long n, unsigned long long T_full, unsigned long long num,
+                                              unsigned long long den) {
     st->T = 0;
     st->removed = 0;
     st->nalive = n;
@@ -96,6 +111,10 @@ __global__ void k_peel_init(PeelDev* st, unsigned long long n, unsigned long lon
     st->error = 0;
 }
 
+__global__ void k_peel_init(PeelDev* st, unsigned long long n, unsigned long long T_full, unsigned long long num, unsigned long long den) {
+    peel_init_dev(st, n, T_full, num, den);
+}
+
 // Device restatement of peel_arith_ok (densest.h:65-72) for the round-0 count.
 __device__ bool peel_arith_ok_dev(unsigned long long n, unsigned long long T0, unsigned long long num, unsigned long long den) {
     if (den < 1 || den > (1ull << 20) || num > (1ull << 20)) return false;
@@ -106,7 +125,7 @@ __device__ bool peel_arith_ok_dev(unsigned long long n, unsigned long long T0, u
 }
 
 // End of a fused round: the rules of batch_peel_loop (densest.h:288-301).
-__global__ void k_peel_end(PeelDev* st, unsigned long long n) {
+__device__ void peel_end_dev(PeelDev* st, unsigned long long n) {
     if (st->done) return;
     const unsigned long long T = st->T, removed = st->removed, nalive = st->nalive;
     const uint32_t round = st->round;
@@ -124,6 +143,43 @@ __global__ void k_peel_end(PeelDev* st, unsigned long long n) {
     st->rounds = round + 1;
     st->round = round + 1;
     if (st->nalive == 0) st->done = 1;
+}
+
+__global__ void k_peel_end(PeelDev* st, unsigned long long n) { peel_end_dev(st, n); }
+
+// Several graphs' peels in shared launches (ga_peel_run_batch): job i is graph i's state with its ranges of the begin
+// (q = 0) and mark (q = 1) launches; every kernel returns at once for a finished state.
+struct PeelJob {
+    uint32_t* tri;
+    uint8_t* alive;
+    uint32_t* removed_round;
+    PeelDev* st;
+    unsigned long long n, T_full, num, den;
+    uint32_t base[2], nblk[2];
+};
+
+__global__ void k_peel_init_many(const PeelJob* __restrict__ jobs, uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        peel_init_dev(jobs[i].st, jobs[i].n, jobs[i].T_full, jobs[i].num, jobs[i].den);
+}
+
+__global__ void k_peel_begin_many(const PeelJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    const PeelJob& J = jobs[map[blockIdx.x]];
+    peel_begin_body<true>(J.tri, (uint32_t)J.n, J.st, blockIdx.x - J.base[0], J.nblk[0]);
+}
+
+__global__ void k_mark_peel_many(const PeelJob* __restrict__ jobs, const uint32_t* __restrict__ map) {
+    const PeelJob& J = jobs[map[blockIdx.x]];
+    mark_peel_body<true>(J.tri, J.alive, J.removed_round, (uint32_t)J.n, 0, 0, 0, 0, J.st, blockIdx.x - J.base[1], J.nblk[1]);
+}
+
+__global__ void k_peel_end_many(const PeelJob* __restrict__ jobs, uint32_t count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) peel_end_dev(jobs[i].st, jobs[i].n);
+}
+
+// Every state's record into one array (one readback per block of rounds).
+__global__ void k_peel_gather(const PeelJob* __restrict__ jobs, uint32_t count, PeelDev* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) out[i] = *jobs[i].st;
 }
 
 static unsigned vgrid(ga_ctx* ctx, uint32_t n) { return ctx_grid(ctx, grid_for(((uint64_t)n + 3) / 4, 256, (unsigned)std::max(ctx->cus, 1) * 8u)); }
@@ -261,6 +317,128 @@ extern "C" int ga_peel_run(ga_peel* p, uint64_t T_full, uint64_t num, uint64_t d
         stats[1] = h->best_den;
         stats[2] = h->best_round;
         stats[3] = h->rounds;
+    }
+    return GA_OK;
+}
+
+// Several graphs' fused peels in shared launches: each round is one begin, one participation launch per bucket shape,
+// one mark and one end launch for all unfinished states, in blocks of rounds with one readback of every state per
+// block. Each state follows exactly the rounds it follows alone.
+extern "C" int ga_peel_run_batch(ga_ctx* ctx, ga_peel* const* peels, uint32_t count, const uint64_t* T_full, uint64_t num, uint64_t den,
+                                 uint64_t* stats, float* ms) {
+    if (!ctx || (count && (!peels || !stats))) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    if (!count) return GA_OK;
+    for (uint32_t i = 0; i < count; i++) {
+        ga_peel* p = peels[i];
+        if (!p || p->ctx->device != ctx->device) return ga_fail(ctx, GA_ERR_ARG, "peel state %u is NULL or on another device", i);
+        if (p->variant != GA_TC_HASH) return ga_fail(ctx, GA_ERR_ARG, "peel state %u: shared launches run the hash variant", i);
+        const uint64_t tf = T_full ? T_full[i] : UINT64_MAX;
+        const char* why = tf != UINT64_MAX ? peel_arith_why(p->g->n, tf, num, den)
+                                           : (den < 1 || den > (1ull << 20) || num > (1ull << 20))
+                                                 ? "eps = num/den needs 1 <= den <= 2^20 and num <= 2^20" : nullptr;
+        if (why) return ga_fail(ctx, GA_ERR_ARITH, "peel state %u: %s", i, why);
+        for (uint32_t j = 0; j < i; j++)
+            if (peels[j] == p) return ga_fail(ctx, GA_ERR_ARG, "peel state %u appears twice", i);
+    }
+    for (uint32_t i = 0; i < count; i++) peels[i]->ctx = ctx;   // idle states: every call on a state ends with a sync
+    std::vector<ga_graph*> gs(count);
+    std::vector<const uint8_t*> al(count);
+    std::vector<uint32_t*> tr(count);
+    std::vector<unsigned long long*> tot(count);
+    std::vector<const uint32_t*> dn(count);
+    for (uint32_t i = 0; i < count; i++) {
+        gs[i] = peels[i]->g;
+        al[i] = peels[i]->alive;
+        tr[i] = peels[i]->tri;
+        tot[i] = &peels[i]->st->T;
+        dn[i] = &peels[i]->st->done;
+    }
+    ga_tc_many* M = nullptr;
+    GA_TRY(ga_tc_participation_many(ctx, gs.data(), al.data(), tr.data(), tot.data(), dn.data(), count, &M));
+    struct Free {
+        ga_tc_many* m;
+        ~Free() { ga_tc_many_free(m); }
+    } free_m{M};
+    // begin and mark ranges: each state its own grid, scaled down together above 8 workgroups per CU
+    const unsigned cus = (unsigned)std::max(ctx->cus, 1);
+    std::vector<PeelJob> jobs(count);
+    std::vector<uint32_t> map[2];
+    uint64_t sum = 0;
+    for (uint32_t i = 0; i < count; i++) sum += vgrid(ctx, peels[i]->g->n);
+    for (uint32_t i = 0; i < count; i++) {
+        ga_peel* p = peels[i];
+        unsigned nb = vgrid(ctx, p->g->n);
+        if (sum > 8ull * cus) nb = std::max(1u, (unsigned)((uint64_t)nb * 8ull * cus / sum));
+        jobs[i] = PeelJob{p->tri, p->alive, p->removed_round, p->st, p->g->n, T_full ? T_full[i] : UINT64_MAX, num, den, {0, 0}, {0, 0}};
+        for (int q = 0; q < 2; q++) {
+            jobs[i].base[q] = (uint32_t)map[q].size();
+            jobs[i].nblk[q] = nb;
+            map[q].insert(map[q].end(), nb, i);
+        }
+    }
+    const size_t jobs_bytes = ((size_t)count * sizeof(PeelJob) + 255) & ~(size_t)255;
+    const size_t map_bytes = ((map[0].size() + map[1].size()) * 4 + 255) & ~(size_t)255;
+    const size_t need = jobs_bytes + map_bytes + (size_t)count * sizeof(PeelDev);
+    if (need > ctx->dcore_bytes) {   // grow-only, shared with the core batches: one call at a time on a context
+        ga_free(ctx->dcore);
+        ctx->dcore = nullptr;
+        ctx->dcore_bytes = 0;
+        GA_TRY(ga_malloc(ctx, &ctx->dcore, need * 2));
+        ctx->dcore_bytes = need * 2;
+    }
+    std::vector<uint8_t> host(jobs_bytes + map_bytes);
+    memcpy(host.data(), jobs.data(), (size_t)count * sizeof(PeelJob));
+    memcpy(host.data() + jobs_bytes, map[0].data(), map[0].size() * 4);
+    memcpy(host.data() + jobs_bytes + map[0].size() * 4, map[1].data(), map[1].size() * 4);
+    char* dbase = (char*)ctx->dcore;
+    const PeelJob* djobs = (const PeelJob*)dbase;
+    const uint32_t* dmap0 = (const uint32_t*)(dbase + jobs_bytes);
+    const uint32_t* dmap1 = dmap0 + map[0].size();
+    PeelDev* dout = (PeelDev*)(dbase + jobs_bytes + map_bytes);
+    GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(dbase, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+    for (uint32_t i = 0; i < count; i++) GA_TRY(ga_peel_reset_state(peels[i]));
+    const unsigned gj = grid_for(count, 256);
+    k_peel_init_many<<<gj, 256, 0, ctx->stream>>>(djobs, count);
+    GA_TRY(ga_launch_check(ctx, "k_peel_init_many"));
+    std::vector<PeelDev> hst(count);
+    int block_rounds = 4;
+    for (int guard = 0;; guard++) {
+        for (int r = 0; r < block_rounds; r++) {
+            k_peel_begin_many<<<(unsigned)map[0].size(), 256, 0, ctx->stream>>>(djobs, dmap0);
+            GA_TRY(ga_launch_check(ctx, "k_peel_begin_many"));
+            GA_TRY(ga_tc_many_launch(ctx, M));
+            k_mark_peel_many<<<(unsigned)map[1].size(), 256, 0, ctx->stream>>>(djobs, dmap1);
+            GA_TRY(ga_launch_check(ctx, "k_mark_peel_many"));
+            k_peel_end_many<<<gj, 256, 0, ctx->stream>>>(djobs, count);
+            GA_TRY(ga_launch_check(ctx, "k_peel_end_many"));
+        }
+        k_peel_gather<<<gj, 256, 0, ctx->stream>>>(djobs, count, dout);
+        GA_TRY(ga_launch_check(ctx, "k_peel_gather"));
+        GA_HIP(ctx, hipMemcpyAsync(hst.data(), dout, (size_t)count * sizeof(PeelDev), hipMemcpyDeviceToHost, ctx->stream));
+        GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        bool all = true;
+        for (uint32_t i = 0; i < count && all; i++) all = hst[i].done != 0;
+        if (all) break;
+        if (guard > 1000000) return ga_fail(ctx, GA_ERR_STATE, "peel batch did not finish");
+        block_rounds = std::min(block_rounds * 2, 64);
+    }
+    GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GA_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    for (uint32_t i = 0; i < count; i++) {
+        const PeelDev& h = hst[i];
+        switch (h.error) {
+            case 0: break;
+            case PEEL_ERR_T: return ga_fail(ctx, GA_ERR_STATE, "peel state %u: device triangle count in round 0 differs from the given count", i);
+            case PEEL_ERR_ARITH: return ga_fail(ctx, GA_ERR_ARITH, "peel state %u: the exact peel arithmetic bounds do not hold", i);
+            default: return ga_fail(ctx, GA_ERR_STATE, "peel state %u: bad removed count from device (0 or more than alive)", i);
+        }
+        stats[4 * i] = h.best_num;
+        stats[4 * i + 1] = h.best_den;
+        stats[4 * i + 2] = h.best_round;
+        stats[4 * i + 3] = h.rounds;
     }
     return GA_OK;
 }

@@ -1827,8 +1827,8 @@ extern "C" int ga_tc_count(ga_ctx* ctx, ga_graph* g, int variant, uint64_t* T, f
     return GA_OK;
 }
 
-// ---- shared launches of ga_tc_count_many (hash variant) ----
-// A launch group: graphs whose bucket runs the same kernel (kind, lanes, probe form, block); one launch covers them.
+// ---- shared launches over several resident graphs (ga_tc_count_many, ga_peel_run_batch) ----
+// A launch group: graphs whose bucket runs the same kernel (kind, lanes, flags, block); one launch covers them.
 struct ManyGroup {
     int kind;    // 0: one lane per item (k_tc_tiny), 1: lane groups with LDS tables, 2: binary search in HBM
     int G, flags;
@@ -1836,11 +1836,125 @@ struct ManyGroup {
     size_t shmem = 0;
     std::vector<TcJob> jobs;
     std::vector<unsigned> grid;   // each job's own grid
+    std::vector<uint32_t> map;    // workgroup -> job (index into the whole job table)
+    unsigned nwg = 0;
 };
+
+struct ga_tc_many {
+    std::vector<ManyGroup> groups;
+    void* dbuf = nullptr;         // job tables and maps on the device (owned unless it is the context's buffer)
+    size_t dbytes = 0;
+    bool own = false;
+    const TcJob* djobs = nullptr;
+    const uint32_t* dmap = nullptr;
+    std::vector<uint8_t> host;    // their host copy: untouched until the caller's stream sync
+    ~ga_tc_many() {
+        if (own) ga_free(dbuf);
+    }
+};
+
+// Adds graph g's buckets as jobs: plain counts (flags 0; lane slots where the tuning says so, one lane per item for
+// bucket 0 when lanes[0] = 1) or participation (F_ALIVE | F_TRI: 4-lane groups for bucket 0, length classes). base
+// holds the graph's pointers; the items, tables and grid of each bucket are filled in as its own launch would.
+static void many_add(const ga_ctx* ctx, ga_tc_many& M, ga_graph* g, const CountView& V, const TcArgs& base, int flags) {
+    const TcPlan& P = *V.P;
+    const unsigned cus = (unsigned)std::max(ctx->cus, 1);
+    auto group_of = [&](int kind, int G, int fl, uint32_t block) -> ManyGroup& {
+        for (auto& gr : M.groups)
+            if (gr.kind == kind && gr.G == G && gr.flags == fl && gr.block == block) return gr;
+        M.groups.push_back(ManyGroup{kind, G, fl, block});
+        return M.groups.back();
+    };
+    uint32_t off = 0;
+    for (int b = 0; b <= NHASH; b++) {   // b = NHASH: the binary-search bucket
+        const uint32_t ni = b < NHASH ? P.nitems[b] : P.nhuge;
+        TcArgs a = base;
+        a.items = P.items + 4ull * off;
+        a.nitems = ni;
+        a.xcd = (P.built.item_order == 3 || P.built.item_order == 5) ? 1u : 0u;
+        off += ni;
+        if (!ni) continue;
+        int G = b < NHASH ? P.tune.lanes[b] : 64;
+        if (G == 1 && flags == 0) {   // bucket 0, one lane per item
+            ManyGroup& gr = group_of(0, 1, 0, (uint32_t)P.tune.block);
+            gr.jobs.push_back(TcJob{a, 0, 0});
+            gr.grid.push_back(ctx_grid(ctx, grid_for(ni, gr.block, cus * 64u)));
+            continue;
+        }
+        if (G == 1) G = 4;   // participation keeps 4-lane groups there
+        bool hashed = b < NHASH;
+        const uint32_t bound = b < NHASH ? std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap) : 0u;
+        int fl = flags | (hashed && flags == 0 && P.tune.probe ? F_SLOTS : 0);
+        HashShape sh = hash_shape(ctx, g, P, (uint32_t)G, hashed, fl, ni, bound);
+        if (!sh.fits) {   // as the graph's own launch: binary search in HBM
+            G = 64;
+            hashed = false;
+            fl = flags;
+            sh = hash_shape(ctx, g, P, 64u, false, fl, ni, bound);
+        }
+        a.ts_max = sh.ts;
+        a.load_shift = sh.shift;
+        ManyGroup& gr = group_of(hashed ? 1 : 2, G, fl, sh.block);
+        gr.shmem = std::max(gr.shmem, sh.shmem);
+        gr.jobs.push_back(TcJob{a, 0, 0});
+        gr.grid.push_back(sh.grid);
+    }
+}
+
+// Workgroup ranges (each job its own grid, scaled down together when a launch would exceed its budget: 64 workgroups
+// per CU, 16 for participation) and the upload of jobs and maps, into the context's grow-only buffer (use_ctx) or a
+// buffer of the object's own.
+static int many_upload(ga_ctx* ctx, ga_tc_many& M, bool use_ctx) {
+    const unsigned cus = (unsigned)std::max(ctx->cus, 1);
+    size_t jobs_total = 0, map_total = 0;
+    for (auto& gr : M.groups) {
+        uint64_t sum = 0;
+        for (unsigned x : gr.grid) sum += x;
+        const bool tc = (gr.flags & F_TRI) != 0;
+        const uint64_t budget = (tc ? 16ull : 64ull) * cus;
+        gr.map.clear();
+        for (size_t j = 0; j < gr.jobs.size(); j++) {
+            unsigned nb = gr.grid[j];
+            if (sum > budget) nb = std::max(1u, (unsigned)((uint64_t)nb * budget / sum));
+            gr.jobs[j].base = (uint32_t)gr.map.size();
+            gr.jobs[j].nblk = nb;
+            gr.map.insert(gr.map.end(), nb, (uint32_t)(jobs_total + j));
+        }
+        gr.nwg = (unsigned)gr.map.size();
+        jobs_total += gr.jobs.size();
+        map_total += gr.map.size();
+    }
+    if (!jobs_total) return GA_OK;
+    const size_t jobs_bytes = (jobs_total * sizeof(TcJob) + 255) & ~(size_t)255;
+    const size_t need = jobs_bytes + map_total * 4;
+    void** buf = use_ctx ? &ctx->dtcm : &M.dbuf;
+    size_t* cap = use_ctx ? &ctx->dtcm_bytes : &M.dbytes;
+    if (need > *cap) {   // grow-only
+        ga_free(*buf);
+        *buf = nullptr;
+        *cap = 0;
+        GA_TRY(ga_malloc(ctx, buf, need * 2));
+        *cap = need * 2;
+    }
+    M.own = !use_ctx;
+    std::vector<uint8_t>& host = use_ctx ? ctx->htcm : M.host;
+    host.resize(need);
+    size_t jo = 0, mo = jobs_bytes;
+    for (const auto& gr : M.groups) {
+        memcpy(host.data() + jo, gr.jobs.data(), gr.jobs.size() * sizeof(TcJob));
+        jo += gr.jobs.size() * sizeof(TcJob);
+        memcpy(host.data() + mo, gr.map.data(), gr.map.size() * 4);
+        mo += gr.map.size() * 4;
+    }
+    GA_HIP(ctx, hipMemcpyAsync(*buf, host.data(), need, hipMemcpyHostToDevice, ctx->stream));
+    M.djobs = (const TcJob*)*buf;
+    M.dmap = (const uint32_t*)((const char*)*buf + jobs_bytes);
+    return GA_OK;
+}
 
 template <int FLAGS>
 static void launch_many_hash(ga_ctx* ctx, int G, bool hashed, unsigned nwg, uint32_t block, size_t shmem, const TcJob* jobs, const uint32_t* map) {
-    if (!hashed) { k_tc_hash_many<64, false, 0><<<nwg, block, shmem, ctx->stream>>>(jobs, map); return; }
+    if (!hashed) { k_tc_hash_many<64, false, FLAGS & ~F_SLOTS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); return; }
     switch (G) {
         case 4: k_tc_hash_many<4, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
         case 8: k_tc_hash_many<8, true, FLAGS><<<nwg, block, shmem, ctx->stream>>>(jobs, map); break;
@@ -1850,22 +1964,27 @@ static void launch_many_hash(ga_ctx* ctx, int G, bool hashed, unsigned nwg, uint
     }
 }
 
+static int many_launch(ga_ctx* ctx, const ga_tc_many& M) {
+    const uint32_t* dmap = M.dmap;
+    for (const auto& gr : M.groups) {
+        const bool hashed = gr.kind == 1;
+        if (gr.kind == 0) k_tc_tiny_many<0><<<gr.nwg, gr.block, 0, ctx->stream>>>(M.djobs, dmap);
+        else if (gr.flags == F_SLOTS) launch_many_hash<F_SLOTS>(ctx, gr.G, hashed, gr.nwg, gr.block, gr.shmem, M.djobs, dmap);
+        else if (gr.flags == (F_ALIVE | F_TRI)) launch_many_hash<F_ALIVE | F_TRI>(ctx, gr.G, hashed, gr.nwg, gr.block, gr.shmem, M.djobs, dmap);
+        else launch_many_hash<0>(ctx, gr.G, hashed, gr.nwg, gr.block, gr.shmem, M.djobs, dmap);
+        GA_TRY(ga_launch_check(ctx, "k_tc_hash_many"));
+        dmap += gr.nwg;
+    }
+    return GA_OK;
+}
+
 // Plain counts of several resident graphs into counts[i] with one launch per bucket shape instead of one launch set per
-// graph. Graphs whose tuning runs the flat kernel are counted by their own launches (solo).
+// graph. Graphs whose tuning runs the flat kernel are counted by their own launches.
 static int count_many_shared(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, unsigned long long* counts) {
-    std::vector<ManyGroup> groups;
+    ga_tc_many M;
     std::vector<uint32_t> solo;
-    auto group_of = [&](int kind, int G, int flags, uint32_t block) -> ManyGroup& {
-        for (auto& gr : groups)
-            if (gr.kind == kind && gr.G == G && gr.flags == flags && gr.block == block) return gr;
-        groups.push_back(ManyGroup{kind, G, flags, block});
-        return groups.back();
-    };
-    const unsigned cus = (unsigned)std::max(ctx->cus, 1);
     for (uint32_t i = 0; i < count; i++) {
-        ga_graph* g = graphs[i];
-        const CountView V = count_view(g);
-        const TcPlan& P = *V.P;
+        const CountView V = count_view(graphs[i]);
         if (flat_on(V)) {
             solo.push_back(i);
             continue;
@@ -1873,99 +1992,51 @@ static int count_many_shared(ga_ctx* ctx, ga_graph* const* graphs, uint32_t coun
         TcArgs base{};
         base.rowptr = V.rowptr;
         base.col = V.col;
-        base.wcol = P.wcol;
-        base.wedge = P.wedge;
-        base.wps = P.wps;
-        base.wlen = P.wlen;
+        base.wcol = V.P->wcol;
+        base.wedge = V.P->wedge;
+        base.wps = V.P->wps;
+        base.wlen = V.P->wlen;
         base.total = counts + i;
-        base.xcd = (P.built.item_order == 3 || P.built.item_order == 5) ? 1u : 0u;
-        uint32_t off = 0;
-        for (int b = 0; b <= NHASH; b++) {   // b = NHASH: the binary-search bucket
-            const uint32_t ni = b < NHASH ? P.nitems[b] : P.nhuge;
-            TcArgs a = base;
-            a.items = P.items + 4ull * off;
-            a.nitems = ni;
-            off += ni;
-            if (!ni) continue;
-            if (b < NHASH && P.tune.lanes[b] == 1) {   // bucket 0, one lane per item
-                ManyGroup& gr = group_of(0, 1, 0, (uint32_t)P.tune.block);
-                gr.jobs.push_back(TcJob{a, 0, 0});
-                gr.grid.push_back(ctx_grid(ctx, grid_for(ni, gr.block, cus * 64u)));
-                continue;
-            }
-            int G = b < NHASH ? P.tune.lanes[b] : 64;
-            bool hashed = b < NHASH;
-            const uint32_t bound = b < NHASH ? std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap) : 0u;
-            int flags = hashed && P.tune.probe ? F_SLOTS : 0;
-            HashShape sh = hash_shape(ctx, g, P, (uint32_t)G, hashed, flags, ni, bound);
-            if (!sh.fits) {   // as the graph's own launch: binary search in HBM
-                G = 64;
-                hashed = false;
-                flags = 0;
-                sh = hash_shape(ctx, g, P, 64u, false, 0, ni, bound);
-            }
-            a.ts_max = sh.ts;
-            a.load_shift = sh.shift;
-            ManyGroup& gr = group_of(hashed ? 1 : 2, G, flags, sh.block);
-            gr.shmem = std::max(gr.shmem, sh.shmem);
-            gr.jobs.push_back(TcJob{a, 0, 0});
-            gr.grid.push_back(sh.grid);
-        }
+        many_add(ctx, M, graphs[i], V, base, 0);
     }
-    // workgroup ranges: each job its own grid, scaled down together when a launch would exceed 64 workgroups per CU
-    size_t jobs_total = 0, map_total = 0;
-    std::vector<std::vector<uint32_t>> maps(groups.size());
-    for (size_t q = 0; q < groups.size(); q++) {
-        ManyGroup& gr = groups[q];
-        uint64_t sum = 0;
-        for (unsigned x : gr.grid) sum += x;
-        const uint64_t budget = 64ull * cus;
-        for (size_t j = 0; j < gr.jobs.size(); j++) {
-            unsigned nb = gr.grid[j];
-            if (sum > budget) nb = std::max(1u, (unsigned)((uint64_t)nb * budget / sum));
-            gr.jobs[j].base = (uint32_t)maps[q].size();
-            gr.jobs[j].nblk = nb;
-            maps[q].insert(maps[q].end(), nb, (uint32_t)(jobs_total + j));
-        }
-        jobs_total += gr.jobs.size();
-        map_total += maps[q].size();
-    }
-    if (jobs_total) {
-        const size_t jobs_bytes = (jobs_total * sizeof(TcJob) + 255) & ~(size_t)255;
-        const size_t need = jobs_bytes + map_total * 4;
-        if (need > ctx->dtcm_bytes) {   // grow-only, used by one call at a time on this context's stream
-            ga_free(ctx->dtcm);
-            ctx->dtcm = nullptr;
-            ctx->dtcm_bytes = 0;
-            GA_TRY(ga_malloc(ctx, &ctx->dtcm, need * 2));
-            ctx->dtcm_bytes = need * 2;
-        }
-        std::vector<uint8_t>& host = ctx->htcm;
-        host.resize(need);
-        size_t jo = 0, mo = jobs_bytes;
-        for (size_t q = 0; q < groups.size(); q++) {
-            memcpy(host.data() + jo, groups[q].jobs.data(), groups[q].jobs.size() * sizeof(TcJob));
-            jo += groups[q].jobs.size() * sizeof(TcJob);
-            memcpy(host.data() + mo, maps[q].data(), maps[q].size() * 4);
-            mo += maps[q].size() * 4;
-        }
-        // ctx->htcm stays untouched until the stream sync that ends ga_tc_count_many
-        GA_HIP(ctx, hipMemcpyAsync(ctx->dtcm, host.data(), need, hipMemcpyHostToDevice, ctx->stream));
-        const TcJob* djobs = (const TcJob*)ctx->dtcm;
-        const uint32_t* dmap = (const uint32_t*)((const char*)ctx->dtcm + jobs_bytes);
-        for (size_t q = 0; q < groups.size(); q++) {
-            const ManyGroup& gr = groups[q];
-            const unsigned nwg = (unsigned)maps[q].size();
-            if (gr.kind == 0) k_tc_tiny_many<0><<<nwg, gr.block, 0, ctx->stream>>>(djobs, dmap);
-            else if (gr.flags & F_SLOTS) launch_many_hash<F_SLOTS>(ctx, gr.G, gr.kind == 1, nwg, gr.block, gr.shmem, djobs, dmap);
-            else launch_many_hash<0>(ctx, gr.G, gr.kind == 1, nwg, gr.block, gr.shmem, djobs, dmap);
-            GA_TRY(ga_launch_check(ctx, "k_tc_hash_many"));
-            dmap += nwg;
-        }
-    }
+    GA_TRY(many_upload(ctx, M, true));   // ctx->htcm stays untouched until the stream sync that ends ga_tc_count_many
+    GA_TRY(many_launch(ctx, M));
     for (uint32_t i : solo) GA_TRY(ga_tc_launch(ctx, graphs[i], GA_TC_HASH, counts + i, nullptr, nullptr, nullptr, nullptr));
     return GA_OK;
 }
+
+// Participation counts (alive masks, per-vertex tri[], triangle total, device `done` flag each) of several graphs, for
+// ga_peel_run_batch: built and uploaded once, launched every round.
+int ga_tc_participation_many(ga_ctx* ctx, ga_graph* const* graphs, const uint8_t* const* alive, uint32_t* const* tri,
+                             unsigned long long* const* total, const uint32_t* const* done, uint32_t count, ga_tc_many** out) {
+    *out = nullptr;
+    ga_tc_many* M = new ga_tc_many;
+    for (uint32_t i = 0; i < count; i++) {
+        int rc = ga_tc_ensure_plan(ctx, graphs[i]);
+        if (rc) { delete M; return rc; }
+        const CountView V = own_view(graphs[i]);
+        TcArgs base{};
+        base.rowptr = V.rowptr;
+        base.col = V.col;
+        base.wcol = V.P->wcol;
+        base.wedge = V.P->wedge;
+        base.wps = V.P->wps;
+        base.wlen = V.P->wlen;
+        base.alive = alive[i];
+        base.tri = tri[i];
+        base.total = total[i];
+        base.done = done[i];
+        many_add(ctx, *M, graphs[i], V, base, F_ALIVE | F_TRI);
+    }
+    int rc = many_upload(ctx, *M, true);   // the context's buffer: one call at a time on a context
+    if (rc) { delete M; return rc; }
+    *out = M;
+    return GA_OK;
+}
+
+int ga_tc_many_launch(ga_ctx* ctx, const ga_tc_many* M) { return many_launch(ctx, *M); }
+
+void ga_tc_many_free(ga_tc_many* M) { delete M; }
 
 extern "C" int ga_tc_count_many(ga_ctx* ctx, ga_graph* const* graphs, uint32_t count, int variant, uint64_t* T, float* ms) {
     if (!ctx || (count && (!graphs || !T))) return GA_ERR_ARG;

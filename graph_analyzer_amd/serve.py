@@ -17,9 +17,11 @@ Shape on one MI355X:
     and frees no device memory;
   * pending core requests (up to `core_batch` of them, on any graphs, each with its own state) are gathered into one
     ga_core_run_batch: every round's three launches cover all of them, so the round latency that bounds one
-    decomposition is paid once per batch instead of once per request.
-`slots=1, count_batch=1, core_batch=1` is the plain form: one request at a time, one count per launch, one
-decomposition at a time, in arrival order.
+    decomposition is paid once per batch instead of once per request;
+  * pending peel requests with the same eps (up to `peel_batch`, on any graphs) likewise go into one ga_peel_run_batch;
+  * pending counts (up to `count_batch`) go into one ga_tc_count_many: one launch per bucket shape for all of them.
+`slots=1, count_batch=1, core_batch=1, peel_batch=1` is the plain form: one request at a time, one count per launch,
+one peel or decomposition at a time, in arrival order.
 """
 from __future__ import annotations
 
@@ -35,7 +37,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from . import TC_HASH, Context, Core, DeviceGraph, GAError, Peel, core_batch
+from . import TC_HASH, Context, Core, DeviceGraph, GAError, Peel, core_batch, peel_batch
 
 KINDS = ("count", "peel", "core")
 
@@ -109,13 +111,14 @@ class _Resident:
 class Server:
     def __init__(self, source: Callable[[object], np.ndarray], slots: int = 16, count_batch: int = 64,
                  states_per_graph: int = 4, budget_bytes: Optional[int] = None, device: int = 0,
-                 core_batch: int = 1024):
-        if slots < 1 or count_batch < 1 or states_per_graph < 1 or core_batch < 1:
-            raise ValueError("slots, count_batch, core_batch and states_per_graph must be >= 1")
+                 core_batch: int = 1024, peel_batch: int = 1024):
+        if slots < 1 or count_batch < 1 or states_per_graph < 1 or core_batch < 1 or peel_batch < 1:
+            raise ValueError("slots, count_batch, core_batch, peel_batch and states_per_graph must be >= 1")
         self.source = source
         self.slots = slots
         self.count_batch = count_batch
         self.core_batch = core_batch
+        self.peel_batch = peel_batch
         self.states_per_graph = states_per_graph
         self.loader = Context(device)
         self.ctxs = [Context(device) for _ in range(slots)]
@@ -213,6 +216,25 @@ class Server:
                 for _, rq in group:
                     rq.inflight += 1
                 return ("count", group, None)
+            if req.kind == "peel" and self.peel_batch > 1:
+                # every pending peel request with this eps whose graph is resident and has a free state goes into one
+                # ga_peel_run_batch (shared launches for all of their rounds)
+                group, states, keep = [], [], collections.deque()
+                while pending and len(group) < self.peel_batch:
+                    q = pending.popleft()
+                    same = q.kind == "peel" and (q.num, q.den) == (req.num, req.den)
+                    rq = self.graphs.get(q.key) if same else None
+                    st = self._take_state(rq, "peel") if rq is not None else None
+                    if st is not None:
+                        group.append((q, rq))
+                        states.append(st)
+                        rq.inflight += 1
+                    else:
+                        keep.append(q)
+                pending.extendleft(reversed(keep))
+                if group:
+                    return ("peel_batch", group, states)
+                continue
             if req.kind == "core" and self.core_batch > 1:
                 # every pending core request whose graph is resident and has a free state goes into one
                 # ga_core_run_batch (shared launches for all of their rounds)
@@ -249,6 +271,10 @@ class Server:
         elif kind == "core_batch":
             res, _ = core_batch(ctx, state, [r.T for _, r in group])
             vals = [(rounds, kmax, digest(core)) for core, rounds, kmax in res]
+        elif kind == "peel_batch":
+            req0 = group[0][0]
+            res, _ = peel_batch(ctx, state, req0.num, req0.den, [r.T for _, r in group])
+            vals = [(pr.best_num, pr.best_den, pr.best_round, pr.rounds, digest(pr.removed_round)) for pr in res]
         elif kind == "peel":
             (req, r), = group
             state.set_ctx(ctx)
@@ -314,9 +340,9 @@ class Server:
                     for _, r in group:
                         r.inflight -= 1
                         r.last = time.perf_counter()
-                    if kind == "core_batch":
+                    if kind in ("core_batch", "peel_batch"):
                         for (_, r), st in zip(group, state):
-                            r.idle["core"].append(st)
+                            r.idle[kind[:4]].append(st)
                     elif state is not None:
                         group[0][1].idle[kind].append(state)
                     if err is not None:

@@ -222,6 +222,13 @@ int ga_peel_download_removed_round(ga_peel* p, uint32_t* removed_round);
  * stats = {best_num, best_den, best_round, rounds}; T_full = the caller's full count for the round-0 check
  * (UINT64_MAX = skip the check). Errors as batch_peel_loop: GA_ERR_ARITH, GA_ERR_STATE. */
 int ga_peel_run(ga_peel* p, uint64_t T_full, uint64_t num, uint64_t den, uint64_t* stats, float* ms);
+/* Fused peels of several resident graphs in shared launches, every round's begin, participation count, mark and end
+ * covering all unfinished states (each state exactly as ga_peel_run runs it alone; the hash variant). peels: distinct
+ * states on ctx's device, moved to ctx. T_full: per-state counts or NULL (UINT64_MAX entries: unknown). stats: 4 per
+ * state {best_num, best_den, best_round, rounds}; removed rounds through ga_peel_download_removed_round. The serving
+ * loop's batched peel. No counterpart in the reference: it peels one graph per process. */
+int ga_peel_run_batch(ga_ctx* ctx, ga_peel* const* peels, uint32_t count, const uint64_t* T_full, uint64_t num, uint64_t den,
+                      uint64_t* stats, float* ms);
 
 /* ---- (k,triangle)-core: backends of core_decomp_loop / core_decomp_loop_inc (dsg.h:665-751, gpu_dsg.cu:88-199) - */
 int ga_core_create(ga_ctx* ctx, ga_graph* g, int incremental, ga_core** out);

@@ -187,3 +187,47 @@ def test_count_many_shared_launches(ctx):
         ctx.set_grid_cap(0)
     for g in dgs:
         g.close()
+
+
+def test_peel_batch_shared_launches(ctx):
+    """ga_peel_run_batch (every round's launches cover all unfinished states) equals each state's own fused peel and
+    the oracle: best density, best round, rounds and every vertex's removal round, at two eps, twice in a row (the
+    second on fewer states), under mixed tunings and a capped grid; a wrong round-0 count is reported."""
+    graphs = mix(30, 33, nmax=3000)
+    graphs.append(np.array(clique(40), np.uint32))
+    oracles = [oracle.OGraph.from_pairs(p) for p in graphs]
+    dgs = [ga.DeviceGraph.from_pairs(ctx, p) for p in graphs]
+    for i, g in enumerate(dgs):
+        if i % 3 == 1:
+            t = ga.TcTuning.from_dict(ga.default_tuning().as_dict())
+            t.lanes[:] = [4, 8, 16, 32, 64] if i % 2 else [1, 32, 16, 64, 64]
+            t.hash_cap = 8 if i % 4 == 1 else 4096   # small cap: the binary-search bucket
+            g.set_tuning(t)
+    Ts = [og.triangles() for og in oracles]
+    peels = [ga.Peel(g) for g in dgs]
+    for rep, (num, den, sel) in enumerate(((1, 10, list(range(len(dgs)))), (1, 20, list(range(0, len(dgs), 2))))):
+        res, ms = ga.peel_batch(ctx, [peels[i] for i in sel], num, den, [Ts[i] for i in sel])
+        assert ms > 0
+        for i, r in zip(sel, res):
+            bn, bd, br, rounds, rr = oracles[i].batch_peel(num, den)
+            assert (r.best_num, r.best_den, r.best_round, r.rounds) == (bn, bd, br, rounds), (rep, i)
+            assert np.array_equal(r.removed_round, rr), (rep, i)
+            alone = ga.Peel(dgs[i])
+            assert alone.run(num, den, T_full=Ts[i]).same(r), (rep, i)
+            alone.close()
+    ctx.set_grid_cap(3)
+    try:
+        res, _ = ga.peel_batch(ctx, peels, 1, 10)
+        for i, r in enumerate(res):
+            assert (r.best_num, r.best_den, r.best_round, r.rounds) == oracles[i].batch_peel(1, 10)[:4], i
+    finally:
+        ctx.set_grid_cap(0)
+    bad = list(Ts)
+    bad[3] += 1
+    with pytest.raises(ga.GAError) as ei:
+        ga.peel_batch(ctx, peels, 1, 10, bad)
+    assert ei.value.code == ga.GA_ERR_STATE
+    for p in peels:
+        p.close()
+    for g in dgs:
+        g.close()

@@ -26,7 +26,7 @@ def test_alone_equals_mix_and_oracle():
     mix.preload(keys)
     got = mix.run(reqs)
     mix.close()
-    alone = serve.Server(source, slots=1, count_batch=1, core_batch=1, states_per_graph=1)
+    alone = serve.Server(source, slots=1, count_batch=1, core_batch=1, peel_batch=1, states_per_graph=1)
     ref = alone.run(reqs, realtime=False)
     alone.close()
     assert [r.rid for r in got] == [r.rid for r in ref]
@@ -66,7 +66,7 @@ def test_footprint_budget_evicts_idle_graphs():
     got = s.run(reqs)
     assert s.evictions > 0 and s.used <= budget
     s.close()
-    alone = serve.Server(source, slots=1, count_batch=1, core_batch=1, states_per_graph=1)
+    alone = serve.Server(source, slots=1, count_batch=1, core_batch=1, peel_batch=1, states_per_graph=1)
     ref = alone.run(reqs, realtime=False)
     alone.close()
     assert [a.value for a in got] == [b.value for b in ref]
@@ -81,7 +81,7 @@ def test_states_created_while_serving():
     got = s.run(reqs)
     assert max(r.nstates["core"] for r in s.graphs.values()) > 1
     s.close()
-    alone = serve.Server(source, slots=1, count_batch=1, core_batch=1, states_per_graph=1)
+    alone = serve.Server(source, slots=1, count_batch=1, core_batch=1, peel_batch=1, states_per_graph=1)
     ref = alone.run(reqs, realtime=False)
     alone.close()
     assert [a.value for a in got] == [b.value for b in ref]

@@ -1840,17 +1840,12 @@ struct ManyGroup {
     unsigned nwg = 0;
 };
 
+// The groups of one set of shared launches; jobs and maps live in the context's grow-only buffer (ctx->dtcm, host copy
+// ctx->htcm), so one such set is in use at a time on a context.
 struct ga_tc_many {
     std::vector<ManyGroup> groups;
-    void* dbuf = nullptr;         // job tables and maps on the device (owned unless it is the context's buffer)
-    size_t dbytes = 0;
-    bool own = false;
     const TcJob* djobs = nullptr;
     const uint32_t* dmap = nullptr;
-    std::vector<uint8_t> host;    // their host copy: untouched until the caller's stream sync
-    ~ga_tc_many() {
-        if (own) ga_free(dbuf);
-    }
 };
 
 // Adds graph g's buckets as jobs: plain counts (flags 0; lane slots where the tuning says so, one lane per item for
@@ -1902,9 +1897,9 @@ static void many_add(const ga_ctx* ctx, ga_tc_many& M, ga_graph* g, const CountV
 }
 
 // Workgroup ranges (each job its own grid, scaled down together when a launch would exceed its budget: 64 workgroups
-// per CU, 16 for participation) and the upload of jobs and maps, into the context's grow-only buffer (use_ctx) or a
-// buffer of the object's own.
-static int many_upload(ga_ctx* ctx, ga_tc_many& M, bool use_ctx) {
+// per CU, 16 for participation) and the upload of jobs and maps into the context's grow-only buffer. The host copy
+// stays untouched until the caller's stream sync.
+static int many_upload(ga_ctx* ctx, ga_tc_many& M) {
     const unsigned cus = (unsigned)std::max(ctx->cus, 1);
     size_t jobs_total = 0, map_total = 0;
     for (auto& gr : M.groups) {
@@ -1927,17 +1922,14 @@ static int many_upload(ga_ctx* ctx, ga_tc_many& M, bool use_ctx) {
     if (!jobs_total) return GA_OK;
     const size_t jobs_bytes = (jobs_total * sizeof(TcJob) + 255) & ~(size_t)255;
     const size_t need = jobs_bytes + map_total * 4;
-    void** buf = use_ctx ? &ctx->dtcm : &M.dbuf;
-    size_t* cap = use_ctx ? &ctx->dtcm_bytes : &M.dbytes;
-    if (need > *cap) {   // grow-only
-        ga_free(*buf);
-        *buf = nullptr;
-        *cap = 0;
-        GA_TRY(ga_malloc(ctx, buf, need * 2));
-        *cap = need * 2;
+    if (need > ctx->dtcm_bytes) {   // grow-only
+        ga_free(ctx->dtcm);
+        ctx->dtcm = nullptr;
+        ctx->dtcm_bytes = 0;
+        GA_TRY(ga_malloc(ctx, &ctx->dtcm, need * 2));
+        ctx->dtcm_bytes = need * 2;
     }
-    M.own = !use_ctx;
-    std::vector<uint8_t>& host = use_ctx ? ctx->htcm : M.host;
+    std::vector<uint8_t>& host = ctx->htcm;
     host.resize(need);
     size_t jo = 0, mo = jobs_bytes;
     for (const auto& gr : M.groups) {
@@ -1946,9 +1938,9 @@ static int many_upload(ga_ctx* ctx, ga_tc_many& M, bool use_ctx) {
         memcpy(host.data() + mo, gr.map.data(), gr.map.size() * 4);
         mo += gr.map.size() * 4;
     }
-    GA_HIP(ctx, hipMemcpyAsync(*buf, host.data(), need, hipMemcpyHostToDevice, ctx->stream));
-    M.djobs = (const TcJob*)*buf;
-    M.dmap = (const uint32_t*)((const char*)*buf + jobs_bytes);
+    GA_HIP(ctx, hipMemcpyAsync(ctx->dtcm, host.data(), need, hipMemcpyHostToDevice, ctx->stream));
+    M.djobs = (const TcJob*)ctx->dtcm;
+    M.dmap = (const uint32_t*)((const char*)ctx->dtcm + jobs_bytes);
     return GA_OK;
 }
 
@@ -1999,7 +1991,7 @@ static int count_many_shared(ga_ctx* ctx, ga_graph* const* graphs, uint32_t coun
         base.total = counts + i;
         many_add(ctx, M, graphs[i], V, base, 0);
     }
-    GA_TRY(many_upload(ctx, M, true));   // ctx->htcm stays untouched until the stream sync that ends ga_tc_count_many
+    GA_TRY(many_upload(ctx, M));   // ctx->htcm stays untouched until the stream sync that ends ga_tc_count_many
     GA_TRY(many_launch(ctx, M));
     for (uint32_t i : solo) GA_TRY(ga_tc_launch(ctx, graphs[i], GA_TC_HASH, counts + i, nullptr, nullptr, nullptr, nullptr));
     return GA_OK;
@@ -2028,7 +2020,7 @@ int ga_tc_participation_many(ga_ctx* ctx, ga_graph* const* graphs, const uint8_t
         base.done = done[i];
         many_add(ctx, *M, graphs[i], V, base, F_ALIVE | F_TRI);
     }
-    int rc = many_upload(ctx, *M, true);   // the context's buffer: one call at a time on a context
+    int rc = many_upload(ctx, *M);
     if (rc) { delete M; return rc; }
     *out = M;
     return GA_OK;

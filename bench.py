@@ -370,8 +370,8 @@ def run_batch(ctx, count, steps, tune):
 
 
 def run_serving(rates, duration):
-    """Load sweep: at each offered rate, the same Poisson arrivals served by continuous batching (16 worker streams,
-    count batching, shared-launch core batches, footprint admission) and by the plain form (one request at a time, one
+    """Load sweep: at each offered rate, the same Poisson arrivals served by continuous batching (4 worker streams,
+    shared-launch count, peel and core batches, footprint admission) and by the plain form (one request at a time, one
     count per launch, one decomposition at a time); every result must be identical in both forms."""
     from graph_analyzer_amd import serve
     source = serve.workload_source(seed=2106, pool=256, rmat_scale=20)
@@ -380,7 +380,7 @@ def run_serving(rates, duration):
     out = {"workload": f"poisson_{duration:g}s_per_rate", "duration_s": duration,
            "mix": "count / peel eps 1/20 / core uniform; 95% on 256 graphs of the batch size law, 5% on 2 R-MAT scale-20 graphs",
            "sweep": []}
-    forms = (("continuous", dict(slots=16, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16)),
+    forms = (("continuous", dict(slots=4, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16)),
              ("plain", dict(slots=1, count_batch=1, core_batch=1, peel_batch=1, states_per_graph=1)))
     rows = {rate: {"offered_rps": rate, "requests": len(sweep[rate])} for rate in rates}
     values = {}

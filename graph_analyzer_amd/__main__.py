@@ -45,7 +45,7 @@ def main(argv=None):
     if args.stage == "serve":   # one Poisson run of the mixed workload through the continuous-batching server
         from graph_analyzer_amd import serve
         source, reqs = serve.mixed_workload(args.rate, args.seconds, pool=args.pool)
-        s = serve.Server(source, slots=16, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16)
+        s = serve.Server(source, slots=4, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16)
         t0 = time.perf_counter()
         s.preload(sorted({q.key for q in reqs}))
         print(f"serve: {len(reqs)} requests at {args.rate:g} req/s, graphs resident in {time.perf_counter() - t0:.1f} s", flush=True)

@@ -3,7 +3,7 @@
     python -m graph_analyzer_amd count|peel|core|support|truss|ingest|prune|greedy
                                  [--workload rmat21|powerlaw_communities|powerlaw_small] [--repeat R] [--num 1 --den 10]
                                  [--passes 10]
-    python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R]
+    python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R] [--tune]
     python -m graph_analyzer_amd greedy_many [--graphs 256] [--passes 10]   (seeded dense graphs of 100-1,000 vertices)
     python -m graph_analyzer_amd probe     (copy bandwidth and random-segment gather rates of the device)
     python -m graph_analyzer_amd serve [--rate 2000] [--seconds 1] [--pool 256]
@@ -41,6 +41,7 @@ def main(argv=None):
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--seconds", type=float, default=1.0)
     ap.add_argument("--pool", type=int, default=256)
+    ap.add_argument("--tune", action="store_true", help="batch: autotune the batched count first (as bench.py does)")
     args = ap.parse_args(argv)
     if args.stage == "serve":   # one Poisson run of the mixed workload through the continuous-batching server
         from graph_analyzer_amd import serve
@@ -110,6 +111,9 @@ def main(argv=None):
         b = ga.Batch(ctx, flat=bp.pairs, offsets=bp.offsets)
         del bp
         print(f"batch {args.graphs} graphs n={b.total_n} m={b.total_m} build {b.build_ms:.1f} ms", flush=True)
+        if args.tune:
+            t, tms = b.tune()
+            print(f"batch tuned: {t.as_dict()} ({tms:.2f} ms)", flush=True)
         for _ in range(args.repeat):
             T, ms = b.count_all()
             print(f"batched count: {ms:.3f} ms, T total {int(T.sum())}", flush=True)

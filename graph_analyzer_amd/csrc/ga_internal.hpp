@@ -31,6 +31,7 @@ struct ga_ctx {
     void* dtcm = nullptr;               // grow-only job tables and workgroup maps of ga_tc_count_many's shared launches
     size_t dtcm_bytes = 0;
     std::vector<uint8_t> htcm;          // their host copy: left untouched until the next call (each call ends with a sync)
+    std::vector<uint8_t> hpeel;         // host copy of ga_peel_run_batch's job tables and maps (same rule)
     uint32_t grid_cap = 0;              // test hook: at most this many workgroups per count / peel / core launch (0: none)
     std::string err;
 };

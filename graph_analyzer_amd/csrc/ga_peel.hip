@@ -387,7 +387,8 @@ extern "C" int ga_peel_run_batch(ga_ctx* ctx, ga_peel* const* peels, uint32_t co
         GA_TRY(ga_malloc(ctx, &ctx->dcore, need * 2));
         ctx->dcore_bytes = need * 2;
     }
-    std::vector<uint8_t> host(jobs_bytes + map_bytes);
+    std::vector<uint8_t>& host = ctx->hpeel;   // outlives an early error return while the copy may be in flight
+    host.resize(jobs_bytes + map_bytes);
     memcpy(host.data(), jobs.data(), (size_t)count * sizeof(PeelJob));
     memcpy(host.data() + jobs_bytes, map[0].data(), map[0].size() * 4);
     memcpy(host.data() + jobs_bytes + map[0].size() * 4, map[1].data(), map[1].size() * 4);

@@ -51,7 +51,8 @@ class TcTuning(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_int32 * 5), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
                 ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32),
                 ("flat_chunk", ctypes.c_int32), ("flat_block", ctypes.c_int32),
-                ("item_order", ctypes.c_int32), ("relabel", ctypes.c_int32), ("probe", ctypes.c_int32)]
+                ("item_order", ctypes.c_int32), ("relabel", ctypes.c_int32), ("probe", ctypes.c_int32),
+                ("waves", ctypes.c_int32)]
 
     @classmethod
     def from_dict(cls, d):
@@ -67,7 +68,8 @@ class TcTuning(ctypes.Structure):
         return {"lanes": list(self.lanes), "block": self.block, "work_chunk": self.work_chunk, "batches": self.batches,
                 "load_shift": self.load_shift, "hash_cap": self.hash_cap, "flat_chunk": self.flat_chunk,
                 "flat_block": self.flat_block,
-                "item_order": self.item_order, "relabel": self.relabel, "probe": self.probe}
+                "item_order": self.item_order, "relabel": self.relabel, "probe": self.probe,
+                "waves": self.waves}
 
 
 _lib = None

@@ -18,6 +18,7 @@ struct ga_ctx {
     int device = 0;
     int cus = 0;
     size_t lds_max = 65536;             // dynamic LDS bytes one workgroup may use
+    size_t lds_cu = 163840;             // LDS bytes of one CU (shared by its resident workgroups)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t* hstat = nullptr;          // pinned host staging for per-round statistics (512 bytes)

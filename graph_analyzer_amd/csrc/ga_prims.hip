@@ -68,6 +68,7 @@ extern "C" int ga_ctx_create(int device, ga_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return GA_ERR_HIP; }
     c->cus = prop.multiProcessorCount;
     c->lds_max = prop.sharedMemPerBlock;
+    if (prop.maxSharedMemoryPerMultiProcessor) c->lds_cu = prop.maxSharedMemoryPerMultiProcessor;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipHostMalloc((void**)&c->hstat, 512, hipHostMallocDefault) != hipSuccess ||

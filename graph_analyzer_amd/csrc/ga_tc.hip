@@ -117,15 +117,21 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t x, uint32_t tbits) {
 // of its probe sequence (its hash bucket, then the next ones cyclically) that had a free slot when it was inserted.
 // A lookup reads whole buckets and stops at the first one with a free slot, so one 16-byte read decides almost every
 // lookup without a loop (at most a quarter of the slots are used at load_shift 2).
-__device__ __forceinline__ uint32_t key_bucket(uint32_t x, uint32_t bbits) {
-    return bbits ? (x * 2654435769u) >> (32u - bbits) : 0u;
+__device__ __forceinline__ uint32_t key_bucket(uint32_t x, uint32_t bbits, uint32_t mul = 2654435769u) {
+    return bbits ? (x * mul) >> (32u - bbits) : 0u;
+}
+
+// Odd multipliers of the lane-slot tables' rebuilds (below): the first is the one every other table uses.
+#define HASH_TRIES 4u
+__device__ __forceinline__ uint32_t hash_mul(uint32_t attempt) {
+    return attempt == 0 ? 2654435769u : attempt == 1 ? 2246822507u : attempt == 2 ? 3266489909u : 668265263u;
 }
 
 // Inserts a key that is not in the table; returns its slot. Ends: a table has at least twice as many slots as keys,
 // and a failed compare-and-swap means another lane has just filled that slot.
-__device__ __forceinline__ uint32_t bucket_insert(uint32_t* tab, uint32_t bbits, uint32_t x) {
+__device__ __forceinline__ uint32_t bucket_insert(uint32_t* tab, uint32_t bbits, uint32_t x, uint32_t mul = 2654435769u) {
     const uint32_t nbm = (1u << bbits) - 1u;
-    uint32_t bk = key_bucket(x, bbits);
+    uint32_t bk = key_bucket(x, bbits, mul);
     for (;;) {
         const uint4 t = ((const uint4*)tab)[bk];
         const uint32_t j = t.x == GA_EMPTY ? 0u : t.y == GA_EMPTY ? 1u : t.z == GA_EMPTY ? 2u : t.w == GA_EMPTY ? 3u : 4u;
@@ -144,9 +150,9 @@ __device__ __forceinline__ uint32_t bucket_pos(const uint4 t, uint32_t x) {
 }
 
 // Slot of x, or GA_EMPTY when x is not in the table. The loop runs only past a full bucket without x.
-__device__ __forceinline__ uint32_t bucket_find(const uint32_t* tab, uint32_t bbits, uint32_t x) {
+__device__ __forceinline__ uint32_t bucket_find(const uint32_t* tab, uint32_t bbits, uint32_t x, uint32_t mul = 2654435769u) {
     const uint32_t nbm = (1u << bbits) - 1u;
-    uint32_t bk = key_bucket(x, bbits);
+    uint32_t bk = key_bucket(x, bbits, mul);
     uint4 t = ((const uint4*)tab)[bk];
     uint32_t j = bucket_pos(t, x);
     bool full = max(max(t.x, t.y), max(t.z, t.w)) != GA_EMPTY;
@@ -295,7 +301,9 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
     }
     uint64_t local = 0;
     const ItemRange ir = item_range(a, gpb, gib, bid, nb);
-    for (uint64_t it = ir.first; it < ir.end; it += ir.step) {
+    // 32-bit item index (one register): the step is taken only while it stays below the end, so it cannot wrap
+    const uint32_t it_end = (uint32_t)ir.end, it_step = (uint32_t)ir.step;
+    for (uint32_t it = (uint32_t)min(ir.first, ir.end); it < it_end; it = it_end - it > it_step ? it + it_step : it_end) {
         const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
         const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK);
         if ((FLAGS & F_ALIVE) && !alive[h]) continue;
@@ -310,7 +318,7 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
                 if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[j];
             }
         }
-        uint32_t tbits = 0, tsz = 1;
+        uint32_t tbits = 0, tsz = 1, mul = hash_mul(0);
         bool spill = true;   // false: every key of N+(h) is in its home bucket, so a lookup reads that bucket only
         if (HASHED) {
             // ceil(log2(dh)) + shift, at least 2, at most the group's reserved slots: the reservation holds the bucket's
@@ -320,25 +328,46 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
             if (tbits < 3) tbits = 3;   // at least two buckets (key_bucket's shift stays below 32; ts_max >= 8 slots)
             const uint32_t tcap = 31u - (uint32_t)__clz(a.ts_max);
             if (tbits > tcap) tbits = tcap;
-            tsz = 1u << tbits;
-            for (uint32_t w = sub; w < tsz / 4u; w += G) {   // 16-byte stores (tab and tix are 16-byte aligned)
-                ((uint4*)tab)[w] = make_uint4(GA_EMPTY, GA_EMPTY, GA_EMPTY, GA_EMPTY);
-                if (FLAGS & F_TRI) ((uint4*)tix)[w] = make_uint4(0, 0, 0, 0);
+            // The lane-slot probe reads home buckets only (no walk in its loop, which keeps it at 8 waves per SIMD by
+            // registers): a table in which a key left its home bucket is rebuilt with the next multiplier and one more
+            // table bit (up to the reservation), at most HASH_TRIES tables in all. Whether a key leaves its home bucket
+            // does not depend on the insertion order: it happens exactly when more than four keys share a home bucket.
+            for (uint32_t attempt = 0;; attempt++) {
+                tsz = 1u << tbits;
+                for (uint32_t w = sub; w < tsz / 4u; w += G) {   // 16-byte stores (tab and tix are 16-byte aligned)
+                    ((uint4*)tab)[w] = make_uint4(GA_EMPTY, GA_EMPTY, GA_EMPTY, GA_EMPTY);
+                    if (FLAGS & F_TRI) ((uint4*)tix)[w] = make_uint4(0, 0, 0, 0);
+                }
+                group_sync();
+                uint32_t moved = 0;   // a key that sits past its home bucket (then lookups must walk on)
+                for (uint32_t k = sub; k < dh; k += G) {
+                    const uint32_t x = col[hs + k];
+                    if ((FLAGS & F_ALIVE) && !alive[x]) continue;
+                    const uint32_t s = bucket_insert(tab, tbits - 2u, x, mul);
+                    if (FLAGS & F_SUP) tix[s] = k;
+                    moved |= (s >> 2) != key_bucket(x, tbits - 2u, mul) ? 1u : 0u;
+                }
+                spill = group_bits<G>(__ballot(moved != 0)) != 0;
+                group_sync();
+                if (!SLOTS || !spill || attempt + 1u == HASH_TRIES) break;
+                mul = hash_mul(attempt + 1u);
+                if (tbits < tcap) tbits++;
             }
-            group_sync();
-            uint32_t moved = 0;   // a key that sits past its home bucket (then lookups must walk on)
-            for (uint32_t k = sub; k < dh; k += G) {
-                const uint32_t x = col[hs + k];
-                if ((FLAGS & F_ALIVE) && !alive[x]) continue;
-                const uint32_t s = bucket_insert(tab, tbits - 2u, x);
-                if (FLAGS & F_SUP) tix[s] = k;
-                moved |= (s >> 2) != key_bucket(x, tbits - 2u) ? 1u : 0u;
-            }
-            spill = group_bits<G>(__ballot(moved != 0)) != 0;
-            group_sync();
         }
         uint32_t hcount = 0;
-        for (uint32_t base = b; base < e; base += G) {
+        if (SLOTS && spill) {
+            // every table tried left a key outside its home bucket (rare): one lane per entry, each element looked up
+            // with the walk past full buckets
+            for (uint32_t j = b + sub; j < e; j += G) {
+                const uint32_t ps = a.wps[j], len = a.wlen[j];
+                uint32_t c = 0;
+                for (uint32_t k = 0; k < len; k++) c += bucket_find(tab, tbits - 2u, col[ps + k], mul) != GA_EMPTY ? 1u : 0u;
+                if (!(FLAGS & F_GRAPH)) local += c;   // per-graph counts: the graphs' counters hold the total
+                hcount += c;
+            }
+        }
+        // (an item taken by the walk above skips the batches)
+        for (uint32_t base = SLOTS && spill ? e : b; base < e; base += G) {
             const uint32_t j = base + sub;
             const uint32_t p = n_p, ps = n_ps;
             uint32_t len = n_len;
@@ -385,47 +414,25 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
 #pragma unroll
                         for (uint32_t u = 0; u < SU; u++) x[u] = lp[min(r + u * kw, last)];   // past the end: masked
                         __builtin_amdgcn_sched_barrier(0);
+                        // the home bucket decides membership (no key left its own): one bucket read and four compares
+                        // per element, the compares going to wave masks (as in the length-class form below)
                         const uint32_t bb = tbits - 2u;
-                        if (!spill) {   // one bucket read and four compares per element (wave masks, as above)
 #pragma unroll
-                            for (uint32_t u0 = 0; u0 < SU; u0 += 4) {
-                                uint4 tb[4];
+                        for (uint32_t u0 = 0; u0 < SU; u0 += 4) {
+                            uint4 tb[4];
 #pragma unroll
-                                for (uint32_t t = 0; t < 4; t++) tb[t] = ((const uint4*)tab)[(x[u0 + t] * 2654435769u) >> (32u - bb)];
+                            for (uint32_t t = 0; t < 4; t++) tb[t] = ((const uint4*)tab)[(x[u0 + t] * mul) >> (32u - bb)];
 #pragma unroll
-                                for (uint32_t t = 0; t < 4; t++) {
-                                    const uint32_t xv = x[u0 + t];
-                                    const uint64_t hm = (__builtin_amdgcn_ballot_w64(tb[t].x == xv) | __builtin_amdgcn_ballot_w64(tb[t].y == xv) |
-                                                         __builtin_amdgcn_ballot_w64(tb[t].z == xv) | __builtin_amdgcn_ballot_w64(tb[t].w == xv)) &
-                                                        __builtin_amdgcn_ballot_w64(r + (u0 + t) * kw <= last);
-                                    c += __builtin_amdgcn_inverse_ballot_w64(hm) ? 1u : 0u;
-                                }
-                            }
-                        } else {   // a key left its home bucket: walk on past a full bucket without x
-                            uint32_t slow = 0;
-#pragma unroll
-                            for (uint32_t u0 = 0; u0 < SU; u0 += 4) {
-                                uint4 tb[4];
-#pragma unroll
-                                for (uint32_t t = 0; t < 4; t++) tb[t] = ((const uint4*)tab)[key_bucket(x[u0 + t], bb)];
-#pragma unroll
-                                for (uint32_t t = 0; t < 4; t++) {
-                                    const uint32_t u = u0 + t, xv = x[u];
-                                    const uint32_t d = min(min(tb[t].x ^ xv, tb[t].y ^ xv), min(tb[t].z ^ xv, tb[t].w ^ xv));
-                                    const uint32_t mx = max(max(tb[t].x, tb[t].y), max(tb[t].z, tb[t].w));
-                                    const uint32_t v = r + u * kw <= last ? 1u : 0u;
-                                    c += (d == 0u ? 1u : 0u) & v;
-                                    slow |= ((d != 0u && mx != GA_EMPTY) ? v : 0u) << u;
-                                }
-                            }
-                            if (slow) {
-#pragma unroll
-                                for (uint32_t u = 0; u < SU; u++)
-                                    if ((slow >> u) & 1u) c += bucket_has_after(tab, bb, x[u], key_bucket(x[u], bb));
+                            for (uint32_t t = 0; t < 4; t++) {
+                                const uint32_t xv = x[u0 + t];
+                                const uint64_t hm = (__builtin_amdgcn_ballot_w64(tb[t].x == xv) | __builtin_amdgcn_ballot_w64(tb[t].y == xv) |
+                                                     __builtin_amdgcn_ballot_w64(tb[t].z == xv) | __builtin_amdgcn_ballot_w64(tb[t].w == xv)) &
+                                                    __builtin_amdgcn_ballot_w64(r + (u0 + t) * kw <= last);
+                                c += __builtin_amdgcn_inverse_ballot_w64(hm) ? 1u : 0u;
                             }
                         }
                     }
-                    local += c;
+                    if (!(FLAGS & F_GRAPH)) local += c;   // per-graph counts: the graphs' counters hold the total
                     if (FLAGS & F_GRAPH) hcount += c;
                 }
                 group_sync();   // staging is rewritten by the next batch
@@ -556,7 +563,7 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
                             }
                         }
                     }
-                    local += c;
+                    if (!(FLAGS & F_GRAPH)) local += c;   // per-graph counts: the graphs' counters hold the total
                     if (FLAGS & (F_TRI | F_GRAPH)) hcount += c;
                     if (FLAGS & (F_TRI | F_SUP)) {   // hits of list k, summed over its W lanes, go to p and edge (h,p)
 #pragma unroll
@@ -594,6 +601,7 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
         for (uint32_t i = threadIdx.x; i < TRI_CACHE; i += blockDim.x)
             if (ccnt[i]) atomicAdd(&a.tri[ckey[i]], ccnt[i]);
     }
+    if (FLAGS & F_GRAPH) return;
     local = wave_sum_u64(local);
     if (lane_id() == 0 && local) atomicAdd(a.total, (unsigned long long)local);
 }
@@ -655,6 +663,7 @@ __device__ __forceinline__ void tc_tiny_body(const TcArgs& a, uint32_t bid, uint
             }
         }
     }
+    if (FLAGS & F_GRAPH) return;   // per-graph counts: the graphs' counters hold the total
     local = wave_sum_u64(local);
     if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
 }
@@ -918,8 +927,17 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
             if (s) atomicAdd(&a.gcnt[g0], s);
         }
     }
+    if (GRAPH) return;   // per-graph counts: the graphs' counters hold the total
     local = wave_sum_u64(local);
     if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
+}
+
+// out += the sum of in[0, n) (out zeroed by the caller).
+__global__ void k_sum_u64(const unsigned long long* __restrict__ in, uint32_t n, unsigned long long* __restrict__ out) {
+    uint64_t local = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) local += in[i];
+    local = wave_sum_u64(local);
+    if (lane_id() == 0 && local) atomicAdd(out, (unsigned long long)local);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1173,6 +1191,7 @@ ga_tc_tuning default_tuning() {
     t.item_order = 1;
     t.relabel = 0;
     t.probe = 1;
+    t.waves = 0;
     return t;
 }
 
@@ -1190,6 +1209,7 @@ bool tuning_valid(const ga_tc_tuning& t) {
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
            t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && (t.relabel == 0 || t.relabel == 1) && (t.probe == 0 || t.probe == 1) &&
+           (t.waves == 0 || (t.waves >= 4 && t.waves <= 8)) &&
            (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
                                   (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
                                    t.flat_chunk / t.flat_block == 16)));
@@ -1456,6 +1476,17 @@ struct HashShape {
     bool fits;   // the tables fit one workgroup's LDS (else the bucket binary-searches N+(h) in HBM)
 };
 
+// Dynamic LDS of a plain or per-graph count launch that holds it to at most `waves` waves per SIMD (0: no cap): a CU
+// has 4 SIMDs and its LDS is shared by its resident workgroups, so asking each workgroup for more than 1/(k+1) of it
+// leaves room for k of them. Fewer items in flight keep more of their probe lists in the L2 (the 4,096-graph batch
+// union: 94.5 ms uncapped, 88.9 ms at 6 waves, 88.6 ms at 5, 96.2 ms at 4).
+static size_t occupancy_lds(const ga_ctx* ctx, uint32_t waves, uint32_t block, size_t shmem) {
+    if (!waves) return shmem;
+    const uint32_t wg = std::max(1u, 4u * waves / std::max(1u, block / 64u));   // workgroups per CU at that cap
+    const size_t need = ctx->lds_cu / (wg + 1u) + 256u;
+    return need > shmem && need <= ctx->lds_max ? need : shmem;
+}
+
 static HashShape hash_shape(const ga_ctx* ctx, const ga_graph* g, const TcPlan& P, uint32_t G, bool hashed, int flags, uint32_t nitems,
                             uint32_t bound) {
     HashShape s;
@@ -1472,6 +1503,7 @@ static HashShape hash_shape(const ga_ctx* ctx, const ga_graph* g, const TcPlan& 
     while (s.block > 64 && (size_t)(s.block / G) * per_group + cache_bytes > 64 * 1024) s.block >>= 1;
     s.shmem = (size_t)(s.block / G) * per_group + cache_bytes;
     s.fits = !hashed || s.shmem <= ctx->lds_max;
+    if (s.fits && (flags & ~(F_GRAPH | F_SLOTS)) == 0) s.shmem = occupancy_lds(ctx, (uint32_t)P.tune.waves, s.block, s.shmem);
     // participation: fewer, longer-lived workgroups, so that each counter cache gathers more items before its flush
     // (64 -> 16 per CU: 27.3 -> 23.0 ms for the R-MAT scale-21 peel, 9.5 -> 8.6 ms on the 34.7M-edge graph)
     s.grid = ctx_grid(ctx, grid_for((uint64_t)nitems * G, s.block, (unsigned)std::max(ctx->cus, 1) * (tc ? 16u : 64u)));
@@ -1502,7 +1534,8 @@ static int launch_tiny(ga_ctx* ctx, ga_graph* g, const CountView& V, const uint3
     a.items = items;
     a.nitems = nitems;
     a.xcd = (V.P->built.item_order == 3 || V.P->built.item_order == 5) ? 1u : 0u;
-    k_tc_tiny<FLAGS><<<ctx_grid(ctx, grid_for(nitems, block, (unsigned)std::max(ctx->cus, 1) * 64u)), block, 0, ctx->stream>>>(a);
+    k_tc_tiny<FLAGS><<<ctx_grid(ctx, grid_for(nitems, block, (unsigned)std::max(ctx->cus, 1) * 64u)), block,
+                       occupancy_lds(ctx, (uint32_t)V.P->tune.waves, block, 0), ctx->stream>>>(a);
     return ga_launch_check(ctx, "k_tc_tiny");
 }
 
@@ -2109,6 +2142,9 @@ static int timed_count(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned l
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, gcnt + ngraphs, nullptr, nullptr, nullptr, nullptr, gof, gcnt));
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    // per-graph counts leave the total to their counters: it is their sum
+    k_sum_u64<<<grid_for(ngraphs, 256, 256), 256, 0, ctx->stream>>>(gcnt, ngraphs, gcnt + ngraphs);
+    GA_TRY(ga_launch_check(ctx, "k_sum_u64"));
     GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, gcnt + ngraphs, 8, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *T = ctx->hstat[0];
@@ -2146,9 +2182,9 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
             if (r > 0) best_here = std::min(best_here, x);
         }
 #ifdef GA_TUNE_TRACE   // build-time switch: print every trial of the coordinate search
-        fprintf(stderr, "tune trial lanes %d %d %d %d %d block %d chunk %d batches %d shift %d flat %d/%d order %d relabel %d probe %d: %.4f ms\n",
+        fprintf(stderr, "tune trial lanes %d %d %d %d %d block %d chunk %d batches %d shift %d flat %d/%d order %d relabel %d probe %d waves %d: %.4f ms\n",
                 t.lanes[0], t.lanes[1], t.lanes[2], t.lanes[3], t.lanes[4], t.block, t.work_chunk, t.batches, t.load_shift, t.flat_chunk,
-                t.flat_block, t.item_order, t.relabel, t.probe, best_here);
+                t.flat_block, t.item_order, t.relabel, t.probe, t.waves, best_here);
 #endif
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
@@ -2183,6 +2219,11 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
             t.probe = 1 - bt.probe;
             GA_TRY(trial(t));
         }
+        // the occupancy cap, before the lane widths, block size, batches and chunk are chosen under it (tried after
+        // them only, the 4,096-graph batch settled on block 512 uncapped: 93 ms against 88.5 ms at block 256, 5 waves),
+        // and again after the block size and load factor below
+        const int wv[4] = {0, 7, 6, 5};   // waves per SIMD (0: no cap)
+        for (int wi = 0; wi < 4; wi++) { ga_tc_tuning t = bt; t.waves = wv[wi]; if (t.waves != bt.waves) GA_TRY(trial(t)); }
         GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
         GA_TRY(ga_tc_ensure_plan(ctx, g, true));
         const CountView V = count_view(g);
@@ -2201,6 +2242,7 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         const int blocks[3] = {128, 256, 512};
         for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.block = blocks[bi]; if (t.block != bt.block) GA_TRY(trial(t)); }
         for (int ls = 1; ls <= 3; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
+        for (int wi = 0; wi < 4; wi++) { ga_tc_tuning t = bt; t.waves = wv[wi]; if (t.waves != bt.waves) GA_TRY(trial(t)); }
         if (pass == 1) break;
         const int bat[3] = {2, 8, 32};
         for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }

@@ -189,6 +189,9 @@ typedef struct ga_tc_tuning {
                               ceil(length / 8) lanes and all of them are read in the same round; 0 = the lists are read
                               by length class (lists of length in (S/2, S] together, one round per class). Participation
                               and support always read by length class */
+    int32_t waves;         /* plain and per-graph counts: 0 = as many waves per SIMD as registers and tables allow;
+                              4..8 = at most this many, held by the workgroups' LDS allocation (fewer items in flight
+                              keep more of their probe lists in the L2) */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
 /* The library's default tuning (what a graph uses until ga_tc_set_tuning or ga_tc_tune). */

@@ -125,6 +125,7 @@ def test_batch_tunings_flat_per_graph(ctx):
         t.block, t.work_chunk, t.batches, t.load_shift, t.hash_cap = 256, 2048, 4, 2, 4096
         t.flat_chunk, t.flat_block, t.item_order = fc, fb, order
         t.probe = i % 2   # per-graph counts in lane slots or by length class
+        t.waves = (0, 4, 5, 6, 7, 8)[i % 6]   # occupancy caps through LDS
         b.set_tuning(t)
         T, _ = b.count_all()
         assert np.array_equal(T, ref), (lanes, fc, fb, order)
@@ -172,6 +173,7 @@ def test_count_many_shared_launches(ctx):
         t.lanes[:] = lanes
         t.flat_chunk, t.flat_block = (flat, 256) if flat else (0, 256)
         t.hash_cap, t.item_order, t.probe, t.relabel = cap, order, probe, i % 3 == 0
+        t.waves = (0, 5, 7, 6)[i % 4]
         g.set_tuning(t)
     own = [g.count(ga.TC_HASH)[0] for g in dgs]
     assert own == ref

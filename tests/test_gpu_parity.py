@@ -194,6 +194,7 @@ def test_tuning_shapes(ctx):
                 t.item_order = (li + shift) % 3
                 t.relabel = (li + block // 64 + shift) % 2   # the renumbered copy for plain counts
                 t.probe = (li + shift + block // 256) % 2   # lane slots or length classes for plain counts
+                t.waves = (0, 5, 6, 7, 8, 4)[(li + block // 64 + shift) % 6]   # occupancy caps through LDS
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)
@@ -218,6 +219,52 @@ def test_tuning_shapes(ctx):
     g.set_tuning(base)
     best, ms = g.tune()
     assert g.count(ga.TC_HASH)[0] == T and ms > 0
+    g.close()
+
+
+# The multipliers of the lane-slot tables' rebuilds, in the order the kernel tries them.
+SLOT_HASH_MULS = (2654435769, 2246822507, 3266489909, 668265263)
+
+
+def colliding_ids(k, bbits=3):
+    """k vertex ids that land in bucket 0 of a table of 2^bbits buckets under every multiplier (and so under fewer
+    buckets too): a vertex with such out-neighbours leaves keys outside their home bucket in every table tried."""
+    x = np.arange(1, 1 << 22, dtype=np.uint64)
+    ok = np.ones(x.size, bool)
+    for m in SLOT_HASH_MULS:
+        ok &= (((x * np.uint64(m)) & np.uint64(0xFFFFFFFF)) >> np.uint64(32 - bbits)) == 0
+    ids = x[ok][:k]
+    assert ids.size == k
+    return [int(v) for v in ids]
+
+
+def test_slot_tables_that_never_settle(ctx):
+    """Lane-slot counts read only a key's home bucket; a table in which a key left its home bucket is rebuilt with
+    other multipliers, and a row whose every table still has such a key is counted by the per-entry walk. Rows whose
+    out-neighbours collide under all multipliers (cliques of 9 and 6 on such ids, with the smallest id first so that it
+    hashes the others) take that walk in the plain and the per-graph counts, at every load factor and occupancy cap."""
+    ids = colliding_ids(13)
+    verts = [0] + ids[:8]
+    pairs = [(verts[a], verts[b]) for a in range(9) for b in range(a + 1, 9)]
+    w = [7] + ids[8:13]
+    pairs += [(w[a], w[b]) for a in range(6) for b in range(a + 1, 6)]
+    raw = np.array(pairs, np.uint32)
+    T = oracle.OGraph.from_pairs(raw).triangles()
+    assert T == 84 + 20   # C(9,3) + C(6,3)
+    g = ga.DeviceGraph.from_pairs(ctx, raw)
+    other = random_graph(np.random.default_rng(5), 300, 0.08)
+    T_other = oracle.OGraph.from_pairs(other).triangles()
+    b = ga.Batch(ctx, [raw, other])   # the colliding graph first: its ids are unshifted in the union
+    for lanes0 in (4, 8, 16):
+        for shift in (1, 2, 3):
+            for waves in (0, 6):
+                t = ga.TcTuning.from_dict(ga.default_tuning().as_dict())
+                t.lanes[0], t.load_shift, t.probe, t.relabel, t.waves = lanes0, shift, 1, 0, waves
+                g.set_tuning(t)
+                assert g.count(ga.TC_HASH)[0] == T, t.as_dict()
+                b.set_tuning(t)
+                assert [int(x) for x in b.count_all()[0]] == [T, T_other], t.as_dict()
+    b.close()
     g.close()
 
 

@@ -417,14 +417,17 @@ extern "C" int ga_batch_download_graph(ga_ctx* ctx, ga_batch* b, uint32_t i, uin
 extern "C" int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms) {
     if (!ctx || !b || !T) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
+    // One copy of the per-graph counters per XCD (workgroups b with the same b mod 8 add to the same copy), so the
+    // items of one graph in flight on different XCDs do not meet on one address; folded into copy 0 after the launch.
     DevBuf cnt;
-    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count + 1) * 8));
+    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count * 8 + 1) * 8));
     unsigned long long* gc = (unsigned long long*)cnt.p;
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double t0 = wall_ms();
-    GA_TRY(ga_fill_u64(ctx, (uint64_t*)gc, (size_t)b->count + 1, 0));
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)gc, (size_t)b->count * 8 + 1, 0));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    GA_TRY(ga_tc_launch(ctx, b->u, GA_TC_HASH, gc + b->count, nullptr, nullptr, nullptr, nullptr, b->gof, gc));
+    GA_TRY(ga_tc_launch(ctx, b->u, GA_TC_HASH, gc + (size_t)b->count * 8, nullptr, nullptr, nullptr, nullptr, b->gof, gc, b->count));
+    GA_TRY(ga_tc_fold_xcd(ctx, gc, b->count));
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     GA_HIP(ctx, hipMemcpyAsync(T, gc, (size_t)b->count * 8, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -481,7 +484,7 @@ extern "C" int ga_batch_tune(ga_ctx* ctx, ga_batch* b, ga_tc_tuning* best, float
     if (!ctx || !b) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
     DevBuf cnt;
-    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count + 1) * 8));
+    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count * 8 + 1) * 8));
     GA_TRY(ga_tc_tune_counts(ctx, b->u, b->gof, (unsigned long long*)cnt.p, b->count, best, best_ms));   // what ga_batch_count runs
     for (ga_graph* g : b->slices) GA_TRY(ga_tc_set_tuning(ctx, g, &b->u->plan.tune));
     return GA_OK;

@@ -137,7 +137,10 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
 // One count of `variant` into *d_total (zeroed by the caller). Optional: alive mask, per-vertex tri, per-edge sup, a device
 // `done` flag that turns the launch into a no-op, and per-graph counts gcnt[gof[h]] for a disjoint union of graphs.
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* d_alive, uint32_t* d_tri,
-                 uint32_t* d_sup, const uint32_t* d_done, const uint32_t* gof = nullptr, unsigned long long* gcnt = nullptr);
+                 uint32_t* d_sup, const uint32_t* d_done, const uint32_t* gof = nullptr, unsigned long long* gcnt = nullptr,
+                 uint32_t gstride = 0);
+// gcnt[g] = the sum of gcnt[g + n * x] over the eight XCD copies x of a per-graph count launched with gstride n.
+int ga_tc_fold_xcd(ga_ctx* ctx, unsigned long long* gcnt, uint32_t n);
 
 // GA_TC_NOSRC / GA_TC_VERTEX count into *total (zeroed by the caller): ga_tc_simple.hip.
 int ga_tc_launch_simple(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* total);

@@ -241,6 +241,8 @@ struct TcArgs {
     const uint32_t* done;
     const uint32_t* gof;
     unsigned long long* gcnt;
+    uint32_t gstride;      // per-graph counts: 0 = gcnt[g]; else each XCD (blockIdx mod 8) adds to its own copy,
+                           // gcnt[g + gstride * x] (the copies are folded after the launch)
     uint32_t xcd;          // items in contiguous chunks per workgroup, each XCD (blockIdx mod 8) one eighth of them
 };
 
@@ -591,7 +593,7 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
             for (uint32_t o = G / 2; o > 0; o >>= 1) hcount += __shfl_xor(hcount, o, G);
             if (sub == 0 && hcount) {
                 if (FLAGS & F_TRI) tri_add<TC>(ckey, ccnt, a.tri, h, hcount);
-                if (FLAGS & F_GRAPH) atomicAdd(&a.gcnt[a.gof[h]], (unsigned long long)hcount);
+                if (FLAGS & F_GRAPH) atomicAdd(&a.gcnt[a.gof[h] + a.gstride * (blockIdx.x & 7u)], (unsigned long long)hcount);
             }
         }
         group_sync();   // every lane is done with the table before the next item clears it
@@ -658,7 +660,7 @@ __device__ __forceinline__ void tc_tiny_body(const TcArgs& a, uint32_t bid, uint
                 const bool mine = c != 0 && gr == gl;
                 const uint64_t same = __ballot(mine);
                 const uint64_t s = wave_sum_u64(mine ? c : 0u);
-                if (lane == leader) atomicAdd(&a.gcnt[gl], (unsigned long long)s);
+                if (lane == leader) atomicAdd(&a.gcnt[gl + a.gstride * (blockIdx.x & 7u)], (unsigned long long)s);
                 todo &= ~same;
             }
         }
@@ -738,6 +740,7 @@ struct FlatArgs {
     const uint32_t* done;
     const uint32_t* gof;           // GRAPH: graph of each vertex and per-graph counts (a disjoint union of graphs)
     unsigned long long* gcnt;
+    uint32_t gstride;              // as TcArgs::gstride
 };
 
 static size_t flat_shmem(uint32_t chunk) {
@@ -913,7 +916,7 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
             local += hit ? 1u : 0u;
             if (GRAPH) {
                 if (!split) cc += hit ? 1u : 0u;
-                else if (hit) atomicAdd(&a.gcnt[a.gof[a.wh[j0 + own[k * T + tid]]]], 1ull);
+                else if (hit) atomicAdd(&a.gcnt[a.gof[a.wh[j0 + own[k * T + tid]]] + a.gstride * (blockIdx.x & 7u)], 1ull);
             }
         }
         if (GRAPH) {   // one atomic per chunk: wave sums in wtot (free since the prefix sum), thread 0 adds them
@@ -924,12 +927,28 @@ __global__ void __launch_bounds__(1024) k_tc_flat(const FlatArgs a) {
         if (GRAPH && tid == 0 && !split) {   // wtot is rewritten only after the next chunk's first two barriers
             unsigned long long s = 0;
             for (uint32_t w = 0; w < T / 64; w++) s += wtot[w];
-            if (s) atomicAdd(&a.gcnt[g0], s);
+            if (s) atomicAdd(&a.gcnt[g0 + a.gstride * (blockIdx.x & 7u)], s);
         }
     }
     if (GRAPH) return;   // per-graph counts: the graphs' counters hold the total
     local = wave_sum_u64(local);
     if (lane == 0 && local) atomicAdd(a.total, (unsigned long long)local);
+}
+
+// Per-graph counts kept per XCD (gcnt[g + n * x], x < 8): gcnt[g] becomes the sum of graph g's eight copies.
+__global__ void k_fold_xcd(unsigned long long* __restrict__ gcnt, uint32_t n) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
+        unsigned long long s = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < 8; x++) s += gcnt[g + (size_t)n * x];
+        gcnt[g] = s;
+    }
+}
+
+int ga_tc_fold_xcd(ga_ctx* ctx, unsigned long long* gcnt, uint32_t n) {
+    if (!n) return GA_OK;
+    k_fold_xcd<<<grid_for(n, 256, 1024), 256, 0, ctx->stream>>>(gcnt, n);
+    return ga_launch_check(ctx, "k_fold_xcd");
 }
 
 // out += the sum of in[0, n) (out zeroed by the caller).
@@ -1546,7 +1565,7 @@ static bool flat_on(const CountView& V) {
 }
 
 static int launch_flat(ga_ctx* ctx, ga_graph* g, const CountView& V, unsigned long long* total, const uint32_t* done, const uint32_t* gof,
-                       unsigned long long* gcnt) {
+                       unsigned long long* gcnt, uint32_t gstride) {
     const TcPlan& P = *V.P;
     FlatArgs f{};
     f.rowptr = V.rowptr;
@@ -1562,6 +1581,7 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, const CountView& V, unsigned lo
     f.done = done;
     f.gof = gof;
     f.gcnt = gcnt;
+    f.gstride = gstride;
     const size_t shmem = flat_shmem(P.chunk_used);
     if (shmem > ctx->lds_max) return ga_fail(ctx, GA_ERR_ARG, "flat chunk %u needs %zu bytes of LDS", P.chunk_used, shmem);
     const uint32_t block = (uint32_t)P.tune.flat_block;
@@ -1750,7 +1770,7 @@ static int ensure_relabel(ga_ctx* ctx, ga_graph* g) {
 
 // Launches one count of `variant` into *d_total (zeroed by the caller). alive/tri/sup as in the flags.
 int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_total, const uint8_t* alive, uint32_t* tri, uint32_t* sup,
-                 const uint32_t* done, const uint32_t* gof, unsigned long long* gcnt) {
+                 const uint32_t* done, const uint32_t* gof, unsigned long long* gcnt, uint32_t gstride) {
     unsigned long long* total = d_total;
     if (variant == GA_TC_AUTO) variant = GA_TC_HASH;
     const int flags = (alive ? F_ALIVE : 0) | (tri ? F_TRI : 0) | (sup ? F_SUP : 0) | (gcnt ? F_GRAPH : 0);
@@ -1772,8 +1792,9 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         a.done = done;
         a.gof = gof;
         a.gcnt = gcnt;
+        a.gstride = gstride;
         if (counts && flat_on(V)) {
-            GA_TRY(launch_flat(ctx, g, V, total, done, gof, gcnt));
+            GA_TRY(launch_flat(ctx, g, V, total, done, gof, gcnt, gstride));
             return flags ? launch_hash<F_GRAPH>(ctx, g, V, a, 3) : launch_hash<0>(ctx, g, V, a, 3);
         }
         switch (flags) {
@@ -2133,19 +2154,21 @@ extern "C" int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t)
     return GA_OK;
 }
 
-// One timed hash count: plain, or per graph of a disjoint union (gof: graph of each vertex; gcnt: ngraphs + 1 device
+// One timed hash count: plain, or per graph of a disjoint union (gof: graph of each vertex; gcnt: 8 * ngraphs + 1 device
 // words, the last one the total) as ga_batch_count runs it.
 static int timed_count(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned long long* gcnt, uint32_t ngraphs, uint64_t* T, float* ms) {
     if (!gcnt) return ga_tc_count(ctx, g, GA_TC_HASH, T, ms);
     GA_TRY(ga_tc_ensure_plan(ctx, g, true));
-    GA_TRY(ga_fill_u64(ctx, (uint64_t*)gcnt, (size_t)ngraphs + 1, 0));
+    // as ga_batch_count: one copy of the counters per XCD (gcnt holds 8 * ngraphs + 1 words), folded after the launch
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)gcnt, (size_t)ngraphs * 8 + 1, 0));
     GA_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, gcnt + ngraphs, nullptr, nullptr, nullptr, nullptr, gof, gcnt));
+    GA_TRY(ga_tc_launch(ctx, g, GA_TC_HASH, gcnt + (size_t)ngraphs * 8, nullptr, nullptr, nullptr, nullptr, gof, gcnt, ngraphs));
+    GA_TRY(ga_tc_fold_xcd(ctx, gcnt, ngraphs));
     GA_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     // per-graph counts leave the total to their counters: it is their sum
-    k_sum_u64<<<grid_for(ngraphs, 256, 256), 256, 0, ctx->stream>>>(gcnt, ngraphs, gcnt + ngraphs);
+    k_sum_u64<<<grid_for(ngraphs, 256, 256), 256, 0, ctx->stream>>>(gcnt, ngraphs, gcnt + (size_t)ngraphs * 8);
     GA_TRY(ga_launch_check(ctx, "k_sum_u64"));
-    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, gcnt + ngraphs, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, gcnt + (size_t)ngraphs * 8, 8, hipMemcpyDeviceToHost, ctx->stream));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *T = ctx->hstat[0];
     if (ms) GA_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
@@ -2167,13 +2190,16 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
     GA_TRY(timed_count(ctx, g, gof, gcnt, ngraphs, &T0, &ms));
     ga_tc_tuning bt = base;
     float bms = 1e30f;
+    // timed counts per trial (the best one is kept): about 15 ms of counting, at least 3 and at most 16, so that a trial
+    // on a graph counted in about a millisecond is not decided by one slow launch
+    const int reps = std::max(3, std::min(16, (int)std::ceil(15.0f / std::max(ms, 0.01f))));
     auto trial = [&](const ga_tc_tuning& t) -> int {
         GA_TRY(ga_tc_set_tuning(ctx, g, &t));
         float best_here = 1e30f;
         // r = 0 is an untimed warm-up: after a plan rebuild the device has idled on the host's work, and the first
         // counts can run slower than the rest (a trial timed on them alone once kept 8-lane groups for d+(h) <= 128 on
         // the 34.7M-edge graph: 1.58 ms against 1.31 ms)
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r <= reps; r++) {
             uint64_t T = 0;
             float x = 0;
             GA_TRY(timed_count(ctx, g, gof, gcnt, ngraphs, &T, &x));
@@ -2189,8 +2215,8 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         if (best_here < bms) { bms = best_here; bt = t; }
         return GA_OK;
     };
-    // coordinate search: relabel and item order, the probe form, lanes per bucket, then block, load shift, batches and
-    // work chunk.
+    // search: relabel, item order, block size and occupancy cap jointly, then one at a time the probe form, lanes per
+    // bucket, block and cap again, load shift, batches and work chunk.
     // The renumbered copy and its plan need about 60 bytes per edge while they are built (sorts included): it is tried
     // only when that fits in half of the free device memory (a 2.6B-edge batch union does not).
     size_t free_b = 0, total_b = 0;
@@ -2206,24 +2232,33 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
     const double t_first = wall_ms();
     for (int pass = 0; pass < 2; pass++) {
         const ga_tc_tuning start = bt;
-        for (int rl = 0; pass == 0 && rl <= (relabel_fits ? 1 : 0); rl++)
-            for (int io = 0; io <= 2; io++) {
-                ga_tc_tuning t = bt;
-                t.relabel = rl;
-                t.item_order = io;
-                if (t.relabel == bt.relabel && t.item_order == bt.item_order) continue;
-                GA_TRY(trial(t));
-            }
+        // item order, block size and occupancy cap together: they decide which items are in flight at once and so
+        // what the L2 holds, and a search one at a time stops short (on the 4,096-graph batch it kept row order, block
+        // 512 and no cap at 93 ms; shape order, block 512 and 6 waves take 89 ms)
+        const int blocks[3] = {128, 256, 512};
+        const int wv[4] = {0, 7, 6, 5};   // waves per SIMD (0: no cap)
+        auto order_block_cap = [&](bool relabel_too) -> int {
+            for (int rl = 0; rl <= (relabel_too && relabel_fits ? 1 : 0); rl++)
+                for (int io = 0; io <= 2; io++)
+                    for (int bi = 0; bi < 3; bi++)
+                        for (int wi = 0; wi < 4; wi++) {
+                            ga_tc_tuning t = bt;
+                            t.relabel = relabel_too ? rl : bt.relabel;
+                            t.item_order = io;
+                            t.block = blocks[bi];
+                            t.waves = wv[wi];
+                            if (t.relabel == bt.relabel && t.item_order == bt.item_order && t.block == bt.block && t.waves == bt.waves)
+                                continue;
+                            GA_TRY(trial(t));
+                        }
+            return GA_OK;
+        };
+        if (pass == 0) GA_TRY(order_block_cap(true));
         if (pass == 0) {   // how the lane groups read a batch's probe lists: lane slots or length classes
             ga_tc_tuning t = bt;
             t.probe = 1 - bt.probe;
             GA_TRY(trial(t));
         }
-        // the occupancy cap, before the lane widths, block size, batches and chunk are chosen under it (tried after
-        // them only, the 4,096-graph batch settled on block 512 uncapped: 93 ms against 88.5 ms at block 256, 5 waves),
-        // and again after the block size and load factor below
-        const int wv[4] = {0, 7, 6, 5};   // waves per SIMD (0: no cap)
-        for (int wi = 0; wi < 4; wi++) { ga_tc_tuning t = bt; t.waves = wv[wi]; if (t.waves != bt.waves) GA_TRY(trial(t)); }
         GA_TRY(ga_tc_set_tuning(ctx, g, &bt));
         GA_TRY(ga_tc_ensure_plan(ctx, g, true));
         const CountView V = count_view(g);
@@ -2239,10 +2274,16 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
                 GA_TRY(trial(t));
             }
         }
-        const int blocks[3] = {128, 256, 512};
-        for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.block = blocks[bi]; if (t.block != bt.block) GA_TRY(trial(t)); }
+        // block size and occupancy cap again under the lane widths found (the cap's steps depend on the block: whole
+        // workgroups per CU)
+        for (int bi = 0; bi < 3; bi++)
+            for (int wi = 0; wi < 4; wi++) {
+                ga_tc_tuning t = bt;
+                t.block = blocks[bi];
+                t.waves = wv[wi];
+                if (t.block != bt.block || t.waves != bt.waves) GA_TRY(trial(t));
+            }
         for (int ls = 1; ls <= 3; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
-        for (int wi = 0; wi < 4; wi++) { ga_tc_tuning t = bt; t.waves = wv[wi]; if (t.waves != bt.waves) GA_TRY(trial(t)); }
         if (pass == 1) break;
         const int bat[3] = {2, 8, 32};
         for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
@@ -2261,6 +2302,9 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
                 GA_TRY(trial(t));
             }
         }
+        // once more with the batches and work chunk found: shape order gained on the batch union only at the larger
+        // work chunks (row order, block 512, no cap led at 2,048 per item)
+        if (pass == 0) GA_TRY(order_block_cap(false));
         if ((bt.block == start.block && bt.load_shift == start.load_shift) || wall_ms() - t_first > 20000.0) break;
     }
     GA_TRY(ga_tc_set_tuning(ctx, g, &bt));

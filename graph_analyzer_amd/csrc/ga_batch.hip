@@ -39,6 +39,7 @@ struct ga_batch {
     BatchPeelDev pd{};
     int graph_rounds = 8;             // peel rounds captured into one HIP graph (0: enqueue every round directly)
     float count_dev_ms = 0;           // device time of the last ga_batch_count's launches
+    unsigned long long* gcnt = nullptr;  // per-graph counters of the batched count: 8 XCD copies + a total word (lazily)
 };
 
 namespace {
@@ -376,6 +377,7 @@ extern "C" void ga_batch_destroy(ga_batch* b) {
     ga_free(b->tri);
     ga_free(b->rr);
     ga_free(b->pstate);
+    ga_free(b->gcnt);
     delete b;
 }
 
@@ -419,9 +421,8 @@ extern "C" int ga_batch_count(ga_ctx* ctx, ga_batch* b, uint64_t* T, float* ms) 
     GA_HIP(ctx, hipSetDevice(ctx->device));
     // One copy of the per-graph counters per XCD (workgroups b with the same b mod 8 add to the same copy), so the
     // items of one graph in flight on different XCDs do not meet on one address; folded into copy 0 after the launch.
-    DevBuf cnt;
-    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count * 8 + 1) * 8));
-    unsigned long long* gc = (unsigned long long*)cnt.p;
+    if (!b->gcnt) GA_TRY(ga_malloc(ctx, (void**)&b->gcnt, ((size_t)b->count * 8 + 1) * 8));   // kept for later counts
+    unsigned long long* gc = b->gcnt;
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double t0 = wall_ms();
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)gc, (size_t)b->count * 8 + 1, 0));
@@ -483,9 +484,8 @@ extern "C" int ga_batch_get_tuning(const ga_batch* b, ga_tc_tuning* out) {
 extern "C" int ga_batch_tune(ga_ctx* ctx, ga_batch* b, ga_tc_tuning* best, float* best_ms) {
     if (!ctx || !b) return GA_ERR_ARG;
     GA_HIP(ctx, hipSetDevice(ctx->device));
-    DevBuf cnt;
-    GA_TRY(ga_malloc(ctx, &cnt.p, ((size_t)b->count * 8 + 1) * 8));
-    GA_TRY(ga_tc_tune_counts(ctx, b->u, b->gof, (unsigned long long*)cnt.p, b->count, best, best_ms));   // what ga_batch_count runs
+    if (!b->gcnt) GA_TRY(ga_malloc(ctx, (void**)&b->gcnt, ((size_t)b->count * 8 + 1) * 8));
+    GA_TRY(ga_tc_tune_counts(ctx, b->u, b->gof, b->gcnt, b->count, best, best_ms));   // what ga_batch_count runs
     for (ga_graph* g : b->slices) GA_TRY(ga_tc_set_tuning(ctx, g, &b->u->plan.tune));
     return GA_OK;
 }

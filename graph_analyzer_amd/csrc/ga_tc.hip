@@ -2285,10 +2285,17 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
             }
         for (int ls = 1; ls <= 3; ls++) { ga_tc_tuning t = bt; t.load_shift = ls; if (ls != bt.load_shift) GA_TRY(trial(t)); }
         if (pass == 1) break;
+        // batches and work chunk together (both cut long rows into items: on the 4,096-graph batch the best chunk
+        // depended on the batches)
         const int bat[3] = {2, 8, 32};
-        for (int bi = 0; bi < 3; bi++) { ga_tc_tuning t = bt; t.batches = bat[bi]; if (t.batches != bt.batches) GA_TRY(trial(t)); }
         const int chunks[3] = {2048, 8192, 32768};
-        for (int ci = 0; ci < 3; ci++) { ga_tc_tuning t = bt; t.work_chunk = chunks[ci]; if (t.work_chunk != bt.work_chunk) GA_TRY(trial(t)); }
+        for (int bi = 0; bi < 3; bi++)
+            for (int ci = 0; ci < 3; ci++) {
+                ga_tc_tuning t = bt;
+                t.batches = bat[bi];
+                t.work_chunk = chunks[ci];
+                if (t.batches != bt.batches || t.work_chunk != bt.work_chunk) GA_TRY(trial(t));
+            }
         if (V.P->nlight) {
             // 4 or 8 units per thread (16 keeps too many records and loads in registers)
             const int fsh[6][2] = {{0, 256}, {1024, 256}, {2048, 512}, {2048, 256}, {4096, 512}, {4096, 1024}};

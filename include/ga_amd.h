@@ -177,9 +177,11 @@ typedef struct ga_tc_tuning {
                               A multiple of flat_block in [flat_block, 8192]; needs hash_cap >= 128 */
     int32_t flat_block;    /* threads per workgroup of the flat kernel (64..1024, multiple of 64) */
     int32_t item_order;    /* 1: work items of a bucket ordered by shape (entries, longest probe list, d+(h)), so the
-                              lane groups of one wavefront run loops of equal length; 0: in row order; 2: by shape
-                              within windows of 4,096 consecutive h ids, windows in id order, each XCD (workgroups
-                              b with the same b mod 8) taking one contiguous eighth of a bucket's items */
+                              lane groups of one wavefront run loops of equal length (in a batch union: within each
+                              graph, graphs in order); 0: in row order; 2: by shape within windows of 4,096
+                              consecutive h ids, windows in id order; 3: as 2, each XCD (workgroups b with the same
+                              b mod 8) taking one contiguous eighth of a bucket's items; 4, 5: as 2, 3 with windows of
+                              65,536 ids */
     int32_t relabel;       /* 1: plain and per-graph counts run on a private copy of the oriented CSR whose vertices
                               are renumbered so that vertices joined by edges in two or more triangles sit together
                               (each vertex is keyed by the smallest id among itself and such neighbours); the

@@ -81,6 +81,30 @@ def algorithmic_bytes(rowptr, col, src):
     return 4 * (n + 1) + 4 * col.size + 4 * int(mins.sum()), int(mins.sum())
 
 
+def bucket_rooflines(g, B_alg, reps=20):
+    """Roofline of each launch of one count: the launch's share of B_alg (4 bytes per probe element of its entries, plus
+    the CSR structure term 4(n+1) + 4m split over the plan's entries) over its mean device time (HIP events between the
+    launches). The shares add up to B_alg."""
+    ent, el, ms = g.bucket_profile(reps)
+    lanes = list(g.get_tuning().lanes)
+    names = ["d+(h)<=8", "d+(h)<=32", "d+(h)<=128", "d+(h)<=512", "d+(h)<=4096", "binary search (d+(h) above the LDS cap)", "flat kernel (light rows)"]
+    flat = ms[6] > 0
+    if flat:   # the flat kernel takes the light rows (buckets 0-2)
+        ent[6], el[6] = sum(ent[:3]), sum(el[:3])
+        ent[:3], el[:3] = [0, 0, 0], [0, 0, 0]
+    nw = sum(ent)
+    struct = B_alg - 4 * sum(el)   # 4(n+1) + 4m
+    out = []
+    for i in range(7):
+        if not ent[i]:
+            continue
+        b = 4 * el[i] + struct * ent[i] / nw
+        ach = b / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else 0.0
+        out.append({"launch": names[i], "lanes": (lanes[i] if i < 5 else (64 if i == 5 else None)), "entries": int(ent[i]),
+                    "probe_elements": int(el[i]), "algorithmic_bytes": b, "ms": ms[i], "achieved": ach, "frac": ach / HBM_PEAK_GBS})
+    return out
+
+
 def load_traffic(workload):
     path = os.path.join(ROOT, "profiles", f"tc_hash_{workload}_traffic.json")
     if not os.path.exists(path):
@@ -488,6 +512,10 @@ def main():
     value = g.m / (wall / args.steps)
     achieved = B_alg / (mean_dev * 1e-3) / 1e9
     log(f"T={T0} ms/step={ms_per_step:.4f} device mean {mean_dev:.4f} ms (min {min(dev_ms):.4f})")
+    per_launch = bucket_rooflines(g, B_alg) if variant == ga.TC_HASH else None
+    for r in per_launch or []:
+        log(f"  launch {r['launch']} ({r['lanes']} lanes): {r['entries']} entries, B {r['algorithmic_bytes'] / 1e9:.3f} GB, "
+            f"{r['ms']:.4f} ms -> {r['achieved']:.0f} GB/s = {100 * r['frac']:.1f} %")
     if args.copy_probe:
         cms = ctx.copy_probe(args.copy_probe << 20)
         log(f"copy probe: {args.copy_probe} MiB read + {args.copy_probe} MiB written in {cms:.3f} ms")
@@ -563,7 +591,8 @@ def main():
                    "tuning": tune_info, "build_ms": g.build_ms, "plan_ms": prep_ms},
         "triangles_per_s": T0 / (wall / args.steps),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": B_alg, "sum_min_outdeg": summin, "kernel_ms_mean": mean_dev},
+                     "traffic": traffic, "algorithmic_bytes": B_alg, "sum_min_outdeg": summin, "kernel_ms_mean": mean_dev,
+                     "per_launch": per_launch},
         "cpu_baseline": cpu,
         "peel": gpu_peel,
         "extras": extras,

@@ -138,6 +138,7 @@ def lib() -> ctypes.CDLL:
             "ga_tc_default_tuning": (ctypes.c_int, [ctypes.POINTER(TcTuning)]),
             "ga_tc_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_plan_info": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
+            "ga_tc_bucket_profile": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, _u64p, ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_vertex": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u8p, _u32p, _u64p]),
             "ga_tc_edge_support": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u32p, _u64p]),
             "ga_peel_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
@@ -424,6 +425,16 @@ class DeviceGraph:
         it = np.zeros(6, np.uint32)
         self.ctx.check(lib().ga_tc_plan_info(self.ctx.handle, self._h, ctypes.byref(nw), _ptr(it, _u32p)), "tc_plan_info")
         return nw.value, it.tolist()
+
+    def bucket_profile(self, reps: int = 20):
+        """Per launch of the plain count (5 hashed buckets, the binary-search bucket, the flat kernel): work entries,
+        probe elements and mean device ms over reps counts."""
+        ent = np.zeros(7, np.uint64)
+        el = np.zeros(7, np.uint64)
+        ms = (ctypes.c_float * 7)()
+        self.ctx.check(lib().ga_tc_bucket_profile(self.ctx.handle, self._h, int(reps), _ptr(ent, _u64p), _ptr(el, _u64p), ms),
+                       "tc_bucket_profile")
+        return ent.tolist(), el.tolist(), list(ms)
 
     def get_tuning(self) -> TcTuning:
         t = TcTuning()

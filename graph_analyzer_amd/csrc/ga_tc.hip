@@ -1608,11 +1608,14 @@ static int launch_hashed(ga_ctx* ctx, ga_graph* g, const CountView& V, const uin
     return launch_hash_bucket<G, true, FLAGS>(ctx, g, V, items, ni, bound, a);
 }
 
+// evs (optional, NHASH + 2 events): evs[b] is recorded before bucket b's launch, evs[NHASH] before the binary-search
+// bucket's and evs[NHASH + 1] after it (per-launch timing of ga_tc_bucket_profile).
 template <int FLAGS>
-static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, int first_bucket = 0) {
+static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, int first_bucket = 0, hipEvent_t* evs = nullptr) {
     const TcPlan& P = *V.P;
     uint32_t off = 0;
     for (int b = 0; b < NHASH; b++) {
+        if (evs) GA_HIP(ctx, hipEventRecord(evs[b], ctx->stream));
         const uint32_t* items = P.items + 4ull * off;
         const uint32_t ni = b < first_bucket ? 0u : P.nitems[b];
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
@@ -1631,7 +1634,10 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, i
         GA_TRY(rc);
         off += P.nitems[b];
     }
-    return launch_hash_bucket<64, false, FLAGS>(ctx, g, V, P.items + 4ull * off, P.nhuge, 0, a);
+    if (evs) GA_HIP(ctx, hipEventRecord(evs[NHASH], ctx->stream));
+    GA_TRY((launch_hash_bucket<64, false, FLAGS>(ctx, g, V, P.items + 4ull * off, P.nhuge, 0, a)));
+    if (evs) GA_HIP(ctx, hipEventRecord(evs[NHASH + 1], ctx->stream));
+    return GA_OK;
 }
 
 template <int FLAGS>
@@ -2127,6 +2133,98 @@ extern "C" int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32
         for (int b = 0; b < NHASH; b++) nitems6[b] = P.nitems[b];
         nitems6[NHASH] = P.nhuge;
     }
+    return GA_OK;
+}
+
+struct BucketStarts {   // first item of each bucket of a plan (by value into k_bucket_work)
+    uint32_t s[NHASH + 2];
+};
+
+namespace {
+// Work of each bucket's items: entries and probe elements (each entry's list is min(d+u, d+v) long). bstart[b] is the
+// first item of bucket b (NHASH hashed buckets, then the binary-search one; bstart[NHASH + 1] = all items).
+__global__ void k_bucket_work(const uint4* __restrict__ items, const uint32_t* __restrict__ wlen, BucketStarts bs,
+                              unsigned long long* __restrict__ out) {
+    for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < bs.s[NHASH + 1]; it += gridDim.x * blockDim.x) {
+        uint32_t b = 0;
+        while (b < NHASH && it >= bs.s[b + 1]) b++;
+        const uint4 im = items[it];
+        const uint32_t ne = im.w & ITEM_NMASK;
+        unsigned long long s = 0;
+        for (uint32_t j = im.y; j < im.y + ne; j++) s += wlen[j];
+        atomicAdd(&out[2 * b], (unsigned long long)ne);
+        atomicAdd(&out[2 * b + 1], s);
+    }
+}
+}  // namespace
+
+// Per-launch profile of the plain count as tuned: for each bucket (NHASH hashed buckets, the binary-search bucket, and
+// the flat kernel in slot NHASH + 1 when it takes the light rows) its work entries, probe elements and the mean device
+// time of its launch over `reps` counts (HIP events between the launches on the library stream). The count itself is
+// verified against the first one.
+extern "C" int ga_tc_bucket_profile(ga_ctx* ctx, ga_graph* g, int reps, uint64_t* entries7, uint64_t* elems7, float* ms7) {
+    if (!ctx || !g || !entries7 || !elems7 || !ms7 || reps < 1) return GA_ERR_ARG;
+    GA_HIP(ctx, hipSetDevice(ctx->device));
+    GA_TRY(ga_tc_ensure_plan(ctx, g, true));
+    const CountView V = count_view(g);
+    const TcPlan& P = *V.P;
+    BucketStarts bs{};
+    for (int b = 0; b < NHASH; b++) bs.s[b + 1] = bs.s[b] + P.nitems[b];
+    bs.s[NHASH + 1] = bs.s[NHASH] + P.nhuge;
+    unsigned long long* d = (unsigned long long*)ctx->dsmall;   // 64 words: [0, 2 * (NHASH + 1)) work, [32] the count
+    GA_TRY(ga_fill_u64(ctx, (uint64_t*)d, 2 * (NHASH + 1), 0));
+    if (bs.s[NHASH + 1]) {
+        k_bucket_work<<<grid_for(bs.s[NHASH + 1], 256, 4096), 256, 0, ctx->stream>>>((const uint4*)P.items, P.wlen, bs, d);
+        GA_TRY(ga_launch_check(ctx, "k_bucket_work"));
+    }
+    GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, d, 2 * (NHASH + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int b = 0; b <= NHASH; b++) {
+        entries7[b] = ctx->hstat[2 * b];
+        elems7[b] = ctx->hstat[2 * b + 1];
+    }
+    entries7[NHASH + 1] = elems7[NHASH + 1] = 0;
+    hipEvent_t ev[NHASH + 4] = {};
+    int rc = GA_OK;
+    for (int i = 0; i < NHASH + 4 && rc == GA_OK; i++)
+        if (hipEventCreate(&ev[i]) != hipSuccess) rc = ga_fail(ctx, GA_ERR_HIP, "hipEventCreate failed");
+    double acc[NHASH + 2] = {};
+    uint64_t T0 = 0;
+    TcArgs a{};
+    a.rowptr = V.rowptr;
+    a.col = V.col;
+    a.wcol = P.wcol;
+    a.wedge = P.wedge;
+    a.wps = P.wps;
+    a.wlen = P.wlen;
+    a.total = d + 32;
+    const bool flat = flat_on(V);
+    for (int r = 0; r <= reps && rc == GA_OK; r++) {   // r = 0: untimed warm-up
+        rc = ga_fill_u64(ctx, (uint64_t*)(d + 32), 1, 0);
+        if (rc == GA_OK && flat) rc = hipEventRecord(ev[NHASH + 2], ctx->stream) == hipSuccess ? GA_OK : GA_ERR_HIP;
+        if (rc == GA_OK && flat) rc = launch_flat(ctx, g, V, d + 32, nullptr, nullptr, nullptr, 0);
+        if (rc == GA_OK && flat) rc = hipEventRecord(ev[NHASH + 3], ctx->stream) == hipSuccess ? GA_OK : GA_ERR_HIP;
+        if (rc == GA_OK) rc = launch_hash<0>(ctx, g, V, a, flat ? 3 : 0, ev);
+        if (rc == GA_OK && hipMemcpyAsync(ctx->hstat, d + 32, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
+        if (rc == GA_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = GA_ERR_HIP;
+        if (rc != GA_OK) break;
+        if (r == 0) T0 = ctx->hstat[0];
+        else if (ctx->hstat[0] != T0) rc = ga_fail(ctx, GA_ERR_STATE, "count changed between profiled runs");
+        for (int b = 0; r > 0 && rc == GA_OK && b <= NHASH; b++) {
+            float x = 0;
+            if (hipEventElapsedTime(&x, ev[b], ev[b + 1]) != hipSuccess) rc = GA_ERR_HIP;
+            acc[b] += x;
+        }
+        if (r > 0 && rc == GA_OK && flat) {
+            float x = 0;
+            if (hipEventElapsedTime(&x, ev[NHASH + 2], ev[NHASH + 3]) != hipSuccess) rc = GA_ERR_HIP;
+            acc[NHASH + 1] += x;
+        }
+    }
+    for (int i = 0; i < NHASH + 4; i++)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    GA_TRY(rc);
+    for (int b = 0; b < NHASH + 2; b++) ms7[b] = (float)(acc[b] / reps);
     return GA_OK;
 }
 

@@ -201,6 +201,10 @@ int ga_tc_default_tuning(ga_tc_tuning* out);
 int ga_tc_set_tuning(ga_ctx* ctx, ga_graph* g, const ga_tc_tuning* t);
 /* Count plan statistics: work entries (edges with min(d+) > 0), items per bucket (5 hashed + 1 binary-search). */
 int ga_tc_plan_info(ga_ctx* ctx, ga_graph* g, uint32_t* nwork, uint32_t* nitems6);
+/* Per-launch profile of the plain count as tuned (no reference counterpart: measurement): for each of the 5 hashed
+ * buckets, the binary-search bucket and the flat kernel (slot 6, light rows, when it is on), the work entries, probe
+ * elements (sum of min(d+u, d+v) over the entries) and the mean device time of the launch over reps counts. */
+int ga_tc_bucket_profile(ga_ctx* ctx, ga_graph* g, int reps, uint64_t* entries7, uint64_t* elems7, float* ms7);
 /* Times candidate tunings on this graph (each verified against the first count) and keeps the fastest. */
 int ga_tc_tune(ga_ctx* ctx, ga_graph* g, ga_tc_tuning* best, float* best_ms);
 /* variant: GA_TC_MERGE or GA_TC_HASH (GA_TC_AUTO = GA_TC_HASH) for the two calls below.

@@ -268,6 +268,27 @@ def test_slot_tables_that_never_settle(ctx):
     g.close()
 
 
+def test_bucket_profile_work(ctx):
+    """The per-launch profile accounts for every work entry of the plan once, and its probe elements add up to the sum
+    of min(d+u, d+v) over the oriented edges (the B_alg term), whatever tuning splits them over the launches."""
+    from graph_analyzer_amd import gen
+    raw = gen.chung_lu(20000, 200000, 2.1, 1.0, seed=9)
+    g = ga.DeviceGraph.from_pairs(ctx, raw)
+    r, c, s = g.download()
+    d = np.diff(r.astype(np.int64))
+    mins = np.minimum(d[s], d[c])
+    for flat, relabel, order in ((0, 0, 1), (2048, 0, 0), (0, 1, 2), (1024, 1, 1)):
+        t = g.get_tuning()
+        t.flat_chunk, t.flat_block, t.relabel, t.item_order = flat, 256, relabel, order
+        g.set_tuning(t)
+        ent, el, ms = g.bucket_profile(2)
+        nw, _ = g.plan_info()
+        assert sum(ent) == nw == int((mins > 0).sum()), (flat, relabel)
+        assert sum(el) == int(mins.sum()), (flat, relabel)
+        assert all(x >= 0 for x in ms) and (ms[6] > 0) == (flat > 0), (flat, relabel, ms)
+    g.close()
+
+
 def test_relabel_counts_on_every_graph_kind(ctx):
     """The renumbered copy (relabel = 1) with the windowed, XCD-contiguous item order counts every graph kind exactly,
     including graphs with isolated vertices, a star, cliques sharing vertices and triangle-free graphs; the batch union

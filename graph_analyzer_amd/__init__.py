@@ -47,6 +47,11 @@ class GAError(RuntimeError):
         self.code = code
 
 
+def _exact_dict(out, st):
+    return {"tri": int(out[0]), "verts": int(out[1]), "set": [int(x) for x in st[: int(out[2])]], "ok": bool(out[3]),
+            "flow_calls": int(out[4]), "components_searched": int(out[5]), "shrinks": int(out[6])}
+
+
 class TcTuning(ctypes.Structure):
     _fields_ = [("lanes", ctypes.c_int32 * 5), ("block", ctypes.c_int32), ("work_chunk", ctypes.c_int32),
                 ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32),
@@ -139,6 +144,8 @@ def lib() -> ctypes.CDLL:
             "ga_tc_tune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(TcTuning), ctypes.POINTER(ctypes.c_float)]),
             "ga_tc_plan_info": (ctypes.c_int, [_vp, _vp, _u32p, _u32p]),
             "ga_tc_bucket_profile": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u64p, _u64p, ctypes.POINTER(ctypes.c_float)]),
+            "ga_core_exact": (ctypes.c_int, [_vp, _vp, _u32p, ctypes.c_int, _u64p, _u32p, _u64p, _u32p]),
+            "ga_exact_densest": (ctypes.c_int, [_vp, _vp, _u64p, _u32p]),
             "ga_tc_vertex": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u8p, _u32p, _u64p]),
             "ga_tc_edge_support": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u32p, _u64p]),
             "ga_peel_create": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
@@ -425,6 +432,30 @@ class DeviceGraph:
         it = np.zeros(6, np.uint32)
         self.ctx.check(lib().ga_tc_plan_info(self.ctx.handle, self._h, ctypes.byref(nw), _ptr(it, _u32p)), "tc_plan_info")
         return nw.value, it.tolist()
+
+    def core_exact(self, core, pruning12: bool = True, seed=None):
+        """CoreExact (dsg.h:454-627) on the core numbers `core`; seed = (tri, verts, vertex set) or None. Returns a dict
+        of tri, verts, set, flow_calls, components_searched, shrinks (GAError when a self-check fails)."""
+        c = np.ascontiguousarray(core, dtype=np.uint32)
+        if c.size != self.n:
+            raise ValueError("core must have n entries")
+        out = np.zeros(7, np.uint64)
+        st = np.zeros(max(self.n, 1), np.uint32)
+        tv = ss = None
+        if seed is not None:
+            tv = np.array([seed[0], seed[1]], np.uint64)
+            ss = np.ascontiguousarray(seed[2], dtype=np.uint32)
+        self.ctx.check(lib().ga_core_exact(self.ctx.handle, self._h, _ptr(c, _u32p), 1 if pruning12 else 0,
+                                           None if tv is None else _ptr(tv, _u64p), None if ss is None else _ptr(ss, _u32p),
+                                           _ptr(out, _u64p), _ptr(st, _u32p)), "core_exact")
+        return _exact_dict(out, st)
+
+    def exact_densest(self):
+        """Exact (dsg.h:402-428): Alg 1 on the whole graph; the same dict as core_exact."""
+        out = np.zeros(7, np.uint64)
+        st = np.zeros(max(self.n, 1), np.uint32)
+        self.ctx.check(lib().ga_exact_densest(self.ctx.handle, self._h, _ptr(out, _u64p), _ptr(st, _u32p)), "exact_densest")
+        return _exact_dict(out, st)
 
     def bucket_profile(self, reps: int = 20):
         """Per launch of the plain count (5 hashed buckets, the binary-search bucket, the flat kernel): work entries,

@@ -263,9 +263,9 @@ struct DsgResult {
     uint64_t tri = 0, verts = 0;          // best subgraph: triangles / vertices (0/0 = graph without triangles)
     std::vector<uint32_t> set;            // its vertices, ascending
     std::vector<Dens> per_pass;           // best density after pass 1, 2, ..., T
-    uint32_t flow_calls = 0;              // max-flow fields of the reference record: always 0 here
-    uint32_t components_searched = 0;
-    uint32_t shrinks = 0;
+    uint32_t flow_calls = 0;              // core_exact / exact_densest: number of max-flow computations
+    uint32_t components_searched = 0;     // core_exact: components that entered the binary search
+    uint32_t shrinks = 0;                 // core_exact: times the network was rebuilt on a higher core
     bool ok = true;
     std::string error;
     Dens dens() const { return Dens{tri, verts}; }
@@ -320,4 +320,66 @@ inline bool core_decomposition(const Graph& g, std::vector<uint32_t>& core, uint
     if (!r.ok) return false;
     if (kmax_out) *kmax_out = (uint32_t)r.kmax;
     return true;
+}
+
+// CoreExact (dsg.h:454-627) on a resident graph: ga_core_exact runs the triangle passes and the induced subgraphs on the
+// device and the max-flow on the host. Same arguments, checks and result record as the reference (flow_calls,
+// components_searched and shrinks included).
+inline DsgResult core_exact_device(const gaa::DeviceGraph& g, const std::vector<uint32_t>& core, bool use_pruning12 = true,
+                                   const DsgResult* seed = nullptr) {
+    DsgResult r;
+    if (core.size() != g.n()) {
+        r.ok = false;
+        r.error = "core[] has the wrong size";
+        return r;
+    }
+    const bool use_seed = seed && seed->ok && seed->verts > 0;
+    if (use_seed && seed->set.size() != seed->verts) {
+        r.ok = false;
+        r.error = "seed is not a valid subgraph of the graph";
+        return r;
+    }
+    const uint64_t tv[2] = {use_seed ? seed->tri : 0, use_seed ? seed->verts : 0};
+    uint64_t out[7] = {};
+    r.set.resize(std::max<uint32_t>(g.n(), 1));
+    const int rc = ga_core_exact(g.ctx(), g.handle(), core.data(), use_pruning12 ? 1 : 0, use_seed ? tv : nullptr,
+                                 use_seed ? seed->set.data() : nullptr, out, r.set.data());
+    r.flow_calls = (uint32_t)out[4];
+    r.components_searched = (uint32_t)out[5];
+    r.shrinks = (uint32_t)out[6];
+    if (rc != GA_OK) {
+        r.ok = false;
+        r.error = ga_ctx_last_error(g.ctx());
+        r.set.clear();
+        return r;
+    }
+    r.tri = out[0];
+    r.verts = out[1];
+    r.set.resize(out[2]);
+    return r;
+}
+
+// The reference's host signatures (dsg.h:402, 454): upload, run.
+inline DsgResult core_exact(const Graph& g, const std::vector<uint32_t>& core, bool use_pruning12 = true,
+                            const DsgResult* seed = nullptr) {
+    return core_exact_device(gaa::DeviceGraph::upload(gaa::default_ctx(), g), core, use_pruning12, seed);
+}
+
+inline DsgResult exact_densest(const Graph& g) {
+    const gaa::DeviceGraph d = gaa::DeviceGraph::upload(gaa::default_ctx(), g);
+    DsgResult r;
+    uint64_t out[7] = {};
+    r.set.resize(std::max<uint32_t>(d.n(), 1));
+    const int rc = ga_exact_densest(d.ctx(), d.handle(), out, r.set.data());
+    r.flow_calls = (uint32_t)out[4];
+    if (rc != GA_OK) {
+        r.ok = false;
+        r.error = ga_ctx_last_error(d.ctx());
+        r.set.clear();
+        return r;
+    }
+    r.tri = out[0];
+    r.verts = out[1];
+    r.set.resize(out[2]);
+    return r;
 }

@@ -305,6 +305,20 @@ int ga_greedy_pp(ga_ctx* ctx, ga_graph* g, uint32_t passes, uint64_t* tri, uint6
 /* The largest vertex count ga_greedy_pp keeps in LDS on this device, and the limit of ga_greedy_pp_many (no counterpart
  * in the reference, whose CPU code has none). */
 uint32_t ga_greedy_max_vertices(ga_ctx* ctx);
+/* core_exact (dsg.h:454-627, DSD paper Alg 4): the densest subgraph for the triangle density, exactly, by max-flow on
+ * the connected components of the pruned cores. core: the (k,triangle)-core numbers of g (host, n entries; from
+ * ga_core_download or core_decomposition). use_pruning12 = 0 is the reference's test hook (start from one triangle).
+ * seed_tri_verts = {tri, verts} and seed_set (verts ids) of another algorithm's result on g (Greedy++), or NULL: it
+ * is checked (ids, no duplicates, triangle count) and then bounds the search from below, as gpu_dsg.cu:348-361 uses it.
+ * out7 = {tri, verts, set size, ok, flow_calls, components_searched, shrinks} (DsgResult); set: capacity n, the best
+ * subgraph's vertices ascending. The triangle passes and induced subgraphs run on the device, the components and the
+ * max-flow (Dinic) on the host. A failed self-check of the reference (inconsistent core numbers, an invalid seed, flow
+ * capacities of 2^59 or more, a min cut not denser than alpha) is GA_ERR_STATE with out7[3] = 0 and the reference's
+ * message in ga_ctx_last_error. */
+int ga_core_exact(ga_ctx* ctx, ga_graph* g, const uint32_t* core, int use_pruning12, const uint64_t* seed_tri_verts,
+                  const uint32_t* seed_set, uint64_t* out7, uint32_t* set);
+/* exact_densest (dsg.h:402-428, Alg 1): the same search on one network of the whole graph. out7 and set as above. */
+int ga_exact_densest(ga_ctx* ctx, ga_graph* g, uint64_t* out7, uint32_t* set);
 /* Greedy++ with `passes` passes on each of `count` graphs of this context, graph by graph equal to ga_greedy_pp: one
  * workgroup per graph in one launch for graphs of up to 1,120 vertices (the LDS-bitmap form on MI355X), one launch
  * per graph for larger ones (each at most ga_greedy_max_vertices). stats: 2 per graph, {tri, verts} of the best density

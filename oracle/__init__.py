@@ -108,6 +108,8 @@ def rlib():
             "ref_induced_subgraph": (_vp, [_vp, _u8p, _u32p, _u32p]),
             "ref_prune_to_core": (_vp, [_vp, _u32p, ctypes.c_uint64, _u32p, _u32p, _u32p]),
             "ref_greedy_pp": (None, [_vp, ctypes.c_uint32, _u64p, _u32p, _u64p]),
+            "ref_core_exact": (None, [_vp, _u32p, ctypes.c_int, _u64p, _u32p, _u64p, _u32p]),
+            "ref_exact_densest": (None, [_vp, _u64p, _u32p]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -126,6 +128,11 @@ def _greedy(fn, h, n, passes):
     if not out[3]:
         raise RuntimeError("greedy_pp: triangle bookkeeping inconsistent")
     return int(out[0]), int(out[1]), st[: int(out[2])], [(int(a), int(b)) for a, b in pp.reshape(-1, 2)]
+
+
+def _dsg_dict(out, st):
+    return {"tri": int(out[0]), "verts": int(out[1]), "set": [int(x) for x in st[: int(out[2])]], "ok": bool(out[3]),
+            "flow_calls": int(out[4]), "components_searched": int(out[5]), "shrinks": int(out[6])}
 
 
 class OGraph:
@@ -325,6 +332,26 @@ class RGraph:
 
     def greedy_pp(self, passes=1):
         return _greedy(rlib().ref_greedy_pp, self._h, self.n, passes)
+
+    def core_exact(self, core, pruning12=True, seed=None):
+        """Reference core_exact: dict of tri, verts, set, ok, flow_calls, components_searched, shrinks."""
+        c = np.ascontiguousarray(core, dtype=np.uint32)
+        out = np.zeros(7, np.uint64)
+        st = np.zeros(max(self.n, 1), np.uint32)
+        if seed is None:
+            rlib().ref_core_exact(self._h, _p(c, _u32p), 1 if pruning12 else 0, None, None, _p(out, _u64p), _p(st, _u32p))
+        else:
+            tv = np.array([seed[0], seed[1]], np.uint64)
+            ss = np.ascontiguousarray(seed[2], dtype=np.uint32)
+            rlib().ref_core_exact(self._h, _p(c, _u32p), 1 if pruning12 else 0, _p(tv, _u64p), _p(ss, _u32p), _p(out, _u64p),
+                                  _p(st, _u32p))
+        return _dsg_dict(out, st)
+
+    def exact_densest(self):
+        out = np.zeros(7, np.uint64)
+        st = np.zeros(max(self.n, 1), np.uint32)
+        rlib().ref_exact_densest(self._h, _p(out, _u64p), _p(st, _u32p))
+        return _dsg_dict(out, st)
 
     def prune_to_core(self, core, k):
         c = np.ascontiguousarray(core, dtype=np.uint32)

@@ -138,4 +138,32 @@ void ref_greedy_pp(const ref_graph* h, uint32_t passes, uint64_t* out, uint32_t*
 
 int ref_peel_arith_ok(uint32_t n, uint64_t T0, uint64_t num, uint64_t den) { return peel_arith_ok(n, T0, num, den) ? 1 : 0; }
 
+// out = {tri, verts, set_size, ok, flow_calls, components_searched, shrinks}; set has room for n ids.
+static void put_dsg(const DsgResult& r, uint64_t* out, uint32_t* set) {
+    out[0] = r.tri;
+    out[1] = r.verts;
+    out[2] = r.set.size();
+    out[3] = r.ok ? 1 : 0;
+    out[4] = r.flow_calls;
+    out[5] = r.components_searched;
+    out[6] = r.shrinks;
+    std::copy(r.set.begin(), r.set.end(), set);
+}
+
+// Reference core_exact (dsg.h:454-627) on the given core numbers; seed (tri, verts, its set) or seed_tv == NULL.
+void ref_core_exact(const ref_graph* h, const uint32_t* core, int use_pruning12, const uint64_t* seed_tv, const uint32_t* seed_set,
+                    uint64_t* out, uint32_t* set) {
+    std::vector<uint32_t> c(core, core + h->g.n);
+    DsgResult seed;
+    if (seed_tv) {
+        seed.tri = seed_tv[0];
+        seed.verts = seed_tv[1];
+        seed.set.assign(seed_set, seed_set + seed_tv[1]);
+    }
+    put_dsg(core_exact(h->g, c, use_pruning12 != 0, seed_tv ? &seed : nullptr), out, set);
+}
+
+// Reference exact_densest (dsg.h:402-428).
+void ref_exact_densest(const ref_graph* h, uint64_t* out, uint32_t* set) { put_dsg(exact_densest(h->g), out, set); }
+
 }  // extern "C"

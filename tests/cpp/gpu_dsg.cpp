@@ -10,15 +10,17 @@
 //   restricted ones; exit 1 otherwise. Then Greedy++ with T passes on the pruned graph (gpu_dsg.cu:344-356):
 //   cpu_greedy_pp_pruned (the oracle) and gpu_greedy_pp_pruned (ga_greedy_pp), rounds = T; the GPU result (density, set
 //   and the best after every pass) must equal the oracle's. --no-greedypp skips it, and so does a pruned graph larger
-//   than ga_greedy_max_vertices (said on stderr).
-// The max-flow part of the pipeline (CoreExact) is not part of this library.
+//   than ga_greedy_max_vertices (said on stderr). Then CoreExact on the pruned graph seeded with the Greedy++ result
+//   (gpu_dsg.cu:348-361): gpu_core_exact_pruned_seeded (core_exact through ga_core_exact), rounds = flow calls. Checked
+//   against the oracle: the set's triangle count, a density no lower than Greedy++'s, and, on pruned graphs of at most
+//   2,000 vertices, the same density from Exact (Alg 1) on the whole pruned graph. --no-exact skips it.
 // Integration test: links the CPU oracle as the checker and as the CPU row.
 #include "../../graph_analyzer_amd/host/dsg.h"
 #include "../../oracle/oracle.hpp"
 
 int main(int argc, char** argv) {
     std::vector<const char*> pos;
-    bool recount = false, burnin = true, verbose = false, gpp = true, greedy_on = true;
+    bool recount = false, burnin = true, verbose = false, gpp = true, greedy_on = true, exact_on = true;
     int repeats = 5;
     uint32_t T_pass = 10;
     for (int i = 1; i < argc; i++) {
@@ -27,13 +29,14 @@ int main(int argc, char** argv) {
         else if (strcmp(argv[i], "--verbose") == 0) verbose = true;
         else if (strcmp(argv[i], "--no-greedypp") == 0) gpp = false;
         else if (strcmp(argv[i], "--no-greedy") == 0) greedy_on = false;
+        else if (strcmp(argv[i], "--no-exact") == 0) exact_on = false;
         else if (strcmp(argv[i], "--T") == 0 && i + 1 < argc) T_pass = (uint32_t)std::max(1, atoi(argv[++i]));
         else if (strcmp(argv[i], "--repeats") == 0 && i + 1 < argc) repeats = std::max(1, atoi(argv[++i]));
         else if (argv[i][0] == '-') {}
         else pos.push_back(argv[i]);
     }
     if (pos.size() != 1) {
-        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedy] [--no-greedypp] [--verbose]\n",
+        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedy] [--no-greedypp] [--no-exact] [--verbose]\n",
                 argv[0]);
         return 2;
     }
@@ -127,6 +130,7 @@ int main(int argc, char** argv) {
         if (dens_gt(Dens{cgr.tri, cgr.verts}, lb)) lb = Dens{cgr.tri, cgr.verts};
     }
     Graph gp;
+    std::vector<uint32_t> core_pruned;
     if (ok) {
         kprune = dens_ceil(lb);
         std::vector<uint32_t> old_id, core_p;
@@ -154,6 +158,7 @@ int main(int argc, char** argv) {
         } else {
             fprintf(stderr, "lower bound %llu/%llu -> keep core >= %llu: pruned graph n=%u m=%u (of n=%u m=%u)\n", (unsigned long long)lb.num,
                     (unsigned long long)lb.den, (unsigned long long)kprune, gp.n, gp.m, og.n, og.m);
+            core_pruned = core_p;
         }
     }
     // ---- Greedy++ on the pruned graph (gpu_dsg.cu:344-356) ----
@@ -187,6 +192,42 @@ int main(int argc, char** argv) {
             fprintf(stderr, "greedy++ (%u passes) on the pruned graph: %llu/%llu\n", T_pass, (unsigned long long)gpr.tri, (unsigned long long)gpr.verts);
         }
     }
+    // ---- CoreExact on the pruned graph, seeded with Greedy++ (gpu_dsg.cu:348-361) ----
+    DsgResult exr;
+    double t_exact = 1e300;
+    bool have_exact = false;
+    if (ok && exact_on) {
+        gaa::DeviceGraph dgp = gaa::DeviceGraph::upload(ctx, gp);
+        for (int i = -1; i < repeats && ok; i++) {
+            gaa::check(ctx, ga_ctx_sync(ctx), "sync");
+            const double te = now_ms();
+            exr = core_exact_device(dgp, core_pruned, true, have_gpp ? &gpr : nullptr);
+            if (i >= 0) t_exact = std::min(t_exact, now_ms() - te);
+            if (!exr.ok) { fprintf(stderr, "core_exact failed: %s\n", exr.error.c_str()); ok = false; }
+        }
+        if (ok) {
+            orc::Graph op;
+            std::vector<uint32_t> oold, ocore;
+            orc::prune_to_core(og, core_cpu, kprune, op, oold, ocore);
+            std::vector<uint8_t> in(op.n, 0);
+            for (uint32_t v : exr.set) in[v] = 1;
+            std::vector<uint32_t> otri;
+            const uint64_t tin = orc::participation(op, in, otri);   // triangles inside the set
+            bool good = exr.set.size() == exr.verts && tin == exr.tri && (!have_gpp || !dens_gt(Dens{gpr.tri, gpr.verts}, exr.dens()));
+            if (good && gp.n <= 2000) {
+                const DsgResult ed = exact_densest(gp);
+                good = ed.ok && dens_cmp(ed.dens(), exr.dens()) == 0;
+            }
+            if (!good) {
+                fprintf(stderr, "MISMATCH: core_exact's result fails the oracle's checks\n");
+                ok = false;
+            } else {
+                have_exact = true;
+                fprintf(stderr, "core_exact on the pruned graph: %llu/%llu, %u flow calls\n", (unsigned long long)exr.tri,
+                        (unsigned long long)exr.verts, exr.flow_calls);
+            }
+        }
+    }
     if (!ok) { fprintf(stderr, "VERIFICATION FAILED - no timing rows printed\n"); return 1; }
     rows.push_back(dsg_row("gpu_core_decomp", path, og.n, og.m, ktri, kverts, best_inc, 0, inc.rounds, kmax));
     rows.push_back(dsg_row("gpu_core_decomp_fused", path, og.n, og.m, ktri, kverts, best_fused, 0, fused_rounds, kmax));
@@ -197,6 +238,7 @@ int main(int argc, char** argv) {
         rows.push_back(dsg_row("cpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp_cpu, 0, T_pass, 0));
         rows.push_back(dsg_row("gpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp, 0, T_pass, 0));
     }
+    if (have_exact) rows.push_back(dsg_row("gpu_core_exact_pruned_seeded", path, gp.n, gp.m, exr.tri, exr.verts, t_exact, 0, exr.flow_calls, 0));
     for (auto& r : rows) puts(r.c_str());
     fflush(stdout);
     fprintf(stderr, "GPU core numbers verified element-wise against CPU Alg 3 (kmax %u, %u rounds)\n", kmax, inc.rounds);

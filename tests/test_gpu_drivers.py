@@ -72,7 +72,7 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     assert rc == 0, err[-2000:]
     rs = {r[0]: r for r in rows(out)}
     assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "cpu_greedy",
-                       "prune_kcore", "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"}
+                       "prune_kcore", "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned", "gpu_core_exact_pruned_seeded"}
     core, kmax = og.core()
     kt, kv = og.core_level_density(core, kmax)
     for name in ("cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused"):
@@ -89,6 +89,11 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     for name in ("cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned"):
         r = rs[name]
         assert (int(r[2]), int(r[3]), int(r[4]), int(r[5]), int(r[10])) == (sub.n, sub.m, gt, gv, 10), r
+    # CoreExact seeded with Greedy++ (the driver checked its set and, on small pruned graphs, Exact's density): exact,
+    # so no lower than Greedy++, at least one flow call
+    r = rs["gpu_core_exact_pruned_seeded"]
+    et, ev = int(r[4]), int(r[5])
+    assert (int(r[2]), int(r[3])) == (sub.n, sub.m) and et * gv >= gt * ev and int(r[10]) >= 1, r
 
 
 @pytest.mark.gpu
@@ -98,7 +103,7 @@ def test_gpu_dsg_driver_no_greedy(graph_file):
     rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "1", "--no-greedy", "--no-greedypp"])
     assert rc == 0, err[-2000:]
     rs = {r[0]: r for r in rows(out)}
-    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "prune_kcore"}
+    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "prune_kcore", "gpu_core_exact_pruned_seeded"}
     core, kmax = og.core()
     kt, kv = og.core_level_density(core, kmax)
     assert int(rs["prune_kcore"][10]) == (kt + kv - 1) // kv

@@ -507,3 +507,31 @@ def test_greedy_fixtures_catch_broken_rules(golden):
         load_differs += _greedy_model(g.n, pairs, 3, use_load=False) != want
     assert ge_differs >= 1 and load_differs >= 1
 
+
+
+def test_exact_fixtures_pinned_by_brute_force():
+    """The CoreExact / Exact fixtures (the reference's own results) on graphs of at most 20 vertices: every recorded best
+    density equals the exhaustive maximum over all vertex subsets, and every recorded set holds the recorded number of
+    triangles. On the larger fixtures the three searches (with and without Pruning1/2, seeded) agree with each other and
+    never fall below the Greedy++ seed."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "exact_vectors.json")
+    with open(path) as f:
+        recs = json.load(f)["graphs"]
+    checked = 0
+    for rec in recs:
+        raw = np.array(rec["raw_pairs"], np.uint32).reshape(-1, 2)
+        og = oracle.OGraph.from_pairs(raw)
+        results = [rec[k] for k in ("core_exact", "core_exact_no_pruning12", "core_exact_seeded", "exact_densest") if k in rec]
+        for res in results:
+            keep = np.zeros(og.n, np.uint8)
+            keep[res["set"]] = 1
+            assert og.participation(keep)[0] == res["tri"] and len(res["set"]) == res["verts"], rec["name"]
+            assert res["tri"] * results[0]["verts"] == results[0]["tri"] * res["verts"], rec["name"]
+        gp = rec["greedy_pp10"]
+        assert results[0]["tri"] * max(gp["verts"], 1) >= gp["tri"] * results[0]["verts"], rec["name"]
+        if og.n <= 20 and og.m:
+            r, c, s = og.arrays()
+            bn, bd = oracle.brute_densest(og.n, np.stack([s, c], 1))
+            assert results[0]["tri"] * bd == bn * results[0]["verts"], rec["name"]
+            checked += 1
+    assert checked >= 25

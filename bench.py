@@ -287,6 +287,19 @@ def run_extras(ctx, g, pairs, cfg, csr, T0):
             raise SystemExit("MISMATCH: GPU greedy_pp differs from the CPU's")
         out["greedy_pp"] = {"passes": passes, "n": sub.n, "m": sub.m, "tri": gt, "verts": gv, "gpu_ms": min(gms), "cpu_greedy_pp_s": t_g}
         log(f"greedy++: {passes} passes on the pruned graph: {gt}/{gv}, GPU {min(gms):.2f} ms, CPU greedy_pp {t_g:.2f} s")
+        # CoreExact on the pruned graph seeded with Greedy++ (gpu_dsg.cu:348-361): triangle passes on the device,
+        # max-flow on the host; wall time. Checked: the set holds the claimed triangles (oracle count) and the density is
+        # no lower than Greedy++'s
+        t0 = time.perf_counter()
+        ex = sub.core_exact(cout, seed=(gt, gv, gs))
+        t_ex = time.perf_counter() - t0
+        keep = np.zeros(sub.n, np.uint8)
+        keep[ex["set"]] = 1
+        if osub.participation(keep)[0] != ex["tri"] or len(ex["set"]) != ex["verts"] or ex["tri"] * gv < gt * ex["verts"]:
+            raise SystemExit("MISMATCH: core_exact's set fails the oracle's check")
+        out["core_exact"] = {"n": sub.n, "m": sub.m, "tri": ex["tri"], "verts": ex["verts"], "flow_calls": ex["flow_calls"],
+                             "wall_s": t_ex, "seed": [gt, gv]}
+        log(f"core_exact (seeded) on the pruned graph: {ex['tri']}/{ex['verts']}, {ex['flow_calls']} flow calls, {t_ex:.2f} s wall")
     sub.close()
     c.close()
     # k-truss: device time of the whole decomposition (support count included); best of 2. No CPU leg here: the

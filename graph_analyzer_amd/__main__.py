@@ -1,6 +1,6 @@
 """Runs one stage of the path on a named synthetic workload and prints its device time and result.
 
-    python -m graph_analyzer_amd count|peel|core|support|truss|ingest|prune|greedy
+    python -m graph_analyzer_amd count|peel|core|support|truss|ingest|prune|greedy|exact
                                  [--workload rmat21|powerlaw_communities|powerlaw_small] [--repeat R] [--num 1 --den 10]
                                  [--passes 10]
     python -m graph_analyzer_amd batch [--graphs 4096] [--repeat R] [--tune]
@@ -30,7 +30,7 @@ WORKLOADS = {"rmat21": gen.rmat21, "powerlaw_communities": gen.powerlaw_communit
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m graph_analyzer_amd")
-    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "greedy", "greedy_many", "batch",
+    ap.add_argument("stage", choices=["count", "peel", "core", "support", "truss", "ingest", "prune", "greedy", "exact", "greedy_many", "batch",
                                       "probe", "serve", "core_batch"])
     ap.add_argument("--workload", default="rmat21", choices=sorted(WORKLOADS))
     ap.add_argument("--repeat", type=int, default=3)
@@ -146,18 +146,25 @@ def main(argv=None):
     T, ms = g.count(ga.TC_HASH)
     print(f"count: T={T} {ms:.3f} ms (first, includes nothing but the count)", flush=True)
     sub = None
-    if args.stage == "greedy":   # Greedy++ on the graph prune_to_core leaves at ceil(kmax-core density)
+    sub_core = None
+    if args.stage in ("greedy", "exact"):   # Greedy++ / CoreExact on the graph prune_to_core leaves at ceil(kmax-core density)
         c = ga.Core(g, True)
         core, _, kmax = c.run(T_full=T)
         Tk, _ = g.vertex_participation((core >= kmax).astype(np.uint8))
         verts = int((core >= kmax).sum())
-        sub, _, _ = c.prune((Tk + verts - 1) // verts if verts else 0)
+        sub, _, sub_core = c.prune((Tk + verts - 1) // verts if verts else 0)
         c.close()
-        print(f"greedy on the pruned graph: n={sub.n} m={sub.m}", flush=True)
+        print(f"{args.stage} on the pruned graph: n={sub.n} m={sub.m}", flush=True)
     for _ in range(args.repeat):
         if args.stage == "greedy":
             t, v, s, _, ms = sub.greedy_pp(args.passes)
             print(f"greedy++ {args.passes} passes: {t}/{v} ({len(s)} vertices) {ms:.3f} ms device", flush=True)
+        elif args.stage == "exact":   # CoreExact seeded with Greedy++ (gpu_dsg.cu:348-361), wall time
+            t, v, s, _, _ = sub.greedy_pp(args.passes)
+            t0 = time.perf_counter()
+            ex = sub.core_exact(sub_core, seed=(t, v, s))
+            print(f"core_exact seeded with {t}/{v}: {ex['tri']}/{ex['verts']}, {ex['flow_calls']} flow calls, "
+                  f"{(time.perf_counter() - t0) * 1e3:.1f} ms wall", flush=True)
         elif args.stage == "count":
             T, ms = g.count(ga.TC_HASH)
             print(f"count: T={T} {ms:.3f} ms", flush=True)

@@ -42,11 +42,14 @@ def main(argv=None):
     ap.add_argument("--seconds", type=float, default=1.0)
     ap.add_argument("--pool", type=int, default=256)
     ap.add_argument("--tune", action="store_true", help="batch: autotune the batched count first (as bench.py does)")
+    ap.add_argument("--slots", type=int, default=4, help="serve: worker streams")
+    ap.add_argument("--core-slots", type=int, default=None, help="serve: workers that may run core work at once (default half of them)")
     args = ap.parse_args(argv)
     if args.stage == "serve":   # one Poisson run of the mixed workload through the continuous-batching server
         from graph_analyzer_amd import serve
         source, reqs = serve.mixed_workload(args.rate, args.seconds, pool=args.pool)
-        s = serve.Server(source, slots=4, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16)
+        s = serve.Server(source, slots=args.slots, count_batch=64, core_batch=1024, peel_batch=1024, states_per_graph=16,
+                         core_slots=args.core_slots)
         t0 = time.perf_counter()
         s.preload(sorted({q.key for q in reqs}))
         print(f"serve: {len(reqs)} requests at {args.rate:g} req/s, graphs resident in {time.perf_counter() - t0:.1f} s", flush=True)

@@ -17,6 +17,10 @@ Shape on one MI355X:
   * stateful requests (peel, core) take one of up to `states_per_graph` state objects of their graph; states are
     created once and moved between worker streams (ga_peel_set_ctx / ga_core_set_ctx), so the steady state allocates
     and frees no device memory;
+  * at most `core_slots` workers (default: half of them) run core work at a time, so counts and peels, which take
+    milliseconds, do not wait behind core batches that take hundreds (same arrivals at 250 / 1,000 / 2,000 req/s,
+    4 workers: p50 21.5 / 708 / 1,152 ms with every worker open to cores, 1.1 / 7.2 / 55 ms with two of them, at
+    191-626 req/s against 203-617);
   * pending core requests (up to `core_batch` of them, on any graphs, each with its own state) are gathered into one
     ga_core_run_batch: every round's three launches cover all of them, so the round latency that bounds one
     decomposition is paid once per batch instead of once per request;
@@ -113,11 +117,16 @@ class _Resident:
 class Server:
     def __init__(self, source: Callable[[object], np.ndarray], slots: int = 4, count_batch: int = 64,
                  states_per_graph: int = 4, budget_bytes: Optional[int] = None, device: int = 0,
-                 core_batch: int = 1024, peel_batch: int = 1024):
+                 core_batch: int = 1024, peel_batch: int = 1024, core_slots: Optional[int] = None):
         if slots < 1 or count_batch < 1 or states_per_graph < 1 or core_batch < 1 or peel_batch < 1:
             raise ValueError("slots, count_batch, core_batch, peel_batch and states_per_graph must be >= 1")
+        if core_slots is None:   # half of the workers (measured best of 1-4 of 4: table in the design notes)
+            core_slots = max(1, slots // 2)
+        if not 1 <= core_slots <= slots:
+            raise ValueError("core_slots must be between 1 and slots")
         self.source = source
         self.slots = slots
+        self.core_slots = core_slots
         self.count_batch = count_batch
         self.core_batch = core_batch
         self.peel_batch = peel_batch
@@ -198,10 +207,13 @@ class Server:
         return self._new_state(r, kind)
 
     # ---- scheduling ----------------------------------------------------------------------------------------
-    def _admit(self, pending: collections.deque, scan: int = 512):
-        """Earliest admissible pending request (and, for a count, every other pending count up to count_batch)."""
+    def _admit(self, pending: collections.deque, scan: int = 512, allow_core: bool = True):
+        """Earliest admissible pending request (and, for a count, every other pending count up to count_batch). Core
+        requests are skipped when allow_core is false (every worker that may run one is busy with one)."""
         # a snapshot: the deque is only changed right before returning, or restored to the same order
         for idx, req in enumerate(list(itertools.islice(pending, scan))):
+            if req.kind == "core" and not allow_core:
+                continue
             r = self.load(req.key)
             if r is None:
                 continue
@@ -310,6 +322,7 @@ class Server:
             t.start()
         pending: collections.deque = collections.deque()
         free = list(range(self.slots - 1, -1, -1))
+        core_busy = 0   # workers running a core task: at most core_slots, so counts and peels never wait behind cores
         out: Dict[int, Result] = {}
         error = None
         t0 = time.perf_counter()
@@ -322,10 +335,12 @@ class Server:
                     i += 1
                 refused = False   # the last admission attempt found nothing it could start
                 while free and pending:
-                    task = self._admit(pending)
+                    task = self._admit(pending, allow_core=core_busy < self.core_slots)
                     if task is None:
                         refused = True
                         break
+                    if task[0] in ("core", "core_batch"):
+                        core_busy += 1
                     todo[free.pop()].put(task)
                 if refused and len(free) == self.slots and i >= len(reqs):
                     # nothing in flight, nothing left to arrive, and the pending requests cannot start
@@ -339,6 +354,8 @@ class Server:
                     w, task, res, err = item
                     free.append(w)
                     kind, group, state = task
+                    if kind in ("core", "core_batch"):
+                        core_busy -= 1
                     for _, r in group:
                         r.inflight -= 1
                         r.last = time.perf_counter()

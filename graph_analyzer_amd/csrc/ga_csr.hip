@@ -310,8 +310,7 @@ static void free_plan(TcPlan& p) {
     ga_free(p.items);
     ga_free(p.wcol);
     ga_free(p.wedge);
-    ga_free(p.wps);
-    ga_free(p.wlen);
+    ga_free(p.wpl);
     ga_free(p.wh);
     ga_free(p.wu);
     ga_free(p.cstart);

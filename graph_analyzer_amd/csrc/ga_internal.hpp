@@ -45,10 +45,10 @@ struct TcPlan {
     uint32_t* items = nullptr;           // 4 words per item {h, begin, rowptr[h], entries | d+(h) << 17}, bucket-major
     uint32_t* wcol = nullptr;            // probe-side vertex per work entry (rows of the hashed vertex, longest first)
     uint32_t* wedge = nullptr;           // oriented edge id of each work entry
-    uint32_t* wps = nullptr;             // rowptr[p] of each work entry (start of the probe list)
-    uint32_t* wlen = nullptr;            // d+(p) of each work entry (length of the probe list)
+    uint2* wpl = nullptr;                // {rowptr[p], d+(p)} of each work entry: start and length of its probe list, one
+                                         // 8-byte load
     uint32_t nwork = 0;                  // work entries (edges with min(d+) > 0)
-    bool entries = false;                // wcol/wedge/wps/wlen/wh hold the sorted work entries (they depend on the CSR only)
+    bool entries = false;                // wcol/wedge/wpl/wh hold the sorted work entries (they depend on the CSR only)
     uint32_t* wh = nullptr;              // hashed vertex h of each work entry
     uint32_t nlight = 0;                 // entries [0, nlight) have d+(h) <= 128 (the flat kernel's class)
     unsigned long long* wu = nullptr;    // flat unit offset of each light entry (nlight + 1, exclusive prefix)

@@ -232,8 +232,7 @@ struct TcArgs {
     const uint32_t* col;
     const uint32_t* wcol;
     const uint32_t* wedge;
-    const uint32_t* wps;     // rowptr[p] of each work entry
-    const uint32_t* wlen;    // d+(p) of each work entry
+    const uint2* wpl;        // {rowptr[p], d+(p)} of each work entry
     const uint8_t* alive;
     uint32_t* tri;
     uint32_t* sup;
@@ -315,8 +314,9 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
         {
             const uint32_t j = b + sub;
             if (j < e) {
-                n_ps = a.wps[j];
-                n_len = a.wlen[j];
+                const uint2 pl = a.wpl[j];
+                n_ps = pl.x;
+                n_len = pl.y;
                 if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[j];
             }
         }
@@ -361,7 +361,8 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
             // every table tried left a key outside its home bucket (rare): one lane per entry, each element looked up
             // with the walk past full buckets
             for (uint32_t j = b + sub; j < e; j += G) {
-                const uint32_t ps = a.wps[j], len = a.wlen[j];
+                const uint2 pl = a.wpl[j];
+                const uint32_t ps = pl.x, len = pl.y;
                 uint32_t c = 0;
                 for (uint32_t k = 0; k < len; k++) c += bucket_find(tab, tbits - 2u, col[ps + k], mul) != GA_EMPTY ? 1u : 0u;
                 if (!(FLAGS & F_GRAPH)) local += c;   // per-graph counts: the graphs' counters hold the total
@@ -393,8 +394,9 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
                     const uint32_t jn = base + G + sub;
                     n_ps = n_len = 0;
                     if (jn < e) {
-                        n_ps = a.wps[jn];
-                        n_len = a.wlen[jn];
+                        const uint2 pl = a.wpl[jn];
+                        n_ps = pl.x;
+                        n_len = pl.y;
                     }
                 }
                 group_sync();
@@ -453,8 +455,9 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
                 const uint32_t jn = base + G + sub;
                 n_p = n_ps = n_len = 0;
                 if (jn < e) {
-                    n_ps = a.wps[jn];
-                    n_len = a.wlen[jn];
+                    const uint2 pl = a.wpl[jn];
+                    n_ps = pl.x;
+                    n_len = pl.y;
                     if (FLAGS & (F_ALIVE | F_TRI)) n_p = a.wcol[jn];
                 }
             }
@@ -626,7 +629,7 @@ __device__ __forceinline__ void tc_tiny_body(const TcArgs& a, uint32_t bid, uint
         if (it < ir.end) {
             const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
             const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK), dh = im.w >> ITEM_DSHIFT;   // 1..8
-            uint32_t nps = a.wps[b], nlen = a.wlen[b];   // every item has an entry; every list has 1..d+(h) elements
+            uint2 npl = a.wpl[b];   // every item has an entry; every list has 1..d+(h) elements
             uint32_t hr[8];
 #pragma unroll
             for (uint32_t k = 0; k < 8; k++) {
@@ -635,11 +638,8 @@ __device__ __forceinline__ void tc_tiny_body(const TcArgs& a, uint32_t bid, uint
             }
             if (FLAGS & F_GRAPH) gr = a.gof[h];
             for (uint32_t j = b; j < e; j++) {
-                const uint32_t ps = nps, len = nlen;
-                if (j + 1u < e) {
-                    nps = a.wps[j + 1u];
-                    nlen = a.wlen[j + 1u];
-                }
+                const uint32_t ps = npl.x, len = npl.y;
+                if (j + 1u < e) npl = a.wpl[j + 1u];
                 uint32_t x[8];
 #pragma unroll
                 for (uint32_t t = 0; t < 8; t++) x[t] = col[ps + min(t, len - 1u)];
@@ -1007,17 +1007,17 @@ __global__ void k_plan_keys(const uint32_t* __restrict__ rowptr, const uint32_t*
 }
 
 // Flat units of each light entry: its row's table (first entry of the row only), one header unit, its probe list.
-__global__ void k_plan_units(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ wlen,
+__global__ void k_plan_units(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint2* __restrict__ wpl,
                              uint32_t nlight, unsigned long long* __restrict__ wt) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
         const uint32_t h = wh[j];
         const bool first = j == 0 || wh[j - 1] != h;
-        wt[j] = (first ? (1ull << flat_bits(rowptr[h + 1] - rowptr[h])) : 0ull) + 1ull + wlen[j];
+        wt[j] = (first ? (1ull << flat_bits(rowptr[h + 1] - rowptr[h])) : 0ull) + 1ull + wpl[j].y;
     }
 }
 
 // Entry records of the flat kernel (see k_tc_flat).
-__global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ wps,
+__global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint2* __restrict__ wpl,
                             const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ hstart, uint32_t nlight,
                             uint4* __restrict__ frec) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
@@ -1025,7 +1025,7 @@ __global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t*
         const uint32_t jf = hstart[h];
         const bool first = jf == j;
         const uint32_t dh = rowptr[h + 1] - rowptr[h];
-        frec[j] = make_uint4((uint32_t)wu[j], first ? rowptr[h] : (uint32_t)(wu[j] - wu[jf]), wps[j],
+        frec[j] = make_uint4((uint32_t)wu[j], first ? rowptr[h] : (uint32_t)(wu[j] - wu[jf]), wpl[j].x,
                              flat_bits(dh) | (dh << 8) | ((first ? 1u : 0u) << 16));
     }
 }
@@ -1044,7 +1044,7 @@ __global__ void k_plan_chunks(const unsigned long long* __restrict__ wu, const u
 
 __global__ void k_plan_fill(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
                             const uint32_t* __restrict__ val, uint32_t nwork, uint32_t* __restrict__ wcol, uint32_t* __restrict__ wedge,
-                            uint32_t* __restrict__ wps, uint32_t* __restrict__ wlen, uint32_t* __restrict__ wh,
+                            uint2* __restrict__ wpl, uint32_t* __restrict__ wh,
                             unsigned long long* __restrict__ wt) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) {
         const uint32_t e = val[j];
@@ -1052,16 +1052,15 @@ __global__ void k_plan_fill(const uint32_t* __restrict__ rowptr, const uint32_t*
         hash_side(rowptr, src[e], col[e], h, p, dmin);
         wcol[j] = p;
         wedge[j] = e;
-        wps[j] = rowptr[p];
-        wlen[j] = dmin;
+        wpl[j] = make_uint2(rowptr[p], dmin);
         wh[j] = h;
         wt[j] = dmin;
     }
 }
 
 // Probe work of each entry, from the kept entries (what k_plan_fill writes into wt).
-__global__ void k_plan_work(const uint32_t* __restrict__ wlen, uint32_t nwork, unsigned long long* __restrict__ wt) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) wt[j] = wlen[j];
+__global__ void k_plan_work(const uint2* __restrict__ wpl, uint32_t nwork, unsigned long long* __restrict__ wt) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) wt[j] = wpl[j].y;
 }
 
 __global__ void k_plan_rowstart(const uint32_t* __restrict__ wh, uint32_t nwork, uint32_t* __restrict__ hstart) {
@@ -1162,7 +1161,7 @@ __global__ void k_plan_items(const uint32_t* __restrict__ flag, const uint32_t* 
 // wavefront otherwise wait, masked off, for the longest of their loops). Group-major order keeps the items in flight
 // on a few graphs, whose lists then stay in cache.
 __global__ void k_plan_item_shape(const uint32_t* __restrict__ items, uint32_t nall, const uint32_t* __restrict__ rowptr,
-                                  const uint32_t* __restrict__ wlen, uint32_t cap, const uint32_t* __restrict__ group, uint32_t gbits,
+                                  const uint2* __restrict__ wpl, uint32_t cap, const uint32_t* __restrict__ group, uint32_t gbits,
                                   uint32_t wshift, unsigned long long* __restrict__ key, uint32_t* __restrict__ val) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) {
         const uint4 im = ((const uint4*)items)[i];
@@ -1171,7 +1170,7 @@ __global__ void k_plan_item_shape(const uint32_t* __restrict__ items, uint32_t n
         const unsigned long long bk = bucket_of(dh, cap);
         const unsigned long long gr = group ? group[h] : wshift ? (h >> wshift) : 0u;   // locality group, or h's id window
         key[i] = (bk << (30 + gbits)) | (gr << 30) | ((unsigned long long)min(e - b, 1023u) << 20) |
-                 ((unsigned long long)min(wlen[b], 1023u) << 10) | min(dh, 1023u);
+                 ((unsigned long long)min(wpl[b].y, 1023u) << 10) | min(dh, 1023u);
         val[i] = i;
     }
 }
@@ -1266,7 +1265,7 @@ static int flat_build_units(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     unsigned long long* wt = (unsigned long long*)wt_b.p;
     k_plan_rowstart<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wh, nlight, hstart);
     GA_TRY(ga_launch_check(ctx, "k_plan_rowstart"));
-    k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wlen, nlight, wt);
+    k_plan_units<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wpl, nlight, wt);
     GA_TRY(ga_launch_check(ctx, "k_plan_units"));
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)wt + nlight, 1, 0));
     GA_TRY(ga_malloc(ctx, (void**)&P.wu, ((size_t)nlight + 1) * 8));
@@ -1276,7 +1275,7 @@ static int flat_build_units(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     P.utot = ctx->hstat[0];
     if (P.utot >= FLAT_UNIT_LIMIT) return GA_OK;   // too many units for 32-bit offsets: lane groups only
     GA_TRY(ga_malloc(ctx, (void**)&P.frec, (size_t)nlight * 16));
-    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wps, P.wu, hstart, nlight, (uint4*)P.frec);
+    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wpl, P.wu, hstart, nlight, (uint4*)P.frec);
     GA_TRY(ga_launch_check(ctx, "k_plan_frec"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GA_OK;
@@ -1313,10 +1312,10 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     if (!keep) {
         ga_free(P.wcol);
         ga_free(P.wedge);
-        ga_free(P.wps);
-        ga_free(P.wlen);
+        ga_free(P.wpl);
         ga_free(P.wh);
-        P.wcol = P.wedge = P.wps = P.wlen = P.wh = nullptr;
+        P.wcol = P.wedge = P.wh = nullptr;
+        P.wpl = nullptr;
         P.nwork = P.nlight = 0;
     }
     ga_free(P.items);
@@ -1357,8 +1356,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     if (!keep) {
         GA_TRY(ga_malloc(ctx, (void**)&P.wcol, std::max<size_t>(nwork, 1) * 4));
         GA_TRY(ga_malloc(ctx, (void**)&P.wedge, std::max<size_t>(nwork, 1) * 4));
-        GA_TRY(ga_malloc(ctx, (void**)&P.wps, std::max<size_t>(nwork, 1) * 4));
-        GA_TRY(ga_malloc(ctx, (void**)&P.wlen, std::max<size_t>(nwork, 1) * 4));
+        GA_TRY(ga_malloc(ctx, (void**)&P.wpl, std::max<size_t>(nwork, 1) * 8));
         GA_TRY(ga_malloc(ctx, (void**)&P.wh, std::max<size_t>(nwork, 1) * 4));
     }
     P.nwork = nwork;
@@ -1382,11 +1380,11 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     uint32_t* fpos = (uint32_t*)fpos_b.p;
     if (!keep) {
         k_plan_fill<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, V.src, V.col, sorted_vals, nwork, P.wcol, P.wedge,
-                                                                         P.wps, P.wlen, wh, wt);
+                                                                         P.wpl, wh, wt);
         GA_TRY(ga_launch_check(ctx, "k_plan_fill"));
         P.entries = true;
     } else {
-        k_plan_work<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(P.wlen, nwork, wt);
+        k_plan_work<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(P.wpl, nwork, wt);
         GA_TRY(ga_launch_check(ctx, "k_plan_work"));
     }
     k_plan_rowstart<<<grid_for(nwork, 256, 8192), 256, 0, ctx->stream>>>(wh, nwork, hstart);
@@ -1439,7 +1437,7 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
         uint32_t gbits = group ? P.group_bits : 0u;
         if (wshift)
             while (gbits < 32 && ((uint64_t)1 << gbits) <= ((uint64_t)n >> wshift)) gbits++;
-        k_plan_item_shape<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, nall, V.rowptr, P.wlen, cap, group, gbits, wshift,
+        k_plan_item_shape<<<grid_for(nall, 256, 8192), 256, 0, ctx->stream>>>(P.items, nall, V.rowptr, P.wpl, cap, group, gbits, wshift,
                                                                               (unsigned long long*)k1.p, (uint32_t*)v1.p);
         GA_TRY(ga_launch_check(ctx, "k_plan_item_shape"));
         unsigned long long* ks = nullptr;
@@ -1789,8 +1787,7 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         a.col = V.col;
         a.wcol = V.P->wcol;
         a.wedge = V.P->wedge;
-        a.wps = V.P->wps;
-        a.wlen = V.P->wlen;
+        a.wpl = V.P->wpl;
         a.alive = alive;
         a.tri = tri;
         a.sup = sup;
@@ -2046,8 +2043,7 @@ static int count_many_shared(ga_ctx* ctx, ga_graph* const* graphs, uint32_t coun
         base.col = V.col;
         base.wcol = V.P->wcol;
         base.wedge = V.P->wedge;
-        base.wps = V.P->wps;
-        base.wlen = V.P->wlen;
+        base.wpl = V.P->wpl;
         base.total = counts + i;
         many_add(ctx, M, graphs[i], V, base, 0);
     }
@@ -2072,8 +2068,7 @@ int ga_tc_participation_many(ga_ctx* ctx, ga_graph* const* graphs, const uint8_t
         base.col = V.col;
         base.wcol = V.P->wcol;
         base.wedge = V.P->wedge;
-        base.wps = V.P->wps;
-        base.wlen = V.P->wlen;
+        base.wpl = V.P->wpl;
         base.alive = alive[i];
         base.tri = tri[i];
         base.total = total[i];
@@ -2143,7 +2138,7 @@ struct BucketStarts {   // first item of each bucket of a plan (by value into k_
 namespace {
 // Work of each bucket's items: entries and probe elements (each entry's list is min(d+u, d+v) long). bstart[b] is the
 // first item of bucket b (NHASH hashed buckets, then the binary-search one; bstart[NHASH + 1] = all items).
-__global__ void k_bucket_work(const uint4* __restrict__ items, const uint32_t* __restrict__ wlen, BucketStarts bs,
+__global__ void k_bucket_work(const uint4* __restrict__ items, const uint2* __restrict__ wpl, BucketStarts bs,
                               unsigned long long* __restrict__ out) {
     for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < bs.s[NHASH + 1]; it += gridDim.x * blockDim.x) {
         uint32_t b = 0;
@@ -2151,7 +2146,7 @@ __global__ void k_bucket_work(const uint4* __restrict__ items, const uint32_t* _
         const uint4 im = items[it];
         const uint32_t ne = im.w & ITEM_NMASK;
         unsigned long long s = 0;
-        for (uint32_t j = im.y; j < im.y + ne; j++) s += wlen[j];
+        for (uint32_t j = im.y; j < im.y + ne; j++) s += wpl[j].y;
         atomicAdd(&out[2 * b], (unsigned long long)ne);
         atomicAdd(&out[2 * b + 1], s);
     }
@@ -2174,7 +2169,7 @@ extern "C" int ga_tc_bucket_profile(ga_ctx* ctx, ga_graph* g, int reps, uint64_t
     unsigned long long* d = (unsigned long long*)ctx->dsmall;   // 64 words: [0, 2 * (NHASH + 1)) work, [32] the count
     GA_TRY(ga_fill_u64(ctx, (uint64_t*)d, 2 * (NHASH + 1), 0));
     if (bs.s[NHASH + 1]) {
-        k_bucket_work<<<grid_for(bs.s[NHASH + 1], 256, 4096), 256, 0, ctx->stream>>>((const uint4*)P.items, P.wlen, bs, d);
+        k_bucket_work<<<grid_for(bs.s[NHASH + 1], 256, 4096), 256, 0, ctx->stream>>>((const uint4*)P.items, P.wpl, bs, d);
         GA_TRY(ga_launch_check(ctx, "k_bucket_work"));
     }
     GA_HIP(ctx, hipMemcpyAsync(ctx->hstat, d, 2 * (NHASH + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -2195,8 +2190,7 @@ extern "C" int ga_tc_bucket_profile(ga_ctx* ctx, ga_graph* g, int reps, uint64_t
     a.col = V.col;
     a.wcol = P.wcol;
     a.wedge = P.wedge;
-    a.wps = P.wps;
-    a.wlen = P.wlen;
+    a.wpl = P.wpl;
     a.total = d + 32;
     const bool flat = flat_on(V);
     for (int r = 0; r <= reps && rc == GA_OK; r++) {   // r = 0: untimed warm-up

@@ -2140,15 +2140,26 @@ namespace {
 // first item of bucket b (NHASH hashed buckets, then the binary-search one; bstart[NHASH + 1] = all items).
 __global__ void k_bucket_work(const uint4* __restrict__ items, const uint2* __restrict__ wpl, BucketStarts bs,
                               unsigned long long* __restrict__ out) {
+    uint64_t ent[NHASH + 1] = {}, el[NHASH + 1] = {};   // this lane's sums per bucket, reduced per wavefront at the end
     for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < bs.s[NHASH + 1]; it += gridDim.x * blockDim.x) {
-        uint32_t b = 0;
-        while (b < NHASH && it >= bs.s[b + 1]) b++;
         const uint4 im = items[it];
         const uint32_t ne = im.w & ITEM_NMASK;
-        unsigned long long s = 0;
+        uint64_t s = 0;
         for (uint32_t j = im.y; j < im.y + ne; j++) s += wpl[j].y;
-        atomicAdd(&out[2 * b], (unsigned long long)ne);
-        atomicAdd(&out[2 * b + 1], s);
+#pragma unroll
+        for (uint32_t b = 0; b <= NHASH; b++)
+            if (it >= bs.s[b] && it < bs.s[b + 1]) {
+                ent[b] += ne;
+                el[b] += s;
+            }
+    }
+#pragma unroll
+    for (uint32_t b = 0; b <= NHASH; b++) {
+        const uint64_t e = wave_sum_u64(ent[b]), l = wave_sum_u64(el[b]);
+        if (lane_id() == 0 && (e || l)) {
+            atomicAdd(&out[2 * b], (unsigned long long)e);
+            atomicAdd(&out[2 * b + 1], (unsigned long long)l);
+        }
     }
 }
 }  // namespace

@@ -13,14 +13,18 @@
 //   than ga_greedy_max_vertices (said on stderr). Then CoreExact on the pruned graph seeded with the Greedy++ result
 //   (gpu_dsg.cu:348-361): gpu_core_exact_pruned_seeded (core_exact through ga_core_exact), rounds = flow calls. Checked
 //   against the oracle: the set's triangle count, a density no lower than Greedy++'s, and, on pruned graphs of at most
-//   2,000 vertices, the same density from Exact (Alg 1) on the whole pruned graph. --no-exact skips it.
+//   2,000 vertices, the same density from Exact (Alg 1) on the whole pruned graph; as gpu_dsg.cu:362-363 the optimum
+//   may not lie below the lower bound nor below Greedy++'s. Then the pipeline rows (gpu_dsg.cu:365-369):
+//   pipeline_gpu_core (fused GPU core + Greedy + prune + Greedy++ + CoreExact) and pipeline_cpu_core (CPU Alg 3 + the
+//   same remainder). --no-exact skips all of that. --verify-full (gpu_dsg.cu:372-385): core_exact on the whole graph
+//   with the CPU core numbers, gpu_core_exact_full, whose optimum must equal the pruned pipeline's.
 // Integration test: links the CPU oracle as the checker and as the CPU row.
 #include "../../graph_analyzer_amd/host/dsg.h"
 #include "../../oracle/oracle.hpp"
 
 int main(int argc, char** argv) {
     std::vector<const char*> pos;
-    bool recount = false, burnin = true, verbose = false, gpp = true, greedy_on = true, exact_on = true;
+    bool recount = false, burnin = true, verbose = false, gpp = true, greedy_on = true, exact_on = true, verify_full = false;
     int repeats = 5;
     uint32_t T_pass = 10;
     for (int i = 1; i < argc; i++) {
@@ -30,13 +34,14 @@ int main(int argc, char** argv) {
         else if (strcmp(argv[i], "--no-greedypp") == 0) gpp = false;
         else if (strcmp(argv[i], "--no-greedy") == 0) greedy_on = false;
         else if (strcmp(argv[i], "--no-exact") == 0) exact_on = false;
+        else if (strcmp(argv[i], "--verify-full") == 0) verify_full = true;
         else if (strcmp(argv[i], "--T") == 0 && i + 1 < argc) T_pass = (uint32_t)std::max(1, atoi(argv[++i]));
         else if (strcmp(argv[i], "--repeats") == 0 && i + 1 < argc) repeats = std::max(1, atoi(argv[++i]));
         else if (argv[i][0] == '-') {}
         else pos.push_back(argv[i]);
     }
     if (pos.size() != 1) {
-        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedy] [--no-greedypp] [--no-exact] [--verbose]\n",
+        fprintf(stderr, "usage: %s graph.txt [--T 10] [--repeats N] [--with-recount] [--no-burnin] [--no-greedy] [--no-greedypp] [--no-exact] [--verify-full] [--verbose]\n",
                 argv[0]);
         return 2;
     }
@@ -213,7 +218,8 @@ int main(int argc, char** argv) {
             for (uint32_t v : exr.set) in[v] = 1;
             std::vector<uint32_t> otri;
             const uint64_t tin = orc::participation(op, in, otri);   // triangles inside the set
-            bool good = exr.set.size() == exr.verts && tin == exr.tri && (!have_gpp || !dens_gt(Dens{gpr.tri, gpr.verts}, exr.dens()));
+            bool good = exr.set.size() == exr.verts && tin == exr.tri && (!have_gpp || !dens_gt(Dens{gpr.tri, gpr.verts}, exr.dens())) &&
+                        dens_cmp(exr.dens(), lb) >= 0;   // gpu_dsg.cu:362-363: not below the lower bound, nor below Greedy++
             if (good && gp.n <= 2000) {
                 const DsgResult ed = exact_densest(gp);
                 good = ed.ok && dens_cmp(ed.dens(), exr.dens()) == 0;
@@ -228,6 +234,23 @@ int main(int argc, char** argv) {
             }
         }
     }
+    // ---- --verify-full: CoreExact on the whole graph with the CPU core numbers (gpu_dsg.cu:372-385) ----
+    DsgResult exf;
+    double t_full = 0;
+    if (ok && have_exact && verify_full) {
+        const double tf = now_ms();
+        exf = core_exact_device(dg, core_cpu);
+        t_full = now_ms() - tf;
+        if (!exf.ok) { fprintf(stderr, "core_exact (full) failed: %s\n", exf.error.c_str()); ok = false; }
+        else if (dens_cmp(exf.dens(), exr.dens()) != 0) {
+            fprintf(stderr, "MISMATCH: full-graph CoreExact %llu/%llu vs pruned pipeline %llu/%llu\n", (unsigned long long)exf.tri,
+                    (unsigned long long)exf.verts, (unsigned long long)exr.tri, (unsigned long long)exr.verts);
+            ok = false;
+        } else {
+            fprintf(stderr, "verify-full: optimum identical (%llu/%llu) with and without pruning\n", (unsigned long long)exf.tri,
+                    (unsigned long long)exf.verts);
+        }
+    }
     if (!ok) { fprintf(stderr, "VERIFICATION FAILED - no timing rows printed\n"); return 1; }
     rows.push_back(dsg_row("gpu_core_decomp", path, og.n, og.m, ktri, kverts, best_inc, 0, inc.rounds, kmax));
     rows.push_back(dsg_row("gpu_core_decomp_fused", path, og.n, og.m, ktri, kverts, best_fused, 0, fused_rounds, kmax));
@@ -238,7 +261,14 @@ int main(int argc, char** argv) {
         rows.push_back(dsg_row("cpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp_cpu, 0, T_pass, 0));
         rows.push_back(dsg_row("gpu_greedy_pp_pruned", path, gp.n, gp.m, gpr.tri, gpr.verts, t_gpp, 0, T_pass, 0));
     }
-    if (have_exact) rows.push_back(dsg_row("gpu_core_exact_pruned_seeded", path, gp.n, gp.m, exr.tri, exr.verts, t_exact, 0, exr.flow_calls, 0));
+    if (have_exact) {
+        rows.push_back(dsg_row("gpu_core_exact_pruned_seeded", path, gp.n, gp.m, exr.tri, exr.verts, t_exact, 0, exr.flow_calls, 0));
+        const double rest = (greedy_on ? t_greedy : 0.0) + t_prune + (have_gpp ? t_gpp : 0.0) + t_exact;
+        rows.push_back(dsg_row("pipeline_gpu_core", path, og.n, og.m, exr.tri, exr.verts, best_fused + rest, 0, fused_rounds, kmax));
+        rows.push_back(dsg_row("pipeline_cpu_core", path, og.n, og.m, exr.tri, exr.verts, t_cpu + rest, 0, 0, kmax));
+    }
+    if (ok && have_exact && verify_full)
+        rows.push_back(dsg_row("gpu_core_exact_full", path, og.n, og.m, exf.tri, exf.verts, t_full, 0, exf.flow_calls, 0));
     for (auto& r : rows) puts(r.c_str());
     fflush(stdout);
     fprintf(stderr, "GPU core numbers verified element-wise against CPU Alg 3 (kmax %u, %u rounds)\n", kmax, inc.rounds);

@@ -68,11 +68,12 @@ def test_gpu_densest_driver(graph_file):
 @pytest.mark.gpu
 def test_gpu_dsg_driver_with_prune(graph_file):
     path, og = graph_file
-    rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "2", "--with-recount"])
+    rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "2", "--with-recount", "--verify-full"])
     assert rc == 0, err[-2000:]
     rs = {r[0]: r for r in rows(out)}
     assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "gpu_core_decomp_recount", "cpu_greedy",
-                       "prune_kcore", "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned", "gpu_core_exact_pruned_seeded"}
+                       "prune_kcore", "cpu_greedy_pp_pruned", "gpu_greedy_pp_pruned", "gpu_core_exact_pruned_seeded",
+                       "pipeline_gpu_core", "pipeline_cpu_core", "gpu_core_exact_full"}
     core, kmax = og.core()
     kt, kv = og.core_level_density(core, kmax)
     for name in ("cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused"):
@@ -94,6 +95,11 @@ def test_gpu_dsg_driver_with_prune(graph_file):
     r = rs["gpu_core_exact_pruned_seeded"]
     et, ev = int(r[4]), int(r[5])
     assert (int(r[2]), int(r[3])) == (sub.n, sub.m) and et * gv >= gt * ev and int(r[10]) >= 1, r
+    # the pipeline rows carry the optimum on the whole graph's n, m; --verify-full finds the same optimum unpruned
+    for name in ("pipeline_gpu_core", "pipeline_cpu_core", "gpu_core_exact_full"):
+        p = rs[name]
+        assert (int(p[2]), int(p[3])) == (og.n, og.m) and int(p[4]) * ev == et * int(p[5]), p
+    assert int(rs["pipeline_gpu_core"][10]) == int(rs["gpu_core_decomp_fused"][10])
 
 
 @pytest.mark.gpu
@@ -103,7 +109,8 @@ def test_gpu_dsg_driver_no_greedy(graph_file):
     rc, out, err = run(["gpu_dsg", path, "--no-burnin", "--repeats", "1", "--no-greedy", "--no-greedypp"])
     assert rc == 0, err[-2000:]
     rs = {r[0]: r for r in rows(out)}
-    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "prune_kcore", "gpu_core_exact_pruned_seeded"}
+    assert set(rs) == {"cpu_core_decomp", "gpu_core_decomp", "gpu_core_decomp_fused", "prune_kcore", "gpu_core_exact_pruned_seeded",
+                       "pipeline_gpu_core", "pipeline_cpu_core"}
     core, kmax = og.core()
     kt, kv = og.core_level_density(core, kmax)
     assert int(rs["prune_kcore"][10]) == (kt + kv - 1) // kv

@@ -333,6 +333,8 @@ extern "C" void ga_graph_destroy(ga_graph* g) {
     ga_free(g->crow);
     ga_free(g->ccol);
     ga_free(g->csrc);
+    ga_free(g->pst);
+    ga_free(g->pcol);
     free_plan(g->plan);
     free_plan(g->cplan);
     delete g;

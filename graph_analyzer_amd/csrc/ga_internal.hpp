@@ -46,7 +46,8 @@ struct TcPlan {
     uint32_t* wcol = nullptr;            // probe-side vertex per work entry (rows of the hashed vertex, longest first)
     uint32_t* wedge = nullptr;           // oriented edge id of each work entry
     uint2* wpl = nullptr;                // {rowptr[p], d+(p)} of each work entry: start and length of its probe list, one
-                                         // 8-byte load
+                                         // 8-byte load (rowptr: the row starts of the column array the counts read)
+    const uint32_t* st_used = nullptr;   // the row starts that items[].z and wpl[].x hold
     uint32_t nwork = 0;                  // work entries (edges with min(d+) > 0)
     bool entries = false;                // wcol/wedge/wpl/wh hold the sorted work entries (they depend on the CSR only)
     uint32_t* wh = nullptr;              // hashed vertex h of each work entry
@@ -73,6 +74,10 @@ struct ga_graph {
     // tune.relabel = 1: plain and per-graph counts use a renumbered copy of the oriented CSR (same orientation, new
     // ids grouping vertices that share triangles) and its own plan; built on demand
     uint32_t *crow = nullptr, *ccol = nullptr, *csrc = nullptr;
+    // tune.relabel = 2: the same rows placed so that no row of at most 32 ids crosses a 128-byte line (longer rows
+    // start on a line when that saves one): row v at pcol[pst[v] .. pst[v] + d+(v)); built with the copy when its
+    // padded length fits in 32 bits and in memory (else relabel 2 reads the unpadded copy)
+    uint32_t *pst = nullptr, *pcol = nullptr;
     TcPlan cplan;
 };
 

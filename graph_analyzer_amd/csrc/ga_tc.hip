@@ -308,7 +308,7 @@ __device__ __forceinline__ void tc_hash_body(const TcArgs& a, uint32_t bid, uint
         const uint4 im = ((const uint4*)a.items)[it];   // {h, first entry, rowptr[h], entries | d+(h) << 17}
         const uint32_t h = im.x, b = im.y, hs = im.z, e = im.y + (im.w & ITEM_NMASK);
         if ((FLAGS & F_ALIVE) && !alive[h]) continue;
-        const uint32_t dh = HASHED ? im.w >> ITEM_DSHIFT : rowptr[h + 1] - hs;
+        const uint32_t dh = HASHED ? im.w >> ITEM_DSHIFT : rowptr[h + 1] - rowptr[h];
         // batch metadata of the first batch, issued before the table is built (later batches: one batch ahead)
         uint32_t n_p = 0, n_ps = 0, n_len = 0;
         {
@@ -1017,27 +1017,27 @@ __global__ void k_plan_units(const uint32_t* __restrict__ rowptr, const uint32_t
 }
 
 // Entry records of the flat kernel (see k_tc_flat).
-__global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ wh, const uint2* __restrict__ wpl,
-                            const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ hstart, uint32_t nlight,
-                            uint4* __restrict__ frec) {
+__global__ void k_plan_frec(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ st, const uint32_t* __restrict__ wh,
+                            const uint2* __restrict__ wpl, const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ hstart,
+                            uint32_t nlight, uint4* __restrict__ frec) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
         const uint32_t h = wh[j];
         const uint32_t jf = hstart[h];
         const bool first = jf == j;
         const uint32_t dh = rowptr[h + 1] - rowptr[h];
-        frec[j] = make_uint4((uint32_t)wu[j], first ? rowptr[h] : (uint32_t)(wu[j] - wu[jf]), wpl[j].x,
+        frec[j] = make_uint4((uint32_t)wu[j], first ? st[h] : (uint32_t)(wu[j] - wu[jf]), wpl[j].x,
                              flat_bits(dh) | (dh << 8) | ((first ? 1u : 0u) << 16));
     }
 }
 
-// cstart[c] = the light entry whose units contain unit c * chunk; chs[c] = rowptr[h] of that entry's row.
-__global__ void k_plan_chunks(const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ rowptr,
+// cstart[c] = the light entry whose units contain unit c * chunk; chs[c] = the row start of that entry's row.
+__global__ void k_plan_chunks(const unsigned long long* __restrict__ wu, const uint32_t* __restrict__ wh, const uint32_t* __restrict__ st,
                               uint32_t nlight, uint32_t chunk, uint32_t* __restrict__ cstart, uint32_t* __restrict__ chs) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nlight; j += gridDim.x * blockDim.x) {
         const unsigned long long a = wu[j], b = wu[j + 1];
         for (unsigned long long c = (a + chunk - 1) / chunk; c * chunk < b; c++) {
             cstart[c] = j;
-            chs[c] = rowptr[wh[j]];
+            chs[c] = st[wh[j]];
         }
     }
 }
@@ -1187,6 +1187,14 @@ __global__ void k_plan_rank(const uint32_t* __restrict__ flag, const uint32_t* _
         if (flag[j]) starts[fpos[j]] = j;
 }
 
+// Row starts of the column array the counts read (the line-aligned copy, or back to the plain one): items[].z and
+// wpl[].x from st, by the item's h and the entry's probe-side vertex.
+__global__ void k_plan_starts(const uint32_t* __restrict__ st, const uint32_t* __restrict__ wcol, uint32_t nwork, uint32_t nall,
+                              uint32_t* __restrict__ items, uint2* __restrict__ wpl) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nwork; j += gridDim.x * blockDim.x) wpl[j].x = st[wcol[j]];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nall; i += gridDim.x * blockDim.x) items[4 * (size_t)i + 2] = st[items[4 * (size_t)i]];
+}
+
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { ga_free(p); }
@@ -1226,7 +1234,7 @@ bool tuning_valid(const ga_tc_tuning& t) {
             return false;
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
-           t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && (t.relabel == 0 || t.relabel == 1) && (t.probe == 0 || t.probe == 1) &&
+           t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && t.relabel >= 0 && t.relabel <= 2 && (t.probe == 0 || t.probe == 1) &&
            (t.waves == 0 || (t.waves >= 4 && t.waves <= 8)) &&
            (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
                                   (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
@@ -1243,15 +1251,19 @@ PlanShape plan_shape(const ga_tc_tuning& t) {
 }  // namespace
 
 // The CSR and the plan one launch reads: the graph's own (participation, support, alive masks, and counts without
-// relabel) or the renumbered copy of the counts.
+// relabel) or the renumbered copy of the counts. The plan is built from rowptr/col/src; the count kernels read kcol,
+// whose rows start at st (the same arrays, or the line-aligned copy of relabel 2).
 struct CountView {
     TcPlan* P;
     const uint32_t *rowptr, *col, *src;
+    const uint32_t *kcol, *st;
 };
-static CountView own_view(ga_graph* g) { return CountView{&g->plan, g->rowptr, g->col, g->src}; }
+static CountView own_view(ga_graph* g) { return CountView{&g->plan, g->rowptr, g->col, g->src, g->col, g->rowptr}; }
 static bool relabel_on(const ga_graph* g) { return g->plan.tune.block && g->plan.tune.relabel; }
 static CountView count_view(ga_graph* g) {
-    return relabel_on(g) ? CountView{&g->cplan, g->crow, g->ccol, g->csrc} : own_view(g);
+    if (!relabel_on(g)) return own_view(g);
+    const bool pad = g->plan.tune.relabel == 2 && g->pst;
+    return CountView{&g->cplan, g->crow, g->ccol, g->csrc, pad ? g->pcol : g->ccol, pad ? g->pst : g->crow};
 }
 
 // Flat units and entry records of the light entries (built the first time the flat kernel is switched on).
@@ -1275,7 +1287,7 @@ static int flat_build_units(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     P.utot = ctx->hstat[0];
     if (P.utot >= FLAT_UNIT_LIMIT) return GA_OK;   // too many units for 32-bit offsets: lane groups only
     GA_TRY(ga_malloc(ctx, (void**)&P.frec, (size_t)nlight * 16));
-    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, P.wh, P.wpl, P.wu, hstart, nlight, (uint4*)P.frec);
+    k_plan_frec<<<grid_for(nlight, 256, 8192), 256, 0, ctx->stream>>>(V.rowptr, V.st, P.wh, P.wpl, P.wu, hstart, nlight, (uint4*)P.frec);
     GA_TRY(ga_launch_check(ctx, "k_plan_frec"));
     GA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return GA_OK;
@@ -1294,7 +1306,7 @@ static int flat_rechunk(ga_ctx* ctx, ga_graph* g, const CountView& V) {
     const uint32_t chunk = (uint32_t)P.tune.flat_chunk;
     const unsigned long long nch = (P.utot + chunk - 1) / chunk;
     GA_TRY(ga_malloc(ctx, (void**)&P.cstart, (nch + 1) * 8));   // cstart (nch + 1) then chs (nch)
-    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.wh, V.rowptr, P.nlight, chunk, P.cstart,
+    k_plan_chunks<<<grid_for(P.nlight, 256, 8192), 256, 0, ctx->stream>>>(P.wu, P.wh, V.st, P.nlight, chunk, P.cstart,
                                                                           P.cstart + nch + 1);
     GA_TRY(ga_launch_check(ctx, "k_plan_chunks"));
     GA_TRY(ga_fill_u32(ctx, P.cstart + nch, 1, P.nlight));
@@ -1451,6 +1463,11 @@ static int build_plan(ga_ctx* ctx, ga_graph* g, const CountView& V) {
         P.items = (uint32_t*)it2.p;
         it2.p = row_order;
     }
+    if (V.st != V.rowptr || (P.st_used && P.st_used != V.rowptr)) {   // starts in the line-aligned copy (or back)
+        if (nwork || nall) k_plan_starts<<<grid_for(std::max(nwork, nall), 256, 8192), 256, 0, ctx->stream>>>(V.st, P.wcol, nwork, nall, P.items, P.wpl);
+        GA_TRY(ga_launch_check(ctx, "k_plan_starts"));
+    }
+    P.st_used = V.st;
     for (int b = 0; b < NHASH; b++) P.nitems[b] = cnt[b];
     P.nhuge = cnt[NHASH];
     P.ready = true;
@@ -1469,7 +1486,7 @@ int ga_tc_ensure_plan(ga_ctx* ctx, ga_graph* g, bool counts_only) {
         P.group = g->plan.group;
         P.group_bits = g->plan.group_bits;
     }
-    if (P.ready && same_plan_params(P.built, P.tune))
+    if (P.ready && same_plan_params(P.built, P.tune) && P.st_used == V.st)
         return P.chunk_used == (uint32_t)P.tune.flat_chunk ? GA_OK : flat_rechunk(ctx, g, V);
     return build_plan(ctx, g, V);
 }
@@ -1567,7 +1584,7 @@ static int launch_flat(ga_ctx* ctx, ga_graph* g, const CountView& V, unsigned lo
     const TcPlan& P = *V.P;
     FlatArgs f{};
     f.rowptr = V.rowptr;
-    f.col = V.col;
+    f.col = V.kcol;
     f.wh = P.wh;
     f.frec = (const uint4*)P.frec;
     f.cstart = P.cstart;
@@ -1707,6 +1724,33 @@ __global__ void k_relabel_unpack(const unsigned long long* __restrict__ key, uin
     }
 }
 
+// the line-aligned copy: edge e of row v goes to pst[v] + (e - rowptr[v])
+__global__ void k_relabel_pad(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ src, const uint32_t* __restrict__ col,
+                              const uint32_t* __restrict__ pst, uint32_t m, uint32_t* __restrict__ pcol) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+        const uint32_t v = src[e];
+        pcol[pst[v] + (e - rowptr[v])] = col[e];
+    }
+}
+
+// Row starts of the line-aligned copy (host, once per graph): rows in id order, a row moved to the next 128-byte line
+// (32 ids) when that makes it cross one line fewer, so rows of at most 32 ids cross none. Returns the padded length,
+// or 0 when it does not fit in 32 bits.
+uint64_t padded_starts(const std::vector<uint32_t>& rowptr, uint32_t n, std::vector<uint32_t>& pst) {
+    pst.assign((size_t)n + 1, 0);
+    uint64_t off = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        const uint64_t d = rowptr[v + 1] - rowptr[v];
+        const uint64_t r = off & 31u;
+        if (d && r && (r + d + 31u) / 32u > (d + 31u) / 32u) off += 32u - r;
+        if (off + d > 0xFFFFFFFFull) return 0;
+        pst[v] = (uint32_t)off;
+        off += d;
+    }
+    pst[n] = (uint32_t)off;
+    return off;
+}
+
 }  // namespace
 
 template <int FLAGS>
@@ -1769,6 +1813,31 @@ static int ensure_relabel(ga_ctx* ctx, ga_graph* g) {
     g->crow = crow;
     g->ccol = ccol;
     g->csrc = csrc;
+    // the line-aligned copy of relabel 2 (left out when it does not fit: relabel 2 then reads the plain copy)
+    std::vector<uint32_t> hrow((size_t)n + 1), hpst;
+    if (hipMemcpy(hrow.data(), crow, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return ga_fail(ctx, GA_ERR_HIP, "relabel: row offsets");
+    const uint64_t plen = padded_starts(hrow, n, hpst);
+    uint32_t *pst = nullptr, *pcol = nullptr;
+    if (plen && !ga_malloc(ctx, (void**)&pst, ((size_t)n + 1) * 4) && !ga_malloc(ctx, (void**)&pcol, std::max<size_t>(plen, 1) * 4)) {
+        int rc2 = hipMemcpy(pst, hpst.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ? ga_fail(ctx, GA_ERR_HIP, "relabel: padded starts")
+                                                                                                       : ga_fill_u32(ctx, pcol, plen, GA_EMPTY);
+        if (!rc2 && m) {
+            k_relabel_pad<<<grid_for(m, 256, 8192), 256, 0, ctx->stream>>>(crow, csrc, ccol, pst, m, pcol);
+            rc2 = ga_launch_check(ctx, "k_relabel_pad");
+        }
+        if (!rc2 && hipStreamSynchronize(ctx->stream) != hipSuccess) rc2 = ga_fail(ctx, GA_ERR_HIP, "relabel: padded copy");
+        if (rc2) {
+            ga_free(pst);
+            ga_free(pcol);
+            return rc2;
+        }
+        g->pst = pst;
+        g->pcol = pcol;
+    } else {
+        ga_free(pst);   // (an allocation failure here leaves only the message; relabel 2 reads the plain copy)
+        ga_free(pcol);
+    }
     return GA_OK;
 }
 
@@ -1784,7 +1853,7 @@ int ga_tc_launch(ga_ctx* ctx, ga_graph* g, int variant, unsigned long long* d_to
         const CountView V = counts ? count_view(g) : own_view(g);
         TcArgs a{};
         a.rowptr = V.rowptr;
-        a.col = V.col;
+        a.col = V.kcol;
         a.wcol = V.P->wcol;
         a.wedge = V.P->wedge;
         a.wpl = V.P->wpl;
@@ -2040,7 +2109,7 @@ static int count_many_shared(ga_ctx* ctx, ga_graph* const* graphs, uint32_t coun
         }
         TcArgs base{};
         base.rowptr = V.rowptr;
-        base.col = V.col;
+        base.col = V.kcol;
         base.wcol = V.P->wcol;
         base.wedge = V.P->wedge;
         base.wpl = V.P->wpl;
@@ -2065,7 +2134,7 @@ int ga_tc_participation_many(ga_ctx* ctx, ga_graph* const* graphs, const uint8_t
         const CountView V = own_view(graphs[i]);
         TcArgs base{};
         base.rowptr = V.rowptr;
-        base.col = V.col;
+        base.col = V.kcol;
         base.wcol = V.P->wcol;
         base.wedge = V.P->wedge;
         base.wpl = V.P->wpl;
@@ -2198,7 +2267,7 @@ extern "C" int ga_tc_bucket_profile(ga_ctx* ctx, ga_graph* g, int reps, uint64_t
     uint64_t T0 = 0;
     TcArgs a{};
     a.rowptr = V.rowptr;
-    a.col = V.col;
+    a.col = V.kcol;
     a.wcol = P.wcol;
     a.wedge = P.wedge;
     a.wpl = P.wpl;
@@ -2320,11 +2389,12 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
     };
     // search: relabel, item order, block size and occupancy cap jointly, then one at a time the probe form, lanes per
     // bucket, block and cap again, load shift, batches and work chunk.
-    // The renumbered copy and its plan need about 60 bytes per edge while they are built (sorts included): it is tried
-    // only when that fits in half of the free device memory (a 2.6B-edge batch union does not).
+    // The renumbered copy and its plan need about 60 bytes per edge while they are built (sorts included), and its
+    // line-aligned copy about 6 more: it is tried only when that fits in half of the free device memory (a 2.6B-edge
+    // batch union does not).
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    const bool relabel_fits = g->crow != nullptr || 60ull * g->m + 16ull * g->n < free_b / 2;
+    const bool relabel_fits = g->crow != nullptr || 66ull * g->m + 20ull * g->n < free_b / 2;
     GA_TRY(trial(base));
     // The lane widths that win at the starting block size and load factor need not win at the block size and load
     // factor found after them (from block 256 and load shift 2, 8-lane groups for d+(h) <= 128 led and were kept at block
@@ -2341,7 +2411,7 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         const int blocks[3] = {128, 256, 512};
         const int wv[4] = {0, 7, 6, 5};   // waves per SIMD (0: no cap)
         auto order_block_cap = [&](bool relabel_too) -> int {
-            for (int rl = 0; rl <= (relabel_too && relabel_fits ? 1 : 0); rl++)
+            for (int rl = 0; rl <= (relabel_too && relabel_fits ? 2 : 0); rl++)
                 for (int io = 0; io <= 2; io++)
                     for (int bi = 0; bi < 3; bi++)
                         for (int wi = 0; wi < 4; wi++) {

@@ -185,8 +185,10 @@ typedef struct ga_tc_tuning {
     int32_t relabel;       /* 1: plain and per-graph counts run on a private copy of the oriented CSR whose vertices
                               are renumbered so that vertices joined by edges in two or more triangles sit together
                               (each vertex is keyed by the smallest id among itself and such neighbours); the
-                              orientation, and so every count, is unchanged. 0: on the CSR itself. Participation,
-                              support and the per-vertex states always use the CSR's own ids */
+                              orientation, and so every count, is unchanged. 2: as 1, with the copy's rows placed so
+                              that no row of at most 32 ids crosses a 128-byte line (about a fifth more column
+                              memory). 0: on the CSR itself. Participation, support and the per-vertex states always
+                              use the CSR's own ids */
     int32_t probe;         /* plain and per-graph counts, lane-group buckets: 1 = every probe list of a batch gets
                               ceil(length / 8) lanes and all of them are read in the same round; 0 = the lists are read
                               by length class (lists of length in (S/2, S] together, one round per class). Participation

@@ -59,7 +59,7 @@ def test_tiny_grids_count_shapes(ctx, cap):
     ctx.set_grid_cap(cap)
     try:
         for lanes, flat, hcap, order, relabel in (([1, 8, 32, 64, 64], (0, 64), 4096, 1, 0), ([4, 16, 16, 32, 64], (2048, 256), 4096, 3, 1),
-                                                  ([1, 4, 8, 16, 32], (512, 64), 24, 2, 0), ([8, 32, 64, 64, 64], (0, 64), 3, 5, 1)):
+                                                  ([1, 4, 8, 16, 32], (512, 64), 24, 2, 0), ([8, 32, 64, 64, 64], (0, 64), 3, 5, 2)):
             t = ga.TcTuning()
             t.lanes[:] = lanes
             t.block, t.work_chunk, t.load_shift, t.batches = 256, 4096, 2, 8

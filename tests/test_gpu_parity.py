@@ -192,7 +192,7 @@ def test_tuning_shapes(ctx):
                 t.block, t.work_chunk, t.load_shift, t.batches = block, chunk, shift, batches
                 t.flat_chunk, t.flat_block = flats[(li + block) % len(flats)]
                 t.item_order = (li + shift) % 3
-                t.relabel = (li + block // 64 + shift) % 2   # the renumbered copy for plain counts
+                t.relabel = (li + block // 64 + shift) % 3   # the renumbered copy (2: line-aligned rows) for plain counts
                 t.probe = (li + shift + block // 256) % 2   # lane slots or length classes for plain counts
                 t.waves = (0, 5, 6, 7, 8, 4)[(li + block // 64 + shift) % 6]   # occupancy caps through LDS
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
@@ -202,7 +202,7 @@ def test_tuning_shapes(ctx):
         for order in (0, 1, 2):
             t = g.get_tuning()
             t.item_order = order
-            t.relabel = order % 2   # participation and support stay on the CSR's own ids whatever relabel says
+            t.relabel = order   # participation and support stay on the CSR's own ids whatever relabel says
             g.set_tuning(t)
             assert np.array_equal(g.vertex_participation()[1], tri_ref), (lanes, order)
             assert np.array_equal(g.edge_support()[1], sup_ref), (lanes, order)
@@ -277,7 +277,7 @@ def test_bucket_profile_work(ctx):
     r, c, s = g.download()
     d = np.diff(r.astype(np.int64))
     mins = np.minimum(d[s], d[c])
-    for flat, relabel, order in ((0, 0, 1), (2048, 0, 0), (0, 1, 2), (1024, 1, 1)):
+    for flat, relabel, order in ((0, 0, 1), (2048, 0, 0), (0, 1, 2), (1024, 1, 1), (0, 2, 1), (2048, 2, 2)):
         t = g.get_tuning()
         t.flat_chunk, t.flat_block, t.relabel, t.item_order = flat, 256, relabel, order
         g.set_tuning(t)
@@ -290,9 +290,10 @@ def test_bucket_profile_work(ctx):
 
 
 def test_relabel_counts_on_every_graph_kind(ctx):
-    """The renumbered copy (relabel = 1) with the windowed, XCD-contiguous item order counts every graph kind exactly,
-    including graphs with isolated vertices, a star, cliques sharing vertices and triangle-free graphs; the batch union
-    keeps each graph's count."""
+    """The renumbered copy (relabel = 1, and its line-aligned rows, relabel = 2) with the windowed, XCD-contiguous item
+    order counts every graph kind exactly, including graphs with isolated vertices, a star, cliques sharing vertices and
+    triangle-free graphs, also when the tuning moves between the plain and the line-aligned rows (the plan's row starts
+    are rewritten each way); the batch union keeps each graph's count."""
     from graph_analyzer_amd import gen
     rng = np.random.default_rng(77)
     cases = [gen.chung_lu(20000, 200000, 2.1, 2.0, seed=8), gen.communities(20000, 300, 10, 120, 0.3, seed=9),
@@ -302,25 +303,31 @@ def test_relabel_counts_on_every_graph_kind(ctx):
         og = oracle.OGraph.from_pairs(raw)
         T = og.triangles()
         g = ga.DeviceGraph.from_pairs(ctx, raw)
-        for order in (1, 2):
-            for lanes0 in (1, 4):
-                t = g.get_tuning()
-                t.relabel, t.item_order, t.lanes[0] = 1, order, lanes0
-                g.set_tuning(t)
-                assert g.count(ga.TC_HASH)[0] == T, (order, lanes0)
-        t = g.get_tuning()
-        t.flat_chunk, t.flat_block = 2048, 256   # the flat kernel on the renumbered copy
+        for relabel in (1, 2, 1, 2):
+            for order in (1, 2):
+                for lanes0 in (1, 4):
+                    t = g.get_tuning()
+                    t.relabel, t.item_order, t.lanes[0] = relabel, order, lanes0
+                    g.set_tuning(t)
+                    assert g.count(ga.TC_HASH)[0] == T, (relabel, order, lanes0)
+        for relabel in (2, 1, 2):
+            t = g.get_tuning()
+            t.flat_chunk, t.flat_block, t.relabel = 2048, 256, relabel   # the flat kernel on the renumbered copy
+            g.set_tuning(t)
+            assert g.count(ga.TC_HASH)[0] == T, relabel
+        t.hash_cap = 3   # rows past the cap: the binary-search bucket on the line-aligned rows
         g.set_tuning(t)
         assert g.count(ga.TC_HASH)[0] == T
         g.close()
     b = ga.Batch(ctx, cases)
     Tb, _ = b.count_all()
-    t = b.get_tuning()
-    t.relabel, t.item_order = 1, 2
-    b.set_tuning(t)
-    Tr, _ = b.count_all()
-    Tp, _ = b.count_all(per_graph=True)
-    assert np.array_equal(Tb, Tr) and np.array_equal(Tb, Tp)
+    for relabel in (1, 2):
+        t = b.get_tuning()
+        t.relabel, t.item_order = relabel, 2
+        b.set_tuning(t)
+        Tr, _ = b.count_all()
+        Tp, _ = b.count_all(per_graph=True)
+        assert np.array_equal(Tb, Tr) and np.array_equal(Tb, Tp), relabel
     assert [int(x) for x in Tb] == [oracle.OGraph.from_pairs(r).triangles() for r in cases]
     b.close()
 

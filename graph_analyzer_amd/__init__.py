@@ -57,7 +57,7 @@ class TcTuning(ctypes.Structure):
                 ("batches", ctypes.c_int32), ("load_shift", ctypes.c_int32), ("hash_cap", ctypes.c_int32),
                 ("flat_chunk", ctypes.c_int32), ("flat_block", ctypes.c_int32),
                 ("item_order", ctypes.c_int32), ("relabel", ctypes.c_int32), ("probe", ctypes.c_int32),
-                ("waves", ctypes.c_int32)]
+                ("waves", ctypes.c_int32), ("streams", ctypes.c_int32)]
 
     @classmethod
     def from_dict(cls, d):
@@ -74,7 +74,7 @@ class TcTuning(ctypes.Structure):
                 "load_shift": self.load_shift, "hash_cap": self.hash_cap, "flat_chunk": self.flat_chunk,
                 "flat_block": self.flat_block,
                 "item_order": self.item_order, "relabel": self.relabel, "probe": self.probe,
-                "waves": self.waves}
+                "waves": self.waves, "streams": self.streams}
 
 
 _lib = None

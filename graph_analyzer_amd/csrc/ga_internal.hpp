@@ -21,6 +21,8 @@ struct ga_ctx {
     size_t lds_cu = 163840;             // LDS bytes of one CU (shared by its resident workgroups)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};   // tune.streams: the count's bucket launches besides `stream`
+    hipEvent_t fork_ev = nullptr, join_ev[3] = {nullptr, nullptr, nullptr};   //   (default priority, created on first use)
     uint64_t* hstat = nullptr;          // pinned host staging for per-round statistics (512 bytes)
     uint64_t* dsmall = nullptr;         // 64 device words for counters of one call (never reallocated)
     void* scratch = nullptr;            // grow-only device scratch for library temporaries (sort / scan storage)

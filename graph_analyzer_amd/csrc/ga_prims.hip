@@ -96,6 +96,11 @@ extern "C" void ga_ctx_destroy(ga_ctx* c) {
     if (c->hstat) (void)hipHostFree(c->hstat);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int i = 0; i < 3; i++) {
+        if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+        if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
+    }
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -1218,6 +1218,7 @@ ga_tc_tuning default_tuning() {
     t.relabel = 0;
     t.probe = 1;
     t.waves = 0;
+    t.streams = 0;
     return t;
 }
 
@@ -1235,7 +1236,7 @@ bool tuning_valid(const ga_tc_tuning& t) {
     return t.block >= 64 && t.block <= 1024 && t.block % 64 == 0 && t.work_chunk >= 64 && t.batches >= 1 && t.batches <= 1024 &&
            t.load_shift >= 1 && t.load_shift <= 3 && t.hash_cap >= 1 && t.hash_cap <= 4096 && t.flat_block >= 64 &&
            t.flat_block <= 1024 && t.flat_block % 64 == 0 && t.item_order >= 0 && t.item_order <= 5 && t.relabel >= 0 && t.relabel <= 2 && (t.probe == 0 || t.probe == 1) &&
-           (t.waves == 0 || (t.waves >= 4 && t.waves <= 8)) &&
+           (t.waves == 0 || (t.waves >= 4 && t.waves <= 8)) && (t.streams == 0 || t.streams == 1) &&
            (t.flat_chunk == 0 || (t.flat_chunk <= 16384 && t.flat_chunk % t.flat_block == 0 &&
                                   (t.flat_chunk / t.flat_block == 4 || t.flat_chunk / t.flat_block == 8 ||
                                    t.flat_chunk / t.flat_block == 16)));
@@ -1623,14 +1624,56 @@ static int launch_hashed(ga_ctx* ctx, ga_graph* g, const CountView& V, const uin
     return launch_hash_bucket<G, true, FLAGS>(ctx, g, V, items, ni, bound, a);
 }
 
+// tune.streams: the bucket launches of one count go to four streams (ctx->stream and three side streams that wait for
+// it), so that one launch's tail and the one-lane kernel of the shortest rows, which uses no LDS, overlap the others;
+// on scope exit ctx->stream is restored and waits for every side stream used (also on an error return).
+struct StreamFork {
+    ga_ctx* ctx;
+    hipStream_t main;
+    bool on = false;
+    bool used[3] = {false, false, false};
+    StreamFork(ga_ctx* c, bool want) : ctx(c), main(c->stream) {
+        if (!want) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(main, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;   // captured: one stream
+        for (int i = 0; i < 3; i++) {
+            if (!c->side[i] && hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) != hipSuccess) return;
+            if (!c->join_ev[i] && hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming) != hipSuccess) return;
+        }
+        if (!c->fork_ev && hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess) return;
+        if (hipEventRecord(c->fork_ev, main) != hipSuccess) return;
+        for (int i = 0; i < 3; i++)
+            if (hipStreamWaitEvent(c->side[i], c->fork_ev, 0) != hipSuccess) return;
+        on = true;
+    }
+    void to(int i) {   // -1: ctx->stream
+        if (!on) return;
+        if (i >= 0) used[i] = true;
+        ctx->stream = i >= 0 ? ctx->side[i] : main;
+    }
+    ~StreamFork() {
+        if (!on) return;
+        ctx->stream = main;
+        for (int i = 0; i < 3; i++)
+            if (used[i]) {
+                (void)hipEventRecord(ctx->join_ev[i], ctx->side[i]);
+                (void)hipStreamWaitEvent(main, ctx->join_ev[i], 0);
+            }
+    }
+};
+
 // evs (optional, NHASH + 2 events): evs[b] is recorded before bucket b's launch, evs[NHASH] before the binary-search
-// bucket's and evs[NHASH + 1] after it (per-launch timing of ga_tc_bucket_profile).
+// bucket's and evs[NHASH + 1] after it (per-launch timing of ga_tc_bucket_profile, always on one stream).
 template <int FLAGS>
 static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, int first_bucket = 0, hipEvent_t* evs = nullptr) {
     const TcPlan& P = *V.P;
+    // bucket 1 (d+(h) <= 32) on ctx->stream, buckets 0 and 2 on side streams 0 and 1, the rest after each other on 2
+    StreamFork fork(ctx, (FLAGS == 0 || FLAGS == F_GRAPH) && !evs && P.tune.streams);
+    static const int stream_of[NHASH + 1] = {0, -1, 1, 2, 2, 2};
     uint32_t off = 0;
     for (int b = 0; b < NHASH; b++) {
         if (evs) GA_HIP(ctx, hipEventRecord(evs[b], ctx->stream));
+        fork.to(stream_of[b]);
         const uint32_t* items = P.items + 4ull * off;
         const uint32_t ni = b < first_bucket ? 0u : P.nitems[b];
         const uint32_t bound = std::min(BUCKET_DEG[b], (uint32_t)P.tune.hash_cap);
@@ -1649,8 +1692,10 @@ static int launch_hash(ga_ctx* ctx, ga_graph* g, const CountView& V, TcArgs a, i
         GA_TRY(rc);
         off += P.nitems[b];
     }
+    fork.to(stream_of[NHASH]);
     if (evs) GA_HIP(ctx, hipEventRecord(evs[NHASH], ctx->stream));
     GA_TRY((launch_hash_bucket<64, false, FLAGS>(ctx, g, V, P.items + 4ull * off, P.nhuge, 0, a)));
+    fork.to(-1);
     if (evs) GA_HIP(ctx, hipEventRecord(evs[NHASH + 1], ctx->stream));
     return GA_OK;
 }
@@ -2430,6 +2475,11 @@ int ga_tc_tune_counts(ga_ctx* ctx, ga_graph* g, const uint32_t* gof, unsigned lo
         if (pass == 0) {   // how the lane groups read a batch's probe lists: lane slots or length classes
             ga_tc_tuning t = bt;
             t.probe = 1 - bt.probe;
+            GA_TRY(trial(t));
+        }
+        {   // the bucket launches on one stream or overlapped on four
+            ga_tc_tuning t = bt;
+            t.streams = 1 - bt.streams;
             GA_TRY(trial(t));
         }
         GA_TRY(ga_tc_set_tuning(ctx, g, &bt));

@@ -196,6 +196,10 @@ typedef struct ga_tc_tuning {
     int32_t waves;         /* plain and per-graph counts: 0 = as many waves per SIMD as registers and tables allow;
                               4..8 = at most this many, held by the workgroups' LDS allocation (fewer items in flight
                               keep more of their probe lists in the L2) */
+    int32_t streams;       /* plain and per-graph counts: 1 = the bucket launches of one count run on four streams
+                              (the caller's and three of the library's, joined back before the call returns), so
+                              that launches overlap; 0 = one after another on the caller's stream. Per-launch
+                              profiles and captured streams always use one stream */
 } ga_tc_tuning;
 int ga_tc_get_tuning(const ga_graph* g, ga_tc_tuning* out);
 /* The library's default tuning (what a graph uses until ga_tc_set_tuning or ga_tc_tune). */

@@ -174,6 +174,7 @@ def test_count_many_shared_launches(ctx):
         t.flat_chunk, t.flat_block = (flat, 256) if flat else (0, 256)
         t.hash_cap, t.item_order, t.probe, t.relabel = cap, order, probe, i % 3
         t.waves = (0, 5, 7, 6)[i % 4]
+        t.streams = i % 2
         g.set_tuning(t)
     own = [g.count(ga.TC_HASH)[0] for g in dgs]
     assert own == ref

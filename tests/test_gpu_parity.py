@@ -195,6 +195,7 @@ def test_tuning_shapes(ctx):
                 t.relabel = (li + block // 64 + shift) % 3   # the renumbered copy (2: line-aligned rows) for plain counts
                 t.probe = (li + shift + block // 256) % 2   # lane slots or length classes for plain counts
                 t.waves = (0, 5, 6, 7, 8, 4)[(li + block // 64 + shift) % 6]   # occupancy caps through LDS
+                t.streams = (li + shift) % 2   # bucket launches overlapped on four streams
                 for cap in (4096, 24, 3):   # small caps send large rows to the binary-search bucket
                     t.hash_cap = cap
                     g.set_tuning(t)

@@ -359,6 +359,11 @@ def run_batch(ctx, count, steps, tune):
             t2 = ga.TcTuning.from_dict(base_t.as_dict())
             t2.flat_chunk, t2.flat_block = fc, fb
             cands.append((f"{name}+flat{fc}/{fb}", t2))
+    # each also with its bucket launches overlapped on four streams (the per-graph path keeps its best tuning)
+    for name, base_t in [("default", ga.default_tuning())] + list(cands):
+        t2 = ga.TcTuning.from_dict(base_t.as_dict())
+        t2.streams = 1 - base_t.streams
+        cands.append((f"{name}+streams{t2.streams}", t2))
     for name, t2 in cands:
         log(f"batch: per-graph count with the {name} tuning")
         b.set_tuning(t2)
